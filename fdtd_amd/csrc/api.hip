@@ -1,0 +1,698 @@
+// api.hip -- the C ABI declared in include/fdtd.h.
+//
+// Replaces the reference's host orchestration, marchingLoop
+// (fdtdrav.h:3425-4387), and its per-step driver, solver (fdtdrav.h:3188-3422),
+// for one device: the z-slab split and ghost-plane copies of the reference
+// collapse into a single domain whose neighbour planes are plain HBM reads.
+// All work goes to one non-blocking stream; a step is two launches (E, H)
+// with no host synchronisation between them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <new>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fdtd.h"
+#include "internal.h"
+
+namespace fdtd {
+hipError_t launch_half_step(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s);
+int tile_rows();
+int tile_cols();
+}  // namespace fdtd
+
+using namespace fdtd;
+
+struct fdtd_ctx {
+    SceneIn in;
+    bool cpml_set = false;
+    bool finalized = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    Built built;
+    KArgs ka{};
+    int e_mode = 0, h_mode = 0;
+    int steps_done = 0;
+    double last_run_ms = 0.0, e_ms = 0.0, h_ms = 0.0;
+    size_t esz = 4;                 // bytes per stored value
+    i64 field_elems = 0;            // per field allocation, guard planes included
+    std::vector<void *> allocs;     // every device allocation
+    void *fields[6] = {};           // allocation bases
+    void *psi[12] = {};
+    size_t psi_elems[12] = {};
+    uint64_t dev_bytes = 0;
+};
+
+namespace {
+
+const char *g_static_err = "no error";
+
+int fail(fdtd_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                      \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(c, e_ == hipErrorOutOfMemory ? FDTD_E_NOMEM : FDTD_E_DEVICE, "%s: %s", \
+                        #expr, hipGetErrorString(e_));                                       \
+    } while (0)
+
+int dev_alloc(fdtd_ctx *c, void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, FDTD_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    c->allocs.push_back(*p);
+    c->dev_bytes += bytes;
+    return FDTD_OK;
+}
+
+int upload(fdtd_ctx *c, void **p, const void *src, size_t bytes)
+{
+    int rc = dev_alloc(c, p, bytes);
+    if (rc) return rc;
+    if (bytes && src) HIPCHK(c, hipMemcpyAsync(*p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return FDTD_OK;
+}
+
+void free_device(fdtd_ctx *c)
+{
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs) (void)hipFree(p);
+    c->allocs.clear();
+    c->dev_bytes = 0;
+    for (auto &f : c->fields) f = nullptr;
+    for (auto &p : c->psi) p = nullptr;
+}
+
+bool box_in_grid(const fdtd_ctx *c, int i, int j, int k)
+{
+    return i >= 0 && j >= 0 && k >= 0 && i < c->in.nx && j < c->in.ny && k < c->in.nz;
+}
+
+// Zero every piece of time-dependent state.
+int zero_state(fdtd_ctx *c)
+{
+    for (int f = 0; f < 6; ++f)
+        HIPCHK(c, hipMemsetAsync(c->fields[f], 0, (size_t)c->field_elems * c->esz, c->stream));
+    for (int p = 0; p < 12; ++p)
+        if (c->psi_elems[p]) HIPCHK(c, hipMemsetAsync(c->psi[p], 0, c->psi_elems[p] * c->esz, c->stream));
+    const size_t np = c->in.probes.size(), nf = c->in.freqs.size();
+    if (np) HIPCHK(c, hipMemsetAsync(c->ka.probe_hist, 0, np * (size_t)c->in.steps_cap * sizeof(float), c->stream));
+    if (np && nf) HIPCHK(c, hipMemsetAsync(c->ka.dft, 0, np * nf * sizeof(double2), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->steps_done = 0;
+    return FDTD_OK;
+}
+
+// Rebuild the material-dependent tables (class map, E/H tables, sources)
+// after a design-eps change; buffers of unchanged size are reused.
+int upload_tables(fdtd_ctx *c);
+
+int choose_chunks(const fdtd_ctx *c, int tiles)
+{
+    // enough workgroups to cover 256 CUs several times over
+    int nchunks = (2048 + tiles - 1) / tiles;
+    nchunks = std::max(1, std::min(nchunks, c->in.nz));
+    return (c->in.nz + nchunks - 1) / nchunks;
+}
+
+int do_finalize(fdtd_ctx *c)
+{
+    const SceneIn &in = c->in;
+    Built &b = c->built;
+    std::string err;
+    if (build_scene(in, b, err) != 0) return fail(c, FDTD_E_INVALID, "%s", err.c_str());
+    c->e_mode = b.e_uniform ? CLS_UNIFORM : b.mode;
+    c->h_mode = b.h_uniform ? CLS_UNIFORM : b.mode;
+    c->esz = in.storage == FDTD_STORE_F32 ? 4 : 2;
+
+    KArgs &a = c->ka;
+    a = KArgs{};
+    a.nx = in.nx; a.ny = in.ny; a.nz = in.nz; a.px = in.px;
+    a.plane = (i64)in.px * in.ny;
+    for (int q = 0; q < 6; ++q) a.L[q] = in.cpml.thick[q];
+    a.Lx = a.L[0] + a.L[1];
+    a.Ly = a.L[2] + a.L[3];
+    a.Lz = a.L[4] + a.L[5];
+    a.d[0] = (float)in.dx; a.d[1] = (float)in.dy; a.d[2] = (float)in.dz;
+    a.steps_cap = in.steps_cap;
+    a.nfreq = (int)in.freqs.size();
+    for (int q = 0; q < 6; ++q) a.probe_unscale[q] = b.probe_unscale[q];
+    a.tiles_x = (in.nx + tile_cols() - 1) / tile_cols();
+    a.tiles_y = (in.ny + tile_rows() - 1) / tile_rows();
+    a.kchunk = choose_chunks(c, a.tiles_x * a.tiles_y);
+
+    // fields: one zero guard plane below and above
+    c->field_elems = a.plane * (in.nz + 2);
+    void **fp[6] = {&a.ex, &a.ey, &a.ez, &a.hx, &a.hy, &a.hz};
+    for (int f = 0; f < 6; ++f) {
+        int rc = dev_alloc(c, &c->fields[f], (size_t)c->field_elems * c->esz);
+        if (rc) return rc;
+        *fp[f] = (char *)c->fields[f] + (size_t)a.plane * c->esz;
+    }
+    // psi slabs
+    const size_t px_elems = (size_t)in.nz * in.ny * a.Lx;
+    const size_t py_elems = (size_t)in.nz * a.Ly * in.px;
+    const size_t pz_elems = (size_t)a.Lz * in.ny * in.px;
+    void **pp[12] = {&a.psi_eyx, &a.psi_ezx, &a.psi_hyx, &a.psi_hzx, &a.psi_exy, &a.psi_ezy,
+                     &a.psi_hxy, &a.psi_hzy, &a.psi_exz, &a.psi_eyz, &a.psi_hxz, &a.psi_hyz};
+    for (int p = 0; p < 12; ++p) {
+        c->psi_elems[p] = p < 4 ? px_elems : p < 8 ? py_elems : pz_elems;
+        int rc = dev_alloc(c, &c->psi[p], std::max<size_t>(c->psi_elems[p], 1) * c->esz);
+        if (rc) return rc;
+        *pp[p] = c->psi[p];
+    }
+    int rc;
+    void *tmp;
+    for (int ax = 0; ax < 3; ++ax) {
+        if ((rc = upload(c, &tmp, b.prof[ax].data(), b.prof[ax].size() * sizeof(float)))) return rc;
+        (ax == 0 ? a.px_prof : ax == 1 ? a.py_prof : a.pz_prof) = (const float4 *)tmp;
+    }
+    if ((rc = upload(c, &tmp, b.e_first.data(), b.e_first.size() * sizeof(int)))) return rc;
+    a.e_first = (const int *)tmp;
+    if ((rc = upload(c, &tmp, b.h_first.data(), b.h_first.size() * sizeof(int)))) return rc;
+    a.h_first = (const int *)tmp;
+    if ((rc = upload(c, &tmp, b.events.data(), b.events.size() * sizeof(Event)))) return rc;
+    a.events = (const Event *)tmp;
+    if ((rc = upload(c, &tmp, b.vpn.data(), b.vpn.size() * sizeof(float)))) return rc;
+    a.vpn = (const float *)tmp;
+    if ((rc = upload(c, &tmp, b.phasor.data(), b.phasor.size() * sizeof(double)))) return rc;
+    a.phasor = (const double2 *)tmp;
+    const size_t np = in.probes.size(), nf = in.freqs.size();
+    if ((rc = dev_alloc(c, &tmp, np * (size_t)in.steps_cap * sizeof(float)))) return rc;
+    a.probe_hist = (float *)tmp;
+    if ((rc = dev_alloc(c, &tmp, np * nf * sizeof(double2)))) return rc;
+    a.dft = (double2 *)tmp;
+    if ((rc = upload_tables(c))) return rc;
+    if ((rc = zero_state(c))) return rc;
+    c->finalized = true;
+    return FDTD_OK;
+}
+
+int upload_tables(fdtd_ctx *c)
+{
+    Built &b = c->built;
+    KArgs &a = c->ka;
+    int rc;
+    void *tmp;
+    if ((rc = upload(c, &tmp, b.etab.data(), b.etab.size() * sizeof(float)))) return rc;
+    a.etab = (const float4 *)tmp;
+    if ((rc = upload(c, &tmp, b.htab.data(), b.htab.size() * sizeof(float)))) return rc;
+    a.htab = (const float4 *)tmp;
+    if ((rc = upload(c, &tmp, b.srcs.data(), b.srcs.size() * sizeof(SrcEdge)))) return rc;
+    a.srcs = (const SrcEdge *)tmp;
+    a.cls = nullptr;
+    if (b.mode == CLS_U16) {
+        std::vector<uint16_t> m16(b.cls.size());
+        for (size_t q = 0; q < b.cls.size(); ++q) m16[q] = (uint16_t)b.cls[q];
+        if ((rc = upload(c, &tmp, m16.data(), m16.size() * sizeof(uint16_t)))) return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        a.cls = tmp;
+    } else if (b.mode == CLS_U32) {
+        if ((rc = upload(c, &tmp, b.cls.data(), b.cls.size() * sizeof(uint32_t)))) return rc;
+        a.cls = tmp;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<uint32_t>().swap(b.cls);   // host copy no longer needed
+    return FDTD_OK;
+}
+
+int run_steps(fdtd_ctx *c, int nsteps, bool profile)
+{
+    if (!c->finalized) return fail(c, FDTD_E_STATE, "fdtd_finalize has not been called");
+    if (nsteps < 0) return fail(c, FDTD_E_INVALID, "nsteps < 0");
+    if (c->steps_done + nsteps > c->in.steps_cap)
+        return fail(c, FDTD_E_RANGE, "run past the finalized horizon (%d + %d > %d)", c->steps_done, nsteps,
+                    c->in.steps_cap);
+    hipEvent_t t0, t1;
+    HIPCHK(c, hipEventCreate(&t0));
+    HIPCHK(c, hipEventCreate(&t1));
+    std::vector<hipEvent_t> evs;
+    if (profile) {
+        evs.resize((size_t)4 * nsteps);
+        for (auto &e : evs) HIPCHK(c, hipEventCreate(&e));
+    }
+    const int m0 = c->steps_done;
+    HIPCHK(c, hipEventRecord(t0, c->stream));
+    for (int s = 0; s < nsteps; ++s) {
+        const int m = m0 + s;
+        if (profile) HIPCHK(c, hipEventRecord(evs[4 * s + 0], c->stream));
+        HIPCHK(c, launch_half_step(true, c->in.storage, c->e_mode, c->ka, m, c->stream));
+        if (profile) HIPCHK(c, hipEventRecord(evs[4 * s + 1], c->stream));
+        if (profile) HIPCHK(c, hipEventRecord(evs[4 * s + 2], c->stream));
+        HIPCHK(c, launch_half_step(false, c->in.storage, c->h_mode, c->ka, m, c->stream));
+        if (profile) HIPCHK(c, hipEventRecord(evs[4 * s + 3], c->stream));
+    }
+    HIPCHK(c, hipEventRecord(t1, c->stream));
+    HIPCHK(c, hipEventSynchronize(t1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, t0, t1));
+    c->last_run_ms = ms;
+    if (profile && nsteps > 0) {
+        double se = 0, sh = 0;
+        for (int s = 0; s < nsteps; ++s) {
+            float a = 0, b = 0;
+            HIPCHK(c, hipEventElapsedTime(&a, evs[4 * s + 0], evs[4 * s + 1]));
+            HIPCHK(c, hipEventElapsedTime(&b, evs[4 * s + 2], evs[4 * s + 3]));
+            se += a;
+            sh += b;
+        }
+        c->e_ms = se / nsteps;
+        c->h_ms = sh / nsteps;
+        for (auto &e : evs) (void)hipEventDestroy(e);
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    c->steps_done = m0 + nsteps;
+    // blow-up detection on the probe samples of this call
+    const size_t np = c->in.probes.size();
+    if (np && nsteps > 0) {
+        std::vector<float> h(np * (size_t)nsteps);
+        HIPCHK(c, hipMemcpy2D(h.data(), (size_t)nsteps * sizeof(float), c->ka.probe_hist + m0,
+                              (size_t)c->in.steps_cap * sizeof(float), (size_t)nsteps * sizeof(float), np,
+                              hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < h.size(); ++q)
+            if (!std::isfinite(h[q]))
+                return fail(c, FDTD_E_NONFINITE, "probe %zu is non-finite at step %zu", q / nsteps,
+                            m0 + q % nsteps);
+    }
+    return FDTD_OK;
+}
+
+uint64_t fnv1a(const unsigned char *p, size_t n, uint64_t h = 1469598103934665603ull)
+{
+    for (size_t q = 0; q < n; ++q) { h ^= p[q]; h *= 1099511628211ull; }
+    return h;
+}
+
+struct CkHeader {
+    char magic[8];
+    int32_t version, nx, ny, nz, storage, step, nprobe, nfreq, steps_cap, pad;
+    uint64_t payload, checksum;
+};
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+const char *fdtd_last_error(const fdtd_ctx *ctx) { return ctx ? ctx->err.c_str() : g_static_err; }
+
+int fdtd_create(const fdtd_grid *g, int storage, fdtd_ctx **out)
+{
+    if (!out) return FDTD_E_INVALID;
+    *out = nullptr;
+    if (!g) return FDTD_E_INVALID;
+    if (g->nx < 4 || g->ny < 4 || g->nz < 4) return FDTD_E_INVALID;
+    if (!(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0)) return FDTD_E_INVALID;
+    if (storage < FDTD_STORE_F32 || storage > FDTD_STORE_BF16) return FDTD_E_INVALID;
+    double eps0 = g->eps0 > 0 ? g->eps0 : 8.854187817e-12;
+    double mu0 = g->mu0 > 0 ? g->mu0 : 4.0 * M_PI * 1e-7;
+    double dt = g->dt;
+    const double cfl = 1.0 / ((1.0 / std::sqrt(mu0 * eps0)) *
+                              std::sqrt(1.0 / (g->dx * g->dx) + 1.0 / (g->dy * g->dy) + 1.0 / (g->dz * g->dz)));
+    if (!(dt > 0)) {
+        if (!(g->courant > 0) || g->courant > 1.0) return FDTD_E_INVALID;
+        dt = g->courant * cfl;
+    } else if (dt > cfl * (1.0 + 1e-12)) {
+        return FDTD_E_INVALID;   // above the 3-D Courant limit: unstable
+    }
+    fdtd_ctx *c = new (std::nothrow) fdtd_ctx();
+    if (!c) return FDTD_E_NOMEM;
+    SceneIn &in = c->in;
+    in.nx = g->nx; in.ny = g->ny; in.nz = g->nz;
+    in.px = (g->nx + 63) / 64 * 64;
+    in.dx = g->dx; in.dy = g->dy; in.dz = g->dz;
+    in.dt = dt; in.eps0 = eps0; in.mu0 = mu0;
+    in.storage = storage;
+    in.cpml = CpmlParams{{0, 0, 0, 0, 0, 0}, 3.0, {0, 0, 0}, 1.5, 5.0, 0.05, 0.0};
+    for (int q = 0; q < 6; ++q) in.design[q] = 0;
+    in.steps_cap = 0;
+    if (hipGetDevice(&c->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        return FDTD_E_DEVICE;
+    }
+    *out = c;
+    return FDTD_OK;
+}
+
+int fdtd_destroy(fdtd_ctx *c)
+{
+    if (!c) return FDTD_E_INVALID;
+    free_device(c);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FDTD_OK;
+}
+
+int fdtd_set_cpml(fdtd_ctx *c, const fdtd_cpml *p)
+{
+    if (!c || !p) return FDTD_E_INVALID;
+    if (c->finalized) return fail(c, FDTD_E_STATE, "scene is finalized");
+    const int n[3] = {c->in.nx, c->in.ny, c->in.nz};
+    for (int a = 0; a < 3; ++a) {
+        const int lo = p->thick[2 * a], hi = p->thick[2 * a + 1];
+        if (lo < 0 || hi < 0 || lo + hi > n[a] - 2)
+            return fail(c, FDTD_E_INVALID, "CPML widths %d+%d do not fit axis %d of %d cells", lo, hi, a, n[a]);
+    }
+    if (!(p->order >= 0) || !(p->kappa_max >= 1) || !(p->alpha_max >= 0) || !(p->alpha_min >= 0))
+        return fail(c, FDTD_E_INVALID, "invalid CPML grading parameters");
+    CpmlParams &q = c->in.cpml;
+    for (int a = 0; a < 6; ++a) q.thick[a] = p->thick[a];
+    q.order = p->order;
+    for (int a = 0; a < 3; ++a) q.sigma_max[a] = p->sigma_max[a];
+    q.sigma_factor = p->sigma_factor > 0 ? p->sigma_factor : 1.5;
+    q.kappa_max = p->kappa_max;
+    q.alpha_max = p->alpha_max;
+    q.alpha_min = p->alpha_min;
+    c->cpml_set = true;
+    return FDTD_OK;
+}
+
+int fdtd_add_box(fdtd_ctx *c, int is, int js, int ks, int ie, int je, int ke, const float eps_r[3],
+                 const float sigma_e[3], const float mu_r[3], const float sigma_m[3])
+{
+    if (!c || !eps_r || !sigma_e || !mu_r || !sigma_m) return FDTD_E_INVALID;
+    if (c->finalized) return fail(c, FDTD_E_STATE, "scene is finalized");
+    for (int q = 0; q < 3; ++q) {
+        if (!(eps_r[q] > 0) || !(mu_r[q] > 0) || !(sigma_e[q] >= 0) || !(sigma_m[q] >= 0) ||
+            !std::isfinite(eps_r[q]) || !std::isfinite(mu_r[q]) || !std::isfinite(sigma_e[q]) ||
+            !std::isfinite(sigma_m[q]))
+            return fail(c, FDTD_E_INVALID, "invalid material values");
+    }
+    Box b;
+    b.b[0] = is; b.b[1] = js; b.b[2] = ks; b.b[3] = ie; b.b[4] = je; b.b[5] = ke;
+    for (int q = 0; q < 3; ++q) {
+        b.mat[q] = eps_r[q];
+        b.mat[3 + q] = sigma_e[q];
+        b.mat[6 + q] = mu_r[q];
+        b.mat[9 + q] = sigma_m[q];
+    }
+    c->in.boxes.push_back(b);
+    return FDTD_OK;
+}
+
+int fdtd_add_lumped_source(fdtd_ctx *c, int dir, int is, int js, int ks, int ie, int je, int ke, double R,
+                           const fdtd_waveform *w)
+{
+    if (!c || !w) return FDTD_E_INVALID;
+    if (c->finalized) return fail(c, FDTD_E_STATE, "scene is finalized");
+    if (dir != 3) return fail(c, FDTD_E_UNSUPPORTED, "only z-directed lumped sources (dir = 3) are built");
+    if (!(R > 0) || !std::isfinite(R)) return fail(c, FDTD_E_INVALID, "source resistance must be > 0");
+    if (w->type < FDTD_WAVE_GAUSSIAN || w->type > FDTD_WAVE_SINE)
+        return fail(c, FDTD_E_INVALID, "unknown waveform type %d", w->type);
+    if ((w->type != FDTD_WAVE_SINE && !(w->tau > 0)) || !std::isfinite(w->amp))
+        return fail(c, FDTD_E_INVALID, "invalid waveform parameters");
+    if (is > ie || js > je || ks >= ke) return fail(c, FDTD_E_INVALID, "empty source box");
+    // every edge must be an updated Ez edge: 0<i<nx-1, 0<j<ny-1, k<nz-1
+    if (is < 1 || ie > c->in.nx - 2 || js < 1 || je > c->in.ny - 2 || ks < 0 || ke > c->in.nz - 1)
+        return fail(c, FDTD_E_RANGE, "source box outside the updated Ez region");
+    Source s;
+    s.b[0] = is; s.b[1] = js; s.b[2] = ks; s.b[3] = ie; s.b[4] = je; s.b[5] = ke;
+    s.R = R;
+    s.wtype = w->type;
+    s.amp = w->amp; s.t0 = w->t0; s.tau = w->tau; s.freq = w->freq;
+    c->in.sources.push_back(s);
+    return FDTD_OK;
+}
+
+int fdtd_add_probe(fdtd_ctx *c, int comp, int i, int j, int k, int *index)
+{
+    if (!c) return FDTD_E_INVALID;
+    if (c->finalized) return fail(c, FDTD_E_STATE, "scene is finalized");
+    if (comp < FDTD_EX || comp > FDTD_HZ) return fail(c, FDTD_E_INVALID, "invalid component %d", comp);
+    if (!box_in_grid(c, i, j, k)) return fail(c, FDTD_E_RANGE, "probe (%d,%d,%d) outside the grid", i, j, k);
+    if (index) *index = (int)c->in.probes.size();
+    c->in.probes.push_back(Probe{comp, i, j, k});
+    return FDTD_OK;
+}
+
+int fdtd_set_dft(fdtd_ctx *c, int nfreq, const double *freqs)
+{
+    if (!c || nfreq < 0 || (nfreq > 0 && !freqs)) return FDTD_E_INVALID;
+    if (c->finalized) return fail(c, FDTD_E_STATE, "scene is finalized");
+    for (int f = 0; f < nfreq; ++f)
+        if (!std::isfinite(freqs[f])) return fail(c, FDTD_E_INVALID, "non-finite DFT frequency");
+    c->in.freqs.assign(freqs, freqs + nfreq);
+    return FDTD_OK;
+}
+
+int fdtd_set_design_region(fdtd_ctx *c, int is, int js, int ks, int ie, int je, int ke)
+{
+    if (!c) return FDTD_E_INVALID;
+    if (c->finalized) return fail(c, FDTD_E_STATE, "scene is finalized");
+    if (is < 0 || js < 0 || ks < 0 || ie > c->in.nx || je > c->in.ny || ke > c->in.nz || is >= ie || js >= je ||
+        ks >= ke)
+        return fail(c, FDTD_E_RANGE, "design region outside the grid or empty");
+    int *d = c->in.design;
+    d[0] = is; d[1] = js; d[2] = ks; d[3] = ie; d[4] = je; d[5] = ke;
+    c->in.design_eps.assign((size_t)(ie - is) * (je - js) * (ke - ks), 1.0f);
+    return FDTD_OK;
+}
+
+int fdtd_set_design_eps(fdtd_ctx *c, const float *eps, size_t n)
+{
+    if (!c || !eps) return FDTD_E_INVALID;
+    if (c->in.design_eps.empty()) return fail(c, FDTD_E_STATE, "no design region");
+    if (n != c->in.design_eps.size()) return fail(c, FDTD_E_INVALID, "design eps has %zu cells, expected %zu", n,
+                                                  c->in.design_eps.size());
+    for (size_t q = 0; q < n; ++q)
+        if (!(eps[q] > 0) || !std::isfinite(eps[q])) return fail(c, FDTD_E_INVALID, "invalid design eps_r");
+    std::memcpy(c->in.design_eps.data(), eps, n * sizeof(float));
+    if (!c->finalized) return FDTD_OK;
+    // rebuild every table on the new materials and restart from step 0
+    free_device(c);
+    c->finalized = false;
+    return do_finalize(c);
+}
+
+int fdtd_finalize(fdtd_ctx *c, int n_steps)
+{
+    if (!c) return FDTD_E_INVALID;
+    if (c->finalized) return fail(c, FDTD_E_STATE, "already finalized");
+    if (n_steps < 1) return fail(c, FDTD_E_INVALID, "n_steps must be >= 1");
+    c->in.steps_cap = n_steps;
+    int rc = do_finalize(c);
+    if (rc) free_device(c);
+    return rc;
+}
+
+int fdtd_run(fdtd_ctx *c, int nsteps)
+{
+    if (!c) return FDTD_E_INVALID;
+    return run_steps(c, nsteps, false);
+}
+
+int fdtd_profile(fdtd_ctx *c, int nsteps)
+{
+    if (!c) return FDTD_E_INVALID;
+    return run_steps(c, nsteps, true);
+}
+
+int fdtd_reset(fdtd_ctx *c)
+{
+    if (!c) return FDTD_E_INVALID;
+    if (!c->finalized) return fail(c, FDTD_E_STATE, "fdtd_finalize has not been called");
+    return zero_state(c);
+}
+
+int fdtd_get_info(const fdtd_ctx *c, fdtd_info *o)
+{
+    if (!c || !o) return FDTD_E_INVALID;
+    std::memset(o, 0, sizeof *o);
+    o->nx = c->in.nx; o->ny = c->in.ny; o->nz = c->in.nz;
+    o->steps_done = c->steps_done;
+    o->steps_capacity = c->in.steps_cap;
+    o->n_probes = (int)c->in.probes.size();
+    o->n_freqs = (int)c->in.freqs.size();
+    o->n_sources = (int)c->in.sources.size();
+    o->n_classes = c->built.ncls;
+    o->storage = c->in.storage;
+    o->dt = c->in.dt;
+    o->last_run_ms = c->last_run_ms;
+    o->e_kernel_ms = c->e_ms;
+    o->h_kernel_ms = c->h_ms;
+    o->device_bytes = c->dev_bytes;
+    // algorithmic bytes per cell: E sweep reads E, H and writes E (9 values);
+    // H sweep likewise; plus one class index per cell in a classed sweep and
+    // psi read+write over the slabs (averaged over the grid).
+    const double s = (double)c->esz;
+    const double fx = (double)(c->in.cpml.thick[0] + c->in.cpml.thick[1]) / c->in.nx;
+    const double fy = (double)(c->in.cpml.thick[2] + c->in.cpml.thick[3]) / c->in.ny;
+    const double fz = (double)(c->in.cpml.thick[4] + c->in.cpml.thick[5]) / c->in.nz;
+    const double psi = 4.0 * s * (fx + fy + fz);
+    const double cls = c->built.mode == CLS_U16 ? 2.0 : c->built.mode == CLS_U32 ? 4.0 : 0.0;
+    o->bytes_per_cell_e = (uint64_t)std::llround(9 * s + (c->e_mode ? cls : 0.0) + psi);
+    o->bytes_per_cell_h = (uint64_t)std::llround(9 * s + (c->h_mode ? cls : 0.0) + psi);
+    return FDTD_OK;
+}
+
+int fdtd_get_probes(fdtd_ctx *c, float *out, size_t n)
+{
+    if (!c || (!out && n)) return FDTD_E_INVALID;
+    if (!c->finalized) return fail(c, FDTD_E_STATE, "not finalized");
+    const size_t np = c->in.probes.size(), ns = (size_t)c->steps_done;
+    if (n < np * ns) return fail(c, FDTD_E_INVALID, "output holds %zu values, need %zu", n, np * ns);
+    if (np && ns)
+        HIPCHK(c, hipMemcpy2D(out, ns * sizeof(float), c->ka.probe_hist, (size_t)c->in.steps_cap * sizeof(float),
+                              ns * sizeof(float), np, hipMemcpyDeviceToHost));
+    return FDTD_OK;
+}
+
+int fdtd_get_dft(fdtd_ctx *c, double *out, size_t n)
+{
+    if (!c || (!out && n)) return FDTD_E_INVALID;
+    if (!c->finalized) return fail(c, FDTD_E_STATE, "not finalized");
+    const size_t cnt = c->in.probes.size() * c->in.freqs.size() * 2;
+    if (n < cnt) return fail(c, FDTD_E_INVALID, "output holds %zu values, need %zu", n, cnt);
+    if (cnt) HIPCHK(c, hipMemcpy(out, c->ka.dft, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    return FDTD_OK;
+}
+
+int fdtd_get_field(fdtd_ctx *c, int comp, float *out, size_t n)
+{
+    if (!c || !out) return FDTD_E_INVALID;
+    if (!c->finalized) return fail(c, FDTD_E_STATE, "not finalized");
+    if (comp < 0 || comp > 5) return fail(c, FDTD_E_INVALID, "invalid component");
+    const int nx = c->in.nx, ny = c->in.ny, nz = c->in.nz, px = c->in.px;
+    const size_t cells = (size_t)nx * ny * nz;
+    if (n < cells) return fail(c, FDTD_E_INVALID, "output too small");
+    std::vector<unsigned char> raw((size_t)px * ny * nz * c->esz);
+    const char *base = (const char *)c->fields[comp] + (size_t)c->ka.plane * c->esz;
+    HIPCHK(c, hipMemcpy(raw.data(), base, raw.size(), hipMemcpyDeviceToHost));
+    const float un = c->built.probe_unscale[comp];
+    for (size_t r = 0; r < (size_t)ny * nz; ++r)
+        for (int i = 0; i < nx; ++i) {
+            const size_t src = r * px + i;
+            float v;
+            if (c->esz == 4) {
+                std::memcpy(&v, &raw[src * 4], 4);
+            } else {
+                uint16_t h;
+                std::memcpy(&h, &raw[src * 2], 2);
+                if (c->in.storage == FDTD_STORE_BF16) {
+                    uint32_t u = (uint32_t)h << 16;
+                    std::memcpy(&v, &u, 4);
+                } else {
+                    _Float16 f;
+                    std::memcpy(&f, &h, 2);
+                    v = (float)f;
+                }
+            }
+            out[r * nx + i] = v * un;
+        }
+    return FDTD_OK;
+}
+
+// ---- checkpoint -------------------------------------------------------------
+int fdtd_checkpoint_size(const fdtd_ctx *c, size_t *bytes)
+{
+    if (!c || !bytes) return FDTD_E_INVALID;
+    if (!c->finalized) return FDTD_E_STATE;
+    size_t p = 6 * (size_t)c->field_elems * c->esz;
+    for (int q = 0; q < 12; ++q) p += c->psi_elems[q] * c->esz;
+    p += c->in.probes.size() * (size_t)c->in.steps_cap * sizeof(float);
+    p += c->in.probes.size() * c->in.freqs.size() * sizeof(double2);
+    *bytes = sizeof(CkHeader) + p;
+    return FDTD_OK;
+}
+
+int fdtd_checkpoint_save(fdtd_ctx *c, void *buf, size_t bytes)
+{
+    if (!c || !buf) return FDTD_E_INVALID;
+    size_t need = 0;
+    int rc = fdtd_checkpoint_size(c, &need);
+    if (rc) return fail(c, rc, "not finalized");
+    if (bytes < need) return fail(c, FDTD_E_INVALID, "checkpoint buffer holds %zu bytes, need %zu", bytes, need);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unsigned char *p = (unsigned char *)buf + sizeof(CkHeader);
+    auto get = [&](const void *dev, size_t n) -> int {
+        if (n) HIPCHK(c, hipMemcpy(p, dev, n, hipMemcpyDeviceToHost));
+        p += n;
+        return FDTD_OK;
+    };
+    for (int f = 0; f < 6; ++f)
+        if ((rc = get(c->fields[f], (size_t)c->field_elems * c->esz))) return rc;
+    for (int q = 0; q < 12; ++q)
+        if ((rc = get(c->psi[q], c->psi_elems[q] * c->esz))) return rc;
+    if ((rc = get(c->ka.probe_hist, c->in.probes.size() * (size_t)c->in.steps_cap * sizeof(float)))) return rc;
+    if ((rc = get(c->ka.dft, c->in.probes.size() * c->in.freqs.size() * sizeof(double2)))) return rc;
+    CkHeader h{};
+    std::memcpy(h.magic, "FDTDCKP1", 8);
+    h.version = 1;
+    h.nx = c->in.nx; h.ny = c->in.ny; h.nz = c->in.nz;
+    h.storage = c->in.storage;
+    h.step = c->steps_done;
+    h.nprobe = (int)c->in.probes.size();
+    h.nfreq = (int)c->in.freqs.size();
+    h.steps_cap = c->in.steps_cap;
+    h.payload = need - sizeof(CkHeader);
+    h.checksum = fnv1a((const unsigned char *)buf + sizeof(CkHeader), h.payload);
+    std::memcpy(buf, &h, sizeof h);
+    return FDTD_OK;
+}
+
+int fdtd_checkpoint_restore(fdtd_ctx *c, const void *buf, size_t bytes)
+{
+    if (!c || !buf) return FDTD_E_INVALID;
+    size_t need = 0;
+    int rc = fdtd_checkpoint_size(c, &need);
+    if (rc) return fail(c, rc, "not finalized");
+    if (bytes < need) return fail(c, FDTD_E_CHECKSUM, "checkpoint truncated");
+    CkHeader h;
+    std::memcpy(&h, buf, sizeof h);
+    if (std::memcmp(h.magic, "FDTDCKP1", 8) != 0 || h.version != 1 || h.nx != c->in.nx || h.ny != c->in.ny ||
+        h.nz != c->in.nz || h.storage != c->in.storage || h.nprobe != (int)c->in.probes.size() ||
+        h.nfreq != (int)c->in.freqs.size() || h.steps_cap != c->in.steps_cap || h.payload != need - sizeof h ||
+        h.step < 0 || h.step > c->in.steps_cap)
+        return fail(c, FDTD_E_CHECKSUM, "checkpoint does not match this context");
+    if (fnv1a((const unsigned char *)buf + sizeof h, h.payload) != h.checksum)
+        return fail(c, FDTD_E_CHECKSUM, "checkpoint checksum mismatch");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const unsigned char *p = (const unsigned char *)buf + sizeof h;
+    auto put = [&](void *dev, size_t n) -> int {
+        if (n) HIPCHK(c, hipMemcpy(dev, p, n, hipMemcpyHostToDevice));
+        p += n;
+        return FDTD_OK;
+    };
+    for (int f = 0; f < 6; ++f)
+        if ((rc = put(c->fields[f], (size_t)c->field_elems * c->esz))) return rc;
+    for (int q = 0; q < 12; ++q)
+        if ((rc = put(c->psi[q], c->psi_elems[q] * c->esz))) return rc;
+    if ((rc = put(c->ka.probe_hist, c->in.probes.size() * (size_t)c->in.steps_cap * sizeof(float)))) return rc;
+    if ((rc = put(c->ka.dft, c->in.probes.size() * c->in.freqs.size() * sizeof(double2)))) return rc;
+    c->steps_done = h.step;
+    return FDTD_OK;
+}
+
+int fdtd_adjoint_gradient(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int checkpoint_every,
+                          double *J_out, double *grad_out, size_t n_grad)
+{
+    (void)obj; (void)n_steps; (void)checkpoint_every; (void)J_out; (void)grad_out; (void)n_grad;
+    if (!c) return FDTD_E_INVALID;
+    return fail(c, FDTD_E_UNSUPPORTED, "adjoint not built yet");
+}
+
+}  // extern "C"

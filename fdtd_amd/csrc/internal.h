@@ -1,0 +1,122 @@
+// internal.h -- structures shared by the host scene builder (scene.cpp), the
+// gfx950 kernels (kernels.hip) and the C ABI (api.hip).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fdtd {
+
+typedef std::int64_t i64;
+
+// Coefficient-class map index width.  Class 0..n-1 index the E and H tables.
+enum ClassMode { CLS_UNIFORM = 0, CLS_U16 = 1, CLS_U32 = 2 };
+
+// One event on a (i, j) column: a source edge or a probe cell at plane k.
+// Events are sorted by column, then k; `col` = j * nx + i.
+struct Event {
+    int k;
+    int kind;   // EV_SOURCE, or EV_PROBE + component (0..5)
+    int id;     // source-edge record or probe index
+    int col;
+};
+enum { EV_SOURCE = 0, EV_PROBE = 8 };
+
+// Per source edge: Ez coefficients with the resistive load folded in
+// (defineSourceZ, fdtdrav.h:1402-1421) and the waveform row.
+struct SrcEdge {
+    float ceze, cezhy, cezhx, cezj;   // cezj already carries the E storage scale
+    int wave;                         // row of the per-node waveform table
+    int pad[3];
+};
+
+// Kernel argument block.  Field pointers point at (i,j,k) = (0,0,0) of an
+// allocation that has one zero guard plane below k = 0 and one above
+// k = nz-1, so every i+-1, j+-1, k+-1 neighbour address is inside it.
+struct KArgs {
+    void *ex, *ey, *ez, *hx, *hy, *hz;
+    // psi slabs: x [nz][ny][Lx], y [nz][Ly][px], z [Lz][ny][px]
+    void *psi_eyx, *psi_ezx, *psi_hyx, *psi_hzx;
+    void *psi_exy, *psi_ezy, *psi_hxy, *psi_hzy;
+    void *psi_exz, *psi_eyz, *psi_hxz, *psi_hyz;
+    const void *cls;           // class map [nz][ny][px] (u16/u32) or null
+    const float4 *etab;        // [class][3] (ce, cb_a, cb_b, 0) for Ex, Ey, Ez
+    const float4 *htab;        // [class][3] (ch, cb_a, cb_b, 0) for Hx, Hy, Hz
+    // 1-D CPML profiles, indexed by the global cell index of the axis:
+    // float4 (b, a, kappa, 0), E at [0,n), H at [n,2n)
+    const float4 *px_prof, *py_prof, *pz_prof;
+    const int *e_first, *h_first;   // per column: first event or -1
+    const Event *events;
+    const SrcEdge *srcs;
+    const float *vpn;          // [wave][steps_cap]
+    float *probe_hist;         // [probe][steps_cap]
+    const double2 *phasor;     // [step][nfreq]
+    double2 *dft;              // [probe][nfreq]
+    float probe_unscale[6];    // stored -> SI per component
+    float d[3];                // dx, dy, dz (cpsi = Cb * d)
+    int nx, ny, nz, px;
+    i64 plane;                 // px * ny
+    int L[6];                  // x-, x+, y-, y+, z-, z+
+    int Lx, Ly, Lz;            // slab widths summed per axis
+    int steps_cap, nfreq;
+    int kchunk;                // planes per z-march block
+    int tiles_x, tiles_y;      // tile grid (for the XCD-aware remap)
+};
+
+// Host-side scene, filled by the fdtd_* scene calls.
+struct Box {
+    int b[6];
+    float mat[12];  // eps_r xyz, sigma_e xyz, mu_r xyz, sigma_m xyz
+};
+struct Source {
+    int b[6];
+    double R;
+    int wtype;
+    double amp, t0, tau, freq;
+};
+struct Probe {
+    int comp, i, j, k;
+};
+
+struct CpmlParams {
+    int thick[6];
+    double order, sigma_max[3], sigma_factor, kappa_max, alpha_max, alpha_min;
+};
+
+// Result of the host build: everything the device needs.
+struct Built {
+    int mode;                       // ClassMode
+    bool e_uniform, h_uniform;
+    int ncls;
+    std::vector<std::uint32_t> cls; // per cell (padded pitch), empty if uniform
+    std::vector<float> etab, htab;  // [class][3][4]
+    std::vector<float> prof[3];     // [2n][4] per axis
+    std::vector<int> e_first, h_first;
+    std::vector<Event> events;
+    std::vector<SrcEdge> srcs;
+    std::vector<float> vpn;         // [wave][steps_cap]
+    std::vector<double> phasor;     // [step][nfreq][2]
+    float probe_unscale[6];
+    float scale[6];                 // stored = SI * scale, per component
+    double sigma_max_used[3];
+};
+
+struct SceneIn {
+    int nx, ny, nz, px;
+    double dx, dy, dz, dt, eps0, mu0;
+    int storage;
+    CpmlParams cpml;
+    std::vector<Box> boxes;
+    std::vector<Source> sources;
+    std::vector<Probe> probes;
+    std::vector<double> freqs;
+    int design[6];
+    std::vector<float> design_eps;
+    int steps_cap;
+};
+
+// scene.cpp
+int build_scene(const SceneIn &in, Built &out, std::string &err);
+
+}  // namespace fdtd

@@ -1,0 +1,245 @@
+/*
+ * fdtd.h -- C ABI of the MI355X (gfx950) 3-D Yee/CPML FDTD engine.
+ *
+ * This is the drop-in replacement for the host orchestration boundary of
+ * ravelec/FDTD, `marchingLoop(...)` (fdtdrav.h:3425-3453), whose ~110
+ * positional arguments are split here into a handle plus one call per
+ * concept: grid, CPML, material boxes, lumped sources, probes, the time-step
+ * loop and the outputs.  Every entry point below names the reference code it
+ * replaces.
+ *
+ * Conventions
+ *   - Every call returns int: FDTD_OK (0) on success, a negative FDTD_E_* code
+ *     otherwise.  Nothing calls exit(); the reference's HANDLE_ERROR did
+ *     (fdtdrav.h:24-33).  fdtd_last_error() gives the text of the last error.
+ *   - Indices are cell indices (i, j, k) on the logical grid
+ *     [0,nx) x [0,ny) x [0,nz), which includes the CPML cells -- the
+ *     reference's NXX x NYY x NZ (fdtdrav.h:3487-3492).  Flat arrays handed in
+ *     or out are x-fastest: idx = i + nx*(j + ny*k).
+ *   - Ownership: the library owns all device memory.  Input arrays are copied
+ *     during the call; output arrays are caller-allocated.
+ *   - Threading: a context is not thread-safe.  Use one host thread per
+ *     context.  A context runs on the HIP device that was current when
+ *     fdtd_create() was called, on one internal stream.
+ *   - Life cycle: create -> set_cpml / add_box / add_lumped_source /
+ *     add_probe / set_dft / set_design_region -> finalize -> run ... ->
+ *     get_* -> destroy.  fdtd_reset() rewinds to step 0 with zero fields.
+ */
+#ifndef FDTD_H
+#define FDTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---------------------------------------------------- */
+#define FDTD_OK             0
+#define FDTD_E_INVALID     -1  /* invalid argument or configuration        */
+#define FDTD_E_STATE       -2  /* call not allowed in the current state    */
+#define FDTD_E_RANGE       -3  /* index or box outside the grid            */
+#define FDTD_E_NOMEM       -4  /* host or device allocation failed         */
+#define FDTD_E_DEVICE      -5  /* HIP runtime error                        */
+#define FDTD_E_UNSUPPORTED -6  /* feature not provided by this build       */
+#define FDTD_E_NONFINITE   -7  /* a probe recorded a non-finite value      */
+#define FDTD_E_CHECKSUM    -8  /* checkpoint blob is corrupt or mismatched */
+
+/* Opaque context: one simulation on one device. */
+typedef struct fdtd_ctx fdtd_ctx;
+
+/* ---- enumerations --------------------------------------------------- */
+/* Field storage precision.  Arithmetic is FP32 in every form. */
+#define FDTD_STORE_F32   0
+#define FDTD_STORE_F16   1  /* FP16 storage of E, H and psi, scaled       */
+#define FDTD_STORE_BF16  2  /* BF16 storage of E, H and psi, scaled       */
+
+/* Field components (probe and field-access selectors). */
+#define FDTD_EX 0
+#define FDTD_EY 1
+#define FDTD_EZ 2
+#define FDTD_HX 3
+#define FDTD_HY 4
+#define FDTD_HZ 5
+
+/* Waveform types; the numbering follows the reference's source_tp
+ * (source.cu:160, fdtdrav.h:4071-4085). */
+#define FDTD_WAVE_GAUSSIAN       1  /* amp*exp(-((t-t0)/tau)^2)                     */
+#define FDTD_WAVE_GAUSS_DERIV    2  /* amp*sqrt(2e)*((t-t0)/tau)*exp(-((t-t0)/tau)^2) */
+#define FDTD_WAVE_SINE           3  /* amp*sin(2*pi*freq*t)                          */
+
+/* ---- parameter blocks ----------------------------------------------- */
+
+/* Grid and constants.  Replaces the constants and dt of source.cu:30-46 and
+ * the sizing of fdtdrav.h:3487-3498 (NUMDEV = 1, so no z padding). */
+typedef struct {
+    int    nx, ny, nz;      /* cells per axis, CPML included                 */
+    double dx, dy, dz;      /* cell sizes [m]                                */
+    double dt;              /* time step [s]; <= 0 selects courant * 3-D CFL  */
+    double courant;         /* Courant factor used when dt <= 0 (source.cu:41-46) */
+    double eps0, mu0;       /* <= 0 selects 8.854187817e-12 and 4*pi*1e-7     */
+} fdtd_grid;
+
+/* CPML parameters.  The reference hard-codes them in setABCPML
+ * (fdtdrav.h:1774-1812); widths come from pml_{x,y,z}_{n,p}
+ * (source.cu:53-59).  A zero width disables that face (PEC backing only). */
+typedef struct {
+    int    thick[6];        /* x-, x+, y-, y+, z-, z+ widths in cells         */
+    double order;           /* polynomial grading m (reference: 3)            */
+    double sigma_max[3];    /* per axis; <= 0 selects sigma_factor*sigma_opt,
+                               sigma_opt = (m+1)/(150*pi*d) (fdtdrav.h:1783-1798) */
+    double sigma_factor;    /* reference: 1.5                                 */
+    double kappa_max;       /* reference: 5                                   */
+    double alpha_max;       /* reference: 0.05                                */
+    double alpha_min;       /* reference: 0                                   */
+} fdtd_cpml;
+
+/* Source waveform.  Replaces setGauss / setSine (fdtdrav.h:1486-1520) and the
+ * source_tp / source_amp / tau / t_0 / source_freq arguments (fdtdrav.h:3430-3433). */
+typedef struct {
+    int    type;            /* FDTD_WAVE_*                                    */
+    double amp;             /* [V]                                            */
+    double t0;              /* [s]                                            */
+    double tau;             /* [s]                                            */
+    double freq;            /* [Hz], FDTD_WAVE_SINE only                      */
+} fdtd_waveform;
+
+/* Adjoint objective: J = sum_t weight[t] * |DFT(probe[t], freq[t])|^2. */
+typedef struct {
+    int           n_terms;
+    const int    *probe;    /* probe index (an E-component probe)            */
+    const int    *freq;     /* DFT frequency index                            */
+    const double *weight;   /* NULL selects 1 for every term                  */
+} fdtd_objective;
+
+/* Run statistics. */
+typedef struct {
+    int      nx, ny, nz;
+    int      steps_done;
+    int      steps_capacity;
+    int      n_probes, n_freqs, n_sources;
+    int      n_classes;          /* distinct E/H coefficient classes           */
+    int      storage;            /* FDTD_STORE_*                               */
+    double   dt;
+    double   last_run_ms;        /* hipEvent time of the last fdtd_run() loop  */
+    double   e_kernel_ms;        /* mean E-kernel launch time (fdtd_profile)   */
+    double   h_kernel_ms;        /* mean H-kernel launch time (fdtd_profile)   */
+    uint64_t device_bytes;       /* device memory held by the context          */
+    uint64_t bytes_per_cell_e;   /* algorithmic bytes per cell, E sweep        */
+    uint64_t bytes_per_cell_h;   /* algorithmic bytes per cell, H sweep        */
+} fdtd_info;
+
+/* ---- life cycle ----------------------------------------------------- */
+
+/* Create a context on the current HIP device.  Replaces marchingLoop's
+ * sizing, P2P set-up and allocation (fdtdrav.h:3455-3498, 3874-4045) for one
+ * device. storage is FDTD_STORE_*. */
+int fdtd_create(const fdtd_grid *grid, int storage, fdtd_ctx **out);
+
+/* Release every resource.  The reference frees nothing (fdtdrav.h:4384). */
+int fdtd_destroy(fdtd_ctx *ctx);
+
+/* Text of the last error on this context (static storage when ctx is NULL). */
+const char *fdtd_last_error(const fdtd_ctx *ctx);
+
+/* ---- scene ---------------------------------------------------------- */
+
+/* CPML parameters; replaces setABCPML's constants (fdtdrav.h:1747-1812). */
+int fdtd_set_cpml(fdtd_ctx *ctx, const fdtd_cpml *cpml);
+
+/* Paint a material box on cells is<=i<ie, js<=j<je, ks<=k<ke; later boxes
+ * win.  Replaces setBrick (fdtdrav.h:1524-1571) and its launch loop
+ * (fdtdrav.h:4104-4128).  Each array holds the x, y, z values. */
+int fdtd_add_box(fdtd_ctx *ctx, int is, int js, int ks, int ie, int je, int ke,
+                 const float eps_r[3], const float sigma_e[3],
+                 const float mu_r[3], const float sigma_m[3]);
+
+/* Resistive lumped voltage source on the edges is<=i<=ie, js<=j<=je,
+ * ks<=k<ke.  Replaces defineSourceZ (fdtdrav.h:1368-1426) with the
+ * injection of calc_e (fdtdrav.h:726-729).  dir must be 3 (z).  Any number
+ * of non-overlapping sources is allowed; the reference had one. */
+int fdtd_add_lumped_source(fdtd_ctx *ctx, int dir, int is, int js, int ks,
+                           int ie, int je, int ke, double resistance,
+                           const fdtd_waveform *wave);
+
+/* Point probe of one component, sampled once per step after that
+ * component's main update and before its CPML correction, as sampleField is
+ * called in calc_e / calc_h (fdtdrav.h:234-246, 740, 347-391).  Returns the
+ * probe index through *index (may be NULL). */
+int fdtd_add_probe(fdtd_ctx *ctx, int comp, int i, int j, int k, int *index);
+
+/* On-the-fly DFT of every probe at nfreq frequencies [Hz]:
+ * X(f) = sum_m s[m] * exp(-j*2*pi*f*m*dt) * dt, accumulated in FP64.  No
+ * counterpart runs in the reference; its cuFFT path calc_otm
+ * (fdtdrav.h:2988-3051) is disabled at fdtdrav.h:4376. */
+int fdtd_set_dft(fdtd_ctx *ctx, int nfreq, const double *freqs);
+
+/* Design region is<=i<ie, js<=j<je, ks<=k<ke whose per-cell relative
+ * permittivity is set by fdtd_set_design_eps and differentiated by
+ * fdtd_adjoint_gradient.  No counterpart in the reference. */
+int fdtd_set_design_region(fdtd_ctx *ctx, int is, int js, int ks,
+                           int ie, int je, int ke);
+
+/* Per-cell isotropic eps_r of the design region (x-fastest, n = its cell
+ * count), painted after every box.  Allowed after finalize: coefficients are
+ * rebuilt and the context is reset to step 0. */
+int fdtd_set_design_eps(fdtd_ctx *ctx, const float *eps_r, size_t n);
+
+/* Build coefficients, CPML profiles and source loading in the reference
+ * order -- averaging, coefficients, CPML, then sources (fdtdrav.h:4134-4290)
+ * -- upload them and allocate the step buffers for n_steps steps
+ * (the reference's n_t_steps, fdtdrav.h:3588). */
+int fdtd_finalize(fdtd_ctx *ctx, int n_steps);
+
+/* ---- time loop ------------------------------------------------------ */
+
+/* Advance nsteps steps (E then H each step, fdtdrav.h:3188-3422 and the
+ * loop of fdtdrav.h:4320-4355).  Blocking; returns FDTD_E_NONFINITE if a
+ * probe went non-finite. */
+int fdtd_run(fdtd_ctx *ctx, int nsteps);
+
+/* Rewind to step 0: zero fields, psi, probe history and DFT accumulators. */
+int fdtd_reset(fdtd_ctx *ctx);
+
+/* Run nsteps steps with a HIP event pair around every E and every H launch
+ * and store the mean launch times in fdtd_info.  Same results as fdtd_run. */
+int fdtd_profile(fdtd_ctx *ctx, int nsteps);
+
+int fdtd_get_info(const fdtd_ctx *ctx, fdtd_info *info);
+
+/* ---- outputs -------------------------------------------------------- */
+
+/* Probe history [probe][step], steps_done columns per row (n >= probes *
+ * steps_done).  Replaces the sampled-buffer read-back (fdtdrav.h:4364-4365). */
+int fdtd_get_probes(fdtd_ctx *ctx, float *out, size_t n);
+
+/* DFT accumulators [probe][freq][re, im] (n >= 2 * probes * freqs). */
+int fdtd_get_dft(fdtd_ctx *ctx, double *out, size_t n);
+
+/* One field component as FP32 SI values on the logical grid (n >= nx*ny*nz). */
+int fdtd_get_field(fdtd_ctx *ctx, int comp, float *out, size_t n);
+
+/* ---- checkpoint / resume (no counterpart in the reference) ---------- */
+
+int fdtd_checkpoint_size(const fdtd_ctx *ctx, size_t *bytes);
+/* Full state: step index, E, H, psi, probe history, DFT accumulators, and a
+ * checksum.  Resume is bitwise equal to an uninterrupted run. */
+int fdtd_checkpoint_save(fdtd_ctx *ctx, void *buf, size_t bytes);
+int fdtd_checkpoint_restore(fdtd_ctx *ctx, const void *buf, size_t bytes);
+
+/* ---- adjoint (no counterpart in the reference) ---------------------- */
+
+/* Run n_steps forward steps from step 0 (checkpointing every
+ * checkpoint_every steps), then the discrete adjoint backward in segments,
+ * recomputing each segment's forward fields from its checkpoint.
+ * J_out receives J; grad_out receives dJ/d(eps_r) of every design cell
+ * (x-fastest over the design region, FP64). */
+int fdtd_adjoint_gradient(fdtd_ctx *ctx, const fdtd_objective *obj,
+                          int n_steps, int checkpoint_every,
+                          double *J_out, double *grad_out, size_t n_grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FDTD_H */

@@ -1,0 +1,656 @@
+/*
+ * fdtd_oracle.cpp -- CPU restatement of ravelec/FDTD's Yee/CPML time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the GPU engine is compared
+ * against (tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg).
+ * The engine in fdtd_amd/ never links or calls it.
+ *
+ * It restates the reference's equations for one device (NUMDEV = 1) on
+ * reference-shaped full-grid arrays, in the reference's order, in FP32 or
+ * FP64 (template parameter R).  Every block cites the code it follows
+ * (fdtdrav.h = H, source.cu = S):
+ *
+ *   raw materials, init          H:919-1094 (setZero), H:1524-1571 (setBrick)
+ *   edge averaging               H:1097-1220 (setSigmaEpsSigmaMu)
+ *   update coefficients          H:1223-1308 (setCoefficients)
+ *   CPML profiles, cpsi, kappa   H:1747-2160 (setABCPML)
+ *   lumped z-source loading      H:1368-1426 (defineSourceZ)
+ *   waveforms                    H:1486-1520 (setGauss, setSine)
+ *   E update + source + sample   H:577-743 (calc_e), CPML H:750-911
+ *   H update + sample            H:249-395 (calc_h), CPML H:398-562
+ *   step order E then H          H:3188-3422 (solver), loop H:4320-4355
+ *
+ * Deliberate differences from the reference, none observable in the probes:
+ *   - The reference initialises fields, sigma and unused coefficients to
+ *     1.2e-38 (H:949-1059).  Here they start at 0.  The only place the
+ *     1.2e-38 mattered is the harmonic mean of sigma_m (H:1185-1213), where
+ *     2ab underflows to 0 in FP32; here a+b == 0 yields 0 explicitly.
+ *   - CPML 1-D profiles, waveforms and DFT phasors are evaluated in FP64 and
+ *     rounded once to R (the reference evaluates them with device powf/expf
+ *     in FP32, an ulp-level difference).
+ *   - Any number of non-overlapping z-sources (the reference has one) and a
+ *     Gaussian-derivative waveform (source_tp = 2 is an empty case at
+ *     H:4077-4079).
+ *   - Point probes instead of sample boxes; an FP64 on-the-fly DFT.
+ *
+ * Built with -ffp-contract=off: every a*b+c is a rounded multiply and a
+ * rounded add, in the operand order written, so an FP32 engine written with
+ * the same order can match it bit for bit.
+ */
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+extern "C" {
+
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy, dz, dt, eps0, mu0;
+    int pml[6];                 /* x-, x+, y-, y+, z-, z+ */
+    double order, sigma_max[3], kappa_max, alpha_max, alpha_min;
+    int nbox;
+    const int *box;             /* 6 per box: is js ks ie je ke (half-open) */
+    const float *box_mat;       /* 12 per box: eps_r xyz, sigma_e xyz, mu_r xyz, sigma_m xyz */
+    int nsrc;
+    const int *src;             /* 6 per source: is js ks ie je ke */
+    const double *src_R;
+    const int *src_wtype;
+    const double *src_wpar;     /* 4 per source: amp t0 tau freq */
+    int nprobe;
+    const int *probe;           /* 4 per probe: comp i j k */
+    int nfreq;
+    const double *freqs;
+    int design[6];              /* is js ks ie je ke, ie <= is when absent */
+    const float *design_eps;
+    int nsteps;                 /* waveform / probe horizon */
+} oracle_scene;
+
+}  // extern "C"
+
+namespace {
+
+typedef std::int64_t i64;
+
+template <class R>
+struct Sim {
+    int NX, NY, NZ;
+    i64 NXY, N;
+    int L[6];
+    int nsteps, step;
+    R dx, dy, dz, dt, eps0, mu0;
+    // fields
+    std::vector<R> Ex, Ey, Ez, Hx, Hy, Hz;
+    // coefficients (reference names)
+    std::vector<R> Cexe, Cexhz, Cexhy, Ceye, Ceyhx, Ceyhz, Ceze, Cezhy, Cezhx, Cezj;
+    std::vector<R> Chxh, Chxey, Chxez, Chyh, Chyez, Chyex, Chzh, Chzex, Chzey;
+    // CPML state
+    std::vector<R> b_ex, a_ex, b_mx, a_mx, b_ey, a_ey, b_my, a_my, b_ez, a_ez, b_mz, a_mz;
+    std::vector<R> Psi_eyx, Psi_ezx, Psi_hyx, Psi_hzx, cpsi_eyx, cpsi_ezx, cpsi_hyx, cpsi_hzx;
+    std::vector<R> Psi_exy, Psi_ezy, Psi_hxy, Psi_hzy, cpsi_exy, cpsi_ezy, cpsi_hxy, cpsi_hzy;
+    std::vector<R> Psi_exz, Psi_eyz, Psi_hxz, Psi_hyz, cpsi_exz, cpsi_eyz, cpsi_hxz, cpsi_hyz;
+    // sources
+    std::vector<int> src_of;          // per cell: source id or -1
+    int nsrc;
+    std::vector<R> vpn;               // [src][step] V(m dt)/(ke-ks)
+    // probes
+    int nprobe, nfreq;
+    std::vector<int> pcomp, pidx;
+    std::vector<double> phist;        // [probe][nsteps]
+    std::vector<double> ph_re, ph_im; // [step][freq]
+    std::vector<double> dft;          // [probe][freq][2]
+
+    i64 id(int i, int j, int k) const { return (i64)i + (i64)NX * ((i64)j + (i64)NY * k); }
+};
+
+template <class R>
+static double waveform(int type, const double *p, double t)
+{
+    const double amp = p[0], t0 = p[1], tau = p[2], f = p[3];
+    switch (type) {
+    case 1: {  // Gaussian, setGauss H:1495
+        double u = (t - t0) / tau;
+        return amp * std::exp(-(u * u));
+    }
+    case 2: {  // Gaussian derivative, peak amp at t0 + tau/sqrt(2)
+        double u = (t - t0) / tau;
+        return amp * std::sqrt(2.0 * std::exp(1.0)) * u * std::exp(-(u * u));
+    }
+    case 3:  // sine, setSine H:1513
+        return amp * std::sin(2.0 * M_PI * f * t);
+    }
+    return 0.0;
+}
+
+template <class R>
+static Sim<R> *build(const oracle_scene *sc)
+{
+    Sim<R> *s = new Sim<R>();
+    const int NX = s->NX = sc->nx, NY = s->NY = sc->ny, NZ = s->NZ = sc->nz;
+    s->NXY = (i64)NX * NY;
+    const i64 N = s->N = s->NXY * NZ;
+    for (int a = 0; a < 6; ++a) s->L[a] = sc->pml[a];
+    s->nsteps = sc->nsteps;
+    s->step = 0;
+    const R dx = s->dx = (R)sc->dx, dy = s->dy = (R)sc->dy, dz = s->dz = (R)sc->dz;
+    const R dt = s->dt = (R)sc->dt, eps_0 = s->eps0 = (R)sc->eps0, mu_0 = s->mu0 = (R)sc->mu0;
+
+    // ---- raw material grids, setZero H:949-972 then setBrick H:1548-1569
+    std::vector<R> m_eps[3], m_se[3], m_mu[3], m_sm[3];
+    for (int c = 0; c < 3; ++c) {
+        m_eps[c].assign(N, (R)1); m_se[c].assign(N, (R)0);
+        m_mu[c].assign(N, (R)1);  m_sm[c].assign(N, (R)0);
+    }
+    for (int b = 0; b < sc->nbox; ++b) {
+        const int *bx = sc->box + 6 * b;
+        const float *bm = sc->box_mat + 12 * b;
+        // cells is <= i < ie, js <= j < je, ks <= k < ke of the grid (H:1548)
+        for (int k = std::max(0, bx[2]); k < std::min(NZ, bx[5]); ++k)
+            for (int j = std::max(0, bx[1]); j < std::min(NY, bx[4]); ++j)
+                for (int i = std::max(0, bx[0]); i < std::min(NX, bx[3]); ++i) {
+                    i64 x = s->id(i, j, k);
+                    for (int c = 0; c < 3; ++c) {
+                        m_eps[c][x] = (R)bm[c];     m_se[c][x] = (R)bm[3 + c];
+                        m_mu[c][x]  = (R)bm[6 + c]; m_sm[c][x] = (R)bm[9 + c];
+                    }
+                }
+    }
+    // design-region per-cell eps_r, painted last (isotropic)
+    {
+        const int *d = sc->design;
+        if (sc->design_eps && d[3] > d[0] && d[4] > d[1] && d[5] > d[2]) {
+            const int dnx = d[3] - d[0], dny = d[4] - d[1];
+            for (int k = d[2]; k < d[5]; ++k)
+                for (int j = d[1]; j < d[4]; ++j)
+                    for (int i = d[0]; i < d[3]; ++i) {
+                        R e = (R)sc->design_eps[(i - d[0]) + (i64)dnx * ((j - d[1]) + (i64)dny * (k - d[2]))];
+                        i64 x = s->id(i, j, k);
+                        m_eps[0][x] = m_eps[1][x] = m_eps[2][x] = e;
+                    }
+        }
+    }
+
+    // ---- edge averaging, H:1127-1214 (guards as in the reference; outside
+    // them the averaged value keeps its init eps=1 / sigma=0 / mu=1).
+    std::vector<R> eps[3], se[3], mu[3], sm[3];
+    for (int c = 0; c < 3; ++c) {
+        eps[c].assign(N, (R)1); se[c].assign(N, (R)0);
+        mu[c].assign(N, (R)1);  sm[c].assign(N, (R)0);
+    }
+    auto harm = [](R a, R b) -> R {
+        R den = a + b;
+        if (den == (R)0) return (R)0;
+        return ((R)2 * a * b) / den;
+    };
+    const i64 NXY = s->NXY;
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                i64 x = s->id(i, j, k);
+                if (j > 0 && k > 0) {       // eps_r_x, H:1135-1144
+                    eps[0][x] = (R)0.25 * (((m_eps[0][x] + m_eps[0][x - NX]) + m_eps[0][x - NXY]) + m_eps[0][x - NX - NXY]);
+                    se[0][x]  = (R)0.25 * (((m_se[0][x] + m_se[0][x - NX]) + m_se[0][x - NXY]) + m_se[0][x - NX - NXY]);
+                }
+                if (i > 0 && k > 0) {       // eps_r_y, H:1150-1161
+                    eps[1][x] = (R)0.25 * (((m_eps[1][x] + m_eps[1][x - 1]) + m_eps[1][x - NXY]) + m_eps[1][x - 1 - NXY]);
+                    se[1][x]  = (R)0.25 * (((m_se[1][x] + m_se[1][x - 1]) + m_se[1][x - NXY]) + m_se[1][x - 1 - NXY]);
+                }
+                if (i > 0 && j > 0) {       // eps_r_z, H:1166-1177
+                    eps[2][x] = (R)0.25 * (((m_eps[2][x] + m_eps[2][x - 1]) + m_eps[2][x - NX]) + m_eps[2][x - 1 - NX]);
+                    se[2][x]  = (R)0.25 * (((m_se[2][x] + m_se[2][x - 1]) + m_se[2][x - NX]) + m_se[2][x - 1 - NX]);
+                }
+                if (i > 0) {                // mu_r_x, H:1181-1187
+                    mu[0][x] = harm(m_mu[0][x], m_mu[0][x - 1]);
+                    sm[0][x] = harm(m_sm[0][x], m_sm[0][x - 1]);
+                }
+                if (j > 0) {                // mu_r_y, H:1192-1199
+                    mu[1][x] = harm(m_mu[1][x], m_mu[1][x - NX]);
+                    sm[1][x] = harm(m_sm[1][x], m_sm[1][x - NX]);
+                }
+                if (k > 0) {                // mu_r_z, H:1204-1213
+                    mu[2][x] = harm(m_mu[2][x], m_mu[2][x - NXY]);
+                    sm[2][x] = harm(m_sm[2][x], m_sm[2][x - NXY]);
+                }
+            }
+    for (int c = 0; c < 3; ++c) {
+        std::vector<R>().swap(m_eps[c]); std::vector<R>().swap(m_se[c]);
+        std::vector<R>().swap(m_mu[c]);  std::vector<R>().swap(m_sm[c]);
+    }
+
+    // ---- coefficients, setCoefficients H:1247-1305
+    auto Z = [N](std::vector<R> &v) { v.assign(N, (R)0); };
+    Z(s->Cexe); Z(s->Cexhz); Z(s->Cexhy); Z(s->Ceye); Z(s->Ceyhx); Z(s->Ceyhz);
+    Z(s->Ceze); Z(s->Cezhy); Z(s->Cezhx); Z(s->Cezj);
+    Z(s->Chxh); Z(s->Chxey); Z(s->Chxez); Z(s->Chyh); Z(s->Chyez); Z(s->Chyex);
+    Z(s->Chzh); Z(s->Chzex); Z(s->Chzey);
+    const R two = (R)2;
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                i64 x = s->id(i, j, k);
+                if (i < NX && j < NY - 1 && k < NZ - 1) {
+                    R den = two * mu[0][x] * mu_0 + dt * sm[0][x];
+                    s->Chxh[x]  = (two * mu[0][x] * mu_0 - dt * sm[0][x]) / den;
+                    s->Chxey[x] = (two * dt) / (den * dz);
+                    s->Chxez[x] = ((R)-2 * dt) / (den * dy);
+                }
+                if (i < NX - 1 && j < NY && k < NZ - 1) {
+                    R den = two * mu[1][x] * mu_0 + dt * sm[1][x];
+                    s->Chyh[x]  = (two * mu[1][x] * mu_0 - dt * sm[1][x]) / den;
+                    s->Chyez[x] = (two * dt) / (den * dx);
+                    s->Chyex[x] = ((R)-2 * dt) / (den * dz);
+                }
+                if (i < NX - 1 && j < NY - 1 && k < NZ) {
+                    R den = two * mu[2][x] * mu_0 + dt * sm[2][x];
+                    s->Chzh[x]  = (two * mu[2][x] * mu_0 - dt * sm[2][x]) / den;
+                    s->Chzex[x] = (two * dt) / (den * dy);
+                    s->Chzey[x] = ((R)-2 * dt) / (den * dx);
+                }
+                if (i < NX - 1 && j < NY - 1 && k < NZ - 1 && j > 0 && k > 0) {
+                    R den = two * eps[0][x] * eps_0 + dt * se[0][x];
+                    s->Cexe[x]  = (two * eps[0][x] * eps_0 - dt * se[0][x]) / den;
+                    s->Cexhz[x] = (two * dt) / (den * dy);
+                    s->Cexhy[x] = ((R)-2 * dt) / (den * dz);
+                }
+                if (i < NX - 1 && j < NY - 1 && k < NZ - 1 && i > 0 && k > 0) {
+                    R den = two * eps[1][x] * eps_0 + dt * se[1][x];
+                    s->Ceye[x]  = (two * eps[1][x] * eps_0 - dt * se[1][x]) / den;
+                    s->Ceyhx[x] = (two * dt) / (den * dz);
+                    s->Ceyhz[x] = ((R)-2 * dt) / (den * dx);
+                }
+                if (i < NX - 1 && j < NY - 1 && k < NZ - 1 && i > 0 && j > 0) {
+                    R den = two * eps[2][x] * eps_0 + dt * se[2][x];
+                    s->Ceze[x]  = (two * eps[2][x] * eps_0 - dt * se[2][x]) / den;
+                    s->Cezhy[x] = (two * dt) / (den * dx);
+                    s->Cezhx[x] = ((R)-2 * dt) / (den * dy);
+                }
+            }
+
+    // ---- CPML, setABCPML H:1747-2155
+    const int Lxn = s->L[0], Lxp = s->L[1], Lyn = s->L[2], Lyp = s->L[3], Lzn = s->L[4], Lzp = s->L[5];
+    const double m = sc->order, kmax = sc->kappa_max, amax = sc->alpha_max, amin = sc->alpha_min;
+    const double e0 = sc->eps0, u0 = sc->mu0, ddt = sc->dt;
+    auto prof = [&](int axis, int n, int Ln, int Lp, double d, std::vector<R> &be, std::vector<R> &ae,
+                    std::vector<R> &bm, std::vector<R> &am, std::vector<R> &ke, std::vector<R> &km) {
+        be.assign(n, (R)0); ae.assign(n, (R)0); bm.assign(n, (R)0); am.assign(n, (R)0);
+        ke.assign(n, (R)1); km.assign(n, (R)1);
+        const double smax = sc->sigma_max[axis];
+        for (int i = 0; i < n; ++i) {
+            double re, rm;
+            if (i < Ln) {                       // low side, H:1863-1864
+                re = (Ln - i - 0.75) / Ln;
+                rm = (Ln - i - 0.25) / Ln;
+            } else if (i >= n - Lp) {           // high side, H:1913-1914
+                re = (i - (n - Lp - 1) - 0.75) / Lp;
+                rm = (i - (n - Lp - 1) - 0.25) / Lp;
+            } else {
+                continue;
+            }
+            double sge = smax * std::pow(re, m);
+            double sgm = smax * std::pow(rm, m) * (u0 / e0);
+            double kpe = 1.0 + (kmax - 1.0) * std::pow(re, m);
+            double kpm = 1.0 + (kmax - 1.0) * std::pow(rm, m);
+            double ale = amin + (amax - amin) * (1.0 - re);
+            double alm = (amin + (amax - amin) * (1.0 - rm)) * (u0 / e0);
+            double vbe = std::exp((-ddt / e0) * ((sge / kpe) + ale));
+            double vbm = std::exp((-ddt / u0) * ((sgm / kpm) + alm));
+            double dne = kpe * (sge + kpe * ale), dnm = kpm * (sgm + kpm * alm);
+            be[i] = (R)vbe;
+            bm[i] = (R)vbm;
+            ae[i] = (R)(dne != 0.0 ? (1.0 / d) * (vbe - 1.0) * sge / dne : 0.0);
+            am[i] = (R)(dnm != 0.0 ? (1.0 / d) * (vbm - 1.0) * sgm / dnm : 0.0);
+            ke[i] = (R)kpe;
+            km[i] = (R)kpm;
+        }
+    };
+    std::vector<R> kex, kmx, key, kmy, kez, kmz;
+    prof(0, NX, Lxn, Lxp, sc->dx, s->b_ex, s->a_ex, s->b_mx, s->a_mx, kex, kmx);
+    prof(1, NY, Lyn, Lyp, sc->dy, s->b_ey, s->a_ey, s->b_my, s->a_my, key, kmy);
+    prof(2, NZ, Lzn, Lzp, sc->dz, s->b_ez, s->a_ez, s->b_mz, s->a_mz, kez, kmz);
+    Z(s->Psi_eyx); Z(s->Psi_ezx); Z(s->Psi_hyx); Z(s->Psi_hzx);
+    Z(s->cpsi_eyx); Z(s->cpsi_ezx); Z(s->cpsi_hyx); Z(s->cpsi_hzx);
+    Z(s->Psi_exy); Z(s->Psi_ezy); Z(s->Psi_hxy); Z(s->Psi_hzy);
+    Z(s->cpsi_exy); Z(s->cpsi_ezy); Z(s->cpsi_hxy); Z(s->cpsi_hzy);
+    Z(s->Psi_exz); Z(s->Psi_eyz); Z(s->Psi_hxz); Z(s->Psi_hyz);
+    Z(s->cpsi_exz); Z(s->cpsi_eyz); Z(s->cpsi_hxz); Z(s->cpsi_hyz);
+    // cpsi = (curl coefficient) * d, then the coefficient /= kappa, over each
+    // setup slab (H:1868-1898 x, H:1967-1998 y, H:2066-2095 z and their p sides).
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                i64 x = s->id(i, j, k);
+                if (i < Lxn || i >= NX - Lxp) {
+                    s->cpsi_eyx[x] = s->Ceyhz[x] * dx;
+                    s->cpsi_ezx[x] = s->Cezhy[x] * dx;
+                    s->cpsi_hyx[x] = s->Chyez[x] * dx;
+                    s->cpsi_hzx[x] = s->Chzey[x] * dx;
+                    s->Ceyhz[x] = s->Ceyhz[x] / kex[i];
+                    s->Cezhy[x] = s->Cezhy[x] / kex[i];
+                    s->Chyez[x] = s->Chyez[x] / kmx[i];
+                    s->Chzey[x] = s->Chzey[x] / kmx[i];
+                }
+                if (j < Lyn || j >= NY - Lyp) {
+                    s->cpsi_exy[x] = s->Cexhz[x] * dy;
+                    s->cpsi_ezy[x] = s->Cezhx[x] * dy;
+                    s->cpsi_hxy[x] = s->Chxez[x] * dy;
+                    s->cpsi_hzy[x] = s->Chzex[x] * dy;
+                    s->Cexhz[x] = s->Cexhz[x] / key[j];
+                    s->Cezhx[x] = s->Cezhx[x] / key[j];
+                    s->Chxez[x] = s->Chxez[x] / kmy[j];
+                    s->Chzex[x] = s->Chzex[x] / kmy[j];
+                }
+                if (k < Lzn || k >= NZ - Lzp) {
+                    s->cpsi_exz[x] = s->Cexhy[x] * dz;
+                    s->cpsi_eyz[x] = s->Ceyhx[x] * dz;
+                    s->cpsi_hxz[x] = s->Chxey[x] * dz;
+                    s->cpsi_hyz[x] = s->Chyex[x] * dz;
+                    s->Cexhy[x] = s->Cexhy[x] / kez[k];
+                    s->Ceyhx[x] = s->Ceyhx[x] / kez[k];
+                    s->Chxey[x] = s->Chxey[x] / kmz[k];
+                    s->Chyex[x] = s->Chyex[x] / kmz[k];
+                }
+            }
+
+    // ---- lumped z-sources, defineSourceZ H:1391-1421 (after the CPML, H:4264)
+    s->nsrc = sc->nsrc;
+    s->src_of.assign(N, -1);
+    s->vpn.assign((size_t)sc->nsrc * (size_t)sc->nsteps, (R)0);
+    for (int q = 0; q < sc->nsrc; ++q) {
+        const int *b = sc->src + 6 * q;
+        const R _is = (R)b[0], _js = (R)b[1], _ks = (R)b[2], _ie = (R)b[3], _je = (R)b[4], _ke = (R)b[5];
+        R r_comp = (((R)1 + _ie) - _is) * (((R)1 + _je) - _js) / (_ke - _ks);
+        r_comp = r_comp * (R)sc->src_R[q];
+        const R rfactor = (dt * dz) / (r_comp * dx * dy);
+        const R rfactor_aux = r_comp * dx * dy;
+        for (int k = b[2]; k < b[5]; ++k)
+            for (int j = b[1]; j <= b[4]; ++j)
+                for (int i = b[0]; i <= b[3]; ++i) {
+                    i64 x = s->id(i, j, k);
+                    s->src_of[x] = q;
+                    R D = two * eps[2][x] * eps_0 + dt * se[2][x] + rfactor;
+                    s->Ceze[x]  = (two * eps[2][x] * eps_0 - dt * se[2][x] - rfactor) / D;
+                    s->Cezhy[x] = (two * dt / dx) / D;
+                    s->Cezhx[x] = ((R)-2 * dt / dy) / D;
+                    s->Cezj[x]  = ((R)-2 * dt / rfactor_aux) / D;
+                }
+        // per-node waveform, setGauss H:1495-1498 (v_mag_factor = ke - ks, H:3596)
+        const int vmag = b[5] - b[2];
+        for (int t = 0; t < sc->nsteps; ++t) {
+            double v = waveform<R>(sc->src_wtype[q], sc->src_wpar + 4 * q, (double)t * sc->dt);
+            s->vpn[(size_t)q * sc->nsteps + t] = (R)v / (R)vmag;
+        }
+    }
+
+    // ---- probes and DFT
+    s->nprobe = sc->nprobe;
+    s->nfreq = sc->nfreq;
+    for (int p = 0; p < sc->nprobe; ++p) {
+        const int *q = sc->probe + 4 * p;
+        s->pcomp.push_back(q[0]);
+        s->pidx.push_back((int)s->id(q[1], q[2], q[3]));
+    }
+    s->phist.assign((size_t)sc->nprobe * sc->nsteps, 0.0);
+    s->dft.assign((size_t)sc->nprobe * sc->nfreq * 2, 0.0);
+    s->ph_re.assign((size_t)sc->nsteps * sc->nfreq, 0.0);
+    s->ph_im.assign((size_t)sc->nsteps * sc->nfreq, 0.0);
+    for (int t = 0; t < sc->nsteps; ++t)
+        for (int f = 0; f < sc->nfreq; ++f) {
+            double th = 2.0 * M_PI * sc->freqs[f] * ((double)t * sc->dt);
+            s->ph_re[(size_t)t * sc->nfreq + f] = std::cos(th) * sc->dt;
+            s->ph_im[(size_t)t * sc->nfreq + f] = -std::sin(th) * sc->dt;
+        }
+
+    Z(s->Ex); Z(s->Ey); Z(s->Ez); Z(s->Hx); Z(s->Hy); Z(s->Hz);
+    return s;
+}
+
+template <class R>
+static void sample(Sim<R> *s, int lo, int hi, int m)
+{
+    for (int p = 0; p < s->nprobe; ++p) {
+        int c = s->pcomp[p];
+        if (c < lo || c > hi) continue;
+        const std::vector<R> *F[6] = {&s->Ex, &s->Ey, &s->Ez, &s->Hx, &s->Hy, &s->Hz};
+        double v = (double)(*F[c])[s->pidx[p]];
+        s->phist[(size_t)p * s->nsteps + m] = v;
+        for (int f = 0; f < s->nfreq; ++f) {
+            double *acc = &s->dft[((size_t)p * s->nfreq + f) * 2];
+            acc[0] = acc[0] + v * s->ph_re[(size_t)m * s->nfreq + f];
+            acc[1] = acc[1] + v * s->ph_im[(size_t)m * s->nfreq + f];
+        }
+    }
+}
+
+// calc_e, H:601-911, for every cell of the single domain.
+template <class R>
+static void step_e(Sim<R> *s, int m)
+{
+    const int NX = s->NX, NY = s->NY, NZ = s->NZ;
+    const i64 NXY = s->NXY;
+    R *Ex = s->Ex.data(), *Ey = s->Ey.data(), *Ez = s->Ez.data();
+    const R *Hx = s->Hx.data(), *Hy = s->Hy.data(), *Hz = s->Hz.data();
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                const i64 x = s->id(i, j, k);
+                if (i < NX - 1 && j < NY - 1 && k < NZ - 1 && j > 0 && k > 0)          // H:665-680
+                    Ex[x] = (s->Cexe[x] * Ex[x] + s->Cexhz[x] * (Hz[x] - Hz[x - NX])) + s->Cexhy[x] * (Hy[x] - Hy[x - NXY]);
+                if (i < NX - 1 && j < NY - 1 && k < NZ - 1 && k > 0 && i > 0)          // H:694-708
+                    Ey[x] = (s->Ceye[x] * Ey[x] + s->Ceyhx[x] * (Hx[x] - Hx[x - NXY])) + s->Ceyhz[x] * (Hz[x] - Hz[x - 1]);
+                if (i < NX - 1 && j < NY - 1 && k < NZ - 1 && j > 0 && i > 0) {        // H:719-735
+                    R v = (s->Ceze[x] * Ez[x] + s->Cezhy[x] * (Hy[x] - Hy[x - 1])) + s->Cezhx[x] * (Hx[x] - Hx[x - NX]);
+                    int q = s->src_of[x];
+                    if (q >= 0) v = v + s->Cezj[x] * s->vpn[(size_t)q * s->nsteps + m];
+                    Ez[x] = v;
+                }
+            }
+    sample(s, 0, 2, m);                                                               // H:740
+    const int Lxn = s->L[0], Lxp = s->L[1], Lyn = s->L[2], Lyp = s->L[3], Lzn = s->L[4], Lzp = s->L[5];
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                const i64 x = s->id(i, j, k);
+                // x_n, x_p: H:752-797
+                if (((i > 0 && i < Lxn) || (i > NX - Lxp - 1 && i < NX - 1)) && j < NY - 1 && k < NZ - 1) {
+                    s->Psi_eyx[x] = s->b_ex[i] * s->Psi_eyx[x] + s->a_ex[i] * (Hz[x] - Hz[x - 1]);
+                    s->Psi_ezx[x] = s->b_ex[i] * s->Psi_ezx[x] + s->a_ex[i] * (Hy[x] - Hy[x - 1]);
+                    if (k > 0) Ey[x] = Ey[x] + s->cpsi_eyx[x] * s->Psi_eyx[x];
+                    if (j > 0) Ez[x] = Ez[x] + s->cpsi_ezx[x] * s->Psi_ezx[x];
+                }
+                // y_n, y_p: H:801-847
+                if (i < NX - 1 && ((j > 0 && j < Lyn) || (j > NY - Lyp - 1 && j < NY - 1)) && k < NZ - 1) {
+                    s->Psi_exy[x] = s->b_ey[j] * s->Psi_exy[x] + s->a_ey[j] * (Hz[x] - Hz[x - NX]);
+                    s->Psi_ezy[x] = s->b_ey[j] * s->Psi_ezy[x] + s->a_ey[j] * (Hx[x] - Hx[x - NX]);
+                    if (k > 0) Ex[x] = Ex[x] + s->cpsi_exy[x] * s->Psi_exy[x];
+                    if (i > 0) Ez[x] = Ez[x] + s->cpsi_ezy[x] * s->Psi_ezy[x];
+                }
+                // z_n, z_p: H:851-911
+                if (i < NX - 1 && j < NY - 1 && ((k > 0 && k < Lzn) || (k > NZ - Lzp - 1 && k < NZ - 1))) {
+                    s->Psi_exz[x] = s->b_ez[k] * s->Psi_exz[x] + s->a_ez[k] * (Hy[x] - Hy[x - NXY]);
+                    s->Psi_eyz[x] = s->b_ez[k] * s->Psi_eyz[x] + s->a_ez[k] * (Hx[x] - Hx[x - NXY]);
+                    if (j > 0) Ex[x] = Ex[x] + s->cpsi_exz[x] * s->Psi_exz[x];
+                    if (i > 0) Ey[x] = Ey[x] + s->cpsi_eyz[x] * s->Psi_eyz[x];
+                }
+            }
+}
+
+// calc_h, H:273-562.
+template <class R>
+static void step_h(Sim<R> *s, int m)
+{
+    const int NX = s->NX, NY = s->NY, NZ = s->NZ;
+    const i64 NXY = s->NXY;
+    const R *Ex = s->Ex.data(), *Ey = s->Ey.data(), *Ez = s->Ez.data();
+    R *Hx = s->Hx.data(), *Hy = s->Hy.data(), *Hz = s->Hz.data();
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                const i64 x = s->id(i, j, k);
+                if (i < NX && j < NY - 1 && k < NZ - 1)                                  // H:328-342
+                    Hx[x] = (s->Chxh[x] * Hx[x] + s->Chxey[x] * (Ey[x + NXY] - Ey[x])) + s->Chxez[x] * (Ez[x + NX] - Ez[x]);
+                if (i < NX - 1 && j < NY && k < NZ - 1)                                  // H:356-368
+                    Hy[x] = (s->Chyh[x] * Hy[x] + s->Chyez[x] * (Ez[x + 1] - Ez[x])) + s->Chyex[x] * (Ex[x + NXY] - Ex[x]);
+                if (i < NX - 1 && j < NY - 1 && k < NZ)                                  // H:381-386
+                    Hz[x] = (s->Chzh[x] * Hz[x] + s->Chzex[x] * (Ex[x + NX] - Ex[x])) + s->Chzey[x] * (Ey[x + 1] - Ey[x]);
+            }
+    sample(s, 3, 5, m);                                                               // H:347-391
+    const int Lxn = s->L[0], Lxp = s->L[1], Lyn = s->L[2], Lyp = s->L[3], Lzn = s->L[4], Lzp = s->L[5];
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                const i64 x = s->id(i, j, k);
+                // x_n, x_p: H:401-442
+                if ((i < Lxn || (i > NX - Lxp - 1 && i < NX - 1)) && j < NY && k < NZ) {
+                    s->Psi_hyx[x] = s->b_mx[i] * s->Psi_hyx[x] + s->a_mx[i] * (Ez[x + 1] - Ez[x]);
+                    s->Psi_hzx[x] = s->b_mx[i] * s->Psi_hzx[x] + s->a_mx[i] * (Ey[x + 1] - Ey[x]);
+                    if (k < NZ - 1) Hy[x] = Hy[x] + s->cpsi_hyx[x] * s->Psi_hyx[x];
+                    if (j < NY - 1) Hz[x] = Hz[x] + s->cpsi_hzx[x] * s->Psi_hzx[x];
+                }
+                // y_n, y_p: H:446-495
+                if (i < NX && (j < Lyn || (j > NY - Lyp - 1 && j < NY - 1)) && k < NZ) {
+                    s->Psi_hxy[x] = s->b_my[j] * s->Psi_hxy[x] + s->a_my[j] * (Ez[x + NX] - Ez[x]);
+                    s->Psi_hzy[x] = s->b_my[j] * s->Psi_hzy[x] + s->a_my[j] * (Ex[x + NX] - Ex[x]);
+                    if (k < NZ - 1) Hx[x] = Hx[x] + s->cpsi_hxy[x] * s->Psi_hxy[x];
+                    if (i < NX - 1) Hz[x] = Hz[x] + s->cpsi_hzy[x] * s->Psi_hzy[x];
+                }
+                // z_n, z_p: H:499-562
+                if (i < NX && j < NY && (k < Lzn || (k > NZ - Lzp - 1 && k < NZ - 1))) {
+                    s->Psi_hxz[x] = s->b_mz[k] * s->Psi_hxz[x] + s->a_mz[k] * (Ey[x + NXY] - Ey[x]);
+                    s->Psi_hyz[x] = s->b_mz[k] * s->Psi_hyz[x] + s->a_mz[k] * (Ex[x + NXY] - Ex[x]);
+                    if (j < NY - 1) Hx[x] = Hx[x] + s->cpsi_hxz[x] * s->Psi_hxz[x];
+                    if (i < NX - 1) Hy[x] = Hy[x] + s->cpsi_hyz[x] * s->Psi_hyz[x];
+                }
+            }
+}
+
+template <class R>
+static const std::vector<R> *lookup(Sim<R> *s, const std::string &n)
+{
+#define O(name) if (n == #name) return &s->name;
+    O(Ex) O(Ey) O(Ez) O(Hx) O(Hy) O(Hz)
+    O(Cexe) O(Cexhz) O(Cexhy) O(Ceye) O(Ceyhx) O(Ceyhz) O(Ceze) O(Cezhy) O(Cezhx) O(Cezj)
+    O(Chxh) O(Chxey) O(Chxez) O(Chyh) O(Chyez) O(Chyex) O(Chzh) O(Chzex) O(Chzey)
+    O(b_ex) O(a_ex) O(b_mx) O(a_mx) O(b_ey) O(a_ey) O(b_my) O(a_my) O(b_ez) O(a_ez) O(b_mz) O(a_mz)
+    O(Psi_eyx) O(Psi_ezx) O(Psi_hyx) O(Psi_hzx) O(cpsi_eyx) O(cpsi_ezx) O(cpsi_hyx) O(cpsi_hzx)
+    O(Psi_exy) O(Psi_ezy) O(Psi_hxy) O(Psi_hzy) O(cpsi_exy) O(cpsi_ezy) O(cpsi_hxy) O(cpsi_hzy)
+    O(Psi_exz) O(Psi_eyz) O(Psi_hxz) O(Psi_hyz) O(cpsi_exz) O(cpsi_eyz) O(cpsi_hxz) O(cpsi_hyz)
+    O(vpn)
+#undef O
+    return nullptr;
+}
+
+struct Handle {
+    int prec;
+    void *sim;
+};
+
+template <class R>
+static int steps_t(Sim<R> *s, int n)
+{
+    for (int t = 0; t < n; ++t) {
+        if (s->step >= s->nsteps) return -3;
+        step_e(s, s->step);
+        step_h(s, s->step);
+        ++s->step;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+/* precision: 32 or 64.  Returns NULL on an invalid scene. */
+void *oracle_create(const oracle_scene *sc, int precision)
+{
+    if (!sc || sc->nx < 3 || sc->ny < 3 || sc->nz < 3 || sc->nsteps < 0) return nullptr;
+    Handle *h = new Handle();
+    h->prec = precision;
+    if (precision == 32) h->sim = build<float>(sc);
+    else if (precision == 64) h->sim = build<double>(sc);
+    else { delete h; return nullptr; }
+    return h;
+}
+
+void oracle_destroy(void *hp)
+{
+    Handle *h = (Handle *)hp;
+    if (!h) return;
+    if (h->prec == 32) delete (Sim<float> *)h->sim;
+    else delete (Sim<double> *)h->sim;
+    delete h;
+}
+
+/* Advance n steps (E then H). */
+int oracle_steps(void *hp, int n)
+{
+    Handle *h = (Handle *)hp;
+    return h->prec == 32 ? steps_t((Sim<float> *)h->sim, n) : steps_t((Sim<double> *)h->sim, n);
+}
+
+int oracle_step_count(void *hp)
+{
+    Handle *h = (Handle *)hp;
+    return h->prec == 32 ? ((Sim<float> *)h->sim)->step : ((Sim<double> *)h->sim)->step;
+}
+
+/* Probe history [probe][nsteps] (all nsteps columns; unrun steps are 0). */
+int oracle_get_probes(void *hp, double *out)
+{
+    Handle *h = (Handle *)hp;
+    const std::vector<double> &v = h->prec == 32 ? ((Sim<float> *)h->sim)->phist : ((Sim<double> *)h->sim)->phist;
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+    return 0;
+}
+
+int oracle_get_dft(void *hp, double *out)
+{
+    Handle *h = (Handle *)hp;
+    const std::vector<double> &v = h->prec == 32 ? ((Sim<float> *)h->sim)->dft : ((Sim<double> *)h->sim)->dft;
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+    return 0;
+}
+
+/* Copy a named internal array (fields, coefficients, profiles, psi, cpsi,
+ * vpn) into out as doubles.  Returns its length, or -1 if unknown. */
+long oracle_get_array(void *hp, const char *name, double *out)
+{
+    Handle *h = (Handle *)hp;
+    std::string n(name);
+    if (h->prec == 32) {
+        const std::vector<float> *v = lookup((Sim<float> *)h->sim, n);
+        if (!v) return -1;
+        if (out) for (size_t i = 0; i < v->size(); ++i) out[i] = (*v)[i];
+        return (long)v->size();
+    }
+    const std::vector<double> *v = lookup((Sim<double> *)h->sim, n);
+    if (!v) return -1;
+    if (out) for (size_t i = 0; i < v->size(); ++i) out[i] = (*v)[i];
+    return (long)v->size();
+}
+
+/* Overwrite a named field array (tests: linearity, dot-product checks). */
+long oracle_set_array(void *hp, const char *name, const double *in)
+{
+    Handle *h = (Handle *)hp;
+    std::string n(name);
+    if (h->prec == 32) {
+        std::vector<float> *v = const_cast<std::vector<float> *>(lookup((Sim<float> *)h->sim, n));
+        if (!v) return -1;
+        for (size_t i = 0; i < v->size(); ++i) (*v)[i] = (float)in[i];
+        return (long)v->size();
+    }
+    std::vector<double> *v = const_cast<std::vector<double> *>(lookup((Sim<double> *)h->sim, n));
+    if (!v) return -1;
+    for (size_t i = 0; i < v->size(); ++i) (*v)[i] = in[i];
+    return (long)v->size();
+}
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker.  The engine never uses it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OracleScene(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("dt", C.c_double), ("eps0", C.c_double), ("mu0", C.c_double),
+                ("pml", C.c_int * 6), ("order", C.c_double), ("sigma_max", C.c_double * 3),
+                ("kappa_max", C.c_double), ("alpha_max", C.c_double), ("alpha_min", C.c_double),
+                ("nbox", C.c_int), ("box", C.POINTER(C.c_int)), ("box_mat", C.POINTER(C.c_float)),
+                ("nsrc", C.c_int), ("src", C.POINTER(C.c_int)), ("src_R", C.POINTER(C.c_double)),
+                ("src_wtype", C.POINTER(C.c_int)), ("src_wpar", C.POINTER(C.c_double)),
+                ("nprobe", C.c_int), ("probe", C.POINTER(C.c_int)),
+                ("nfreq", C.c_int), ("freqs", C.POINTER(C.c_double)),
+                ("design", C.c_int * 6), ("design_eps", C.POINTER(C.c_float)),
+                ("nsteps", C.c_int)]
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = C.CDLL(LIB_PATH)
+        lib.oracle_create.restype = C.c_void_p
+        lib.oracle_create.argtypes = [C.POINTER(OracleScene), C.c_int]
+        lib.oracle_destroy.argtypes = [C.c_void_p]
+        lib.oracle_steps.argtypes = [C.c_void_p, C.c_int]
+        lib.oracle_step_count.argtypes = [C.c_void_p]
+        lib.oracle_get_probes.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        lib.oracle_get_dft.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        lib.oracle_get_array.restype = C.c_long
+        lib.oracle_get_array.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]
+        lib.oracle_set_array.restype = C.c_long
+        lib.oracle_set_array.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]
+        _lib = lib
+    return _lib
+
+
+def _sigma_max(sc):
+    """Resolve the CPML sigma_max the way the engine does (<= 0: reference
+    default sigma_factor (m+1) / (150 pi d), fdtdrav.h:1783-1798)."""
+    out = []
+    for a, d in enumerate((sc.dx, sc.dy, sc.dz)):
+        s = sc.cpml.sigma_max[a]
+        if not s > 0:
+            s = sc.cpml.sigma_factor * (sc.cpml.order + 1.0) / (150.0 * np.pi * d)
+        out.append(s)
+    return out
+
+
+class Oracle:
+    """CPU restatement of the reference for one Scene, FP32 or FP64."""
+
+    def __init__(self, sc, precision=64, nsteps=None):
+        lib = load()
+        self.sc = sc
+        self.nsteps = int(nsteps if nsteps is not None else sc.steps)
+        # keep every array alive for the lifetime of the handle
+        self._keep = []
+
+        def arr(a, ctype, np_dtype):
+            a = np.ascontiguousarray(a, dtype=np_dtype)
+            if a.size == 0:
+                a = np.zeros(1, dtype=np_dtype)
+            self._keep.append(a)
+            return a.ctypes.data_as(C.POINTER(ctype))
+
+        boxes = [list(b.ijk) for b in sc.boxes]
+        mats = [list(b.eps_r) + list(b.sigma_e) + list(b.mu_r) + list(b.sigma_m) for b in sc.boxes]
+        src = [list(s.ijk) for s in sc.sources]
+        wpar = [[s.amp, s.t0, s.tau, s.freq] for s in sc.sources]
+        os_ = OracleScene()
+        os_.nx, os_.ny, os_.nz = sc.nx, sc.ny, sc.nz
+        os_.dx, os_.dy, os_.dz, os_.dt = sc.dx, sc.dy, sc.dz, sc.dt
+        os_.eps0, os_.mu0 = sc.eps0, sc.mu0
+        os_.pml = (C.c_int * 6)(*sc.cpml.thick)
+        os_.order = sc.cpml.order
+        os_.sigma_max = (C.c_double * 3)(*_sigma_max(sc))
+        os_.kappa_max, os_.alpha_max, os_.alpha_min = sc.cpml.kappa_max, sc.cpml.alpha_max, sc.cpml.alpha_min
+        os_.nbox = len(boxes)
+        os_.box = arr(np.array(boxes, dtype=np.int32).ravel(), C.c_int, np.int32)
+        os_.box_mat = arr(np.array(mats, dtype=np.float32).ravel(), C.c_float, np.float32)
+        os_.nsrc = len(src)
+        os_.src = arr(np.array(src, dtype=np.int32).ravel(), C.c_int, np.int32)
+        os_.src_R = arr([s.R for s in sc.sources], C.c_double, np.float64)
+        os_.src_wtype = arr([s.wtype for s in sc.sources], C.c_int, np.int32)
+        os_.src_wpar = arr(np.array(wpar, dtype=np.float64).ravel(), C.c_double, np.float64)
+        os_.nprobe = len(sc.probes)
+        os_.probe = arr(np.array(sc.probes, dtype=np.int32).ravel(), C.c_int, np.int32)
+        os_.nfreq = len(sc.freqs)
+        os_.freqs = arr(sc.freqs, C.c_double, np.float64)
+        if sc.design is not None and sc.design_eps is not None:
+            os_.design = (C.c_int * 6)(*sc.design)
+            os_.design_eps = arr(sc.design_eps, C.c_float, np.float32)
+        else:
+            os_.design = (C.c_int * 6)(0, 0, 0, 0, 0, 0)
+            os_.design_eps = C.POINTER(C.c_float)()
+        os_.nsteps = self.nsteps
+        self._scene = os_
+        self.lib = lib
+        self.h = lib.oracle_create(C.byref(os_), int(precision))
+        if not self.h:
+            raise ValueError("oracle rejected the scene")
+
+    def close(self):
+        if self.h:
+            self.lib.oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def steps(self, n):
+        rc = self.lib.oracle_steps(self.h, int(n))
+        if rc != 0:
+            raise RuntimeError(f"oracle_steps failed ({rc})")
+
+    @property
+    def steps_done(self):
+        return self.lib.oracle_step_count(self.h)
+
+    def probes(self):
+        out = np.zeros((len(self.sc.probes), self.nsteps), dtype=np.float64)
+        self.lib.oracle_get_probes(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out[:, :self.steps_done]
+
+    def dft(self):
+        out = np.zeros((len(self.sc.probes), len(self.sc.freqs), 2), dtype=np.float64)
+        self.lib.oracle_get_dft(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out[..., 0] + 1j * out[..., 1]
+
+    def array(self, name):
+        n = self.lib.oracle_get_array(self.h, name.encode(), None)
+        if n < 0:
+            raise KeyError(name)
+        out = np.zeros(n, dtype=np.float64)
+        self.lib.oracle_get_array(self.h, name.encode(), out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def field(self, name):
+        return self.array(name).reshape(self.sc.nz, self.sc.ny, self.sc.nx)
+
+    def set_array(self, name, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        n = self.lib.oracle_set_array(self.h, name.encode(), v.ctypes.data_as(C.POINTER(C.c_double)))
+        if n < 0:
+            raise KeyError(name)

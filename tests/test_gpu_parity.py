@@ -1,0 +1,146 @@
+"""GPU parity: the HIP engine against the CPU oracle on the same seeded scenes.
+
+FP32 storage must reproduce the oracle's FP32 restatement of the reference
+bit for bit (same operations, same order, no contraction) and the FP64
+restatement within 1e-5 relative L2 on every probe series.
+"""
+import numpy as np
+import pytest
+
+from fdtd_amd import scenes
+from fdtd_amd.scenes import EX, EY, EZ, HX, HY, HZ
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz"]
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.linalg.norm(a - b, axis=-1) / np.maximum(np.linalg.norm(b, axis=-1), 1e-300)
+
+
+def scene_vacuum():
+    sc = scenes.yee64(n=24, steps=120, L=5, n_probes=4, seed=11)
+    # probes of every component, one inside the CPML and one on a face
+    sc.probes += [(EX, 12, 12, 12), (EY, 12, 12, 12), (HX, 12, 12, 12), (HY, 13, 11, 12),
+                  (HZ, 11, 13, 12), (EZ, 2, 12, 12), (EZ, 0, 5, 5)]
+    return sc
+
+
+def scene_dielectric():
+    sc = scenes.dielectric(nx=40, ny=36, nz=32, steps=150, L=6, n_boxes=6, n_src=2, n_probes=8,
+                           n_freqs=4, seed=21, name="dielectric-small")
+    sc.cpml = scenes.graded_cpml((5, 6, 4, 7, 3, 5), sc.dx)
+    return sc
+
+
+def scene_magnetic():
+    """Anisotropic and magnetic lossy boxes (classed H sweep), three waveform
+    types, a two-edge source, the reference's default CPML grading."""
+    d = 1e-3
+    dt = scenes.courant_dt(d, d, d, 0.9)
+    sc = scenes.Scene(name="magnetic", nx=30, ny=28, nz=26, dx=d, dy=d, dz=d, dt=dt, steps=140,
+                      cpml=scenes.Cpml(thick=(5, 5, 5, 5, 5, 5)))
+    sc.boxes = [scenes.Box((8, 7, 6, 20, 18, 16), (2.0, 3.0, 4.0), (0.01, 0.0, 0.02), (1.5, 2.0, 1.0),
+                           (100.0, 0.0, 50.0)),
+                scenes.Box((14, 12, 10, 24, 22, 20), (5.0, 5.0, 5.0), (0.0, 0.0, 0.0), (3.0, 3.0, 3.0),
+                           (0.0, 0.0, 0.0))]
+    sc.sources = [scenes.Source((10, 10, 9, 10, 10, 11), 50.0, scenes.WAVE_GAUSSIAN, 1.0, 30 * dt, 8 * dt),
+                  scenes.Source((18, 16, 12, 18, 16, 13), 75.0, scenes.WAVE_SINE, 0.5, 0.0, 1.0, 20e9),
+                  scenes.z_edge(12, 20, 14, dt)]
+    sc.probes = [(EZ, 15, 14, 13), (HX, 15, 14, 13), (HY, 16, 12, 11), (HZ, 9, 9, 9), (EX, 20, 20, 20),
+                 (EY, 7, 15, 18)]
+    sc.freqs = [5e9, 12e9]
+    return sc
+
+
+def scene_design_u32():
+    """A design region of per-cell eps_r large enough for 32-bit classes."""
+    return scenes.adjoint(nx=56, ny=56, nz=44, steps=60, L=4, d=1e-3, design=(48, 48, 36),
+                          eps_range=(1.0, 4.0), seed=31, name="design-u32")
+
+
+SCENES = {"vacuum": scene_vacuum, "dielectric": scene_dielectric, "magnetic": scene_magnetic,
+          "design_u32": scene_design_u32}
+
+
+@pytest.fixture(scope="module")
+def oracle_mod():
+    from oracle import oracle
+    return oracle
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_fp32_bitwise_and_fp64_parity(name, engine, oracle_mod):
+    sc = SCENES[name]()
+    print(f"[parity] scene={name} grid={sc.nx}x{sc.ny}x{sc.nz} steps={sc.steps}", flush=True)
+    sim = engine.Simulation.from_scene(sc, "f32")
+    info = sim.info()
+    sim.run(sc.steps)
+    gp = sim.probes()
+    o32 = oracle_mod.Oracle(sc, 32)
+    o32.steps(sc.steps)
+    o64 = oracle_mod.Oracle(sc, 64)
+    o64.steps(sc.steps)
+    p32, p64 = o32.probes(), o64.probes()
+    assert gp.shape == p32.shape
+    # bit-exact against the FP32 restatement, probes and whole fields
+    mism = np.flatnonzero(gp.astype(np.float64).ravel() != p32.ravel())
+    assert mism.size == 0, f"{mism.size} probe samples differ, first at {mism[:5]}"
+    for c, nm in enumerate(FIELDS):
+        g = sim.field(c)
+        o = o32.field(nm).astype(np.float32)
+        bad = np.flatnonzero(g.ravel() != o.ravel())
+        assert bad.size == 0, f"{nm}: {bad.size} cells differ, max |d| = {np.abs(g - o).max()}"
+    # FP64 restatement: 1e-5 relative L2 on every probe series that carries
+    # signal; a series that is zero by symmetry (e.g. Hz beside a centred
+    # z-source) is held to 1e-5 of the largest series instead.
+    n64 = np.linalg.norm(p64, axis=1)
+    sig = n64 >= 1e-3 * n64.max()
+    r = rel_l2(gp[sig], p64[sig])
+    assert np.all(r <= 1e-5), f"relative L2 vs FP64 {r.max():.3e}"
+    dn = np.linalg.norm(gp.astype(np.float64) - p64, axis=1)
+    assert np.all(dn[~sig] <= 1e-5 * n64.max())
+    if sc.freqs:
+        gd, od = sim.dft(), o32.dft()
+        np.testing.assert_array_equal(gd, od)
+    if name == "design_u32":
+        assert info["n_classes"] > 65536
+    sim.close()
+
+
+def test_resume_from_checkpoint_is_bitwise(engine):
+    sc = scene_dielectric()
+    a = engine.Simulation.from_scene(sc, "f32")
+    a.run(sc.steps)
+    ref = a.probes()
+    ref_ez = a.field(EZ)
+    b = engine.Simulation.from_scene(sc, "f32")
+    b.run(60)
+    blob = b.checkpoint_save()
+    b.run(30)            # diverge, then rewind
+    b.checkpoint_restore(blob)
+    assert b.info()["steps_done"] == 60
+    b.run(sc.steps - 60)
+    np.testing.assert_array_equal(b.probes(), ref)
+    np.testing.assert_array_equal(b.field(EZ), ref_ez)
+    # a flipped byte is caught by the checksum
+    bad = bytearray(blob)
+    bad[len(bad) // 2] ^= 0x40
+    with pytest.raises(engine.FdtdError) as ei:
+        b.checkpoint_restore(bytes(bad))
+    assert ei.value.code == -8
+
+
+def test_reset_and_profile_reproduce_run(engine):
+    sc = scene_vacuum()
+    sim = engine.Simulation.from_scene(sc, "f32")
+    sim.run(sc.steps)
+    ref = sim.probes()
+    sim.reset()
+    sim.profile(sc.steps)
+    np.testing.assert_array_equal(sim.probes(), ref)
+    inf = sim.info()
+    assert inf["e_kernel_ms"] > 0 and inf["h_kernel_ms"] > 0
