@@ -55,7 +55,8 @@ def run_form(sc, storage, steps, warmup, profile_steps):
     wall = time.perf_counter() - t0
     inf = sim.info()
     ev_ms = inf["last_run_ms"]
-    sim.profile(profile_steps)
+    if profile_steps > 0:
+        sim.profile(profile_steps)
     prof = sim.info()
     probes = sim.probes()[:, warmup:warmup + steps]
     sim.close()
@@ -93,7 +94,20 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=50)
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only time the plain HBM copy kernel (measurement calibration)")
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30)
+    ap.add_argument("--copy-iters", type=int, default=20)
     args = ap.parse_args()
+    if args.copy_probe:
+        import ctypes
+        lib = ctypes.CDLL(os.path.join(ROOT, "fdtd_amd", "libhbmcopy.so"))
+        lib.hbm_copy_gbs.restype = ctypes.c_double
+        lib.hbm_copy_gbs.argtypes = [ctypes.c_uint64, ctypes.c_int]
+        gbs = lib.hbm_copy_gbs(args.copy_bytes, args.copy_iters)
+        print(json.dumps({"copy_gbs": round(gbs, 1), "bytes_each_way": args.copy_bytes,
+                          "iters": args.copy_iters, "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
               file=sys.stderr)
@@ -108,7 +122,7 @@ def main():
     dom_e = f32["e_ms"] >= f32["h_ms"]
     kms = f32["e_ms"] if dom_e else f32["h_ms"]
     bpc = be if dom_e else bh
-    achieved = bpc * cells / (kms * 1e-3) / 1e9
+    achieved = bpc * cells / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     step_gbs = (be + bh) * cells / (ms_per_step * 1e-3) / 1e9
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "Mcell-updates/s", "n_gpus": 1,
@@ -125,8 +139,8 @@ def main():
                      "bytes_per_cell": round(bpc, 3), "kernel_ms": round(kms, 5)},
         "kernels": {"e_ms": round(f32["e_ms"], 5), "h_ms": round(f32["h_ms"], 5),
                     "b_alg_e": round(be, 3), "b_alg_h": round(bh, 3),
-                    "e_frac": round(be * cells / (f32["e_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                    "h_frac": round(bh * cells / (f32["h_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "e_frac": round(be * cells / (max(f32["e_ms"], 1e-9) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "h_frac": round(bh * cells / (max(f32["h_ms"], 1e-9) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     "step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "wall_s": round(f32["wall_s"], 3),
     }
@@ -143,8 +157,8 @@ def main():
                        "probe_rel_l2_vs_fp32_max": float(f"{rel.max():.3e}"),
                        "probe_rel_l2_vs_fp32_median": float(f"{np.median(rel):.3e}"),
                        "e_ms": round(f16["e_ms"], 5), "h_ms": round(f16["h_ms"], 5),
-                       "e_frac": round(e16 * cells / (f16["e_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                       "h_frac": round(h16 * cells / (f16["h_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                       "e_frac": round(e16 * cells / (max(f16["e_ms"], 1e-9) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "h_frac": round(h16 * cells / (max(f16["h_ms"], 1e-9) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(out), flush=True)

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfdtd_amd.so")
-SOURCES = ["api.hip", "kernels.hip", "scene.cpp"]
+SOURCES = ["api.hip", "kernels.hip", "adjoint.hip", "scene.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -29,17 +29,25 @@ def hipcc():
     return path if os.path.exists(path) else "hipcc"
 
 
-def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
+COPY_LIB = os.path.join(HERE, "libhbmcopy.so")
+
+
+def _hipcc(out, srcs, verbose):
+    tmp = out + ".tmp"
     cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + srcs
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libfdtd_amd.so")
-    os.replace(tmp, LIB)
+        raise RuntimeError(f"hipcc failed building {os.path.basename(out)}")
+    os.replace(tmp, out)
+
+
+def build(verbose=False):
+    """Build the engine library and the copy-kernel measurement library."""
+    _hipcc(LIB, [os.path.join(CSRC, s) for s in SOURCES], verbose)
+    _hipcc(COPY_LIB, [os.path.join(CSRC, "hbm_copy.hip")], verbose)
     return LIB
 
 

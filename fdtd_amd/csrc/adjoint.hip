@@ -1,0 +1,419 @@
+// adjoint.hip -- reverse-time (discrete adjoint) kernels for gfx950.
+//
+// The forward time step is linear in (E, H, psi_e, psi_h), so its adjoint is
+// the exact transpose.  Reverse step m, for the state after forward step m:
+//   adj_e_kernel  (E cells, backward differences, like the forward E sweep)
+//       [finalize of the E half-step of step m+1: psi_e adjoint, dJ/deps on
+//        the design edges, dJ/dE <- Ce dJ/dE]  then the transpose of the H
+//        half-step of step m gathered into dJ/dE, then dJ/d(probe sample m).
+//   adj_src_fixup (one thread) the lumped-source edges' own Ez coefficients.
+//   adj_h_kernel  (H cells, forward differences, like the forward H sweep)
+//       [dJ/dH <- Ch dJ/dH, psi_h adjoint] then the transpose of the E
+//        half-step of step m gathered into dJ/dH.
+// Every transposed term is a gather, so no two threads write one address and
+// no atomics are needed.  Forward operators: fdtdrav.h:665-911 (E) and
+// fdtdrav.h:328-562 (H); edge averaging fdtdrav.h:1135-1177.
+#include <hip/hip_runtime.h>
+#include <climits>
+#include <cstdint>
+
+#include "device_common.h"
+#include "internal.h"
+
+namespace fdtd {
+
+namespace {
+
+const float4 NOPROF = {0.f, 0.f, 1.f, 0.f};
+
+__device__ __forceinline__ bool xs_h(const KArgs &a, int i) { return i < a.L[0] || (i >= a.nx - a.L[1] && i < a.nx - 1); }
+__device__ __forceinline__ bool ys_h(const KArgs &a, int j) { return j < a.L[2] || (j >= a.ny - a.L[3] && j < a.ny - 1); }
+__device__ __forceinline__ bool zs_h(const KArgs &a, int k) { return k < a.L[4] || (k >= a.nz - a.L[5] && k < a.nz - 1); }
+__device__ __forceinline__ bool xs_e(const KArgs &a, int i) { return (i > 0 && i < a.L[0]) || (i >= a.nx - a.L[1] && i < a.nx - 1); }
+__device__ __forceinline__ bool ys_e(const KArgs &a, int j) { return (j > 0 && j < a.L[2]) || (j >= a.ny - a.L[3] && j < a.ny - 1); }
+__device__ __forceinline__ bool zs_e(const KArgs &a, int k) { return (k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && k < a.nz - 1); }
+__device__ __forceinline__ int slab_x(const KArgs &a, int i) { return i < a.L[0] ? i : i - (a.nx - a.L[1]) + a.L[0]; }
+__device__ __forceinline__ int slab_y(const KArgs &a, int j) { return j < a.L[2] ? j : j - (a.ny - a.L[3]) + a.L[2]; }
+__device__ __forceinline__ int slab_z(const KArgs &a, int k) { return k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4]; }
+
+template <int CM>
+__device__ __forceinline__ void coefs(const KArgs &a, const float4 *tab, bool uni, i64 c, float4 &cx, float4 &cy,
+                                      float4 &cz)
+{
+    const int cl = uni ? 0 : class_of<CM>(a.cls, c);
+    cx = tab[cl * 3 + 0];
+    cy = tab[cl * 3 + 1];
+    cz = tab[cl * 3 + 2];
+}
+
+// Transposed H half-step quantities of H cell (i,j,k): the factors that
+// multiply each E difference it reads (Hx: dzEy -> sy, dyEz -> sz; Hy: dxEz
+// -> tz, dzEx -> tx; Hz: dyEx -> rx, dxEy -> ry), main term plus psi_h.
+struct HQ {
+    float sy, sz, tz, tx, rx, ry;
+};
+
+template <int CM>
+__device__ HQ hquant(const KArgs &a, const AArgs &b, bool huni, int i, int j, int k)
+{
+    HQ q{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
+    const i64 c = (i64)k * a.plane + (i64)j * px + i;
+    float4 hx, hy, hz;
+    coefs<CM>(a, a.htab, huni, c, hx, hy, hz);
+    const bool ux = j < ny - 1 && k < nz - 1, uy = i < nx - 1 && k < nz - 1, uz = i < nx - 1 && j < ny - 1;
+    const bool xs = xs_h(a, i), ys = ys_h(a, j), zs = zs_h(a, k);
+    const float4 pxm = xs ? a.px_prof[nx + i] : NOPROF;
+    const float4 pym = ys ? a.py_prof[ny + j] : NOPROF;
+    const float4 pzm = zs ? a.pz_prof[nz + k] : NOPROF;
+    const float hbx = b.hb[0][c], hby = b.hb[1][c], hbz = b.hb[2][c];
+    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    if (ux) {
+        q.sy = (hx.y / pzm.z) * hbx;
+        q.sz = (hx.z / pym.z) * hbx;
+    }
+    if (uy) {
+        q.tz = (hy.y / pxm.z) * hby;
+        q.tx = (hy.z / pzm.z) * hby;
+    }
+    if (uz) {
+        q.rx = (hz.y / pym.z) * hbz;
+        q.ry = (hz.z / pxm.z) * hbz;
+    }
+    if (xs) {
+        const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
+        q.tz += pxm.y * (b.pb[2][s] + (uy ? (hy.y * dx) * hby : 0.f));   // psi_hyx
+        q.ry += pxm.y * (b.pb[3][s] + (uz ? (hz.z * dx) * hbz : 0.f));   // psi_hzx
+    }
+    if (ys) {
+        const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
+        q.sz += pym.y * (b.pb[6][s] + (ux ? (hx.z * dy) * hbx : 0.f));   // psi_hxy
+        q.rx += pym.y * (b.pb[7][s] + (uz ? (hz.y * dy) * hbz : 0.f));   // psi_hzy
+    }
+    if (zs) {
+        const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
+        q.sy += pzm.y * (b.pb[10][s] + (ux ? (hx.y * dz) * hbx : 0.f));  // psi_hxz
+        q.tx += pzm.y * (b.pb[11][s] + (uy ? (hy.z * dz) * hby : 0.f));  // psi_hyz
+    }
+    return q;
+}
+
+// Transposed E half-step quantities of E cell (i,j,k): Ex: dyHz -> ax, dzHy
+// -> bx; Ey: dzHx -> ay, dxHz -> by; Ez: dxHy -> az, dyHx -> bz.
+struct EQ {
+    float ax, bx, ay, by, az, bz;
+};
+
+template <int CM>
+__device__ EQ equant(const KArgs &a, const AArgs &b, bool euni, int i, int j, int k)
+{
+    EQ q{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
+    if (i >= nx - 1 || j >= ny - 1 || k >= nz - 1) return q;   // no E update, no psi_e
+    const i64 c = (i64)k * a.plane + (i64)j * px + i;
+    float4 ex, ey, ez;
+    coefs<CM>(a, a.etab, euni, c, ex, ey, ez);
+    const bool ux = j > 0 && k > 0, uy = i > 0 && k > 0, uz = i > 0 && j > 0;
+    const bool xs = xs_e(a, i), ys = ys_e(a, j), zs = zs_e(a, k);
+    const float4 pxe = xs ? a.px_prof[i] : NOPROF;
+    const float4 pye = ys ? a.py_prof[j] : NOPROF;
+    const float4 pze = zs ? a.pz_prof[k] : NOPROF;
+    const float ebx = b.eb[0][c], eby = b.eb[1][c], ebz = b.eb[2][c];
+    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    if (ux) {
+        q.ax = (ex.y / pye.z) * ebx;
+        q.bx = (ex.z / pze.z) * ebx;
+    }
+    if (uy) {
+        q.ay = (ey.y / pze.z) * eby;
+        q.by = (ey.z / pxe.z) * eby;
+    }
+    if (uz) {
+        q.az = (ez.y / pxe.z) * ebz;
+        q.bz = (ez.z / pye.z) * ebz;
+    }
+    if (xs) {
+        const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
+        q.by += pxe.y * (b.pb[0][s] + (uy ? (ey.z * dx) * eby : 0.f));   // psi_eyx
+        q.az += pxe.y * (b.pb[1][s] + (uz ? (ez.y * dx) * ebz : 0.f));   // psi_ezx
+    }
+    if (ys) {
+        const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
+        q.ax += pye.y * (b.pb[4][s] + (ux ? (ex.y * dy) * ebx : 0.f));   // psi_exy
+        q.bz += pye.y * (b.pb[5][s] + (uz ? (ez.z * dy) * ebz : 0.f));   // psi_ezy
+    }
+    if (zs) {
+        const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
+        q.bx += pze.y * (b.pb[8][s] + (ux ? (ex.z * dz) * ebx : 0.f));   // psi_exz
+        q.ay += pze.y * (b.pb[9][s] + (uy ? (ey.y * dz) * eby : 0.f));   // psi_eyz
+    }
+    return q;
+}
+
+}  // namespace
+
+// fstep >= 0: finalize the E half-step of forward step fstep first.
+// sten: gather the transposed H half-step of step m and inject dJ/ds[m].
+template <int CM>
+__global__ void __launch_bounds__(TX * BY)
+    adj_e_kernel(const KArgs a, const AArgs b, const int m, const int fstep, const int sten, const int euni,
+                 const int huni)
+{
+    int tx, ty, tz;
+    tile_of(a, tx, ty, tz);
+    const int i = tx * TX + (int)(threadIdx.x & (TX - 1));
+    const int j = ty * BY + (int)(threadIdx.x / TX);
+    const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
+    if (i >= nx || j >= ny) return;
+    const int k0 = tz * a.kchunk;
+    const int k1 = min(nz, k0 + a.kchunk);
+    const bool ilo = i > 0, ihi = i < nx - 1, jlo = j > 0, jhi = j < ny - 1;
+    const int mycol = j * nx + i;
+    int ev = first_event(a, a.e_first, mycol, k0);
+    int ek = ev >= 0 ? a.events[ev].k : INT_MAX;
+    const bool in_box = i >= b.sb[0] && i < b.sb[0] + b.sb[3] && j >= b.sb[1] && j < b.sb[1] + b.sb[4];
+    const i64 nb = (i64)b.sb[3] * b.sb[4] * b.sb[5];
+    const bool xs = xs_e(a, i), ys = ys_e(a, j);
+    const float4 pxe = xs ? a.px_prof[i] : NOPROF;
+    const float4 pye = ys ? a.py_prof[j] : NOPROF;
+    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+
+    HQ qkm{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (sten && k0 > 0) qkm = hquant<CM>(a, b, huni, i, j, k0 - 1);
+    for (int k = k0; k < k1; ++k) {
+        const i64 c = (i64)k * a.plane + (i64)j * px + i;
+        const bool klo = k > 0, khi = k < nz - 1;
+        const bool ux = ihi && jlo && jhi && klo && khi;
+        const bool uy = ilo && ihi && jhi && klo && khi;
+        const bool uz = ilo && ihi && jlo && jhi && khi;
+        float ebx = b.eb[0][c], eby = b.eb[1][c], ebz = b.eb[2][c];
+        // source edge and probe events of this cell
+        float cez_src = 0.f;
+        bool is_src = false;
+        int pev = -1;
+        if (k == ek) {
+            pev = ev;
+            while (a.events[ev].k == k && a.events[ev].col == mycol) {
+                if (a.events[ev].kind == EV_SOURCE) {
+                    is_src = true;
+                    cez_src = a.srcs[a.events[ev].id].ceze;
+                }
+                ++ev;
+            }
+            ek = (a.events[ev].col == mycol) ? a.events[ev].k : INT_MAX;
+        }
+        if (fstep >= 0 && (ux || uy || uz)) {
+            float4 ex, ey, ez;
+            coefs<CM>(a, a.etab, euni, c, ex, ey, ez);
+            const bool zs = zs_e(a, k);
+            if (xs && jhi && khi) {
+                const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
+                b.pb[0][s] = pxe.x * (b.pb[0][s] + (uy ? (ey.z * dx) * eby : 0.f));
+                b.pb[1][s] = pxe.x * (b.pb[1][s] + (uz ? (ez.y * dx) * ebz : 0.f));
+            }
+            if (ys && ihi && khi) {
+                const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
+                b.pb[4][s] = pye.x * (b.pb[4][s] + (ux ? (ex.y * dy) * ebx : 0.f));
+                b.pb[5][s] = pye.x * (b.pb[5][s] + (uz ? (ez.z * dy) * ebz : 0.f));
+            }
+            if (zs && ihi && jhi) {
+                const float4 pze = a.pz_prof[k];
+                const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
+                b.pb[8][s] = pze.x * (b.pb[8][s] + (ux ? (ex.z * dz) * ebx : 0.f));
+                b.pb[9][s] = pze.x * (b.pb[9][s] + (uy ? (ey.y * dz) * eby : 0.f));
+            }
+            if (in_box && k >= b.sb[2] && k < b.sb[2] + b.sb[5]) {
+                const i64 q = (i - b.sb[0]) + (i64)b.sb[3] * ((j - b.sb[1]) + (i64)b.sb[4] * (k - b.sb[2]));
+                const float *h0 = b.hist + (i64)(fstep - b.hist_base) * 3 * nb;
+                const float *h1 = h0 + 3 * nb;
+                const float e_[3] = {ebx, eby, ebz};
+                const bool u_[3] = {ux, uy, uz};
+                for (int cc = 0; cc < 3; ++cc) {
+                    const float4 dc = b.dcoef[cc * nb + q];
+                    if (!u_[cc] || (dc.x == 0.f && dc.y == 0.f)) continue;
+                    const double em = h0[cc * nb + q], ep = h1[cc * nb + q];
+                    const double dE = (double)dc.x * em - (double)dc.y * (ep - (double)dc.z * em);
+                    b.gedge[cc * nb + q] += (double)e_[cc] * dE;
+                }
+            }
+            if (ux) ebx = ex.x * ebx;
+            if (uy) eby = ey.x * eby;
+            if (uz) ebz = (is_src ? cez_src : ez.x) * ebz;
+        }
+        if (sten) {
+            const HQ q0 = hquant<CM>(a, b, huni, i, j, k);
+            if (ux) {
+                const HQ qj = hquant<CM>(a, b, huni, i, j - 1, k);
+                ebx += (qkm.tx - q0.tx) + (qj.rx - q0.rx);
+                if (uz) ebz += qj.sz - q0.sz;
+            }
+            if (uy || uz) {
+                const HQ qi = hquant<CM>(a, b, huni, i - 1, j, k);
+                if (uy) eby += (qkm.sy - q0.sy) + (qi.ry - q0.ry);
+                if (uz) ebz += qi.tz - q0.tz;
+            }
+            if (uz && !ux) {
+                const HQ qj = hquant<CM>(a, b, huni, i, j - 1, k);
+                ebz += qj.sz - q0.sz;
+            }
+            qkm = q0;
+            if (pev >= 0) {   // dJ/d(probe sample m) of E probes on this cell
+                for (int e = pev; a.events[e].k == k && a.events[e].col == mycol; ++e) {
+                    const int kind = a.events[e].kind;
+                    if (kind < EV_PROBE || kind > EV_PROBE + 2) continue;
+                    const float gv = b.g[(i64)a.events[e].id * b.gsteps + m];
+                    if (kind == EV_PROBE) ebx += gv;
+                    else if (kind == EV_PROBE + 1) eby += gv;
+                    else ebz += gv;
+                }
+            }
+        }
+        if (ux) b.eb[0][c] = ebx;
+        if (uy) b.eb[1][c] = eby;
+        if (uz) b.eb[2][c] = ebz;
+    }
+}
+
+template <int CM>
+__global__ void __launch_bounds__(TX * BY)
+    adj_h_kernel(const KArgs a, const AArgs b, const int euni, const int huni)
+{
+    int tx, ty, tz;
+    tile_of(a, tx, ty, tz);
+    const int i = tx * TX + (int)(threadIdx.x & (TX - 1));
+    const int j = ty * BY + (int)(threadIdx.x / TX);
+    const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
+    if (i >= nx || j >= ny) return;
+    const int k0 = tz * a.kchunk;
+    const int k1 = min(nz, k0 + a.kchunk);
+    const bool ihi = i < nx - 1, jhi = j < ny - 1;
+    const bool xs = xs_h(a, i), ys = ys_h(a, j);
+    const float4 pxm = xs ? a.px_prof[nx + i] : NOPROF;
+    const float4 pym = ys ? a.py_prof[ny + j] : NOPROF;
+    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    EQ q0 = equant<CM>(a, b, euni, i, j, k0);
+    for (int k = k0; k < k1; ++k) {
+        const i64 c = (i64)k * a.plane + (i64)j * px + i;
+        const bool khi = k < nz - 1;
+        const bool ux = jhi && khi, uy = ihi && khi, uz = ihi && jhi;
+        const EQ qk = khi ? equant<CM>(a, b, euni, i, j, k + 1) : EQ{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        float4 hx, hy, hz;
+        coefs<CM>(a, a.htab, huni, c, hx, hy, hz);
+        float hbx = b.hb[0][c], hby = b.hb[1][c], hbz = b.hb[2][c];
+        // psi_h adjoint and dJ/dH <- Ch dJ/dH (the H half-step, pointwise part)
+        if (xs) {
+            const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
+            b.pb[2][s] = pxm.x * (b.pb[2][s] + (uy ? (hy.y * dx) * hby : 0.f));
+            b.pb[3][s] = pxm.x * (b.pb[3][s] + (uz ? (hz.z * dx) * hbz : 0.f));
+        }
+        if (ys) {
+            const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
+            b.pb[6][s] = pym.x * (b.pb[6][s] + (ux ? (hx.z * dy) * hbx : 0.f));
+            b.pb[7][s] = pym.x * (b.pb[7][s] + (uz ? (hz.y * dy) * hbz : 0.f));
+        }
+        if (zs_h(a, k)) {
+            const float4 pzm = a.pz_prof[nz + k];
+            const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
+            b.pb[10][s] = pzm.x * (b.pb[10][s] + (ux ? (hx.y * dz) * hbx : 0.f));
+            b.pb[11][s] = pzm.x * (b.pb[11][s] + (uy ? (hy.z * dz) * hby : 0.f));
+        }
+        if (ux) hbx = hx.x * hbx;
+        if (uy) hby = hy.x * hby;
+        if (uz) hbz = hz.x * hbz;
+        // transposed E half-step gathered from the E cells that read this H
+        if (ux) {
+            const EQ qj = equant<CM>(a, b, euni, i, j + 1, k);
+            hbx += (q0.ay - qk.ay) + (q0.bz - qj.bz);
+            if (uz) hbz += q0.ax - qj.ax;
+        }
+        if (uy || uz) {
+            const EQ qi = equant<CM>(a, b, euni, i + 1, j, k);
+            if (uy) hby += (q0.bx - qk.bx) + (q0.az - qi.az);
+            if (uz) hbz += q0.by - qi.by;
+        }
+        if (uz && !ux) {
+            const EQ qj = equant<CM>(a, b, euni, i, j + 1, k);
+            hbz += q0.ax - qj.ax;
+        }
+        if (ux) b.hb[0][c] = hbx;
+        if (uy) b.hb[1][c] = hby;
+        if (uz) b.hb[2][c] = hbz;
+        q0 = qk;
+    }
+}
+
+// The H-side gather used the material Ez coefficients on every edge; a
+// lumped-source edge updates Ez with its own (defineSourceZ).  One thread
+// adds the difference for every source edge, in order (deterministic).
+template <int CM>
+__global__ void adj_src_fixup(const KArgs a, const AArgs b, const int euni)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    for (int s = 0; s < b.nsrc_edges; ++s) {
+        const int i = b.src_cell[3 * s], j = b.src_cell[3 * s + 1], k = b.src_cell[3 * s + 2];
+        const i64 c = (i64)k * a.plane + (i64)j * a.px + i;
+        float4 ex, ey, ez;
+        coefs<CM>(a, a.etab, euni, c, ex, ey, ez);
+        const float4 pxe = xs_e(a, i) ? a.px_prof[i] : NOPROF;
+        const float4 pye = ys_e(a, j) ? a.py_prof[j] : NOPROF;
+        const SrcEdge se = a.srcs[s];
+        const float ebz = b.eb[2][c];
+        const float daz = se.cezhy * ebz - (ez.y / pxe.z) * ebz;   // multiplies Hy(x) - Hy(x-i)
+        const float dbz = se.cezhx * ebz - (ez.z / pye.z) * ebz;   // multiplies Hx(x) - Hx(x-j)
+        const bool khi = k < nz - 1;
+        if (i < nx - 1 && khi) b.hb[1][c] += daz;
+        if (i - 1 < nx - 1 && khi) b.hb[1][c - 1] -= daz;
+        if (j < ny - 1 && khi) b.hb[0][c] += dbz;
+        if (j - 1 < ny - 1 && khi) b.hb[0][c - a.px] -= dbz;
+    }
+}
+
+// E of the design box into snapshot slot (step - hist_base), SI units.
+template <class T>
+__global__ void snap_kernel(const KArgs a, float *hist, const int slot)
+{
+    const int bx = a.sb[3], by = a.sb[4], bz = a.sb[5];
+    const i64 nb = (i64)bx * by * bz;
+    const i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb) return;
+    const int i = (int)(t % bx), j = (int)((t / bx) % by), k = (int)(t / ((i64)bx * by));
+    const i64 c = (i64)(a.sb[2] + k) * a.plane + (i64)(a.sb[1] + j) * a.px + (a.sb[0] + i);
+    float *o = hist + (i64)slot * 3 * nb + t;
+    o[0] = ld((const T *)a.ex + c) * a.probe_unscale[0];
+    o[nb] = ld((const T *)a.ey + c) * a.probe_unscale[1];
+    o[2 * nb] = ld((const T *)a.ez + c) * a.probe_unscale[2];
+}
+
+hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a, const AArgs &b, int m, int fstep,
+                      int sten, hipStream_t s)
+{
+    const int nt = a.tiles_x * a.tiles_y * ((a.nz + a.kchunk - 1) / a.kchunk);
+    dim3 grid(nt), block(TX * BY);
+    if (which == 0) {
+        if (mode == CLS_U32) hipLaunchKernelGGL((adj_e_kernel<CLS_U32>), grid, block, 0, s, a, b, m, fstep, sten, euni, huni);
+        else if (mode == CLS_U16) hipLaunchKernelGGL((adj_e_kernel<CLS_U16>), grid, block, 0, s, a, b, m, fstep, sten, euni, huni);
+        else hipLaunchKernelGGL((adj_e_kernel<CLS_UNIFORM>), grid, block, 0, s, a, b, m, fstep, sten, 1, 1);
+    } else if (which == 1) {
+        if (mode == CLS_U32) hipLaunchKernelGGL((adj_h_kernel<CLS_U32>), grid, block, 0, s, a, b, euni, huni);
+        else if (mode == CLS_U16) hipLaunchKernelGGL((adj_h_kernel<CLS_U16>), grid, block, 0, s, a, b, euni, huni);
+        else hipLaunchKernelGGL((adj_h_kernel<CLS_UNIFORM>), grid, block, 0, s, a, b, 1, 1);
+    } else {
+        if (mode == CLS_U32) hipLaunchKernelGGL((adj_src_fixup<CLS_U32>), dim3(1), dim3(64), 0, s, a, b, euni);
+        else if (mode == CLS_U16) hipLaunchKernelGGL((adj_src_fixup<CLS_U16>), dim3(1), dim3(64), 0, s, a, b, euni);
+        else hipLaunchKernelGGL((adj_src_fixup<CLS_UNIFORM>), dim3(1), dim3(64), 0, s, a, b, 1);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_snap(int storage, const KArgs &a, float *hist, int slot, hipStream_t s)
+{
+    const i64 nb = (i64)a.sb[3] * a.sb[4] * a.sb[5];
+    dim3 grid((unsigned)((nb + 255) / 256)), block(256);
+    if (storage == 1) hipLaunchKernelGGL((snap_kernel<_Float16>), grid, block, 0, s, a, hist, slot);
+    else if (storage == 2) hipLaunchKernelGGL((snap_kernel<bf16_t>), grid, block, 0, s, a, hist, slot);
+    else hipLaunchKernelGGL((snap_kernel<float>), grid, block, 0, s, a, hist, slot);
+    return hipGetLastError();
+}
+
+}  // namespace fdtd

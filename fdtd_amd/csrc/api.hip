@@ -22,6 +22,9 @@
 
 namespace fdtd {
 hipError_t launch_half_step(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s);
+hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a, const AArgs &b, int m, int fstep,
+                      int sten, hipStream_t s);
+hipError_t launch_snap(int storage, const KArgs &a, float *hist, int slot, hipStream_t s);
 int tile_rows();
 int tile_cols();
 }  // namespace fdtd
@@ -47,6 +50,13 @@ struct fdtd_ctx {
     void *psi[12] = {};
     size_t psi_elems[12] = {};
     uint64_t dev_bytes = 0;
+    // SI coefficient tables for the adjoint
+    const float4 *etab_si = nullptr, *htab_si = nullptr;
+    const SrcEdge *srcs_si = nullptr;
+    const int *src_cell = nullptr;
+    const float4 *dcoef = nullptr;
+    double adj_fwd_ms = 0.0, adj_total_ms = 0.0;
+    int adj_segments = 0;
 };
 
 namespace {
@@ -161,6 +171,8 @@ int do_finalize(fdtd_ctx *c)
     a.tiles_x = (in.nx + tile_cols() - 1) / tile_cols();
     a.tiles_y = (in.ny + tile_rows() - 1) / tile_rows();
     a.kchunk = choose_chunks(c, a.tiles_x * a.tiles_y);
+    a.rec = 1;
+    a.snap = nullptr;
 
     // fields: one zero guard plane below and above
     c->field_elems = a.plane * (in.nz + 2);
@@ -221,6 +233,19 @@ int upload_tables(fdtd_ctx *c)
     a.htab = (const float4 *)tmp;
     if ((rc = upload(c, &tmp, b.srcs.data(), b.srcs.size() * sizeof(SrcEdge)))) return rc;
     a.srcs = (const SrcEdge *)tmp;
+    if ((rc = upload(c, &tmp, b.etab_si.data(), b.etab_si.size() * sizeof(float)))) return rc;
+    c->etab_si = (const float4 *)tmp;
+    if ((rc = upload(c, &tmp, b.htab_si.data(), b.htab_si.size() * sizeof(float)))) return rc;
+    c->htab_si = (const float4 *)tmp;
+    if ((rc = upload(c, &tmp, b.srcs_si.data(), b.srcs_si.size() * sizeof(SrcEdge)))) return rc;
+    c->srcs_si = (const SrcEdge *)tmp;
+    if ((rc = upload(c, &tmp, b.src_cell.data(), b.src_cell.size() * sizeof(int)))) return rc;
+    c->src_cell = (const int *)tmp;
+    c->dcoef = nullptr;
+    if (!b.dcoef.empty()) {
+        if ((rc = upload(c, &tmp, b.dcoef.data(), b.dcoef.size() * sizeof(float)))) return rc;
+        c->dcoef = (const float4 *)tmp;
+    }
     a.cls = nullptr;
     if (b.mode == CLS_U16) {
         std::vector<uint16_t> m16(b.cls.size());
@@ -312,6 +337,255 @@ struct CkHeader {
 };
 
 }  // namespace
+
+
+// ---- adjoint orchestration --------------------------------------------------
+namespace {
+
+// Device memory the forward state occupies (fields + psi), per checkpoint.
+size_t state_bytes(const fdtd_ctx *c)
+{
+    size_t b = 6 * (size_t)c->field_elems * c->esz;
+    for (int q = 0; q < 12; ++q) b += c->psi_elems[q] * c->esz;
+    return b;
+}
+
+int copy_state(fdtd_ctx *c, char *dst, bool save)
+{
+    size_t off = 0;
+    for (int f = 0; f < 6; ++f) {
+        const size_t n = (size_t)c->field_elems * c->esz;
+        if (save) HIPCHK(c, hipMemcpyAsync(dst + off, c->fields[f], n, hipMemcpyDeviceToDevice, c->stream));
+        else HIPCHK(c, hipMemcpyAsync(c->fields[f], dst + off, n, hipMemcpyDeviceToDevice, c->stream));
+        off += n;
+    }
+    for (int q = 0; q < 12; ++q) {
+        const size_t n = c->psi_elems[q] * c->esz;
+        if (!n) continue;
+        if (save) HIPCHK(c, hipMemcpyAsync(dst + off, c->psi[q], n, hipMemcpyDeviceToDevice, c->stream));
+        else HIPCHK(c, hipMemcpyAsync(c->psi[q], dst + off, n, hipMemcpyDeviceToDevice, c->stream));
+        off += n;
+    }
+    return FDTD_OK;
+}
+
+struct DevBuf {
+    std::vector<void *> ptrs;
+    ~DevBuf()
+    {
+        // an early return may leave work queued on these buffers
+        if (!ptrs.empty()) (void)hipDeviceSynchronize();
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    void *get(size_t bytes)
+    {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        ptrs.push_back(p);
+        return p;
+    }
+};
+
+}  // namespace
+
+int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, double *J_out, double *grad_out,
+                size_t n_grad)
+{
+    if (!c->finalized) return fail(c, FDTD_E_STATE, "fdtd_finalize has not been called");
+    const SceneIn &in = c->in;
+    if (in.design_eps.empty() || !c->dcoef) return fail(c, FDTD_E_STATE, "no design region");
+    if (n_steps < 1 || n_steps > in.steps_cap) return fail(c, FDTD_E_RANGE, "n_steps outside [1, %d]", in.steps_cap);
+    const int *d = in.design;
+    const int dnx = d[3] - d[0], dny = d[4] - d[1], dnz = d[5] - d[2];
+    if (n_grad < (size_t)dnx * dny * dnz) return fail(c, FDTD_E_INVALID, "gradient buffer too small");
+    const int *L = in.cpml.thick;
+    const int nx = in.nx, ny = in.ny, nz = in.nz;
+    // the dilated design box must be updated, CPML-free E edges
+    if (d[0] < std::max(1, L[0]) || d[3] >= nx - std::max(1, L[1]) || d[1] < std::max(1, L[2]) ||
+        d[4] >= ny - std::max(1, L[3]) || d[2] < std::max(1, L[4]) || d[5] >= nz - std::max(1, L[5]))
+        return fail(c, FDTD_E_RANGE, "design region (dilated by one edge) must lie outside the CPML");
+    for (const Source &s : in.sources)
+        for (int k = s.b[2]; k < s.b[5]; ++k)
+            for (int j = s.b[1]; j <= s.b[4]; ++j)
+                for (int i = s.b[0]; i <= s.b[3]; ++i)
+                    if (i >= d[0] && i <= d[3] && j >= d[1] && j <= d[4] && k >= d[2] && k < d[5])
+                        return fail(c, FDTD_E_INVALID, "a source edge touches the design region");
+    const int np = (int)in.probes.size(), nf = (int)in.freqs.size();
+    if (obj->n_terms < 1) return fail(c, FDTD_E_INVALID, "empty objective");
+    for (int t = 0; t < obj->n_terms; ++t) {
+        const int p = obj->probe[t], f = obj->freq[t];
+        if (p < 0 || p >= np || f < 0 || f >= nf) return fail(c, FDTD_E_RANGE, "objective term %d out of range", t);
+        const Probe &pr = in.probes[p];
+        if (pr.comp > 2) return fail(c, FDTD_E_INVALID, "objective probes must sample E");
+        if (pr.i < std::max(1, L[0]) || pr.i >= nx - std::max(1, L[1]) || pr.j < std::max(1, L[2]) ||
+            pr.j >= ny - std::max(1, L[3]) || pr.k < std::max(1, L[4]) || pr.k >= nz - std::max(1, L[5]))
+            return fail(c, FDTD_E_INVALID, "objective probe %d lies in the CPML or on a face", p);
+    }
+    const int n = n_steps;
+    int Kseg = K > 0 ? std::min(K, n) : n;
+    const int nseg = (n + Kseg - 1) / Kseg;
+    const int bx = dnx + 1, by = dny + 1, bz = dnz + 1;
+    const i64 nb = (i64)bx * by * bz;
+    const size_t hist_bytes = (size_t)(Kseg + 1) * 3 * nb * sizeof(float);
+    const size_t ck_bytes = state_bytes(c);
+    size_t freeb = 0, totb = 0;
+    HIPCHK(c, hipMemGetInfo(&freeb, &totb));
+    const size_t adj_bytes = 6 * (size_t)c->field_elems * 4 + [&] {
+        size_t t = 0;
+        for (int q = 0; q < 12; ++q) t += c->psi_elems[q] * 4;
+        return t;
+    }();
+    const size_t need = hist_bytes + (size_t)(nseg - 1) * ck_bytes + adj_bytes + (size_t)np * n * 4 + 3 * nb * 8;
+    if ((double)need > 0.9 * (double)freeb)
+        return fail(c, FDTD_E_NOMEM, "adjoint needs %.2f GB of device memory, %.2f GB free; use a larger "
+                    "checkpoint interval", need / 1e9, freeb / 1e9);
+
+    DevBuf mem;
+    AArgs b{};
+    for (int q = 0; q < 3; ++q) {
+        char *e = (char *)mem.get((size_t)c->field_elems * 4);
+        char *h = (char *)mem.get((size_t)c->field_elems * 4);
+        if (!e || !h) return fail(c, FDTD_E_NOMEM, "adjoint field allocation failed");
+        HIPCHK(c, hipMemsetAsync(e, 0, (size_t)c->field_elems * 4, c->stream));
+        HIPCHK(c, hipMemsetAsync(h, 0, (size_t)c->field_elems * 4, c->stream));
+        b.eb[q] = (float *)(e + (size_t)c->ka.plane * 4);
+        b.hb[q] = (float *)(h + (size_t)c->ka.plane * 4);
+    }
+    for (int q = 0; q < 12; ++q) {
+        b.pb[q] = (float *)mem.get(std::max<size_t>(c->psi_elems[q], 1) * 4);
+        if (!b.pb[q]) return fail(c, FDTD_E_NOMEM, "adjoint psi allocation failed");
+        HIPCHK(c, hipMemsetAsync(b.pb[q], 0, std::max<size_t>(c->psi_elems[q], 1) * 4, c->stream));
+    }
+    float *hist = (float *)mem.get(hist_bytes);
+    double *gedge = (double *)mem.get(3 * nb * sizeof(double));
+    float *gdev = (float *)mem.get((size_t)std::max(np, 1) * n * sizeof(float));
+    char *pool = nseg > 1 ? (char *)mem.get((size_t)(nseg - 1) * ck_bytes) : nullptr;
+    if (!hist || !gedge || !gdev || (nseg > 1 && !pool)) return fail(c, FDTD_E_NOMEM, "adjoint buffers");
+    HIPCHK(c, hipMemsetAsync(hist, 0, hist_bytes, c->stream));
+    HIPCHK(c, hipMemsetAsync(gedge, 0, 3 * nb * sizeof(double), c->stream));
+    b.gsteps = n;
+    b.g = gdev;
+    b.hist = hist;
+    b.dcoef = c->dcoef;
+    b.gedge = gedge;
+    b.sb[0] = d[0]; b.sb[1] = d[1]; b.sb[2] = d[2]; b.sb[3] = bx; b.sb[4] = by; b.sb[5] = bz;
+    b.src_cell = c->src_cell;
+    b.nsrc_edges = (int)c->built.srcs.size();
+
+    int rc = zero_state(c);
+    if (rc) return rc;
+    KArgs fa = c->ka;               // forward arguments
+    fa.sb[0] = d[0]; fa.sb[1] = d[1]; fa.sb[2] = d[2]; fa.sb[3] = bx; fa.sb[4] = by; fa.sb[5] = bz;
+    KArgs aa = c->ka;               // adjoint arguments: SI tables
+    aa.etab = c->etab_si;
+    aa.htab = c->htab_si;
+    aa.srcs = c->srcs_si;
+    const int mode = c->built.mode;
+    const int euni = c->built.e_uniform ? 1 : 0, huni = c->built.h_uniform ? 1 : 0;
+
+    hipEvent_t t0, t1, t2;
+    HIPCHK(c, hipEventCreate(&t0));
+    HIPCHK(c, hipEventCreate(&t1));
+    HIPCHK(c, hipEventCreate(&t2));
+    HIPCHK(c, hipEventRecord(t0, c->stream));
+    // ---- forward pass: DFT at the probes; checkpoints at segment starts, or
+    // the whole design history when there is a single segment
+    fa.rec = 1;
+    fa.snap = nseg == 1 ? hist : nullptr;
+    fa.snap_base = 0;
+    for (int m = 0; m < n; ++m) {
+        if (nseg > 1 && m % Kseg == 0 && m > 0)
+            if ((rc = copy_state(c, pool + (size_t)(m / Kseg - 1) * ck_bytes, true))) return rc;
+        HIPCHK(c, launch_half_step(true, in.storage, c->e_mode, fa, m, c->stream));
+        HIPCHK(c, launch_half_step(false, in.storage, c->h_mode, fa, m, c->stream));
+    }
+    HIPCHK(c, hipEventRecord(t1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->steps_done = n;
+    // ---- objective J = sum w |X|^2 and dJ/ds[m] = sum w 2 Re(conj(X) phasor[m])
+    std::vector<double> X((size_t)np * nf * 2);
+    if (np && nf) HIPCHK(c, hipMemcpy(X.data(), c->ka.dft, X.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<float> g((size_t)np * n, 0.0f);
+    std::vector<double> gd((size_t)np * n, 0.0);
+    double J = 0.0;
+    for (int t = 0; t < obj->n_terms; ++t) {
+        const int p = obj->probe[t], f = obj->freq[t];
+        const double w = obj->weight ? obj->weight[t] : 1.0;
+        const double xr = X[((size_t)p * nf + f) * 2], xi = X[((size_t)p * nf + f) * 2 + 1];
+        J += w * (xr * xr + xi * xi);
+        for (int m = 0; m < n; ++m) {
+            const double pr = c->built.phasor[((size_t)m * nf + f) * 2], pi = c->built.phasor[((size_t)m * nf + f) * 2 + 1];
+            gd[(size_t)p * n + m] += w * 2.0 * (xr * pr + xi * pi);
+        }
+    }
+    for (size_t q = 0; q < g.size(); ++q) g[q] = (float)gd[q];
+    HIPCHK(c, hipMemcpyAsync(gdev, g.data(), g.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    // ---- reverse sweep, segment by segment
+    for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+        const int s0 = sgi * Kseg, s1 = std::min(n, s0 + Kseg);
+        if (nseg > 1) {
+            if (sgi == 0) {
+                for (int f = 0; f < 6; ++f)
+                    HIPCHK(c, hipMemsetAsync(c->fields[f], 0, (size_t)c->field_elems * c->esz, c->stream));
+                for (int q = 0; q < 12; ++q)
+                    if (c->psi_elems[q]) HIPCHK(c, hipMemsetAsync(c->psi[q], 0, c->psi_elems[q] * c->esz, c->stream));
+            } else if ((rc = copy_state(c, pool + (size_t)(sgi - 1) * ck_bytes, false))) {
+                return rc;
+            }
+            KArgs ra = fa;
+            ra.rec = 0;
+            ra.snap = hist;
+            ra.snap_base = s0;
+            HIPCHK(c, launch_snap(in.storage, ra, hist, 0, c->stream));
+            for (int m = s0; m < s1; ++m) {
+                HIPCHK(c, launch_half_step(true, in.storage, c->e_mode, ra, m, c->stream));
+                HIPCHK(c, launch_half_step(false, in.storage, c->h_mode, ra, m, c->stream));
+            }
+        }
+        b.hist_base = s0;
+        for (int m = s1 - 1; m >= s0; --m) {
+            HIPCHK(c, launch_adj(0, mode, euni, huni, aa, b, m, m < s1 - 1 ? m + 1 : -1, 1, c->stream));
+            HIPCHK(c, launch_adj(1, mode, euni, huni, aa, b, m, -1, 1, c->stream));
+            if (b.nsrc_edges) HIPCHK(c, launch_adj(2, mode, euni, huni, aa, b, m, -1, 1, c->stream));
+        }
+        HIPCHK(c, launch_adj(0, mode, euni, huni, aa, b, s0, s0, 0, c->stream));
+    }
+    HIPCHK(c, hipEventRecord(t2, c->stream));
+    HIPCHK(c, hipEventSynchronize(t2));
+    float ms1 = 0.f, ms2 = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms1, t0, t1));
+    HIPCHK(c, hipEventElapsedTime(&ms2, t0, t2));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipEventDestroy(t2);
+    c->adj_fwd_ms = ms1;
+    c->adj_total_ms = ms2;
+    c->adj_segments = nseg;
+    // ---- chain rule through the 4-cell averages (fdtdrav.h:1135-1177)
+    std::vector<double> ge((size_t)3 * nb);
+    HIPCHK(c, hipMemcpy(ge.data(), gedge, ge.size() * sizeof(double), hipMemcpyDeviceToHost));
+    auto q = [&](int a_, int b_, int c_) { return (size_t)(a_ + (i64)bx * (b_ + (i64)by * c_)); };
+    for (int k = 0; k < dnz; ++k)
+        for (int j = 0; j < dny; ++j)
+            for (int i = 0; i < dnx; ++i) {
+                double acc = 0.0;
+                for (int u = 0; u < 2; ++u)
+                    for (int v = 0; v < 2; ++v) {
+                        acc += ge[q(i, j + u, k + v)];
+                        acc += ge[(size_t)nb + q(i + u, j, k + v)];
+                        acc += ge[(size_t)2 * nb + q(i + u, j + v, k)];
+                    }
+                grad_out[i + (size_t)dnx * (j + (size_t)dny * k)] = 0.25 * acc;
+            }
+    *J_out = J;
+    // leave the context as after an uninterrupted forward run of n steps
+    // when there was a single segment; otherwise rewind it
+    if (nseg > 1) return zero_state(c);
+    return FDTD_OK;
+}
 
 // ============================================================================
 extern "C" {
@@ -544,6 +818,9 @@ int fdtd_get_info(const fdtd_ctx *c, fdtd_info *o)
     const double fz = (double)(c->in.cpml.thick[4] + c->in.cpml.thick[5]) / c->in.nz;
     const double psi = 4.0 * s * (fx + fy + fz);
     const double cls = c->built.mode == CLS_U16 ? 2.0 : c->built.mode == CLS_U32 ? 4.0 : 0.0;
+    o->adjoint_forward_ms = c->adj_fwd_ms;
+    o->adjoint_total_ms = c->adj_total_ms;
+    o->adjoint_segments = c->adj_segments;
     o->bytes_per_cell_e = (uint64_t)std::llround(9 * s + (c->e_mode ? cls : 0.0) + psi);
     o->bytes_per_cell_h = (uint64_t)std::llround(9 * s + (c->h_mode ? cls : 0.0) + psi);
     return FDTD_OK;
@@ -690,9 +967,8 @@ int fdtd_checkpoint_restore(fdtd_ctx *c, const void *buf, size_t bytes)
 int fdtd_adjoint_gradient(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int checkpoint_every,
                           double *J_out, double *grad_out, size_t n_grad)
 {
-    (void)obj; (void)n_steps; (void)checkpoint_every; (void)J_out; (void)grad_out; (void)n_grad;
-    if (!c) return FDTD_E_INVALID;
-    return fail(c, FDTD_E_UNSUPPORTED, "adjoint not built yet");
+    if (!c || !obj || !J_out || !grad_out) return FDTD_E_INVALID;
+    return run_adjoint(c, obj, n_steps, checkpoint_every, J_out, grad_out, n_grad);
 }
 
 }  // extern "C"

@@ -1,0 +1,61 @@
+// device_common.h -- storage conversions and workgroup mapping shared by the
+// forward kernels (kernels.hip) and the adjoint kernels (adjoint.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "internal.h"
+
+namespace fdtd {
+
+constexpr int TX = 64;   // tile width (one wavefront row)
+constexpr int BY = 4;    // tile rows per workgroup
+
+// ---- storage types ------------------------------------------------------
+struct bf16_t {
+    std::uint16_t v;
+};
+
+__device__ __forceinline__ float ld(const float *p) { return *p; }
+__device__ __forceinline__ void st(float *p, float v) { *p = v; }
+__device__ __forceinline__ float ld(const _Float16 *p) { return (float)*p; }
+__device__ __forceinline__ void st(_Float16 *p, float v) { *p = (_Float16)v; }
+__device__ __forceinline__ float ld(const bf16_t *p) { return __uint_as_float((std::uint32_t)p->v << 16); }
+__device__ __forceinline__ void st(bf16_t *p, float v)
+{
+    std::uint32_t u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
+    p->v = (std::uint16_t)(u >> 16);
+}
+
+template <int CM>
+__device__ __forceinline__ int class_of(const void *cls, i64 c)
+{
+    if (CM == CLS_U16) return ((const std::uint16_t *)cls)[c];
+    if (CM == CLS_U32) return (int)((const std::uint32_t *)cls)[c];
+    return 0;
+}
+
+// Workgroup -> tile: XCD x (= blockIdx % 8 under round-robin dispatch) gets
+// the contiguous tile range [start_x, start_x + count_x).
+__device__ __forceinline__ void tile_of(const KArgs &a, int &tx, int &ty, int &tz)
+{
+    const int nt = a.tiles_x * a.tiles_y * ((a.nz + a.kchunk - 1) / a.kchunk);
+    const int b = blockIdx.x;
+    const int q = nt >> 3, r = nt & 7, x = b & 7;
+    const int t = x * q + (x < r ? x : r) + (b >> 3);
+    tx = t % a.tiles_x;
+    ty = (t / a.tiles_x) % a.tiles_y;
+    tz = t / (a.tiles_x * a.tiles_y);
+}
+
+
+__device__ __forceinline__ int first_event(const KArgs &a, const int *first, int col, int k0)
+{
+    int ev = first[col];
+    if (ev < 0) return -1;
+    while (a.events[ev].col == col && a.events[ev].k < k0) ++ev;
+    return a.events[ev].col == col ? ev : -1;
+}
+
+}  // namespace fdtd

@@ -62,6 +62,28 @@ struct KArgs {
     int steps_cap, nfreq;
     int kchunk;                // planes per z-march block
     int tiles_x, tiles_y;      // tile grid (for the XCD-aware remap)
+    int rec;                   // record probes / DFT (0 during adjoint recompute)
+    // design-region snapshot: E^{m+1} of the E edges in the box
+    // [sb[0], sb[0]+sb[3]) x [sb[1], sb[1]+sb[4]) x [sb[2], sb[2]+sb[5]) is
+    // written in SI units to snap[((m+1-snap_base)*3 + comp)*nb + q]
+    float *snap;
+    int sb[6];
+    int snap_base;
+};
+
+// Adjoint argument block (FP32 SI adjoint state; same layouts as the forward).
+struct AArgs {
+    float *eb[3], *hb[3];      // dJ/dE, dJ/dH (guard-plane layout of the fields)
+    float *pb[12];             // dJ/dpsi: eyx ezx hyx hzx exy ezy hxy hzy exz eyz hxz hyz
+    const float *g;            // [probe][gsteps] dJ/d(probe sample m)
+    int gsteps;
+    const float *hist;         // [slot][3][nb] E snapshots (SI), slot = step - hist_base
+    int hist_base;
+    const float4 *dcoef;       // [3][nb] (dCe/deps, 2 eps0/den, Ce, 0); zero off the design
+    double *gedge;             // [3][nb] dJ/d(averaged eps) per E edge
+    int sb[6];                 // dilated design box (i0, j0, k0, bx, by, bz)
+    const int *src_cell;       // per source edge (i, j, k)
+    int nsrc_edges;
 };
 
 // Host-side scene, filled by the fdtd_* scene calls.
@@ -90,13 +112,17 @@ struct Built {
     bool e_uniform, h_uniform;
     int ncls;
     std::vector<std::uint32_t> cls; // per cell (padded pitch), empty if uniform
-    std::vector<float> etab, htab;  // [class][3][4]
+    std::vector<float> etab, htab;  // [class][3][4], in stored (scaled) units
+    std::vector<float> etab_si, htab_si;   // the same in SI units (adjoint)
     std::vector<float> prof[3];     // [2n][4] per axis
     std::vector<int> e_first, h_first;
     std::vector<Event> events;
     std::vector<SrcEdge> srcs;
+    std::vector<SrcEdge> srcs_si;   // SI coefficients (adjoint)
+    std::vector<int> src_cell;      // per source edge: (i, j, k) triples
     std::vector<float> vpn;         // [wave][steps_cap]
     std::vector<double> phasor;     // [step][nfreq][2]
+    std::vector<float> dcoef;       // [3][nb][4] design-edge eps derivatives (adjoint)
     float probe_unscale[6];
     float scale[6];                 // stored = SI * scale, per component
     double sigma_max_used[3];

@@ -23,58 +23,10 @@
 #include <climits>
 #include <cstdint>
 
+#include "device_common.h"
 #include "internal.h"
 
 namespace fdtd {
-
-constexpr int TX = 64;   // tile width (one wavefront row)
-constexpr int BY = 4;    // tile rows per workgroup
-
-// ---- storage types ------------------------------------------------------
-struct bf16_t {
-    std::uint16_t v;
-};
-
-__device__ __forceinline__ float ld(const float *p) { return *p; }
-__device__ __forceinline__ void st(float *p, float v) { *p = v; }
-__device__ __forceinline__ float ld(const _Float16 *p) { return (float)*p; }
-__device__ __forceinline__ void st(_Float16 *p, float v) { *p = (_Float16)v; }
-__device__ __forceinline__ float ld(const bf16_t *p) { return __uint_as_float((std::uint32_t)p->v << 16); }
-__device__ __forceinline__ void st(bf16_t *p, float v)
-{
-    std::uint32_t u = __float_as_uint(v);
-    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
-    p->v = (std::uint16_t)(u >> 16);
-}
-
-template <int CM>
-__device__ __forceinline__ int class_of(const void *cls, i64 c)
-{
-    if (CM == CLS_U16) return ((const std::uint16_t *)cls)[c];
-    if (CM == CLS_U32) return (int)((const std::uint32_t *)cls)[c];
-    return 0;
-}
-
-// Workgroup -> tile: XCD x (= blockIdx % 8 under round-robin dispatch) gets
-// the contiguous tile range [start_x, start_x + count_x).
-__device__ __forceinline__ void tile_of(const KArgs &a, int &tx, int &ty, int &tz)
-{
-    const int nt = a.tiles_x * a.tiles_y * ((a.nz + a.kchunk - 1) / a.kchunk);
-    const int b = blockIdx.x;
-    const int q = nt >> 3, r = nt & 7, x = b & 7;
-    const int t = x * q + (x < r ? x : r) + (b >> 3);
-    tx = t % a.tiles_x;
-    ty = (t / a.tiles_x) % a.tiles_y;
-    tz = t / (a.tiles_x * a.tiles_y);
-}
-
-__device__ __forceinline__ int first_event(const KArgs &a, const int *first, int col, int k0)
-{
-    int ev = first[col];
-    if (ev < 0) return -1;
-    while (a.events[ev].col == col && a.events[ev].k < k0) ++ev;
-    return a.events[ev].col == col ? ev : -1;
-}
 
 __device__ __forceinline__ void record_probe(const KArgs &a, int p, int comp, float stored, int m)
 {
@@ -130,6 +82,8 @@ __global__ void __launch_bounds__(TX * BY) e_kernel(const KArgs a, const int m)
 
     float hx_km = ld(hx + col + (i64)(k0 - 1) * plane);
     float hy_km = ld(hy + col + (i64)(k0 - 1) * plane);
+    const bool in_snap = a.snap != nullptr && i >= a.sb[0] && i < a.sb[0] + a.sb[3] && j >= a.sb[1] &&
+                         j < a.sb[1] + a.sb[4];
 
     for (int k = k0; k < k1; ++k) {
         const i64 c = col + (i64)k * plane;
@@ -176,7 +130,7 @@ __global__ void __launch_bounds__(TX * BY) e_kernel(const KArgs a, const int m)
                         ezn = ((s.ceze * ez0 + s.cezhy * (hyc - hy_im)) + s.cezhx * (hxc - hx_jm)) +
                               s.cezj * a.vpn[(i64)s.wave * a.steps_cap + m];
                     }
-                } else {
+                } else if (a.rec) {
                     const int comp = e.kind - EV_PROBE;
                     record_probe(a, e.id, comp, comp == 0 ? exn : comp == 1 ? eyn : ezn, m);
                 }
@@ -223,6 +177,15 @@ __global__ void __launch_bounds__(TX * BY) e_kernel(const KArgs a, const int m)
         if (upd_x) st(ex + c, exn);
         if (upd_y) st(ey + c, eyn);
         if (upd_z) st(ez + c, ezn);
+        if (in_snap && k >= a.sb[2] && k < a.sb[2] + a.sb[5]) {
+            // stored-precision value, as the next step reads it, in SI units
+            const i64 nb = (i64)a.sb[3] * a.sb[4] * a.sb[5];
+            const i64 q = (i - a.sb[0]) + (i64)a.sb[3] * ((j - a.sb[1]) + (i64)a.sb[4] * (k - a.sb[2]));
+            float *o = a.snap + (i64)(m + 1 - a.snap_base) * 3 * nb + q;
+            o[0] = ld(ex + c) * a.probe_unscale[0];
+            o[nb] = ld(ey + c) * a.probe_unscale[1];
+            o[2 * nb] = ld(ez + c) * a.probe_unscale[2];
+        }
         hx_km = hxc;
         hy_km = hyc;
     }
@@ -308,7 +271,7 @@ __global__ void __launch_bounds__(TX * BY) h_kernel(const KArgs a, const int m)
             while (ev >= 0 && a.events[ev].k == k && a.events[ev].col == mycol) {
                 const Event e = a.events[ev];
                 const int comp = e.kind - EV_PROBE;
-                record_probe(a, e.id, comp, comp == 3 ? hxn : comp == 4 ? hyn : hzn, m);
+                if (a.rec) record_probe(a, e.id, comp, comp == 3 ? hxn : comp == 4 ? hyn : hzn, m);
                 ++ev;
             }
             ek = (a.events[ev].col == mycol) ? a.events[ev].k : INT_MAX;

@@ -292,6 +292,8 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
     const int ntab = out.ncls;
     out.etab.assign((size_t)ntab * 12, 0.0f);
     out.htab.assign((size_t)ntab * 12, 0.0f);
+    out.etab_si.assign((size_t)ntab * 12, 0.0f);
+    out.htab_si.assign((size_t)ntab * 12, 0.0f);
     for (int c = 0; c < ntab; ++c) {
         const float *t = tuples[(size_t)(ntab == 1 ? 0 : c)].data();
         for (int comp = 0; comp < 3; ++comp) {
@@ -303,6 +305,8 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
             h[0] = t[9 + comp * 3 + 0];
             h[1] = t[9 + comp * 3 + 1];
             h[2] = t[9 + comp * 3 + 2];
+            std::memcpy(&out.etab_si[(size_t)c * 12 + comp * 4], e, 4 * sizeof(float));
+            std::memcpy(&out.htab_si[(size_t)c * 12 + comp * 4], h, 4 * sizeof(float));
             if (scaled) {
                 e[1] = (float)((double)e[1] * sc[comp] / sc[hcomp[comp][0]]);
                 e[2] = (float)((double)e[2] * sc[comp] / sc[hcomp[comp][1]]);
@@ -359,8 +363,9 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
     // eps_z / sigma_z of that edge (defineSourceZ, fdtdrav.h:1391-1421)
     out.events.clear();
     out.srcs.clear();
+    out.srcs_si.clear();
+    out.src_cell.clear();
     std::vector<Event> ev_e, ev_h;
-    std::vector<int> src_at;  // sparse overlap check
     std::map<i64, int> owner;
     out.vpn.assign(in.sources.size() * (size_t)in.steps_cap, 0.0f);
     for (size_t q = 0; q < in.sources.size(); ++q) {
@@ -390,6 +395,10 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
                     se.cezhy = (two * C.dt / C.dx) / D;
                     se.cezhx = (-2.0f * C.dt / C.dy) / D;
                     se.cezj = (-2.0f * C.dt / raux) / D;
+                    out.srcs_si.push_back(se);
+                    out.src_cell.push_back(i);
+                    out.src_cell.push_back(j);
+                    out.src_cell.push_back(k);
                     if (scaled) {
                         se.cezhy = (float)((double)se.cezhy * sc[2] / sc[4]);
                         se.cezhx = (float)((double)se.cezhx * sc[2] / sc[3]);
@@ -432,6 +441,56 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
         out.events.push_back(ev_h[q]);
     }
     out.events.push_back(Event{1 << 30, -1, -1, -1});
+
+    // ---- design edges: derivative of each E edge's update w.r.t. its
+    // averaged eps (fdtdrav.h:1135-1177, 1271-1305):  with den = 2 eps eps0 +
+    // dt sigma,  dCe/deps = 4 eps0 dt sigma / den^2 and dCb/deps = -Cb 2 eps0 /
+    // den, so dE'/deps = dCe E - (2 eps0/den)(E' - Ce E) on an edge that is
+    // neither a source edge nor in the CPML.  Stored over the design box
+    // dilated by one edge on each high side; zero where no design cell
+    // enters the edge's average.
+    out.dcoef.clear();
+    if (!in.design_eps.empty()) {
+        const int *d = in.design;
+        const int bx = d[3] - d[0] + 1, by = d[4] - d[1] + 1, bz = d[5] - d[2] + 1;
+        const i64 nb = (i64)bx * by * bz;
+        out.dcoef.assign((size_t)3 * nb * 4, 0.0f);
+        auto in_design = [&](int i, int j, int k) {
+            return i >= d[0] && i < d[3] && j >= d[1] && j < d[4] && k >= d[2] && k < d[5];
+        };
+        for (int kk = 0; kk < bz; ++kk)
+            for (int jj = 0; jj < by; ++jj)
+                for (int ii = 0; ii < bx; ++ii) {
+                    const int i = d[0] + ii, j = d[1] + jj, k = d[2] + kk;
+                    if (i < 1 || j < 1 || k < 1 || i >= nx - 1 || j >= ny - 1 || k >= nz - 1) continue;
+                    const size_t x = (size_t)(i + nx * (j + (i64)ny * k));
+                    const i64 q = ii + (i64)bx * (jj + (i64)by * kk);
+                    // the four cells of each component's average
+                    const size_t cx[4] = {x, x - nx, x - nxy, x - nx - nxy};
+                    const size_t cy[4] = {x, x - 1, x - nxy, x - 1 - nxy};
+                    const size_t cz[4] = {x, x - 1, x - nx, x - 1 - nx};
+                    const bool dx_[3] = {in_design(i, j, k) || in_design(i, j - 1, k) || in_design(i, j, k - 1) ||
+                                             in_design(i, j - 1, k - 1),
+                                         in_design(i, j, k) || in_design(i - 1, j, k) || in_design(i, j, k - 1) ||
+                                             in_design(i - 1, j, k - 1),
+                                         in_design(i, j, k) || in_design(i - 1, j, k) || in_design(i, j - 1, k) ||
+                                             in_design(i - 1, j - 1, k)};
+                    const size_t *cs[3] = {cx, cy, cz};
+                    for (int comp = 0; comp < 3; ++comp) {
+                        if (!dx_[comp]) continue;
+                        const float e = avg4(mats[lab[cs[comp][0]]].v[comp], mats[lab[cs[comp][1]]].v[comp],
+                                             mats[lab[cs[comp][2]]].v[comp], mats[lab[cs[comp][3]]].v[comp]);
+                        const float sg = avg4(mats[lab[cs[comp][0]]].v[3 + comp], mats[lab[cs[comp][1]]].v[3 + comp],
+                                              mats[lab[cs[comp][2]]].v[3 + comp], mats[lab[cs[comp][3]]].v[3 + comp]);
+                        const float ce = tuples[cellcls[x]][comp * 3];
+                        const double den = 2.0 * (double)e * in.eps0 + in.dt * (double)sg;
+                        float *o = &out.dcoef[((size_t)comp * nb + q) * 4];
+                        o[0] = (float)(4.0 * in.eps0 * in.dt * (double)sg / (den * den));
+                        o[1] = (float)(2.0 * in.eps0 / den);
+                        o[2] = ce;
+                    }
+                }
+    }
 
     // ---- DFT phasors exp(-j 2 pi f t_m) dt, t_m = m dt
     const int nf = (int)in.freqs.size();

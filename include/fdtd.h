@@ -125,6 +125,10 @@ typedef struct {
     double   last_run_ms;        /* hipEvent time of the last fdtd_run() loop  */
     double   e_kernel_ms;        /* mean E-kernel launch time (fdtd_profile)   */
     double   h_kernel_ms;        /* mean H-kernel launch time (fdtd_profile)   */
+    double   adjoint_forward_ms; /* last fdtd_adjoint_gradient: forward pass   */
+    double   adjoint_total_ms;   /* ... forward + recompute + reverse sweep     */
+    int      adjoint_segments;   /* ... checkpoint segments used                */
+    int      pad0;
     uint64_t device_bytes;       /* device memory held by the context          */
     uint64_t bytes_per_cell_e;   /* algorithmic bytes per cell, E sweep        */
     uint64_t bytes_per_cell_h;   /* algorithmic bytes per cell, H sweep        */

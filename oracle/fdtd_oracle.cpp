@@ -39,6 +39,7 @@
  */
 #include <algorithm>
 #include <cmath>
+#include <immintrin.h>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -103,8 +104,82 @@ struct Sim {
     std::vector<double> ph_re, ph_im; // [step][freq]
     std::vector<double> dft;          // [probe][freq][2]
 
+    // storage-rounding model (FP32 oracle only): 0 none, 16 FP16, 17 BF16,
+    // applied to every stored E, H and psi value in scaled units; cells of
+    // `keep` (when non-empty) are left unrounded.
+    // averaged eps_r / sigma_e per E component (kept for the adjoint's
+    // coefficient derivatives) and the design region
+    std::vector<R> eps_avg[3], se_avg[3];
+    int design[6] = {0, 0, 0, 0, 0, 0};
+    // adjoint state: eb/hb = dJ/dE, dJ/dH; pbe/pbh = dJ/dpsi (reference psi names)
+    std::vector<R> eb[3], hb[3];
+    std::vector<R> pb_eyx, pb_ezx, pb_hyx, pb_hzx, pb_exy, pb_ezy, pb_hxy, pb_hzy, pb_exz, pb_eyz, pb_hxz, pb_hyz;
+    std::vector<R> obj_g;             // [probe][nsteps] dJ/d(probe sample)
+    std::vector<double> gedge[3];     // dJ/d(averaged eps) per E edge of the dilated design box
+    std::vector<R> hist;              // [m][3][dilated design box] forward E history
+
+    int quant = 0;
+    float qsE = 1.0f, qsH = 1.0f;
+    int qblock = 64;
+    std::vector<char> keep;
+
     i64 id(int i, int j, int k) const { return (i64)i + (i64)NX * ((i64)j + (i64)NY * k); }
 };
+
+static inline float round_store(float v, int mode)
+{
+    if (mode == 16) return _cvtsh_ss(_cvtss_sh(v, 0));   // FP16, round to nearest even
+    if (mode == 17) {                                     // BF16, round to nearest even
+        std::uint32_t u;
+        std::memcpy(&u, &v, 4);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        u &= 0xFFFF0000u;
+        std::memcpy(&v, &u, 4);
+    }
+    return v;
+}
+
+template <class R>
+static void quantize(Sim<R> *s, bool efield)
+{
+    if (s->quant == 0) return;
+    const float sc = efield ? s->qsE : s->qsH;    // fields
+    const float sp = efield ? s->qsH : s->qsE;    // psi of this sweep holds curl-operand units
+    std::vector<R> *f[3] = {efield ? &s->Ex : &s->Hx, efield ? &s->Ey : &s->Hy, efield ? &s->Ez : &s->Hz};
+    std::vector<R> *p[6] = {efield ? &s->Psi_eyx : &s->Psi_hyx, efield ? &s->Psi_ezx : &s->Psi_hzx,
+                            efield ? &s->Psi_exy : &s->Psi_hxy, efield ? &s->Psi_ezy : &s->Psi_hzy,
+                            efield ? &s->Psi_exz : &s->Psi_hxz, efield ? &s->Psi_eyz : &s->Psi_hyz};
+    const bool k = !s->keep.empty();
+    if (s->quant == 18) {
+        // block-scaled int16: one power-of-two scale per aligned run of
+        // qblock cells along x, mantissas rounded to nearest in [-32767, 32767]
+        const int B = s->qblock;
+        const int nb = (s->NX + B - 1) / B;
+        std::vector<R> *all[9] = {f[0], f[1], f[2], p[0], p[1], p[2], p[3], p[4], p[5]};
+#pragma omp parallel for schedule(static)
+        for (i64 r = 0; r < (i64)s->NY * s->NZ; ++r)
+            for (int a = 0; a < 9; ++a) {
+                R *row = all[a]->data() + r * s->NX;
+                for (int b = 0; b < nb; ++b) {
+                    const int i0 = b * B, i1 = std::min(s->NX, i0 + B);
+                    double mx = 0.0;
+                    for (int i = i0; i < i1; ++i) mx = std::max(mx, std::fabs((double)row[i]));
+                    if (mx == 0.0) continue;
+                    int e;
+                    std::frexp(mx / 32767.0, &e);      // 2^e >= mx/32767
+                    const double q = std::ldexp(1.0, e);
+                    for (int i = i0; i < i1; ++i) row[i] = (R)(std::nearbyint((double)row[i] / q) * q);
+                }
+            }
+        return;
+    }
+#pragma omp parallel for schedule(static)
+    for (i64 x = 0; x < s->N; ++x) {
+        if (k && s->keep[x]) continue;
+        for (int c = 0; c < 3; ++c) (*f[c])[x] = (R)(round_store((float)(*f[c])[x] * sc, s->quant) / sc);
+        for (int c = 0; c < 6; ++c) (*p[c])[x] = (R)(round_store((float)(*p[c])[x] * sp, s->quant) / sp);
+    }
+}
 
 template <class R>
 static double waveform(int type, const double *p, double t)
@@ -218,6 +293,10 @@ static Sim<R> *build(const oracle_scene *sc)
     for (int c = 0; c < 3; ++c) {
         std::vector<R>().swap(m_eps[c]); std::vector<R>().swap(m_se[c]);
         std::vector<R>().swap(m_mu[c]);  std::vector<R>().swap(m_sm[c]);
+    }
+    for (int q = 0; q < 6; ++q) s->design[q] = sc->design[q];
+    if (sc->design_eps && sc->design[3] > sc->design[0]) {
+        for (int c = 0; c < 3; ++c) { s->eps_avg[c] = eps[c]; s->se_avg[c] = se[c]; }
     }
 
     // ---- coefficients, setCoefficients H:1247-1305
@@ -476,7 +555,7 @@ static void step_e(Sim<R> *s, int m)
                     if (j > 0) Ex[x] = Ex[x] + s->cpsi_exz[x] * s->Psi_exz[x];
                     if (i > 0) Ey[x] = Ey[x] + s->cpsi_eyz[x] * s->Psi_eyz[x];
                 }
-            }
+            }    quantize(s, true);
 }
 
 // calc_h, H:273-562.
@@ -527,7 +606,7 @@ static void step_h(Sim<R> *s, int m)
                     if (j < NY - 1) Hx[x] = Hx[x] + s->cpsi_hxz[x] * s->Psi_hxz[x];
                     if (i < NX - 1) Hy[x] = Hy[x] + s->cpsi_hyz[x] * s->Psi_hyz[x];
                 }
-            }
+            }    quantize(s, false);
 }
 
 template <class R>
@@ -542,8 +621,292 @@ static const std::vector<R> *lookup(Sim<R> *s, const std::string &n)
     O(Psi_exy) O(Psi_ezy) O(Psi_hxy) O(Psi_hzy) O(cpsi_exy) O(cpsi_ezy) O(cpsi_hxy) O(cpsi_hzy)
     O(Psi_exz) O(Psi_eyz) O(Psi_hxz) O(Psi_hyz) O(cpsi_exz) O(cpsi_eyz) O(cpsi_hxz) O(cpsi_hyz)
     O(vpn)
+    O(pb_eyx) O(pb_ezx) O(pb_hyx) O(pb_hzx) O(pb_exy) O(pb_ezy) O(pb_hxy) O(pb_hzy)
+    O(pb_exz) O(pb_eyz) O(pb_hxz) O(pb_hyz) O(obj_g)
+    if (n == "eb_x") return &s->eb[0];
+    if (n == "eb_y") return &s->eb[1];
+    if (n == "eb_z") return &s->eb[2];
+    if (n == "hb_x") return &s->hb[0];
+    if (n == "hb_y") return &s->hb[1];
+    if (n == "hb_z") return &s->hb[2];
 #undef O
     return nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint (no counterpart in the reference).  The time step is
+// linear in the state (E, H, psi_e, psi_h), so the adjoint is its exact
+// transpose, written here in scatter form on the full-grid arrays: every
+// forward term  F(c) += C * (G(c+d) - G(c))  becomes
+// Gb(c+d) += C * Fb(c), Gb(c) -= C * Fb(c).  Reverse step m undoes the H
+// half-step, injects dJ/d(probe sample m), then undoes the E half-step and
+// accumulates dJ/d(averaged eps) on the design edges.
+template <class R>
+static void adj_alloc(Sim<R> *s)
+{
+    auto Z = [s](std::vector<R> &v) { v.assign(s->N, (R)0); };
+    for (int c = 0; c < 3; ++c) { Z(s->eb[c]); Z(s->hb[c]); }
+    Z(s->pb_eyx); Z(s->pb_ezx); Z(s->pb_hyx); Z(s->pb_hzx); Z(s->pb_exy); Z(s->pb_ezy);
+    Z(s->pb_hxy); Z(s->pb_hzy); Z(s->pb_exz); Z(s->pb_eyz); Z(s->pb_hxz); Z(s->pb_hyz);
+    if (s->obj_g.size() != (size_t)s->nprobe * s->nsteps) s->obj_g.assign((size_t)s->nprobe * s->nsteps, (R)0);
+}
+
+template <class R>
+static void reverse_h(Sim<R> *s)
+{
+    const int NX = s->NX, NY = s->NY, NZ = s->NZ;
+    const i64 NXY = s->NXY;
+    R *ebx = s->eb[0].data(), *eby = s->eb[1].data(), *ebz = s->eb[2].data();
+    R *hbx = s->hb[0].data(), *hby = s->hb[1].data(), *hbz = s->hb[2].data();
+    const int Lxn = s->L[0], Lxp = s->L[1], Lyn = s->L[2], Lyp = s->L[3], Lzn = s->L[4], Lzp = s->L[5];
+    // scatter (serial: neighbouring cells write the same E entries)
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                const i64 c = s->id(i, j, k);
+                if (j < NY - 1 && k < NZ - 1) {          // Hx main term
+                    const R t1 = s->Chxey[c] * hbx[c], t2 = s->Chxez[c] * hbx[c];
+                    eby[c + NXY] += t1; eby[c] -= t1;
+                    ebz[c + NX] += t2; ebz[c] -= t2;
+                }
+                if (i < NX - 1 && k < NZ - 1) {          // Hy
+                    const R t1 = s->Chyez[c] * hby[c], t2 = s->Chyex[c] * hby[c];
+                    ebz[c + 1] += t1; ebz[c] -= t1;
+                    ebx[c + NXY] += t2; ebx[c] -= t2;
+                }
+                if (i < NX - 1 && j < NY - 1) {          // Hz
+                    const R t1 = s->Chzex[c] * hbz[c], t2 = s->Chzey[c] * hbz[c];
+                    ebx[c + NX] += t1; ebx[c] -= t1;
+                    eby[c + 1] += t2; eby[c] -= t2;
+                }
+                if ((i < Lxn || (i > NX - Lxp - 1 && i < NX - 1)) && j < NY && k < NZ) {
+                    R u1 = s->pb_hyx[c] + (k < NZ - 1 ? s->cpsi_hyx[c] * hby[c] : (R)0);
+                    R u2 = s->pb_hzx[c] + (j < NY - 1 ? s->cpsi_hzx[c] * hbz[c] : (R)0);
+                    ebz[c + 1] += s->a_mx[i] * u1; ebz[c] -= s->a_mx[i] * u1;
+                    eby[c + 1] += s->a_mx[i] * u2; eby[c] -= s->a_mx[i] * u2;
+                    s->pb_hyx[c] = s->b_mx[i] * u1;
+                    s->pb_hzx[c] = s->b_mx[i] * u2;
+                }
+                if (i < NX && (j < Lyn || (j > NY - Lyp - 1 && j < NY - 1)) && k < NZ) {
+                    R u1 = s->pb_hxy[c] + (k < NZ - 1 ? s->cpsi_hxy[c] * hbx[c] : (R)0);
+                    R u2 = s->pb_hzy[c] + (i < NX - 1 ? s->cpsi_hzy[c] * hbz[c] : (R)0);
+                    ebz[c + NX] += s->a_my[j] * u1; ebz[c] -= s->a_my[j] * u1;
+                    ebx[c + NX] += s->a_my[j] * u2; ebx[c] -= s->a_my[j] * u2;
+                    s->pb_hxy[c] = s->b_my[j] * u1;
+                    s->pb_hzy[c] = s->b_my[j] * u2;
+                }
+                if (i < NX && j < NY && (k < Lzn || (k > NZ - Lzp - 1 && k < NZ - 1))) {
+                    R u1 = s->pb_hxz[c] + (j < NY - 1 ? s->cpsi_hxz[c] * hbx[c] : (R)0);
+                    R u2 = s->pb_hyz[c] + (i < NX - 1 ? s->cpsi_hyz[c] * hby[c] : (R)0);
+                    eby[c + NXY] += s->a_mz[k] * u1; eby[c] -= s->a_mz[k] * u1;
+                    ebx[c + NXY] += s->a_mz[k] * u2; ebx[c] -= s->a_mz[k] * u2;
+                    s->pb_hxz[c] = s->b_mz[k] * u1;
+                    s->pb_hyz[c] = s->b_mz[k] * u2;
+                }
+            }
+    // pointwise: dJ/dH(old) = Ch * dJ/dH(new) on updated cells
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                const i64 c = s->id(i, j, k);
+                if (j < NY - 1 && k < NZ - 1) hbx[c] = s->Chxh[c] * hbx[c];
+                if (i < NX - 1 && k < NZ - 1) hby[c] = s->Chyh[c] * hby[c];
+                if (i < NX - 1 && j < NY - 1) hbz[c] = s->Chzh[c] * hbz[c];
+            }
+}
+
+// dilated design box: edges i in [d0, d3], j in [d1, d4], k in [d2, d5]
+template <class R>
+static void dbox(const Sim<R> *s, int &bx, int &by, int &bz)
+{
+    bx = s->design[3] - s->design[0] + 1;
+    by = s->design[4] - s->design[1] + 1;
+    bz = s->design[5] - s->design[2] + 1;
+}
+
+template <class R>
+static void snap(Sim<R> *s, int m)
+{
+    int bx, by, bz;
+    dbox(s, bx, by, bz);
+    const i64 nb = (i64)bx * by * bz;
+    const std::vector<R> *F[3] = {&s->Ex, &s->Ey, &s->Ez};
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < bz; ++k)
+            for (int j = 0; j < by; ++j)
+                for (int i = 0; i < bx; ++i)
+                    s->hist[((size_t)m * 3 + c) * nb + i + (i64)bx * (j + (i64)by * k)] =
+                        (*F[c])[s->id(s->design[0] + i, s->design[1] + j, s->design[2] + k)];
+}
+
+template <class R>
+static void reverse_e(Sim<R> *s, int m)
+{
+    const int NX = s->NX, NY = s->NY, NZ = s->NZ;
+    const i64 NXY = s->NXY;
+    R *ebx = s->eb[0].data(), *eby = s->eb[1].data(), *ebz = s->eb[2].data();
+    R *hbx = s->hb[0].data(), *hby = s->hb[1].data(), *hbz = s->hb[2].data();
+    const int Lxn = s->L[0], Lxp = s->L[1], Lyn = s->L[2], Lyp = s->L[3], Lzn = s->L[4], Lzp = s->L[5];
+    // parameter gradient on the design edges: dE'/deps = dCe*E^m - beta*(E^{m+1} - Ce*E^m)
+    if (!s->hist.empty()) {
+        int bx, by, bz;
+        dbox(s, bx, by, bz);
+        const i64 nb = (i64)bx * by * bz;
+        const R *CE[3] = {s->Cexe.data(), s->Ceye.data(), s->Ceze.data()};
+        for (int c = 0; c < 3; ++c)
+            for (int k = 0; k < bz; ++k)
+                for (int j = 0; j < by; ++j)
+                    for (int i = 0; i < bx; ++i) {
+                        const int gi = s->design[0] + i, gj = s->design[1] + j, gk = s->design[2] + k;
+                        bool upd = c == 0 ? (gi < NX - 1 && gj > 0 && gj < NY - 1 && gk > 0 && gk < NZ - 1)
+                                 : c == 1 ? (gi > 0 && gi < NX - 1 && gj < NY - 1 && gk > 0 && gk < NZ - 1)
+                                          : (gi > 0 && gi < NX - 1 && gj > 0 && gj < NY - 1 && gk < NZ - 1);
+                        if (!upd) continue;
+                        const i64 x = s->id(gi, gj, gk), q = i + (i64)bx * (j + (i64)by * k);
+                        const double e0 = (double)s->eps0, dt = (double)s->dt;
+                        const double ep = (double)s->eps_avg[c][x], sg = (double)s->se_avg[c][x];
+                        const double den = 2.0 * ep * e0 + dt * sg;
+                        const double beta = 2.0 * e0 / den, dce = 4.0 * e0 * dt * sg / (den * den);
+                        const double em = (double)s->hist[((size_t)m * 3 + c) * nb + q];
+                        const double ep1 = (double)s->hist[((size_t)(m + 1) * 3 + c) * nb + q];
+                        const double ce = (double)CE[c][x];
+                        s->gedge[c][q] += (double)s->eb[c][x] * (dce * em - beta * (ep1 - ce * em));
+                    }
+    }
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                const i64 x = s->id(i, j, k);
+                if (i < NX - 1 && j < NY - 1 && k < NZ - 1 && j > 0 && k > 0) {   // Ex
+                    const R t1 = s->Cexhz[x] * ebx[x], t2 = s->Cexhy[x] * ebx[x];
+                    hbz[x] += t1; hbz[x - NX] -= t1;
+                    hby[x] += t2; hby[x - NXY] -= t2;
+                }
+                if (i < NX - 1 && j < NY - 1 && k < NZ - 1 && k > 0 && i > 0) {   // Ey
+                    const R t1 = s->Ceyhx[x] * eby[x], t2 = s->Ceyhz[x] * eby[x];
+                    hbx[x] += t1; hbx[x - NXY] -= t1;
+                    hbz[x] += t2; hbz[x - 1] -= t2;
+                }
+                if (i < NX - 1 && j < NY - 1 && k < NZ - 1 && j > 0 && i > 0) {   // Ez
+                    const R t1 = s->Cezhy[x] * ebz[x], t2 = s->Cezhx[x] * ebz[x];
+                    hby[x] += t1; hby[x - 1] -= t1;
+                    hbx[x] += t2; hbx[x - NX] -= t2;
+                }
+                if (((i > 0 && i < Lxn) || (i > NX - Lxp - 1 && i < NX - 1)) && j < NY - 1 && k < NZ - 1) {
+                    R w1 = s->pb_eyx[x] + (k > 0 ? s->cpsi_eyx[x] * eby[x] : (R)0);
+                    R w2 = s->pb_ezx[x] + (j > 0 ? s->cpsi_ezx[x] * ebz[x] : (R)0);
+                    hbz[x] += s->a_ex[i] * w1; hbz[x - 1] -= s->a_ex[i] * w1;
+                    hby[x] += s->a_ex[i] * w2; hby[x - 1] -= s->a_ex[i] * w2;
+                    s->pb_eyx[x] = s->b_ex[i] * w1;
+                    s->pb_ezx[x] = s->b_ex[i] * w2;
+                }
+                if (i < NX - 1 && ((j > 0 && j < Lyn) || (j > NY - Lyp - 1 && j < NY - 1)) && k < NZ - 1) {
+                    R w1 = s->pb_exy[x] + (k > 0 ? s->cpsi_exy[x] * ebx[x] : (R)0);
+                    R w2 = s->pb_ezy[x] + (i > 0 ? s->cpsi_ezy[x] * ebz[x] : (R)0);
+                    hbz[x] += s->a_ey[j] * w1; hbz[x - NX] -= s->a_ey[j] * w1;
+                    hbx[x] += s->a_ey[j] * w2; hbx[x - NX] -= s->a_ey[j] * w2;
+                    s->pb_exy[x] = s->b_ey[j] * w1;
+                    s->pb_ezy[x] = s->b_ey[j] * w2;
+                }
+                if (i < NX - 1 && j < NY - 1 && ((k > 0 && k < Lzn) || (k > NZ - Lzp - 1 && k < NZ - 1))) {
+                    R w1 = s->pb_exz[x] + (j > 0 ? s->cpsi_exz[x] * ebx[x] : (R)0);
+                    R w2 = s->pb_eyz[x] + (i > 0 ? s->cpsi_eyz[x] * eby[x] : (R)0);
+                    hby[x] += s->a_ez[k] * w1; hby[x - NXY] -= s->a_ez[k] * w1;
+                    hbx[x] += s->a_ez[k] * w2; hbx[x - NXY] -= s->a_ez[k] * w2;
+                    s->pb_exz[x] = s->b_ez[k] * w1;
+                    s->pb_eyz[x] = s->b_ez[k] * w2;
+                }
+            }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                const i64 x = s->id(i, j, k);
+                if (i < NX - 1 && j < NY - 1 && k < NZ - 1 && j > 0 && k > 0) ebx[x] = s->Cexe[x] * ebx[x];
+                if (i < NX - 1 && j < NY - 1 && k < NZ - 1 && k > 0 && i > 0) eby[x] = s->Ceye[x] * eby[x];
+                if (i < NX - 1 && j < NY - 1 && k < NZ - 1 && j > 0 && i > 0) ebz[x] = s->Ceze[x] * ebz[x];
+            }
+}
+
+// one reverse step m: H half, probe injection, E half
+template <class R>
+static void reverse_step(Sim<R> *s, int m)
+{
+    reverse_h(s);
+    std::vector<R> *EB[3] = {&s->eb[0], &s->eb[1], &s->eb[2]};
+    for (int p = 0; p < s->nprobe; ++p) {
+        const int c = s->pcomp[p];
+        if (c > 2) continue;
+        (*EB[c])[s->pidx[p]] += s->obj_g[(size_t)p * s->nsteps + m];
+    }
+    reverse_e(s, m);
+}
+
+template <class R>
+static void reset_state(Sim<R> *s)
+{
+    auto Z = [](std::vector<R> &v) { std::fill(v.begin(), v.end(), (R)0); };
+    Z(s->Ex); Z(s->Ey); Z(s->Ez); Z(s->Hx); Z(s->Hy); Z(s->Hz);
+    Z(s->Psi_eyx); Z(s->Psi_ezx); Z(s->Psi_hyx); Z(s->Psi_hzx); Z(s->Psi_exy); Z(s->Psi_ezy);
+    Z(s->Psi_hxy); Z(s->Psi_hzy); Z(s->Psi_exz); Z(s->Psi_eyz); Z(s->Psi_hxz); Z(s->Psi_hyz);
+    std::fill(s->phist.begin(), s->phist.end(), 0.0);
+    std::fill(s->dft.begin(), s->dft.end(), 0.0);
+    s->step = 0;
+}
+
+// J = sum_t w_t |DFT(p_t, f_t)|^2 and dJ/d(eps_r) of every design cell.
+template <class R>
+static int adjoint_gradient(Sim<R> *s, int nt, const int *tp, const int *tf, const double *tw, double *J,
+                            double *grad)
+{
+    if (s->eps_avg[0].empty()) return -1;   // no design region
+    const int M = s->nsteps;
+    int bx, by, bz;
+    dbox(s, bx, by, bz);
+    const i64 nb = (i64)bx * by * bz;
+    reset_state(s);
+    s->hist.assign((size_t)(M + 1) * 3 * nb, (R)0);
+    snap(s, 0);
+    for (int m = 0; m < M; ++m) {
+        step_e(s, m);
+        step_h(s, m);
+        snap(s, m + 1);
+    }
+    s->step = M;
+    // objective and its derivative w.r.t. every probe sample
+    double Jv = 0.0;
+    adj_alloc(s);
+    for (int t = 0; t < nt; ++t) {
+        const double *X = &s->dft[((size_t)tp[t] * s->nfreq + tf[t]) * 2];
+        const double w = tw ? tw[t] : 1.0;
+        Jv += w * (X[0] * X[0] + X[1] * X[1]);
+        for (int m = 0; m < M; ++m) {
+            const double pr = s->ph_re[(size_t)m * s->nfreq + tf[t]], pi = s->ph_im[(size_t)m * s->nfreq + tf[t]];
+            // d|X|^2/ds = 2 Re(conj(X) * phasor)
+            s->obj_g[(size_t)tp[t] * M + m] += (R)(w * 2.0 * (X[0] * pr + X[1] * pi));
+        }
+    }
+    *J = Jv;
+    for (int c = 0; c < 3; ++c) s->gedge[c].assign(nb, 0.0);
+    for (int m = M - 1; m >= 0; --m) reverse_step(s, m);
+    // chain rule through the 4-cell averages (fdtdrav.h:1135-1177)
+    const int *d = s->design;
+    const int dnx = d[3] - d[0], dny = d[4] - d[1], dnz = d[5] - d[2];
+    for (int k = 0; k < dnz; ++k)
+        for (int j = 0; j < dny; ++j)
+            for (int i = 0; i < dnx; ++i) {
+                auto q = [&](int a, int b, int c) { return a + (i64)bx * (b + (i64)by * c); };
+                double g = 0.0;
+                for (int b = 0; b < 2; ++b)
+                    for (int c = 0; c < 2; ++c) {
+                        g += s->gedge[0][q(i, j + b, k + c)];   // Ex edges (i, j..j+1, k..k+1)
+                        g += s->gedge[1][q(i + b, j, k + c)];   // Ey edges (i..i+1, j, k..k+1)
+                        g += s->gedge[2][q(i + b, j + c, k)];   // Ez edges (i..i+1, j..j+1, k)
+                    }
+                grad[i + (i64)dnx * (j + (i64)dny * k)] = 0.25 * g;
+            }
+    return 0;
 }
 
 struct Handle {
@@ -634,6 +997,71 @@ long oracle_get_array(void *hp, const char *name, double *out)
     if (!v) return -1;
     if (out) for (size_t i = 0; i < v->size(); ++i) out[i] = (*v)[i];
     return (long)v->size();
+}
+
+/* J = sum_t w_t |DFT(probe_t, freq_t)|^2 and dJ/d(eps_r) per design cell,
+ * by a forward run of nsteps that records E on the design edges, then the
+ * reverse sweep.  FP64 oracle only. */
+int oracle_adjoint(void *hp, int nt, const int *tp, const int *tf, const double *tw, double *J, double *grad)
+{
+    Handle *h = (Handle *)hp;
+    if (h->prec != 64) return -1;
+    return adjoint_gradient((Sim<double> *)h->sim, nt, tp, tf, tw, J, grad);
+}
+
+/* Linear-operator access for transpose (dot-product) tests: allocate the
+ * adjoint arrays, run one forward step m or one reverse step m on the
+ * current arrays (set through oracle_set_array). */
+int oracle_adjoint_alloc(void *hp)
+{
+    Handle *h = (Handle *)hp;
+    if (h->prec == 32) adj_alloc((Sim<float> *)h->sim);
+    else adj_alloc((Sim<double> *)h->sim);
+    return 0;
+}
+
+int oracle_forward_step(void *hp, int m)
+{
+    Handle *h = (Handle *)hp;
+    if (h->prec == 32) { step_e((Sim<float> *)h->sim, m); step_h((Sim<float> *)h->sim, m); }
+    else { step_e((Sim<double> *)h->sim, m); step_h((Sim<double> *)h->sim, m); }
+    return 0;
+}
+
+int oracle_reverse_step(void *hp, int m)
+{
+    Handle *h = (Handle *)hp;
+    if (h->prec == 32) reverse_step((Sim<float> *)h->sim, m);
+    else reverse_step((Sim<double> *)h->sim, m);
+    return 0;
+}
+
+/* Storage-rounding model for error studies (FP32 oracle only): mode 0/16/17,
+ * stored = value * scale (sE for E, sH for H); cells within Chebyshev radius
+ * `keep_radius` of a source edge stay unrounded (keep_radius < 0: none). */
+int oracle_set_quant(void *hp, int mode, double sE, double sH, int keep_radius)
+{
+    Handle *h = (Handle *)hp;
+    if (h->prec != 32) return -1;
+    Sim<float> *s = (Sim<float> *)h->sim;
+    s->quant = mode >= 18 ? 18 : mode;
+    if (mode > 18) s->qblock = mode - 18;   // 18 + block size selects the block length
+    s->qsE = (float)sE;
+    s->qsH = (float)sH;
+    s->keep.clear();
+    if (keep_radius >= 0) {
+        s->keep.assign(s->N, 0);
+        for (int k = 0; k < s->NZ; ++k)
+            for (int j = 0; j < s->NY; ++j)
+                for (int i = 0; i < s->NX; ++i) {
+                    if (s->src_of[s->id(i, j, k)] < 0) continue;
+                    for (int c = std::max(0, k - keep_radius); c <= std::min(s->NZ - 1, k + keep_radius); ++c)
+                        for (int b = std::max(0, j - keep_radius); b <= std::min(s->NY - 1, j + keep_radius); ++b)
+                            for (int a = std::max(0, i - keep_radius); a <= std::min(s->NX - 1, i + keep_radius); ++a)
+                                s->keep[s->id(a, b, c)] = 1;
+                }
+    }
+    return 0;
 }
 
 /* Overwrite a named field array (tests: linearity, dot-product checks). */

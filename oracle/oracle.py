@@ -52,6 +52,13 @@ def load():
         lib.oracle_get_dft.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         lib.oracle_get_array.restype = C.c_long
         lib.oracle_get_array.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]
+        lib.oracle_adjoint.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+        lib.oracle_adjoint_alloc.argtypes = [C.c_void_p]
+        lib.oracle_forward_step.argtypes = [C.c_void_p, C.c_int]
+        lib.oracle_reverse_step.argtypes = [C.c_void_p, C.c_int]
+        lib.oracle_set_quant.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]
         lib.oracle_set_array.restype = C.c_long
         lib.oracle_set_array.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]
         _lib = lib
@@ -164,6 +171,41 @@ class Oracle:
 
     def field(self, name):
         return self.array(name).reshape(self.sc.nz, self.sc.ny, self.sc.nx)
+
+    def set_quant(self, mode, sE=1.0, sH=1.0, keep_radius=-1):
+        """Round every stored E/H/psi value to FP16 (16) or BF16 (17) in
+        scaled units (FP32 oracle only) -- a model of reduced-precision
+        storage for error studies."""
+        if self.lib.oracle_set_quant(self.h, int(mode), float(sE), float(sH), int(keep_radius)) != 0:
+            raise ValueError("quantisation needs the FP32 oracle")
+
+    def adjoint(self, terms):
+        """FP64 discrete adjoint: terms = [(probe, freq_index, weight)].
+        Returns (J, dJ/d eps_r per design cell, x-fastest)."""
+        sc = self.sc
+        n = len(terms)
+        tp = (C.c_int * n)(*[t[0] for t in terms])
+        tf = (C.c_int * n)(*[t[1] for t in terms])
+        tw = (C.c_double * n)(*[t[2] for t in terms])
+        d = sc.design
+        g = np.zeros((d[3] - d[0]) * (d[4] - d[1]) * (d[5] - d[2]), dtype=np.float64)
+        J = C.c_double()
+        rc = self.lib.oracle_adjoint(self.h, n, tp, tf, tw, C.byref(J),
+                                     g.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc != 0:
+            raise RuntimeError("oracle adjoint needs the FP64 oracle and a design region")
+        return J.value, g
+
+    def adjoint_alloc(self):
+        self.lib.oracle_adjoint_alloc(self.h)
+
+    def forward_step(self, m):
+        """One E+H step on the current arrays (no bookkeeping)."""
+        self.lib.oracle_forward_step(self.h, int(m))
+
+    def reverse_step(self, m):
+        """One reverse (transposed) step on the adjoint arrays."""
+        self.lib.oracle_reverse_step(self.h, int(m))
 
     def set_array(self, name, values):
         v = np.ascontiguousarray(values, dtype=np.float64).ravel()

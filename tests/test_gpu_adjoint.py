@@ -1,0 +1,68 @@
+"""GPU discrete adjoint: against the FP64 CPU adjoint, between checkpoint
+intervals, and against central finite differences of GPU forward runs."""
+import numpy as np
+import pytest
+
+from fdtd_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def small_design(**kw):
+    base = dict(nx=28, ny=24, nz=22, steps=160, L=4, d=1e-3, design=(6, 5, 4), eps_range=(1.0, 4.0),
+                seed=9, probe_freqs=(12e9, 20e9), name="adjoint-small")
+    base.update(kw)
+    return scenes.adjoint(**base)
+
+
+TERMS = [(0, 0, 1.0), (0, 1, 0.5)]
+
+
+def objective(sim):
+    X = sim.dft()
+    return sum(w * abs(X[p, f]) ** 2 for p, f, w in TERMS)
+
+
+def test_adjoint_matches_fp64_oracle(engine):
+    from oracle.oracle import Oracle
+    sc = small_design()
+    print(f"[adjoint] {sc.nx}x{sc.ny}x{sc.nz} steps={sc.steps}", flush=True)
+    sim = engine.Simulation.from_scene(sc, "f32")
+    J, g = sim.adjoint_gradient(TERMS, sc.steps, 0)
+    Jo, go = Oracle(sc, 64).adjoint(TERMS)
+    assert abs(J - Jo) / Jo < 1e-4
+    rel = np.linalg.norm(g - go) / np.linalg.norm(go)
+    print(f"[adjoint] J gpu {J:.6e} fp64 {Jo:.6e}; gradient rel L2 {rel:.2e}", flush=True)
+    assert rel < 1e-3
+
+
+def test_checkpointed_equals_stored_history(engine):
+    sc = small_design(steps=150)
+    a = engine.Simulation.from_scene(sc, "f32")
+    J0, g0 = a.adjoint_gradient(TERMS, sc.steps, 0)
+    assert a.info()["adjoint_segments"] == 1
+    J1, g1 = a.adjoint_gradient(TERMS, sc.steps, 32)
+    assert a.info()["adjoint_segments"] == 5
+    assert J0 == J1
+    np.testing.assert_array_equal(g0, g1)
+
+
+def test_adjoint_vs_central_differences(engine):
+    sc = small_design()
+    sim = engine.Simulation.from_scene(sc, "f32")
+    _, g = sim.adjoint_gradient(TERMS, sc.steps, 0)
+    eps = sc.design_eps.astype(np.float32)
+    rng = np.random.default_rng(4)
+    cells = rng.choice(eps.size, 4, replace=False)
+    for q in cells:
+        vals, ev = [], []
+        for sgn in (1, -1):
+            e2 = eps.copy()
+            e2[q] = np.float32(eps[q] + sgn * 5e-2)
+            ev.append(float(e2[q]))
+            sim.set_design_eps(e2)
+            sim.run(sc.steps)
+            vals.append(objective(sim))
+        fd = (vals[0] - vals[1]) / (ev[0] - ev[1])
+        print(f"[adjoint] cell {q}: adjoint {g[q]:.6e} fd {fd:.6e}", flush=True)
+        assert abs(g[q] - fd) <= 2e-3 * np.abs(g).max()
