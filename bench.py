@@ -29,22 +29,21 @@ METRIC = "Mcell-updates/s and % HBM peak; FP16-store vs FP32 at stated probe err
 
 
 def b_alg(sc, info, storage_bytes):
-    """Algorithmic HBM bytes per cell for the E and the H sweep.
+    """Algorithmic HBM bytes per cell-update of one time step.
 
-    Each sweep reads and writes its own three components and reads the other
-    three (9 values); a classed sweep reads one class index per cell; each
-    CPML psi (2 per axis per sweep) is read and written over its slab."""
+    One-kernel step: every field component is read once and written once (12
+    values); a classed E update reads one class index per cell; each CPML psi
+    (4 per axis) is read and written over its slab (fraction 2L/N of the
+    axis).  With the two half-step kernels each sweep reads and writes its
+    own three components and reads the other three (18 values per step)."""
     s = storage_bytes
     f = [(sc.cpml.thick[0] + sc.cpml.thick[1]) / sc.nx, (sc.cpml.thick[2] + sc.cpml.thick[3]) / sc.ny,
          (sc.cpml.thick[4] + sc.cpml.thick[5]) / sc.nz]
-    psi = 4.0 * s * sum(f)
-    cls = {0: 0.0}
+    psi = 8.0 * s * sum(f)
     ncls = info["n_classes"]
     cls_bytes = 0.0 if ncls <= 1 else (2.0 if ncls <= 65536 else 4.0)
-    e = 9 * s + cls_bytes + psi
-    h = 9 * s + psi   # H coefficients are uniform in these workloads (mu_r = 1, sigma_m = 0)
-    del cls
-    return e, h
+    fields = 12 * s if info["fused"] else 18 * s
+    return fields + cls_bytes + psi
 
 
 def run_form(sc, storage, steps, warmup, profile_steps):
@@ -61,7 +60,7 @@ def run_form(sc, storage, steps, warmup, profile_steps):
     probes = sim.probes()[:, warmup:warmup + steps]
     sim.close()
     return dict(wall_s=wall, event_ms=ev_ms, e_ms=prof["e_kernel_ms"], h_ms=prof["h_kernel_ms"],
-                info=inf, probes=probes)
+                step_ms=prof["step_kernel_ms"], info=inf, probes=probes)
 
 
 def cpu_baseline(sc, seconds_target=15.0):
@@ -85,6 +84,64 @@ def cpu_baseline(sc, seconds_target=15.0):
                       f"the full {sc.nx}x{sc.ny}x{sc.nz} {sc.name} grid ({dt:.1f} s)"}
 
 
+def adjoint_report(args):
+    """Adjoint gradient of J = sum |DFT_Ez(probe, f)|^2 w.r.t. the design
+    cells' eps_r: wall time against a plain forward run, and the gradient
+    against central finite differences (h = 1e-3, FP32 forward runs) on
+    seeded cells and, for the small workload, against the FP64 CPU adjoint."""
+    wl = args.workload if args.workload.startswith("adjoint") else "adjoint128"
+    sc = scenes.WORKLOADS[wl]()
+    terms = [(p, f, 1.0) for p in range(len(sc.probes)) for f in range(len(sc.freqs))]
+    sim = api.Simulation.from_scene(sc, args.storage)
+    sim.run(sc.steps)
+    fwd_ms = sim.info()["last_run_ms"]
+    sim.reset()
+    t0 = time.perf_counter()
+    J, g = sim.adjoint_gradient(terms, sc.steps, args.checkpoint_every)
+    wall = time.perf_counter() - t0
+    inf = sim.info()
+    out = {"workload": wl, "grid": [sc.nx, sc.ny, sc.nz], "steps": sc.steps,
+           "design_cells": int(g.size), "storage": args.storage, "J": J,
+           "forward_ms": round(fwd_ms, 3), "adjoint_total_ms": round(inf["adjoint_total_ms"], 3),
+           "adjoint_forward_ms": round(inf["adjoint_forward_ms"], 3),
+           "ratio_vs_forward": round(inf["adjoint_total_ms"] / fwd_ms, 3),
+           "checkpoint_every": args.checkpoint_every, "segments": inf["adjoint_segments"],
+           "wall_s": round(wall, 3)}
+    # central differences on seeded cells, FP32 forward runs
+    rng = np.random.default_rng(sc.seed)
+    cells = rng.choice(g.size, min(args.fd_cells, g.size), replace=False)
+    fsim = sim if args.storage == "f32" else api.Simulation.from_scene(sc, "f32")
+    if args.storage != "f32":
+        _, g32 = fsim.adjoint_gradient(terms, sc.steps, args.checkpoint_every)
+        out["grad_rel_l2_vs_fp32_adjoint"] = float(np.linalg.norm(g - g32) / np.linalg.norm(g32))
+    eps = sc.design_eps.astype(np.float32)
+    h = 1e-3
+    errs = []
+    for q in cells:
+        vals, ev = [], []
+        for sgn in (1, -1):
+            e2 = eps.copy()
+            e2[q] = np.float32(eps[q] + sgn * h)
+            ev.append(float(e2[q]))
+            fsim.set_design_eps(e2)
+            fsim.run(sc.steps)
+            X = fsim.dft()
+            vals.append(sum(w * abs(X[p, f]) ** 2 for p, f, w in terms))
+        fd = (vals[0] - vals[1]) / (ev[0] - ev[1])
+        errs.append(abs(g[q] - fd) / max(abs(fd), 1e-300))
+    gmax = float(np.abs(g).max())
+    out["fd"] = {"cells": len(cells), "h": h, "rel_err_median": float(np.median(errs)),
+                 "rel_err_max": float(np.max(errs))}
+    if wl == "adjoint128" and not args.no_cpu:
+        from oracle.oracle import Oracle
+        t0 = time.perf_counter()
+        Jo, go = Oracle(sc, 64).adjoint(terms)
+        out["fp64_cpu_adjoint"] = {"grad_rel_l2": float(np.linalg.norm(g - go) / np.linalg.norm(go)),
+                                   "J_rel": abs(J - Jo) / Jo, "cpu_s": round(time.perf_counter() - t0, 1)}
+    out["grad_max_abs"] = gmax
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +155,12 @@ def main():
                     help="only time the plain HBM copy kernel (measurement calibration)")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--copy-iters", type=int, default=20)
+    ap.add_argument("--copy-variants", action="store_true")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="adjoint-gradient report for --workload adjoint128 / adjoint768")
+    ap.add_argument("--checkpoint-every", type=int, default=64)
+    ap.add_argument("--fd-cells", type=int, default=16)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
     args = ap.parse_args()
     if args.copy_probe:
         import ctypes
@@ -105,8 +168,18 @@ def main():
         lib.hbm_copy_gbs.restype = ctypes.c_double
         lib.hbm_copy_gbs.argtypes = [ctypes.c_uint64, ctypes.c_int]
         gbs = lib.hbm_copy_gbs(args.copy_bytes, args.copy_iters)
-        print(json.dumps({"copy_gbs": round(gbs, 1), "bytes_each_way": args.copy_bytes,
-                          "iters": args.copy_iters, "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}))
+        out = {"copy_gbs": round(gbs, 1), "bytes_each_way": args.copy_bytes,
+               "iters": args.copy_iters, "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
+        if args.copy_variants:
+            lib.hbm_copy_variant_gbs.restype = ctypes.c_double
+            lib.hbm_copy_variant_gbs.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
+            names = ["grid_stride", "u1", "u2", "u4", "u8", "u1_nt", "u2_nt", "u4_nt", "u8_nt"]
+            out["variants_gbs"] = {nm: round(lib.hbm_copy_variant_gbs(args.copy_bytes, args.copy_iters, v), 1)
+                                   for v, nm in enumerate(names)}
+        print(json.dumps(out))
+        return
+    if args.adjoint:
+        print(json.dumps(adjoint_report(args)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
@@ -118,12 +191,10 @@ def main():
     f32 = run_form(sc, "f32", args.steps, args.warmup, args.profile_steps)
     ms_per_step = f32["event_ms"] / args.steps
     value = cells * args.steps / (f32["event_ms"] * 1e-3) / 1e6
-    be, bh = b_alg(sc, f32["info"], 4)
-    dom_e = f32["e_ms"] >= f32["h_ms"]
-    kms = f32["e_ms"] if dom_e else f32["h_ms"]
-    bpc = be if dom_e else bh
+    bpc = b_alg(sc, f32["info"], 4)
+    kms = f32["step_ms"]
     achieved = bpc * cells / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
-    step_gbs = (be + bh) * cells / (ms_per_step * 1e-3) / 1e9
+    step_gbs = bpc * cells / (ms_per_step * 1e-3) / 1e9
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "Mcell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
@@ -135,19 +206,15 @@ def main():
                    "coefficient_classes": f32["info"]["n_classes"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "e_kernel" if dom_e else "h_kernel",
+                     "kernel": "step_kernel" if f32["info"]["fused"] else "e_kernel+h_kernel",
                      "bytes_per_cell": round(bpc, 3), "kernel_ms": round(kms, 5)},
-        "kernels": {"e_ms": round(f32["e_ms"], 5), "h_ms": round(f32["h_ms"], 5),
-                    "b_alg_e": round(be, 3), "b_alg_h": round(bh, 3),
-                    "e_frac": round(be * cells / (max(f32["e_ms"], 1e-9) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                    "h_frac": round(bh * cells / (max(f32["h_ms"], 1e-9) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                    "step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "timed_loop_frac": round(step_gbs / HBM_PEAK_GBS, 4),
         "wall_s": round(f32["wall_s"], 3),
     }
     if not args.no_fp16:
         f16 = run_form(sc, "f16", args.steps, args.warmup, args.profile_steps)
         v16 = cells * args.steps / (f16["event_ms"] * 1e-3) / 1e6
-        e16, h16 = b_alg(sc, f16["info"], 2)
+        b16 = b_alg(sc, f16["info"], 2)
         p32 = f32["probes"].astype(np.float64)
         p16 = f16["probes"].astype(np.float64)
         live = np.linalg.norm(p32, axis=1) > 0
@@ -156,9 +223,8 @@ def main():
                        "speedup_vs_fp32": round(v16 / value, 3),
                        "probe_rel_l2_vs_fp32_max": float(f"{rel.max():.3e}"),
                        "probe_rel_l2_vs_fp32_median": float(f"{np.median(rel):.3e}"),
-                       "e_ms": round(f16["e_ms"], 5), "h_ms": round(f16["h_ms"], 5),
-                       "e_frac": round(e16 * cells / (max(f16["e_ms"], 1e-9) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                       "h_frac": round(h16 * cells / (max(f16["h_ms"], 1e-9) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                       "kernel_ms": round(f16["step_ms"], 5), "bytes_per_cell": round(b16, 3),
+                       "frac": round(b16 * cells / (max(f16["step_ms"], 1e-9) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(out), flush=True)

@@ -79,7 +79,8 @@ class FdtdInfo(C.Structure):
                 ("n_probes", C.c_int), ("n_freqs", C.c_int), ("n_sources", C.c_int),
                 ("n_classes", C.c_int), ("storage", C.c_int), ("dt", C.c_double),
                 ("last_run_ms", C.c_double), ("e_kernel_ms", C.c_double),
-                ("h_kernel_ms", C.c_double), ("adjoint_forward_ms", C.c_double),
+                ("h_kernel_ms", C.c_double), ("step_kernel_ms", C.c_double),
+                ("fused", C.c_int), ("pad1", C.c_int), ("adjoint_forward_ms", C.c_double),
                 ("adjoint_total_ms", C.c_double), ("adjoint_segments", C.c_int), ("pad0", C.c_int),
                 ("device_bytes", C.c_uint64),
                 ("bytes_per_cell_e", C.c_uint64), ("bytes_per_cell_h", C.c_uint64)]

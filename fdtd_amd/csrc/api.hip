@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <new>
 #include <cstring>
 #include <string>
@@ -25,6 +26,8 @@ hipError_t launch_half_step(bool efield, int storage, int mode, const KArgs &a, 
 hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a, const AArgs &b, int m, int fstep,
                       int sten, hipStream_t s);
 hipError_t launch_snap(int storage, const KArgs &a, float *hist, int slot, hipStream_t s);
+hipError_t launch_step(int storage, int mode, const KArgs &a, int m, hipStream_t s);
+void step_tiles(int nx, int ny, int &tiles_x, int &tiles_y);
 int tile_rows();
 int tile_cols();
 }  // namespace fdtd
@@ -46,8 +49,16 @@ struct fdtd_ctx {
     size_t esz = 4;                 // bytes per stored value
     i64 field_elems = 0;            // per field allocation, guard planes included
     std::vector<void *> allocs;     // every device allocation
-    void *fields[6] = {};           // allocation bases
-    void *psi[12] = {};
+    // Two buffer sets (ping-pong for the one-kernel step): fields[s][comp]
+    // are allocation bases; psi[s][q] holds the E-side psi of set s and, in
+    // set 0 only, the H-side psi (updated in place).  `cur` is the set that
+    // holds the current state.
+    void *fields_set[2][6] = {};
+    void *psi_set[2][12] = {};
+    int cur = 0;
+    bool fused = true;              // one kernel per step (else E and H half-step kernels)
+    KArgs ka_step{};                // tile geometry of the step kernel
+    double step_ms = 0.0;
     size_t psi_elems[12] = {};
     uint64_t dev_bytes = 0;
     // SI coefficient tables for the adjoint
@@ -110,9 +121,16 @@ void free_device(fdtd_ctx *c)
     for (void *p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
     c->dev_bytes = 0;
-    for (auto &f : c->fields) f = nullptr;
-    for (auto &p : c->psi) p = nullptr;
+    for (int s = 0; s < 2; ++s) {
+        for (auto &f : c->fields_set[s]) f = nullptr;
+        for (auto &p : c->psi_set[s]) p = nullptr;
+    }
+    c->cur = 0;
 }
+
+// The buffers holding the current state.
+void *fld(const fdtd_ctx *c, int f) { return c->fields_set[c->cur][f]; }
+void *psv(const fdtd_ctx *c, int q) { return (q % 4) < 2 ? c->psi_set[c->cur][q] : c->psi_set[0][q]; }
 
 bool box_in_grid(const fdtd_ctx *c, int i, int j, int k)
 {
@@ -122,10 +140,15 @@ bool box_in_grid(const fdtd_ctx *c, int i, int j, int k)
 // Zero every piece of time-dependent state.
 int zero_state(fdtd_ctx *c)
 {
-    for (int f = 0; f < 6; ++f)
-        HIPCHK(c, hipMemsetAsync(c->fields[f], 0, (size_t)c->field_elems * c->esz, c->stream));
-    for (int p = 0; p < 12; ++p)
-        if (c->psi_elems[p]) HIPCHK(c, hipMemsetAsync(c->psi[p], 0, c->psi_elems[p] * c->esz, c->stream));
+    for (int s = 0; s < 2; ++s) {
+        for (int f = 0; f < 6; ++f)
+            if (c->fields_set[s][f])
+                HIPCHK(c, hipMemsetAsync(c->fields_set[s][f], 0, (size_t)c->field_elems * c->esz, c->stream));
+        for (int p = 0; p < 12; ++p)
+            if (c->psi_elems[p] && c->psi_set[s][p])
+                HIPCHK(c, hipMemsetAsync(c->psi_set[s][p], 0, c->psi_elems[p] * c->esz, c->stream));
+    }
+    c->cur = 0;
     const size_t np = c->in.probes.size(), nf = c->in.freqs.size();
     if (np) HIPCHK(c, hipMemsetAsync(c->ka.probe_hist, 0, np * (size_t)c->in.steps_cap * sizeof(float), c->stream));
     if (np && nf) HIPCHK(c, hipMemsetAsync(c->ka.dft, 0, np * nf * sizeof(double2), c->stream));
@@ -141,7 +164,8 @@ int upload_tables(fdtd_ctx *c);
 int choose_chunks(const fdtd_ctx *c, int tiles)
 {
     // enough workgroups to cover 256 CUs several times over
-    int nchunks = (2048 + tiles - 1) / tiles;
+    const char *ov = std::getenv("FDTD_CHUNKS");
+    int nchunks = ov ? std::atoi(ov) : (2048 + tiles - 1) / tiles;
     nchunks = std::max(1, std::min(nchunks, c->in.nz));
     return (c->in.nz + nchunks - 1) / nchunks;
 }
@@ -174,25 +198,44 @@ int do_finalize(fdtd_ctx *c)
     a.rec = 1;
     a.snap = nullptr;
 
-    // fields: one zero guard plane below and above
+    // fields: one zero guard plane below and above; a second set for the
+    // one-kernel step
+    // The half-step kernels are the default; FDTD_KERNEL=fused selects the
+    // one-kernel step (same results, measured slower on MI355X).
+    const char *ks = std::getenv("FDTD_KERNEL");
+    c->fused = ks && std::strcmp(ks, "fused") == 0;
     c->field_elems = a.plane * (in.nz + 2);
-    void **fp[6] = {&a.ex, &a.ey, &a.ez, &a.hx, &a.hy, &a.hz};
-    for (int f = 0; f < 6; ++f) {
-        int rc = dev_alloc(c, &c->fields[f], (size_t)c->field_elems * c->esz);
-        if (rc) return rc;
-        *fp[f] = (char *)c->fields[f] + (size_t)a.plane * c->esz;
-    }
+    for (int s = 0; s < (c->fused ? 2 : 1); ++s)
+        for (int f = 0; f < 6; ++f) {
+            int rc = dev_alloc(c, &c->fields_set[s][f], (size_t)c->field_elems * c->esz);
+            if (rc) return rc;
+        }
     // psi slabs
     const size_t px_elems = (size_t)in.nz * in.ny * a.Lx;
     const size_t py_elems = (size_t)in.nz * a.Ly * in.px;
     const size_t pz_elems = (size_t)a.Lz * in.ny * in.px;
     void **pp[12] = {&a.psi_eyx, &a.psi_ezx, &a.psi_hyx, &a.psi_hzx, &a.psi_exy, &a.psi_ezy,
                      &a.psi_hxy, &a.psi_hzy, &a.psi_exz, &a.psi_eyz, &a.psi_hxz, &a.psi_hyz};
+    (void)pp;
     for (int p = 0; p < 12; ++p) {
         c->psi_elems[p] = p < 4 ? px_elems : p < 8 ? py_elems : pz_elems;
-        int rc = dev_alloc(c, &c->psi[p], std::max<size_t>(c->psi_elems[p], 1) * c->esz);
-        if (rc) return rc;
-        *pp[p] = c->psi[p];
+        const bool eside = (p % 4) < 2;
+        for (int s = 0; s < (c->fused && eside ? 2 : 1); ++s) {
+            int rc = dev_alloc(c, &c->psi_set[s][p], std::max<size_t>(c->psi_elems[p], 1) * c->esz);
+            if (rc) return rc;
+        }
+    }
+    a.e_uni = b.e_uniform ? 1 : 0;
+    a.h_uni = b.h_uniform ? 1 : 0;
+    c->ka_step = a;
+    step_tiles(in.nx, in.ny, c->ka_step.tiles_x, c->ka_step.tiles_y);
+    {
+        // about 1536 workgroups (4 wavefronts each): 24 wavefronts per CU
+        const int tiles = c->ka_step.tiles_x * c->ka_step.tiles_y;
+        const char *kc = std::getenv("FDTD_STEP_CHUNKS");
+        int nch = kc ? std::atoi(kc) : (1536 + tiles - 1) / tiles;
+        nch = std::max(1, std::min(nch, in.nz));
+        c->ka_step.kchunk = (in.nz + nch - 1) / nch;
     }
     int rc;
     void *tmp;
@@ -206,6 +249,10 @@ int do_finalize(fdtd_ctx *c)
     a.h_first = (const int *)tmp;
     if ((rc = upload(c, &tmp, b.events.data(), b.events.size() * sizeof(Event)))) return rc;
     a.events = (const Event *)tmp;
+    if ((rc = upload(c, &tmp, b.s_first.data(), b.s_first.size() * sizeof(int)))) return rc;
+    a.s_first = (const int *)tmp;
+    if ((rc = upload(c, &tmp, b.s_events.data(), b.s_events.size() * sizeof(Event)))) return rc;
+    a.s_events = (const Event *)tmp;
     if ((rc = upload(c, &tmp, b.vpn.data(), b.vpn.size() * sizeof(float)))) return rc;
     a.vpn = (const float *)tmp;
     if ((rc = upload(c, &tmp, b.phasor.data(), b.phasor.size() * sizeof(double)))) return rc;
@@ -262,6 +309,61 @@ int upload_tables(fdtd_ctx *c)
     return FDTD_OK;
 }
 
+// Point a kernel argument block at the current buffer set (read) and the
+// other set (write).  The half-step kernels update the current set in place.
+void bind_sets(const fdtd_ctx *c, KArgs &a)
+{
+    const int r = c->cur, w = c->fused ? 1 - c->cur : c->cur;
+    const size_t g = (size_t)c->ka.plane * c->esz;   // guard plane
+    void **rd[6] = {&a.ex, &a.ey, &a.ez, &a.hx, &a.hy, &a.hz};
+    void **wr[6] = {&a.nex, &a.ney, &a.nez, &a.nhx, &a.nhy, &a.nhz};
+    for (int f = 0; f < 6; ++f) {
+        *rd[f] = (char *)c->fields_set[r][f] + g;
+        *wr[f] = (char *)c->fields_set[w][f] + g;
+    }
+    void **pr[12] = {&a.psi_eyx, &a.psi_ezx, &a.psi_hyx, &a.psi_hzx, &a.psi_exy, &a.psi_ezy,
+                     &a.psi_hxy, &a.psi_hzy, &a.psi_exz, &a.psi_eyz, &a.psi_hxz, &a.psi_hyz};
+    for (int p = 0; p < 12; ++p) *pr[p] = (p % 4) < 2 ? c->psi_set[r][p] : c->psi_set[0][p];
+    a.npsi_eyx = c->psi_set[w][0];
+    a.npsi_ezx = c->psi_set[w][1];
+    a.npsi_exy = c->psi_set[w][4];
+    a.npsi_ezy = c->psi_set[w][5];
+    a.npsi_exz = c->psi_set[w][8];
+    a.npsi_eyz = c->psi_set[w][9];
+}
+
+// One time step m.  base carries the per-call flags (rec, snapshot); the
+// tile geometry and buffer pointers are filled in here.
+hipError_t do_step(fdtd_ctx *c, const KArgs &base, int m, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+                   hipEvent_t ev2 = nullptr, hipEvent_t ev3 = nullptr)
+{
+    hipError_t e;
+    if (c->fused) {
+        KArgs a = base;
+        a.tiles_x = c->ka_step.tiles_x;
+        a.tiles_y = c->ka_step.tiles_y;
+        a.kchunk = c->ka_step.kchunk;
+        bind_sets(c, a);
+        if (ev0) (void)hipEventRecord(ev0, c->stream);
+        e = launch_step(c->in.storage, c->built.mode, a, m, c->stream);
+        if (ev1) (void)hipEventRecord(ev1, c->stream);
+        c->cur = 1 - c->cur;
+        return e;
+    }
+    KArgs a = base;
+    a.tiles_x = c->ka.tiles_x;
+    a.tiles_y = c->ka.tiles_y;
+    a.kchunk = c->ka.kchunk;
+    bind_sets(c, a);
+    if (ev0) (void)hipEventRecord(ev0, c->stream);
+    if ((e = launch_half_step(true, c->in.storage, c->e_mode, a, m, c->stream)) != hipSuccess) return e;
+    if (ev1) (void)hipEventRecord(ev1, c->stream);
+    if (ev2) (void)hipEventRecord(ev2, c->stream);
+    e = launch_half_step(false, c->in.storage, c->h_mode, a, m, c->stream);
+    if (ev3) (void)hipEventRecord(ev3, c->stream);
+    return e;
+}
+
 int run_steps(fdtd_ctx *c, int nsteps, bool profile)
 {
     if (!c->finalized) return fail(c, FDTD_E_STATE, "fdtd_finalize has not been called");
@@ -281,12 +383,10 @@ int run_steps(fdtd_ctx *c, int nsteps, bool profile)
     HIPCHK(c, hipEventRecord(t0, c->stream));
     for (int s = 0; s < nsteps; ++s) {
         const int m = m0 + s;
-        if (profile) HIPCHK(c, hipEventRecord(evs[4 * s + 0], c->stream));
-        HIPCHK(c, launch_half_step(true, c->in.storage, c->e_mode, c->ka, m, c->stream));
-        if (profile) HIPCHK(c, hipEventRecord(evs[4 * s + 1], c->stream));
-        if (profile) HIPCHK(c, hipEventRecord(evs[4 * s + 2], c->stream));
-        HIPCHK(c, launch_half_step(false, c->in.storage, c->h_mode, c->ka, m, c->stream));
-        if (profile) HIPCHK(c, hipEventRecord(evs[4 * s + 3], c->stream));
+        if (profile)
+            HIPCHK(c, do_step(c, c->ka, m, evs[4 * s + 0], evs[4 * s + 1], evs[4 * s + 2], evs[4 * s + 3]));
+        else
+            HIPCHK(c, do_step(c, c->ka, m));
     }
     HIPCHK(c, hipEventRecord(t1, c->stream));
     HIPCHK(c, hipEventSynchronize(t1));
@@ -298,12 +398,18 @@ int run_steps(fdtd_ctx *c, int nsteps, bool profile)
         for (int s = 0; s < nsteps; ++s) {
             float a = 0, b = 0;
             HIPCHK(c, hipEventElapsedTime(&a, evs[4 * s + 0], evs[4 * s + 1]));
-            HIPCHK(c, hipEventElapsedTime(&b, evs[4 * s + 2], evs[4 * s + 3]));
+            if (!c->fused) HIPCHK(c, hipEventElapsedTime(&b, evs[4 * s + 2], evs[4 * s + 3]));
             se += a;
             sh += b;
         }
-        c->e_ms = se / nsteps;
-        c->h_ms = sh / nsteps;
+        if (c->fused) {
+            c->step_ms = se / nsteps;
+            c->e_ms = c->h_ms = 0.0;
+        } else {
+            c->e_ms = se / nsteps;
+            c->h_ms = sh / nsteps;
+            c->step_ms = c->e_ms + c->h_ms;
+        }
         for (auto &e : evs) (void)hipEventDestroy(e);
     }
     (void)hipEventDestroy(t0);
@@ -355,15 +461,15 @@ int copy_state(fdtd_ctx *c, char *dst, bool save)
     size_t off = 0;
     for (int f = 0; f < 6; ++f) {
         const size_t n = (size_t)c->field_elems * c->esz;
-        if (save) HIPCHK(c, hipMemcpyAsync(dst + off, c->fields[f], n, hipMemcpyDeviceToDevice, c->stream));
-        else HIPCHK(c, hipMemcpyAsync(c->fields[f], dst + off, n, hipMemcpyDeviceToDevice, c->stream));
+        if (save) HIPCHK(c, hipMemcpyAsync(dst + off, fld(c, f), n, hipMemcpyDeviceToDevice, c->stream));
+        else HIPCHK(c, hipMemcpyAsync(fld(c, f), dst + off, n, hipMemcpyDeviceToDevice, c->stream));
         off += n;
     }
     for (int q = 0; q < 12; ++q) {
         const size_t n = c->psi_elems[q] * c->esz;
         if (!n) continue;
-        if (save) HIPCHK(c, hipMemcpyAsync(dst + off, c->psi[q], n, hipMemcpyDeviceToDevice, c->stream));
-        else HIPCHK(c, hipMemcpyAsync(c->psi[q], dst + off, n, hipMemcpyDeviceToDevice, c->stream));
+        if (save) HIPCHK(c, hipMemcpyAsync(dst + off, psv(c, q), n, hipMemcpyDeviceToDevice, c->stream));
+        else HIPCHK(c, hipMemcpyAsync(psv(c, q), dst + off, n, hipMemcpyDeviceToDevice, c->stream));
         off += n;
     }
     return FDTD_OK;
@@ -499,8 +605,7 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     for (int m = 0; m < n; ++m) {
         if (nseg > 1 && m % Kseg == 0 && m > 0)
             if ((rc = copy_state(c, pool + (size_t)(m / Kseg - 1) * ck_bytes, true))) return rc;
-        HIPCHK(c, launch_half_step(true, in.storage, c->e_mode, fa, m, c->stream));
-        HIPCHK(c, launch_half_step(false, in.storage, c->h_mode, fa, m, c->stream));
+        HIPCHK(c, do_step(c, fa, m));
     }
     HIPCHK(c, hipEventRecord(t1, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -529,9 +634,9 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
         if (nseg > 1) {
             if (sgi == 0) {
                 for (int f = 0; f < 6; ++f)
-                    HIPCHK(c, hipMemsetAsync(c->fields[f], 0, (size_t)c->field_elems * c->esz, c->stream));
+                    HIPCHK(c, hipMemsetAsync(fld(c, f), 0, (size_t)c->field_elems * c->esz, c->stream));
                 for (int q = 0; q < 12; ++q)
-                    if (c->psi_elems[q]) HIPCHK(c, hipMemsetAsync(c->psi[q], 0, c->psi_elems[q] * c->esz, c->stream));
+                    if (c->psi_elems[q]) HIPCHK(c, hipMemsetAsync(psv(c, q), 0, c->psi_elems[q] * c->esz, c->stream));
             } else if ((rc = copy_state(c, pool + (size_t)(sgi - 1) * ck_bytes, false))) {
                 return rc;
             }
@@ -539,11 +644,9 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
             ra.rec = 0;
             ra.snap = hist;
             ra.snap_base = s0;
+            bind_sets(c, ra);
             HIPCHK(c, launch_snap(in.storage, ra, hist, 0, c->stream));
-            for (int m = s0; m < s1; ++m) {
-                HIPCHK(c, launch_half_step(true, in.storage, c->e_mode, ra, m, c->stream));
-                HIPCHK(c, launch_half_step(false, in.storage, c->h_mode, ra, m, c->stream));
-            }
+            for (int m = s0; m < s1; ++m) HIPCHK(c, do_step(c, ra, m));
         }
         b.hist_base = s0;
         for (int m = s1 - 1; m >= s0; --m) {
@@ -808,6 +911,8 @@ int fdtd_get_info(const fdtd_ctx *c, fdtd_info *o)
     o->last_run_ms = c->last_run_ms;
     o->e_kernel_ms = c->e_ms;
     o->h_kernel_ms = c->h_ms;
+    o->step_kernel_ms = c->step_ms;
+    o->fused = c->fused ? 1 : 0;
     o->device_bytes = c->dev_bytes;
     // algorithmic bytes per cell: E sweep reads E, H and writes E (9 values);
     // H sweep likewise; plus one class index per cell in a classed sweep and
@@ -857,7 +962,7 @@ int fdtd_get_field(fdtd_ctx *c, int comp, float *out, size_t n)
     const size_t cells = (size_t)nx * ny * nz;
     if (n < cells) return fail(c, FDTD_E_INVALID, "output too small");
     std::vector<unsigned char> raw((size_t)px * ny * nz * c->esz);
-    const char *base = (const char *)c->fields[comp] + (size_t)c->ka.plane * c->esz;
+    const char *base = (const char *)c->fields_set[c->cur][comp] + (size_t)c->ka.plane * c->esz;
     HIPCHK(c, hipMemcpy(raw.data(), base, raw.size(), hipMemcpyDeviceToHost));
     const float un = c->built.probe_unscale[comp];
     for (size_t r = 0; r < (size_t)ny * nz; ++r)
@@ -911,9 +1016,9 @@ int fdtd_checkpoint_save(fdtd_ctx *c, void *buf, size_t bytes)
         return FDTD_OK;
     };
     for (int f = 0; f < 6; ++f)
-        if ((rc = get(c->fields[f], (size_t)c->field_elems * c->esz))) return rc;
+        if ((rc = get(fld(c, f), (size_t)c->field_elems * c->esz))) return rc;
     for (int q = 0; q < 12; ++q)
-        if ((rc = get(c->psi[q], c->psi_elems[q] * c->esz))) return rc;
+        if ((rc = get(psv(c, q), c->psi_elems[q] * c->esz))) return rc;
     if ((rc = get(c->ka.probe_hist, c->in.probes.size() * (size_t)c->in.steps_cap * sizeof(float)))) return rc;
     if ((rc = get(c->ka.dft, c->in.probes.size() * c->in.freqs.size() * sizeof(double2)))) return rc;
     CkHeader h{};
@@ -955,9 +1060,9 @@ int fdtd_checkpoint_restore(fdtd_ctx *c, const void *buf, size_t bytes)
         return FDTD_OK;
     };
     for (int f = 0; f < 6; ++f)
-        if ((rc = put(c->fields[f], (size_t)c->field_elems * c->esz))) return rc;
+        if ((rc = put(fld(c, f), (size_t)c->field_elems * c->esz))) return rc;
     for (int q = 0; q < 12; ++q)
-        if ((rc = put(c->psi[q], c->psi_elems[q] * c->esz))) return rc;
+        if ((rc = put(psv(c, q), c->psi_elems[q] * c->esz))) return rc;
     if ((rc = put(c->ka.probe_hist, c->in.probes.size() * (size_t)c->in.steps_cap * sizeof(float)))) return rc;
     if ((rc = put(c->ka.dft, c->in.probes.size() * c->in.freqs.size() * sizeof(double2)))) return rc;
     c->steps_done = h.step;

@@ -28,6 +28,28 @@ __device__ __forceinline__ void st(bf16_t *p, float v)
     p->v = (std::uint16_t)(u >> 16);
 }
 
+// v rounded to the storage type and back (what the next read will see)
+__device__ __forceinline__ float rnd(float v, const float *) { return v; }
+__device__ __forceinline__ float rnd(float v, const _Float16 *) { return (float)(_Float16)v; }
+__device__ __forceinline__ float rnd(float v, const bf16_t *)
+{
+    std::uint32_t u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xFFFF0000u);
+}
+
+// per-component scale from a kernel-argument array without dynamic indexing
+__device__ __forceinline__ float pick6(const float (&v)[6], int c)
+{
+    float r = v[0];
+    r = c == 1 ? v[1] : r;
+    r = c == 2 ? v[2] : r;
+    r = c == 3 ? v[3] : r;
+    r = c == 4 ? v[4] : r;
+    r = c == 5 ? v[5] : r;
+    return r;
+}
+
 template <int CM>
 __device__ __forceinline__ int class_of(const void *cls, i64 c)
 {

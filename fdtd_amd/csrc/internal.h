@@ -23,6 +23,12 @@ struct Event {
 };
 enum { EV_SOURCE = 0, EV_PROBE = 8 };
 
+// Rows of a thread's strip in the one-kernel step.  The step kernel keys its
+// events by (column, strip) with the row in bits 8.. of `kind`; a source on
+// the first row of a strip is also listed, as row STRIP_R, in the strip
+// below (whose halo row recomputes it).
+constexpr int STRIP_R = 2;
+
 // Per source edge: Ez coefficients with the resistive load folded in
 // (defineSourceZ, fdtdrav.h:1402-1421) and the waveform row.
 struct SrcEdge {
@@ -36,10 +42,16 @@ struct SrcEdge {
 // k = nz-1, so every i+-1, j+-1, k+-1 neighbour address is inside it.
 struct KArgs {
     void *ex, *ey, *ez, *hx, *hy, *hz;
+    // the fused step kernel reads the set above and writes this one (the
+    // half-step kernels update in place: same pointers)
+    void *nex, *ney, *nez, *nhx, *nhy, *nhz;
     // psi slabs: x [nz][ny][Lx], y [nz][Ly][px], z [Lz][ny][px]
     void *psi_eyx, *psi_ezx, *psi_hyx, *psi_hzx;
     void *psi_exy, *psi_ezy, *psi_hxy, *psi_hzy;
     void *psi_exz, *psi_eyz, *psi_hxz, *psi_hyz;
+    // E-side psi written by the fused step kernel (H-side psi is updated in
+    // place: only the owning tile reads or writes it)
+    void *npsi_eyx, *npsi_ezx, *npsi_exy, *npsi_ezy, *npsi_exz, *npsi_eyz;
     const void *cls;           // class map [nz][ny][px] (u16/u32) or null
     const float4 *etab;        // [class][3] (ce, cb_a, cb_b, 0) for Ex, Ey, Ez
     const float4 *htab;        // [class][3] (ch, cb_a, cb_b, 0) for Hx, Hy, Hz
@@ -48,6 +60,8 @@ struct KArgs {
     const float4 *px_prof, *py_prof, *pz_prof;
     const int *e_first, *h_first;   // per column: first event or -1
     const Event *events;
+    const int *s_first;             // per (strip, column): first strip event or -1
+    const Event *s_events;
     const SrcEdge *srcs;
     const float *vpn;          // [wave][steps_cap]
     float *probe_hist;         // [probe][steps_cap]
@@ -63,6 +77,7 @@ struct KArgs {
     int kchunk;                // planes per z-march block
     int tiles_x, tiles_y;      // tile grid (for the XCD-aware remap)
     int rec;                   // record probes / DFT (0 during adjoint recompute)
+    int e_uni, h_uni;          // E / H coefficients are one class (no class-map read)
     // design-region snapshot: E^{m+1} of the E edges in the box
     // [sb[0], sb[0]+sb[3]) x [sb[1], sb[1]+sb[4]) x [sb[2], sb[2]+sb[5]) is
     // written in SI units to snap[((m+1-snap_base)*3 + comp)*nb + q]
@@ -117,6 +132,8 @@ struct Built {
     std::vector<float> prof[3];     // [2n][4] per axis
     std::vector<int> e_first, h_first;
     std::vector<Event> events;
+    std::vector<int> s_first;       // [ny/STRIP_R][nx] first strip event or -1
+    std::vector<Event> s_events;
     std::vector<SrcEdge> srcs;
     std::vector<SrcEdge> srcs_si;   // SI coefficients (adjoint)
     std::vector<int> src_cell;      // per source edge: (i, j, k) triples

@@ -28,9 +28,80 @@
 
 namespace fdtd {
 
+// Non-temporal access for the streams only the owning thread touches (the
+// swept field's own values): they are read once and written once per sweep.
+__device__ __forceinline__ float ldn(const float *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void stn(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ float ldn(const _Float16 *p) { return (float)__builtin_nontemporal_load(p); }
+__device__ __forceinline__ void stn(_Float16 *p, float v) { __builtin_nontemporal_store((_Float16)v, p); }
+__device__ __forceinline__ float ldn(const bf16_t *p)
+{
+    const std::uint16_t u = __builtin_nontemporal_load((const std::uint16_t *)p);
+    return __uint_as_float((std::uint32_t)u << 16);
+}
+__device__ __forceinline__ void stn(bf16_t *p, float v)
+{
+    std::uint32_t u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    __builtin_nontemporal_store((std::uint16_t)(u >> 16), (std::uint16_t *)p);
+}
+
+// Everything the E update of one cell reads from memory, loaded a plane
+// ahead of its use so that the next plane's loads are in flight while this
+// plane is computed.
+struct EPlane {
+    float hxc, hyc, hzc, hy_im, hz_im, hx_jm, hz_jm, ex0, ey0, ez0;
+    int cl;
+};
+
+template <class T, int CM>
+__device__ __forceinline__ EPlane e_load(const KArgs &a, const T *hx, const T *hy, const T *hz, const T *ex,
+                                         const T *ey, const T *ez, i64 c, int px)
+{
+    EPlane p;
+    p.hxc = ld(hx + c);
+    p.hyc = ld(hy + c);
+    p.hzc = ld(hz + c);
+    p.hy_im = ld(hy + c - 1);
+    p.hz_im = ld(hz + c - 1);
+    p.hx_jm = ld(hx + c - px);
+    p.hz_jm = ld(hz + c - px);
+    p.ex0 = ldn(ex + c);
+    p.ey0 = ldn(ey + c);
+    p.ez0 = ldn(ez + c);
+    p.cl = CM != CLS_UNIFORM ? class_of<CM>(a.cls, c) : 0;
+    return p;
+}
+
+// Everything the H update of one cell reads (E of planes k and k+1 around
+// it, its own H), loaded a plane ahead.
+struct HPlane {
+    float ex_kp, ey_kp, ezc, ez_ip, ey_ip, ez_jp, ex_jp, hx0, hy0, hz0;
+    int cl;
+};
+
+template <class T, int CM>
+__device__ __forceinline__ HPlane h_load(const KArgs &a, const T *ex, const T *ey, const T *ez, const T *hx,
+                                         const T *hy, const T *hz, i64 c, i64 plane, int px)
+{
+    HPlane p;
+    p.ex_kp = ld(ex + c + plane);
+    p.ey_kp = ld(ey + c + plane);
+    p.ezc = ld(ez + c);
+    p.ez_ip = ld(ez + c + 1);
+    p.ey_ip = ld(ey + c + 1);
+    p.ez_jp = ld(ez + c + px);
+    p.ex_jp = ld(ex + c + px);
+    p.hx0 = ldn(hx + c);
+    p.hy0 = ldn(hy + c);
+    p.hz0 = ldn(hz + c);
+    p.cl = CM != CLS_UNIFORM ? class_of<CM>(a.cls, c) : 0;
+    return p;
+}
+
 __device__ __forceinline__ void record_probe(const KArgs &a, int p, int comp, float stored, int m)
 {
-    const float v = stored * a.probe_unscale[comp];
+    const float v = stored * pick6(a.probe_unscale, comp);
     a.probe_hist[(i64)p * a.steps_cap + m] = v;
     const double dv = (double)v;
     for (int f = 0; f < a.nfreq; ++f) {
@@ -85,16 +156,18 @@ __global__ void __launch_bounds__(TX * BY) e_kernel(const KArgs a, const int m)
     const bool in_snap = a.snap != nullptr && i >= a.sb[0] && i < a.sb[0] + a.sb[3] && j >= a.sb[1] &&
                          j < a.sb[1] + a.sb[4];
 
+    EPlane nxt = e_load<T, CM>(a, hx, hy, hz, ex, ey, ez, col + (i64)k0 * plane, px);
     for (int k = k0; k < k1; ++k) {
         const i64 c = col + (i64)k * plane;
-        const float hxc = ld(hx + c), hyc = ld(hy + c), hzc = ld(hz + c);
-        const float hy_im = ld(hy + c - 1), hz_im = ld(hz + c - 1);
-        const float hx_jm = ld(hx + c - px), hz_jm = ld(hz + c - px);
+        const EPlane cur = nxt;
+        if (k + 1 < k1) nxt = e_load<T, CM>(a, hx, hy, hz, ex, ey, ez, c + plane, px);
+        const float hxc = cur.hxc, hyc = cur.hyc, hzc = cur.hzc;
+        const float hy_im = cur.hy_im, hz_im = cur.hz_im;
+        const float hx_jm = cur.hx_jm, hz_jm = cur.hz_jm;
         if (CM != CLS_UNIFORM) {
-            const int cl = class_of<CM>(a.cls, c);
-            cx = a.etab[cl * 3 + 0];
-            cy = a.etab[cl * 3 + 1];
-            cz = a.etab[cl * 3 + 2];
+            cx = a.etab[cur.cl * 3 + 0];
+            cy = a.etab[cur.cl * 3 + 1];
+            cz = a.etab[cur.cl * 3 + 2];
         }
         const bool klo = k > 0, khi = k < nz - 1;
         const bool upd_x = ihi && jlo && jhi && klo && khi;
@@ -115,7 +188,7 @@ __global__ void __launch_bounds__(TX * BY) e_kernel(const KArgs a, const int m)
             cby_hx = cby_hx / pze.z;
         }
 
-        const float ex0 = ld(ex + c), ey0 = ld(ey + c), ez0 = ld(ez + c);
+        const float ex0 = cur.ex0, ey0 = cur.ey0, ez0 = cur.ez0;
         float exn = ex0, eyn = ey0, ezn = ez0;
         if (upd_x) exn = (cx.x * ex0 + cbx_hz * (hzc - hz_jm)) + cbx_hy * (hyc - hy_km);
         if (upd_y) eyn = (cy.x * ey0 + cby_hx * (hxc - hx_km)) + cby_hz * (hzc - hz_im);
@@ -174,9 +247,9 @@ __global__ void __launch_bounds__(TX * BY) e_kernel(const KArgs a, const int m)
             if (jlo) exn = exn + (cx.z * dz) * pex;
             if (ilo) eyn = eyn + (cy.y * dz) * pey;
         }
-        if (upd_x) st(ex + c, exn);
-        if (upd_y) st(ey + c, eyn);
-        if (upd_z) st(ez + c, ezn);
+        if (upd_x) stn(ex + c, exn);
+        if (upd_y) stn(ey + c, eyn);
+        if (upd_z) stn(ez + c, ezn);
         if (in_snap && k >= a.sb[2] && k < a.sb[2] + a.sb[5]) {
             // stored-precision value, as the next step reads it, in SI units
             const i64 nb = (i64)a.sb[3] * a.sb[4] * a.sb[5];
@@ -230,18 +303,20 @@ __global__ void __launch_bounds__(TX * BY) h_kernel(const KArgs a, const int m)
 
     float ex_c = ld(ex + col + (i64)k0 * plane);
     float ey_c = ld(ey + col + (i64)k0 * plane);
+    HPlane nxt = h_load<T, CM>(a, ex, ey, ez, hx, hy, hz, col + (i64)k0 * plane, plane, px);
 
     for (int k = k0; k < k1; ++k) {
         const i64 c = col + (i64)k * plane;
-        const float ex_kp = ld(ex + c + plane), ey_kp = ld(ey + c + plane);
-        const float ezc = ld(ez + c);
-        const float ez_ip = ld(ez + c + 1), ey_ip = ld(ey + c + 1);
-        const float ez_jp = ld(ez + c + px), ex_jp = ld(ex + c + px);
+        const HPlane cur = nxt;
+        if (k + 1 < k1) nxt = h_load<T, CM>(a, ex, ey, ez, hx, hy, hz, c + plane, plane, px);
+        const float ex_kp = cur.ex_kp, ey_kp = cur.ey_kp;
+        const float ezc = cur.ezc;
+        const float ez_ip = cur.ez_ip, ey_ip = cur.ey_ip;
+        const float ez_jp = cur.ez_jp, ex_jp = cur.ex_jp;
         if (CM != CLS_UNIFORM) {
-            const int cl = class_of<CM>(a.cls, c);
-            cx = a.htab[cl * 3 + 0];
-            cy = a.htab[cl * 3 + 1];
-            cz = a.htab[cl * 3 + 2];
+            cx = a.htab[cur.cl * 3 + 0];
+            cy = a.htab[cur.cl * 3 + 1];
+            cz = a.htab[cur.cl * 3 + 2];
         }
         const bool khi = k < nz - 1;
         const bool upd_x = jhi && khi;
@@ -261,7 +336,7 @@ __global__ void __launch_bounds__(TX * BY) h_kernel(const KArgs a, const int m)
             chy_ex = chy_ex / pzm.z;
         }
 
-        const float hx0 = ld(hx + c), hy0 = ld(hy + c), hz0 = ld(hz + c);
+        const float hx0 = cur.hx0, hy0 = cur.hy0, hz0 = cur.hz0;
         float hxn = hx0, hyn = hy0, hzn = hz0;
         if (upd_x) hxn = (cx.x * hx0 + chx_ey * (ey_kp - ey_c)) + chx_ez * (ez_jp - ezc);
         if (upd_y) hyn = (cy.x * hy0 + chy_ez * (ez_ip - ezc)) + chy_ex * (ex_kp - ex_c);
@@ -311,9 +386,9 @@ __global__ void __launch_bounds__(TX * BY) h_kernel(const KArgs a, const int m)
             if (jhi) hxn = hxn + (cx.y * dz) * phx;
             if (ihi) hyn = hyn + (cy.z * dz) * phy;
         }
-        if (upd_x) st(hx + c, hxn);
-        if (upd_y) st(hy + c, hyn);
-        if (upd_z) st(hz + c, hzn);
+        if (upd_x) stn(hx + c, hxn);
+        if (upd_y) stn(hy + c, hyn);
+        if (upd_z) stn(hz + c, hzn);
         ex_c = ex_kp;
         ey_c = ey_kp;
     }

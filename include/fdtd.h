@@ -125,6 +125,9 @@ typedef struct {
     double   last_run_ms;        /* hipEvent time of the last fdtd_run() loop  */
     double   e_kernel_ms;        /* mean E-kernel launch time (fdtd_profile)   */
     double   h_kernel_ms;        /* mean H-kernel launch time (fdtd_profile)   */
+    double   step_kernel_ms;     /* mean time of one step's launches            */
+    int      fused;              /* 1: one kernel per step; 0: E and H kernels  */
+    int      pad1;
     double   adjoint_forward_ms; /* last fdtd_adjoint_gradient: forward pass   */
     double   adjoint_total_ms;   /* ... forward + recompute + reverse sweep     */
     int      adjoint_segments;   /* ... checkpoint segments used                */
