@@ -163,9 +163,12 @@ int upload_tables(fdtd_ctx *c);
 
 int choose_chunks(const fdtd_ctx *c, int tiles)
 {
-    // enough workgroups to cover 256 CUs several times over
+    // Many short z-marches: ~32k workgroups keep every CU's wave slots
+    // full through the tail of the launch, while a march of >= 8 planes
+    // keeps the re-read of the plane below (above) each chunk under 1/8.
     const char *ov = std::getenv("FDTD_CHUNKS");
-    int nchunks = ov ? std::atoi(ov) : (2048 + tiles - 1) / tiles;
+    int nchunks = ov ? std::atoi(ov)
+                     : std::min((32768 + tiles - 1) / tiles, c->in.nz / 8);
     nchunks = std::max(1, std::min(nchunks, c->in.nz));
     return (c->in.nz + nchunks - 1) / nchunks;
 }

@@ -9,7 +9,10 @@
 namespace fdtd {
 
 constexpr int TX = 64;   // tile width (one wavefront row)
-constexpr int BY = 4;    // tile rows per workgroup
+#ifndef FDTD_TILE_ROWS
+#define FDTD_TILE_ROWS 4
+#endif
+constexpr int BY = FDTD_TILE_ROWS;   // tile rows per workgroup
 
 // ---- storage types ------------------------------------------------------
 struct bf16_t {

@@ -9,6 +9,18 @@
 // each of the 8 XCDs owns one contiguous band of tile rows, which keeps the
 // halo lines of a tile in the L2 of the XCD that fetched them.
 //
+// The march is software-pipelined two planes deep: every value plane k reads
+// (field operands, its own old values, the slab psi) is fetched into one of
+// two register sets while plane k-1 is computed from the other, and the
+// class index of plane k+1 rides along so that its coefficient-table entries
+// are requested one plane ahead.  The loop is unrolled by two so the sets
+// alternate without register copies (a copy of a register whose load is in
+// flight would make the wave wait for it), and nothing is fetched behind a
+// plane's own loads except the next coefficients: the in-order memory
+// counter then lets plane k's compute start while plane k+1 is in flight.
+// The per-plane z-profile is uniform across the workgroup and comes through
+// the scalar data cache (read only).
+//
 // Per cell the kernels do exactly the reference's arithmetic, in its operand
 // order, with no contraction (the library is built with -ffp-contract=off):
 //   E: calc_e      fdtdrav.h:665-735, CPML fdtdrav.h:750-911
@@ -25,6 +37,18 @@
 
 #include "device_common.h"
 #include "internal.h"
+
+// Occupancy floors of the two sweep kernels (waves per SIMD).
+#ifdef FDTD_E_WAVES
+#define E_OCC __attribute__((amdgpu_waves_per_eu(FDTD_E_WAVES, FDTD_E_WAVES)))
+#else
+#define E_OCC
+#endif
+#ifdef FDTD_H_WAVES
+#define H_OCC __attribute__((amdgpu_waves_per_eu(FDTD_H_WAVES, FDTD_H_WAVES)))
+#else
+#define H_OCC
+#endif
 
 namespace fdtd {
 
@@ -46,57 +70,56 @@ __device__ __forceinline__ void stn(bf16_t *p, float v)
     __builtin_nontemporal_store((std::uint16_t)(u >> 16), (std::uint16_t *)p);
 }
 
-// Everything the E update of one cell reads from memory, loaded a plane
-// ahead of its use so that the next plane's loads are in flight while this
-// plane is computed.
-struct EPlane {
-    float hxc, hyc, hzc, hy_im, hz_im, hx_jm, hz_jm, ex0, ey0, ez0;
-    int cl;
-};
-
-template <class T, int CM>
-__device__ __forceinline__ EPlane e_load(const KArgs &a, const T *hx, const T *hy, const T *hz, const T *ex,
-                                         const T *ey, const T *ez, i64 c, int px)
+// Byte-offset addressing.  Every array a sweep touches is smaller than 4 GiB
+// (checked when the scene is finalized), so a 32-bit byte offset in one
+// vector register, added to a scalar base, addresses every stream of a
+// plane; field offsets are taken from the start of the allocation (the guard
+// plane below k = 0) so that they are never negative.
+template <class T>
+__device__ __forceinline__ const T *at(const T *base, std::uint32_t off)
 {
-    EPlane p;
-    p.hxc = ld(hx + c);
-    p.hyc = ld(hy + c);
-    p.hzc = ld(hz + c);
-    p.hy_im = ld(hy + c - 1);
-    p.hz_im = ld(hz + c - 1);
-    p.hx_jm = ld(hx + c - px);
-    p.hz_jm = ld(hz + c - px);
-    p.ex0 = ldn(ex + c);
-    p.ey0 = ldn(ey + c);
-    p.ez0 = ldn(ez + c);
-    p.cl = CM != CLS_UNIFORM ? class_of<CM>(a.cls, c) : 0;
-    return p;
+    return (const T *)((const char *)base + off);
+}
+template <class T>
+__device__ __forceinline__ T *at(T *base, std::uint32_t off)
+{
+    return (T *)((char *)base + off);
 }
 
-// Everything the H update of one cell reads (E of planes k and k+1 around
-// it, its own H), loaded a plane ahead.
-struct HPlane {
-    float ex_kp, ey_kp, ezc, ez_ip, ey_ip, ez_jp, ex_jp, hx0, hy0, hz0;
-    int cl;
+// Read-only, workgroup-uniform table entry through the scalar data cache.
+typedef __attribute__((address_space(4))) const float const_float;
+__device__ __forceinline__ float4 uload(const float4 *p)
+{
+    const const_float *q = (const const_float *)p;
+    return make_float4(q[0], q[1], q[2], q[3]);
+}
+
+// Coefficient-table entries of one cell: (c, cb_a, cb_b, -) per component.
+struct Coef {
+    float4 x, y, z;
 };
 
-template <class T, int CM>
-__device__ __forceinline__ HPlane h_load(const KArgs &a, const T *ex, const T *ey, const T *ez, const T *hx,
-                                         const T *hy, const T *hz, i64 c, i64 plane, int px)
+template <int CM>
+__device__ __forceinline__ Coef coef_of(const float4 *tab, int cl)
 {
-    HPlane p;
-    p.ex_kp = ld(ex + c + plane);
-    p.ey_kp = ld(ey + c + plane);
-    p.ezc = ld(ez + c);
-    p.ez_ip = ld(ez + c + 1);
-    p.ey_ip = ld(ey + c + 1);
-    p.ez_jp = ld(ez + c + px);
-    p.ex_jp = ld(ex + c + px);
-    p.hx0 = ldn(hx + c);
-    p.hy0 = ldn(hy + c);
-    p.hz0 = ldn(hz + c);
-    p.cl = CM != CLS_UNIFORM ? class_of<CM>(a.cls, c) : 0;
-    return p;
+    Coef t;
+    t.x = tab[cl * 3 + 0];
+    t.y = tab[cl * 3 + 1];
+    t.z = tab[cl * 3 + 2];
+    return t;
+}
+
+template <int CM>
+__device__ __forceinline__ int class_at(const void *cls, std::uint32_t cell)
+{
+    if (CM == CLS_U16) return *at((const std::uint16_t *)cls, cell * 2u);
+    if (CM == CLS_U32) return (int)*at((const std::uint32_t *)cls, cell * 4u);
+    return 0;
+}
+
+__device__ __forceinline__ int first_k(const KArgs &a, int ev, int col)
+{
+    return (ev >= 0 && a.events[ev].col == col) ? a.events[ev].k : INT_MAX;
 }
 
 __device__ __forceinline__ void record_probe(const KArgs &a, int p, int comp, float stored, int m)
@@ -114,283 +137,462 @@ __device__ __forceinline__ void record_probe(const KArgs &a, int p, int comp, fl
 }
 
 // ---- E half-step ----------------------------------------------------------
+
+// Column constants of one thread in the E sweep.
+struct ECol {
+    int i, j, sx, sy, mycol;
+    i64 col;
+    std::uint32_t o0;    // byte offset of (i, j, 0) from the guard-plane base
+    std::uint32_t ob;    // bytes per plane
+    bool ilo, ihi, jlo, jhi, xs, ys, in_snap;
+    float4 pxe, pye;
+};
+
+// Everything the E update of plane k reads from memory.
+struct EPlane {
+    float hxc, hyc, hzc, hy_im, hz_im, hx_jm, hz_jm, ex0, ey0, ez0;
+    float p_yx, p_zx, p_xy, p_zy, p_xz, p_yz;   // old psi of the slabs the cell is in
+    int cln;                                      // class index of plane k+1
+};
+
+__device__ __forceinline__ bool e_zslab(const KArgs &a, int k)
+{
+    return (k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && k < a.nz - 1);   // fdtdrav.h:853,879
+}
+
 template <class T, int CM>
-__global__ void __launch_bounds__(TX * BY) e_kernel(const KArgs a, const int m)
+__device__ __forceinline__ void e_fetch(EPlane &p, const KArgs &a, const ECol &cc, const T *ex, const T *ey,
+                                        const T *ez, const T *hx, const T *hy, const T *hz, int k, int k1)
+{
+    constexpr std::uint32_t S = sizeof(T);
+    const std::uint32_t o = cc.o0 + (std::uint32_t)k * cc.ob;
+    const std::uint32_t ojm = o - (std::uint32_t)a.px * S;
+    const int px = a.px, ny = a.ny;
+    p.hxc = ld(at(hx, o));
+    p.hyc = ld(at(hy, o));
+    p.hzc = ld(at(hz, o));
+    p.hy_im = ld(at(hy, o - S));
+    p.hz_im = ld(at(hz, o - S));
+    p.hx_jm = ld(at(hx, ojm));
+    p.hz_jm = ld(at(hz, ojm));
+    p.ex0 = ldn(at(ex, o));
+    p.ey0 = ldn(at(ey, o));
+    p.ez0 = ldn(at(ez, o));
+    // class map (no guard planes): the entry of plane k+1
+    p.cln = (CM != CLS_UNIFORM && k + 1 < k1)
+                ? class_at<CM>(a.cls, (std::uint32_t)(k + 1) * (std::uint32_t)a.plane + (std::uint32_t)cc.col)
+                : 0;
+    const bool khi = k < a.nz - 1;
+    p.p_yx = p.p_zx = p.p_xy = p.p_zy = p.p_xz = p.p_yz = 0.f;
+    if (cc.xs && cc.jhi && khi) {
+        const std::uint32_t q = (((std::uint32_t)k * ny + cc.j) * a.Lx + cc.sx) * S;
+        p.p_yx = ld(at((const T *)a.psi_eyx, q));
+        p.p_zx = ld(at((const T *)a.psi_ezx, q));
+    }
+    if (cc.ys && cc.ihi && khi) {
+        const std::uint32_t q = (((std::uint32_t)k * a.Ly + cc.sy) * px + cc.i) * S;
+        p.p_xy = ld(at((const T *)a.psi_exy, q));
+        p.p_zy = ld(at((const T *)a.psi_ezy, q));
+    }
+    if (e_zslab(a, k) && cc.ihi && cc.jhi) {
+        const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
+        const std::uint32_t q = (((std::uint32_t)sz * ny + cc.j) * px + cc.i) * S;
+        p.p_xz = ld(at((const T *)a.psi_exz, q));
+        p.p_yz = ld(at((const T *)a.psi_eyz, q));
+    }
+}
+
+template <class T, int CM>
+__device__ __forceinline__ void e_update(const EPlane &p, const Coef &t, const KArgs &a, const ECol &cc, T *ex,
+                                         T *ey, T *ez, int k, int m, float &hx_km, float &hy_km, int &ev,
+                                         int &ek)
+{
+    const int nz = a.nz;
+    constexpr std::uint32_t S = sizeof(T);
+    const std::uint32_t o = cc.o0 + (std::uint32_t)k * cc.ob;
+    const float hxc = p.hxc, hyc = p.hyc, hzc = p.hzc;
+    const float hy_im = p.hy_im, hz_im = p.hz_im;
+    const float hx_jm = p.hx_jm, hz_jm = p.hz_jm;
+    const float4 cx = t.x, cy = t.y, cz = t.z;
+    const bool klo = k > 0, khi = k < nz - 1;
+    const bool upd_x = cc.ihi && cc.jlo && cc.jhi && klo && khi;
+    const bool upd_y = cc.ilo && cc.ihi && cc.jhi && klo && khi;
+    const bool upd_z = cc.ilo && cc.ihi && cc.jlo && cc.jhi && khi;
+    const bool zs = e_zslab(a, k);
+
+    // curl coefficients, divided by kappa on the slab cells (setABCPML)
+    float cbx_hz = cx.y, cbx_hy = cx.z;
+    float cby_hx = cy.y, cby_hz = cy.z;
+    float cbz_hy = cz.y, cbz_hx = cz.z;
+    float4 pze = make_float4(0.f, 0.f, 1.f, 0.f);
+    if (cc.xs) { cby_hz = cby_hz / cc.pxe.z; cbz_hy = cbz_hy / cc.pxe.z; }
+    if (cc.ys) { cbx_hz = cbx_hz / cc.pye.z; cbz_hx = cbz_hx / cc.pye.z; }
+    if (zs) {
+        pze = uload(a.pz_prof + k);
+        cbx_hy = cbx_hy / pze.z;
+        cby_hx = cby_hx / pze.z;
+    }
+
+    const float ex0 = p.ex0, ey0 = p.ey0, ez0 = p.ez0;
+    float exn = ex0, eyn = ey0, ezn = ez0;
+    if (upd_x) exn = (cx.x * ex0 + cbx_hz * (hzc - hz_jm)) + cbx_hy * (hyc - hy_km);
+    if (upd_y) eyn = (cy.x * ey0 + cby_hx * (hxc - hx_km)) + cby_hz * (hzc - hz_im);
+    if (upd_z) ezn = (cz.x * ez0 + cbz_hy * (hyc - hy_im)) + cbz_hx * (hxc - hx_jm);
+
+    if (k == ek) {   // source edges and probes on this cell
+        while (ev >= 0 && a.events[ev].k == k && a.events[ev].col == cc.mycol) {
+            const Event e = a.events[ev];
+            if (e.kind == EV_SOURCE) {
+                if (upd_z) {
+                    const SrcEdge s = a.srcs[e.id];
+                    ezn = ((s.ceze * ez0 + s.cezhy * (hyc - hy_im)) + s.cezhx * (hxc - hx_jm)) +
+                          s.cezj * a.vpn[(i64)s.wave * a.steps_cap + m];
+                }
+            } else if (a.rec) {
+                const int comp = e.kind - EV_PROBE;
+                record_probe(a, e.id, comp, comp == 0 ? exn : comp == 1 ? eyn : ezn, m);
+            }
+            ++ev;
+        }
+        ek = first_k(a, ev, cc.mycol);
+    }
+
+    // CPML psi recursions and corrections, fdtdrav.h:750-911
+    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    if (cc.xs && cc.jhi && khi) {
+        const std::uint32_t q = (((std::uint32_t)k * a.ny + cc.j) * a.Lx + cc.sx) * S;
+        const float pey = cc.pxe.x * p.p_yx + cc.pxe.y * (hzc - hz_im);
+        const float pez = cc.pxe.x * p.p_zx + cc.pxe.y * (hyc - hy_im);
+        st(at((T *)a.psi_eyx, q), pey);
+        st(at((T *)a.psi_ezx, q), pez);
+        if (klo) eyn = eyn + (cy.z * dx) * pey;
+        if (cc.jlo) ezn = ezn + (cz.y * dx) * pez;
+    }
+    if (cc.ys && cc.ihi && khi) {
+        const std::uint32_t q = (((std::uint32_t)k * a.Ly + cc.sy) * a.px + cc.i) * S;
+        const float pex = cc.pye.x * p.p_xy + cc.pye.y * (hzc - hz_jm);
+        const float pez = cc.pye.x * p.p_zy + cc.pye.y * (hxc - hx_jm);
+        st(at((T *)a.psi_exy, q), pex);
+        st(at((T *)a.psi_ezy, q), pez);
+        if (klo) exn = exn + (cx.y * dy) * pex;
+        if (cc.ilo) ezn = ezn + (cz.z * dy) * pez;
+    }
+    if (zs && cc.ihi && cc.jhi) {
+        const int sz = k < a.L[4] ? k : k - (nz - a.L[5]) + a.L[4];
+        const std::uint32_t q = (((std::uint32_t)sz * a.ny + cc.j) * a.px + cc.i) * S;
+        const float pex = pze.x * p.p_xz + pze.y * (hyc - hy_km);
+        const float pey = pze.x * p.p_yz + pze.y * (hxc - hx_km);
+        st(at((T *)a.psi_exz, q), pex);
+        st(at((T *)a.psi_eyz, q), pey);
+        if (cc.jlo) exn = exn + (cx.z * dz) * pex;
+        if (cc.ilo) eyn = eyn + (cy.y * dz) * pey;
+    }
+    if (upd_x) stn(at(ex, o), exn);
+    if (upd_y) stn(at(ey, o), eyn);
+    if (upd_z) stn(at(ez, o), ezn);
+    if (cc.in_snap && k >= a.sb[2] && k < a.sb[2] + a.sb[5]) {
+        // stored-precision value, as the next step reads it, in SI units
+        const i64 nb = (i64)a.sb[3] * a.sb[4] * a.sb[5];
+        const i64 q = (cc.i - a.sb[0]) + (i64)a.sb[3] * ((cc.j - a.sb[1]) + (i64)a.sb[4] * (k - a.sb[2]));
+        float *o = a.snap + (i64)(m + 1 - a.snap_base) * 3 * nb + q;
+        o[0] = (upd_x ? rnd(exn, ex) : ex0) * a.probe_unscale[0];
+        o[nb] = (upd_y ? rnd(eyn, ex) : ey0) * a.probe_unscale[1];
+        o[2 * nb] = (upd_z ? rnd(ezn, ex) : ez0) * a.probe_unscale[2];
+    }
+    hx_km = hxc;
+    hy_km = hyc;
+}
+
+template <class T, int CM>
+__global__ void __launch_bounds__(TX * BY) E_OCC e_kernel(const KArgs a, const int m)
 {
     int tx, ty, tz;
     tile_of(a, tx, ty, tz);
-    const int i = tx * TX + (int)(threadIdx.x & (TX - 1));
-    const int j = ty * BY + (int)(threadIdx.x / TX);
-    const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
+    ECol cc;
+    cc.i = tx * TX + (int)(threadIdx.x & (TX - 1));
+    cc.j = ty * BY + (int)(threadIdx.x / TX);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int i = cc.i, j = cc.j;
     if (i >= nx || j >= ny) return;
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
-    const i64 plane = a.plane;
-    const i64 col = (i64)j * px + i;
+    cc.col = (i64)j * a.px + i;
+    cc.mycol = j * nx + i;
+    cc.ob = (std::uint32_t)a.plane * (std::uint32_t)sizeof(T);
+    cc.o0 = (std::uint32_t)cc.col * (std::uint32_t)sizeof(T) + cc.ob;
 
-    T *__restrict__ ex = (T *)a.ex;
-    T *__restrict__ ey = (T *)a.ey;
-    T *__restrict__ ez = (T *)a.ez;
-    const T *__restrict__ hx = (const T *)a.hx;
-    const T *__restrict__ hy = (const T *)a.hy;
-    const T *__restrict__ hz = (const T *)a.hz;
+    // allocation bases (guard plane below k = 0)
+    T *__restrict__ ex = (T *)a.ex - a.plane;
+    T *__restrict__ ey = (T *)a.ey - a.plane;
+    T *__restrict__ ez = (T *)a.ez - a.plane;
+    const T *__restrict__ hx = (const T *)a.hx - a.plane;
+    const T *__restrict__ hy = (const T *)a.hy - a.plane;
+    const T *__restrict__ hz = (const T *)a.hz - a.plane;
 
     // column-constant predicates (reference region guards)
-    const bool ilo = i > 0, ihi = i < nx - 1, jlo = j > 0, jhi = j < ny - 1;
-    const bool xs = (ilo && i < a.L[0]) || (i >= nx - a.L[1] && ihi);   // E x-slab, fdtdrav.h:752,777
-    const bool ys = (jlo && j < a.L[2]) || (j >= ny - a.L[3] && jhi);   // E y-slab, fdtdrav.h:803,827
-    const int sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
-    const int sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
-    const float4 pxe = xs ? a.px_prof[i] : make_float4(0.f, 0.f, 1.f, 0.f);
-    const float4 pye = ys ? a.py_prof[j] : make_float4(0.f, 0.f, 1.f, 0.f);
-    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    cc.ilo = i > 0;
+    cc.ihi = i < nx - 1;
+    cc.jlo = j > 0;
+    cc.jhi = j < ny - 1;
+    cc.xs = (cc.ilo && i < a.L[0]) || (i >= nx - a.L[1] && cc.ihi);   // E x-slab, fdtdrav.h:752,777
+    cc.ys = (cc.jlo && j < a.L[2]) || (j >= ny - a.L[3] && cc.jhi);   // E y-slab, fdtdrav.h:803,827
+    cc.sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
+    cc.sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
+    cc.pxe = cc.xs ? a.px_prof[i] : make_float4(0.f, 0.f, 1.f, 0.f);
+    cc.pye = cc.ys ? a.py_prof[j] : make_float4(0.f, 0.f, 1.f, 0.f);
+    cc.in_snap = a.snap != nullptr && i >= a.sb[0] && i < a.sb[0] + a.sb[3] && j >= a.sb[1] &&
+                 j < a.sb[1] + a.sb[4];
 
-    float4 cx = a.etab[0], cy = a.etab[1], cz = a.etab[2];
+    int ev = first_event(a, a.e_first, cc.mycol, k0);
+    int ek = first_k(a, ev, cc.mycol);
 
-    const int mycol = j * nx + i;
-    int ev = first_event(a, a.e_first, mycol, k0);
-    int ek = ev >= 0 ? a.events[ev].k : INT_MAX;
+    float hx_km = ld(at(hx, cc.o0 + (std::uint32_t)(k0 - 1) * cc.ob));
+    float hy_km = ld(at(hy, cc.o0 + (std::uint32_t)(k0 - 1) * cc.ob));
 
-    float hx_km = ld(hx + col + (i64)(k0 - 1) * plane);
-    float hy_km = ld(hy + col + (i64)(k0 - 1) * plane);
-    const bool in_snap = a.snap != nullptr && i >= a.sb[0] && i < a.sb[0] + a.sb[3] && j >= a.sb[1] &&
-                         j < a.sb[1] + a.sb[4];
-
-    EPlane nxt = e_load<T, CM>(a, hx, hy, hz, ex, ey, ez, col + (i64)k0 * plane, px);
-    for (int k = k0; k < k1; ++k) {
-        const i64 c = col + (i64)k * plane;
-        const EPlane cur = nxt;
-        if (k + 1 < k1) nxt = e_load<T, CM>(a, hx, hy, hz, ex, ey, ez, c + plane, px);
-        const float hxc = cur.hxc, hyc = cur.hyc, hzc = cur.hzc;
-        const float hy_im = cur.hy_im, hz_im = cur.hz_im;
-        const float hx_jm = cur.hx_jm, hz_jm = cur.hz_jm;
-        if (CM != CLS_UNIFORM) {
-            cx = a.etab[cur.cl * 3 + 0];
-            cy = a.etab[cur.cl * 3 + 1];
-            cz = a.etab[cur.cl * 3 + 2];
-        }
-        const bool klo = k > 0, khi = k < nz - 1;
-        const bool upd_x = ihi && jlo && jhi && klo && khi;
-        const bool upd_y = ilo && ihi && jhi && klo && khi;
-        const bool upd_z = ilo && ihi && jlo && jhi && khi;
-        const bool zs = (klo && k < a.L[4]) || (k >= nz - a.L[5] && khi);   // fdtdrav.h:853,879
-
-        // curl coefficients, divided by kappa on the slab cells (setABCPML)
-        float cbx_hz = cx.y, cbx_hy = cx.z;
-        float cby_hx = cy.y, cby_hz = cy.z;
-        float cbz_hy = cz.y, cbz_hx = cz.z;
-        float4 pze = make_float4(0.f, 0.f, 1.f, 0.f);
-        if (xs) { cby_hz = cby_hz / pxe.z; cbz_hy = cbz_hy / pxe.z; }
-        if (ys) { cbx_hz = cbx_hz / pye.z; cbz_hx = cbz_hx / pye.z; }
-        if (zs) {
-            pze = a.pz_prof[k];
-            cbx_hy = cbx_hy / pze.z;
-            cby_hx = cby_hx / pze.z;
-        }
-
-        const float ex0 = cur.ex0, ey0 = cur.ey0, ez0 = cur.ez0;
-        float exn = ex0, eyn = ey0, ezn = ez0;
-        if (upd_x) exn = (cx.x * ex0 + cbx_hz * (hzc - hz_jm)) + cbx_hy * (hyc - hy_km);
-        if (upd_y) eyn = (cy.x * ey0 + cby_hx * (hxc - hx_km)) + cby_hz * (hzc - hz_im);
-        if (upd_z) ezn = (cz.x * ez0 + cbz_hy * (hyc - hy_im)) + cbz_hx * (hxc - hx_jm);
-
-        if (k == ek) {   // source edges and probes on this cell
-            while (ev >= 0 && a.events[ev].k == k && a.events[ev].col == mycol) {
-                const Event e = a.events[ev];
-                if (e.kind == EV_SOURCE) {
-                    if (upd_z) {
-                        const SrcEdge s = a.srcs[e.id];
-                        ezn = ((s.ceze * ez0 + s.cezhy * (hyc - hy_im)) + s.cezhx * (hxc - hx_jm)) +
-                              s.cezj * a.vpn[(i64)s.wave * a.steps_cap + m];
-                    }
-                } else if (a.rec) {
-                    const int comp = e.kind - EV_PROBE;
-                    record_probe(a, e.id, comp, comp == 0 ? exn : comp == 1 ? eyn : ezn, m);
-                }
-                ++ev;
-            }
-            ek = (a.events[ev].col == mycol) ? a.events[ev].k : INT_MAX;
-        }
-
-        // CPML psi recursions and corrections, fdtdrav.h:750-911
-        if (xs && jhi && khi) {
-            const i64 q = ((i64)k * ny + j) * a.Lx + sx;
-            T *pa = (T *)a.psi_eyx + q;
-            T *pb = (T *)a.psi_ezx + q;
-            const float pey = pxe.x * ld(pa) + pxe.y * (hzc - hz_im);
-            const float pez = pxe.x * ld(pb) + pxe.y * (hyc - hy_im);
-            st(pa, pey);
-            st(pb, pez);
-            if (klo) eyn = eyn + (cy.z * dx) * pey;
-            if (jlo) ezn = ezn + (cz.y * dx) * pez;
-        }
-        if (ys && ihi && khi) {
-            const i64 q = ((i64)k * a.Ly + sy) * px + i;
-            T *pa = (T *)a.psi_exy + q;
-            T *pb = (T *)a.psi_ezy + q;
-            const float pex = pye.x * ld(pa) + pye.y * (hzc - hz_jm);
-            const float pez = pye.x * ld(pb) + pye.y * (hxc - hx_jm);
-            st(pa, pex);
-            st(pb, pez);
-            if (klo) exn = exn + (cx.y * dy) * pex;
-            if (ilo) ezn = ezn + (cz.z * dy) * pez;
-        }
-        if (zs && ihi && jhi) {
-            const int sz = k < a.L[4] ? k : k - (nz - a.L[5]) + a.L[4];
-            const i64 q = ((i64)sz * ny + j) * px + i;
-            T *pa = (T *)a.psi_exz + q;
-            T *pb = (T *)a.psi_eyz + q;
-            const float pex = pze.x * ld(pa) + pze.y * (hyc - hy_km);
-            const float pey = pze.x * ld(pb) + pze.y * (hxc - hx_km);
-            st(pa, pex);
-            st(pb, pey);
-            if (jlo) exn = exn + (cx.z * dz) * pex;
-            if (ilo) eyn = eyn + (cy.y * dz) * pey;
-        }
-        if (upd_x) stn(ex + c, exn);
-        if (upd_y) stn(ey + c, eyn);
-        if (upd_z) stn(ez + c, ezn);
-        if (in_snap && k >= a.sb[2] && k < a.sb[2] + a.sb[5]) {
-            // stored-precision value, as the next step reads it, in SI units
-            const i64 nb = (i64)a.sb[3] * a.sb[4] * a.sb[5];
-            const i64 q = (i - a.sb[0]) + (i64)a.sb[3] * ((j - a.sb[1]) + (i64)a.sb[4] * (k - a.sb[2]));
-            float *o = a.snap + (i64)(m + 1 - a.snap_base) * 3 * nb + q;
-            o[0] = ld(ex + c) * a.probe_unscale[0];
-            o[nb] = ld(ey + c) * a.probe_unscale[1];
-            o[2 * nb] = ld(ez + c) * a.probe_unscale[2];
-        }
-        hx_km = hxc;
-        hy_km = hyc;
+    EPlane A, B;
+    Coef tA, tB;
+    if (CM == CLS_UNIFORM) {
+        tA = coef_of<CM>(a.etab, 0);
+        tB = tA;
+    } else {
+        tA = coef_of<CM>(a.etab, class_at<CM>(a.cls, (std::uint32_t)k0 * (std::uint32_t)a.plane + (std::uint32_t)cc.col));
+    }
+    e_fetch<T, CM>(A, a, cc, ex, ey, ez, hx, hy, hz, k0, k1);
+    int k = k0;
+    // Steady state: the fetches are unconditional, so on every path the
+    // next plane's loads are the younger ones when this plane's are awaited.
+    for (; k + 2 < k1; k += 2) {
+        e_fetch<T, CM>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        if (CM != CLS_UNIFORM) tB = coef_of<CM>(a.etab, A.cln);
+        e_update<T, CM>(A, tA, a, cc, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
+        e_fetch<T, CM>(A, a, cc, ex, ey, ez, hx, hy, hz, k + 2, k1);
+        if (CM != CLS_UNIFORM) tA = coef_of<CM>(a.etab, B.cln);
+        e_update<T, CM>(B, tB, a, cc, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
+    }
+    // last one or two planes of the chunk
+    if (k + 1 < k1) {
+        e_fetch<T, CM>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        if (CM != CLS_UNIFORM) tB = coef_of<CM>(a.etab, A.cln);
+        e_update<T, CM>(A, tA, a, cc, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
+        e_update<T, CM>(B, tB, a, cc, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
+    } else {
+        e_update<T, CM>(A, tA, a, cc, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
     }
 }
 
 // ---- H half-step ----------------------------------------------------------
+
+struct HCol {
+    int i, j, sx, sy, mycol;
+    i64 col;
+    std::uint32_t o0;    // byte offset of (i, j, 0) from the guard-plane base
+    std::uint32_t ob;    // bytes per plane
+    bool ihi, jhi, xs, ys;
+    float4 pxm, pym;
+};
+
+// Everything the H update of plane k reads (E of planes k and k+1 around
+// it, its own H, slab psi).
+struct HPlane {
+    float ex_kp, ey_kp, ezc, ez_ip, ey_ip, ez_jp, ex_jp, hx0, hy0, hz0;
+    float p_yx, p_zx, p_xy, p_zy, p_xz, p_yz;
+    int cln;
+};
+
+__device__ __forceinline__ bool h_zslab(const KArgs &a, int k)
+{
+    return k < a.L[4] || (k >= a.nz - a.L[5] && k < a.nz - 1);   // fdtdrav.h:500,531
+}
+
 template <class T, int CM>
-__global__ void __launch_bounds__(TX * BY) h_kernel(const KArgs a, const int m)
+__device__ __forceinline__ void h_fetch(HPlane &p, const KArgs &a, const HCol &cc, const T *ex, const T *ey,
+                                        const T *ez, const T *hx, const T *hy, const T *hz, int k, int k1)
+{
+    constexpr std::uint32_t S = sizeof(T);
+    const std::uint32_t o = cc.o0 + (std::uint32_t)k * cc.ob;
+    const std::uint32_t okp = o + cc.ob;
+    const std::uint32_t ojp = o + (std::uint32_t)a.px * S;
+    const int px = a.px, ny = a.ny;
+    p.ex_kp = ld(at(ex, okp));
+    p.ey_kp = ld(at(ey, okp));
+    p.ezc = ld(at(ez, o));
+    p.ez_ip = ld(at(ez, o + S));
+    p.ey_ip = ld(at(ey, o + S));
+    p.ez_jp = ld(at(ez, ojp));
+    p.ex_jp = ld(at(ex, ojp));
+    p.hx0 = ldn(at(hx, o));
+    p.hy0 = ldn(at(hy, o));
+    p.hz0 = ldn(at(hz, o));
+    p.cln = (CM != CLS_UNIFORM && k + 1 < k1)
+                ? class_at<CM>(a.cls, (std::uint32_t)(k + 1) * (std::uint32_t)a.plane + (std::uint32_t)cc.col)
+                : 0;
+    p.p_yx = p.p_zx = p.p_xy = p.p_zy = p.p_xz = p.p_yz = 0.f;
+    if (cc.xs) {
+        const std::uint32_t q = (((std::uint32_t)k * ny + cc.j) * a.Lx + cc.sx) * S;
+        p.p_yx = ld(at((const T *)a.psi_hyx, q));
+        p.p_zx = ld(at((const T *)a.psi_hzx, q));
+    }
+    if (cc.ys) {
+        const std::uint32_t q = (((std::uint32_t)k * a.Ly + cc.sy) * px + cc.i) * S;
+        p.p_xy = ld(at((const T *)a.psi_hxy, q));
+        p.p_zy = ld(at((const T *)a.psi_hzy, q));
+    }
+    if (h_zslab(a, k)) {
+        const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
+        const std::uint32_t q = (((std::uint32_t)sz * ny + cc.j) * px + cc.i) * S;
+        p.p_xz = ld(at((const T *)a.psi_hxz, q));
+        p.p_yz = ld(at((const T *)a.psi_hyz, q));
+    }
+}
+
+template <class T, int CM>
+__device__ __forceinline__ void h_update(const HPlane &p, const Coef &t, const KArgs &a, const HCol &cc, T *hx,
+                                         T *hy, T *hz, int k, int m, float &ex_c, float &ey_c, int &ev, int &ek)
+{
+    const int nz = a.nz;
+    constexpr std::uint32_t S = sizeof(T);
+    const std::uint32_t o = cc.o0 + (std::uint32_t)k * cc.ob;
+    const float ex_kp = p.ex_kp, ey_kp = p.ey_kp;
+    const float ezc = p.ezc;
+    const float ez_ip = p.ez_ip, ey_ip = p.ey_ip;
+    const float ez_jp = p.ez_jp, ex_jp = p.ex_jp;
+    const float4 cx = t.x, cy = t.y, cz = t.z;
+    const bool khi = k < nz - 1;
+    const bool upd_x = cc.jhi && khi;
+    const bool upd_y = cc.ihi && khi;
+    const bool upd_z = cc.ihi && cc.jhi;
+    const bool zs = h_zslab(a, k);
+
+    float chx_ey = cx.y, chx_ez = cx.z;
+    float chy_ez = cy.y, chy_ex = cy.z;
+    float chz_ex = cz.y, chz_ey = cz.z;
+    float4 pzm = make_float4(0.f, 0.f, 1.f, 0.f);
+    if (cc.xs) { chy_ez = chy_ez / cc.pxm.z; chz_ey = chz_ey / cc.pxm.z; }
+    if (cc.ys) { chx_ez = chx_ez / cc.pym.z; chz_ex = chz_ex / cc.pym.z; }
+    if (zs) {
+        pzm = uload(a.pz_prof + nz + k);
+        chx_ey = chx_ey / pzm.z;
+        chy_ex = chy_ex / pzm.z;
+    }
+
+    const float hx0 = p.hx0, hy0 = p.hy0, hz0 = p.hz0;
+    float hxn = hx0, hyn = hy0, hzn = hz0;
+    if (upd_x) hxn = (cx.x * hx0 + chx_ey * (ey_kp - ey_c)) + chx_ez * (ez_jp - ezc);
+    if (upd_y) hyn = (cy.x * hy0 + chy_ez * (ez_ip - ezc)) + chy_ex * (ex_kp - ex_c);
+    if (upd_z) hzn = (cz.x * hz0 + chz_ex * (ex_jp - ex_c)) + chz_ey * (ey_ip - ey_c);
+
+    if (k == ek) {   // probes on this cell
+        while (ev >= 0 && a.events[ev].k == k && a.events[ev].col == cc.mycol) {
+            const Event e = a.events[ev];
+            const int comp = e.kind - EV_PROBE;
+            if (a.rec) record_probe(a, e.id, comp, comp == 3 ? hxn : comp == 4 ? hyn : hzn, m);
+            ++ev;
+        }
+        ek = first_k(a, ev, cc.mycol);
+    }
+
+    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    if (cc.xs) {
+        const std::uint32_t q = (((std::uint32_t)k * a.ny + cc.j) * a.Lx + cc.sx) * S;
+        const float phy = cc.pxm.x * p.p_yx + cc.pxm.y * (ez_ip - ezc);
+        const float phz = cc.pxm.x * p.p_zx + cc.pxm.y * (ey_ip - ey_c);
+        st(at((T *)a.psi_hyx, q), phy);
+        st(at((T *)a.psi_hzx, q), phz);
+        if (khi) hyn = hyn + (cy.y * dx) * phy;
+        if (cc.jhi) hzn = hzn + (cz.z * dx) * phz;
+    }
+    if (cc.ys) {
+        const std::uint32_t q = (((std::uint32_t)k * a.Ly + cc.sy) * a.px + cc.i) * S;
+        const float phx = cc.pym.x * p.p_xy + cc.pym.y * (ez_jp - ezc);
+        const float phz = cc.pym.x * p.p_zy + cc.pym.y * (ex_jp - ex_c);
+        st(at((T *)a.psi_hxy, q), phx);
+        st(at((T *)a.psi_hzy, q), phz);
+        if (khi) hxn = hxn + (cx.z * dy) * phx;
+        if (cc.ihi) hzn = hzn + (cz.y * dy) * phz;
+    }
+    if (zs) {
+        const int sz = k < a.L[4] ? k : k - (nz - a.L[5]) + a.L[4];
+        const std::uint32_t q = (((std::uint32_t)sz * a.ny + cc.j) * a.px + cc.i) * S;
+        const float phx = pzm.x * p.p_xz + pzm.y * (ey_kp - ey_c);
+        const float phy = pzm.x * p.p_yz + pzm.y * (ex_kp - ex_c);
+        st(at((T *)a.psi_hxz, q), phx);
+        st(at((T *)a.psi_hyz, q), phy);
+        if (cc.jhi) hxn = hxn + (cx.y * dz) * phx;
+        if (cc.ihi) hyn = hyn + (cy.z * dz) * phy;
+    }
+    if (upd_x) stn(at(hx, o), hxn);
+    if (upd_y) stn(at(hy, o), hyn);
+    if (upd_z) stn(at(hz, o), hzn);
+    ex_c = ex_kp;
+    ey_c = ey_kp;
+}
+
+template <class T, int CM>
+__global__ void __launch_bounds__(TX * BY) H_OCC h_kernel(const KArgs a, const int m)
 {
     int tx, ty, tz;
     tile_of(a, tx, ty, tz);
-    const int i = tx * TX + (int)(threadIdx.x & (TX - 1));
-    const int j = ty * BY + (int)(threadIdx.x / TX);
-    const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
+    HCol cc;
+    cc.i = tx * TX + (int)(threadIdx.x & (TX - 1));
+    cc.j = ty * BY + (int)(threadIdx.x / TX);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int i = cc.i, j = cc.j;
     if (i >= nx || j >= ny) return;
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
-    const i64 plane = a.plane;
-    const i64 col = (i64)j * px + i;
+    cc.col = (i64)j * a.px + i;
+    cc.mycol = j * nx + i;
+    cc.ob = (std::uint32_t)a.plane * (std::uint32_t)sizeof(T);
+    cc.o0 = (std::uint32_t)cc.col * (std::uint32_t)sizeof(T) + cc.ob;
 
-    const T *__restrict__ ex = (const T *)a.ex;
-    const T *__restrict__ ey = (const T *)a.ey;
-    const T *__restrict__ ez = (const T *)a.ez;
-    T *__restrict__ hx = (T *)a.hx;
-    T *__restrict__ hy = (T *)a.hy;
-    T *__restrict__ hz = (T *)a.hz;
+    // allocation bases (guard plane below k = 0)
+    const T *__restrict__ ex = (const T *)a.ex - a.plane;
+    const T *__restrict__ ey = (const T *)a.ey - a.plane;
+    const T *__restrict__ ez = (const T *)a.ez - a.plane;
+    T *__restrict__ hx = (T *)a.hx - a.plane;
+    T *__restrict__ hy = (T *)a.hy - a.plane;
+    T *__restrict__ hz = (T *)a.hz - a.plane;
 
-    const bool ihi = i < nx - 1, jhi = j < ny - 1;
-    const bool xs = i < a.L[0] || (i >= nx - a.L[1] && ihi);   // H x-slab, fdtdrav.h:402,424
-    const bool ys = j < a.L[2] || (j >= ny - a.L[3] && jhi);   // H y-slab, fdtdrav.h:447,471
-    const int sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
-    const int sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
-    const float4 pxm = xs ? a.px_prof[nx + i] : make_float4(0.f, 0.f, 1.f, 0.f);
-    const float4 pym = ys ? a.py_prof[ny + j] : make_float4(0.f, 0.f, 1.f, 0.f);
-    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    cc.ihi = i < nx - 1;
+    cc.jhi = j < ny - 1;
+    cc.xs = i < a.L[0] || (i >= nx - a.L[1] && cc.ihi);   // H x-slab, fdtdrav.h:402,424
+    cc.ys = j < a.L[2] || (j >= ny - a.L[3] && cc.jhi);   // H y-slab, fdtdrav.h:447,471
+    cc.sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
+    cc.sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
+    cc.pxm = cc.xs ? a.px_prof[nx + i] : make_float4(0.f, 0.f, 1.f, 0.f);
+    cc.pym = cc.ys ? a.py_prof[ny + j] : make_float4(0.f, 0.f, 1.f, 0.f);
 
-    float4 cx = a.htab[0], cy = a.htab[1], cz = a.htab[2];
+    int ev = first_event(a, a.h_first, cc.mycol, k0);
+    int ek = first_k(a, ev, cc.mycol);
 
-    const int mycol = j * nx + i;
-    int ev = first_event(a, a.h_first, mycol, k0);
-    int ek = ev >= 0 ? a.events[ev].k : INT_MAX;
+    float ex_c = ld(at(ex, cc.o0 + (std::uint32_t)k0 * cc.ob));
+    float ey_c = ld(at(ey, cc.o0 + (std::uint32_t)k0 * cc.ob));
 
-    float ex_c = ld(ex + col + (i64)k0 * plane);
-    float ey_c = ld(ey + col + (i64)k0 * plane);
-    HPlane nxt = h_load<T, CM>(a, ex, ey, ez, hx, hy, hz, col + (i64)k0 * plane, plane, px);
-
-    for (int k = k0; k < k1; ++k) {
-        const i64 c = col + (i64)k * plane;
-        const HPlane cur = nxt;
-        if (k + 1 < k1) nxt = h_load<T, CM>(a, ex, ey, ez, hx, hy, hz, c + plane, plane, px);
-        const float ex_kp = cur.ex_kp, ey_kp = cur.ey_kp;
-        const float ezc = cur.ezc;
-        const float ez_ip = cur.ez_ip, ey_ip = cur.ey_ip;
-        const float ez_jp = cur.ez_jp, ex_jp = cur.ex_jp;
-        if (CM != CLS_UNIFORM) {
-            cx = a.htab[cur.cl * 3 + 0];
-            cy = a.htab[cur.cl * 3 + 1];
-            cz = a.htab[cur.cl * 3 + 2];
-        }
-        const bool khi = k < nz - 1;
-        const bool upd_x = jhi && khi;
-        const bool upd_y = ihi && khi;
-        const bool upd_z = ihi && jhi;
-        const bool zs = k < a.L[4] || (k >= nz - a.L[5] && khi);   // fdtdrav.h:500,531
-
-        float chx_ey = cx.y, chx_ez = cx.z;
-        float chy_ez = cy.y, chy_ex = cy.z;
-        float chz_ex = cz.y, chz_ey = cz.z;
-        float4 pzm = make_float4(0.f, 0.f, 1.f, 0.f);
-        if (xs) { chy_ez = chy_ez / pxm.z; chz_ey = chz_ey / pxm.z; }
-        if (ys) { chx_ez = chx_ez / pym.z; chz_ex = chz_ex / pym.z; }
-        if (zs) {
-            pzm = a.pz_prof[nz + k];
-            chx_ey = chx_ey / pzm.z;
-            chy_ex = chy_ex / pzm.z;
-        }
-
-        const float hx0 = cur.hx0, hy0 = cur.hy0, hz0 = cur.hz0;
-        float hxn = hx0, hyn = hy0, hzn = hz0;
-        if (upd_x) hxn = (cx.x * hx0 + chx_ey * (ey_kp - ey_c)) + chx_ez * (ez_jp - ezc);
-        if (upd_y) hyn = (cy.x * hy0 + chy_ez * (ez_ip - ezc)) + chy_ex * (ex_kp - ex_c);
-        if (upd_z) hzn = (cz.x * hz0 + chz_ex * (ex_jp - ex_c)) + chz_ey * (ey_ip - ey_c);
-
-        if (k == ek) {   // probes on this cell
-            while (ev >= 0 && a.events[ev].k == k && a.events[ev].col == mycol) {
-                const Event e = a.events[ev];
-                const int comp = e.kind - EV_PROBE;
-                if (a.rec) record_probe(a, e.id, comp, comp == 3 ? hxn : comp == 4 ? hyn : hzn, m);
-                ++ev;
-            }
-            ek = (a.events[ev].col == mycol) ? a.events[ev].k : INT_MAX;
-        }
-
-        if (xs) {
-            const i64 q = ((i64)k * ny + j) * a.Lx + sx;
-            T *pa = (T *)a.psi_hyx + q;
-            T *pb = (T *)a.psi_hzx + q;
-            const float phy = pxm.x * ld(pa) + pxm.y * (ez_ip - ezc);
-            const float phz = pxm.x * ld(pb) + pxm.y * (ey_ip - ey_c);
-            st(pa, phy);
-            st(pb, phz);
-            if (khi) hyn = hyn + (cy.y * dx) * phy;
-            if (jhi) hzn = hzn + (cz.z * dx) * phz;
-        }
-        if (ys) {
-            const i64 q = ((i64)k * a.Ly + sy) * px + i;
-            T *pa = (T *)a.psi_hxy + q;
-            T *pb = (T *)a.psi_hzy + q;
-            const float phx = pym.x * ld(pa) + pym.y * (ez_jp - ezc);
-            const float phz = pym.x * ld(pb) + pym.y * (ex_jp - ex_c);
-            st(pa, phx);
-            st(pb, phz);
-            if (khi) hxn = hxn + (cx.z * dy) * phx;
-            if (ihi) hzn = hzn + (cz.y * dy) * phz;
-        }
-        if (zs) {
-            const int sz = k < a.L[4] ? k : k - (nz - a.L[5]) + a.L[4];
-            const i64 q = ((i64)sz * ny + j) * px + i;
-            T *pa = (T *)a.psi_hxz + q;
-            T *pb = (T *)a.psi_hyz + q;
-            const float phx = pzm.x * ld(pa) + pzm.y * (ey_kp - ey_c);
-            const float phy = pzm.x * ld(pb) + pzm.y * (ex_kp - ex_c);
-            st(pa, phx);
-            st(pb, phy);
-            if (jhi) hxn = hxn + (cx.y * dz) * phx;
-            if (ihi) hyn = hyn + (cy.z * dz) * phy;
-        }
-        if (upd_x) stn(hx + c, hxn);
-        if (upd_y) stn(hy + c, hyn);
-        if (upd_z) stn(hz + c, hzn);
-        ex_c = ex_kp;
-        ey_c = ey_kp;
+    HPlane A, B;
+    Coef tA, tB;
+    if (CM == CLS_UNIFORM) {
+        tA = coef_of<CM>(a.htab, 0);
+        tB = tA;
+    } else {
+        tA = coef_of<CM>(a.htab, class_at<CM>(a.cls, (std::uint32_t)k0 * (std::uint32_t)a.plane + (std::uint32_t)cc.col));
+    }
+    h_fetch<T, CM>(A, a, cc, ex, ey, ez, hx, hy, hz, k0, k1);
+    int k = k0;
+    for (; k + 2 < k1; k += 2) {
+        h_fetch<T, CM>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        if (CM != CLS_UNIFORM) tB = coef_of<CM>(a.htab, A.cln);
+        h_update<T, CM>(A, tA, a, cc, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
+        h_fetch<T, CM>(A, a, cc, ex, ey, ez, hx, hy, hz, k + 2, k1);
+        if (CM != CLS_UNIFORM) tA = coef_of<CM>(a.htab, B.cln);
+        h_update<T, CM>(B, tB, a, cc, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
+    }
+    if (k + 1 < k1) {
+        h_fetch<T, CM>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        if (CM != CLS_UNIFORM) tB = coef_of<CM>(a.htab, A.cln);
+        h_update<T, CM>(A, tA, a, cc, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
+        h_update<T, CM>(B, tB, a, cc, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
+    } else {
+        h_update<T, CM>(A, tA, a, cc, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
     }
 }
 
