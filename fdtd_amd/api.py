@@ -16,6 +16,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfdtd_amd.so")
 
 FDTD_OK = 0
+FDTD_E_INVALID, FDTD_E_STATE, FDTD_E_RANGE, FDTD_E_NOMEM = -1, -2, -3, -4
+FDTD_E_DEVICE, FDTD_E_UNSUPPORTED, FDTD_E_NONFINITE, FDTD_E_CHECKSUM = -5, -6, -7, -8
 ERRORS = {-1: "FDTD_E_INVALID", -2: "FDTD_E_STATE", -3: "FDTD_E_RANGE", -4: "FDTD_E_NOMEM",
           -5: "FDTD_E_DEVICE", -6: "FDTD_E_UNSUPPORTED", -7: "FDTD_E_NONFINITE",
           -8: "FDTD_E_CHECKSUM"}

@@ -208,6 +208,12 @@ int do_finalize(fdtd_ctx *c)
     const char *ks = std::getenv("FDTD_KERNEL");
     c->fused = ks && std::strcmp(ks, "fused") == 0;
     c->field_elems = a.plane * (in.nz + 2);
+    // The sweep kernels address every array with 32-bit byte offsets.
+    const size_t cls_bytes = (size_t)a.plane * in.nz * (b.mode == CLS_U32 ? 4 : 2);
+    if ((size_t)c->field_elems * c->esz >= ((size_t)1 << 32) || cls_bytes >= ((size_t)1 << 32))
+        return fail(c, FDTD_E_UNSUPPORTED,
+                    "grid too large: each field array (with its two guard planes) and the class map "
+                    "must be under 4 GiB (%zu bytes per field)", (size_t)c->field_elems * c->esz);
     for (int s = 0; s < (c->fused ? 2 : 1); ++s)
         for (int f = 0; f < 6; ++f) {
             int rc = dev_alloc(c, &c->fields_set[s][f], (size_t)c->field_elems * c->esz);

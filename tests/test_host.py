@@ -1,0 +1,186 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every function include/fdtd.h declares, the Python binding matches the
+header, argument validation in fdtd_create, the seeded scene generator and
+the benchmark's bookkeeping."""
+import ctypes as C
+import math
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "fdtd.h")
+
+
+def header_functions():
+    """Name -> number of parameters of every function declared in fdtd.h."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"(?:\bint\s+|\bconst char\s*\*\s*|\bsize_t\s+|\bdouble\s+)(fdtd_\w+)\s*\(([^)]*)\)\s*;",
+                         text):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from fdtd_amd import api, build
+    if not os.path.exists(api.LIB_PATH):
+        build.build()
+    return api.load_library()
+
+
+def test_header_declares_the_boundary():
+    fns = header_functions()
+    # the drop-in boundary for marchingLoop (fdtdrav.h:3425-3453)
+    for name in ("fdtd_create", "fdtd_destroy", "fdtd_last_error", "fdtd_set_cpml", "fdtd_add_box",
+                 "fdtd_add_lumped_source", "fdtd_add_probe", "fdtd_set_dft", "fdtd_finalize", "fdtd_run",
+                 "fdtd_get_probes", "fdtd_get_dft", "fdtd_checkpoint_save", "fdtd_checkpoint_restore",
+                 "fdtd_adjoint_gradient"):
+        assert name in fns, name
+
+
+def test_library_exports_every_header_function(lib):
+    fns = header_functions()
+    assert len(fns) >= 15
+    missing = [n for n in fns if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_matches_header(lib):
+    from fdtd_amd import api
+    fns = header_functions()
+    assert set(api.EXPORTS) == set(fns)
+    for name, nargs in fns.items():
+        assert len(api.EXPORTS[name][1]) == nargs, name
+
+
+def _grid(**kw):
+    from fdtd_amd import api
+    g = api.FdtdGrid()
+    g.nx, g.ny, g.nz = kw.get("n", (16, 16, 16))
+    g.dx = g.dy = g.dz = kw.get("d", 1e-3)
+    g.dt = kw.get("dt", 0.0)
+    g.courant = kw.get("courant", 0.99)
+    return g
+
+
+@pytest.mark.parametrize("bad", [dict(n=(3, 16, 16)), dict(d=0.0), dict(courant=1.5),
+                                 dict(dt=1e-9), dict(courant=0.0)])
+def test_create_rejects_invalid_grids(lib, bad):
+    from fdtd_amd import api
+    ctx = C.c_void_p()
+    rc = lib.fdtd_create(C.byref(_grid(**bad)), 0, C.byref(ctx))
+    assert rc == api.FDTD_E_INVALID
+    assert not ctx.value
+
+
+def test_create_rejects_unknown_storage(lib):
+    from fdtd_amd import api
+    ctx = C.c_void_p()
+    assert lib.fdtd_create(C.byref(_grid()), 7, C.byref(ctx)) == api.FDTD_E_INVALID
+
+
+def test_create_without_device_fails_loudly(lib):
+    """On a host with no HIP device the engine refuses to start (there is no
+    CPU fallback); with a device it creates a context."""
+    from fdtd_amd import api
+    ctx = C.c_void_p()
+    rc = lib.fdtd_create(C.byref(_grid()), 0, C.byref(ctx))
+    if rc == api.FDTD_OK:
+        assert lib.fdtd_destroy(ctx) == api.FDTD_OK
+    else:
+        assert rc == api.FDTD_E_DEVICE and not ctx.value
+
+
+def test_splitmix64_published_vectors():
+    from fdtd_amd.scenes import SplitMix64
+    r = SplitMix64(1234567)
+    assert [r.next() for _ in range(5)] == [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                                            4593380528125082431, 16408922859458223821]
+    assert SplitMix64(0).next() == 0xE220A8397B1DCDAF
+
+
+def test_courant_dt_values():
+    from fdtd_amd import scenes
+    # 0.99 x the 3-D Courant limit at 1 mm and at 0.5 mm cells
+    assert abs(scenes.courant_dt(1e-3, 1e-3, 1e-3, 0.99) - 1.9066e-12) < 5e-17
+    assert abs(scenes.courant_dt(5e-4, 5e-4, 5e-4, 0.99) - 0.9533e-12) < 5e-17
+
+
+def test_workloads_match_their_recipes():
+    from fdtd_amd import scenes
+    sc = scenes.yee64()
+    assert (sc.nx, sc.ny, sc.nz, sc.steps, sc.cpml.thick[0], len(sc.sources), len(sc.probes)) == \
+        (64, 64, 64, 500, 10, 1, 8)
+    assert sc.sources[0].ijk == (32, 32, 32, 32, 32, 33)
+    sc = scenes.dielectric256()
+    assert (sc.nx, sc.steps, sc.cpml.thick[0], len(sc.boxes), len(sc.sources), len(sc.probes),
+            len(sc.freqs), sc.seed) == (256, 2000, 12, 64, 4, 64, 16, 1002)
+    assert abs(sc.freqs[0] - 1e9) < 1 and abs(sc.freqs[-1] - 30e9) < 1
+    sc = scenes.slab512()
+    assert (sc.nx, sc.ny, sc.nz, sc.steps, sc.cpml.thick[0], len(sc.boxes), len(sc.sources),
+            len(sc.probes), sc.seed) == (512, 512, 256, 3000, 16, 256, 8, 128, 1003)
+    sc = scenes.adjoint128()
+    assert sc.design == (48, 48, 24, 80, 80, 40) and sc.steps == 1000 and sc.freqs == [10e9]
+    assert sc.probes[0][1] == 80 - 1 + 20
+    assert 1.0 <= sc.design_eps.min() and sc.design_eps.max() < 4.0
+    # sigma_max = 0.8 (m + 1) / (eta0 dx)
+    assert abs(sc.cpml.sigma_max[0] - 0.8 * 4 / (scenes.ETA0 * 1e-3)) < 1e-9
+
+
+def test_scene_generation_is_deterministic():
+    from fdtd_amd import scenes
+    a, b = scenes.slab512(), scenes.slab512()
+    assert a.boxes == b.boxes and a.sources == b.sources and a.probes == b.probes
+    # positions stay in the interior band [L+2, N-L-3], probes never on a source edge
+    edges = {(s.ijk[0], s.ijk[1], s.ijk[2]) for s in a.sources}
+    for (_, i, j, k) in a.probes:
+        assert 18 <= i <= 512 - 19 and 18 <= j <= 512 - 19 and 18 <= k <= 256 - 19
+        assert (i, j, k) not in edges
+    assert len(set(a.probes)) == len(a.probes)
+
+
+def test_bench_byte_model():
+    sys.path.insert(0, ROOT)
+    import bench
+    from fdtd_amd import scenes
+    sc = scenes.slab512()
+    info = {"n_classes": 8160, "fused": 0}
+    assert bench.b_alg(sc, info, 4) == pytest.approx(82.0)
+    assert bench.b_alg(sc, info, 2) == pytest.approx(42.0)
+    assert bench.b_alg(sc, {"n_classes": 8160, "fused": 1}, 4) == pytest.approx(58.0)
+    sc = scenes.yee64()
+    assert bench.b_alg(sc, {"n_classes": 1, "fused": 0}, 4) == pytest.approx(102.0)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_copy_probe_library_exports():
+    from fdtd_amd import build
+    path = build.COPY_LIB
+    if not os.path.exists(path):
+        build.build()
+    lib = C.CDLL(path)
+    assert hasattr(lib, "hbm_copy_gbs") and hasattr(lib, "hbm_copy_variant_gbs")
+
+
+def test_product_path_has_no_oracle_dependency():
+    """The engine package never imports the oracle (test infrastructure)."""
+    pkg = os.path.join(ROOT, "fdtd_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".cpp", ".h")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert not re.search(r"^\s*(import|from)\s+oracle|oracle_\w+\s*\(|liboracle", text, re.M), f
