@@ -25,6 +25,16 @@ sys.path.insert(0, ROOT)
 from fdtd_amd import api, scenes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+
+# HBM bytes per time step measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (separate runs) on MI355X, scaled by what the same counters read
+# for a copy kernel of known size (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0);
+# summaries and commands in profiles/split_sweeps_slab512_hbm_traffic.md.
+# Keyed by (workload, storage, kernel path); re-measure after kernel changes.
+MEASURED_TRAFFIC = {
+    ("slab512", "f32", "split"): (965003.6 * 2.0 + 903933.3 + 914452.4 * 2.0 + 912249.9) * 1024,
+    ("slab512", "f16", "split"): (523694.1 * 2.0 + 451680.9 + 458411.5 * 2.0 + 455733.8) * 1024,
+}
 METRIC = "Mcell-updates/s and % HBM peak; FP16-store vs FP32 at stated probe error"
 
 
@@ -46,6 +56,12 @@ def b_alg(sc, info, storage_bytes):
     return fields + cls_bytes + psi
 
 
+def _traffic(workload, storage, info):
+    """Measured HBM bytes per step for this workload and kernel path, or None."""
+    t = MEASURED_TRAFFIC.get((workload, storage, "fused" if info["fused"] else "split"))
+    return None if t is None else round(t)
+
+
 def run_form(sc, storage, steps, warmup, profile_steps):
     sim = api.Simulation.from_scene(sc, storage, n_steps=warmup + steps + profile_steps)
     sim.run(warmup)
@@ -58,9 +74,10 @@ def run_form(sc, storage, steps, warmup, profile_steps):
         sim.profile(profile_steps)
     prof = sim.info()
     probes = sim.probes()[:, warmup:warmup + steps]
+    dft = sim.dft()
     sim.close()
     return dict(wall_s=wall, event_ms=ev_ms, e_ms=prof["e_kernel_ms"], h_ms=prof["h_kernel_ms"],
-                step_ms=prof["step_kernel_ms"], info=inf, probes=probes)
+                step_ms=prof["step_kernel_ms"], info=inf, probes=probes, dft=dft)
 
 
 def cpu_baseline(sc, seconds_target=15.0):
@@ -162,7 +179,7 @@ def main():
     ap.add_argument("--fd-cells", type=int, default=16)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
     args = ap.parse_args()
-    if args.copy_probe:
+    if args.copy_probe or args.copy_variants:
         import ctypes
         lib = ctypes.CDLL(os.path.join(ROOT, "fdtd_amd", "libhbmcopy.so"))
         lib.hbm_copy_gbs.restype = ctypes.c_double
@@ -205,7 +222,8 @@ def main():
                    "dft_freqs": len(sc.freqs), "seed": sc.seed,
                    "coefficient_classes": f32["info"]["n_classes"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": _traffic(sc.name, "f32", f32["info"]),
                      "kernel": "step_kernel" if f32["info"]["fused"] else "e_kernel+h_kernel",
                      "bytes_per_cell": round(bpc, 3), "kernel_ms": round(kms, 5)},
         "timed_loop_frac": round(step_gbs / HBM_PEAK_GBS, 4),
@@ -217,13 +235,22 @@ def main():
         b16 = b_alg(sc, f16["info"], 2)
         p32 = f32["probes"].astype(np.float64)
         p16 = f16["probes"].astype(np.float64)
-        live = np.linalg.norm(p32, axis=1) > 0
-        rel = np.linalg.norm(p16 - p32, axis=1)[live] / np.linalg.norm(p32, axis=1)[live]
+        # probes carrying signal (a probe the pulse has not reached is noise
+        # over noise); the FP32 run is the reference: it is within 1e-5 of the
+        # FP64 oracle on every scene the tests check
+        n32 = np.linalg.norm(p32, axis=1)
+        live = n32 >= 1e-3 * np.abs(p32).max(initial=0.0) * np.sqrt(max(p32.shape[1], 1))
+        rel = np.linalg.norm(p16 - p32, axis=1)[live] / n32[live]
+        d32, d16 = f32["dft"], f16["dft"]
+        dft_err = float(np.abs(d16 - d32).max() / np.abs(d32).max()) if d32.size and np.abs(d32).max() > 0 else None
         out["fp16"] = {"value": round(v16, 1), "ms_per_step": round(f16["event_ms"] / args.steps, 5),
                        "speedup_vs_fp32": round(v16 / value, 3),
-                       "probe_rel_l2_vs_fp32_max": float(f"{rel.max():.3e}"),
-                       "probe_rel_l2_vs_fp32_median": float(f"{np.median(rel):.3e}"),
+                       "probe_rel_l2_vs_fp32_max": float(f"{rel.max():.3e}") if rel.size else None,
+                       "probe_rel_l2_vs_fp32_median": float(f"{np.median(rel):.3e}") if rel.size else None,
+                       "dft_rel_err_vs_fp32_max": float(f"{dft_err:.3e}") if dft_err is not None else None,
+                       "probes_compared": int(live.sum()),
                        "kernel_ms": round(f16["step_ms"], 5), "bytes_per_cell": round(b16, 3),
+                       "traffic": _traffic(sc.name, "f16", f16["info"]),
                        "frac": round(b16 * cells / (max(f16["step_ms"], 1e-9) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc)

@@ -66,3 +66,19 @@ def test_adjoint_vs_central_differences(engine):
         fd = (vals[0] - vals[1]) / (ev[0] - ev[1])
         print(f"[adjoint] cell {q}: adjoint {g[q]:.6e} fd {fd:.6e}", flush=True)
         assert abs(g[q] - fd) <= 2e-3 * np.abs(g).max()
+
+
+def test_torch_autograd_binding(engine):
+    """The torch.autograd.Function returns the run's J and, on backward, the
+    engine's adjoint gradient."""
+    import torch
+    from fdtd_amd.autograd import design_objective
+    sc = small_design(steps=120)
+    sim = engine.Simulation.from_scene(sc, "f32")
+    eps = torch.tensor(sc.design_eps, dtype=torch.float64, requires_grad=True)
+    J = design_objective(eps, sim, TERMS, sc.steps, 32)
+    J.backward()
+    sim2 = engine.Simulation.from_scene(sc, "f32")
+    J2, g2 = sim2.adjoint_gradient(TERMS, sc.steps, 32)
+    assert abs(float(J) - J2) <= 1e-6 * J2
+    np.testing.assert_array_equal(eps.grad.numpy(), g2)
