@@ -27,9 +27,11 @@ hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a, c
                       int sten, hipStream_t s);
 hipError_t launch_snap(int storage, const KArgs &a, float *hist, int slot, hipStream_t s);
 hipError_t launch_step(int storage, int mode, const KArgs &a, int m, hipStream_t s);
+hipError_t launch_pair_half_step(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s);
 void step_tiles(int nx, int ny, int &tiles_x, int &tiles_y);
 int tile_rows();
 int tile_cols();
+int pair_tile_cols();
 }  // namespace fdtd
 
 using namespace fdtd;
@@ -57,6 +59,8 @@ struct fdtd_ctx {
     void *psi_set[2][12] = {};
     int cur = 0;
     bool fused = true;              // one kernel per step (else E and H half-step kernels)
+    bool pair = false;              // 16-bit storage: two cells per thread in the half-step kernels
+    int pair_tiles_x = 0, pair_kchunk = 0;
     KArgs ka_step{};                // tile geometry of the step kernel
     double step_ms = 0.0;
     size_t psi_elems[12] = {};
@@ -198,6 +202,12 @@ int do_finalize(fdtd_ctx *c)
     a.tiles_x = (in.nx + tile_cols() - 1) / tile_cols();
     a.tiles_y = (in.ny + tile_rows() - 1) / tile_rows();
     a.kchunk = choose_chunks(c, a.tiles_x * a.tiles_y);
+    // 16-bit storage sweeps a cell pair per thread (128-cell tile rows);
+    // FDTD_PAIR=0 selects the one-cell kernels instead (same stored bits)
+    const char *pv = std::getenv("FDTD_PAIR");
+    c->pair = in.storage != FDTD_STORE_F32 && !(pv && std::strcmp(pv, "0") == 0);
+    c->pair_tiles_x = (in.nx + pair_tile_cols() - 1) / pair_tile_cols();
+    c->pair_kchunk = choose_chunks(c, c->pair_tiles_x * a.tiles_y);
     a.rec = 1;
     a.snap = nullptr;
 
@@ -360,15 +370,16 @@ hipError_t do_step(fdtd_ctx *c, const KArgs &base, int m, hipEvent_t ev0 = nullp
         return e;
     }
     KArgs a = base;
-    a.tiles_x = c->ka.tiles_x;
+    a.tiles_x = c->pair ? c->pair_tiles_x : c->ka.tiles_x;
     a.tiles_y = c->ka.tiles_y;
-    a.kchunk = c->ka.kchunk;
+    a.kchunk = c->pair ? c->pair_kchunk : c->ka.kchunk;
     bind_sets(c, a);
+    auto half = c->pair ? launch_pair_half_step : launch_half_step;
     if (ev0) (void)hipEventRecord(ev0, c->stream);
-    if ((e = launch_half_step(true, c->in.storage, c->e_mode, a, m, c->stream)) != hipSuccess) return e;
+    if ((e = half(true, c->in.storage, c->e_mode, a, m, c->stream)) != hipSuccess) return e;
     if (ev1) (void)hipEventRecord(ev1, c->stream);
     if (ev2) (void)hipEventRecord(ev2, c->stream);
-    e = launch_half_step(false, c->in.storage, c->h_mode, a, m, c->stream);
+    e = half(false, c->in.storage, c->h_mode, a, m, c->stream);
     if (ev3) (void)hipEventRecord(ev3, c->stream);
     return e;
 }

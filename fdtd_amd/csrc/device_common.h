@@ -42,7 +42,8 @@ __device__ __forceinline__ float rnd(float v, const bf16_t *)
 }
 
 // per-component scale from a kernel-argument array without dynamic indexing
-__device__ __forceinline__ float pick6(const float (&v)[6], int c)
+template <class V>
+__device__ __forceinline__ float pick6(const V &v, int c)
 {
     float r = v[0];
     r = c == 1 ? v[1] : r;

@@ -37,6 +37,7 @@
 
 #include "device_common.h"
 #include "internal.h"
+#include "sweep_common.h"
 
 // Occupancy floors of the two sweep kernels (waves per SIMD).
 #ifdef FDTD_E_WAVES
@@ -51,90 +52,6 @@
 #endif
 
 namespace fdtd {
-
-// Non-temporal access for the streams only the owning thread touches (the
-// swept field's own values): they are read once and written once per sweep.
-__device__ __forceinline__ float ldn(const float *p) { return __builtin_nontemporal_load(p); }
-__device__ __forceinline__ void stn(float *p, float v) { __builtin_nontemporal_store(v, p); }
-__device__ __forceinline__ float ldn(const _Float16 *p) { return (float)__builtin_nontemporal_load(p); }
-__device__ __forceinline__ void stn(_Float16 *p, float v) { __builtin_nontemporal_store((_Float16)v, p); }
-__device__ __forceinline__ float ldn(const bf16_t *p)
-{
-    const std::uint16_t u = __builtin_nontemporal_load((const std::uint16_t *)p);
-    return __uint_as_float((std::uint32_t)u << 16);
-}
-__device__ __forceinline__ void stn(bf16_t *p, float v)
-{
-    std::uint32_t u = __float_as_uint(v);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    __builtin_nontemporal_store((std::uint16_t)(u >> 16), (std::uint16_t *)p);
-}
-
-// Byte-offset addressing.  Every array a sweep touches is smaller than 4 GiB
-// (checked when the scene is finalized), so a 32-bit byte offset in one
-// vector register, added to a scalar base, addresses every stream of a
-// plane; field offsets are taken from the start of the allocation (the guard
-// plane below k = 0) so that they are never negative.
-template <class T>
-__device__ __forceinline__ const T *at(const T *base, std::uint32_t off)
-{
-    return (const T *)((const char *)base + off);
-}
-template <class T>
-__device__ __forceinline__ T *at(T *base, std::uint32_t off)
-{
-    return (T *)((char *)base + off);
-}
-
-// Read-only, workgroup-uniform table entry through the scalar data cache.
-typedef __attribute__((address_space(4))) const float const_float;
-__device__ __forceinline__ float4 uload(const float4 *p)
-{
-    const const_float *q = (const const_float *)p;
-    return make_float4(q[0], q[1], q[2], q[3]);
-}
-
-// Coefficient-table entries of one cell: (c, cb_a, cb_b, -) per component.
-struct Coef {
-    float4 x, y, z;
-};
-
-template <int CM>
-__device__ __forceinline__ Coef coef_of(const float4 *tab, int cl)
-{
-    Coef t;
-    t.x = tab[cl * 3 + 0];
-    t.y = tab[cl * 3 + 1];
-    t.z = tab[cl * 3 + 2];
-    return t;
-}
-
-template <int CM>
-__device__ __forceinline__ int class_at(const void *cls, std::uint32_t cell)
-{
-    if (CM == CLS_U16) return *at((const std::uint16_t *)cls, cell * 2u);
-    if (CM == CLS_U32) return (int)*at((const std::uint32_t *)cls, cell * 4u);
-    return 0;
-}
-
-__device__ __forceinline__ int first_k(const KArgs &a, int ev, int col)
-{
-    return (ev >= 0 && a.events[ev].col == col) ? a.events[ev].k : INT_MAX;
-}
-
-__device__ __forceinline__ void record_probe(const KArgs &a, int p, int comp, float stored, int m)
-{
-    const float v = stored * pick6(a.probe_unscale, comp);
-    a.probe_hist[(i64)p * a.steps_cap + m] = v;
-    const double dv = (double)v;
-    for (int f = 0; f < a.nfreq; ++f) {
-        const double2 ph = a.phasor[(i64)m * a.nfreq + f];
-        double2 acc = a.dft[(i64)p * a.nfreq + f];
-        acc.x = acc.x + dv * ph.x;
-        acc.y = acc.y + dv * ph.y;
-        a.dft[(i64)p * a.nfreq + f] = acc;
-    }
-}
 
 // ---- E half-step ----------------------------------------------------------
 
@@ -164,6 +81,7 @@ template <class T, int CM>
 __device__ __forceinline__ void e_fetch(EPlane &p, const KArgs &a, const ECol &cc, const T *ex, const T *ey,
                                         const T *ez, const T *hx, const T *hy, const T *hz, int k, int k1)
 {
+    ConstKArgs &ra = rare_args(a);
     constexpr std::uint32_t S = sizeof(T);
     const std::uint32_t o = cc.o0 + (std::uint32_t)k * cc.ob;
     const std::uint32_t ojm = o - (std::uint32_t)a.px * S;
@@ -185,20 +103,20 @@ __device__ __forceinline__ void e_fetch(EPlane &p, const KArgs &a, const ECol &c
     const bool khi = k < a.nz - 1;
     p.p_yx = p.p_zx = p.p_xy = p.p_zy = p.p_xz = p.p_yz = 0.f;
     if (cc.xs && cc.jhi && khi) {
-        const std::uint32_t q = (((std::uint32_t)k * ny + cc.j) * a.Lx + cc.sx) * S;
-        p.p_yx = ld(at((const T *)a.psi_eyx, q));
-        p.p_zx = ld(at((const T *)a.psi_ezx, q));
+        const std::uint32_t q = (((std::uint32_t)k * ny + cc.j) * ra.Lx + cc.sx) * S;
+        p.p_yx = ld(at((const T *)ra.psi_eyx, q));
+        p.p_zx = ld(at((const T *)ra.psi_ezx, q));
     }
     if (cc.ys && cc.ihi && khi) {
-        const std::uint32_t q = (((std::uint32_t)k * a.Ly + cc.sy) * px + cc.i) * S;
-        p.p_xy = ld(at((const T *)a.psi_exy, q));
-        p.p_zy = ld(at((const T *)a.psi_ezy, q));
+        const std::uint32_t q = (((std::uint32_t)k * ra.Ly + cc.sy) * px + cc.i) * S;
+        p.p_xy = ld(at((const T *)ra.psi_exy, q));
+        p.p_zy = ld(at((const T *)ra.psi_ezy, q));
     }
     if (e_zslab(a, k) && cc.ihi && cc.jhi) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
         const std::uint32_t q = (((std::uint32_t)sz * ny + cc.j) * px + cc.i) * S;
-        p.p_xz = ld(at((const T *)a.psi_exz, q));
-        p.p_yz = ld(at((const T *)a.psi_eyz, q));
+        p.p_xz = ld(at((const T *)ra.psi_exz, q));
+        p.p_yz = ld(at((const T *)ra.psi_eyz, q));
     }
 }
 
@@ -207,6 +125,7 @@ __device__ __forceinline__ void e_update(const EPlane &p, const Coef &t, const K
                                          T *ey, T *ez, int k, int m, float &hx_km, float &hy_km, int &ev,
                                          int &ek)
 {
+    ConstKArgs &ra = rare_args(a);
     const int nz = a.nz;
     constexpr std::uint32_t S = sizeof(T);
     const std::uint32_t o = cc.o0 + (std::uint32_t)k * cc.ob;
@@ -240,65 +159,48 @@ __device__ __forceinline__ void e_update(const EPlane &p, const Coef &t, const K
     if (upd_z) ezn = (cz.x * ez0 + cbz_hy * (hyc - hy_im)) + cbz_hx * (hxc - hx_jm);
 
     if (k == ek) {   // source edges and probes on this cell
-        while (ev >= 0 && a.events[ev].k == k && a.events[ev].col == cc.mycol) {
-            const Event e = a.events[ev];
-            if (e.kind == EV_SOURCE) {
-                if (upd_z) {
-                    const SrcEdge s = a.srcs[e.id];
-                    ezn = ((s.ceze * ez0 + s.cezhy * (hyc - hy_im)) + s.cezhx * (hxc - hx_jm)) +
-                          s.cezj * a.vpn[(i64)s.wave * a.steps_cap + m];
-                }
-            } else if (a.rec) {
-                const int comp = e.kind - EV_PROBE;
-                record_probe(a, e.id, comp, comp == 0 ? exn : comp == 1 ? eyn : ezn, m);
-            }
-            ++ev;
-        }
-        ek = first_k(a, ev, cc.mycol);
+        const EvE r = e_events(ra, ev, k, cc.mycol, m, upd_z, ez0, hyc, hy_im, hxc, hx_jm, exn, eyn, ezn);
+        ev = r.ev;
+        ek = r.ek;
+        ezn = r.ezn;
     }
 
     // CPML psi recursions and corrections, fdtdrav.h:750-911
-    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    const float dx = ra.d[0], dy = ra.d[1], dz = ra.d[2];
     if (cc.xs && cc.jhi && khi) {
-        const std::uint32_t q = (((std::uint32_t)k * a.ny + cc.j) * a.Lx + cc.sx) * S;
+        const std::uint32_t q = (((std::uint32_t)k * ra.ny + cc.j) * ra.Lx + cc.sx) * S;
         const float pey = cc.pxe.x * p.p_yx + cc.pxe.y * (hzc - hz_im);
         const float pez = cc.pxe.x * p.p_zx + cc.pxe.y * (hyc - hy_im);
-        st(at((T *)a.psi_eyx, q), pey);
-        st(at((T *)a.psi_ezx, q), pez);
+        st(at((T *)ra.psi_eyx, q), pey);
+        st(at((T *)ra.psi_ezx, q), pez);
         if (klo) eyn = eyn + (cy.z * dx) * pey;
         if (cc.jlo) ezn = ezn + (cz.y * dx) * pez;
     }
     if (cc.ys && cc.ihi && khi) {
-        const std::uint32_t q = (((std::uint32_t)k * a.Ly + cc.sy) * a.px + cc.i) * S;
+        const std::uint32_t q = (((std::uint32_t)k * ra.Ly + cc.sy) * ra.px + cc.i) * S;
         const float pex = cc.pye.x * p.p_xy + cc.pye.y * (hzc - hz_jm);
         const float pez = cc.pye.x * p.p_zy + cc.pye.y * (hxc - hx_jm);
-        st(at((T *)a.psi_exy, q), pex);
-        st(at((T *)a.psi_ezy, q), pez);
+        st(at((T *)ra.psi_exy, q), pex);
+        st(at((T *)ra.psi_ezy, q), pez);
         if (klo) exn = exn + (cx.y * dy) * pex;
         if (cc.ilo) ezn = ezn + (cz.z * dy) * pez;
     }
     if (zs && cc.ihi && cc.jhi) {
         const int sz = k < a.L[4] ? k : k - (nz - a.L[5]) + a.L[4];
-        const std::uint32_t q = (((std::uint32_t)sz * a.ny + cc.j) * a.px + cc.i) * S;
+        const std::uint32_t q = (((std::uint32_t)sz * ra.ny + cc.j) * ra.px + cc.i) * S;
         const float pex = pze.x * p.p_xz + pze.y * (hyc - hy_km);
         const float pey = pze.x * p.p_yz + pze.y * (hxc - hx_km);
-        st(at((T *)a.psi_exz, q), pex);
-        st(at((T *)a.psi_eyz, q), pey);
+        st(at((T *)ra.psi_exz, q), pex);
+        st(at((T *)ra.psi_eyz, q), pey);
         if (cc.jlo) exn = exn + (cx.z * dz) * pex;
         if (cc.ilo) eyn = eyn + (cy.y * dz) * pey;
     }
     if (upd_x) stn(at(ex, o), exn);
     if (upd_y) stn(at(ey, o), eyn);
     if (upd_z) stn(at(ez, o), ezn);
-    if (cc.in_snap && k >= a.sb[2] && k < a.sb[2] + a.sb[5]) {
-        // stored-precision value, as the next step reads it, in SI units
-        const i64 nb = (i64)a.sb[3] * a.sb[4] * a.sb[5];
-        const i64 q = (cc.i - a.sb[0]) + (i64)a.sb[3] * ((cc.j - a.sb[1]) + (i64)a.sb[4] * (k - a.sb[2]));
-        float *o = a.snap + (i64)(m + 1 - a.snap_base) * 3 * nb + q;
-        o[0] = (upd_x ? rnd(exn, ex) : ex0) * a.probe_unscale[0];
-        o[nb] = (upd_y ? rnd(eyn, ex) : ey0) * a.probe_unscale[1];
-        o[2 * nb] = (upd_z ? rnd(ezn, ex) : ez0) * a.probe_unscale[2];
-    }
+    if (cc.in_snap && k >= a.sb[2] && k < a.sb[2] + a.sb[5])   // stored-precision value, as the next step reads it
+        e_snapshot(ra, cc.i, cc.j, k, m, upd_x ? rnd(exn, ex) : ex0, upd_y ? rnd(eyn, ex) : ey0,
+                   upd_z ? rnd(ezn, ex) : ez0);
     hx_km = hxc;
     hy_km = hyc;
 }
@@ -408,6 +310,7 @@ template <class T, int CM>
 __device__ __forceinline__ void h_fetch(HPlane &p, const KArgs &a, const HCol &cc, const T *ex, const T *ey,
                                         const T *ez, const T *hx, const T *hy, const T *hz, int k, int k1)
 {
+    ConstKArgs &ra = rare_args(a);
     constexpr std::uint32_t S = sizeof(T);
     const std::uint32_t o = cc.o0 + (std::uint32_t)k * cc.ob;
     const std::uint32_t okp = o + cc.ob;
@@ -428,20 +331,20 @@ __device__ __forceinline__ void h_fetch(HPlane &p, const KArgs &a, const HCol &c
                 : 0;
     p.p_yx = p.p_zx = p.p_xy = p.p_zy = p.p_xz = p.p_yz = 0.f;
     if (cc.xs) {
-        const std::uint32_t q = (((std::uint32_t)k * ny + cc.j) * a.Lx + cc.sx) * S;
-        p.p_yx = ld(at((const T *)a.psi_hyx, q));
-        p.p_zx = ld(at((const T *)a.psi_hzx, q));
+        const std::uint32_t q = (((std::uint32_t)k * ny + cc.j) * ra.Lx + cc.sx) * S;
+        p.p_yx = ld(at((const T *)ra.psi_hyx, q));
+        p.p_zx = ld(at((const T *)ra.psi_hzx, q));
     }
     if (cc.ys) {
-        const std::uint32_t q = (((std::uint32_t)k * a.Ly + cc.sy) * px + cc.i) * S;
-        p.p_xy = ld(at((const T *)a.psi_hxy, q));
-        p.p_zy = ld(at((const T *)a.psi_hzy, q));
+        const std::uint32_t q = (((std::uint32_t)k * ra.Ly + cc.sy) * px + cc.i) * S;
+        p.p_xy = ld(at((const T *)ra.psi_hxy, q));
+        p.p_zy = ld(at((const T *)ra.psi_hzy, q));
     }
     if (h_zslab(a, k)) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
         const std::uint32_t q = (((std::uint32_t)sz * ny + cc.j) * px + cc.i) * S;
-        p.p_xz = ld(at((const T *)a.psi_hxz, q));
-        p.p_yz = ld(at((const T *)a.psi_hyz, q));
+        p.p_xz = ld(at((const T *)ra.psi_hxz, q));
+        p.p_yz = ld(at((const T *)ra.psi_hyz, q));
     }
 }
 
@@ -449,6 +352,7 @@ template <class T, int CM>
 __device__ __forceinline__ void h_update(const HPlane &p, const Coef &t, const KArgs &a, const HCol &cc, T *hx,
                                          T *hy, T *hz, int k, int m, float &ex_c, float &ey_c, int &ev, int &ek)
 {
+    ConstKArgs &ra = rare_args(a);
     const int nz = a.nz;
     constexpr std::uint32_t S = sizeof(T);
     const std::uint32_t o = cc.o0 + (std::uint32_t)k * cc.ob;
@@ -482,41 +386,37 @@ __device__ __forceinline__ void h_update(const HPlane &p, const Coef &t, const K
     if (upd_z) hzn = (cz.x * hz0 + chz_ex * (ex_jp - ex_c)) + chz_ey * (ey_ip - ey_c);
 
     if (k == ek) {   // probes on this cell
-        while (ev >= 0 && a.events[ev].k == k && a.events[ev].col == cc.mycol) {
-            const Event e = a.events[ev];
-            const int comp = e.kind - EV_PROBE;
-            if (a.rec) record_probe(a, e.id, comp, comp == 3 ? hxn : comp == 4 ? hyn : hzn, m);
-            ++ev;
-        }
-        ek = first_k(a, ev, cc.mycol);
+        const EvH r = h_events(ra, ev, k, cc.mycol, m, hxn, hyn, hzn);
+        ev = r.ev;
+        ek = r.ek;
     }
 
-    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    const float dx = ra.d[0], dy = ra.d[1], dz = ra.d[2];
     if (cc.xs) {
-        const std::uint32_t q = (((std::uint32_t)k * a.ny + cc.j) * a.Lx + cc.sx) * S;
+        const std::uint32_t q = (((std::uint32_t)k * ra.ny + cc.j) * ra.Lx + cc.sx) * S;
         const float phy = cc.pxm.x * p.p_yx + cc.pxm.y * (ez_ip - ezc);
         const float phz = cc.pxm.x * p.p_zx + cc.pxm.y * (ey_ip - ey_c);
-        st(at((T *)a.psi_hyx, q), phy);
-        st(at((T *)a.psi_hzx, q), phz);
+        st(at((T *)ra.psi_hyx, q), phy);
+        st(at((T *)ra.psi_hzx, q), phz);
         if (khi) hyn = hyn + (cy.y * dx) * phy;
         if (cc.jhi) hzn = hzn + (cz.z * dx) * phz;
     }
     if (cc.ys) {
-        const std::uint32_t q = (((std::uint32_t)k * a.Ly + cc.sy) * a.px + cc.i) * S;
+        const std::uint32_t q = (((std::uint32_t)k * ra.Ly + cc.sy) * ra.px + cc.i) * S;
         const float phx = cc.pym.x * p.p_xy + cc.pym.y * (ez_jp - ezc);
         const float phz = cc.pym.x * p.p_zy + cc.pym.y * (ex_jp - ex_c);
-        st(at((T *)a.psi_hxy, q), phx);
-        st(at((T *)a.psi_hzy, q), phz);
+        st(at((T *)ra.psi_hxy, q), phx);
+        st(at((T *)ra.psi_hzy, q), phz);
         if (khi) hxn = hxn + (cx.z * dy) * phx;
         if (cc.ihi) hzn = hzn + (cz.y * dy) * phz;
     }
     if (zs) {
         const int sz = k < a.L[4] ? k : k - (nz - a.L[5]) + a.L[4];
-        const std::uint32_t q = (((std::uint32_t)sz * a.ny + cc.j) * a.px + cc.i) * S;
+        const std::uint32_t q = (((std::uint32_t)sz * ra.ny + cc.j) * ra.px + cc.i) * S;
         const float phx = pzm.x * p.p_xz + pzm.y * (ey_kp - ey_c);
         const float phy = pzm.x * p.p_yz + pzm.y * (ex_kp - ex_c);
-        st(at((T *)a.psi_hxz, q), phx);
-        st(at((T *)a.psi_hyz, q), phy);
+        st(at((T *)ra.psi_hxz, q), phx);
+        st(at((T *)ra.psi_hyz, q), phy);
         if (cc.jhi) hxn = hxn + (cx.y * dz) * phx;
         if (cc.ihi) hyn = hyn + (cy.z * dz) * phy;
     }

@@ -1,0 +1,637 @@
+// kernels_pair.hip -- the E and H half-step kernels for 16-bit storage
+// (FP16 / BF16), two adjacent x cells per thread.
+//
+// With 2-byte values a one-cell-per-thread sweep moves half the bytes per
+// load instruction of the FP32 sweep and runs out of loads in flight long
+// before HBM is busy.  Here a thread owns the cell pair (i, i+1), i even, of
+// a 128 x BY tile: every field stream of a plane is one 4-byte load per lane
+// holding both cells (the same bytes per instruction as the FP32 sweep), the
+// pair stays packed in one register until it is used, and the two
+// x-neighbour values that fall outside the pair (i-1 for the E sweep, i+2 for
+// the H sweep) are single 2-byte loads.  The z-march, the two-deep software
+// pipeline, the class-table prefetch, the XCD tile mapping and the per-cell
+// arithmetic are those of kernels.hip; only the packing differs, so the
+// stored bits equal the one-cell kernels' bit for bit.
+//   E: calc_e  fdtdrav.h:665-735, CPML fdtdrav.h:750-911
+//   H: calc_h  fdtdrav.h:328-386, CPML fdtdrav.h:398-562
+#include <hip/hip_runtime.h>
+#include <climits>
+#include <cstdint>
+
+#include "device_common.h"
+#include "internal.h"
+#include "sweep_common.h"
+
+namespace fdtd {
+
+constexpr int TXP = 2 * TX;   // cells per tile row
+
+typedef std::uint32_t u32;
+
+// ---- packed pairs: cell i in the low half, cell i+1 in the high half -------
+__device__ __forceinline__ float lo_f(u32 u, const _Float16 *)
+{
+    return (float)__builtin_bit_cast(_Float16, (std::uint16_t)(u & 0xFFFFu));
+}
+__device__ __forceinline__ float hi_f(u32 u, const _Float16 *)
+{
+    return (float)__builtin_bit_cast(_Float16, (std::uint16_t)(u >> 16));
+}
+__device__ __forceinline__ float lo_f(u32 u, const bf16_t *) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float hi_f(u32 u, const bf16_t *) { return __uint_as_float(u & 0xFFFF0000u); }
+
+// storage bits of v (round to nearest even, as the one-cell kernels store)
+__device__ __forceinline__ u32 bits16(float v, const _Float16 *)
+{
+    return (u32)__builtin_bit_cast(std::uint16_t, (_Float16)v);
+}
+__device__ __forceinline__ u32 bits16(float v, const bf16_t *)
+{
+    u32 u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+
+template <class T>
+__device__ __forceinline__ u32 ld2(const T *base, u32 off)
+{
+    return *(const u32 *)((const char *)base + off);
+}
+template <class T>
+__device__ __forceinline__ u32 ldn2(const T *base, u32 off)
+{
+    return __builtin_nontemporal_load((const u32 *)((const char *)base + off));
+}
+template <class T>
+__device__ __forceinline__ void stn2(T *base, u32 off, u32 v)
+{
+    __builtin_nontemporal_store(v, (u32 *)((char *)base + off));
+}
+
+// new value of a pair: updated halves replaced, the others keep their bits
+template <class T>
+__device__ __forceinline__ u32 merge2(u32 old, bool ua, float va, bool ub, float vb)
+{
+    const u32 lo = ua ? bits16(va, (const T *)nullptr) : (old & 0xFFFFu);
+    const u32 hi = ub ? bits16(vb, (const T *)nullptr) : (old >> 16);
+    return lo | (hi << 16);
+}
+
+// class indices of the pair at cell index `cell` (even)
+template <int CM>
+__device__ __forceinline__ void class_pair(const void *cls, u32 cell, int &ca, int &cb)
+{
+    if (CM == CLS_U16) {
+        const u32 u = *at((const u32 *)cls, cell * 2u);
+        ca = (int)(u & 0xFFFFu);
+        cb = (int)(u >> 16);
+    } else if (CM == CLS_U32) {
+        const uint2 u = *(const uint2 *)((const char *)cls + cell * 4u);
+        ca = (int)u.x;
+        cb = (int)u.y;
+    } else {
+        ca = cb = 0;
+    }
+}
+
+struct CoefPair {
+    Coef a, b;
+};
+
+template <int CM>
+__device__ __forceinline__ CoefPair coef_pair(const float4 *tab, int ca, int cb)
+{
+    CoefPair t;
+    t.a = coef_of<CM>(tab, ca);
+    t.b = coef_of<CM>(tab, cb);
+    return t;
+}
+
+// Per-cell column constants (cell 0 = i, cell 1 = i+1).
+struct PCell {
+    int i, sx, mycol;
+    bool ilo, ihi, xs, valid, in_snap;
+    float4 px;   // CPML x profile of this cell (E or H nodes)
+};
+
+// Row constants of a thread.
+struct PRow {
+    int j, sy;
+    i64 col;      // j * px + i (cell 0)
+    u32 o0, ob;   // byte offset of cell 0 at k = 0 from the guard-plane base; bytes per plane
+    bool jlo, jhi, ys;
+    float4 py;
+};
+
+// ============================ E sweep ======================================
+
+struct EPair {
+    u32 hx, hy, hz, hx_jm, hz_jm, ex, ey, ez;   // packed pairs
+    float hy_im, hz_im;                          // H at i-1 (x neighbour of cell 0)
+    float p_yx[2], p_zx[2], p_xy[2], p_zy[2], p_xz[2], p_yz[2];
+    int cla, clb;                                // class indices of plane k+1
+};
+
+template <class T, int CM>
+__device__ __forceinline__ void ep_fetch(EPair &p, const KArgs &a, const PRow &r, const PCell (&c)[2], const T *ex,
+                                         const T *ey, const T *ez, const T *hx, const T *hy, const T *hz, int k,
+                                         int k1)
+{
+    ConstKArgs &ra = rare_args(a);
+    constexpr u32 S = sizeof(T);
+    const u32 o = r.o0 + (u32)k * r.ob;
+    const u32 ojm = o - (u32)a.px * S;
+    p.hx = ld2(hx, o);
+    p.hy = ld2(hy, o);
+    p.hz = ld2(hz, o);
+    p.hx_jm = ld2(hx, ojm);
+    p.hz_jm = ld2(hz, ojm);
+    p.hy_im = ld(at(hy, o - S));
+    p.hz_im = ld(at(hz, o - S));
+    p.ex = ldn2(ex, o);
+    p.ey = ldn2(ey, o);
+    p.ez = ldn2(ez, o);
+    p.cla = p.clb = 0;
+    if (CM != CLS_UNIFORM && k + 1 < k1)
+        class_pair<CM>(a.cls, (u32)(k + 1) * (u32)a.plane + (u32)r.col, p.cla, p.clb);
+    const bool khi = k < a.nz - 1;
+    for (int q = 0; q < 2; ++q) {
+        p.p_yx[q] = p.p_zx[q] = p.p_xy[q] = p.p_zy[q] = p.p_xz[q] = p.p_yz[q] = 0.f;
+        if (c[q].xs && r.jhi && khi) {
+            const u32 qq = (((u32)k * ra.ny + r.j) * ra.Lx + c[q].sx) * S;
+            p.p_yx[q] = ld(at((const T *)ra.psi_eyx, qq));
+            p.p_zx[q] = ld(at((const T *)ra.psi_ezx, qq));
+        }
+    }
+    if (r.ys && khi) {
+        const u32 qq = (((u32)k * ra.Ly + r.sy) * ra.px + c[0].i) * S;
+        const u32 u = ld2((const T *)ra.psi_exy, qq), v = ld2((const T *)ra.psi_ezy, qq);
+        p.p_xy[0] = lo_f(u, ex);
+        p.p_xy[1] = hi_f(u, ex);
+        p.p_zy[0] = lo_f(v, ex);
+        p.p_zy[1] = hi_f(v, ex);
+    }
+    if (((k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && khi)) && r.jhi) {
+        const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
+        const u32 qq = (((u32)sz * ra.ny + r.j) * ra.px + c[0].i) * S;
+        const u32 u = ld2((const T *)ra.psi_exz, qq), v = ld2((const T *)ra.psi_eyz, qq);
+        p.p_xz[0] = lo_f(u, ex);
+        p.p_xz[1] = hi_f(u, ex);
+        p.p_yz[0] = lo_f(v, ex);
+        p.p_yz[1] = hi_f(v, ex);
+    }
+}
+
+// E update of one cell of the pair; the arithmetic of kernels.hip e_update.
+template <class T>
+__device__ __forceinline__ void e_cell(const KArgs &a, const PRow &r, const PCell &c, int q, const EPair &p,
+                                       const Coef &t, int k, int m, float hxc, float hyc, float hzc, float hy_im,
+                                       float hz_im, float hx_jm, float hz_jm, float ex0, float ey0, float ez0,
+                                       float hx_km, float hy_km, int &ev, int &ek, bool zs, const float4 &pze,
+                                       float &exn, float &eyn, float &ezn, bool &ux, bool &uy, bool &uz)
+{
+    ConstKArgs &ra = rare_args(a);
+    const int nz = a.nz;
+    const float4 cx = t.x, cy = t.y, cz = t.z;
+    const bool klo = k > 0, khi = k < nz - 1;
+    ux = c.valid && c.ihi && r.jlo && r.jhi && klo && khi;
+    uy = c.valid && c.ilo && c.ihi && r.jhi && klo && khi;
+    uz = c.valid && c.ilo && c.ihi && r.jlo && r.jhi && khi;
+
+    float cbx_hz = cx.y, cbx_hy = cx.z;
+    float cby_hx = cy.y, cby_hz = cy.z;
+    float cbz_hy = cz.y, cbz_hx = cz.z;
+    if (c.xs) { cby_hz = cby_hz / c.px.z; cbz_hy = cbz_hy / c.px.z; }
+    if (r.ys) { cbx_hz = cbx_hz / r.py.z; cbz_hx = cbz_hx / r.py.z; }
+    if (zs) { cbx_hy = cbx_hy / pze.z; cby_hx = cby_hx / pze.z; }
+
+    exn = ex0;
+    eyn = ey0;
+    ezn = ez0;
+    if (ux) exn = (cx.x * ex0 + cbx_hz * (hzc - hz_jm)) + cbx_hy * (hyc - hy_km);
+    if (uy) eyn = (cy.x * ey0 + cby_hx * (hxc - hx_km)) + cby_hz * (hzc - hz_im);
+    if (uz) ezn = (cz.x * ez0 + cbz_hy * (hyc - hy_im)) + cbz_hx * (hxc - hx_jm);
+
+    if (k == ek) {   // source edges and probes on this cell
+        const EvE rr = e_events(ra, ev, k, c.mycol, m, uz, ez0, hyc, hy_im, hxc, hx_jm, exn, eyn, ezn);
+        ev = rr.ev;
+        ek = rr.ek;
+        ezn = rr.ezn;
+    }
+
+    constexpr u32 S = sizeof(T);
+    const float dx = ra.d[0], dy = ra.d[1], dz = ra.d[2];
+    if (c.xs && r.jhi && khi) {
+        const u32 qq = (((u32)k * ra.ny + r.j) * ra.Lx + c.sx) * S;
+        const float pey = c.px.x * p.p_yx[q] + c.px.y * (hzc - hz_im);
+        const float pez = c.px.x * p.p_zx[q] + c.px.y * (hyc - hy_im);
+        st(at((T *)ra.psi_eyx, qq), pey);
+        st(at((T *)ra.psi_ezx, qq), pez);
+        if (klo) eyn = eyn + (cy.z * dx) * pey;
+        if (r.jlo) ezn = ezn + (cz.y * dx) * pez;
+    }
+    if (r.ys && c.valid && c.ihi && khi) {
+        const u32 qq = (((u32)k * ra.Ly + r.sy) * ra.px + c.i) * S;
+        const float pex = r.py.x * p.p_xy[q] + r.py.y * (hzc - hz_jm);
+        const float pez = r.py.x * p.p_zy[q] + r.py.y * (hxc - hx_jm);
+        st(at((T *)ra.psi_exy, qq), pex);
+        st(at((T *)ra.psi_ezy, qq), pez);
+        if (klo) exn = exn + (cx.y * dy) * pex;
+        if (c.ilo) ezn = ezn + (cz.z * dy) * pez;
+    }
+    if (zs && c.valid && c.ihi && r.jhi) {
+        const int sz = k < a.L[4] ? k : k - (nz - a.L[5]) + a.L[4];
+        const u32 qq = (((u32)sz * ra.ny + r.j) * ra.px + c.i) * S;
+        const float pex = pze.x * p.p_xz[q] + pze.y * (hyc - hy_km);
+        const float pey = pze.x * p.p_yz[q] + pze.y * (hxc - hx_km);
+        st(at((T *)ra.psi_exz, qq), pex);
+        st(at((T *)ra.psi_eyz, qq), pey);
+        if (r.jlo) exn = exn + (cx.z * dz) * pex;
+        if (c.ilo) eyn = eyn + (cy.y * dz) * pey;
+    }
+    if (c.in_snap && k >= a.sb[2] && k < a.sb[2] + a.sb[5]) {
+        const T *tp = nullptr;
+        e_snapshot(ra, c.i, r.j, k, m, ux ? rnd(exn, tp) : ex0, uy ? rnd(eyn, tp) : ey0, uz ? rnd(ezn, tp) : ez0);
+    }
+}
+
+template <class T, int CM>
+__device__ __forceinline__ void ep_update(const EPair &p, const CoefPair &t, const KArgs &a, const PRow &r,
+                                          const PCell (&c)[2], T *ex, T *ey, T *ez, int k, int m, u32 &hx_km,
+                                          u32 &hy_km, int (&ev)[2], int (&ek)[2])
+{
+    const T *tp = nullptr;
+    const u32 o = r.o0 + (u32)k * r.ob;
+    const bool zs = (k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && k < a.nz - 1);   // fdtdrav.h:853,879
+    const float4 pze = zs ? uload(a.pz_prof + k) : make_float4(0.f, 0.f, 1.f, 0.f);
+    const float hx0 = lo_f(p.hx, tp), hx1 = hi_f(p.hx, tp);
+    const float hy0 = lo_f(p.hy, tp), hy1 = hi_f(p.hy, tp);
+    const float hz0 = lo_f(p.hz, tp), hz1 = hi_f(p.hz, tp);
+    float ex_a, ey_a, ez_a, ex_b, ey_b, ez_b;
+    bool uxa, uya, uza, uxb, uyb, uzb;
+    e_cell<T>(a, r, c[0], 0, p, t.a, k, m, hx0, hy0, hz0, p.hy_im, p.hz_im, lo_f(p.hx_jm, tp), lo_f(p.hz_jm, tp),
+              lo_f(p.ex, tp), lo_f(p.ey, tp), lo_f(p.ez, tp), lo_f(hx_km, tp), lo_f(hy_km, tp), ev[0], ek[0], zs,
+              pze, ex_a, ey_a, ez_a, uxa, uya, uza);
+    e_cell<T>(a, r, c[1], 1, p, t.b, k, m, hx1, hy1, hz1, hy0, hz0, hi_f(p.hx_jm, tp), hi_f(p.hz_jm, tp),
+              hi_f(p.ex, tp), hi_f(p.ey, tp), hi_f(p.ez, tp), hi_f(hx_km, tp), hi_f(hy_km, tp), ev[1], ek[1], zs,
+              pze, ex_b, ey_b, ez_b, uxb, uyb, uzb);
+    if (uxa || uxb) stn2(ex, o, merge2<T>(p.ex, uxa, ex_a, uxb, ex_b));
+    if (uya || uyb) stn2(ey, o, merge2<T>(p.ey, uya, ey_a, uyb, ey_b));
+    if (uza || uzb) stn2(ez, o, merge2<T>(p.ez, uza, ez_a, uzb, ez_b));
+    hx_km = p.hx;
+    hy_km = p.hy;
+}
+
+template <class T, int CM>
+__global__ void __launch_bounds__(TX * BY) e_pair_kernel(const KArgs a, const int m)
+{
+    int tx, ty, tz;
+    tile_of(a, tx, ty, tz);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int i0 = tx * TXP + 2 * (int)(threadIdx.x & (TX - 1));
+    PRow r;
+    r.j = ty * BY + (int)(threadIdx.x / TX);
+    if (i0 >= nx || r.j >= ny) return;
+    const int k0 = tz * a.kchunk;
+    const int k1 = min(nz, k0 + a.kchunk);
+    const int j = r.j;
+    r.col = (i64)j * a.px + i0;
+    r.ob = (u32)a.plane * (u32)sizeof(T);
+    r.o0 = (u32)r.col * (u32)sizeof(T) + r.ob;
+    r.jlo = j > 0;
+    r.jhi = j < ny - 1;
+    r.ys = (r.jlo && j < a.L[2]) || (j >= ny - a.L[3] && r.jhi);   // E y-slab, fdtdrav.h:803,827
+    r.sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
+    r.py = r.ys ? a.py_prof[j] : make_float4(0.f, 0.f, 1.f, 0.f);
+
+    PCell c[2];
+    for (int q = 0; q < 2; ++q) {
+        const int i = i0 + q;
+        c[q].i = i;
+        c[q].valid = i < nx;
+        c[q].ilo = i > 0;
+        c[q].ihi = i < nx - 1;
+        c[q].xs = c[q].valid && ((c[q].ilo && i < a.L[0]) || (i >= nx - a.L[1] && c[q].ihi));   // fdtdrav.h:752,777
+        c[q].sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
+        c[q].px = c[q].xs ? a.px_prof[i] : make_float4(0.f, 0.f, 1.f, 0.f);
+        c[q].mycol = j * nx + i;
+        c[q].in_snap = c[q].valid && a.snap != nullptr && i >= a.sb[0] && i < a.sb[0] + a.sb[3] &&
+                       j >= a.sb[1] && j < a.sb[1] + a.sb[4];
+    }
+
+    T *__restrict__ ex = (T *)a.ex - a.plane;
+    T *__restrict__ ey = (T *)a.ey - a.plane;
+    T *__restrict__ ez = (T *)a.ez - a.plane;
+    const T *__restrict__ hx = (const T *)a.hx - a.plane;
+    const T *__restrict__ hy = (const T *)a.hy - a.plane;
+    const T *__restrict__ hz = (const T *)a.hz - a.plane;
+
+    int ev[2], ek[2];
+    for (int q = 0; q < 2; ++q) {
+        ev[q] = c[q].valid ? first_event(a, a.e_first, c[q].mycol, k0) : -1;
+        ek[q] = first_k(a, ev[q], c[q].mycol);
+    }
+
+    u32 hx_km = ld2(hx, r.o0 + (u32)(k0 - 1) * r.ob);
+    u32 hy_km = ld2(hy, r.o0 + (u32)(k0 - 1) * r.ob);
+
+    EPair A, B;
+    CoefPair tA, tB;
+    if (CM == CLS_UNIFORM) {
+        tA = coef_pair<CM>(a.etab, 0, 0);
+        tB = tA;
+    } else {
+        int ca, cb;
+        class_pair<CM>(a.cls, (u32)k0 * (u32)a.plane + (u32)r.col, ca, cb);
+        tA = coef_pair<CM>(a.etab, ca, cb);
+    }
+    ep_fetch<T, CM>(A, a, r, c, ex, ey, ez, hx, hy, hz, k0, k1);
+    int k = k0;
+    for (; k + 2 < k1; k += 2) {
+        ep_fetch<T, CM>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        if (CM != CLS_UNIFORM) tB = coef_pair<CM>(a.etab, A.cla, A.clb);
+        ep_update<T, CM>(A, tA, a, r, c, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
+        ep_fetch<T, CM>(A, a, r, c, ex, ey, ez, hx, hy, hz, k + 2, k1);
+        if (CM != CLS_UNIFORM) tA = coef_pair<CM>(a.etab, B.cla, B.clb);
+        ep_update<T, CM>(B, tB, a, r, c, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
+    }
+    if (k + 1 < k1) {
+        ep_fetch<T, CM>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        if (CM != CLS_UNIFORM) tB = coef_pair<CM>(a.etab, A.cla, A.clb);
+        ep_update<T, CM>(A, tA, a, r, c, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
+        ep_update<T, CM>(B, tB, a, r, c, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
+    } else {
+        ep_update<T, CM>(A, tA, a, r, c, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
+    }
+}
+
+// ============================ H sweep ======================================
+
+struct HPair {
+    u32 ex_kp, ey_kp, ez, ez_jp, ex_jp, hx, hy, hz;   // packed pairs
+    float ez_i2, ey_i2;                                // E at i+2 (x neighbour of cell 1)
+    float p_yx[2], p_zx[2], p_xy[2], p_zy[2], p_xz[2], p_yz[2];
+    int cla, clb;
+};
+
+__device__ __forceinline__ bool hp_zslab(const KArgs &a, int k)
+{
+    return k < a.L[4] || (k >= a.nz - a.L[5] && k < a.nz - 1);   // fdtdrav.h:500,531
+}
+
+template <class T, int CM>
+__device__ __forceinline__ void hp_fetch(HPair &p, const KArgs &a, const PRow &r, const PCell (&c)[2], const T *ex,
+                                         const T *ey, const T *ez, const T *hx, const T *hy, const T *hz, int k,
+                                         int k1)
+{
+    ConstKArgs &ra = rare_args(a);
+    constexpr u32 S = sizeof(T);
+    const u32 o = r.o0 + (u32)k * r.ob;
+    const u32 okp = o + r.ob;
+    const u32 ojp = o + (u32)a.px * S;
+    p.ex_kp = ld2(ex, okp);
+    p.ey_kp = ld2(ey, okp);
+    p.ez = ld2(ez, o);
+    p.ez_jp = ld2(ez, ojp);
+    p.ex_jp = ld2(ex, ojp);
+    p.ez_i2 = ld(at(ez, o + 2 * S));
+    p.ey_i2 = ld(at(ey, o + 2 * S));
+    p.hx = ldn2(hx, o);
+    p.hy = ldn2(hy, o);
+    p.hz = ldn2(hz, o);
+    p.cla = p.clb = 0;
+    if (CM != CLS_UNIFORM && k + 1 < k1)
+        class_pair<CM>(a.cls, (u32)(k + 1) * (u32)a.plane + (u32)r.col, p.cla, p.clb);
+    for (int q = 0; q < 2; ++q) {
+        p.p_yx[q] = p.p_zx[q] = p.p_xy[q] = p.p_zy[q] = p.p_xz[q] = p.p_yz[q] = 0.f;
+        if (c[q].xs) {
+            const u32 qq = (((u32)k * ra.ny + r.j) * ra.Lx + c[q].sx) * S;
+            p.p_yx[q] = ld(at((const T *)ra.psi_hyx, qq));
+            p.p_zx[q] = ld(at((const T *)ra.psi_hzx, qq));
+        }
+    }
+    if (r.ys) {
+        const u32 qq = (((u32)k * ra.Ly + r.sy) * ra.px + c[0].i) * S;
+        const u32 u = ld2((const T *)ra.psi_hxy, qq), v = ld2((const T *)ra.psi_hzy, qq);
+        p.p_xy[0] = lo_f(u, ex);
+        p.p_xy[1] = hi_f(u, ex);
+        p.p_zy[0] = lo_f(v, ex);
+        p.p_zy[1] = hi_f(v, ex);
+    }
+    if (hp_zslab(a, k)) {
+        const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
+        const u32 qq = (((u32)sz * ra.ny + r.j) * ra.px + c[0].i) * S;
+        const u32 u = ld2((const T *)ra.psi_hxz, qq), v = ld2((const T *)ra.psi_hyz, qq);
+        p.p_xz[0] = lo_f(u, ex);
+        p.p_xz[1] = hi_f(u, ex);
+        p.p_yz[0] = lo_f(v, ex);
+        p.p_yz[1] = hi_f(v, ex);
+    }
+}
+
+// H update of one cell of the pair; the arithmetic of kernels.hip h_update.
+template <class T>
+__device__ __forceinline__ void h_cell(const KArgs &a, const PRow &r, const PCell &c, int q, const HPair &p,
+                                       const Coef &t, int k, int m, float ex_kp, float ey_kp, float ezc,
+                                       float ez_ip, float ey_ip, float ez_jp, float ex_jp, float hx0, float hy0,
+                                       float hz0, float ex_c, float ey_c, int &ev, int &ek, bool zs,
+                                       const float4 &pzm, float &hxn, float &hyn, float &hzn, bool &ux, bool &uy,
+                                       bool &uz)
+{
+    ConstKArgs &ra = rare_args(a);
+    const int nz = a.nz;
+    const float4 cx = t.x, cy = t.y, cz = t.z;
+    const bool khi = k < nz - 1;
+    ux = c.valid && r.jhi && khi;
+    uy = c.valid && c.ihi && khi;
+    uz = c.valid && c.ihi && r.jhi;
+
+    float chx_ey = cx.y, chx_ez = cx.z;
+    float chy_ez = cy.y, chy_ex = cy.z;
+    float chz_ex = cz.y, chz_ey = cz.z;
+    if (c.xs) { chy_ez = chy_ez / c.px.z; chz_ey = chz_ey / c.px.z; }
+    if (r.ys) { chx_ez = chx_ez / r.py.z; chz_ex = chz_ex / r.py.z; }
+    if (zs) { chx_ey = chx_ey / pzm.z; chy_ex = chy_ex / pzm.z; }
+
+    hxn = hx0;
+    hyn = hy0;
+    hzn = hz0;
+    if (ux) hxn = (cx.x * hx0 + chx_ey * (ey_kp - ey_c)) + chx_ez * (ez_jp - ezc);
+    if (uy) hyn = (cy.x * hy0 + chy_ez * (ez_ip - ezc)) + chy_ex * (ex_kp - ex_c);
+    if (uz) hzn = (cz.x * hz0 + chz_ex * (ex_jp - ex_c)) + chz_ey * (ey_ip - ey_c);
+
+    if (k == ek) {   // probes on this cell
+        const EvH rr = h_events(ra, ev, k, c.mycol, m, hxn, hyn, hzn);
+        ev = rr.ev;
+        ek = rr.ek;
+    }
+
+    constexpr u32 S = sizeof(T);
+    const float dx = ra.d[0], dy = ra.d[1], dz = ra.d[2];
+    if (c.xs) {
+        const u32 qq = (((u32)k * ra.ny + r.j) * ra.Lx + c.sx) * S;
+        const float phy = c.px.x * p.p_yx[q] + c.px.y * (ez_ip - ezc);
+        const float phz = c.px.x * p.p_zx[q] + c.px.y * (ey_ip - ey_c);
+        st(at((T *)ra.psi_hyx, qq), phy);
+        st(at((T *)ra.psi_hzx, qq), phz);
+        if (khi) hyn = hyn + (cy.y * dx) * phy;
+        if (r.jhi) hzn = hzn + (cz.z * dx) * phz;
+    }
+    if (r.ys && c.valid) {
+        const u32 qq = (((u32)k * ra.Ly + r.sy) * ra.px + c.i) * S;
+        const float phx = r.py.x * p.p_xy[q] + r.py.y * (ez_jp - ezc);
+        const float phz = r.py.x * p.p_zy[q] + r.py.y * (ex_jp - ex_c);
+        st(at((T *)ra.psi_hxy, qq), phx);
+        st(at((T *)ra.psi_hzy, qq), phz);
+        if (khi) hxn = hxn + (cx.z * dy) * phx;
+        if (c.ihi) hzn = hzn + (cz.y * dy) * phz;
+    }
+    if (zs && c.valid) {
+        const int sz = k < a.L[4] ? k : k - (nz - a.L[5]) + a.L[4];
+        const u32 qq = (((u32)sz * ra.ny + r.j) * ra.px + c.i) * S;
+        const float phx = pzm.x * p.p_xz[q] + pzm.y * (ey_kp - ey_c);
+        const float phy = pzm.x * p.p_yz[q] + pzm.y * (ex_kp - ex_c);
+        st(at((T *)ra.psi_hxz, qq), phx);
+        st(at((T *)ra.psi_hyz, qq), phy);
+        if (r.jhi) hxn = hxn + (cx.y * dz) * phx;
+        if (c.ihi) hyn = hyn + (cy.z * dz) * phy;
+    }
+}
+
+template <class T, int CM>
+__device__ __forceinline__ void hp_update(const HPair &p, const CoefPair &t, const KArgs &a, const PRow &r,
+                                          const PCell (&c)[2], T *hx, T *hy, T *hz, int k, int m, u32 &ex_c,
+                                          u32 &ey_c, int (&ev)[2], int (&ek)[2])
+{
+    const T *tp = nullptr;
+    const u32 o = r.o0 + (u32)k * r.ob;
+    const bool zs = hp_zslab(a, k);
+    const float4 pzm = zs ? uload(a.pz_prof + a.nz + k) : make_float4(0.f, 0.f, 1.f, 0.f);
+    const float ez0 = lo_f(p.ez, tp), ez1 = hi_f(p.ez, tp);
+    const float exc0 = lo_f(ex_c, tp), exc1 = hi_f(ex_c, tp);
+    const float eyc0 = lo_f(ey_c, tp), eyc1 = hi_f(ey_c, tp);
+    float hx_a, hy_a, hz_a, hx_b, hy_b, hz_b;
+    bool uxa, uya, uza, uxb, uyb, uzb;
+    h_cell<T>(a, r, c[0], 0, p, t.a, k, m, lo_f(p.ex_kp, tp), lo_f(p.ey_kp, tp), ez0, ez1, eyc1,
+              lo_f(p.ez_jp, tp), lo_f(p.ex_jp, tp), lo_f(p.hx, tp), lo_f(p.hy, tp), lo_f(p.hz, tp), exc0, eyc0,
+              ev[0], ek[0], zs, pzm, hx_a, hy_a, hz_a, uxa, uya, uza);
+    h_cell<T>(a, r, c[1], 1, p, t.b, k, m, hi_f(p.ex_kp, tp), hi_f(p.ey_kp, tp), ez1, p.ez_i2, p.ey_i2,
+              hi_f(p.ez_jp, tp), hi_f(p.ex_jp, tp), hi_f(p.hx, tp), hi_f(p.hy, tp), hi_f(p.hz, tp), exc1, eyc1,
+              ev[1], ek[1], zs, pzm, hx_b, hy_b, hz_b, uxb, uyb, uzb);
+    if (uxa || uxb) stn2(hx, o, merge2<T>(p.hx, uxa, hx_a, uxb, hx_b));
+    if (uya || uyb) stn2(hy, o, merge2<T>(p.hy, uya, hy_a, uyb, hy_b));
+    if (uza || uzb) stn2(hz, o, merge2<T>(p.hz, uza, hz_a, uzb, hz_b));
+    ex_c = p.ex_kp;
+    ey_c = p.ey_kp;
+}
+
+template <class T, int CM>
+__global__ void __launch_bounds__(TX * BY) h_pair_kernel(const KArgs a, const int m)
+{
+    int tx, ty, tz;
+    tile_of(a, tx, ty, tz);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int i0 = tx * TXP + 2 * (int)(threadIdx.x & (TX - 1));
+    PRow r;
+    r.j = ty * BY + (int)(threadIdx.x / TX);
+    if (i0 >= nx || r.j >= ny) return;
+    const int k0 = tz * a.kchunk;
+    const int k1 = min(nz, k0 + a.kchunk);
+    const int j = r.j;
+    r.col = (i64)j * a.px + i0;
+    r.ob = (u32)a.plane * (u32)sizeof(T);
+    r.o0 = (u32)r.col * (u32)sizeof(T) + r.ob;
+    r.jlo = j > 0;
+    r.jhi = j < ny - 1;
+    r.ys = j < a.L[2] || (j >= ny - a.L[3] && r.jhi);   // H y-slab, fdtdrav.h:447,471
+    r.sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
+    r.py = r.ys ? a.py_prof[ny + j] : make_float4(0.f, 0.f, 1.f, 0.f);
+
+    PCell c[2];
+    for (int q = 0; q < 2; ++q) {
+        const int i = i0 + q;
+        c[q].i = i;
+        c[q].valid = i < nx;
+        c[q].ilo = i > 0;
+        c[q].ihi = i < nx - 1;
+        c[q].xs = c[q].valid && (i < a.L[0] || (i >= nx - a.L[1] && c[q].ihi));   // fdtdrav.h:402,424
+        c[q].sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
+        c[q].px = c[q].xs ? a.px_prof[nx + i] : make_float4(0.f, 0.f, 1.f, 0.f);
+        c[q].mycol = j * nx + i;
+        c[q].in_snap = false;
+    }
+
+    const T *__restrict__ ex = (const T *)a.ex - a.plane;
+    const T *__restrict__ ey = (const T *)a.ey - a.plane;
+    const T *__restrict__ ez = (const T *)a.ez - a.plane;
+    T *__restrict__ hx = (T *)a.hx - a.plane;
+    T *__restrict__ hy = (T *)a.hy - a.plane;
+    T *__restrict__ hz = (T *)a.hz - a.plane;
+
+    int ev[2], ek[2];
+    for (int q = 0; q < 2; ++q) {
+        ev[q] = c[q].valid ? first_event(a, a.h_first, c[q].mycol, k0) : -1;
+        ek[q] = first_k(a, ev[q], c[q].mycol);
+    }
+
+    u32 ex_c = ld2(ex, r.o0 + (u32)k0 * r.ob);
+    u32 ey_c = ld2(ey, r.o0 + (u32)k0 * r.ob);
+
+    HPair A, B;
+    CoefPair tA, tB;
+    if (CM == CLS_UNIFORM) {
+        tA = coef_pair<CM>(a.htab, 0, 0);
+        tB = tA;
+    } else {
+        int ca, cb;
+        class_pair<CM>(a.cls, (u32)k0 * (u32)a.plane + (u32)r.col, ca, cb);
+        tA = coef_pair<CM>(a.htab, ca, cb);
+    }
+    hp_fetch<T, CM>(A, a, r, c, ex, ey, ez, hx, hy, hz, k0, k1);
+    int k = k0;
+    for (; k + 2 < k1; k += 2) {
+        hp_fetch<T, CM>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        if (CM != CLS_UNIFORM) tB = coef_pair<CM>(a.htab, A.cla, A.clb);
+        hp_update<T, CM>(A, tA, a, r, c, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
+        hp_fetch<T, CM>(A, a, r, c, ex, ey, ez, hx, hy, hz, k + 2, k1);
+        if (CM != CLS_UNIFORM) tA = coef_pair<CM>(a.htab, B.cla, B.clb);
+        hp_update<T, CM>(B, tB, a, r, c, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
+    }
+    if (k + 1 < k1) {
+        hp_fetch<T, CM>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        if (CM != CLS_UNIFORM) tB = coef_pair<CM>(a.htab, A.cla, A.clb);
+        hp_update<T, CM>(A, tA, a, r, c, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
+        hp_update<T, CM>(B, tB, a, r, c, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
+    } else {
+        hp_update<T, CM>(A, tA, a, r, c, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
+    }
+}
+
+// ---- launchers --------------------------------------------------------------
+template <class T>
+static hipError_t launch_pair_t(bool efield, int mode, const KArgs &a, int m, hipStream_t s)
+{
+    const int nt = a.tiles_x * a.tiles_y * ((a.nz + a.kchunk - 1) / a.kchunk);
+    dim3 grid(nt), block(TX * BY);
+    if (efield) {
+        if (mode == CLS_UNIFORM) hipLaunchKernelGGL((e_pair_kernel<T, CLS_UNIFORM>), grid, block, 0, s, a, m);
+        else if (mode == CLS_U16) hipLaunchKernelGGL((e_pair_kernel<T, CLS_U16>), grid, block, 0, s, a, m);
+        else hipLaunchKernelGGL((e_pair_kernel<T, CLS_U32>), grid, block, 0, s, a, m);
+    } else {
+        if (mode == CLS_UNIFORM) hipLaunchKernelGGL((h_pair_kernel<T, CLS_UNIFORM>), grid, block, 0, s, a, m);
+        else if (mode == CLS_U16) hipLaunchKernelGGL((h_pair_kernel<T, CLS_U16>), grid, block, 0, s, a, m);
+        else hipLaunchKernelGGL((h_pair_kernel<T, CLS_U32>), grid, block, 0, s, a, m);
+    }
+    return hipGetLastError();
+}
+
+// storage: 1 FP16, 2 BF16.  a.tiles_x counts 128-cell tile columns.
+hipError_t launch_pair_half_step(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s)
+{
+    if (storage == 2) return launch_pair_t<bf16_t>(efield, mode, a, m, s);
+    return launch_pair_t<_Float16>(efield, mode, a, m, s);
+}
+
+int pair_tile_cols() { return TXP; }
+
+}  // namespace fdtd

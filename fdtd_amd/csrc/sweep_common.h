@@ -1,0 +1,190 @@
+// sweep_common.h -- device helpers shared by the E/H sweep kernels
+// (kernels.hip: one cell per thread; kernels_pair.hip: two cells per thread
+// for 16-bit storage).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <climits>
+#include <cstdint>
+
+#include "device_common.h"
+#include "internal.h"
+
+namespace fdtd {
+
+// Non-temporal access for the streams only the owning thread touches (the
+// swept field's own values): they are read once and written once per sweep.
+__device__ __forceinline__ float ldn(const float *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void stn(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ float ldn(const _Float16 *p) { return (float)__builtin_nontemporal_load(p); }
+__device__ __forceinline__ void stn(_Float16 *p, float v) { __builtin_nontemporal_store((_Float16)v, p); }
+__device__ __forceinline__ float ldn(const bf16_t *p)
+{
+    const std::uint16_t u = __builtin_nontemporal_load((const std::uint16_t *)p);
+    return __uint_as_float((std::uint32_t)u << 16);
+}
+__device__ __forceinline__ void stn(bf16_t *p, float v)
+{
+    std::uint32_t u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    __builtin_nontemporal_store((std::uint16_t)(u >> 16), (std::uint16_t *)p);
+}
+
+// Byte-offset addressing.  Every array a sweep touches is smaller than 4 GiB
+// (checked when the scene is finalized), so a 32-bit byte offset in one
+// vector register, added to a scalar base, addresses every stream of a
+// plane; field offsets are taken from the start of the allocation (the guard
+// plane below k = 0) so that they are never negative.
+template <class T>
+__device__ __forceinline__ const T *at(const T *base, std::uint32_t off)
+{
+    return (const T *)((const char *)base + off);
+}
+template <class T>
+__device__ __forceinline__ T *at(T *base, std::uint32_t off)
+{
+    return (T *)((char *)base + off);
+}
+
+// Read-only, workgroup-uniform table entry through the scalar data cache.
+typedef __attribute__((address_space(4))) const float const_float;
+__device__ __forceinline__ float4 uload(const float4 *p)
+{
+    const const_float *q = (const const_float *)p;
+    return make_float4(q[0], q[1], q[2], q[3]);
+}
+
+// Coefficient-table entries of one cell: (c, cb_a, cb_b, -) per component.
+struct Coef {
+    float4 x, y, z;
+};
+
+template <int CM>
+__device__ __forceinline__ Coef coef_of(const float4 *tab, int cl)
+{
+    Coef t;
+    t.x = tab[cl * 3 + 0];
+    t.y = tab[cl * 3 + 1];
+    t.z = tab[cl * 3 + 2];
+    return t;
+}
+
+template <int CM>
+__device__ __forceinline__ int class_at(const void *cls, std::uint32_t cell)
+{
+    if (CM == CLS_U16) return *at((const std::uint16_t *)cls, cell * 2u);
+    if (CM == CLS_U32) return (int)*at((const std::uint32_t *)cls, cell * 4u);
+    return 0;
+}
+
+template <class A>
+__device__ __forceinline__ int first_k(const A &a, int ev, int col)
+{
+    return (ev >= 0 && a.events[ev].col == col) ? a.events[ev].k : INT_MAX;
+}
+
+template <class A>
+__device__ __forceinline__ void record_probe(const A &a, int p, int comp, float stored, int m)
+{
+    const float v = stored * pick6(a.probe_unscale, comp);
+    a.probe_hist[(i64)p * a.steps_cap + m] = v;
+    const double dv = (double)v;
+    for (int f = 0; f < a.nfreq; ++f) {
+        const double2 ph = a.phasor[(i64)m * a.nfreq + f];
+        double2 acc = a.dft[(i64)p * a.nfreq + f];
+        acc.x = acc.x + dv * ph.x;
+        acc.y = acc.y + dv * ph.y;
+        a.dft[(i64)p * a.nfreq + f] = acc;
+    }
+}
+
+
+// ---- rare paths ------------------------------------------------------------
+// Cells carrying a source edge or a probe are a few hundred per sweep.  Their
+// handling reads a dozen further kernel arguments (event, source, waveform,
+// probe and DFT tables).  Read through `a` directly, the compiler hoists those
+// loads out of the z-march and holds the addresses in scalar registers for
+// the whole loop, spilling the loop's own scalars to vector lanes; read
+// through rare_args() they are loaded inside the rare branch each time.
+
+// The argument block seen through a pointer the compiler cannot follow.  It
+// is the first parameter of every sweep kernel, so it sits at the start of
+// the kernel-argument segment (taking the parameter's own address instead
+// would make the compiler copy the block to scratch).
+typedef __attribute__((address_space(4))) const KArgs ConstKArgs;
+__device__ __forceinline__ ConstKArgs &rare_args(const KArgs &)
+{
+    ConstKArgs *p = (ConstKArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return *p;
+}
+
+struct EvE {
+    int ev, ek;
+    float ezn;
+};
+
+// Source edges and E probes of one cell at plane k (calc_e's source term,
+// fdtdrav.h:726-729, and sampleField, fdtdrav.h:740).  Returns the next
+// event, its plane and Ez after the source term.
+template <class A>
+__device__ __forceinline__ EvE e_events(const A &a, int ev, int k, int col, int m, bool upd_z,
+                                                  float ez0, float hyc, float hy_im, float hxc, float hx_jm,
+                                                  float exn, float eyn, float ezn)
+{
+    while (ev >= 0 && a.events[ev].k == k && a.events[ev].col == col) {
+        const Event e = a.events[ev];
+        if (e.kind == EV_SOURCE) {
+            if (upd_z) {
+                const SrcEdge s = a.srcs[e.id];
+                ezn = ((s.ceze * ez0 + s.cezhy * (hyc - hy_im)) + s.cezhx * (hxc - hx_jm)) +
+                      s.cezj * a.vpn[(i64)s.wave * a.steps_cap + m];
+            }
+        } else if (a.rec) {
+            const int comp = e.kind - EV_PROBE;
+            record_probe(a, e.id, comp, comp == 0 ? exn : comp == 1 ? eyn : ezn, m);
+        }
+        ++ev;
+    }
+    EvE r;
+    r.ev = ev;
+    r.ek = first_k(a, ev, col);
+    r.ezn = ezn;
+    return r;
+}
+
+struct EvH {
+    int ev, ek;
+};
+
+// H probes of one cell at plane k (sampleField in calc_h, fdtdrav.h:347-391).
+template <class A>
+__device__ __forceinline__ EvH h_events(const A &a, int ev, int k, int col, int m, float hxn,
+                                                  float hyn, float hzn)
+{
+    while (ev >= 0 && a.events[ev].k == k && a.events[ev].col == col) {
+        const Event e = a.events[ev];
+        const int comp = e.kind - EV_PROBE;
+        if (a.rec) record_probe(a, e.id, comp, comp == 3 ? hxn : comp == 4 ? hyn : hzn, m);
+        ++ev;
+    }
+    EvH r;
+    r.ev = ev;
+    r.ek = first_k(a, ev, col);
+    return r;
+}
+
+// Design-region snapshot of E^{m+1} (stored precision, SI units) for the
+// adjoint; only runs when the adjoint's forward pass asks for it.
+template <class A>
+__device__ __forceinline__ void e_snapshot(const A &a, int i, int j, int k, int m, float vx, float vy,
+                                                     float vz)
+{
+    const i64 nb = (i64)a.sb[3] * a.sb[4] * a.sb[5];
+    const i64 q = (i - a.sb[0]) + (i64)a.sb[3] * ((j - a.sb[1]) + (i64)a.sb[4] * (k - a.sb[2]));
+    float *o = a.snap + (i64)(m + 1 - a.snap_base) * 3 * nb + q;
+    o[0] = vx * a.probe_unscale[0];
+    o[nb] = vy * a.probe_unscale[1];
+    o[2 * nb] = vz * a.probe_unscale[2];
+}
+
+}  // namespace fdtd

@@ -66,3 +66,54 @@ def test_scaled_storage_matches_rounding_model(engine, storage, mode):
     # engine is much closer to the FP64 result than to a run with the H
     # scale off by 2x
     assert e_model.max() < 4 * e_store.max()
+
+
+def _pair_scenes():
+    from test_gpu_parity import SCENES
+    odd = scenes.dielectric(nx=41, ny=37, nz=30, steps=90, L=5, n_boxes=5, n_src=2, n_probes=6, n_freqs=2,
+                            seed=23, name="odd-nx")
+    odd.probes += [(scenes.EX, 40, 20, 15), (scenes.EY, 39, 20, 15), (scenes.HZ, 0, 18, 14), (scenes.HY, 40, 5, 5)]
+    out = {"odd_nx": odd}
+    for name in ("vacuum", "magnetic", "design_u32"):
+        out[name] = SCENES[name]()
+    return out
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_pair_kernels_store_the_one_cell_bits(engine, storage, monkeypatch):
+    """The two-cells-per-thread sweeps (default for 16-bit storage) write the
+    same bits as the one-cell sweeps (FDTD_PAIR=0): probes, DFT, fields."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    for name, sc in _pair_scenes().items():
+        print(f"[pair] storage={storage} scene={name} grid={sc.nx}x{sc.ny}x{sc.nz} steps={sc.steps}", flush=True)
+        res = []
+        for pair in ("1", "0"):
+            monkeypatch.setenv("FDTD_PAIR", pair)
+            sim = engine.Simulation.from_scene(sc, storage)
+            sim.run(sc.steps)
+            res.append((sim.probes(), sim.dft() if sc.freqs else None,
+                        [sim.field(c) for c in range(6)]))
+            sim.close()
+        (p1, d1, f1), (p0, d0, f0) = res
+        np.testing.assert_array_equal(p1, p0)
+        if d1 is not None:
+            np.testing.assert_array_equal(d1, d0)
+        for c in range(6):
+            np.testing.assert_array_equal(f1[c], f0[c], err_msg=f"{name} component {c}")
+
+
+def test_pair_kernels_adjoint_snapshots(engine, monkeypatch):
+    """The adjoint's forward recompute goes through the pair sweeps for
+    16-bit storage: same gradient as with the one-cell sweeps."""
+    sc = scenes.adjoint(nx=30, ny=26, nz=22, steps=100, L=4, d=1e-3, design=(7, 5, 4), eps_range=(1.0, 4.0),
+                        seed=9, probe_freqs=(12e9,), name="adjoint-pair")
+    out = []
+    for pair in ("1", "0"):
+        monkeypatch.setenv("FDTD_PAIR", pair)
+        sim = engine.Simulation.from_scene(sc, "f16")
+        out.append(sim.adjoint_gradient([(0, 0, 1.0)], sc.steps, 32))
+        sim.close()
+    assert out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[0][1], out[1][1])
