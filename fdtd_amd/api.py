@@ -85,7 +85,8 @@ class FdtdInfo(C.Structure):
                 ("fused", C.c_int), ("pad1", C.c_int), ("adjoint_forward_ms", C.c_double),
                 ("adjoint_total_ms", C.c_double), ("adjoint_segments", C.c_int), ("pad0", C.c_int),
                 ("device_bytes", C.c_uint64),
-                ("bytes_per_cell_e", C.c_uint64), ("bytes_per_cell_h", C.c_uint64)]
+                ("bytes_per_cell_e", C.c_uint64), ("bytes_per_cell_h", C.c_uint64),
+                ("adjoint_device_bytes", C.c_uint64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -68,17 +68,23 @@ __device__ HQ hquant(const KArgs &a, const AArgs &b, bool huni, int i, int j, in
     const float4 pzm = zs ? a.pz_prof[nz + k] : NOPROF;
     const float hbx = b.hb[0][c], hby = b.hb[1][c], hbz = b.hb[2][c];
     const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    // curl coefficients over kappa, divided on the slab cells only (kappa is
+    // 1 elsewhere and x / 1 = x exactly)
+    float hxy = hx.y, hxz = hx.z, hyy = hy.y, hyz = hy.z, hzy = hz.y, hzz = hz.z;
+    if (zs) { hxy = hxy / pzm.z; hyz = hyz / pzm.z; }
+    if (ys) { hxz = hxz / pym.z; hzy = hzy / pym.z; }
+    if (xs) { hyy = hyy / pxm.z; hzz = hzz / pxm.z; }
     if (ux) {
-        q.sy = (hx.y / pzm.z) * hbx;
-        q.sz = (hx.z / pym.z) * hbx;
+        q.sy = hxy * hbx;
+        q.sz = hxz * hbx;
     }
     if (uy) {
-        q.tz = (hy.y / pxm.z) * hby;
-        q.tx = (hy.z / pzm.z) * hby;
+        q.tz = hyy * hby;
+        q.tx = hyz * hby;
     }
     if (uz) {
-        q.rx = (hz.y / pym.z) * hbz;
-        q.ry = (hz.z / pxm.z) * hbz;
+        q.rx = hzy * hbz;
+        q.ry = hzz * hbz;
     }
     if (xs) {
         const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
@@ -120,17 +126,21 @@ __device__ EQ equant(const KArgs &a, const AArgs &b, bool euni, int i, int j, in
     const float4 pze = zs ? a.pz_prof[k] : NOPROF;
     const float ebx = b.eb[0][c], eby = b.eb[1][c], ebz = b.eb[2][c];
     const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    float exy = ex.y, exz = ex.z, eyy = ey.y, eyz = ey.z, ezy = ez.y, ezz = ez.z;
+    if (ys) { exy = exy / pye.z; ezz = ezz / pye.z; }
+    if (zs) { exz = exz / pze.z; eyy = eyy / pze.z; }
+    if (xs) { eyz = eyz / pxe.z; ezy = ezy / pxe.z; }
     if (ux) {
-        q.ax = (ex.y / pye.z) * ebx;
-        q.bx = (ex.z / pze.z) * ebx;
+        q.ax = exy * ebx;
+        q.bx = exz * ebx;
     }
     if (uy) {
-        q.ay = (ey.y / pze.z) * eby;
-        q.by = (ey.z / pxe.z) * eby;
+        q.ay = eyy * eby;
+        q.by = eyz * eby;
     }
     if (uz) {
-        q.az = (ez.y / pxe.z) * ebz;
-        q.bz = (ez.z / pye.z) * ebz;
+        q.az = ezy * ebz;
+        q.bz = ezz * ebz;
     }
     if (xs) {
         const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
@@ -152,41 +162,67 @@ __device__ EQ equant(const KArgs &a, const AArgs &b, bool euni, int i, int j, in
 
 }  // namespace
 
+// Tiles of the adjoint sweeps.  A thread's gather needs the transposed
+// quantities of its cell and of two neighbours in the same plane ((i-1, j)
+// and (i, j-1) for the E sweep, (i+1, j) and (i, j+1) for the H sweep); the
+// plane neighbour (k-1 / k+1) is carried in registers along the z-march.
+// Each thread computes its own cell's quantities once per plane and shares
+// them through LDS: a wave covers TXA = 63 columns plus one halo lane (the
+// column before the tile for the E sweep, after it for the H sweep) that only
+// computes, and the edge row of the tile computes its missing row directly.
+// The shared values are exactly those the neighbour computes for itself.
+constexpr int TXA = TX - 1;
+
+// Whether the adjoint sweeps share through LDS for rows of nx cells.
+inline bool adj_shares(int nx) { return nx >= 8 * TX; }
+
 // fstep >= 0: finalize the E half-step of forward step fstep first.
 // sten: gather the transposed H half-step of step m and inject dJ/ds[m].
-template <int CM>
+template <int CM, bool SHARE>
 __global__ void __launch_bounds__(TX * BY)
     adj_e_kernel(const KArgs a, const AArgs b, const int m, const int fstep, const int sten, const int euni,
                  const int huni)
 {
+    __shared__ HQ sq[BY][TX];
     int tx, ty, tz;
     tile_of(a, tx, ty, tz);
-    const int i = tx * TX + (int)(threadIdx.x & (TX - 1));
-    const int j = ty * BY + (int)(threadIdx.x / TX);
+    const int lane = (int)(threadIdx.x & (TX - 1));
+    const int row = (int)(threadIdx.x / TX);
+    // SHARE: lane 0 is the halo column before the tile; otherwise every lane
+    // owns a cell and computes its neighbours' quantities itself
+    const int i = SHARE ? tx * TXA + lane - 1 : tx * TX + lane;
+    const int j = ty * BY + row;
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
-    if (i >= nx || j >= ny) return;
+    const bool valid = i >= 0 && i < nx && j < ny;   // a cell whose quantities a neighbour reads
+    const bool own = valid && (!SHARE || lane > 0);  // a cell this thread updates
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
     const bool ilo = i > 0, ihi = i < nx - 1, jlo = j > 0, jhi = j < ny - 1;
     const int mycol = j * nx + i;
-    int ev = first_event(a, a.e_first, mycol, k0);
+    int ev = own ? first_event(a, a.e_first, mycol, k0) : -1;
     int ek = ev >= 0 ? a.events[ev].k : INT_MAX;
     const bool in_box = i >= b.sb[0] && i < b.sb[0] + b.sb[3] && j >= b.sb[1] && j < b.sb[1] + b.sb[4];
     const i64 nb = (i64)b.sb[3] * b.sb[4] * b.sb[5];
-    const bool xs = xs_e(a, i), ys = ys_e(a, j);
+    const bool xs = own && xs_e(a, i), ys = own && ys_e(a, j);
     const float4 pxe = xs ? a.px_prof[i] : NOPROF;
     const float4 pye = ys ? a.py_prof[j] : NOPROF;
     const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    const HQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 
-    HQ qkm{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (sten && k0 > 0) qkm = hquant<CM>(a, b, huni, i, j, k0 - 1);
+    HQ qkm = zq;
+    if (own && sten && k0 > 0) qkm = hquant<CM>(a, b, huni, i, j, k0 - 1);
     for (int k = k0; k < k1; ++k) {
         const i64 c = (i64)k * a.plane + (i64)j * px + i;
         const bool klo = k > 0, khi = k < nz - 1;
-        const bool ux = ihi && jlo && jhi && klo && khi;
-        const bool uy = ilo && ihi && jhi && klo && khi;
-        const bool uz = ilo && ihi && jlo && jhi && khi;
-        float ebx = b.eb[0][c], eby = b.eb[1][c], ebz = b.eb[2][c];
+        const bool ux = own && ihi && jlo && jhi && klo && khi;
+        const bool uy = own && ilo && ihi && jhi && klo && khi;
+        const bool uz = own && ilo && ihi && jlo && jhi && khi;
+        float ebx = 0.f, eby = 0.f, ebz = 0.f;
+        if (own) {
+            ebx = b.eb[0][c];
+            eby = b.eb[1][c];
+            ebz = b.eb[2][c];
+        }
         // source edge and probe events of this cell
         float cez_src = 0.f;
         bool is_src = false;
@@ -241,32 +277,36 @@ __global__ void __launch_bounds__(TX * BY)
             if (uz) ebz = (is_src ? cez_src : ez.x) * ebz;
         }
         if (sten) {
-            const HQ q0 = hquant<CM>(a, b, huni, i, j, k);
-            if (ux) {
-                const HQ qj = hquant<CM>(a, b, huni, i, j - 1, k);
-                ebx += (qkm.tx - q0.tx) + (qj.rx - q0.rx);
-                if (uz) ebz += qj.sz - q0.sz;
+            const HQ q0 = (SHARE ? valid : own) ? hquant<CM>(a, b, huni, i, j, k) : zq;
+            if (SHARE) {
+                sq[row][lane] = q0;
+                __syncthreads();
             }
-            if (uy || uz) {
-                const HQ qi = hquant<CM>(a, b, huni, i - 1, j, k);
+            if (own) {
+                // (i-1, j, k) and (i, j-1, k)
+                const HQ qi = SHARE ? sq[row][lane - 1] : (ilo && (uy || uz) ? hquant<CM>(a, b, huni, i - 1, j, k) : zq);
+                const HQ qj = (SHARE && row > 0) ? sq[row - 1][lane]
+                                                 : (jlo && (ux || uz) ? hquant<CM>(a, b, huni, i, j - 1, k) : zq);
+                if (ux) {
+                    ebx += (qkm.tx - q0.tx) + (qj.rx - q0.rx);
+                    if (uz) ebz += qj.sz - q0.sz;
+                }
                 if (uy) eby += (qkm.sy - q0.sy) + (qi.ry - q0.ry);
                 if (uz) ebz += qi.tz - q0.tz;
-            }
-            if (uz && !ux) {
-                const HQ qj = hquant<CM>(a, b, huni, i, j - 1, k);
-                ebz += qj.sz - q0.sz;
-            }
-            qkm = q0;
-            if (pev >= 0) {   // dJ/d(probe sample m) of E probes on this cell
-                for (int e = pev; a.events[e].k == k && a.events[e].col == mycol; ++e) {
-                    const int kind = a.events[e].kind;
-                    if (kind < EV_PROBE || kind > EV_PROBE + 2) continue;
-                    const float gv = b.g[(i64)a.events[e].id * b.gsteps + m];
-                    if (kind == EV_PROBE) ebx += gv;
-                    else if (kind == EV_PROBE + 1) eby += gv;
-                    else ebz += gv;
+                if (uz && !ux) ebz += qj.sz - q0.sz;
+                qkm = q0;
+                if (pev >= 0) {   // dJ/d(probe sample m) of E probes on this cell
+                    for (int e = pev; a.events[e].k == k && a.events[e].col == mycol; ++e) {
+                        const int kind = a.events[e].kind;
+                        if (kind < EV_PROBE || kind > EV_PROBE + 2) continue;
+                        const float gv = b.g[(i64)a.events[e].id * b.gsteps + m];
+                        if (kind == EV_PROBE) ebx += gv;
+                        else if (kind == EV_PROBE + 1) eby += gv;
+                        else ebz += gv;
+                    }
                 }
             }
+            if (SHARE) __syncthreads();
         }
         if (ux) b.eb[0][c] = ebx;
         if (uy) b.eb[1][c] = eby;
@@ -274,67 +314,82 @@ __global__ void __launch_bounds__(TX * BY)
     }
 }
 
-template <int CM>
+template <int CM, bool SHARE>
 __global__ void __launch_bounds__(TX * BY)
     adj_h_kernel(const KArgs a, const AArgs b, const int euni, const int huni)
 {
+    __shared__ EQ sq[BY][TX];
     int tx, ty, tz;
     tile_of(a, tx, ty, tz);
-    const int i = tx * TX + (int)(threadIdx.x & (TX - 1));
-    const int j = ty * BY + (int)(threadIdx.x / TX);
+    const int lane = (int)(threadIdx.x & (TX - 1));
+    const int row = (int)(threadIdx.x / TX);
+    // SHARE: lane 63 is the halo column after the tile
+    const int i = SHARE ? tx * TXA + lane : tx * TX + lane;
+    const int j = ty * BY + row;
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
-    if (i >= nx || j >= ny) return;
+    const bool valid = i < nx && j < ny;
+    const bool own = valid && (!SHARE || lane < TX - 1);
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
     const bool ihi = i < nx - 1, jhi = j < ny - 1;
-    const bool xs = xs_h(a, i), ys = ys_h(a, j);
+    const bool xs = own && xs_h(a, i), ys = own && ys_h(a, j);
     const float4 pxm = xs ? a.px_prof[nx + i] : NOPROF;
     const float4 pym = ys ? a.py_prof[ny + j] : NOPROF;
     const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
-    EQ q0 = equant<CM>(a, b, euni, i, j, k0);
+    const EQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    EQ q0 = (SHARE ? valid : own) ? equant<CM>(a, b, euni, i, j, k0) : zq;
     for (int k = k0; k < k1; ++k) {
         const i64 c = (i64)k * a.plane + (i64)j * px + i;
         const bool khi = k < nz - 1;
-        const bool ux = jhi && khi, uy = ihi && khi, uz = ihi && jhi;
-        const EQ qk = khi ? equant<CM>(a, b, euni, i, j, k + 1) : EQ{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        float4 hx, hy, hz;
-        coefs<CM>(a, a.htab, huni, c, hx, hy, hz);
-        float hbx = b.hb[0][c], hby = b.hb[1][c], hbz = b.hb[2][c];
-        // psi_h adjoint and dJ/dH <- Ch dJ/dH (the H half-step, pointwise part)
-        if (xs) {
-            const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
-            b.pb[2][s] = pxm.x * (b.pb[2][s] + (uy ? (hy.y * dx) * hby : 0.f));
-            b.pb[3][s] = pxm.x * (b.pb[3][s] + (uz ? (hz.z * dx) * hbz : 0.f));
+        const bool ux = own && jhi && khi, uy = own && ihi && khi, uz = own && ihi && jhi;
+        const EQ qk = ((SHARE ? valid : own) && khi) ? equant<CM>(a, b, euni, i, j, k + 1) : zq;
+        float hbx = 0.f, hby = 0.f, hbz = 0.f;
+        if (own) {
+            float4 hx, hy, hz;
+            coefs<CM>(a, a.htab, huni, c, hx, hy, hz);
+            hbx = b.hb[0][c];
+            hby = b.hb[1][c];
+            hbz = b.hb[2][c];
+            // psi_h adjoint and dJ/dH <- Ch dJ/dH (the H half-step, pointwise part)
+            if (xs) {
+                const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
+                b.pb[2][s] = pxm.x * (b.pb[2][s] + (uy ? (hy.y * dx) * hby : 0.f));
+                b.pb[3][s] = pxm.x * (b.pb[3][s] + (uz ? (hz.z * dx) * hbz : 0.f));
+            }
+            if (ys) {
+                const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
+                b.pb[6][s] = pym.x * (b.pb[6][s] + (ux ? (hx.z * dy) * hbx : 0.f));
+                b.pb[7][s] = pym.x * (b.pb[7][s] + (uz ? (hz.y * dy) * hbz : 0.f));
+            }
+            if (zs_h(a, k)) {
+                const float4 pzm = a.pz_prof[nz + k];
+                const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
+                b.pb[10][s] = pzm.x * (b.pb[10][s] + (ux ? (hx.y * dz) * hbx : 0.f));
+                b.pb[11][s] = pzm.x * (b.pb[11][s] + (uy ? (hy.z * dz) * hby : 0.f));
+            }
+            if (ux) hbx = hx.x * hbx;
+            if (uy) hby = hy.x * hby;
+            if (uz) hbz = hz.x * hbz;
         }
-        if (ys) {
-            const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
-            b.pb[6][s] = pym.x * (b.pb[6][s] + (ux ? (hx.z * dy) * hbx : 0.f));
-            b.pb[7][s] = pym.x * (b.pb[7][s] + (uz ? (hz.y * dy) * hbz : 0.f));
-        }
-        if (zs_h(a, k)) {
-            const float4 pzm = a.pz_prof[nz + k];
-            const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
-            b.pb[10][s] = pzm.x * (b.pb[10][s] + (ux ? (hx.y * dz) * hbx : 0.f));
-            b.pb[11][s] = pzm.x * (b.pb[11][s] + (uy ? (hy.z * dz) * hby : 0.f));
-        }
-        if (ux) hbx = hx.x * hbx;
-        if (uy) hby = hy.x * hby;
-        if (uz) hbz = hz.x * hbz;
         // transposed E half-step gathered from the E cells that read this H
-        if (ux) {
-            const EQ qj = equant<CM>(a, b, euni, i, j + 1, k);
-            hbx += (q0.ay - qk.ay) + (q0.bz - qj.bz);
-            if (uz) hbz += q0.ax - qj.ax;
+        if (SHARE) {
+            sq[row][lane] = q0;
+            __syncthreads();
         }
-        if (uy || uz) {
-            const EQ qi = equant<CM>(a, b, euni, i + 1, j, k);
+        if (own) {
+            // (i+1, j, k) and (i, j+1, k)
+            const EQ qi = SHARE ? sq[row][lane + 1] : (ihi && (uy || uz) ? equant<CM>(a, b, euni, i + 1, j, k) : zq);
+            const EQ qj = (SHARE && row < BY - 1) ? sq[row + 1][lane]
+                                                  : (jhi && (ux || uz) ? equant<CM>(a, b, euni, i, j + 1, k) : zq);
+            if (ux) {
+                hbx += (q0.ay - qk.ay) + (q0.bz - qj.bz);
+                if (uz) hbz += q0.ax - qj.ax;
+            }
             if (uy) hby += (q0.bx - qk.bx) + (q0.az - qi.az);
             if (uz) hbz += q0.by - qi.by;
+            if (uz && !ux) hbz += q0.ax - qj.ax;
         }
-        if (uz && !ux) {
-            const EQ qj = equant<CM>(a, b, euni, i, j + 1, k);
-            hbz += q0.ax - qj.ax;
-        }
+        if (SHARE) __syncthreads();
         if (ux) b.hb[0][c] = hbx;
         if (uy) b.hb[1][c] = hby;
         if (uz) b.hb[2][c] = hbz;
@@ -385,24 +440,33 @@ __global__ void snap_kernel(const KArgs a, float *hist, const int slot)
     o[2 * nb] = ld((const T *)a.ez + c) * a.probe_unscale[2];
 }
 
-hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a, const AArgs &b, int m, int fstep,
+hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, const AArgs &b, int m, int fstep,
                       int sten, hipStream_t s)
 {
+    // LDS sharing costs one halo column per 64 and rounds nx up to 63-column
+    // tiles: used once the rows are wide enough for that to be small
+    const bool share = adj_shares(a0.nx);
+    KArgs a = a0;
+    a.tiles_x = share ? (a.nx + TXA - 1) / TXA : (a.nx + TX - 1) / TX;
     const int nt = a.tiles_x * a.tiles_y * ((a.nz + a.kchunk - 1) / a.kchunk);
     dim3 grid(nt), block(TX * BY);
+#define ADJ_E(CMV, SH, EU, HU) hipLaunchKernelGGL((adj_e_kernel<CMV, SH>), grid, block, 0, s, a, b, m, fstep, sten, EU, HU)
+#define ADJ_H(CMV, SH, EU, HU) hipLaunchKernelGGL((adj_h_kernel<CMV, SH>), grid, block, 0, s, a, b, EU, HU)
     if (which == 0) {
-        if (mode == CLS_U32) hipLaunchKernelGGL((adj_e_kernel<CLS_U32>), grid, block, 0, s, a, b, m, fstep, sten, euni, huni);
-        else if (mode == CLS_U16) hipLaunchKernelGGL((adj_e_kernel<CLS_U16>), grid, block, 0, s, a, b, m, fstep, sten, euni, huni);
-        else hipLaunchKernelGGL((adj_e_kernel<CLS_UNIFORM>), grid, block, 0, s, a, b, m, fstep, sten, 1, 1);
+        if (mode == CLS_U32) { if (share) ADJ_E(CLS_U32, true, euni, huni); else ADJ_E(CLS_U32, false, euni, huni); }
+        else if (mode == CLS_U16) { if (share) ADJ_E(CLS_U16, true, euni, huni); else ADJ_E(CLS_U16, false, euni, huni); }
+        else { if (share) ADJ_E(CLS_UNIFORM, true, 1, 1); else ADJ_E(CLS_UNIFORM, false, 1, 1); }
     } else if (which == 1) {
-        if (mode == CLS_U32) hipLaunchKernelGGL((adj_h_kernel<CLS_U32>), grid, block, 0, s, a, b, euni, huni);
-        else if (mode == CLS_U16) hipLaunchKernelGGL((adj_h_kernel<CLS_U16>), grid, block, 0, s, a, b, euni, huni);
-        else hipLaunchKernelGGL((adj_h_kernel<CLS_UNIFORM>), grid, block, 0, s, a, b, 1, 1);
+        if (mode == CLS_U32) { if (share) ADJ_H(CLS_U32, true, euni, huni); else ADJ_H(CLS_U32, false, euni, huni); }
+        else if (mode == CLS_U16) { if (share) ADJ_H(CLS_U16, true, euni, huni); else ADJ_H(CLS_U16, false, euni, huni); }
+        else { if (share) ADJ_H(CLS_UNIFORM, true, 1, 1); else ADJ_H(CLS_UNIFORM, false, 1, 1); }
     } else {
         if (mode == CLS_U32) hipLaunchKernelGGL((adj_src_fixup<CLS_U32>), dim3(1), dim3(64), 0, s, a, b, euni);
         else if (mode == CLS_U16) hipLaunchKernelGGL((adj_src_fixup<CLS_U16>), dim3(1), dim3(64), 0, s, a, b, euni);
         else hipLaunchKernelGGL((adj_src_fixup<CLS_UNIFORM>), dim3(1), dim3(64), 0, s, a, b, 1);
     }
+#undef ADJ_E
+#undef ADJ_H
     return hipGetLastError();
 }
 

@@ -61,6 +61,7 @@ struct fdtd_ctx {
     bool fused = true;              // one kernel per step (else E and H half-step kernels)
     bool pair = false;              // 16-bit storage: two cells per thread in the half-step kernels
     int pair_tiles_x = 0, pair_kchunk = 0;
+    int kchunk_h = 0;               // planes per block of the one-cell H sweep (E: ka.kchunk)
     KArgs ka_step{};                // tile geometry of the step kernel
     double step_ms = 0.0;
     size_t psi_elems[12] = {};
@@ -72,6 +73,7 @@ struct fdtd_ctx {
     const float4 *dcoef = nullptr;
     double adj_fwd_ms = 0.0, adj_total_ms = 0.0;
     int adj_segments = 0;
+    uint64_t adj_bytes = 0;         // peak device memory of the last adjoint run
 };
 
 namespace {
@@ -165,14 +167,17 @@ int zero_state(fdtd_ctx *c)
 // after a design-eps change; buffers of unchanged size are reused.
 int upload_tables(fdtd_ctx *c);
 
-int choose_chunks(const fdtd_ctx *c, int tiles)
+// Planes per z-march block.  Many short marches keep every CU's wave slots
+// full through the tail of a launch; each block re-reads one plane of the
+// carried z-neighbour, so a block of `planes` costs 1/planes extra on that
+// stream.  Measured on slab512: the FP32 E sweep is fastest at 4 planes, the
+// FP32 H sweep at 2, the 16-bit pair sweeps at 8 (FDTD_CHUNKS overrides all
+// three with a number of blocks along z).
+int choose_chunks(const fdtd_ctx *c, int tiles, int planes)
 {
-    // Many short z-marches: ~32k workgroups keep every CU's wave slots
-    // full through the tail of the launch, while a march of >= 8 planes
-    // keeps the re-read of the plane below (above) each chunk under 1/8.
     const char *ov = std::getenv("FDTD_CHUNKS");
     int nchunks = ov ? std::atoi(ov)
-                     : std::min((32768 + tiles - 1) / tiles, c->in.nz / 8);
+                     : std::min((32 * 1024 * 8 / planes + tiles - 1) / tiles, c->in.nz / planes);
     nchunks = std::max(1, std::min(nchunks, c->in.nz));
     return (c->in.nz + nchunks - 1) / nchunks;
 }
@@ -201,13 +206,14 @@ int do_finalize(fdtd_ctx *c)
     for (int q = 0; q < 6; ++q) a.probe_unscale[q] = b.probe_unscale[q];
     a.tiles_x = (in.nx + tile_cols() - 1) / tile_cols();
     a.tiles_y = (in.ny + tile_rows() - 1) / tile_rows();
-    a.kchunk = choose_chunks(c, a.tiles_x * a.tiles_y);
+    a.kchunk = choose_chunks(c, a.tiles_x * a.tiles_y, 4);
+    c->kchunk_h = choose_chunks(c, a.tiles_x * a.tiles_y, 2);
     // 16-bit storage sweeps a cell pair per thread (128-cell tile rows);
     // FDTD_PAIR=0 selects the one-cell kernels instead (same stored bits)
     const char *pv = std::getenv("FDTD_PAIR");
     c->pair = in.storage != FDTD_STORE_F32 && !(pv && std::strcmp(pv, "0") == 0);
     c->pair_tiles_x = (in.nx + pair_tile_cols() - 1) / pair_tile_cols();
-    c->pair_kchunk = choose_chunks(c, c->pair_tiles_x * a.tiles_y);
+    c->pair_kchunk = choose_chunks(c, c->pair_tiles_x * a.tiles_y, 8);
     a.rec = 1;
     a.snap = nullptr;
 
@@ -379,6 +385,7 @@ hipError_t do_step(fdtd_ctx *c, const KArgs &base, int m, hipEvent_t ev0 = nullp
     if ((e = half(true, c->in.storage, c->e_mode, a, m, c->stream)) != hipSuccess) return e;
     if (ev1) (void)hipEventRecord(ev1, c->stream);
     if (ev2) (void)hipEventRecord(ev2, c->stream);
+    if (!c->pair) a.kchunk = c->kchunk_h;
     e = half(false, c->in.storage, c->h_mode, a, m, c->stream);
     if (ev3) (void)hipEventRecord(ev3, c->stream);
     return e;
@@ -497,20 +504,22 @@ int copy_state(fdtd_ctx *c, char *dst, bool save)
 
 struct DevBuf {
     std::vector<void *> ptrs;
+    uint64_t bytes = 0;
     ~DevBuf()
     {
         // an early return may leave work queued on these buffers
         if (!ptrs.empty()) (void)hipDeviceSynchronize();
         for (void *p : ptrs) (void)hipFree(p);
     }
-    void *get(size_t bytes)
+    void *get(size_t bytes_)
     {
         void *p = nullptr;
-        if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) {
+        if (hipMalloc(&p, std::max<size_t>(bytes_, 256)) != hipSuccess) {
             (void)hipGetLastError();
             return nullptr;
         }
         ptrs.push_back(p);
+        bytes += std::max<size_t>(bytes_, 256);
         return p;
     }
 };
@@ -606,6 +615,11 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     KArgs fa = c->ka;               // forward arguments
     fa.sb[0] = d[0]; fa.sb[1] = d[1]; fa.sb[2] = d[2]; fa.sb[3] = bx; fa.sb[4] = by; fa.sb[5] = bz;
     KArgs aa = c->ka;               // adjoint arguments: SI tables
+    if (const char *ov = std::getenv("FDTD_ADJ_CHUNKS")) {
+        // z-blocks of the adjoint sweeps (default: those of the forward E sweep)
+        const int nch = std::max(1, std::min(std::atoi(ov), nz));
+        aa.kchunk = (nz + nch - 1) / nch;
+    }
     aa.etab = c->etab_si;
     aa.htab = c->htab_si;
     aa.srcs = c->srcs_si;
@@ -687,6 +701,7 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     c->adj_fwd_ms = ms1;
     c->adj_total_ms = ms2;
     c->adj_segments = nseg;
+    c->adj_bytes = c->dev_bytes + mem.bytes;
     // ---- chain rule through the 4-cell averages (fdtdrav.h:1135-1177)
     std::vector<double> ge((size_t)3 * nb);
     HIPCHK(c, hipMemcpy(ge.data(), gedge, ge.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -946,6 +961,7 @@ int fdtd_get_info(const fdtd_ctx *c, fdtd_info *o)
     o->adjoint_forward_ms = c->adj_fwd_ms;
     o->adjoint_total_ms = c->adj_total_ms;
     o->adjoint_segments = c->adj_segments;
+    o->adjoint_device_bytes = c->adj_bytes;
     o->bytes_per_cell_e = (uint64_t)std::llround(9 * s + (c->e_mode ? cls : 0.0) + psi);
     o->bytes_per_cell_h = (uint64_t)std::llround(9 * s + (c->h_mode ? cls : 0.0) + psi);
     return FDTD_OK;

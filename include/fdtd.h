@@ -135,6 +135,8 @@ typedef struct {
     uint64_t device_bytes;       /* device memory held by the context          */
     uint64_t bytes_per_cell_e;   /* algorithmic bytes per cell, E sweep        */
     uint64_t bytes_per_cell_h;   /* algorithmic bytes per cell, H sweep        */
+    uint64_t adjoint_device_bytes; /* last fdtd_adjoint_gradient: peak device
+                                    memory, context plus adjoint buffers      */
 } fdtd_info;
 
 /* ---- life cycle ----------------------------------------------------- */
