@@ -62,6 +62,10 @@ struct fdtd_ctx {
     bool pair = false;              // 16-bit storage: two cells per thread in the half-step kernels
     int pair_tiles_x = 0, pair_kchunk = 0;
     int kchunk_h = 0;               // planes per block of the one-cell H sweep (E: ka.kchunk)
+    // per sweep (0 = E, 1 = H): interior tile box and the general variant's tile list
+    int box[2][6] = {};
+    const int *glist[2] = {nullptr, nullptr};
+    int nglist[2] = {0, 0};
     KArgs ka_step{};                // tile geometry of the step kernel
     double step_ms = 0.0;
     size_t psi_elems[12] = {};
@@ -182,6 +186,33 @@ int choose_chunks(const fdtd_ctx *c, int tiles, int planes)
     return (c->in.nz + nchunks - 1) / nchunks;
 }
 
+void interior_box(const fdtd_ctx *c, int tile_cols, int kchunk, int ib[6]);
+
+// Interior box and general tile list of one sweep (tiles tile_cols wide,
+// z-blocks of kchunk planes).
+int sweep_split(fdtd_ctx *c, int sweep, int tile_cols, int kchunk)
+{
+    int *ib = c->box[sweep];
+    interior_box(c, tile_cols, kchunk, ib);
+    c->glist[sweep] = nullptr;
+    c->nglist[sweep] = 0;
+    if (ib[3] == ib[0]) return FDTD_OK;
+    const int tx = (c->in.nx + tile_cols - 1) / tile_cols, ty = c->ka.tiles_y;
+    const int tz = (c->in.nz + kchunk - 1) / kchunk;
+    std::vector<int> list;
+    for (int z = 0; z < tz; ++z)
+        for (int y = 0; y < ty; ++y)
+            for (int x = 0; x < tx; ++x)
+                if (!(x >= ib[0] && x < ib[3] && y >= ib[1] && y < ib[4] && z >= ib[2] && z < ib[5]))
+                    list.push_back(x + tx * (y + ty * z));
+    void *d = nullptr;
+    int rc = upload(c, &d, list.data(), list.size() * sizeof(int));
+    if (rc) return rc;
+    c->glist[sweep] = (const int *)d;
+    c->nglist[sweep] = (int)list.size();
+    return FDTD_OK;
+}
+
 int do_finalize(fdtd_ctx *c)
 {
     const SceneIn &in = c->in;
@@ -235,6 +266,13 @@ int do_finalize(fdtd_ctx *c)
             int rc = dev_alloc(c, &c->fields_set[s][f], (size_t)c->field_elems * c->esz);
             if (rc) return rc;
         }
+    // interior / general split of the two sweeps
+    {
+        const int tcols = c->pair ? pair_tile_cols() : tile_cols();
+        int rc = sweep_split(c, 0, tcols, c->pair ? c->pair_kchunk : a.kchunk);
+        if (!rc) rc = sweep_split(c, 1, tcols, c->pair ? c->pair_kchunk : c->kchunk_h);
+        if (rc) return rc;
+    }
     // psi slabs
     const size_t px_elems = (size_t)in.nz * in.ny * a.Lx;
     const size_t py_elems = (size_t)in.nz * a.Ly * in.px;
@@ -357,6 +395,31 @@ void bind_sets(const fdtd_ctx *c, KArgs &a)
     a.npsi_eyz = c->psi_set[w][9];
 }
 
+// Tile box of a sweep (tiles of tile_cols x tile_rows() cells, z-blocks of
+// kchunk planes) whose cells all lie off the CPML slabs and off the outer
+// faces: there the interior variant of the sweep runs, with no slab or face
+// predicates in its code.  FDTD_INTERIOR=0 disables the split.
+void interior_box(const fdtd_ctx *c, int tile_cols, int kchunk, int ib[6])
+{
+    for (int q = 0; q < 6; ++q) ib[q] = 0;
+    const char *ev = std::getenv("FDTD_INTERIOR");
+    if (ev && std::strcmp(ev, "0") == 0) return;
+    const SceneIn &in = c->in;
+    const int *L = in.cpml.thick;
+    const int i0 = std::max(L[0], 1), i1 = in.nx - std::max(L[1], 1);
+    const int j0 = std::max(L[2], 1), j1 = in.ny - std::max(L[3], 1);
+    const int k0 = std::max(L[4], 1), k1 = in.nz - std::max(L[5], 1);
+    const int ty = tile_rows();
+    ib[0] = (i0 + tile_cols - 1) / tile_cols;
+    ib[3] = std::max(ib[0], i1 / tile_cols);
+    ib[1] = (j0 + ty - 1) / ty;
+    ib[4] = std::max(ib[1], j1 / ty);
+    ib[2] = (k0 + kchunk - 1) / kchunk;
+    ib[5] = std::max(ib[2], k1 / kchunk);
+    if (ib[3] == ib[0] || ib[4] == ib[1] || ib[5] == ib[2])
+        for (int q = 0; q < 6; ++q) ib[q] = 0;
+}
+
 // One time step m.  base carries the per-call flags (rec, snapshot); the
 // tile geometry and buffer pointers are filled in here.
 hipError_t do_step(fdtd_ctx *c, const KArgs &base, int m, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
@@ -381,11 +444,18 @@ hipError_t do_step(fdtd_ctx *c, const KArgs &base, int m, hipEvent_t ev0 = nullp
     a.kchunk = c->pair ? c->pair_kchunk : c->ka.kchunk;
     bind_sets(c, a);
     auto half = c->pair ? launch_pair_half_step : launch_half_step;
+    auto split = [&](int sweep) {
+        for (int q = 0; q < 6; ++q) a.ib[q] = c->box[sweep][q];
+        a.glist = c->glist[sweep];
+        a.nglist = c->nglist[sweep];
+    };
+    split(0);
     if (ev0) (void)hipEventRecord(ev0, c->stream);
     if ((e = half(true, c->in.storage, c->e_mode, a, m, c->stream)) != hipSuccess) return e;
     if (ev1) (void)hipEventRecord(ev1, c->stream);
     if (ev2) (void)hipEventRecord(ev2, c->stream);
     if (!c->pair) a.kchunk = c->kchunk_h;
+    split(1);
     e = half(false, c->in.storage, c->h_mode, a, m, c->stream);
     if (ev3) (void)hipEventRecord(ev3, c->stream);
     return e;

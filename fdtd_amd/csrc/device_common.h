@@ -76,6 +76,41 @@ __device__ __forceinline__ void tile_of(const KArgs &a, int &tx, int &ty, int &t
 }
 
 
+// Workgroup -> tile of the interior box, with the same XCD-aware remap.
+__device__ __forceinline__ void tile_of_box(const KArgs &a, int &tx, int &ty, int &tz)
+{
+    const int bx = a.ib[3] - a.ib[0], by = a.ib[4] - a.ib[1];
+    const int nt = bx * by * (a.ib[5] - a.ib[2]);
+    const int b = blockIdx.x;
+    const int q = nt >> 3, r = nt & 7, x = b & 7;
+    const int t = x * q + (x < r ? x : r) + (b >> 3);
+    tx = a.ib[0] + t % bx;
+    ty = a.ib[1] + (t / bx) % by;
+    tz = a.ib[2] + t / (bx * by);
+}
+
+// Workgroup -> tile of the general variant: from its tile list when there
+// is an interior box, else the whole grid.
+__device__ __forceinline__ void tile_of_general(const KArgs &a, int &tx, int &ty, int &tz)
+{
+    if (a.nglist == 0) {
+        tile_of(a, tx, ty, tz);
+        return;
+    }
+    const int nt = a.nglist;
+    const int b = blockIdx.x;
+    const int q = nt >> 3, r = nt & 7, x = b & 7;
+    const int t = a.glist[x * q + (x < r ? x : r) + (b >> 3)];
+    tx = t % a.tiles_x;
+    ty = (t / a.tiles_x) % a.tiles_y;
+    tz = t / (a.tiles_x * a.tiles_y);
+}
+
+__device__ __forceinline__ bool in_box(const KArgs &a, int tx, int ty, int tz)
+{
+    return tx >= a.ib[0] && tx < a.ib[3] && ty >= a.ib[1] && ty < a.ib[4] && tz >= a.ib[2] && tz < a.ib[5];
+}
+
 __device__ __forceinline__ int first_event(const KArgs &a, const int *first, int col, int k0)
 {
     int ev = first[col];

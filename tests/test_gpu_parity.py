@@ -62,8 +62,19 @@ def scene_design_u32():
                           eps_range=(1.0, 4.0), seed=31, name="design-u32")
 
 
+def scene_wide():
+    """Wide enough in x for an interior tile box in both the 64-column and
+    the 128-column (16-bit pair) sweeps, so both kernel variants run; a
+    source and probes inside the box, probes in the CPML too."""
+    sc = scenes.dielectric(nx=300, ny=40, nz=36, steps=100, L=5, n_boxes=6, n_src=2, n_probes=8, n_freqs=2,
+                           seed=41, name="wide")
+    sc.sources.append(scenes.z_edge(150, 20, 18, sc.dt))
+    sc.probes += [(EZ, 140, 20, 18), (EX, 100, 12, 10), (HY, 200, 30, 20), (EZ, 2, 20, 18), (HX, 298, 20, 18)]
+    return sc
+
+
 SCENES = {"vacuum": scene_vacuum, "dielectric": scene_dielectric, "magnetic": scene_magnetic,
-          "design_u32": scene_design_u32}
+          "design_u32": scene_design_u32, "wide": scene_wide}
 
 
 @pytest.fixture(scope="module")
@@ -123,6 +134,25 @@ def test_kernel_paths_and_chunking_bitwise(path, engine, oracle_mod, monkeypatch
     print(f"[parity] kernel={kernel} chunks={chunks}", flush=True)
     sim = engine.Simulation.from_scene(sc, "f32")
     assert sim.info()["fused"] == (kernel == "fused")
+    sim.run(sc.steps)
+    o32 = oracle_mod.Oracle(sc, 32)
+    o32.steps(sc.steps)
+    np.testing.assert_array_equal(sim.probes().astype(np.float64), o32.probes())
+    for c, nm in enumerate(FIELDS):
+        np.testing.assert_array_equal(sim.field(c), o32.field(nm).astype(np.float32))
+
+
+@pytest.mark.parametrize("interior,chunks", [(None, None), ("0", None), (None, "5"), (None, "36")])
+def test_interior_split_and_z_blocks_bitwise(interior, chunks, engine, oracle_mod, monkeypatch):
+    """The interior/general split of each sweep and any z-blocking give the
+    FP32 oracle's results bit for bit (wide scene: the box is non-empty)."""
+    if interior:
+        monkeypatch.setenv("FDTD_INTERIOR", interior)
+    if chunks:
+        monkeypatch.setenv("FDTD_CHUNKS", chunks)
+    sc = scene_wide()
+    print(f"[parity] wide interior={interior} chunks={chunks}", flush=True)
+    sim = engine.Simulation.from_scene(sc, "f32")
     sim.run(sc.steps)
     o32 = oracle_mod.Oracle(sc, 32)
     o32.steps(sc.steps)
