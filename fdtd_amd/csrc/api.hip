@@ -64,6 +64,8 @@ struct fdtd_ctx {
     int kchunk_h = 0;               // planes per block of the one-cell H sweep (E: ka.kchunk)
     // per sweep (0 = E, 1 = H): interior tile box and the general variant's tile list
     int box[2][6] = {};
+    const int *xlist[2] = {nullptr, nullptr};
+    int nxlist[2] = {0, 0};
     const int *glist[2] = {nullptr, nullptr};
     int nglist[2] = {0, 0};
     KArgs ka_step{};                // tile geometry of the step kernel
@@ -188,28 +190,41 @@ int choose_chunks(const fdtd_ctx *c, int tiles, int planes)
 
 void interior_box(const fdtd_ctx *c, int tile_cols, int kchunk, int ib[6]);
 
-// Interior box and general tile list of one sweep (tiles tile_cols wide,
-// z-blocks of kchunk planes).
+// Interior box and the x-edge and general tile lists of one sweep (tiles
+// tile_cols wide, z-blocks of kchunk planes).  FDTD_XEDGE=0 leaves the
+// x-edge tiles to the general variant.
 int sweep_split(fdtd_ctx *c, int sweep, int tile_cols, int kchunk)
 {
     int *ib = c->box[sweep];
     interior_box(c, tile_cols, kchunk, ib);
-    c->glist[sweep] = nullptr;
-    c->nglist[sweep] = 0;
+    c->xlist[sweep] = c->glist[sweep] = nullptr;
+    c->nxlist[sweep] = c->nglist[sweep] = 0;
     if (ib[3] == ib[0]) return FDTD_OK;
+    const char *xe = std::getenv("FDTD_XEDGE");
+    const bool xedge = !(xe && std::strcmp(xe, "0") == 0);
     const int tx = (c->in.nx + tile_cols - 1) / tile_cols, ty = c->ka.tiles_y;
     const int tz = (c->in.nz + kchunk - 1) / kchunk;
-    std::vector<int> list;
+    std::vector<int> xl, gl;
     for (int z = 0; z < tz; ++z)
         for (int y = 0; y < ty; ++y)
-            for (int x = 0; x < tx; ++x)
-                if (!(x >= ib[0] && x < ib[3] && y >= ib[1] && y < ib[4] && z >= ib[2] && z < ib[5]))
-                    list.push_back(x + tx * (y + ty * z));
+            for (int x = 0; x < tx; ++x) {
+                const bool yz_in = y >= ib[1] && y < ib[4] && z >= ib[2] && z < ib[5];
+                const bool x_in = x >= ib[0] && x < ib[3];
+                if (yz_in && x_in) continue;
+                (yz_in && xedge ? xl : gl).push_back(x + tx * (y + ty * z));
+            }
     void *d = nullptr;
-    int rc = upload(c, &d, list.data(), list.size() * sizeof(int));
-    if (rc) return rc;
-    c->glist[sweep] = (const int *)d;
-    c->nglist[sweep] = (int)list.size();
+    int rc;
+    if (!xl.empty()) {
+        if ((rc = upload(c, &d, xl.data(), xl.size() * sizeof(int)))) return rc;
+        c->xlist[sweep] = (const int *)d;
+        c->nxlist[sweep] = (int)xl.size();
+    }
+    if (!gl.empty()) {
+        if ((rc = upload(c, &d, gl.data(), gl.size() * sizeof(int)))) return rc;
+        c->glist[sweep] = (const int *)d;
+        c->nglist[sweep] = (int)gl.size();
+    }
     return FDTD_OK;
 }
 
@@ -446,6 +461,8 @@ hipError_t do_step(fdtd_ctx *c, const KArgs &base, int m, hipEvent_t ev0 = nullp
     auto half = c->pair ? launch_pair_half_step : launch_half_step;
     auto split = [&](int sweep) {
         for (int q = 0; q < 6; ++q) a.ib[q] = c->box[sweep][q];
+        a.xlist = c->xlist[sweep];
+        a.nxlist = c->nxlist[sweep];
         a.glist = c->glist[sweep];
         a.nglist = c->nglist[sweep];
     };

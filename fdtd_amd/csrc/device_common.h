@@ -89,26 +89,29 @@ __device__ __forceinline__ void tile_of_box(const KArgs &a, int &tx, int &ty, in
     tz = a.ib[2] + t / (bx * by);
 }
 
-// Workgroup -> tile of the general variant: from its tile list when there
-// is an interior box, else the whole grid.
-__device__ __forceinline__ void tile_of_general(const KArgs &a, int &tx, int &ty, int &tz)
+// Workgroup -> tile of a tile list, with the same XCD-aware remap.
+__device__ __forceinline__ void tile_of_list(const KArgs &a, const int *list, int nt, int &tx, int &ty, int &tz)
 {
-    if (a.nglist == 0) {
-        tile_of(a, tx, ty, tz);
-        return;
-    }
-    const int nt = a.nglist;
     const int b = blockIdx.x;
     const int q = nt >> 3, r = nt & 7, x = b & 7;
-    const int t = a.glist[x * q + (x < r ? x : r) + (b >> 3)];
+    const int t = list[x * q + (x < r ? x : r) + (b >> 3)];
     tx = t % a.tiles_x;
     ty = (t / a.tiles_x) % a.tiles_y;
     tz = t / (a.tiles_x * a.tiles_y);
 }
 
-__device__ __forceinline__ bool in_box(const KArgs &a, int tx, int ty, int tz)
+// Sweep-kernel variants by the region of their tiles.
+enum SweepRegion { V_IN = 0, V_XE = 1, V_GEN = 2 };
+
+// Workgroup -> tile of variant V: the interior box, the x-edge list, or the
+// general list (the whole grid when there is no box).
+template <int V>
+__device__ __forceinline__ void tile_of_region(const KArgs &a, int &tx, int &ty, int &tz)
 {
-    return tx >= a.ib[0] && tx < a.ib[3] && ty >= a.ib[1] && ty < a.ib[4] && tz >= a.ib[2] && tz < a.ib[5];
+    if (V == V_IN) tile_of_box(a, tx, ty, tz);
+    else if (V == V_XE) tile_of_list(a, a.xlist, a.nxlist, tx, ty, tz);
+    else if (a.nglist == 0) tile_of(a, tx, ty, tz);
+    else tile_of_list(a, a.glist, a.nglist, tx, ty, tz);
 }
 
 __device__ __forceinline__ int first_event(const KArgs &a, const int *first, int col, int k0)

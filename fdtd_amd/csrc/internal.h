@@ -78,13 +78,17 @@ struct KArgs {
     int tiles_x, tiles_y;      // tile grid (for the XCD-aware remap)
     // Interior tile box [ib0, ib3) x [ib1, ib4) x [ib2, ib5) of (tile x,
     // tile y, z-block): every cell of it lies off the CPML slabs and the
-    // outer faces.  The interior variant of a sweep covers exactly the box,
-    // the general variant every other tile (empty box: the general variant
-    // covers everything).
+    // outer faces.  Three variants of a sweep share the grid: the interior
+    // variant covers exactly the box; the x-edge variant the tiles beside it
+    // in x (same tile rows and z-blocks), which can meet only the x-slabs and
+    // x-faces; the general variant every other tile (empty box: the general
+    // variant covers everything).
     int ib[6];
-    // Tiles of the general variant when the box is not empty (flat index
-    // tx + tiles_x (ty + tiles_y tz)), so that they spread evenly over the
-    // XCDs instead of leaving the box's share of the grid idle.
+    // Tile lists of the x-edge and general variants when the box is not
+    // empty (flat index tx + tiles_x (ty + tiles_y tz)), so that they spread
+    // evenly over the XCDs.
+    const int *xlist;
+    int nxlist;
     const int *glist;
     int nglist;
     int rec;                   // record probes / DFT (0 during adjoint recompute)

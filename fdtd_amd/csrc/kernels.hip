@@ -77,7 +77,7 @@ __device__ __forceinline__ bool e_zslab(const KArgs &a, int k)
     return (k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && k < a.nz - 1);   // fdtdrav.h:853,879
 }
 
-template <class T, int CM, bool IN>
+template <class T, int CM, int V>
 __device__ __forceinline__ void e_fetch(EPlane &p, const KArgs &a, const ECol &cc, const T *ex, const T *ey,
                                         const T *ez, const T *hx, const T *hy, const T *hz, int k, int k1)
 {
@@ -100,7 +100,7 @@ __device__ __forceinline__ void e_fetch(EPlane &p, const KArgs &a, const ECol &c
     p.cln = (CM != CLS_UNIFORM && k + 1 < k1)
                 ? class_at<CM>(a.cls, (std::uint32_t)(k + 1) * (std::uint32_t)a.plane + (std::uint32_t)cc.col)
                 : 0;
-    const bool khi = IN || k < a.nz - 1;
+    const bool khi = V != V_GEN || k < a.nz - 1;
     p.p_yx = p.p_zx = p.p_xy = p.p_zy = p.p_xz = p.p_yz = 0.f;
     if (cc.xs && cc.jhi && khi) {
         const std::uint32_t q = (((std::uint32_t)k * ny + cc.j) * ra.Lx + cc.sx) * S;
@@ -112,7 +112,7 @@ __device__ __forceinline__ void e_fetch(EPlane &p, const KArgs &a, const ECol &c
         p.p_xy = ld(at((const T *)ra.psi_exy, q));
         p.p_zy = ld(at((const T *)ra.psi_ezy, q));
     }
-    if (!IN && e_zslab(a, k) && cc.ihi && cc.jhi) {
+    if (V == V_GEN && e_zslab(a, k) && cc.ihi && cc.jhi) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
         const std::uint32_t q = (((std::uint32_t)sz * ny + cc.j) * px + cc.i) * S;
         p.p_xz = ld(at((const T *)ra.psi_exz, q));
@@ -120,7 +120,7 @@ __device__ __forceinline__ void e_fetch(EPlane &p, const KArgs &a, const ECol &c
     }
 }
 
-template <class T, int CM, bool IN>
+template <class T, int CM, int V>
 __device__ __forceinline__ void e_update(const EPlane &p, const Coef &t, const KArgs &a, const ECol &cc, T *ex,
                                          T *ey, T *ez, int k, int m, float &hx_km, float &hy_km, int &ev,
                                          int &ek)
@@ -133,11 +133,11 @@ __device__ __forceinline__ void e_update(const EPlane &p, const Coef &t, const K
     const float hy_im = p.hy_im, hz_im = p.hz_im;
     const float hx_jm = p.hx_jm, hz_jm = p.hz_jm;
     const float4 cx = t.x, cy = t.y, cz = t.z;
-    const bool klo = IN || k > 0, khi = IN || k < nz - 1;
+    const bool klo = V != V_GEN || k > 0, khi = V != V_GEN || k < nz - 1;
     const bool upd_x = cc.ihi && cc.jlo && cc.jhi && klo && khi;
     const bool upd_y = cc.ilo && cc.ihi && cc.jhi && klo && khi;
     const bool upd_z = cc.ilo && cc.ihi && cc.jlo && cc.jhi && khi;
-    const bool zs = !IN && e_zslab(a, k);
+    const bool zs = V == V_GEN && e_zslab(a, k);
 
     // curl coefficients, divided by kappa on the slab cells (setABCPML)
     float cbx_hz = cx.y, cbx_hy = cx.z;
@@ -201,19 +201,15 @@ __device__ __forceinline__ void e_update(const EPlane &p, const Coef &t, const K
     if (cc.in_snap && k >= a.sb[2] && k < a.sb[2] + a.sb[5])   // stored-precision value, as the next step reads it
         e_snapshot(ra, cc.i, cc.j, k, m, upd_x ? rnd(exn, ex) : ex0, upd_y ? rnd(eyn, ex) : ey0,
                    upd_z ? rnd(ezn, ex) : ez0);
-    hx_km = hxc;
-    hy_km = hyc;
+    hx_km = carry(hxc);
+    hy_km = carry(hyc);
 }
 
-template <class T, int CM, bool IN>
+template <class T, int CM, int V>
 __global__ void __launch_bounds__(TX * BY) E_OCC e_kernel(const KArgs a, const int m)
 {
     int tx, ty, tz;
-    if (IN) {
-        tile_of_box(a, tx, ty, tz);
-    } else {
-        tile_of_general(a, tx, ty, tz);
-    }
+    tile_of_region<V>(a, tx, ty, tz);
     ECol cc;
     cc.i = tx * TX + (int)(threadIdx.x & (TX - 1));
     cc.j = ty * BY + (int)(threadIdx.x / TX);
@@ -236,12 +232,12 @@ __global__ void __launch_bounds__(TX * BY) E_OCC e_kernel(const KArgs a, const i
     const T *__restrict__ hz = (const T *)a.hz - a.plane;
 
     // column-constant predicates (reference region guards)
-    cc.ilo = IN || i > 0;
-    cc.ihi = IN || i < nx - 1;
-    cc.jlo = IN || j > 0;
-    cc.jhi = IN || j < ny - 1;
-    cc.xs = !IN && ((cc.ilo && i < a.L[0]) || (i >= nx - a.L[1] && cc.ihi));   // E x-slab, fdtdrav.h:752,777
-    cc.ys = !IN && ((cc.jlo && j < a.L[2]) || (j >= ny - a.L[3] && cc.jhi));   // E y-slab, fdtdrav.h:803,827
+    cc.ilo = V == V_IN || i > 0;
+    cc.ihi = V == V_IN || i < nx - 1;
+    cc.jlo = V != V_GEN || j > 0;
+    cc.jhi = V != V_GEN || j < ny - 1;
+    cc.xs = V != V_IN && ((cc.ilo && i < a.L[0]) || (i >= nx - a.L[1] && cc.ihi));   // E x-slab, fdtdrav.h:752,777
+    cc.ys = V == V_GEN && ((cc.jlo && j < a.L[2]) || (j >= ny - a.L[3] && cc.jhi));   // E y-slab, fdtdrav.h:803,827
     cc.sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
     cc.sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
     cc.pxe = cc.xs ? a.px_prof[i] : make_float4(0.f, 0.f, 1.f, 0.f);
@@ -263,26 +259,26 @@ __global__ void __launch_bounds__(TX * BY) E_OCC e_kernel(const KArgs a, const i
     } else {
         tA = coef_of<CM>(a.etab, class_at<CM>(a.cls, (std::uint32_t)k0 * (std::uint32_t)a.plane + (std::uint32_t)cc.col));
     }
-    e_fetch<T, CM, IN>(A, a, cc, ex, ey, ez, hx, hy, hz, k0, k1);
+    e_fetch<T, CM, V>(A, a, cc, ex, ey, ez, hx, hy, hz, k0, k1);
     int k = k0;
     // Steady state: the fetches are unconditional, so on every path the
     // next plane's loads are the younger ones when this plane's are awaited.
     for (; k + 2 < k1; k += 2) {
-        e_fetch<T, CM, IN>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        e_fetch<T, CM, V>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_of<CM>(a.etab, A.cln);
-        e_update<T, CM, IN>(A, tA, a, cc, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
-        e_fetch<T, CM, IN>(A, a, cc, ex, ey, ez, hx, hy, hz, k + 2, k1);
+        e_update<T, CM, V>(A, tA, a, cc, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
+        e_fetch<T, CM, V>(A, a, cc, ex, ey, ez, hx, hy, hz, k + 2, k1);
         if (CM != CLS_UNIFORM) tA = coef_of<CM>(a.etab, B.cln);
-        e_update<T, CM, IN>(B, tB, a, cc, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
+        e_update<T, CM, V>(B, tB, a, cc, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
     }
     // last one or two planes of the chunk
     if (k + 1 < k1) {
-        e_fetch<T, CM, IN>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        e_fetch<T, CM, V>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_of<CM>(a.etab, A.cln);
-        e_update<T, CM, IN>(A, tA, a, cc, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
-        e_update<T, CM, IN>(B, tB, a, cc, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
+        e_update<T, CM, V>(A, tA, a, cc, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
+        e_update<T, CM, V>(B, tB, a, cc, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
     } else {
-        e_update<T, CM, IN>(A, tA, a, cc, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
+        e_update<T, CM, V>(A, tA, a, cc, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
     }
 }
 
@@ -310,7 +306,7 @@ __device__ __forceinline__ bool h_zslab(const KArgs &a, int k)
     return k < a.L[4] || (k >= a.nz - a.L[5] && k < a.nz - 1);   // fdtdrav.h:500,531
 }
 
-template <class T, int CM, bool IN>
+template <class T, int CM, int V>
 __device__ __forceinline__ void h_fetch(HPlane &p, const KArgs &a, const HCol &cc, const T *ex, const T *ey,
                                         const T *ez, const T *hx, const T *hy, const T *hz, int k, int k1)
 {
@@ -344,7 +340,7 @@ __device__ __forceinline__ void h_fetch(HPlane &p, const KArgs &a, const HCol &c
         p.p_xy = ld(at((const T *)ra.psi_hxy, q));
         p.p_zy = ld(at((const T *)ra.psi_hzy, q));
     }
-    if (!IN && h_zslab(a, k)) {
+    if (V == V_GEN && h_zslab(a, k)) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
         const std::uint32_t q = (((std::uint32_t)sz * ny + cc.j) * px + cc.i) * S;
         p.p_xz = ld(at((const T *)ra.psi_hxz, q));
@@ -352,7 +348,7 @@ __device__ __forceinline__ void h_fetch(HPlane &p, const KArgs &a, const HCol &c
     }
 }
 
-template <class T, int CM, bool IN>
+template <class T, int CM, int V>
 __device__ __forceinline__ void h_update(const HPlane &p, const Coef &t, const KArgs &a, const HCol &cc, T *hx,
                                          T *hy, T *hz, int k, int m, float &ex_c, float &ey_c, int &ev, int &ek)
 {
@@ -365,11 +361,11 @@ __device__ __forceinline__ void h_update(const HPlane &p, const Coef &t, const K
     const float ez_ip = p.ez_ip, ey_ip = p.ey_ip;
     const float ez_jp = p.ez_jp, ex_jp = p.ex_jp;
     const float4 cx = t.x, cy = t.y, cz = t.z;
-    const bool khi = IN || k < nz - 1;
+    const bool khi = V != V_GEN || k < nz - 1;
     const bool upd_x = cc.jhi && khi;
     const bool upd_y = cc.ihi && khi;
     const bool upd_z = cc.ihi && cc.jhi;
-    const bool zs = !IN && h_zslab(a, k);
+    const bool zs = V == V_GEN && h_zslab(a, k);
 
     float chx_ey = cx.y, chx_ez = cx.z;
     float chy_ez = cy.y, chy_ex = cy.z;
@@ -427,19 +423,15 @@ __device__ __forceinline__ void h_update(const HPlane &p, const Coef &t, const K
     if (upd_x) stn(at(hx, o), hxn);
     if (upd_y) stn(at(hy, o), hyn);
     if (upd_z) stn(at(hz, o), hzn);
-    ex_c = ex_kp;
-    ey_c = ey_kp;
+    ex_c = carry(ex_kp);
+    ey_c = carry(ey_kp);
 }
 
-template <class T, int CM, bool IN>
+template <class T, int CM, int V>
 __global__ void __launch_bounds__(TX * BY) H_OCC h_kernel(const KArgs a, const int m)
 {
     int tx, ty, tz;
-    if (IN) {
-        tile_of_box(a, tx, ty, tz);
-    } else {
-        tile_of_general(a, tx, ty, tz);
-    }
+    tile_of_region<V>(a, tx, ty, tz);
     HCol cc;
     cc.i = tx * TX + (int)(threadIdx.x & (TX - 1));
     cc.j = ty * BY + (int)(threadIdx.x / TX);
@@ -461,10 +453,10 @@ __global__ void __launch_bounds__(TX * BY) H_OCC h_kernel(const KArgs a, const i
     T *__restrict__ hy = (T *)a.hy - a.plane;
     T *__restrict__ hz = (T *)a.hz - a.plane;
 
-    cc.ihi = IN || i < nx - 1;
-    cc.jhi = IN || j < ny - 1;
-    cc.xs = !IN && (i < a.L[0] || (i >= nx - a.L[1] && cc.ihi));   // H x-slab, fdtdrav.h:402,424
-    cc.ys = !IN && (j < a.L[2] || (j >= ny - a.L[3] && cc.jhi));   // H y-slab, fdtdrav.h:447,471
+    cc.ihi = V == V_IN || i < nx - 1;
+    cc.jhi = V != V_GEN || j < ny - 1;
+    cc.xs = V != V_IN && (i < a.L[0] || (i >= nx - a.L[1] && cc.ihi));   // H x-slab, fdtdrav.h:402,424
+    cc.ys = V == V_GEN && (j < a.L[2] || (j >= ny - a.L[3] && cc.jhi));   // H y-slab, fdtdrav.h:447,471
     cc.sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
     cc.sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
     cc.pxm = cc.xs ? a.px_prof[nx + i] : make_float4(0.f, 0.f, 1.f, 0.f);
@@ -484,51 +476,55 @@ __global__ void __launch_bounds__(TX * BY) H_OCC h_kernel(const KArgs a, const i
     } else {
         tA = coef_of<CM>(a.htab, class_at<CM>(a.cls, (std::uint32_t)k0 * (std::uint32_t)a.plane + (std::uint32_t)cc.col));
     }
-    h_fetch<T, CM, IN>(A, a, cc, ex, ey, ez, hx, hy, hz, k0, k1);
+    h_fetch<T, CM, V>(A, a, cc, ex, ey, ez, hx, hy, hz, k0, k1);
     int k = k0;
     for (; k + 2 < k1; k += 2) {
-        h_fetch<T, CM, IN>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        h_fetch<T, CM, V>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_of<CM>(a.htab, A.cln);
-        h_update<T, CM, IN>(A, tA, a, cc, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
-        h_fetch<T, CM, IN>(A, a, cc, ex, ey, ez, hx, hy, hz, k + 2, k1);
+        h_update<T, CM, V>(A, tA, a, cc, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
+        h_fetch<T, CM, V>(A, a, cc, ex, ey, ez, hx, hy, hz, k + 2, k1);
         if (CM != CLS_UNIFORM) tA = coef_of<CM>(a.htab, B.cln);
-        h_update<T, CM, IN>(B, tB, a, cc, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
+        h_update<T, CM, V>(B, tB, a, cc, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
     }
     if (k + 1 < k1) {
-        h_fetch<T, CM, IN>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        h_fetch<T, CM, V>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_of<CM>(a.htab, A.cln);
-        h_update<T, CM, IN>(A, tA, a, cc, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
-        h_update<T, CM, IN>(B, tB, a, cc, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
+        h_update<T, CM, V>(A, tA, a, cc, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
+        h_update<T, CM, V>(B, tB, a, cc, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
     } else {
-        h_update<T, CM, IN>(A, tA, a, cc, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
+        h_update<T, CM, V>(A, tA, a, cc, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
     }
 }
 
 // ---- launchers --------------------------------------------------------------
-template <class T, bool IN>
+template <class T, int V>
 static void launch_v(bool efield, int mode, const KArgs &a, int m, int nt, hipStream_t s)
 {
     dim3 grid(nt), block(TX * BY);
     if (efield) {
-        if (mode == CLS_UNIFORM) hipLaunchKernelGGL((e_kernel<T, CLS_UNIFORM, IN>), grid, block, 0, s, a, m);
-        else if (mode == CLS_U16) hipLaunchKernelGGL((e_kernel<T, CLS_U16, IN>), grid, block, 0, s, a, m);
-        else hipLaunchKernelGGL((e_kernel<T, CLS_U32, IN>), grid, block, 0, s, a, m);
+        if (mode == CLS_UNIFORM) hipLaunchKernelGGL((e_kernel<T, CLS_UNIFORM, V>), grid, block, 0, s, a, m);
+        else if (mode == CLS_U16) hipLaunchKernelGGL((e_kernel<T, CLS_U16, V>), grid, block, 0, s, a, m);
+        else hipLaunchKernelGGL((e_kernel<T, CLS_U32, V>), grid, block, 0, s, a, m);
     } else {
-        if (mode == CLS_UNIFORM) hipLaunchKernelGGL((h_kernel<T, CLS_UNIFORM, IN>), grid, block, 0, s, a, m);
-        else if (mode == CLS_U16) hipLaunchKernelGGL((h_kernel<T, CLS_U16, IN>), grid, block, 0, s, a, m);
-        else hipLaunchKernelGGL((h_kernel<T, CLS_U32, IN>), grid, block, 0, s, a, m);
+        if (mode == CLS_UNIFORM) hipLaunchKernelGGL((h_kernel<T, CLS_UNIFORM, V>), grid, block, 0, s, a, m);
+        else if (mode == CLS_U16) hipLaunchKernelGGL((h_kernel<T, CLS_U16, V>), grid, block, 0, s, a, m);
+        else hipLaunchKernelGGL((h_kernel<T, CLS_U32, V>), grid, block, 0, s, a, m);
     }
 }
 
-// Interior variant over the box a.ib, general variant over every tile
-// (skipping the box).
+// Interior variant over the box a.ib, x-edge and general variants over
+// their tile lists; without a box the general variant covers every tile.
 template <class T>
 static hipError_t launch_t(bool efield, int mode, const KArgs &a, int m, hipStream_t s)
 {
-    const int nt = a.tiles_x * a.tiles_y * ((a.nz + a.kchunk - 1) / a.kchunk);
     const int nb = (a.ib[3] - a.ib[0]) * (a.ib[4] - a.ib[1]) * (a.ib[5] - a.ib[2]);
-    if (nb > 0) launch_v<T, true>(efield, mode, a, m, nb, s);
-    launch_v<T, false>(efield, mode, a, m, a.nglist ? a.nglist : nt, s);
+    if (nb > 0) {
+        launch_v<T, V_IN>(efield, mode, a, m, nb, s);
+        if (a.nxlist > 0) launch_v<T, V_XE>(efield, mode, a, m, a.nxlist, s);
+        if (a.nglist > 0) launch_v<T, V_GEN>(efield, mode, a, m, a.nglist, s);
+    } else {
+        launch_v<T, V_GEN>(efield, mode, a, m, a.tiles_x * a.tiles_y * ((a.nz + a.kchunk - 1) / a.kchunk), s);
+    }
     return hipGetLastError();
 }
 

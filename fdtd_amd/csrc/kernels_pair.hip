@@ -132,7 +132,7 @@ struct EPair {
     int cla, clb;                                // class indices of plane k+1
 };
 
-template <class T, int CM, bool IN>
+template <class T, int CM, int V>
 __device__ __forceinline__ void ep_fetch(EPair &p, const KArgs &a, const PRow &r, const PCell (&c)[2], const T *ex,
                                          const T *ey, const T *ez, const T *hx, const T *hy, const T *hz, int k,
                                          int k1)
@@ -154,7 +154,7 @@ __device__ __forceinline__ void ep_fetch(EPair &p, const KArgs &a, const PRow &r
     p.cla = p.clb = 0;
     if (CM != CLS_UNIFORM && k + 1 < k1)
         class_pair<CM>(a.cls, (u32)(k + 1) * (u32)a.plane + (u32)r.col, p.cla, p.clb);
-    const bool khi = IN || k < a.nz - 1;
+    const bool khi = V != V_GEN || k < a.nz - 1;
     for (int q = 0; q < 2; ++q) {
         p.p_yx[q] = p.p_zx[q] = p.p_xy[q] = p.p_zy[q] = p.p_xz[q] = p.p_yz[q] = 0.f;
         if (c[q].xs && r.jhi && khi) {
@@ -171,7 +171,7 @@ __device__ __forceinline__ void ep_fetch(EPair &p, const KArgs &a, const PRow &r
         p.p_zy[0] = lo_f(v, ex);
         p.p_zy[1] = hi_f(v, ex);
     }
-    if (!IN && ((k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && khi)) && r.jhi) {
+    if (V == V_GEN && ((k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && khi)) && r.jhi) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
         const u32 qq = (((u32)sz * ra.ny + r.j) * ra.px + c[0].i) * S;
         const u32 u = ld2((const T *)ra.psi_exz, qq), v = ld2((const T *)ra.psi_eyz, qq);
@@ -183,7 +183,7 @@ __device__ __forceinline__ void ep_fetch(EPair &p, const KArgs &a, const PRow &r
 }
 
 // E update of one cell of the pair; the arithmetic of kernels.hip e_update.
-template <class T, bool IN>
+template <class T, int V>
 __device__ __forceinline__ void e_cell(const KArgs &a, const PRow &r, const PCell &c, int q, const EPair &p,
                                        const Coef &t, int k, int m, float hxc, float hyc, float hzc, float hy_im,
                                        float hz_im, float hx_jm, float hz_jm, float ex0, float ey0, float ez0,
@@ -193,7 +193,7 @@ __device__ __forceinline__ void e_cell(const KArgs &a, const PRow &r, const PCel
     ConstKArgs &ra = rare_args(a);
     const int nz = a.nz;
     const float4 cx = t.x, cy = t.y, cz = t.z;
-    const bool klo = IN || k > 0, khi = IN || k < nz - 1;
+    const bool klo = V != V_GEN || k > 0, khi = V != V_GEN || k < nz - 1;
     ux = c.valid && c.ihi && r.jlo && r.jhi && klo && khi;
     uy = c.valid && c.ilo && c.ihi && r.jhi && klo && khi;
     uz = c.valid && c.ilo && c.ihi && r.jlo && r.jhi && khi;
@@ -255,42 +255,38 @@ __device__ __forceinline__ void e_cell(const KArgs &a, const PRow &r, const PCel
     }
 }
 
-template <class T, int CM, bool IN>
+template <class T, int CM, int V>
 __device__ __forceinline__ void ep_update(const EPair &p, const CoefPair &t, const KArgs &a, const PRow &r,
                                           const PCell (&c)[2], T *ex, T *ey, T *ez, int k, int m, u32 &hx_km,
                                           u32 &hy_km, int (&ev)[2], int (&ek)[2])
 {
     const T *tp = nullptr;
     const u32 o = r.o0 + (u32)k * r.ob;
-    const bool zs = !IN && ((k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && k < a.nz - 1));   // fdtdrav.h:853,879
+    const bool zs = V == V_GEN && ((k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && k < a.nz - 1));   // fdtdrav.h:853,879
     const float4 pze = zs ? uload(a.pz_prof + k) : make_float4(0.f, 0.f, 1.f, 0.f);
     const float hx0 = lo_f(p.hx, tp), hx1 = hi_f(p.hx, tp);
     const float hy0 = lo_f(p.hy, tp), hy1 = hi_f(p.hy, tp);
     const float hz0 = lo_f(p.hz, tp), hz1 = hi_f(p.hz, tp);
     float ex_a, ey_a, ez_a, ex_b, ey_b, ez_b;
     bool uxa, uya, uza, uxb, uyb, uzb;
-    e_cell<T, IN>(a, r, c[0], 0, p, t.a, k, m, hx0, hy0, hz0, p.hy_im, p.hz_im, lo_f(p.hx_jm, tp), lo_f(p.hz_jm, tp),
+    e_cell<T, V>(a, r, c[0], 0, p, t.a, k, m, hx0, hy0, hz0, p.hy_im, p.hz_im, lo_f(p.hx_jm, tp), lo_f(p.hz_jm, tp),
               lo_f(p.ex, tp), lo_f(p.ey, tp), lo_f(p.ez, tp), lo_f(hx_km, tp), lo_f(hy_km, tp), ev[0], ek[0], zs,
               pze, ex_a, ey_a, ez_a, uxa, uya, uza);
-    e_cell<T, IN>(a, r, c[1], 1, p, t.b, k, m, hx1, hy1, hz1, hy0, hz0, hi_f(p.hx_jm, tp), hi_f(p.hz_jm, tp),
+    e_cell<T, V>(a, r, c[1], 1, p, t.b, k, m, hx1, hy1, hz1, hy0, hz0, hi_f(p.hx_jm, tp), hi_f(p.hz_jm, tp),
               hi_f(p.ex, tp), hi_f(p.ey, tp), hi_f(p.ez, tp), hi_f(hx_km, tp), hi_f(hy_km, tp), ev[1], ek[1], zs,
               pze, ex_b, ey_b, ez_b, uxb, uyb, uzb);
     if (uxa || uxb) stn2(ex, o, merge2<T>(p.ex, uxa, ex_a, uxb, ex_b));
     if (uya || uyb) stn2(ey, o, merge2<T>(p.ey, uya, ey_a, uyb, ey_b));
     if (uza || uzb) stn2(ez, o, merge2<T>(p.ez, uza, ez_a, uzb, ez_b));
-    hx_km = p.hx;
-    hy_km = p.hy;
+    hx_km = carry(p.hx);
+    hy_km = carry(p.hy);
 }
 
-template <class T, int CM, bool IN>
+template <class T, int CM, int V>
 __global__ void __launch_bounds__(TX * BY) e_pair_kernel(const KArgs a, const int m)
 {
     int tx, ty, tz;
-    if (IN) {
-        tile_of_box(a, tx, ty, tz);
-    } else {
-        tile_of_general(a, tx, ty, tz);
-    }
+    tile_of_region<V>(a, tx, ty, tz);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int i0 = tx * TXP + 2 * (int)(threadIdx.x & (TX - 1));
     PRow r;
@@ -302,9 +298,9 @@ __global__ void __launch_bounds__(TX * BY) e_pair_kernel(const KArgs a, const in
     r.col = (i64)j * a.px + i0;
     r.ob = (u32)a.plane * (u32)sizeof(T);
     r.o0 = (u32)r.col * (u32)sizeof(T) + r.ob;
-    r.jlo = IN || j > 0;
-    r.jhi = IN || j < ny - 1;
-    r.ys = !IN && ((r.jlo && j < a.L[2]) || (j >= ny - a.L[3] && r.jhi));   // E y-slab, fdtdrav.h:803,827
+    r.jlo = V != V_GEN || j > 0;
+    r.jhi = V != V_GEN || j < ny - 1;
+    r.ys = V == V_GEN && ((r.jlo && j < a.L[2]) || (j >= ny - a.L[3] && r.jhi));   // E y-slab, fdtdrav.h:803,827
     r.sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
     r.py = r.ys ? a.py_prof[j] : make_float4(0.f, 0.f, 1.f, 0.f);
 
@@ -312,10 +308,10 @@ __global__ void __launch_bounds__(TX * BY) e_pair_kernel(const KArgs a, const in
     for (int q = 0; q < 2; ++q) {
         const int i = i0 + q;
         c[q].i = i;
-        c[q].valid = IN || i < nx;
-        c[q].ilo = IN || i > 0;
-        c[q].ihi = IN || i < nx - 1;
-        c[q].xs = !IN && c[q].valid && ((c[q].ilo && i < a.L[0]) || (i >= nx - a.L[1] && c[q].ihi));   // fdtdrav.h:752,777
+        c[q].valid = V == V_IN || i < nx;
+        c[q].ilo = V == V_IN || i > 0;
+        c[q].ihi = V == V_IN || i < nx - 1;
+        c[q].xs = V != V_IN && c[q].valid && ((c[q].ilo && i < a.L[0]) || (i >= nx - a.L[1] && c[q].ihi));   // fdtdrav.h:752,777
         c[q].sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
         c[q].px = c[q].xs ? a.px_prof[i] : make_float4(0.f, 0.f, 1.f, 0.f);
         c[q].mycol = j * nx + i;
@@ -349,23 +345,23 @@ __global__ void __launch_bounds__(TX * BY) e_pair_kernel(const KArgs a, const in
         class_pair<CM>(a.cls, (u32)k0 * (u32)a.plane + (u32)r.col, ca, cb);
         tA = coef_pair<CM>(a.etab, ca, cb);
     }
-    ep_fetch<T, CM, IN>(A, a, r, c, ex, ey, ez, hx, hy, hz, k0, k1);
+    ep_fetch<T, CM, V>(A, a, r, c, ex, ey, ez, hx, hy, hz, k0, k1);
     int k = k0;
     for (; k + 2 < k1; k += 2) {
-        ep_fetch<T, CM, IN>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        ep_fetch<T, CM, V>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_pair<CM>(a.etab, A.cla, A.clb);
-        ep_update<T, CM, IN>(A, tA, a, r, c, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
-        ep_fetch<T, CM, IN>(A, a, r, c, ex, ey, ez, hx, hy, hz, k + 2, k1);
+        ep_update<T, CM, V>(A, tA, a, r, c, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
+        ep_fetch<T, CM, V>(A, a, r, c, ex, ey, ez, hx, hy, hz, k + 2, k1);
         if (CM != CLS_UNIFORM) tA = coef_pair<CM>(a.etab, B.cla, B.clb);
-        ep_update<T, CM, IN>(B, tB, a, r, c, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
+        ep_update<T, CM, V>(B, tB, a, r, c, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
     }
     if (k + 1 < k1) {
-        ep_fetch<T, CM, IN>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        ep_fetch<T, CM, V>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_pair<CM>(a.etab, A.cla, A.clb);
-        ep_update<T, CM, IN>(A, tA, a, r, c, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
-        ep_update<T, CM, IN>(B, tB, a, r, c, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
+        ep_update<T, CM, V>(A, tA, a, r, c, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
+        ep_update<T, CM, V>(B, tB, a, r, c, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
     } else {
-        ep_update<T, CM, IN>(A, tA, a, r, c, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
+        ep_update<T, CM, V>(A, tA, a, r, c, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
     }
 }
 
@@ -383,7 +379,7 @@ __device__ __forceinline__ bool hp_zslab(const KArgs &a, int k)
     return k < a.L[4] || (k >= a.nz - a.L[5] && k < a.nz - 1);   // fdtdrav.h:500,531
 }
 
-template <class T, int CM, bool IN>
+template <class T, int CM, int V>
 __device__ __forceinline__ void hp_fetch(HPair &p, const KArgs &a, const PRow &r, const PCell (&c)[2], const T *ex,
                                          const T *ey, const T *ez, const T *hx, const T *hy, const T *hz, int k,
                                          int k1)
@@ -422,7 +418,7 @@ __device__ __forceinline__ void hp_fetch(HPair &p, const KArgs &a, const PRow &r
         p.p_zy[0] = lo_f(v, ex);
         p.p_zy[1] = hi_f(v, ex);
     }
-    if (!IN && hp_zslab(a, k)) {
+    if (V == V_GEN && hp_zslab(a, k)) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
         const u32 qq = (((u32)sz * ra.ny + r.j) * ra.px + c[0].i) * S;
         const u32 u = ld2((const T *)ra.psi_hxz, qq), v = ld2((const T *)ra.psi_hyz, qq);
@@ -434,7 +430,7 @@ __device__ __forceinline__ void hp_fetch(HPair &p, const KArgs &a, const PRow &r
 }
 
 // H update of one cell of the pair; the arithmetic of kernels.hip h_update.
-template <class T, bool IN>
+template <class T, int V>
 __device__ __forceinline__ void h_cell(const KArgs &a, const PRow &r, const PCell &c, int q, const HPair &p,
                                        const Coef &t, int k, int m, float ex_kp, float ey_kp, float ezc,
                                        float ez_ip, float ey_ip, float ez_jp, float ex_jp, float hx0, float hy0,
@@ -445,7 +441,7 @@ __device__ __forceinline__ void h_cell(const KArgs &a, const PRow &r, const PCel
     ConstKArgs &ra = rare_args(a);
     const int nz = a.nz;
     const float4 cx = t.x, cy = t.y, cz = t.z;
-    const bool khi = IN || k < nz - 1;
+    const bool khi = V != V_GEN || k < nz - 1;
     ux = c.valid && r.jhi && khi;
     uy = c.valid && c.ihi && khi;
     uz = c.valid && c.ihi && r.jhi;
@@ -502,42 +498,38 @@ __device__ __forceinline__ void h_cell(const KArgs &a, const PRow &r, const PCel
     }
 }
 
-template <class T, int CM, bool IN>
+template <class T, int CM, int V>
 __device__ __forceinline__ void hp_update(const HPair &p, const CoefPair &t, const KArgs &a, const PRow &r,
                                           const PCell (&c)[2], T *hx, T *hy, T *hz, int k, int m, u32 &ex_c,
                                           u32 &ey_c, int (&ev)[2], int (&ek)[2])
 {
     const T *tp = nullptr;
     const u32 o = r.o0 + (u32)k * r.ob;
-    const bool zs = !IN && hp_zslab(a, k);
+    const bool zs = V == V_GEN && hp_zslab(a, k);
     const float4 pzm = zs ? uload(a.pz_prof + a.nz + k) : make_float4(0.f, 0.f, 1.f, 0.f);
     const float ez0 = lo_f(p.ez, tp), ez1 = hi_f(p.ez, tp);
     const float exc0 = lo_f(ex_c, tp), exc1 = hi_f(ex_c, tp);
     const float eyc0 = lo_f(ey_c, tp), eyc1 = hi_f(ey_c, tp);
     float hx_a, hy_a, hz_a, hx_b, hy_b, hz_b;
     bool uxa, uya, uza, uxb, uyb, uzb;
-    h_cell<T, IN>(a, r, c[0], 0, p, t.a, k, m, lo_f(p.ex_kp, tp), lo_f(p.ey_kp, tp), ez0, ez1, eyc1,
+    h_cell<T, V>(a, r, c[0], 0, p, t.a, k, m, lo_f(p.ex_kp, tp), lo_f(p.ey_kp, tp), ez0, ez1, eyc1,
               lo_f(p.ez_jp, tp), lo_f(p.ex_jp, tp), lo_f(p.hx, tp), lo_f(p.hy, tp), lo_f(p.hz, tp), exc0, eyc0,
               ev[0], ek[0], zs, pzm, hx_a, hy_a, hz_a, uxa, uya, uza);
-    h_cell<T, IN>(a, r, c[1], 1, p, t.b, k, m, hi_f(p.ex_kp, tp), hi_f(p.ey_kp, tp), ez1, p.ez_i2, p.ey_i2,
+    h_cell<T, V>(a, r, c[1], 1, p, t.b, k, m, hi_f(p.ex_kp, tp), hi_f(p.ey_kp, tp), ez1, p.ez_i2, p.ey_i2,
               hi_f(p.ez_jp, tp), hi_f(p.ex_jp, tp), hi_f(p.hx, tp), hi_f(p.hy, tp), hi_f(p.hz, tp), exc1, eyc1,
               ev[1], ek[1], zs, pzm, hx_b, hy_b, hz_b, uxb, uyb, uzb);
     if (uxa || uxb) stn2(hx, o, merge2<T>(p.hx, uxa, hx_a, uxb, hx_b));
     if (uya || uyb) stn2(hy, o, merge2<T>(p.hy, uya, hy_a, uyb, hy_b));
     if (uza || uzb) stn2(hz, o, merge2<T>(p.hz, uza, hz_a, uzb, hz_b));
-    ex_c = p.ex_kp;
-    ey_c = p.ey_kp;
+    ex_c = carry(p.ex_kp);
+    ey_c = carry(p.ey_kp);
 }
 
-template <class T, int CM, bool IN>
+template <class T, int CM, int V>
 __global__ void __launch_bounds__(TX * BY) h_pair_kernel(const KArgs a, const int m)
 {
     int tx, ty, tz;
-    if (IN) {
-        tile_of_box(a, tx, ty, tz);
-    } else {
-        tile_of_general(a, tx, ty, tz);
-    }
+    tile_of_region<V>(a, tx, ty, tz);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int i0 = tx * TXP + 2 * (int)(threadIdx.x & (TX - 1));
     PRow r;
@@ -549,9 +541,9 @@ __global__ void __launch_bounds__(TX * BY) h_pair_kernel(const KArgs a, const in
     r.col = (i64)j * a.px + i0;
     r.ob = (u32)a.plane * (u32)sizeof(T);
     r.o0 = (u32)r.col * (u32)sizeof(T) + r.ob;
-    r.jlo = IN || j > 0;
-    r.jhi = IN || j < ny - 1;
-    r.ys = !IN && (j < a.L[2] || (j >= ny - a.L[3] && r.jhi));   // H y-slab, fdtdrav.h:447,471
+    r.jlo = V != V_GEN || j > 0;
+    r.jhi = V != V_GEN || j < ny - 1;
+    r.ys = V == V_GEN && (j < a.L[2] || (j >= ny - a.L[3] && r.jhi));   // H y-slab, fdtdrav.h:447,471
     r.sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
     r.py = r.ys ? a.py_prof[ny + j] : make_float4(0.f, 0.f, 1.f, 0.f);
 
@@ -559,10 +551,10 @@ __global__ void __launch_bounds__(TX * BY) h_pair_kernel(const KArgs a, const in
     for (int q = 0; q < 2; ++q) {
         const int i = i0 + q;
         c[q].i = i;
-        c[q].valid = IN || i < nx;
-        c[q].ilo = IN || i > 0;
-        c[q].ihi = IN || i < nx - 1;
-        c[q].xs = !IN && c[q].valid && (i < a.L[0] || (i >= nx - a.L[1] && c[q].ihi));   // fdtdrav.h:402,424
+        c[q].valid = V == V_IN || i < nx;
+        c[q].ilo = V == V_IN || i > 0;
+        c[q].ihi = V == V_IN || i < nx - 1;
+        c[q].xs = V != V_IN && c[q].valid && (i < a.L[0] || (i >= nx - a.L[1] && c[q].ihi));   // fdtdrav.h:402,424
         c[q].sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
         c[q].px = c[q].xs ? a.px_prof[nx + i] : make_float4(0.f, 0.f, 1.f, 0.f);
         c[q].mycol = j * nx + i;
@@ -595,49 +587,53 @@ __global__ void __launch_bounds__(TX * BY) h_pair_kernel(const KArgs a, const in
         class_pair<CM>(a.cls, (u32)k0 * (u32)a.plane + (u32)r.col, ca, cb);
         tA = coef_pair<CM>(a.htab, ca, cb);
     }
-    hp_fetch<T, CM, IN>(A, a, r, c, ex, ey, ez, hx, hy, hz, k0, k1);
+    hp_fetch<T, CM, V>(A, a, r, c, ex, ey, ez, hx, hy, hz, k0, k1);
     int k = k0;
     for (; k + 2 < k1; k += 2) {
-        hp_fetch<T, CM, IN>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        hp_fetch<T, CM, V>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_pair<CM>(a.htab, A.cla, A.clb);
-        hp_update<T, CM, IN>(A, tA, a, r, c, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
-        hp_fetch<T, CM, IN>(A, a, r, c, ex, ey, ez, hx, hy, hz, k + 2, k1);
+        hp_update<T, CM, V>(A, tA, a, r, c, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
+        hp_fetch<T, CM, V>(A, a, r, c, ex, ey, ez, hx, hy, hz, k + 2, k1);
         if (CM != CLS_UNIFORM) tA = coef_pair<CM>(a.htab, B.cla, B.clb);
-        hp_update<T, CM, IN>(B, tB, a, r, c, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
+        hp_update<T, CM, V>(B, tB, a, r, c, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
     }
     if (k + 1 < k1) {
-        hp_fetch<T, CM, IN>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        hp_fetch<T, CM, V>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_pair<CM>(a.htab, A.cla, A.clb);
-        hp_update<T, CM, IN>(A, tA, a, r, c, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
-        hp_update<T, CM, IN>(B, tB, a, r, c, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
+        hp_update<T, CM, V>(A, tA, a, r, c, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
+        hp_update<T, CM, V>(B, tB, a, r, c, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
     } else {
-        hp_update<T, CM, IN>(A, tA, a, r, c, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
+        hp_update<T, CM, V>(A, tA, a, r, c, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
     }
 }
 
 // ---- launchers --------------------------------------------------------------
-template <class T, bool IN>
+template <class T, int V>
 static void launch_pair_v(bool efield, int mode, const KArgs &a, int m, int nt, hipStream_t s)
 {
     dim3 grid(nt), block(TX * BY);
     if (efield) {
-        if (mode == CLS_UNIFORM) hipLaunchKernelGGL((e_pair_kernel<T, CLS_UNIFORM, IN>), grid, block, 0, s, a, m);
-        else if (mode == CLS_U16) hipLaunchKernelGGL((e_pair_kernel<T, CLS_U16, IN>), grid, block, 0, s, a, m);
-        else hipLaunchKernelGGL((e_pair_kernel<T, CLS_U32, IN>), grid, block, 0, s, a, m);
+        if (mode == CLS_UNIFORM) hipLaunchKernelGGL((e_pair_kernel<T, CLS_UNIFORM, V>), grid, block, 0, s, a, m);
+        else if (mode == CLS_U16) hipLaunchKernelGGL((e_pair_kernel<T, CLS_U16, V>), grid, block, 0, s, a, m);
+        else hipLaunchKernelGGL((e_pair_kernel<T, CLS_U32, V>), grid, block, 0, s, a, m);
     } else {
-        if (mode == CLS_UNIFORM) hipLaunchKernelGGL((h_pair_kernel<T, CLS_UNIFORM, IN>), grid, block, 0, s, a, m);
-        else if (mode == CLS_U16) hipLaunchKernelGGL((h_pair_kernel<T, CLS_U16, IN>), grid, block, 0, s, a, m);
-        else hipLaunchKernelGGL((h_pair_kernel<T, CLS_U32, IN>), grid, block, 0, s, a, m);
+        if (mode == CLS_UNIFORM) hipLaunchKernelGGL((h_pair_kernel<T, CLS_UNIFORM, V>), grid, block, 0, s, a, m);
+        else if (mode == CLS_U16) hipLaunchKernelGGL((h_pair_kernel<T, CLS_U16, V>), grid, block, 0, s, a, m);
+        else hipLaunchKernelGGL((h_pair_kernel<T, CLS_U32, V>), grid, block, 0, s, a, m);
     }
 }
 
 template <class T>
 static hipError_t launch_pair_t(bool efield, int mode, const KArgs &a, int m, hipStream_t s)
 {
-    const int nt = a.tiles_x * a.tiles_y * ((a.nz + a.kchunk - 1) / a.kchunk);
     const int nb = (a.ib[3] - a.ib[0]) * (a.ib[4] - a.ib[1]) * (a.ib[5] - a.ib[2]);
-    if (nb > 0) launch_pair_v<T, true>(efield, mode, a, m, nb, s);
-    launch_pair_v<T, false>(efield, mode, a, m, a.nglist ? a.nglist : nt, s);
+    if (nb > 0) {
+        launch_pair_v<T, V_IN>(efield, mode, a, m, nb, s);
+        if (a.nxlist > 0) launch_pair_v<T, V_XE>(efield, mode, a, m, a.nxlist, s);
+        if (a.nglist > 0) launch_pair_v<T, V_GEN>(efield, mode, a, m, a.nglist, s);
+    } else {
+        launch_pair_v<T, V_GEN>(efield, mode, a, m, a.tiles_x * a.tiles_y * ((a.nz + a.kchunk - 1) / a.kchunk), s);
+    }
     return hipGetLastError();
 }
 

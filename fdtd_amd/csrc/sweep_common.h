@@ -11,6 +11,24 @@
 
 namespace fdtd {
 
+// Copy of a carried z-operand (the previous plane's H for E, the next
+// plane's E for H) into a register of its own.  Left to itself the compiler
+// keeps the carried value in its load register and renames the next fetch
+// instead, which costs a register copy -- and a wait for that fetch -- at the
+// loop back-edge.  The source is already consumed when this runs.
+__device__ __forceinline__ float carry(float v)
+{
+    float r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+__device__ __forceinline__ std::uint32_t carry(std::uint32_t v)
+{
+    std::uint32_t r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
 // Non-temporal access for the streams only the owning thread touches (the
 // swept field's own values): they are read once and written once per sweep.
 __device__ __forceinline__ float ldn(const float *p) { return __builtin_nontemporal_load(p); }
