@@ -62,12 +62,8 @@ struct fdtd_ctx {
     bool pair = false;              // 16-bit storage: two cells per thread in the half-step kernels
     int pair_tiles_x = 0, pair_kchunk = 0;
     int kchunk_h = 0;               // planes per block of the one-cell H sweep (E: ka.kchunk)
-    // per sweep (0 = E, 1 = H): interior tile box and the general variant's tile list
-    int box[2][6] = {};
-    const int *xlist[2] = {nullptr, nullptr};
-    int nxlist[2] = {0, 0};
-    const int *glist[2] = {nullptr, nullptr};
-    int nglist[2] = {0, 0};
+    // per sweep (0 = E, 1 = H): tiles grouped by region variant
+    int box[2][6] = {};             // interior tile box per sweep (0 = E, 1 = H)
     KArgs ka_step{};                // tile geometry of the step kernel
     double step_ms = 0.0;
     size_t psi_elems[12] = {};
@@ -188,44 +184,39 @@ int choose_chunks(const fdtd_ctx *c, int tiles, int planes)
     return (c->in.nz + nchunks - 1) / nchunks;
 }
 
-void interior_box(const fdtd_ctx *c, int tile_cols, int kchunk, int ib[6]);
-
-// Interior box and the x-edge and general tile lists of one sweep (tiles
-// tile_cols wide, z-blocks of kchunk planes).  FDTD_XEDGE=0 leaves the
-// x-edge tiles to the general variant.
-int sweep_split(fdtd_ctx *c, int sweep, int tile_cols, int kchunk)
+// Interior tile box of one sweep (tiles tile_cols wide and tile_rows() high,
+// z-blocks of kchunk planes): along each axis, the run of tiles whose cell
+// range reaches neither a CPML slab nor an outer face (nor the grid's end).
+// The x-edge variant takes the tiles beside the box in x, the general
+// variant the rows and z-blocks outside it: z-edge variants of their own
+// measured slower on 512x512x256 (their launch tails cost more than their
+// leaner code saves).  FDTD_INTERIOR=0, or an axis without interior tiles,
+// leaves the box empty and every tile to the general variant.
+void sweep_split(fdtd_ctx *c, int sweep, int tile_cols, int kchunk)
 {
+    const SceneIn &in = c->in;
+    const int *L = in.cpml.thick;
     int *ib = c->box[sweep];
-    interior_box(c, tile_cols, kchunk, ib);
-    c->xlist[sweep] = c->glist[sweep] = nullptr;
-    c->nxlist[sweep] = c->nglist[sweep] = 0;
-    if (ib[3] == ib[0]) return FDTD_OK;
-    const char *xe = std::getenv("FDTD_XEDGE");
-    const bool xedge = !(xe && std::strcmp(xe, "0") == 0);
-    const int tx = (c->in.nx + tile_cols - 1) / tile_cols, ty = c->ka.tiles_y;
-    const int tz = (c->in.nz + kchunk - 1) / kchunk;
-    std::vector<int> xl, gl;
-    for (int z = 0; z < tz; ++z)
-        for (int y = 0; y < ty; ++y)
-            for (int x = 0; x < tx; ++x) {
-                const bool yz_in = y >= ib[1] && y < ib[4] && z >= ib[2] && z < ib[5];
-                const bool x_in = x >= ib[0] && x < ib[3];
-                if (yz_in && x_in) continue;
-                (yz_in && xedge ? xl : gl).push_back(x + tx * (y + ty * z));
+    for (int q = 0; q < 6; ++q) ib[q] = 0;
+    const char *ev = std::getenv("FDTD_INTERIOR");
+    if (ev && std::strcmp(ev, "0") == 0) return;
+    const int by = tile_rows();
+    const int nts[3] = {(in.nx + tile_cols - 1) / tile_cols, (in.ny + by - 1) / by, (in.nz + kchunk - 1) / kchunk};
+    const int w[3] = {tile_cols, by, kchunk}, nn[3] = {in.nx, in.ny, in.nz};
+    int box[6];
+    for (int ax = 0; ax < 3; ++ax) {
+        const int lo_cells = std::max(L[2 * ax], 1), hi_cells = nn[ax] - std::max(L[2 * ax + 1], 1);
+        int lo = nts[ax], hi = 0;
+        for (int t = 0; t < nts[ax]; ++t)
+            if (t * w[ax] >= lo_cells && (t + 1) * w[ax] <= hi_cells) {
+                lo = std::min(lo, t);
+                hi = t + 1;
             }
-    void *d = nullptr;
-    int rc;
-    if (!xl.empty()) {
-        if ((rc = upload(c, &d, xl.data(), xl.size() * sizeof(int)))) return rc;
-        c->xlist[sweep] = (const int *)d;
-        c->nxlist[sweep] = (int)xl.size();
+        if (hi <= lo) return;
+        box[ax] = lo;
+        box[ax + 3] = hi;
     }
-    if (!gl.empty()) {
-        if ((rc = upload(c, &d, gl.data(), gl.size() * sizeof(int)))) return rc;
-        c->glist[sweep] = (const int *)d;
-        c->nglist[sweep] = (int)gl.size();
-    }
-    return FDTD_OK;
+    for (int q = 0; q < 6; ++q) ib[q] = box[q];
 }
 
 int do_finalize(fdtd_ctx *c)
@@ -284,9 +275,8 @@ int do_finalize(fdtd_ctx *c)
     // interior / general split of the two sweeps
     {
         const int tcols = c->pair ? pair_tile_cols() : tile_cols();
-        int rc = sweep_split(c, 0, tcols, c->pair ? c->pair_kchunk : a.kchunk);
-        if (!rc) rc = sweep_split(c, 1, tcols, c->pair ? c->pair_kchunk : c->kchunk_h);
-        if (rc) return rc;
+        sweep_split(c, 0, tcols, c->pair ? c->pair_kchunk : a.kchunk);
+        sweep_split(c, 1, tcols, c->pair ? c->pair_kchunk : c->kchunk_h);
     }
     // psi slabs
     const size_t px_elems = (size_t)in.nz * in.ny * a.Lx;
@@ -410,31 +400,6 @@ void bind_sets(const fdtd_ctx *c, KArgs &a)
     a.npsi_eyz = c->psi_set[w][9];
 }
 
-// Tile box of a sweep (tiles of tile_cols x tile_rows() cells, z-blocks of
-// kchunk planes) whose cells all lie off the CPML slabs and off the outer
-// faces: there the interior variant of the sweep runs, with no slab or face
-// predicates in its code.  FDTD_INTERIOR=0 disables the split.
-void interior_box(const fdtd_ctx *c, int tile_cols, int kchunk, int ib[6])
-{
-    for (int q = 0; q < 6; ++q) ib[q] = 0;
-    const char *ev = std::getenv("FDTD_INTERIOR");
-    if (ev && std::strcmp(ev, "0") == 0) return;
-    const SceneIn &in = c->in;
-    const int *L = in.cpml.thick;
-    const int i0 = std::max(L[0], 1), i1 = in.nx - std::max(L[1], 1);
-    const int j0 = std::max(L[2], 1), j1 = in.ny - std::max(L[3], 1);
-    const int k0 = std::max(L[4], 1), k1 = in.nz - std::max(L[5], 1);
-    const int ty = tile_rows();
-    ib[0] = (i0 + tile_cols - 1) / tile_cols;
-    ib[3] = std::max(ib[0], i1 / tile_cols);
-    ib[1] = (j0 + ty - 1) / ty;
-    ib[4] = std::max(ib[1], j1 / ty);
-    ib[2] = (k0 + kchunk - 1) / kchunk;
-    ib[5] = std::max(ib[2], k1 / kchunk);
-    if (ib[3] == ib[0] || ib[4] == ib[1] || ib[5] == ib[2])
-        for (int q = 0; q < 6; ++q) ib[q] = 0;
-}
-
 // One time step m.  base carries the per-call flags (rec, snapshot); the
 // tile geometry and buffer pointers are filled in here.
 hipError_t do_step(fdtd_ctx *c, const KArgs &base, int m, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
@@ -461,10 +426,6 @@ hipError_t do_step(fdtd_ctx *c, const KArgs &base, int m, hipEvent_t ev0 = nullp
     auto half = c->pair ? launch_pair_half_step : launch_half_step;
     auto split = [&](int sweep) {
         for (int q = 0; q < 6; ++q) a.ib[q] = c->box[sweep][q];
-        a.xlist = c->xlist[sweep];
-        a.nxlist = c->nxlist[sweep];
-        a.glist = c->glist[sweep];
-        a.nglist = c->nglist[sweep];
     };
     split(0);
     if (ev0) (void)hipEventRecord(ev0, c->stream);

@@ -89,29 +89,52 @@ __device__ __forceinline__ void tile_of_box(const KArgs &a, int &tx, int &ty, in
     tz = a.ib[2] + t / (bx * by);
 }
 
-// Workgroup -> tile of a tile list, with the same XCD-aware remap.
-__device__ __forceinline__ void tile_of_list(const KArgs &a, const int *list, int nt, int &tx, int &ty, int &tz)
+// XCD-aware remap of workgroup b over nt tiles (as in tile_of).
+__device__ __forceinline__ int remap_tile(int nt)
 {
     const int b = blockIdx.x;
     const int q = nt >> 3, r = nt & 7, x = b & 7;
-    const int t = list[x * q + (x < r ? x : r) + (b >> 3)];
-    tx = t % a.tiles_x;
-    ty = (t / a.tiles_x) % a.tiles_y;
-    tz = t / (a.tiles_x * a.tiles_y);
+    return x * q + (x < r ? x : r) + (b >> 3);
 }
 
-// Sweep-kernel variants by the region of their tiles.
-enum SweepRegion { V_IN = 0, V_XE = 1, V_GEN = 2 };
-
-// Workgroup -> tile of variant V: the interior box, the x-edge list, or the
-// general list (the whole grid when there is no box).
+// Workgroup -> tile of region variant V, computed from the interior box (no
+// tile lists: the many short workgroups then start without a dependent
+// load).  V_X: tile columns outside [ib0, ib3) of the box's rows and
+// z-blocks.  V_GEN: every z-block outside [ib2, ib5), then the rows outside
+// [ib1, ib4) of the box's z-blocks; the whole grid when there is no box.
 template <int V>
 __device__ __forceinline__ void tile_of_region(const KArgs &a, int &tx, int &ty, int &tz)
 {
-    if (V == V_IN) tile_of_box(a, tx, ty, tz);
-    else if (V == V_XE) tile_of_list(a, a.xlist, a.nxlist, tx, ty, tz);
-    else if (a.nglist == 0) tile_of(a, tx, ty, tz);
-    else tile_of_list(a, a.glist, a.nglist, tx, ty, tz);
+    const int *ib = a.ib;
+    if (V == V_IN) {
+        tile_of_box(a, tx, ty, tz);
+    } else if (V == V_X) {
+        const int ex = ib[0] + a.tiles_x - ib[3], by = ib[4] - ib[1];
+        const int t = remap_tile(ex * by * (ib[5] - ib[2]));
+        const int xe = t % ex;
+        tx = xe < ib[0] ? xe : ib[3] + (xe - ib[0]);
+        ty = ib[1] + (t / ex) % by;
+        tz = ib[2] + t / (ex * by);
+    } else if (ib[3] == ib[0]) {
+        tile_of(a, tx, ty, tz);
+    } else {
+        const int txy = a.tiles_x * a.tiles_y, tzs = (a.nz + a.kchunk - 1) / a.kchunk;
+        const int bz = ib[5] - ib[2], ey = a.tiles_y - (ib[4] - ib[1]);
+        const int n1 = txy * (tzs - bz);
+        const int t = remap_tile(n1 + a.tiles_x * ey * bz);
+        if (t < n1) {
+            const int ze = t / txy;
+            tx = t % a.tiles_x;
+            ty = (t % txy) / a.tiles_x;
+            tz = ze < ib[2] ? ze : ib[5] + (ze - ib[2]);
+        } else {
+            const int u = t - n1;
+            const int ye = (u / a.tiles_x) % ey;
+            tx = u % a.tiles_x;
+            ty = ye < ib[1] ? ye : ib[4] + (ye - ib[1]);
+            tz = ib[2] + u / (a.tiles_x * ey);
+        }
+    }
 }
 
 __device__ __forceinline__ int first_event(const KArgs &a, const int *first, int col, int k0)

@@ -37,6 +37,10 @@ struct SrcEdge {
     int pad[3];
 };
 
+// Sweep-kernel region variants: bit V_X / V_Y / V_Z set when the tiles of
+// the variant can meet an x / y / z CPML slab or outer face.
+enum SweepRegion { V_IN = 0, V_X = 1, V_Y = 2, V_Z = 4, V_GEN = 7 };
+
 // Kernel argument block.  Field pointers point at (i,j,k) = (0,0,0) of an
 // allocation that has one zero guard plane below k = 0 and one above
 // k = nz-1, so every i+-1, j+-1, k+-1 neighbour address is inside it.
@@ -78,19 +82,11 @@ struct KArgs {
     int tiles_x, tiles_y;      // tile grid (for the XCD-aware remap)
     // Interior tile box [ib0, ib3) x [ib1, ib4) x [ib2, ib5) of (tile x,
     // tile y, z-block): every cell of it lies off the CPML slabs and the
-    // outer faces.  Three variants of a sweep share the grid: the interior
-    // variant covers exactly the box; the x-edge variant the tiles beside it
-    // in x (same tile rows and z-blocks), which can meet only the x-slabs and
-    // x-faces; the general variant every other tile (empty box: the general
-    // variant covers everything).
+    // outer faces.  With a box, three region variants share the sweep: V_IN
+    // over the box, V_X over the tiles beside it in x (same rows and
+    // z-blocks), V_GEN over the rows and z-blocks outside it.  Without one
+    // (all zero), V_GEN covers every tile.
     int ib[6];
-    // Tile lists of the x-edge and general variants when the box is not
-    // empty (flat index tx + tiles_x (ty + tiles_y tz)), so that they spread
-    // evenly over the XCDs.
-    const int *xlist;
-    int nxlist;
-    const int *glist;
-    int nglist;
     int rec;                   // record probes / DFT (0 during adjoint recompute)
     int e_uni, h_uni;          // E / H coefficients are one class (no class-map read)
     // design-region snapshot: E^{m+1} of the E edges in the box
@@ -100,6 +96,22 @@ struct KArgs {
     int sb[6];
     int snap_base;
 };
+
+// Tile counts of the three region variants of a sweep (tile_of_region).
+inline void region_tiles(const KArgs &a, int &n_in, int &n_x, int &n_gen)
+{
+    const int *ib = a.ib;
+    const int tzs = (a.nz + a.kchunk - 1) / a.kchunk;
+    if (ib[3] == ib[0]) {
+        n_in = n_x = 0;
+        n_gen = a.tiles_x * a.tiles_y * tzs;
+        return;
+    }
+    const int bx = ib[3] - ib[0], by = ib[4] - ib[1], bz = ib[5] - ib[2];
+    n_in = bx * by * bz;
+    n_x = (a.tiles_x - bx) * by * bz;
+    n_gen = a.tiles_x * a.tiles_y * (tzs - bz) + a.tiles_x * (a.tiles_y - by) * bz;
+}
 
 // Adjoint argument block (FP32 SI adjoint state; same layouts as the forward).
 struct AArgs {

@@ -100,7 +100,7 @@ __device__ __forceinline__ void e_fetch(EPlane &p, const KArgs &a, const ECol &c
     p.cln = (CM != CLS_UNIFORM && k + 1 < k1)
                 ? class_at<CM>(a.cls, (std::uint32_t)(k + 1) * (std::uint32_t)a.plane + (std::uint32_t)cc.col)
                 : 0;
-    const bool khi = V != V_GEN || k < a.nz - 1;
+    const bool khi = !(V & V_Z) || k < a.nz - 1;
     p.p_yx = p.p_zx = p.p_xy = p.p_zy = p.p_xz = p.p_yz = 0.f;
     if (cc.xs && cc.jhi && khi) {
         const std::uint32_t q = (((std::uint32_t)k * ny + cc.j) * ra.Lx + cc.sx) * S;
@@ -112,7 +112,7 @@ __device__ __forceinline__ void e_fetch(EPlane &p, const KArgs &a, const ECol &c
         p.p_xy = ld(at((const T *)ra.psi_exy, q));
         p.p_zy = ld(at((const T *)ra.psi_ezy, q));
     }
-    if (V == V_GEN && e_zslab(a, k) && cc.ihi && cc.jhi) {
+    if ((V & V_Z) && e_zslab(a, k) && cc.ihi && cc.jhi) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
         const std::uint32_t q = (((std::uint32_t)sz * ny + cc.j) * px + cc.i) * S;
         p.p_xz = ld(at((const T *)ra.psi_exz, q));
@@ -133,11 +133,11 @@ __device__ __forceinline__ void e_update(const EPlane &p, const Coef &t, const K
     const float hy_im = p.hy_im, hz_im = p.hz_im;
     const float hx_jm = p.hx_jm, hz_jm = p.hz_jm;
     const float4 cx = t.x, cy = t.y, cz = t.z;
-    const bool klo = V != V_GEN || k > 0, khi = V != V_GEN || k < nz - 1;
+    const bool klo = !(V & V_Z) || k > 0, khi = !(V & V_Z) || k < nz - 1;
     const bool upd_x = cc.ihi && cc.jlo && cc.jhi && klo && khi;
     const bool upd_y = cc.ilo && cc.ihi && cc.jhi && klo && khi;
     const bool upd_z = cc.ilo && cc.ihi && cc.jlo && cc.jhi && khi;
-    const bool zs = V == V_GEN && e_zslab(a, k);
+    const bool zs = (V & V_Z) && e_zslab(a, k);
 
     // curl coefficients, divided by kappa on the slab cells (setABCPML)
     float cbx_hz = cx.y, cbx_hy = cx.z;
@@ -232,12 +232,12 @@ __global__ void __launch_bounds__(TX * BY) E_OCC e_kernel(const KArgs a, const i
     const T *__restrict__ hz = (const T *)a.hz - a.plane;
 
     // column-constant predicates (reference region guards)
-    cc.ilo = V == V_IN || i > 0;
-    cc.ihi = V == V_IN || i < nx - 1;
-    cc.jlo = V != V_GEN || j > 0;
-    cc.jhi = V != V_GEN || j < ny - 1;
-    cc.xs = V != V_IN && ((cc.ilo && i < a.L[0]) || (i >= nx - a.L[1] && cc.ihi));   // E x-slab, fdtdrav.h:752,777
-    cc.ys = V == V_GEN && ((cc.jlo && j < a.L[2]) || (j >= ny - a.L[3] && cc.jhi));   // E y-slab, fdtdrav.h:803,827
+    cc.ilo = !(V & V_X) || i > 0;
+    cc.ihi = !(V & V_X) || i < nx - 1;
+    cc.jlo = !(V & V_Y) || j > 0;
+    cc.jhi = !(V & V_Y) || j < ny - 1;
+    cc.xs = (V & V_X) && ((cc.ilo && i < a.L[0]) || (i >= nx - a.L[1] && cc.ihi));   // E x-slab, fdtdrav.h:752,777
+    cc.ys = (V & V_Y) && ((cc.jlo && j < a.L[2]) || (j >= ny - a.L[3] && cc.jhi));   // E y-slab, fdtdrav.h:803,827
     cc.sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
     cc.sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
     cc.pxe = cc.xs ? a.px_prof[i] : make_float4(0.f, 0.f, 1.f, 0.f);
@@ -340,7 +340,7 @@ __device__ __forceinline__ void h_fetch(HPlane &p, const KArgs &a, const HCol &c
         p.p_xy = ld(at((const T *)ra.psi_hxy, q));
         p.p_zy = ld(at((const T *)ra.psi_hzy, q));
     }
-    if (V == V_GEN && h_zslab(a, k)) {
+    if ((V & V_Z) && h_zslab(a, k)) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
         const std::uint32_t q = (((std::uint32_t)sz * ny + cc.j) * px + cc.i) * S;
         p.p_xz = ld(at((const T *)ra.psi_hxz, q));
@@ -361,11 +361,11 @@ __device__ __forceinline__ void h_update(const HPlane &p, const Coef &t, const K
     const float ez_ip = p.ez_ip, ey_ip = p.ey_ip;
     const float ez_jp = p.ez_jp, ex_jp = p.ex_jp;
     const float4 cx = t.x, cy = t.y, cz = t.z;
-    const bool khi = V != V_GEN || k < nz - 1;
+    const bool khi = !(V & V_Z) || k < nz - 1;
     const bool upd_x = cc.jhi && khi;
     const bool upd_y = cc.ihi && khi;
     const bool upd_z = cc.ihi && cc.jhi;
-    const bool zs = V == V_GEN && h_zslab(a, k);
+    const bool zs = (V & V_Z) && h_zslab(a, k);
 
     float chx_ey = cx.y, chx_ez = cx.z;
     float chy_ez = cy.y, chy_ex = cy.z;
@@ -453,10 +453,10 @@ __global__ void __launch_bounds__(TX * BY) H_OCC h_kernel(const KArgs a, const i
     T *__restrict__ hy = (T *)a.hy - a.plane;
     T *__restrict__ hz = (T *)a.hz - a.plane;
 
-    cc.ihi = V == V_IN || i < nx - 1;
-    cc.jhi = V != V_GEN || j < ny - 1;
-    cc.xs = V != V_IN && (i < a.L[0] || (i >= nx - a.L[1] && cc.ihi));   // H x-slab, fdtdrav.h:402,424
-    cc.ys = V == V_GEN && (j < a.L[2] || (j >= ny - a.L[3] && cc.jhi));   // H y-slab, fdtdrav.h:447,471
+    cc.ihi = !(V & V_X) || i < nx - 1;
+    cc.jhi = !(V & V_Y) || j < ny - 1;
+    cc.xs = (V & V_X) && (i < a.L[0] || (i >= nx - a.L[1] && cc.ihi));   // H x-slab, fdtdrav.h:402,424
+    cc.ys = (V & V_Y) && (j < a.L[2] || (j >= ny - a.L[3] && cc.jhi));   // H y-slab, fdtdrav.h:447,471
     cc.sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
     cc.sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
     cc.pxm = cc.xs ? a.px_prof[nx + i] : make_float4(0.f, 0.f, 1.f, 0.f);
@@ -512,19 +512,16 @@ static void launch_v(bool efield, int mode, const KArgs &a, int m, int nt, hipSt
     }
 }
 
-// Interior variant over the box a.ib, x-edge and general variants over
-// their tile lists; without a box the general variant covers every tile.
+// One launch per non-empty region variant: interior box, x-edge tiles,
+// general tiles.
 template <class T>
 static hipError_t launch_t(bool efield, int mode, const KArgs &a, int m, hipStream_t s)
 {
-    const int nb = (a.ib[3] - a.ib[0]) * (a.ib[4] - a.ib[1]) * (a.ib[5] - a.ib[2]);
-    if (nb > 0) {
-        launch_v<T, V_IN>(efield, mode, a, m, nb, s);
-        if (a.nxlist > 0) launch_v<T, V_XE>(efield, mode, a, m, a.nxlist, s);
-        if (a.nglist > 0) launch_v<T, V_GEN>(efield, mode, a, m, a.nglist, s);
-    } else {
-        launch_v<T, V_GEN>(efield, mode, a, m, a.tiles_x * a.tiles_y * ((a.nz + a.kchunk - 1) / a.kchunk), s);
-    }
+    int nb, nx, ng;
+    region_tiles(a, nb, nx, ng);
+    if (nb > 0) launch_v<T, V_IN>(efield, mode, a, m, nb, s);
+    if (nx > 0) launch_v<T, V_X>(efield, mode, a, m, nx, s);
+    if (ng > 0) launch_v<T, V_GEN>(efield, mode, a, m, ng, s);
     return hipGetLastError();
 }
 

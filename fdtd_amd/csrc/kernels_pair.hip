@@ -154,7 +154,7 @@ __device__ __forceinline__ void ep_fetch(EPair &p, const KArgs &a, const PRow &r
     p.cla = p.clb = 0;
     if (CM != CLS_UNIFORM && k + 1 < k1)
         class_pair<CM>(a.cls, (u32)(k + 1) * (u32)a.plane + (u32)r.col, p.cla, p.clb);
-    const bool khi = V != V_GEN || k < a.nz - 1;
+    const bool khi = !(V & V_Z) || k < a.nz - 1;
     for (int q = 0; q < 2; ++q) {
         p.p_yx[q] = p.p_zx[q] = p.p_xy[q] = p.p_zy[q] = p.p_xz[q] = p.p_yz[q] = 0.f;
         if (c[q].xs && r.jhi && khi) {
@@ -171,7 +171,7 @@ __device__ __forceinline__ void ep_fetch(EPair &p, const KArgs &a, const PRow &r
         p.p_zy[0] = lo_f(v, ex);
         p.p_zy[1] = hi_f(v, ex);
     }
-    if (V == V_GEN && ((k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && khi)) && r.jhi) {
+    if ((V & V_Z) && ((k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && khi)) && r.jhi) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
         const u32 qq = (((u32)sz * ra.ny + r.j) * ra.px + c[0].i) * S;
         const u32 u = ld2((const T *)ra.psi_exz, qq), v = ld2((const T *)ra.psi_eyz, qq);
@@ -193,7 +193,7 @@ __device__ __forceinline__ void e_cell(const KArgs &a, const PRow &r, const PCel
     ConstKArgs &ra = rare_args(a);
     const int nz = a.nz;
     const float4 cx = t.x, cy = t.y, cz = t.z;
-    const bool klo = V != V_GEN || k > 0, khi = V != V_GEN || k < nz - 1;
+    const bool klo = !(V & V_Z) || k > 0, khi = !(V & V_Z) || k < nz - 1;
     ux = c.valid && c.ihi && r.jlo && r.jhi && klo && khi;
     uy = c.valid && c.ilo && c.ihi && r.jhi && klo && khi;
     uz = c.valid && c.ilo && c.ihi && r.jlo && r.jhi && khi;
@@ -262,7 +262,7 @@ __device__ __forceinline__ void ep_update(const EPair &p, const CoefPair &t, con
 {
     const T *tp = nullptr;
     const u32 o = r.o0 + (u32)k * r.ob;
-    const bool zs = V == V_GEN && ((k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && k < a.nz - 1));   // fdtdrav.h:853,879
+    const bool zs = (V & V_Z) && ((k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && k < a.nz - 1));   // fdtdrav.h:853,879
     const float4 pze = zs ? uload(a.pz_prof + k) : make_float4(0.f, 0.f, 1.f, 0.f);
     const float hx0 = lo_f(p.hx, tp), hx1 = hi_f(p.hx, tp);
     const float hy0 = lo_f(p.hy, tp), hy1 = hi_f(p.hy, tp);
@@ -298,9 +298,9 @@ __global__ void __launch_bounds__(TX * BY) e_pair_kernel(const KArgs a, const in
     r.col = (i64)j * a.px + i0;
     r.ob = (u32)a.plane * (u32)sizeof(T);
     r.o0 = (u32)r.col * (u32)sizeof(T) + r.ob;
-    r.jlo = V != V_GEN || j > 0;
-    r.jhi = V != V_GEN || j < ny - 1;
-    r.ys = V == V_GEN && ((r.jlo && j < a.L[2]) || (j >= ny - a.L[3] && r.jhi));   // E y-slab, fdtdrav.h:803,827
+    r.jlo = !(V & V_Y) || j > 0;
+    r.jhi = !(V & V_Y) || j < ny - 1;
+    r.ys = (V & V_Y) && ((r.jlo && j < a.L[2]) || (j >= ny - a.L[3] && r.jhi));   // E y-slab, fdtdrav.h:803,827
     r.sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
     r.py = r.ys ? a.py_prof[j] : make_float4(0.f, 0.f, 1.f, 0.f);
 
@@ -308,10 +308,10 @@ __global__ void __launch_bounds__(TX * BY) e_pair_kernel(const KArgs a, const in
     for (int q = 0; q < 2; ++q) {
         const int i = i0 + q;
         c[q].i = i;
-        c[q].valid = V == V_IN || i < nx;
-        c[q].ilo = V == V_IN || i > 0;
-        c[q].ihi = V == V_IN || i < nx - 1;
-        c[q].xs = V != V_IN && c[q].valid && ((c[q].ilo && i < a.L[0]) || (i >= nx - a.L[1] && c[q].ihi));   // fdtdrav.h:752,777
+        c[q].valid = !(V & V_X) || i < nx;
+        c[q].ilo = !(V & V_X) || i > 0;
+        c[q].ihi = !(V & V_X) || i < nx - 1;
+        c[q].xs = (V & V_X) && c[q].valid && ((c[q].ilo && i < a.L[0]) || (i >= nx - a.L[1] && c[q].ihi));   // fdtdrav.h:752,777
         c[q].sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
         c[q].px = c[q].xs ? a.px_prof[i] : make_float4(0.f, 0.f, 1.f, 0.f);
         c[q].mycol = j * nx + i;
@@ -418,7 +418,7 @@ __device__ __forceinline__ void hp_fetch(HPair &p, const KArgs &a, const PRow &r
         p.p_zy[0] = lo_f(v, ex);
         p.p_zy[1] = hi_f(v, ex);
     }
-    if (V == V_GEN && hp_zslab(a, k)) {
+    if ((V & V_Z) && hp_zslab(a, k)) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
         const u32 qq = (((u32)sz * ra.ny + r.j) * ra.px + c[0].i) * S;
         const u32 u = ld2((const T *)ra.psi_hxz, qq), v = ld2((const T *)ra.psi_hyz, qq);
@@ -441,7 +441,7 @@ __device__ __forceinline__ void h_cell(const KArgs &a, const PRow &r, const PCel
     ConstKArgs &ra = rare_args(a);
     const int nz = a.nz;
     const float4 cx = t.x, cy = t.y, cz = t.z;
-    const bool khi = V != V_GEN || k < nz - 1;
+    const bool khi = !(V & V_Z) || k < nz - 1;
     ux = c.valid && r.jhi && khi;
     uy = c.valid && c.ihi && khi;
     uz = c.valid && c.ihi && r.jhi;
@@ -505,7 +505,7 @@ __device__ __forceinline__ void hp_update(const HPair &p, const CoefPair &t, con
 {
     const T *tp = nullptr;
     const u32 o = r.o0 + (u32)k * r.ob;
-    const bool zs = V == V_GEN && hp_zslab(a, k);
+    const bool zs = (V & V_Z) && hp_zslab(a, k);
     const float4 pzm = zs ? uload(a.pz_prof + a.nz + k) : make_float4(0.f, 0.f, 1.f, 0.f);
     const float ez0 = lo_f(p.ez, tp), ez1 = hi_f(p.ez, tp);
     const float exc0 = lo_f(ex_c, tp), exc1 = hi_f(ex_c, tp);
@@ -541,9 +541,9 @@ __global__ void __launch_bounds__(TX * BY) h_pair_kernel(const KArgs a, const in
     r.col = (i64)j * a.px + i0;
     r.ob = (u32)a.plane * (u32)sizeof(T);
     r.o0 = (u32)r.col * (u32)sizeof(T) + r.ob;
-    r.jlo = V != V_GEN || j > 0;
-    r.jhi = V != V_GEN || j < ny - 1;
-    r.ys = V == V_GEN && (j < a.L[2] || (j >= ny - a.L[3] && r.jhi));   // H y-slab, fdtdrav.h:447,471
+    r.jlo = !(V & V_Y) || j > 0;
+    r.jhi = !(V & V_Y) || j < ny - 1;
+    r.ys = (V & V_Y) && (j < a.L[2] || (j >= ny - a.L[3] && r.jhi));   // H y-slab, fdtdrav.h:447,471
     r.sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
     r.py = r.ys ? a.py_prof[ny + j] : make_float4(0.f, 0.f, 1.f, 0.f);
 
@@ -551,10 +551,10 @@ __global__ void __launch_bounds__(TX * BY) h_pair_kernel(const KArgs a, const in
     for (int q = 0; q < 2; ++q) {
         const int i = i0 + q;
         c[q].i = i;
-        c[q].valid = V == V_IN || i < nx;
-        c[q].ilo = V == V_IN || i > 0;
-        c[q].ihi = V == V_IN || i < nx - 1;
-        c[q].xs = V != V_IN && c[q].valid && (i < a.L[0] || (i >= nx - a.L[1] && c[q].ihi));   // fdtdrav.h:402,424
+        c[q].valid = !(V & V_X) || i < nx;
+        c[q].ilo = !(V & V_X) || i > 0;
+        c[q].ihi = !(V & V_X) || i < nx - 1;
+        c[q].xs = (V & V_X) && c[q].valid && (i < a.L[0] || (i >= nx - a.L[1] && c[q].ihi));   // fdtdrav.h:402,424
         c[q].sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
         c[q].px = c[q].xs ? a.px_prof[nx + i] : make_float4(0.f, 0.f, 1.f, 0.f);
         c[q].mycol = j * nx + i;
@@ -626,14 +626,11 @@ static void launch_pair_v(bool efield, int mode, const KArgs &a, int m, int nt, 
 template <class T>
 static hipError_t launch_pair_t(bool efield, int mode, const KArgs &a, int m, hipStream_t s)
 {
-    const int nb = (a.ib[3] - a.ib[0]) * (a.ib[4] - a.ib[1]) * (a.ib[5] - a.ib[2]);
-    if (nb > 0) {
-        launch_pair_v<T, V_IN>(efield, mode, a, m, nb, s);
-        if (a.nxlist > 0) launch_pair_v<T, V_XE>(efield, mode, a, m, a.nxlist, s);
-        if (a.nglist > 0) launch_pair_v<T, V_GEN>(efield, mode, a, m, a.nglist, s);
-    } else {
-        launch_pair_v<T, V_GEN>(efield, mode, a, m, a.tiles_x * a.tiles_y * ((a.nz + a.kchunk - 1) / a.kchunk), s);
-    }
+    int nb, nx, ng;
+    region_tiles(a, nb, nx, ng);
+    if (nb > 0) launch_pair_v<T, V_IN>(efield, mode, a, m, nb, s);
+    if (nx > 0) launch_pair_v<T, V_X>(efield, mode, a, m, nx, s);
+    if (ng > 0) launch_pair_v<T, V_GEN>(efield, mode, a, m, ng, s);
     return hipGetLastError();
 }
 
