@@ -44,6 +44,11 @@ EXPORTS = {
     "fdtd_get_probes": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
     "fdtd_get_dft": (C.c_int, [_P, C.POINTER(C.c_double), C.c_size_t]),
     "fdtd_get_field": (C.c_int, [_P, C.c_int, C.POINTER(C.c_float), C.c_size_t]),
+    "fdtd_set_port": (C.c_int, [_P] + [C.c_int] * 14),
+    "fdtd_get_port": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t]),
+    "fdtd_port_spectrum": (C.c_int, [_P, C.c_double, C.c_int, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), C.c_size_t]),
+    "fdtd_save_port": (C.c_int, [_P, C.c_char_p, C.c_char_p]),
     "fdtd_checkpoint_size": (C.c_int, [_P, C.POINTER(C.c_size_t)]),
     "fdtd_checkpoint_save": (C.c_int, [_P, _P, C.c_size_t]),
     "fdtd_checkpoint_restore": (C.c_int, [_P, _P, C.c_size_t]),
@@ -191,6 +196,13 @@ class Simulation:
         self._check(self.lib.fdtd_set_dft(self.h, len(f), _dptr(f)))
         self.n_freqs = len(f)
 
+    def set_port(self, vbox, ibox, vdir=3, idir=3):
+        """Port voltage box and current box (is, js, ks, ie, je, ke, inclusive
+        indices as the reference's sampled boxes).  Adds the probes the two
+        reductions read after the caller's own."""
+        self._check(self.lib.fdtd_set_port(self.h, vdir, *vbox, idir, *ibox))
+        self.n_probes = self.info()["n_probes"]
+
     def set_design_region(self, ijk):
         self._check(self.lib.fdtd_set_design_region(self.h, *ijk))
         self.design_cells = (ijk[3] - ijk[0]) * (ijk[4] - ijk[1]) * (ijk[5] - ijk[2])
@@ -228,6 +240,27 @@ class Simulation:
         out = np.zeros((self.n_probes, self.n_freqs, 2), dtype=np.float64)
         self._check(self.lib.fdtd_get_dft(self.h, _dptr(out), out.size))
         return out[..., 0] + 1j * out[..., 1]
+
+    def port(self):
+        """(V, I): port voltage and averaged current per step (float32)."""
+        steps = self.info()["steps_done"]
+        v = np.zeros(steps, dtype=np.float32)
+        i = np.zeros(steps, dtype=np.float32)
+        self._check(self.lib.fdtd_get_port(self.h, _fptr(v), _fptr(i), steps))
+        return v, i
+
+    def port_spectrum(self, z0, freqs) -> dict:
+        """V(f), I(f), S11 and Zin (complex128 arrays) at the given frequencies."""
+        f = np.ascontiguousarray(freqs, dtype=np.float64)
+        out = np.zeros((len(f), 8), dtype=np.float64)
+        self._check(self.lib.fdtd_port_spectrum(self.h, float(z0), len(f), _dptr(f), _dptr(out),
+                                                out.size))
+        c = out[:, 0::2] + 1j * out[:, 1::2]
+        return {"V": c[:, 0], "I": c[:, 1], "S11": c[:, 2], "Zin": c[:, 3]}
+
+    def save_port(self, vpath, ipath):
+        """results1003.dat / results1004.dat style text files of V and I."""
+        self._check(self.lib.fdtd_save_port(self.h, os.fsencode(vpath), os.fsencode(ipath)))
 
     def field(self, comp) -> np.ndarray:
         out = np.zeros((self.nz, self.ny, self.nx), dtype=np.float32)

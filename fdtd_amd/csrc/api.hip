@@ -47,6 +47,9 @@ struct fdtd_ctx {
     KArgs ka{};
     int e_mode = 0, h_mode = 0;
     int steps_done = 0;
+    bool has_port = false;          // fdtd_set_port: its samples are probes port_base ..
+    int port_base = 0;
+    PortPlan port;
     double last_run_ms = 0.0, e_ms = 0.0, h_ms = 0.0;
     size_t esz = 4;                 // bytes per stored value
     i64 field_elems = 0;            // per field allocation, guard planes included
@@ -1024,6 +1027,72 @@ int fdtd_get_probes(fdtd_ctx *c, float *out, size_t n)
     if (np && ns)
         HIPCHK(c, hipMemcpy2D(out, ns * sizeof(float), c->ka.probe_hist, (size_t)c->in.steps_cap * sizeof(float),
                               ns * sizeof(float), np, hipMemcpyDeviceToHost));
+    return FDTD_OK;
+}
+
+int fdtd_set_port(fdtd_ctx *c, int vdir, int vis, int vjs, int vks, int vie, int vje, int vke, int idir, int iis,
+                  int ijs, int iks, int iie, int ije, int ike)
+{
+    if (!c) return FDTD_E_INVALID;
+    if (c->finalized) return fail(c, FDTD_E_STATE, "scene is finalized");
+    if (c->has_port) return fail(c, FDTD_E_STATE, "the port is already set (one port per context)");
+    PortDef d{vdir, {vis, vjs, vks, vie, vje, vke}, idir, {iis, ijs, iks, iie, ije, ike}};
+    std::string err;
+    const SceneIn &in = c->in;
+    int rc = port_plan(d, in.nx, in.ny, in.nz, (float)in.dx, (float)in.dy, (float)in.dz, c->port, err);
+    if (rc != FDTD_OK) return fail(c, rc, "%s", err.c_str());
+    c->port_base = (int)c->in.probes.size();
+    c->in.probes.insert(c->in.probes.end(), c->port.samples.begin(), c->port.samples.end());
+    c->has_port = true;
+    return FDTD_OK;
+}
+
+// V and averaged I of the steps run so far (host copies of the port's probe
+// histories, reduced by port_series).
+static int port_read(fdtd_ctx *c, std::vector<float> &v, std::vector<float> &cur)
+{
+    if (!c->finalized) return fail(c, FDTD_E_STATE, "not finalized");
+    if (!c->has_port) return fail(c, FDTD_E_STATE, "no port was set");
+    const size_t np = c->port.samples.size(), ns = (size_t)c->steps_done;
+    v.assign(ns, 0.f);
+    cur.assign(ns, 0.f);
+    if (!ns) return FDTD_OK;
+    std::vector<float> hist(np * ns);
+    HIPCHK(c, hipMemcpy2D(hist.data(), ns * sizeof(float), c->ka.probe_hist + (size_t)c->port_base * c->in.steps_cap,
+                          (size_t)c->in.steps_cap * sizeof(float), ns * sizeof(float), np, hipMemcpyDeviceToHost));
+    port_series(c->port, hist.data(), ns, v.data(), cur.data());
+    return FDTD_OK;
+}
+
+int fdtd_get_port(fdtd_ctx *c, float *v, float *i, size_t n)
+{
+    if (!c || ((!v || !i) && n)) return FDTD_E_INVALID;
+    std::vector<float> vv, ii;
+    if (int rc = port_read(c, vv, ii)) return rc;
+    if (n < vv.size()) return fail(c, FDTD_E_INVALID, "output holds %zu values, need %zu", n, vv.size());
+    std::copy(vv.begin(), vv.end(), v);
+    std::copy(ii.begin(), ii.end(), i);
+    return FDTD_OK;
+}
+
+int fdtd_port_spectrum(fdtd_ctx *c, double z0, int nfreq, const double *freqs, double *out, size_t n)
+{
+    if (!c || nfreq < 0 || (nfreq && (!freqs || !out))) return FDTD_E_INVALID;
+    if (!(z0 > 0.0) || !std::isfinite(z0)) return fail(c, FDTD_E_INVALID, "z0 must be positive");
+    if (n < 8 * (size_t)nfreq) return fail(c, FDTD_E_INVALID, "output holds %zu values, need %zu", n, 8 * (size_t)nfreq);
+    std::vector<float> vv, ii;
+    if (int rc = port_read(c, vv, ii)) return rc;
+    port_spectrum(vv.data(), ii.data(), vv.size(), c->in.dt, z0, nfreq, freqs, out);
+    return FDTD_OK;
+}
+
+int fdtd_save_port(fdtd_ctx *c, const char *vpath, const char *ipath)
+{
+    if (!c || !vpath || !ipath) return FDTD_E_INVALID;
+    std::vector<float> vv, ii;
+    if (int rc = port_read(c, vv, ii)) return rc;
+    if (port_save(vv.data(), vv.size(), vpath) != FDTD_OK) return fail(c, FDTD_E_INVALID, "cannot write %s", vpath);
+    if (port_save(ii.data(), ii.size(), ipath) != FDTD_OK) return fail(c, FDTD_E_INVALID, "cannot write %s", ipath);
     return FDTD_OK;
 }
 

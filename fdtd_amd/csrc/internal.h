@@ -189,4 +189,27 @@ struct SceneIn {
 // scene.cpp
 int build_scene(const SceneIn &in, Built &out, std::string &err);
 
+// port.cpp: port voltage / current boxes (fdtd_set_port) and the probes the
+// reductions read.  v: voltage samples in summation order; ia..id: the four
+// line sums of the current loop, I = ca*sum(ia) + cb*sum(ib) - cc*sum(ic) -
+// cd*sum(id).  Sample indices are into `samples`.
+struct PortDef {
+    int vdir, vbox[6];   // is, js, ks, ie, je, ke
+    int idir, ibox[6];
+};
+struct PortPlan {
+    PortDef def;
+    std::vector<Probe> samples;
+    std::vector<int> v, ia, ib, ic, id;
+    float vscale, ca, cb, cc, cd;
+};
+int port_plan(const PortDef &d, int nx, int ny, int nz, float dx, float dy, float dz, PortPlan &p,
+              std::string &err);
+// hist: [sample][ns] probe histories of plan.samples; v, cur: [ns]
+void port_series(const PortPlan &p, const float *hist, size_t ns, float *v, float *cur);
+// out: [nfreq][8] = Re/Im of V(f), I(f), S11, Zin
+void port_spectrum(const float *v, const float *cur, size_t ns, double dt, double z0, int nfreq,
+                   const double *freqs, double *out);
+int port_save(const float *x, size_t ns, const char *path);
+
 }  // namespace fdtd

@@ -229,6 +229,42 @@ int fdtd_get_dft(fdtd_ctx *ctx, double *out, size_t n);
 /* One field component as FP32 SI values on the logical grid (n >= nx*ny*nz). */
 int fdtd_get_field(fdtd_ctx *ctx, int comp, float *out, size_t n);
 
+/* ---- port voltage / current (the reference's results1003/1004.dat) ---- */
+
+/* Port of the run: the sampled-voltage box and the sampled-current box, each
+ * given as is, js, ks, ie, je, ke with the reference's inclusive indices
+ * (calc_index of the v_* / i_* extents, fdtdrav.h:3614-3650) and a direction
+ * (1 x, 2 y, 3 z).
+ *   V[m] = -dz / ((ie-is+1)(je-js+1)) * sum Ez over is<=i<=ie, js<=j<=je,
+ *          ks<=k<ke   (voltMgpu case 3, fdtdrav.h:2252-2281; the reference
+ *          samples only Ez, so vdir must be 3);
+ *   I[m] = loop integral of H around the current box at the box's last
+ *          plane along idir (currMgpu, fdtdrav.h:2301-2487), averaged with
+ *          the next sample (correcI, fdtdrav.h:2490-2507).
+ * The field values the two reductions read are sampled as probes (appended
+ * after the caller's; they count in fdtd_get_probes).  One port per context,
+ * set before fdtd_finalize.  Replaces voltMgpu / currMgpu / sampledAdj /
+ * correcI of calcVoltCurrent (fdtdrav.h:3125-3185). */
+int fdtd_set_port(fdtd_ctx *ctx, int vdir, int vis, int vjs, int vks, int vie, int vje, int vke,
+                  int idir, int iis, int ijs, int iks, int iie, int ije, int ike);
+
+/* V[m] and the averaged I[m] of the steps run so far, in volts and amperes
+ * (FP32, the reference's summation order).  n >= steps_done. */
+int fdtd_get_port(fdtd_ctx *ctx, float *v, float *i, size_t n);
+
+/* Port spectrum at nfreq frequencies (Hz): out[8q .. 8q+7] = Re, Im of V(f),
+ * I(f), S11 and Zin, with X(f) = sum_m x[m] exp(-2 pi i f m dt) (cuFFT's R2C
+ * bin k at f = k / (n dt)) and c_zinS11's algebra (fdtdrav.h:2510-2546):
+ * a, b = (V +- z0 I) / (2 sqrt(z0)), S11 = b / a, Zin = z0 (1 + S11) / (1 - S11).
+ * Replaces calc_otm (fdtdrav.h:2988-3051, disabled in the reference).
+ * z0 > 0; n >= 8 * nfreq. */
+int fdtd_port_spectrum(fdtd_ctx *ctx, double z0, int nfreq, const double *freqs, double *out, size_t n);
+
+/* Writes V and the averaged I as saveTDomain does (fdtdrav.h:2703-2717): one
+ * "%e \n" line per step, to vpath and ipath (results1003.dat and
+ * results1004.dat in the reference's working directory). */
+int fdtd_save_port(fdtd_ctx *ctx, const char *vpath, const char *ipath);
+
 /* ---- checkpoint / resume (no counterpart in the reference) ---------- */
 
 int fdtd_checkpoint_size(const fdtd_ctx *ctx, size_t *bytes);

@@ -43,3 +43,41 @@ def test_pins_catch_planted_mistake(name, tmp_path):
                         "-k", "not reflection"], cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
     assert r.returncode == 1, r.stdout[-2000:] + r.stderr[-2000:]
     assert "failed" in r.stdout
+
+
+# ---- port reductions (oracle/port.py) -------------------------------------------
+
+PORT_SRC = os.path.join(ROOT, "oracle", "port.py")
+PORT_MUTANTS = {
+    # z-directed loop one plane too high
+    "loop_plane": ('                if k == ke - 1:\n                    acc("yp"', '                if k == ke:\n                    acc("yp"'),
+    # one line of the loop with the wrong sign
+    "loop_sign": ('- f32(dx) * sums["xn"])\n    else:', '+ f32(dx) * sums["xn"])\n    else:'),
+    # x-directed loop's z-line one row inside the box
+    "loop_row": ('acc("zp", H(HZ, cnx - 2, cny - 1, k))', 'acc("zp", H(HZ, cnx - 2, cny - 2, k))'),
+    # y-directed loop's x-line on the wrong row
+    "loop_col": ('acc("xp", H(HX, i, cny - 2, k))', 'acc("xp", H(HX, i, cny - 1, k))'),
+    # voltage with the sign of -dz dropped
+    "volt_sign": ("csuf = f32(-f32(dz))", "csuf = f32(f32(dz))"),
+    # cross-section count off by one
+    "volt_area": ("(_je - _js + f32(1)))", "(_je - _js + f32(2)))"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(PORT_MUTANTS))
+def test_port_pins_catch_planted_mistake(name, tmp_path):
+    old, new = PORT_MUTANTS[name]
+    text = open(PORT_SRC).read()
+    assert text.count(old) == 1, f"mutation site for {name} not found"
+    pkg = tmp_path / "oracle"
+    pkg.mkdir()
+    for f in ("__init__.py", "oracle.py"):
+        (pkg / f).write_text(open(os.path.join(ROOT, "oracle", f)).read())
+    (pkg / "port.py").write_text(text.replace(old, new))
+    env = dict(os.environ, PYTHONPATH=f"{tmp_path}{os.pathsep}{ROOT}",
+               ORACLE_LIB=os.path.join(ROOT, "oracle", "liboracle.so"))
+    r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider", "-m", "not gpu",
+                        os.path.join(ROOT, "tests", "test_port.py")], cwd=tmp_path, env=env,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 1, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "failed" in r.stdout
