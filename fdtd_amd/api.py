@@ -32,6 +32,8 @@ EXPORTS = {
     "fdtd_set_cpml": (C.c_int, [_P, _P]),
     "fdtd_add_box": (C.c_int, [_P] + [C.c_int] * 6 + [C.POINTER(C.c_float)] * 4),
     "fdtd_add_lumped_source": (C.c_int, [_P, C.c_int] + [C.c_int] * 6 + [C.c_double, _P]),
+    "fdtd_add_plate": (C.c_int, [_P, C.c_int] + [C.c_int] * 6 + [C.POINTER(C.c_float)]),
+    "fdtd_add_resistor": (C.c_int, [_P, C.c_int] + [C.c_int] * 6 + [C.c_double]),
     "fdtd_add_probe": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "fdtd_set_dft": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
     "fdtd_set_design_region": (C.c_int, [_P] + [C.c_int] * 6),
@@ -185,6 +187,13 @@ class Simulation:
         w = FdtdWaveform(wtype, amp, t0, tau, freq)
         self._check(self.lib.fdtd_add_lumped_source(self.h, direction, *ijk, R, C.byref(w)))
 
+    def add_plate(self, axis, ijk, sigma):
+        sg = (C.c_float * 3)(*sigma)
+        self._check(self.lib.fdtd_add_plate(self.h, axis, *ijk, sg))
+
+    def add_resistor(self, ijk, R, direction=3):
+        self._check(self.lib.fdtd_add_resistor(self.h, direction, *ijk, float(R)))
+
     def add_probe(self, comp, i, j, k) -> int:
         idx = C.c_int()
         self._check(self.lib.fdtd_add_probe(self.h, comp, i, j, k, C.byref(idx)))
@@ -302,8 +311,14 @@ class Simulation:
             sim.add_box(b.ijk, b.eps_r, b.sigma_e, b.mu_r, b.sigma_m)
         for s in sc.sources:
             sim.add_lumped_source(s.ijk, s.R, s.wtype, s.amp, s.t0, s.tau, s.freq)
+        for p in getattr(sc, "plates", []):
+            sim.add_plate(p.axis, p.ijk, p.sigma)
+        for r in getattr(sc, "resistors", []):
+            sim.add_resistor(r.ijk, r.R)
         for (comp, i, j, k) in sc.probes:
             sim.add_probe(comp, i, j, k)
+        if getattr(sc, "port", None) is not None:
+            sim.set_port(sc.port.vbox, sc.port.ibox, vdir=3, idir=sc.port.idir)
         if sc.freqs:
             sim.set_dft(sc.freqs)
         if sc.design is not None:

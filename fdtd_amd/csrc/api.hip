@@ -877,6 +877,37 @@ int fdtd_add_box(fdtd_ctx *c, int is, int js, int ks, int ie, int je, int ke, co
     return FDTD_OK;
 }
 
+int fdtd_add_plate(fdtd_ctx *c, int axis, int is, int js, int ks, int ie, int je, int ke, const float sigma[3])
+{
+    if (!c || !sigma) return FDTD_E_INVALID;
+    if (c->finalized) return fail(c, FDTD_E_STATE, "scene is finalized");
+    if (axis < 1 || axis > 3) return fail(c, FDTD_E_INVALID, "plate axis must be 1, 2 or 3");
+    for (int q = 0; q < 3; ++q)
+        if (!(sigma[q] >= 0) || !std::isfinite(sigma[q])) return fail(c, FDTD_E_INVALID, "invalid plate sigma");
+    if (is < 0 || js < 0 || ks < 0 || ie >= c->in.nx || je >= c->in.ny || ke >= c->in.nz || is > ie || js > je ||
+        ks > ke)
+        return fail(c, FDTD_E_RANGE, "plate (%d,%d,%d)-(%d,%d,%d) outside the grid", is, js, ks, ie, je, ke);
+    Plate p{axis, {is, js, ks, ie, je, ke}, {sigma[0], sigma[1], sigma[2]}};
+    c->in.plates.push_back(p);
+    return FDTD_OK;
+}
+
+int fdtd_add_resistor(fdtd_ctx *c, int dir, int is, int js, int ks, int ie, int je, int ke, double R)
+{
+    if (!c) return FDTD_E_INVALID;
+    if (c->finalized) return fail(c, FDTD_E_STATE, "scene is finalized");
+    if (dir != 3) return fail(c, FDTD_E_UNSUPPORTED, "only z-directed resistors (dir = 3) are built");
+    if (!(R > 0) || !std::isfinite(R)) return fail(c, FDTD_E_INVALID, "resistance must be > 0");
+    const SceneIn &in = c->in;
+    if (is < 1 || js < 1 || ks < 0 || ie > in.nx - 2 || je > in.ny - 2 || ke > in.nz - 1 || is > ie || js > je ||
+        ks >= ke)
+        return fail(c, FDTD_E_RANGE, "resistor edges (%d,%d,%d)-(%d,%d,%d) outside the updated Ez edges", is, js, ks,
+                    ie, je, ke);
+    Resistor r{{is, js, ks, ie, je, ke}, R};
+    c->in.resistors.push_back(r);
+    return FDTD_OK;
+}
+
 int fdtd_add_lumped_source(fdtd_ctx *c, int dir, int is, int js, int ks, int ie, int je, int ke, double R,
                            const fdtd_waveform *w)
 {

@@ -142,6 +142,14 @@ struct Source {
 struct Probe {
     int comp, i, j, k;
 };
+struct Plate {
+    int axis, b[6];
+    float sigma[3];
+};
+struct Resistor {
+    int b[6];
+    double R;
+};
 
 struct CpmlParams {
     int thick[6];
@@ -179,6 +187,8 @@ struct SceneIn {
     CpmlParams cpml;
     std::vector<Box> boxes;
     std::vector<Source> sources;
+    std::vector<Plate> plates;
+    std::vector<Resistor> resistors;
     std::vector<Probe> probes;
     std::vector<double> freqs;
     int design[6];

@@ -12,6 +12,8 @@
 //   coefficients        setCoefficients      fdtdrav.h:1247-1305
 //   CPML profiles       setABCPML            fdtdrav.h:1774-2155
 //   source loading      defineSourceZ        fdtdrav.h:1391-1421
+//   conducting plates   definePlateX/Y/Z     fdtdrav.h:1574-1678
+//   lumped resistors    defineResistor       fdtdrav.h:1429-1484
 //   per-node waveform   setGauss / setSine   fdtdrav.h:1486-1520
 #include <hip/hip_runtime.h>
 
@@ -84,7 +86,7 @@ inline void hcoef(const Consts &c, float mu, float sig, float da, float db, floa
 }
 
 struct KeyHash {
-    size_t operator()(const std::array<std::uint32_t, 8> &k) const
+    size_t operator()(const std::array<std::uint32_t, 12> &k) const
     {
         std::uint64_t h = 1469598103934665603ull;
         for (std::uint32_t v : k) { h ^= v; h *= 1099511628211ull; h ^= h >> 29; }
@@ -143,16 +145,98 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
     }
     if (mats.size() >= 0xFFFFFFF0u) { err = "too many materials"; return -1; }
 
+    // ---- plates: averaged sigma_e overrides of the edges in each plate, in
+    // call order (definePlateX/Y/Z, fdtdrav.h:1574-1678, applied between
+    // setSigmaEpsSigmaMu and setCoefficients, fdtdrav.h:4183-4210)
+    std::unordered_map<i64, float> sig_over[3];
+    for (const Plate &pl : in.plates) {
+        const int *b = pl.b;
+        auto put = [&](int comp, int i0, int i1, int j0, int j1, int k0, int k1) {   // inclusive ranges
+            for (int k = k0; k <= k1; ++k)
+                for (int j = j0; j <= j1; ++j)
+                    for (int i = i0; i <= i1; ++i) sig_over[comp][i + nx * (j + (i64)ny * k)] = pl.sigma[comp];
+        };
+        if (pl.axis == 1) {          // plane i = is
+            put(1, b[0], b[0], b[1], b[4] - 1, b[2], b[5]);
+            put(2, b[0], b[0], b[1], b[4], b[2], b[5] - 1);
+        } else if (pl.axis == 2) {   // plane j = js
+            put(2, b[0], b[3], b[1], b[1], b[2], b[5] - 1);
+            put(0, b[0], b[3] - 1, b[1], b[1], b[2], b[5]);
+        } else {                     // plane k = ks
+            put(0, b[0], b[3] - 1, b[1], b[4], b[2], b[2]);
+            put(1, b[0], b[3], b[1], b[4] - 1, b[2], b[2]);
+        }
+    }
+    auto edge_sigma = [&](int comp, size_t x, float avg) {
+        if (sig_over[comp].empty()) return avg;
+        auto it = sig_over[comp].find((i64)x);
+        return it == sig_over[comp].end() ? avg : it->second;
+    };
+    // ---- resistors: Ez edges with the resistor-loaded coefficients
+    // (defineResistor, fdtdrav.h:1429-1484; after the source and the CPML,
+    // so a resistor edge keeps its curl coefficients undivided by kappa --
+    // resistors are therefore kept off the CPML slabs)
+    std::unordered_map<i64, int> res_of;
+    {
+        const int *L = in.cpml.thick;
+        for (size_t r = 0; r < in.resistors.size(); ++r) {
+            const int *b = in.resistors[r].b;
+            if (b[0] < L[0] || b[3] >= nx - L[1] || b[1] < L[2] || b[4] >= ny - L[3] || b[2] < L[4] ||
+                b[5] > nz - L[5]) {
+                err = "resistor inside a CPML slab";
+                return -1;
+            }
+            for (int k = b[2]; k < b[5]; ++k)
+                for (int j = b[1]; j <= b[4]; ++j)
+                    for (int i = b[0]; i <= b[3]; ++i) res_of[i + nx * (j + (i64)ny * k)] = (int)r;
+        }
+        for (const Source &sr : in.sources)
+            for (int k = sr.b[2]; k < sr.b[5]; ++k)
+                for (int j = sr.b[1]; j <= sr.b[4]; ++j)
+                    for (int i = sr.b[0]; i <= sr.b[3]; ++i)
+                        if (res_of.count(i + nx * (j + (i64)ny * k))) {
+                            err = "a resistor overlaps a lumped source";
+                            return -1;
+                        }
+        if (!in.design_eps.empty() && (!in.plates.empty() || !in.resistors.empty())) {
+            // the adjoint's edge derivatives assume the averaged materials
+            const int *d = in.design;
+            auto near = [&](i64 cell) {
+                const int i = (int)(cell % nx), j = (int)((cell / nx) % ny), k = (int)(cell / nxy);
+                return i >= d[0] - 1 && i <= d[3] && j >= d[1] - 1 && j <= d[4] && k >= d[2] - 1 && k <= d[5];
+            };
+            for (int c = 0; c < 3; ++c)
+                for (const auto &kv : sig_over[c])
+                    if (near(kv.first)) { err = "a plate touches the design region"; return -1; }
+            for (const auto &kv : res_of)
+                if (near(kv.first)) { err = "a resistor touches the design region"; return -1; }
+        }
+    }
+    // resistor-loaded Ez coefficients (defineResistor's operand order)
+    auto rcoef = [&](const Resistor &rs, float e, float sg, float *o) {
+        const float _is = (float)rs.b[0], _js = (float)rs.b[1], _ks = (float)rs.b[2];
+        const float _ie = (float)rs.b[3], _je = (float)rs.b[4], _ke = (float)rs.b[5];
+        float r_comp = ((1.0f + _ie) - _is) * ((1.0f + _je) - _js) / (_ke - _ks);
+        r_comp = r_comp * (float)rs.R;
+        const float rfactor = (C.dt * C.dz) / (r_comp * C.dx * C.dy);
+        const float two = 2.0f;
+        const float D = two * e * C.eps0 + C.dt * sg + rfactor;
+        o[0] = (two * e * C.eps0 - C.dt * sg - rfactor) / D;
+        o[1] = (two * C.dt / C.dx) / D;
+        o[2] = (-2.0f * C.dt / C.dy) / D;
+    };
+
     // ---- per-cell coefficient tuples, deduplicated by value
     // The tuple of a cell depends on 7 labels -- (i,j,k) and its -x, -y, -z,
-    // -x-y, -y-z, -x-z neighbours -- and on which faces the cell lies on.
-    // Components not updated at a cell get the vacuum values (never read).
+    // -x-y, -y-z, -x-z neighbours -- on which faces the cell lies on, and on
+    // its plate sigma overrides and resistor.  Components not updated at a
+    // cell get the vacuum values (never read).
     const std::uint32_t NONE = 0xFFFFFFFFu;
-    std::unordered_map<std::array<std::uint32_t, 8>, int, KeyHash> key2cls;
+    std::unordered_map<std::array<std::uint32_t, 12>, int, KeyHash> key2cls;
     std::unordered_map<std::array<std::uint32_t, 18>, int, TupHash> tup2cls;
     std::vector<std::array<float, 18>> tuples;
     std::vector<std::uint32_t> cellcls((size_t)N);
-    std::array<std::uint32_t, 8> prev;
+    std::array<std::uint32_t, 12> prev;
     prev.fill(NONE - 1);
     int prev_cls = -1;
     float vac[18];
@@ -168,7 +252,7 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 const size_t x = (size_t)(i + nx * (j + (i64)ny * k));
-                std::array<std::uint32_t, 8> key;
+                std::array<std::uint32_t, 12> key;
                 key[0] = lab[x];
                 key[1] = i > 0 ? lab[x - 1] : NONE;
                 key[2] = j > 0 ? lab[x - nx] : NONE;
@@ -178,6 +262,18 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
                 key[6] = (i > 0 && k > 0) ? lab[x - 1 - nxy] : NONE;
                 key[7] = (std::uint32_t)((i == 0) | (i == nx - 1) << 1 | (j == 0) << 2 | (j == ny - 1) << 3 |
                                          (k == 0) << 4 | (k == nz - 1) << 5);
+                for (int c = 0; c < 3; ++c) {
+                    key[8 + c] = NONE;
+                    if (!sig_over[c].empty()) {
+                        auto it = sig_over[c].find((i64)x);
+                        if (it != sig_over[c].end()) std::memcpy(&key[8 + c], &it->second, 4);
+                    }
+                }
+                key[11] = 0;
+                if (!res_of.empty()) {
+                    auto it = res_of.find((i64)x);
+                    if (it != res_of.end()) key[11] = (std::uint32_t)it->second + 1u;
+                }
                 if (key == prev) { cellcls[x] = (std::uint32_t)prev_cls; continue; }
                 auto it = key2cls.find(key);
                 int cid;
@@ -191,20 +287,21 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
                     // Ex: i<nx-1, 0<j<ny-1, 0<k<nz-1; labels (i,j,k),(j-1),(k-1),(j-1,k-1)
                     if (i < nx - 1 && j > 0 && j < ny - 1 && k > 0 && k < nz - 1) {
                         float e = avg4(m0.v[0], M(key[2]).v[0], M(key[3]).v[0], M(key[5]).v[0]);
-                        float s = avg4(m0.v[3], M(key[2]).v[3], M(key[3]).v[3], M(key[5]).v[3]);
+                        float s = edge_sigma(0, x, avg4(m0.v[3], M(key[2]).v[3], M(key[3]).v[3], M(key[5]).v[3]));
                         ecoef(C, e, s, C.dy, C.dz, t + 0);
                     }
                     // Ey: 0<i<nx-1, j<ny-1, 0<k<nz-1; labels (i,j,k),(i-1),(k-1),(i-1,k-1)
                     if (i > 0 && i < nx - 1 && j < ny - 1 && k > 0 && k < nz - 1) {
                         float e = avg4(m0.v[1], M(key[1]).v[1], M(key[3]).v[1], M(key[6]).v[1]);
-                        float s = avg4(m0.v[4], M(key[1]).v[4], M(key[3]).v[4], M(key[6]).v[4]);
+                        float s = edge_sigma(1, x, avg4(m0.v[4], M(key[1]).v[4], M(key[3]).v[4], M(key[6]).v[4]));
                         ecoef(C, e, s, C.dz, C.dx, t + 3);
                     }
                     // Ez: 0<i<nx-1, 0<j<ny-1, k<nz-1; labels (i,j,k),(i-1),(j-1),(i-1,j-1)
                     if (i > 0 && i < nx - 1 && j > 0 && j < ny - 1 && k < nz - 1) {
                         float e = avg4(m0.v[2], M(key[1]).v[2], M(key[2]).v[2], M(key[4]).v[2]);
-                        float s = avg4(m0.v[5], M(key[1]).v[5], M(key[2]).v[5], M(key[4]).v[5]);
-                        ecoef(C, e, s, C.dx, C.dy, t + 6);
+                        float s = edge_sigma(2, x, avg4(m0.v[5], M(key[1]).v[5], M(key[2]).v[5], M(key[4]).v[5]));
+                        if (key[11]) rcoef(in.resistors[key[11] - 1], e, s, t + 6);
+                        else ecoef(C, e, s, C.dx, C.dy, t + 6);
                     }
                     // Hx: j<ny-1, k<nz-1; mu_x harmonic over (i, i-1) when i > 0
                     if (j < ny - 1 && k < nz - 1) {
@@ -386,8 +483,8 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
                     const size_t x = (size_t)cell;
                     float e = avg4(mats[lab[x]].v[2], mats[lab[x - 1]].v[2], mats[lab[x - nx]].v[2],
                                    mats[lab[x - 1 - nx]].v[2]);
-                    float sg = avg4(mats[lab[x]].v[5], mats[lab[x - 1]].v[5], mats[lab[x - nx]].v[5],
-                                    mats[lab[x - 1 - nx]].v[5]);
+                    float sg = edge_sigma(2, x, avg4(mats[lab[x]].v[5], mats[lab[x - 1]].v[5], mats[lab[x - nx]].v[5],
+                                                     mats[lab[x - 1 - nx]].v[5]));
                     const float two = 2.0f;
                     float D = two * e * C.eps0 + C.dt * sg + rfactor;
                     SrcEdge se{};

@@ -97,6 +97,32 @@ class Source:
 
 
 @dataclass
+class Plate:
+    """Zero-thickness conducting plate (definePlateX/Y/Z, fdtdrav.h:1574-1678):
+    axis 1/2/3 and the inclusive index box of its calc_index'ed extents."""
+    axis: int
+    ijk: Tuple[int, int, int, int, int, int]
+    sigma: Tuple[float, float, float]
+
+
+@dataclass
+class Resistor:
+    """Lumped resistor on the Ez edges is<=i<=ie, js<=j<=je, ks<=k<ke
+    (defineResistor, fdtdrav.h:1429-1484)."""
+    ijk: Tuple[int, int, int, int, int, int]
+    R: float
+
+
+@dataclass
+class Port:
+    """Sampled-voltage box (z) and sampled-current box of direction idir,
+    inclusive indices (fdtdrav.h:3614-3650)."""
+    vbox: Tuple[int, int, int, int, int, int]
+    ibox: Tuple[int, int, int, int, int, int]
+    idir: int = 3
+
+
+@dataclass
 class Scene:
     name: str
     nx: int
@@ -112,6 +138,9 @@ class Scene:
     sources: List[Source] = field(default_factory=list)
     probes: List[Tuple[int, int, int, int]] = field(default_factory=list)   # (comp, i, j, k)
     freqs: List[float] = field(default_factory=list)
+    plates: List[Plate] = field(default_factory=list)
+    resistors: List[Resistor] = field(default_factory=list)
+    port: Optional[Port] = None
     design: Optional[Tuple[int, int, int, int, int, int]] = None
     design_eps: Optional[np.ndarray] = None      # float32, x-fastest over the design region
     eps0: float = EPS0
@@ -127,6 +156,7 @@ class Scene:
                 "cell_m": [self.dx, self.dy, self.dz], "dt_s": self.dt, "steps": self.steps,
                 "cpml": list(self.cpml.thick), "boxes": len(self.boxes),
                 "sources": len(self.sources), "probes": len(self.probes),
+                "plates": len(self.plates), "resistors": len(self.resistors),
                 "dft_freqs": len(self.freqs), "seed": self.seed}
 
 
@@ -268,6 +298,57 @@ def adjoint768(**kw) -> Scene:
                              probe_freqs=(5e9, 10e9, 15e9, 20e9), n_probes=8, seed=1005,
                              name="adjoint768"), **kw})
 
+
+
+def patch_antenna(steps=3000) -> Scene:
+    """The reference's own scene (source.cu:17-226): a microstrip-fed patch on
+    a 60 x 100 x 3 cell substrate (eps_r 2.2, sigma 4e-4), ground plane, feed
+    line and patch as z-plates with sigma 1e10, a 50-ohm z-source of 7 x 3
+    edges driven by a Gaussian (nc = 20), the port 10 cells down the feed
+    line.  CPML 10 and 15 cells of air on every side.  Constants as the
+    reference's: pi = 3.1415, Courant factor 1, FP32 cell sizes.
+
+    The bounding box of source.cu:256-273 compares every maximum with `<`,
+    which would keep the first object's maximum; here the maximum over the
+    objects is taken (x 60 dx, y 100 dy, z 3 dz), as the loop intends.
+    """
+    f32 = np.float32
+    eps0 = f32(8.854187817e-12)
+    pi = f32(3.1415)
+    mu0 = f32(f32(4 * pi) * f32(1e-7))
+    c = f32(1.0 / math.sqrt(float(f32(mu0 * eps0))))
+    dx, dy, dz = f32(0.389e-3), f32(0.4e-3), f32(0.265e-3)
+    dt = f32(1.0 / (float(c) * math.sqrt(1.0 / float(dx) ** 2 + 1.0 / float(dy) ** 2 + 1.0 / float(dz) ** 2)))
+    pml, air = 10, 15
+    o = pml + air
+
+    def ci(coord, d):   # calc_index, fdtdrav.h:198-206
+        return int(round(float(f32(f32(coord) / d))))
+
+    X, Y, Z = ci(60 * dx, dx), ci(100 * dy, dy), ci(3 * dz, dz)
+    nx, ny, nz = X + 2 * o, Y + 2 * o, Z + 2 * o
+    brick = Box(ijk=(o, o, o, o + ci(60 * dx, dx), o + ci(100 * dy, dy), o + ci(3 * dz, dz)),
+                eps_r=(2.2, 2.2, 2.2), sigma_e=(0.0004, 0.0004, 0.0004),
+                mu_r=(1.0, 1.0, 1.0), sigma_m=(1.2e-38, 1.2e-38, 1.2e-38))
+    pec = [((0.0, 0.0, 0.0), (60 * dx, 100 * dy, 0.0)),
+           ((7.535e-3, 0.0, 3 * dz), (9.869e-3, 50 * dy, 3 * dz)),
+           ((5.445e-3, 50 * dy, 3 * dz), (17.895e-3, 50 * dy + 16e-3, 3 * dz))]
+    plates = [Plate(axis=3, ijk=(o + ci(a[0], dx), o + ci(a[1], dy), o + ci(a[2], dz),
+                                 o + ci(b[0], dx), o + ci(b[1], dy), o + ci(b[2], dz)),
+                    sigma=(1e10, 1e10, 1e10)) for a, b in pec]
+    tau = f32(f32(20 * dy) / f32(2 * c))
+    src = Source(ijk=(o + ci(7.535e-3, dx), o + ci(0.0, dy), o + ci(0.0, dz),
+                      o + ci(9.869e-3, dx), o + ci(0.0, dy), o + ci(3 * dz, dz)),
+                 R=50.0, wtype=WAVE_GAUSSIAN, amp=1.0, t0=float(f32(3 * tau)), tau=float(tau), freq=1e8)
+    vbox = (o + ci(7.535e-3, dx), o + ci(10 * dy, dy), o + ci(0.0, dz),
+            o + ci(9.869e-3, dx), o + ci(10 * dy, dy), o + ci(3 * dz, dz))
+    ibox = (o + ci(7.535e-3, dx), o + ci(10 * dy, dy), o + ci(3 * dz, dz),
+            o + ci(9.869e-3, dx), o + ci(10 * dy, dy), o + ci(3 * dz, dz))
+    cp = Cpml(thick=(pml,) * 6, order=3.0, sigma_max=(0.0, 0.0, 0.0), kappa_max=5.0, alpha_max=0.05,
+              alpha_min=0.0, sigma_factor=1.5)
+    return Scene(name="patch_antenna", nx=nx, ny=ny, nz=nz, dx=float(dx), dy=float(dy), dz=float(dz),
+                 dt=float(dt), steps=steps, cpml=cp, boxes=[brick], sources=[src], plates=plates,
+                 port=Port(vbox=vbox, ibox=ibox, idir=2), eps0=float(eps0), mu0=float(mu0), seed=0)
 
 WORKLOADS = {"yee64": yee64, "dielectric256": dielectric256, "slab512": slab512,
              "adjoint128": adjoint128, "adjoint768": adjoint768}

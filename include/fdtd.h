@@ -172,6 +172,28 @@ int fdtd_add_lumped_source(fdtd_ctx *ctx, int dir, int is, int js, int ks,
                            int ie, int je, int ke, double resistance,
                            const fdtd_waveform *wave);
 
+/* Zero-thickness conducting plate: overrides the averaged sigma_e of the
+ * edges lying in the plate after the edge averaging and before the
+ * coefficients, in call order (later plates win), as definePlateX / Y / Z
+ * do (fdtdrav.h:1574-1678; marchingLoop applies definePlateZ,
+ * fdtdrav.h:4183-4210).  axis 3: the plane k = ks, Ex edges is<=i<ie,
+ * js<=j<=je get sigma[0] and Ey edges is<=i<=ie, js<=j<je get sigma[1];
+ * axis 1 (plane i = is: Ey edges j<je, k<=ke get sigma[1], Ez edges j<=je,
+ * k<ke get sigma[2]) and axis 2 (plane j = js: Ez edges i<=ie, k<ke get
+ * sigma[2], Ex edges i<ie, k<=ke get sigma[0]) likewise.  sigma >= 0 (S/m;
+ * the reference's PEC plates use 1e10). */
+int fdtd_add_plate(fdtd_ctx *ctx, int axis, int is, int js, int ks, int ie, int je, int ke,
+                   const float sigma[3]);
+
+/* Lumped resistor on the Ez edges is<=i<=ie, js<=j<=je, ks<=k<ke: the Ez
+ * coefficients of those edges are replaced with the resistor-loaded ones
+ * of defineResistor (fdtdrav.h:1429-1484; the reference applies it after
+ * the source and the CPML, fdtdrav.h:4296-4309).  dir must be 3 (the
+ * reference's kernel loads Ez whatever its direction argument).  R > 0.
+ * Resistors may not overlap a lumped source or a CPML slab. */
+int fdtd_add_resistor(fdtd_ctx *ctx, int dir, int is, int js, int ks, int ie, int je, int ke,
+                      double resistance);
+
 /* Point probe of one component, sampled once per step after that
  * component's main update and before its CPML correction, as sampleField is
  * called in calc_e / calc_h (fdtdrav.h:234-246, 740, 347-391).  Returns the

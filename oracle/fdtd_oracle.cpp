@@ -68,6 +68,12 @@ typedef struct {
     int design[6];              /* is js ks ie je ke, ie <= is when absent */
     const float *design_eps;
     int nsteps;                 /* waveform / probe horizon */
+    int nplate;
+    const int *plate;           /* 7 per plate: axis is js ks ie je ke */
+    const float *plate_sigma;   /* 3 per plate: sigma_e x y z */
+    int nres;
+    const int *res;             /* 6 per resistor: is js ks ie je ke (z-directed) */
+    const double *res_R;
 } oracle_scene;
 
 }  // extern "C"
@@ -290,6 +296,29 @@ static Sim<R> *build(const oracle_scene *sc)
                     sm[2][x] = harm(m_sm[2][x], m_sm[2][x - NXY]);
                 }
             }
+    // ---- conducting plates, definePlateX/Y/Z H:1574-1678 (between the
+    // averaging and the coefficients, H:4183-4210), each kernel's thread
+    // conditions over every cell, plates in order
+    for (int p = 0; p < sc->nplate; ++p) {
+        const int *b = sc->plate + 7 * p;
+        const int axis = b[0], is = b[1], js = b[2], ks = b[3], ie = b[4], je = b[5], ke = b[6];
+        const float *sg = sc->plate_sigma + 3 * p;
+        for (int k = 0; k < NZ; ++k)
+            for (int j = 0; j < NY; ++j)
+                for (int i = 0; i < NX; ++i) {
+                    i64 x = s->id(i, j, k);
+                    if (axis == 1) {            // definePlateX H:1584-1596
+                        if (i == is && j >= js && j < je && k >= ks && k <= ke) se[1][x] = (R)sg[1];
+                        if (i == is && j >= js && j <= je && k >= ks && k < ke) se[2][x] = (R)sg[2];
+                    } else if (axis == 2) {     // definePlateY H:1618-1631
+                        if (i >= is && i <= ie && j == js && k >= ks && k < ke) se[2][x] = (R)sg[2];
+                        if (i >= is && i < ie && j == js && k >= ks && k <= ke) se[0][x] = (R)sg[0];
+                    } else {                    // definePlateZ H:1655-1672
+                        if (i >= is && i < ie && j >= js && j <= je && k == ks) se[0][x] = (R)sg[0];
+                        if (i >= is && i <= ie && j >= js && j < je && k == ks) se[1][x] = (R)sg[1];
+                    }
+                }
+    }
     for (int c = 0; c < 3; ++c) {
         std::vector<R>().swap(m_eps[c]); std::vector<R>().swap(m_se[c]);
         std::vector<R>().swap(m_mu[c]);  std::vector<R>().swap(m_sm[c]);
@@ -461,6 +490,26 @@ static Sim<R> *build(const oracle_scene *sc)
             double v = waveform<R>(sc->src_wtype[q], sc->src_wpar + 4 * q, (double)t * sc->dt);
             s->vpn[(size_t)q * sc->nsteps + t] = (R)v / (R)vmag;
         }
+    }
+
+    // ---- lumped resistors, defineResistor H:1429-1484 (after the sources,
+    // H:4296-4309; the kernel loads Ez whatever its direction argument)
+    for (int q = 0; q < sc->nres; ++q) {
+        const int *b = sc->res + 6 * q;
+        const R _is = (R)b[0], _js = (R)b[1], _ks = (R)b[2], _ie = (R)b[3], _je = (R)b[4], _ke = (R)b[5];
+        R r_comp = (((R)1 + _ie) - _is) * (((R)1 + _je) - _js) / (_ke - _ks);
+        r_comp = r_comp * (R)sc->res_R[q];
+        const R rfactor = (dt * dz) / (r_comp * dx * dy);
+        for (int k = 0; k < NZ; ++k)
+            for (int j = 0; j < NY; ++j)
+                for (int i = 0; i < NX; ++i) {
+                    if (!(i >= b[0] && i <= b[3] && j >= b[1] && j <= b[4] && k >= b[2] && k < b[5])) continue;
+                    i64 x = s->id(i, j, k);
+                    R D = two * eps[2][x] * eps_0 + dt * se[2][x] + rfactor;
+                    s->Ceze[x]  = (two * eps[2][x] * eps_0 - dt * se[2][x] - rfactor) / D;
+                    s->Cezhy[x] = (two * dt / dx) / D;
+                    s->Cezhx[x] = ((R)-2 * dt / dy) / D;
+                }
     }
 
     // ---- probes and DFT

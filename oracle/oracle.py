@@ -27,7 +27,9 @@ class OracleScene(C.Structure):
                 ("nprobe", C.c_int), ("probe", C.POINTER(C.c_int)),
                 ("nfreq", C.c_int), ("freqs", C.POINTER(C.c_double)),
                 ("design", C.c_int * 6), ("design_eps", C.POINTER(C.c_float)),
-                ("nsteps", C.c_int)]
+                ("nsteps", C.c_int),
+                ("nplate", C.c_int), ("plate", C.POINTER(C.c_int)), ("plate_sigma", C.POINTER(C.c_float)),
+                ("nres", C.c_int), ("res", C.POINTER(C.c_int)), ("res_R", C.POINTER(C.c_double))]
 
 
 def build():
@@ -127,6 +129,14 @@ class Oracle:
             os_.design = (C.c_int * 6)(0, 0, 0, 0, 0, 0)
             os_.design_eps = C.POINTER(C.c_float)()
         os_.nsteps = self.nsteps
+        plates = getattr(sc, "plates", []) or []
+        res = getattr(sc, "resistors", []) or []
+        os_.nplate = len(plates)
+        os_.plate = arr(np.array([[p.axis, *p.ijk] for p in plates], dtype=np.int32).ravel(), C.c_int, np.int32)
+        os_.plate_sigma = arr(np.array([p.sigma for p in plates], dtype=np.float32).ravel(), C.c_float, np.float32)
+        os_.nres = len(res)
+        os_.res = arr(np.array([r.ijk for r in res], dtype=np.int32).ravel(), C.c_int, np.int32)
+        os_.res_R = arr([r.R for r in res], C.c_double, np.float64)
         self._scene = os_
         self.lib = lib
         self.h = lib.oracle_create(C.byref(os_), int(precision))

@@ -1,0 +1,164 @@
+"""Conducting plates and lumped resistors (definePlateX/Y/Z, defineResistor)
+and the reference's own patch-antenna scene.
+
+CPU part (oracle, FP64), pinned by geometry and physics rather than by the
+reference's formulas:
+  * a plate changes exactly the E edges that lie in its rectangle (an Ex
+    edge spans [i, i+1] x {j} x {k}, and so on) -- for each plate axis;
+  * a plate of sigma 1e10 is a conductor: the tangential E on it stays
+    below 1e-6 of the field around it;
+  * a resistor edge with the port around it obeys I = -V/R minus the gap's
+    displacement current, step by step (its update as a current balance).
+GPU part: the engine bit for bit against the FP32 oracle on a scene with
+plates, a resistor and a port, and on the patch-antenna scene.
+"""
+import numpy as np
+import pytest
+
+from fdtd_amd import scenes
+from oracle import port as oport
+from oracle.oracle import Oracle
+
+EPS0 = scenes.EPS0
+
+
+def _scene(n=18, steps=60, plates=(), resistors=(), src=(9, 9, 5), L=4, probes=(), boxes=(),
+           d=(1e-3, 1.2e-3, 0.8e-3)):
+    dt = scenes.courant_dt(*d, 0.99)
+    return scenes.Scene(name="loads", nx=n, ny=n, nz=n, dx=d[0], dy=d[1], dz=d[2], dt=dt, steps=steps,
+                        cpml=scenes.graded_cpml(L, d[0]), sources=[scenes.z_edge(*src, dt)],
+                        plates=list(plates), resistors=list(resistors), probes=list(probes),
+                        boxes=list(boxes), seed=5)
+
+
+def _on_plate(comp, i, j, k, axis, b):
+    """Does the E edge (comp, i, j, k) lie in the plate's rectangle?"""
+    is_, js, ks, ie, je, ke = b
+    lo = [i, j, k]
+    hi = [i, j, k]
+    hi[comp] += 1                      # the edge spans one cell along its own axis
+    a = axis - 1                       # plate normal
+    if comp == a:
+        return False                   # normal edges are not tangential
+    if not (lo[a] == b[a] == hi[a]):
+        return False
+    rng = [(is_, ie), (js, je), (ks, ke)]
+    return all(rng[q][0] <= lo[q] and hi[q] <= rng[q][1] for q in range(3) if q != a)
+
+
+@pytest.mark.parametrize("axis,box", [(3, (5, 6, 8, 11, 10, 8)), (1, (7, 4, 5, 7, 12, 11)),
+                                      (2, (4, 9, 6, 12, 9, 10))])
+def test_plate_changes_exactly_the_edges_on_it(axis, box):
+    sc = _scene(plates=[scenes.Plate(axis=axis, ijk=box, sigma=(5.0, 7.0, 9.0))], steps=1)
+    o = Oracle(sc, precision=64)
+    vac = _scene(steps=1)
+    ov = Oracle(vac, precision=64)
+    n = sc.nx
+    for comp, name in enumerate(("Cexe", "Ceye", "Ceze")):
+        a = o.array(name).reshape(n, n, n)
+        v = ov.array(name).reshape(n, n, n)
+        changed = {(i, j, k) for k, j, i in zip(*np.nonzero(a != v))}
+        # the source edge changes Ceze on its own; leave it out
+        changed.discard((9, 9, 5))
+        expect = {(i, j, k) for k in range(1, n - 1) for j in range(1, n - 1) for i in range(1, n - 1)
+                  if _on_plate(comp, i, j, k, axis, box)}
+        assert changed == expect, (name, sorted(changed ^ expect)[:6])
+        sig = (5.0, 7.0, 9.0)[comp]
+        for (i, j, k) in list(expect)[:3]:   # the closed form with the plate's sigma (vacuum eps)
+            ce = (2 * EPS0 - sc.dt * sig) / (2 * EPS0 + sc.dt * sig)
+            assert abs(a[k, j, i] - ce) < 1e-12
+
+
+def test_pec_plate_holds_tangential_e_at_zero():
+    box = (4, 4, 10, 14, 14, 10)
+    probes = [(c, i, j, 10) for c in (0, 1) for i in (7, 9, 11) for j in (7, 9, 11)]
+    near = [(c, i, j, 8) for c in (0, 1) for i in (7, 9, 11) for j in (7, 9, 11)]
+    sc = _scene(plates=[scenes.Plate(axis=3, ijk=box, sigma=(1e10, 1e10, 1e10))], steps=120,
+                probes=probes + near)
+    o = Oracle(sc, precision=64)
+    o.steps(sc.steps)
+    h = o.probes()
+    on, off = np.abs(h[:len(probes)]).max(), np.abs(h[len(probes):]).max()
+    assert off > 0
+    assert on < 1e-6 * off
+
+
+def test_resistor_balance_with_the_port():
+    R = 75.0
+    res = scenes.Resistor(ijk=(11, 9, 5, 11, 9, 6), R=R)
+    sc = _scene(resistors=[res], steps=100)
+    cells = oport.cells(res.ijk, res.ijk, 3, sc.nz)
+    sc.probes = cells
+    o = Oracle(sc, precision=64)
+    o.steps(sc.steps)
+    hist = o.probes()
+    look = {c: hist[q] for q, c in enumerate(cells)}
+    get = lambda comp, i, j, k: look[(comp, i, j, k)]
+    v = oport.voltage(get, res.ijk, sc.dz, sc.nz, dtype=np.float64)
+    raw = oport.current_raw(get, res.ijk, 3, sc.dx, sc.dy, sc.dz, sc.nz, dtype=np.float64)
+    c_gap = EPS0 * sc.dx * sc.dy / sc.dz
+    pred = -c_gap / sc.dt * np.diff(v) - (v[1:] + v[:-1]) / (2 * R)
+    scale = np.max(np.abs(raw))
+    assert scale > 0
+    assert np.max(np.abs(raw[:-1] - pred)) < 1e-9 * scale
+
+
+def test_patch_antenna_scene_matches_source_cu():
+    sc = scenes.patch_antenna(steps=10)
+    # 60 x 100 x 3 cells of object, 15 air + 10 CPML on every side
+    assert (sc.nx, sc.ny, sc.nz) == (110, 150, 53)
+    assert [p.ijk for p in sc.plates] == [(25, 25, 25, 85, 125, 25), (44, 25, 28, 50, 75, 28),
+                                          (39, 75, 28, 71, 115, 28)]
+    assert sc.sources[0].ijk == (44, 25, 25, 50, 25, 28)
+    assert sc.port.vbox == (44, 35, 25, 50, 35, 28) and sc.port.ibox == (44, 35, 28, 50, 35, 28)
+    assert sc.boxes[0].ijk == (25, 25, 25, 85, 125, 28)
+
+
+# ---- GPU -------------------------------------------------------------------------
+
+def _engine_vs_oracle(sc):
+    from fdtd_amd import api
+    sim = api.Simulation.from_scene(sc, "f32")
+    sim.run(sc.steps)
+    v, i = sim.port()
+    pr = sim.probes()
+    sim.close()
+    p = sc.port
+    cells = oport.cells(p.vbox, p.ibox, p.idir, sc.nz)
+    sc2 = scenes.Scene(**{**sc.__dict__, "probes": list(sc.probes) + cells, "port": None})
+    o = Oracle(sc2, precision=32)
+    o.steps(sc.steps)
+    h = o.probes()
+    look = {c: h[len(sc.probes) + q] for q, c in enumerate(cells)}
+    get = lambda comp, i_, j, k: look[(comp, i_, j, k)]
+    vo = oport.voltage(get, p.vbox, sc.dz, sc.nz)
+    io = oport.correct(oport.current_raw(get, p.ibox, p.idir, sc.dx, sc.dy, sc.dz, sc.nz))
+    return v, i, pr[:len(sc.probes)], vo, io, h[:len(sc.probes)].astype(np.float32)
+
+
+@pytest.mark.gpu
+def test_plates_resistor_port_bitwise_vs_oracle():
+    sc = _scene(n=22, steps=150, src=(10, 11, 8),
+                plates=[scenes.Plate(axis=3, ijk=(5, 5, 12, 16, 16, 12), sigma=(1e10, 1e10, 1e10)),
+                        scenes.Plate(axis=1, ijk=(15, 6, 6, 15, 14, 11), sigma=(3.0, 2e3, 1e10)),
+                        scenes.Plate(axis=2, ijk=(6, 15, 5, 13, 15, 10), sigma=(50.0, 1.0, 0.5))],
+                resistors=[scenes.Resistor(ijk=(13, 11, 7, 13, 12, 9), R=100.0)],
+                boxes=[scenes.Box(ijk=(6, 6, 4, 15, 16, 8), eps_r=(3.0, 3.0, 3.0), sigma_e=(0.01, 0.01, 0.01))],
+                probes=[(2, 12, 12, 10), (0, 8, 9, 13), (1, 14, 10, 9)])
+    sc.port = scenes.Port(vbox=(10, 11, 8, 10, 11, 9), ibox=(10, 11, 8, 10, 11, 9), idir=3)
+    print("loads case: 22^3, 3 plates, 1 resistor, port", flush=True)
+    v, i, pr, vo, io, po = _engine_vs_oracle(sc)
+    assert np.max(np.abs(vo)) > 0
+    assert np.array_equal(pr, po)
+    assert np.array_equal(v, vo)
+    assert np.array_equal(i, io)
+
+
+@pytest.mark.gpu
+def test_patch_antenna_bitwise_vs_oracle():
+    sc = scenes.patch_antenna(steps=300)
+    print("patch antenna: 110x150x53, 300 steps", flush=True)
+    v, i, _, vo, io, _ = _engine_vs_oracle(sc)
+    assert np.max(np.abs(vo)) > 0 and np.max(np.abs(io)) > 0
+    assert np.array_equal(v, vo)
+    assert np.array_equal(i, io)

@@ -81,3 +81,45 @@ def test_port_pins_catch_planted_mistake(name, tmp_path):
                        capture_output=True, text=True, timeout=600)
     assert r.returncode == 1, r.stdout[-2000:] + r.stderr[-2000:]
     assert "failed" in r.stdout
+
+
+# ---- plates and resistors (oracle/fdtd_oracle.cpp, pinned by tests/test_loads.py) ----
+
+LOAD_MUTANTS = {
+    # z-plate Ex edges taken with i <= ie (one edge past the plate)
+    "plate_z_range": ("if (i >= is && i < ie && j >= js && j <= je && k == ks) se[0][x]",
+                      "if (i >= is && i <= ie && j >= js && j <= je && k == ks) se[0][x]"),
+    # x-plate Ez edges stop one row short
+    "plate_x_range": ("if (i == is && j >= js && j <= je && k >= ks && k < ke) se[2][x]",
+                      "if (i == is && j >= js && j < je && k >= ks && k < ke) se[2][x]"),
+    # y-plate Ex edges get the z sigma
+    "plate_y_sigma": ("if (i >= is && i < ie && j == js && k >= ks && k <= ke) se[0][x] = (R)sg[0];",
+                      "if (i >= is && i < ie && j == js && k >= ks && k <= ke) se[0][x] = (R)sg[2];"),
+    # resistor factor with dx in place of dz (transposed cell size)
+    "resistor_cell": ("const R rfactor = (dt * dz) / (r_comp * dx * dy);\n        for (int k = 0; k < NZ; ++k)",
+                      "const R rfactor = (dt * dx) / (r_comp * dx * dy);\n        for (int k = 0; k < NZ; ++k)"),
+    # resistor loss entering Ce with the wrong sign
+    "resistor_sign": ("s->Ceze[x]  = (two * eps[2][x] * eps_0 - dt * se[2][x] - rfactor) / D;\n"
+                      "                    s->Cezhy[x] = (two * dt / dx) / D;\n"
+                      "                    s->Cezhx[x] = ((R)-2 * dt / dy) / D;\n                }\n    }",
+                      "s->Ceze[x]  = (two * eps[2][x] * eps_0 - dt * se[2][x] + rfactor) / D;\n"
+                      "                    s->Cezhy[x] = (two * dt / dx) / D;\n"
+                      "                    s->Cezhx[x] = ((R)-2 * dt / dy) / D;\n                }\n    }"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(LOAD_MUTANTS))
+def test_load_pins_catch_planted_mistake(name, tmp_path):
+    old, new = LOAD_MUTANTS[name]
+    text = open(SRC).read()
+    assert text.count(old) == 1, f"mutation site for {name} not found"
+    src = tmp_path / "fdtd_oracle.cpp"
+    src.write_text(text.replace(old, new))
+    lib = tmp_path / "liboracle.so"
+    subprocess.run(["g++"] + FLAGS + ["-o", str(lib), str(src)], check=True)
+    env = dict(os.environ, ORACLE_LIB=str(lib))
+    r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider", "-m", "not gpu",
+                        os.path.join(ROOT, "tests", "test_loads.py")], cwd=ROOT, env=env,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 1, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "failed" in r.stdout
