@@ -53,7 +53,7 @@ struct HQ {
     float sy, sz, tz, tx, rx, ry;
 };
 
-template <int CM>
+template <int CM, bool IN>
 __device__ HQ hquant(const KArgs &a, const AArgs &b, bool huni, int i, int j, int k)
 {
     HQ q{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -61,8 +61,9 @@ __device__ HQ hquant(const KArgs &a, const AArgs &b, bool huni, int i, int j, in
     const i64 c = (i64)k * a.plane + (i64)j * px + i;
     float4 hx, hy, hz;
     coefs<CM>(a, a.htab, huni, c, hx, hy, hz);
-    const bool ux = j < ny - 1 && k < nz - 1, uy = i < nx - 1 && k < nz - 1, uz = i < nx - 1 && j < ny - 1;
-    const bool xs = xs_h(a, i), ys = ys_h(a, j), zs = zs_h(a, k);
+    const bool ux = IN || (j < ny - 1 && k < nz - 1), uy = IN || (i < nx - 1 && k < nz - 1);
+    const bool uz = IN || (i < nx - 1 && j < ny - 1);
+    const bool xs = !IN && xs_h(a, i), ys = !IN && ys_h(a, j), zs = !IN && zs_h(a, k);
     const float4 pxm = xs ? a.px_prof[nx + i] : NOPROF;
     const float4 pym = ys ? a.py_prof[ny + j] : NOPROF;
     const float4 pzm = zs ? a.pz_prof[nz + k] : NOPROF;
@@ -110,17 +111,17 @@ struct EQ {
     float ax, bx, ay, by, az, bz;
 };
 
-template <int CM>
+template <int CM, bool IN>
 __device__ EQ equant(const KArgs &a, const AArgs &b, bool euni, int i, int j, int k)
 {
     EQ q{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
-    if (i >= nx - 1 || j >= ny - 1 || k >= nz - 1) return q;   // no E update, no psi_e
+    if (!IN && (i >= nx - 1 || j >= ny - 1 || k >= nz - 1)) return q;   // no E update, no psi_e
     const i64 c = (i64)k * a.plane + (i64)j * px + i;
     float4 ex, ey, ez;
     coefs<CM>(a, a.etab, euni, c, ex, ey, ez);
-    const bool ux = j > 0 && k > 0, uy = i > 0 && k > 0, uz = i > 0 && j > 0;
-    const bool xs = xs_e(a, i), ys = ys_e(a, j), zs = zs_e(a, k);
+    const bool ux = IN || (j > 0 && k > 0), uy = IN || (i > 0 && k > 0), uz = IN || (i > 0 && j > 0);
+    const bool xs = !IN && xs_e(a, i), ys = !IN && ys_e(a, j), zs = !IN && zs_e(a, k);
     const float4 pxe = xs ? a.px_prof[i] : NOPROF;
     const float4 pye = ys ? a.py_prof[j] : NOPROF;
     const float4 pze = zs ? a.pz_prof[k] : NOPROF;
@@ -178,14 +179,15 @@ inline bool adj_shares(int nx) { return nx >= 8 * TX; }
 
 // fstep >= 0: finalize the E half-step of forward step fstep first.
 // sten: gather the transposed H half-step of step m and inject dJ/ds[m].
-template <int CM, bool SHARE>
+template <int CM, bool SHARE, int V>
 __global__ void __launch_bounds__(TX * BY)
     adj_e_kernel(const KArgs a, const AArgs b, const int m, const int fstep, const int sten, const int euni,
                  const int huni)
 {
+    constexpr bool IN = V == V_IN;   // interior tiles: no slab, face or range predicates
     __shared__ HQ sq[BY][TX];
     int tx, ty, tz;
-    tile_of(a, tx, ty, tz);
+    tile_of_region<V>(a, tx, ty, tz);
     const int lane = (int)(threadIdx.x & (TX - 1));
     const int row = (int)(threadIdx.x / TX);
     // SHARE: lane 0 is the halo column before the tile; otherwise every lane
@@ -193,27 +195,27 @@ __global__ void __launch_bounds__(TX * BY)
     const int i = SHARE ? tx * TXA + lane - 1 : tx * TX + lane;
     const int j = ty * BY + row;
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
-    const bool valid = i >= 0 && i < nx && j < ny;   // a cell whose quantities a neighbour reads
-    const bool own = valid && (!SHARE || lane > 0);  // a cell this thread updates
+    const bool valid = IN || (i >= 0 && i < nx && j < ny);   // a cell whose quantities a neighbour reads
+    const bool own = valid && (!SHARE || lane > 0);          // a cell this thread updates
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
-    const bool ilo = i > 0, ihi = i < nx - 1, jlo = j > 0, jhi = j < ny - 1;
+    const bool ilo = IN || i > 0, ihi = IN || i < nx - 1, jlo = IN || j > 0, jhi = IN || j < ny - 1;
     const int mycol = j * nx + i;
     int ev = own ? first_event(a, a.e_first, mycol, k0) : -1;
     int ek = ev >= 0 ? a.events[ev].k : INT_MAX;
     const bool in_box = i >= b.sb[0] && i < b.sb[0] + b.sb[3] && j >= b.sb[1] && j < b.sb[1] + b.sb[4];
     const i64 nb = (i64)b.sb[3] * b.sb[4] * b.sb[5];
-    const bool xs = own && xs_e(a, i), ys = own && ys_e(a, j);
+    const bool xs = !IN && own && xs_e(a, i), ys = !IN && own && ys_e(a, j);
     const float4 pxe = xs ? a.px_prof[i] : NOPROF;
     const float4 pye = ys ? a.py_prof[j] : NOPROF;
     const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
     const HQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 
     HQ qkm = zq;
-    if (own && sten && k0 > 0) qkm = hquant<CM>(a, b, huni, i, j, k0 - 1);
+    if (own && sten && k0 > 0) qkm = hquant<CM, IN>(a, b, huni, i, j, k0 - 1);
     for (int k = k0; k < k1; ++k) {
         const i64 c = (i64)k * a.plane + (i64)j * px + i;
-        const bool klo = k > 0, khi = k < nz - 1;
+        const bool klo = IN || k > 0, khi = IN || k < nz - 1;
         const bool ux = own && ihi && jlo && jhi && klo && khi;
         const bool uy = own && ilo && ihi && jhi && klo && khi;
         const bool uz = own && ilo && ihi && jlo && jhi && khi;
@@ -241,7 +243,7 @@ __global__ void __launch_bounds__(TX * BY)
         if (fstep >= 0 && (ux || uy || uz)) {
             float4 ex, ey, ez;
             coefs<CM>(a, a.etab, euni, c, ex, ey, ez);
-            const bool zs = zs_e(a, k);
+            const bool zs = !IN && zs_e(a, k);
             if (xs && jhi && khi) {
                 const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
                 b.pb[0][s] = pxe.x * (b.pb[0][s] + (uy ? (ey.z * dx) * eby : 0.f));
@@ -277,16 +279,16 @@ __global__ void __launch_bounds__(TX * BY)
             if (uz) ebz = (is_src ? cez_src : ez.x) * ebz;
         }
         if (sten) {
-            const HQ q0 = (SHARE ? valid : own) ? hquant<CM>(a, b, huni, i, j, k) : zq;
+            const HQ q0 = (SHARE ? valid : own) ? hquant<CM, IN>(a, b, huni, i, j, k) : zq;
             if (SHARE) {
                 sq[row][lane] = q0;
                 __syncthreads();
             }
             if (own) {
                 // (i-1, j, k) and (i, j-1, k)
-                const HQ qi = SHARE ? sq[row][lane - 1] : (ilo && (uy || uz) ? hquant<CM>(a, b, huni, i - 1, j, k) : zq);
+                const HQ qi = SHARE ? sq[row][lane - 1] : (ilo && (uy || uz) ? hquant<CM, IN>(a, b, huni, i - 1, j, k) : zq);
                 const HQ qj = (SHARE && row > 0) ? sq[row - 1][lane]
-                                                 : (jlo && (ux || uz) ? hquant<CM>(a, b, huni, i, j - 1, k) : zq);
+                                                 : (jlo && (ux || uz) ? hquant<CM, IN>(a, b, huni, i, j - 1, k) : zq);
                 if (ux) {
                     ebx += (qkm.tx - q0.tx) + (qj.rx - q0.rx);
                     if (uz) ebz += qj.sz - q0.sz;
@@ -314,35 +316,36 @@ __global__ void __launch_bounds__(TX * BY)
     }
 }
 
-template <int CM, bool SHARE>
+template <int CM, bool SHARE, int V>
 __global__ void __launch_bounds__(TX * BY)
     adj_h_kernel(const KArgs a, const AArgs b, const int euni, const int huni)
 {
+    constexpr bool IN = V == V_IN;   // interior tiles: no slab, face or range predicates
     __shared__ EQ sq[BY][TX];
     int tx, ty, tz;
-    tile_of(a, tx, ty, tz);
+    tile_of_region<V>(a, tx, ty, tz);
     const int lane = (int)(threadIdx.x & (TX - 1));
     const int row = (int)(threadIdx.x / TX);
     // SHARE: lane 63 is the halo column after the tile
     const int i = SHARE ? tx * TXA + lane : tx * TX + lane;
     const int j = ty * BY + row;
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
-    const bool valid = i < nx && j < ny;
+    const bool valid = IN || (i < nx && j < ny);
     const bool own = valid && (!SHARE || lane < TX - 1);
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
-    const bool ihi = i < nx - 1, jhi = j < ny - 1;
-    const bool xs = own && xs_h(a, i), ys = own && ys_h(a, j);
+    const bool ihi = IN || i < nx - 1, jhi = IN || j < ny - 1;
+    const bool xs = !IN && own && xs_h(a, i), ys = !IN && own && ys_h(a, j);
     const float4 pxm = xs ? a.px_prof[nx + i] : NOPROF;
     const float4 pym = ys ? a.py_prof[ny + j] : NOPROF;
     const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
     const EQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    EQ q0 = (SHARE ? valid : own) ? equant<CM>(a, b, euni, i, j, k0) : zq;
+    EQ q0 = (SHARE ? valid : own) ? equant<CM, IN>(a, b, euni, i, j, k0) : zq;
     for (int k = k0; k < k1; ++k) {
         const i64 c = (i64)k * a.plane + (i64)j * px + i;
-        const bool khi = k < nz - 1;
+        const bool khi = IN || k < nz - 1;
         const bool ux = own && jhi && khi, uy = own && ihi && khi, uz = own && ihi && jhi;
-        const EQ qk = ((SHARE ? valid : own) && khi) ? equant<CM>(a, b, euni, i, j, k + 1) : zq;
+        const EQ qk = ((SHARE ? valid : own) && khi) ? equant<CM, IN>(a, b, euni, i, j, k + 1) : zq;
         float hbx = 0.f, hby = 0.f, hbz = 0.f;
         if (own) {
             float4 hx, hy, hz;
@@ -361,7 +364,7 @@ __global__ void __launch_bounds__(TX * BY)
                 b.pb[6][s] = pym.x * (b.pb[6][s] + (ux ? (hx.z * dy) * hbx : 0.f));
                 b.pb[7][s] = pym.x * (b.pb[7][s] + (uz ? (hz.y * dy) * hbz : 0.f));
             }
-            if (zs_h(a, k)) {
+            if (!IN && zs_h(a, k)) {
                 const float4 pzm = a.pz_prof[nz + k];
                 const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
                 b.pb[10][s] = pzm.x * (b.pb[10][s] + (ux ? (hx.y * dz) * hbx : 0.f));
@@ -378,9 +381,9 @@ __global__ void __launch_bounds__(TX * BY)
         }
         if (own) {
             // (i+1, j, k) and (i, j+1, k)
-            const EQ qi = SHARE ? sq[row][lane + 1] : (ihi && (uy || uz) ? equant<CM>(a, b, euni, i + 1, j, k) : zq);
+            const EQ qi = SHARE ? sq[row][lane + 1] : (ihi && (uy || uz) ? equant<CM, IN>(a, b, euni, i + 1, j, k) : zq);
             const EQ qj = (SHARE && row < BY - 1) ? sq[row + 1][lane]
-                                                  : (jhi && (ux || uz) ? equant<CM>(a, b, euni, i, j + 1, k) : zq);
+                                                  : (jhi && (ux || uz) ? equant<CM, IN>(a, b, euni, i, j + 1, k) : zq);
             if (ux) {
                 hbx += (q0.ay - qk.ay) + (q0.bz - qj.bz);
                 if (uz) hbz += q0.ax - qj.ax;
@@ -440,6 +443,35 @@ __global__ void snap_kernel(const KArgs a, float *hist, const int slot)
     o[2 * nb] = ld((const T *)a.ez + c) * a.probe_unscale[2];
 }
 
+// Interior tile box of the adjoint tiling (tiles tcols wide, rows BY, z-blocks
+// of a.kchunk planes): the own cells of a tile and the neighbour quantities it
+// evaluates (one cell beyond it on each side: the E gather reads H at i-1,
+// j-1, k-1, the H gather E at i+1, j+1, k+1) all lie off the CPML slabs and
+// the outer faces.  All zero when an axis has no such tile.
+void adj_box(const KArgs &a, int tcols, int ib[6])
+{
+    const int n[3] = {a.nx, a.ny, a.nz}, w[3] = {tcols, BY, a.kchunk};
+    const int nt[3] = {a.tiles_x, a.tiles_y, (a.nz + a.kchunk - 1) / a.kchunk};
+    for (int q = 0; q < 6; ++q) ib[q] = 0;
+    int box[6];
+    for (int ax = 0; ax < 3; ++ax) {
+        const int lo_c = (a.L[2 * ax] > 1 ? a.L[2 * ax] : 1) + 1;
+        const int hi_c = n[ax] - (a.L[2 * ax + 1] > 1 ? a.L[2 * ax + 1] : 1) - 1;
+        int lo = nt[ax], hi = 0;
+        for (int t = 0; t < nt[ax]; ++t)
+            if (t * w[ax] >= lo_c && (t + 1) * w[ax] <= hi_c) {
+                lo = lo < t ? lo : t;
+                hi = t + 1;
+            }
+        if (hi <= lo) return;
+        box[ax] = lo;
+        box[ax + 3] = hi;
+    }
+    for (int q = 0; q < 6; ++q) ib[q] = box[q];
+}
+
+// which: 0 adjoint E sweep, 1 adjoint H sweep, 2 source fix-up.  a0.ib[0] <
+// 0 asks for the general variant alone (no interior split).
 hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, const AArgs &b, int m, int fstep,
                       int sten, hipStream_t s)
 {
@@ -448,23 +480,39 @@ hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, 
     const bool share = adj_shares(a0.nx);
     KArgs a = a0;
     a.tiles_x = share ? (a.nx + TXA - 1) / TXA : (a.nx + TX - 1) / TX;
-    const int nt = a.tiles_x * a.tiles_y * ((a.nz + a.kchunk - 1) / a.kchunk);
-    dim3 grid(nt), block(TX * BY);
-#define ADJ_E(CMV, SH, EU, HU) hipLaunchKernelGGL((adj_e_kernel<CMV, SH>), grid, block, 0, s, a, b, m, fstep, sten, EU, HU)
-#define ADJ_H(CMV, SH, EU, HU) hipLaunchKernelGGL((adj_h_kernel<CMV, SH>), grid, block, 0, s, a, b, EU, HU)
-    if (which == 0) {
-        if (mode == CLS_U32) { if (share) ADJ_E(CLS_U32, true, euni, huni); else ADJ_E(CLS_U32, false, euni, huni); }
-        else if (mode == CLS_U16) { if (share) ADJ_E(CLS_U16, true, euni, huni); else ADJ_E(CLS_U16, false, euni, huni); }
-        else { if (share) ADJ_E(CLS_UNIFORM, true, 1, 1); else ADJ_E(CLS_UNIFORM, false, 1, 1); }
-    } else if (which == 1) {
-        if (mode == CLS_U32) { if (share) ADJ_H(CLS_U32, true, euni, huni); else ADJ_H(CLS_U32, false, euni, huni); }
-        else if (mode == CLS_U16) { if (share) ADJ_H(CLS_U16, true, euni, huni); else ADJ_H(CLS_U16, false, euni, huni); }
-        else { if (share) ADJ_H(CLS_UNIFORM, true, 1, 1); else ADJ_H(CLS_UNIFORM, false, 1, 1); }
-    } else {
+    if (which == 2) {
         if (mode == CLS_U32) hipLaunchKernelGGL((adj_src_fixup<CLS_U32>), dim3(1), dim3(64), 0, s, a, b, euni);
         else if (mode == CLS_U16) hipLaunchKernelGGL((adj_src_fixup<CLS_U16>), dim3(1), dim3(64), 0, s, a, b, euni);
         else hipLaunchKernelGGL((adj_src_fixup<CLS_UNIFORM>), dim3(1), dim3(64), 0, s, a, b, 1);
+        return hipGetLastError();
     }
+    if (a0.ib[0] < 0) {
+        for (int q = 0; q < 6; ++q) a.ib[q] = 0;
+    } else {
+        adj_box(a, share ? TXA : TX, a.ib);
+    }
+    int n_in, n_x, n_gen;
+    region_tiles(a, n_in, n_x, n_gen);
+#define ADJ_E(CMV, SH, V, NT, EU, HU) \
+    hipLaunchKernelGGL((adj_e_kernel<CMV, SH, V>), dim3(NT), dim3(TX * BY), 0, s, a, b, m, fstep, sten, EU, HU)
+#define ADJ_H(CMV, SH, V, NT, EU, HU) \
+    hipLaunchKernelGGL((adj_h_kernel<CMV, SH, V>), dim3(NT), dim3(TX * BY), 0, s, a, b, EU, HU)
+#define ADJ_V(MAC, V, NT)                                                                  \
+    do {                                                                                   \
+        if (mode == CLS_U32) { if (share) MAC(CLS_U32, true, V, NT, euni, huni); else MAC(CLS_U32, false, V, NT, euni, huni); } \
+        else if (mode == CLS_U16) { if (share) MAC(CLS_U16, true, V, NT, euni, huni); else MAC(CLS_U16, false, V, NT, euni, huni); } \
+        else { if (share) MAC(CLS_UNIFORM, true, V, NT, 1, 1); else MAC(CLS_UNIFORM, false, V, NT, 1, 1); } \
+    } while (0)
+    if (which == 0) {
+        if (n_in) ADJ_V(ADJ_E, V_IN, n_in);
+        if (n_x) ADJ_V(ADJ_E, V_X, n_x);
+        if (n_gen) ADJ_V(ADJ_E, V_GEN, n_gen);
+    } else {
+        if (n_in) ADJ_V(ADJ_H, V_IN, n_in);
+        if (n_x) ADJ_V(ADJ_H, V_X, n_x);
+        if (n_gen) ADJ_V(ADJ_H, V_GEN, n_gen);
+    }
+#undef ADJ_V
 #undef ADJ_E
 #undef ADJ_H
     return hipGetLastError();

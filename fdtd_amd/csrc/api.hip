@@ -674,6 +674,11 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     aa.etab = c->etab_si;
     aa.htab = c->htab_si;
     aa.srcs = c->srcs_si;
+    // interior / general split of the adjoint sweeps (FDTD_INTERIOR=0: general only)
+    {
+        const char *ev = std::getenv("FDTD_INTERIOR");
+        aa.ib[0] = (ev && std::strcmp(ev, "0") == 0) ? -1 : 0;
+    }
     const int mode = c->built.mode;
     const int euni = c->built.e_uniform ? 1 : 0, huni = c->built.h_uniform ? 1 : 0;
 
