@@ -310,7 +310,7 @@ class Simulation:
         for b in sc.boxes:
             sim.add_box(b.ijk, b.eps_r, b.sigma_e, b.mu_r, b.sigma_m)
         for s in sc.sources:
-            sim.add_lumped_source(s.ijk, s.R, s.wtype, s.amp, s.t0, s.tau, s.freq)
+            sim.add_lumped_source(s.ijk, s.R, s.wtype, s.amp, s.t0, s.tau, s.freq, direction=s.dir)
         for p in getattr(sc, "plates", []):
             sim.add_plate(p.axis, p.ijk, p.sigma)
         for r in getattr(sc, "resistors", []):

@@ -918,17 +918,27 @@ int fdtd_add_lumped_source(fdtd_ctx *c, int dir, int is, int js, int ks, int ie,
 {
     if (!c || !w) return FDTD_E_INVALID;
     if (c->finalized) return fail(c, FDTD_E_STATE, "scene is finalized");
-    if (dir != 3) return fail(c, FDTD_E_UNSUPPORTED, "only z-directed lumped sources (dir = 3) are built");
+    if (dir != 3 && dir != 1) return fail(c, FDTD_E_UNSUPPORTED, "lumped sources are x- (dir = 1) or z-directed (dir = 3)");
     if (!(R > 0) || !std::isfinite(R)) return fail(c, FDTD_E_INVALID, "source resistance must be > 0");
     if (w->type < FDTD_WAVE_GAUSSIAN || w->type > FDTD_WAVE_SINE)
         return fail(c, FDTD_E_INVALID, "unknown waveform type %d", w->type);
     if ((w->type != FDTD_WAVE_SINE && !(w->tau > 0)) || !std::isfinite(w->amp))
         return fail(c, FDTD_E_INVALID, "invalid waveform parameters");
-    if (is > ie || js > je || ks >= ke) return fail(c, FDTD_E_INVALID, "empty source box");
-    // every edge must be an updated Ez edge: 0<i<nx-1, 0<j<ny-1, k<nz-1
-    if (is < 1 || ie > c->in.nx - 2 || js < 1 || je > c->in.ny - 2 || ks < 0 || ke > c->in.nz - 1)
-        return fail(c, FDTD_E_RANGE, "source box outside the updated Ez region");
+    if (dir == 1) {
+        // defineSourceX's loaded edges: is<=i<ie, js<=j<=ie, ks<=k<=ke, all updated Ex
+        // edges (i<nx-1, 0<j<ny-1, 0<k<nz-1); calc_e's Ez injection still runs on the
+        // box's Ez edges is<=i<=ie, js<=j<=je, ks<=k<ke (none when ks == ke)
+        if (is >= ie || js > ie || js > je || ks > ke) return fail(c, FDTD_E_INVALID, "empty x-source box (needs ie > is)");
+        if (is < 1 || ie > c->in.nx - 2 || js < 1 || std::max(ie, je) > c->in.ny - 2 || ks < 1 || ke > c->in.nz - 2)
+            return fail(c, FDTD_E_RANGE, "x-source box outside the updated Ex / Ez region");
+    } else {
+        if (is > ie || js > je || ks >= ke) return fail(c, FDTD_E_INVALID, "empty source box");
+        // every edge must be an updated Ez edge: 0<i<nx-1, 0<j<ny-1, k<nz-1
+        if (is < 1 || ie > c->in.nx - 2 || js < 1 || je > c->in.ny - 2 || ks < 0 || ke > c->in.nz - 1)
+            return fail(c, FDTD_E_RANGE, "source box outside the updated Ez region");
+    }
     Source s;
+    s.dir = dir;
     s.b[0] = is; s.b[1] = js; s.b[2] = ks; s.b[3] = ie; s.b[4] = je; s.b[5] = ke;
     s.R = R;
     s.wtype = w->type;

@@ -138,6 +138,7 @@ struct Source {
     double R;
     int wtype;
     double amp, t0, tau, freq;
+    int dir = 3;   // 3: z-directed (loaded, driven Ez edges); 1: x-directed (loaded Ex edges)
 };
 struct Probe {
     int comp, i, j, k;

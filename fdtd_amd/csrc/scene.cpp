@@ -14,6 +14,8 @@
 //   source loading      defineSourceZ        fdtdrav.h:1391-1421
 //   conducting plates   definePlateX/Y/Z     fdtdrav.h:1574-1678
 //   lumped resistors    defineResistor       fdtdrav.h:1429-1484
+//   x-directed source   defineSourceX        fdtdrav.h:1311-1366 (a load only:
+//                       calc_e's Ex injection is commented out, fdtdrav.h:681)
 //   per-node waveform   setGauss / setSine   fdtdrav.h:1486-1520
 #include <hip/hip_runtime.h>
 
@@ -86,7 +88,7 @@ inline void hcoef(const Consts &c, float mu, float sig, float da, float db, floa
 }
 
 struct KeyHash {
-    size_t operator()(const std::array<std::uint32_t, 12> &k) const
+    size_t operator()(const std::array<std::uint32_t, 13> &k) const
     {
         std::uint64_t h = 1469598103934665603ull;
         for (std::uint32_t v : k) { h ^= v; h *= 1099511628211ull; h ^= h >> 29; }
@@ -212,6 +214,53 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
                 if (near(kv.first)) { err = "a resistor touches the design region"; return -1; }
         }
     }
+    // ---- x-directed sources: Ex edges with defineSourceX's loaded
+    // coefficients (fdtdrav.h:1311-1366; its j range ends at ie), after the
+    // CPML like the z-sources -- kept off the CPML slabs for the same reason
+    // as resistors
+    std::unordered_map<i64, int> xs_of;
+    {
+        const int *L = in.cpml.thick;
+        for (size_t q = 0; q < in.sources.size(); ++q) {
+            if (in.sources[q].dir != 1) continue;
+            const int *b = in.sources[q].b;
+            if (b[0] < L[0] || b[3] > nx - L[1] - 1 || b[1] < L[2] || std::max(b[3], b[4]) > ny - L[3] - 1 ||
+                b[2] < L[4] || b[5] > nz - L[5] - 1) {
+                err = "x-directed source inside a CPML slab";
+                return -1;
+            }
+            for (int k = b[2]; k <= b[5]; ++k)
+                for (int j = b[1]; j <= b[3]; ++j)
+                    for (int i = b[0]; i < b[3]; ++i) {
+                        const i64 cell = i + nx * (j + (i64)ny * k);
+                        if (xs_of.count(cell)) { err = "x-directed sources overlap"; return -1; }
+                        xs_of[cell] = (int)q;
+                    }
+        }
+        if (!in.design_eps.empty()) {
+            const int *d = in.design;
+            for (const auto &kv : xs_of) {
+                const i64 cell = kv.first;
+                const int i = (int)(cell % nx), j = (int)((cell / nx) % ny), k = (int)(cell / nxy);
+                if (i >= d[0] - 1 && i <= d[3] && j >= d[1] - 1 && j <= d[4] && k >= d[2] - 1 && k <= d[5]) {
+                    err = "an x-directed source touches the design region";
+                    return -1;
+                }
+            }
+        }
+    }
+    auto xcoef = [&](const Source &xs, float e, float sg, float *o) {
+        const float _is = (float)xs.b[0], _js = (float)xs.b[1], _ks = (float)xs.b[2];
+        const float _ie = (float)xs.b[3], _je = (float)xs.b[4], _ke = (float)xs.b[5];
+        float r_comp = ((1.0f + _je) - _js) * ((1.0f + _ke) - _ks) / (_ie - _is);
+        r_comp = r_comp * (float)xs.R;
+        const float rfactor = (C.dt * C.dx) / (r_comp * C.dz * C.dy);
+        const float two = 2.0f;
+        const float D = two * e * C.eps0 + C.dt * sg + rfactor;
+        o[0] = (two * e * C.eps0 - C.dt * sg - rfactor) / D;
+        o[1] = (two * C.dt / C.dy) / D;
+        o[2] = (-2.0f * C.dt / C.dz) / D;
+    };
     // resistor-loaded Ez coefficients (defineResistor's operand order)
     auto rcoef = [&](const Resistor &rs, float e, float sg, float *o) {
         const float _is = (float)rs.b[0], _js = (float)rs.b[1], _ks = (float)rs.b[2];
@@ -232,11 +281,11 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
     // its plate sigma overrides and resistor.  Components not updated at a
     // cell get the vacuum values (never read).
     const std::uint32_t NONE = 0xFFFFFFFFu;
-    std::unordered_map<std::array<std::uint32_t, 12>, int, KeyHash> key2cls;
+    std::unordered_map<std::array<std::uint32_t, 13>, int, KeyHash> key2cls;
     std::unordered_map<std::array<std::uint32_t, 18>, int, TupHash> tup2cls;
     std::vector<std::array<float, 18>> tuples;
     std::vector<std::uint32_t> cellcls((size_t)N);
-    std::array<std::uint32_t, 12> prev;
+    std::array<std::uint32_t, 13> prev;
     prev.fill(NONE - 1);
     int prev_cls = -1;
     float vac[18];
@@ -252,7 +301,7 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 const size_t x = (size_t)(i + nx * (j + (i64)ny * k));
-                std::array<std::uint32_t, 12> key;
+                std::array<std::uint32_t, 13> key;
                 key[0] = lab[x];
                 key[1] = i > 0 ? lab[x - 1] : NONE;
                 key[2] = j > 0 ? lab[x - nx] : NONE;
@@ -274,6 +323,11 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
                     auto it = res_of.find((i64)x);
                     if (it != res_of.end()) key[11] = (std::uint32_t)it->second + 1u;
                 }
+                key[12] = 0;
+                if (!xs_of.empty()) {
+                    auto it = xs_of.find((i64)x);
+                    if (it != xs_of.end()) key[12] = (std::uint32_t)it->second + 1u;
+                }
                 if (key == prev) { cellcls[x] = (std::uint32_t)prev_cls; continue; }
                 auto it = key2cls.find(key);
                 int cid;
@@ -288,7 +342,8 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
                     if (i < nx - 1 && j > 0 && j < ny - 1 && k > 0 && k < nz - 1) {
                         float e = avg4(m0.v[0], M(key[2]).v[0], M(key[3]).v[0], M(key[5]).v[0]);
                         float s = edge_sigma(0, x, avg4(m0.v[3], M(key[2]).v[3], M(key[3]).v[3], M(key[5]).v[3]));
-                        ecoef(C, e, s, C.dy, C.dz, t + 0);
+                        if (key[12]) xcoef(in.sources[key[12] - 1], e, s, t + 0);
+                        else ecoef(C, e, s, C.dy, C.dz, t + 0);
                     }
                     // Ey: 0<i<nx-1, j<ny-1, 0<k<nz-1; labels (i,j,k),(i-1),(k-1),(i-1,k-1)
                     if (i > 0 && i < nx - 1 && j < ny - 1 && k > 0 && k < nz - 1) {
@@ -456,8 +511,12 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
         }
     }
 
-    // ---- sources: one record per edge, coefficients from the averaged
-    // eps_z / sigma_z of that edge (defineSourceZ, fdtdrav.h:1391-1421)
+    // ---- sources: one record per driven Ez edge, coefficients from the
+    // averaged eps_z / sigma_z of that edge (defineSourceZ,
+    // fdtdrav.h:1391-1421).  An x-directed source leaves the Ez coefficients
+    // as setCoefficients wrote them and keeps setZero's Cezj = 1.2e-38
+    // (fdtdrav.h:1016): calc_e injects on the box's Ez edges whatever the
+    // source direction (fdtdrav.h:726-729).
     out.events.clear();
     out.srcs.clear();
     out.srcs_si.clear();
@@ -486,12 +545,21 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
                     float sg = edge_sigma(2, x, avg4(mats[lab[x]].v[5], mats[lab[x - 1]].v[5], mats[lab[x - nx]].v[5],
                                                      mats[lab[x - 1 - nx]].v[5]));
                     const float two = 2.0f;
-                    float D = two * e * C.eps0 + C.dt * sg + rfactor;
                     SrcEdge se{};
-                    se.ceze = (two * e * C.eps0 - C.dt * sg - rfactor) / D;
-                    se.cezhy = (two * C.dt / C.dx) / D;
-                    se.cezhx = (-2.0f * C.dt / C.dy) / D;
-                    se.cezj = (-2.0f * C.dt / raux) / D;
+                    if (s.dir == 1) {
+                        float o[3];
+                        ecoef(C, e, sg, C.dx, C.dy, o);
+                        se.ceze = o[0];
+                        se.cezhy = o[1];
+                        se.cezhx = o[2];
+                        se.cezj = 1.2e-38f;
+                    } else {
+                        float D = two * e * C.eps0 + C.dt * sg + rfactor;
+                        se.ceze = (two * e * C.eps0 - C.dt * sg - rfactor) / D;
+                        se.cezhy = (two * C.dt / C.dx) / D;
+                        se.cezhx = (-2.0f * C.dt / C.dy) / D;
+                        se.cezj = (-2.0f * C.dt / raux) / D;
+                    }
                     out.srcs_si.push_back(se);
                     out.src_cell.push_back(i);
                     out.src_cell.push_back(j);
@@ -506,6 +574,7 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
                     out.srcs.push_back(se);
                 }
         const int vmag = b[5] - b[2];
+        if (vmag == 0) continue;   // an x-directed source one plane thick drives no Ez edge
         for (int t = 0; t < in.steps_cap; ++t) {
             double v = waveform(s.wtype, s.amp, s.t0, s.tau, s.freq, (double)t * in.dt);
             out.vpn[q * (size_t)in.steps_cap + t] = (float)v / (float)vmag;

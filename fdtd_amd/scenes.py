@@ -94,6 +94,7 @@ class Source:
     t0: float
     tau: float
     freq: float = 0.0
+    dir: int = 3      # 3: z-directed (defineSourceZ); 1: x-directed (defineSourceX: loaded Ex edges)
 
 
 @dataclass

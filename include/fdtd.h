@@ -164,10 +164,17 @@ int fdtd_add_box(fdtd_ctx *ctx, int is, int js, int ks, int ie, int je, int ke,
                  const float eps_r[3], const float sigma_e[3],
                  const float mu_r[3], const float sigma_m[3]);
 
-/* Resistive lumped voltage source on the edges is<=i<=ie, js<=j<=je,
- * ks<=k<ke.  Replaces defineSourceZ (fdtdrav.h:1368-1426) with the
- * injection of calc_e (fdtdrav.h:726-729).  dir must be 3 (z).  Any number
- * of non-overlapping sources is allowed; the reference had one. */
+/* Resistive lumped voltage source.  dir 3: the Ez edges is<=i<=ie,
+ * js<=j<=je, ks<=k<ke; replaces defineSourceZ (fdtdrav.h:1368-1426) with the
+ * injection of calc_e (fdtdrav.h:726-729).  Any number of non-overlapping
+ * z-sources is allowed; the reference had one.
+ * dir 1 reproduces defineSourceX (fdtdrav.h:1311-1366) as marchingLoop runs
+ * it: the Ex edges is<=i<ie, js<=j<=ie (the reference bounds j by ie),
+ * ks<=k<=ke get the source-loaded coefficients; calc_e's Ex injection is
+ * commented out (fdtdrav.h:681), and its Ez injection still adds
+ * 1.2e-38 * V(t)/(ke-ks) (setZero's Cezj, fdtdrav.h:1016) on the box's Ez
+ * edges is<=i<=ie, js<=j<=je, ks<=k<ke.  Needs ie > is; refused at
+ * fdtd_finalize inside a CPML slab. */
 int fdtd_add_lumped_source(fdtd_ctx *ctx, int dir, int is, int js, int ks,
                            int ie, int je, int ke, double resistance,
                            const fdtd_waveform *wave);

@@ -15,6 +15,7 @@
  *   update coefficients          H:1223-1308 (setCoefficients)
  *   CPML profiles, cpsi, kappa   H:1747-2160 (setABCPML)
  *   lumped z-source loading      H:1368-1426 (defineSourceZ)
+ *   lumped x-source loading      H:1311-1366 (defineSourceX)
  *   waveforms                    H:1486-1520 (setGauss, setSine)
  *   E update + source + sample   H:577-743 (calc_e), CPML H:750-911
  *   H update + sample            H:249-395 (calc_h), CPML H:398-562
@@ -74,6 +75,7 @@ typedef struct {
     int nres;
     const int *res;             /* 6 per resistor: is js ks ie je ke (z-directed) */
     const double *res_R;
+    const int *src_dir;         /* per source: 3 (z, defineSourceZ) or 1 (x, defineSourceX); NULL: all 3 */
 } oracle_scene;
 
 }  // extern "C"
@@ -469,6 +471,39 @@ static Sim<R> *build(const oracle_scene *sc)
     for (int q = 0; q < sc->nsrc; ++q) {
         const int *b = sc->src + 6 * q;
         const R _is = (R)b[0], _js = (R)b[1], _ks = (R)b[2], _ie = (R)b[3], _je = (R)b[4], _ke = (R)b[5];
+        if (sc->src_dir && sc->src_dir[q] == 1) {
+            // defineSourceX H:1346-1362 over the whole grid, its j bound being ie
+            // (H:1352); calc_e's Ex injection is commented out (H:681), but its Ez
+            // injection (H:726-729) still runs on the box's Ez edges with the
+            // Cezj that setZero left, 1.2e-38 (H:1016), and v_mag = ke - ks
+            R r_comp = (((R)1 + _je) - _js) * (((R)1 + _ke) - _ks) / (_ie - _is);
+            r_comp = r_comp * (R)sc->src_R[q];
+            const R rfactor = (dt * dx) / (r_comp * dz * dy);
+            for (int k = 0; k < NZ; ++k)
+                for (int j = 0; j < NY; ++j)
+                    for (int i = 0; i < NX; ++i) {
+                        if (!(i >= b[0] && i < b[3] && j >= b[1] && j <= b[3] && k >= b[2] && k <= b[5])) continue;
+                        i64 x = s->id(i, j, k);
+                        R D = two * eps[0][x] * eps_0 + dt * se[0][x] + rfactor;
+                        s->Cexe[x]  = (two * eps[0][x] * eps_0 - dt * se[0][x] - rfactor) / D;
+                        s->Cexhz[x] = (two * dt / dy) / D;
+                        s->Cexhy[x] = ((R)-2 * dt / dz) / D;
+                    }
+            const int vmag = b[5] - b[2];
+            if (vmag == 0) continue;
+            for (int k = b[2]; k < b[5]; ++k)
+                for (int j = b[1]; j <= b[4]; ++j)
+                    for (int i = b[0]; i <= b[3]; ++i) {
+                        i64 x = s->id(i, j, k);
+                        s->src_of[x] = q;
+                        s->Cezj[x] = (R)(float)1.2e-38;
+                    }
+            for (int t = 0; t < sc->nsteps; ++t) {
+                double v = waveform<R>(sc->src_wtype[q], sc->src_wpar + 4 * q, (double)t * sc->dt);
+                s->vpn[(size_t)q * sc->nsteps + t] = (R)v / (R)vmag;
+            }
+            continue;
+        }
         R r_comp = (((R)1 + _ie) - _is) * (((R)1 + _je) - _js) / (_ke - _ks);
         r_comp = r_comp * (R)sc->src_R[q];
         const R rfactor = (dt * dz) / (r_comp * dx * dy);

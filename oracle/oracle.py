@@ -29,7 +29,8 @@ class OracleScene(C.Structure):
                 ("design", C.c_int * 6), ("design_eps", C.POINTER(C.c_float)),
                 ("nsteps", C.c_int),
                 ("nplate", C.c_int), ("plate", C.POINTER(C.c_int)), ("plate_sigma", C.POINTER(C.c_float)),
-                ("nres", C.c_int), ("res", C.POINTER(C.c_int)), ("res_R", C.POINTER(C.c_double))]
+                ("nres", C.c_int), ("res", C.POINTER(C.c_int)), ("res_R", C.POINTER(C.c_double)),
+                ("src_dir", C.POINTER(C.c_int))]
 
 
 def build():
@@ -118,6 +119,7 @@ class Oracle:
         os_.src_R = arr([s.R for s in sc.sources], C.c_double, np.float64)
         os_.src_wtype = arr([s.wtype for s in sc.sources], C.c_int, np.int32)
         os_.src_wpar = arr(np.array(wpar, dtype=np.float64).ravel(), C.c_double, np.float64)
+        os_.src_dir = arr([getattr(s, "dir", 3) for s in sc.sources], C.c_int, np.int32)
         os_.nprobe = len(sc.probes)
         os_.probe = arr(np.array(sc.probes, dtype=np.int32).ravel(), C.c_int, np.int32)
         os_.nfreq = len(sc.freqs)

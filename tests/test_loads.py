@@ -1,5 +1,6 @@
-"""Conducting plates and lumped resistors (definePlateX/Y/Z, defineResistor)
-and the reference's own patch-antenna scene.
+"""Conducting plates, lumped resistors and x-directed lumped sources
+(definePlateX/Y/Z, defineResistor, defineSourceX) and the reference's own
+patch-antenna scene.
 
 CPU part (oracle, FP64), pinned by geometry and physics rather than by the
 reference's formulas:
@@ -8,7 +9,12 @@ reference's formulas:
   * a plate of sigma 1e10 is a conductor: the tangential E on it stays
     below 1e-6 of the field around it;
   * a resistor edge with the port around it obeys I = -V/R minus the gap's
-    displacement current, step by step (its update as a current balance).
+    displacement current, step by step (its update as a current balance);
+  * an x-directed source edge obeys the same balance along x, with the
+    source's R split over (1+je-js)(1+ke-ks) parallel columns of (ie-is)
+    series edges; its loaded edges are is<=i<ie, js<=j<=ie, ks<=k<=ke (the
+    reference bounds j by ie), and the only drive it adds is the Ez
+    injection through setZero's Cezj = 1.2e-38 on the box's Ez edges.
 GPU part: the engine bit for bit against the FP32 oracle on a scene with
 plates, a resistor and a port, and on the patch-antenna scene.
 """
@@ -103,6 +109,61 @@ def test_resistor_balance_with_the_port():
     assert np.max(np.abs(raw[:-1] - pred)) < 1e-9 * scale
 
 
+def _xsource(ijk, R=60.0, amp=1.0, dt=None):
+    return scenes.Source(ijk=ijk, R=R, dir=1, **scenes.gauss_deriv(dt, amp))
+
+
+def test_xsource_loads_exactly_its_edges():
+    box = (6, 7, 8, 10, 9, 9)                      # je = 9 < ie = 10
+    sc = _scene(steps=1)
+    sc.sources.append(_xsource(box, dt=sc.dt))
+    o = Oracle(sc, precision=64)
+    ov = Oracle(_scene(steps=1), precision=64)
+    n = sc.nx
+    changed = {(i, j, k) for k, j, i in zip(*np.nonzero(o.array("Cexe").reshape(n, n, n)
+                                                        != ov.array("Cexe").reshape(n, n, n)))}
+    expect = {(i, j, k) for i in range(6, 10) for j in range(7, 11) for k in (8, 9)}
+    assert changed == expect, sorted(changed ^ expect)[:6]
+    for name in ("Ceze", "Cezhy", "Cezhx", "Ceye"):
+        assert np.array_equal(o.array(name), ov.array(name)), name
+    cezj = o.array("Cezj").reshape(n, n, n) - ov.array("Cezj").reshape(n, n, n)
+    inj = {(i, j, k) for k, j, i in zip(*np.nonzero(cezj))}
+    assert inj == {(i, 7 + dj, 8) for i in range(6, 11) for dj in range(3)}
+
+
+def test_xsource_edge_balance():
+    R = 60.0
+    box = (8, 9, 9, 10, 10, 10)                    # 2 x 2 parallel columns of 2 series edges
+    sc = _scene(steps=100)
+    sc.sources.append(_xsource(box, R=R, dt=sc.dt))
+    i, j, k = 8, 9, 9
+    sc.probes = [(0, i, j, k), (5, i, j, k), (5, i, j - 1, k), (4, i, j, k), (4, i, j, k - 1)]
+    o = Oracle(sc, precision=64)
+    o.steps(sc.steps)
+    ex, hz, hzm, hy, hym = o.probes()
+    r_comp = R * (1 + box[4] - box[1]) * (1 + box[5] - box[2]) / (box[3] - box[0])
+    circ = (hz - hzm)[:-1] * sc.dz - (hy - hym)[:-1] * sc.dy
+    pred = EPS0 * sc.dy * sc.dz / sc.dt * np.diff(ex) + sc.dx * (ex[1:] + ex[:-1]) / (2 * r_comp)
+    scale = np.max(np.abs(circ))
+    assert scale > 0
+    assert np.max(np.abs(circ - pred)) < 1e-9 * scale
+
+
+def test_xsource_alone_drives_only_through_tiny_cezj():
+    box = (7, 7, 8, 10, 8, 10)
+    sc = _scene(steps=1)
+    sc.sources = [_xsource(box, dt=sc.dt, amp=1e3)]
+    o = Oracle(sc, precision=64)
+    o.steps(1)
+    n = sc.nx
+    ez = o.array("Ez").reshape(n, n, n)
+    nz = {(i, j, k) for k, j, i in zip(*np.nonzero(ez))}
+    assert nz == {(i, j, k) for i in range(7, 11) for j in (7, 8) for k in (8, 9)}
+    vals = ez[ez != 0]
+    assert np.all(vals == vals[0]) and 0 < abs(vals[0]) < 1e-30
+    assert not np.any(o.array("Ex")) and not np.any(o.array("Ey"))
+
+
 def test_patch_antenna_scene_matches_source_cu():
     sc = scenes.patch_antenna(steps=10)
     # 60 x 100 x 3 cells of object, 15 air + 10 CPML on every side
@@ -162,3 +223,49 @@ def test_patch_antenna_bitwise_vs_oracle():
     assert np.max(np.abs(vo)) > 0 and np.max(np.abs(io)) > 0
     assert np.array_equal(v, vo)
     assert np.array_equal(i, io)
+
+
+@pytest.mark.gpu
+def test_xsource_bitwise_vs_oracle():
+    """An x-directed source beside a z-source (its loaded edges then carry
+    the field), and the x-source alone (fields of order 1e-38, denormal in
+    FP32)."""
+    from fdtd_amd import api
+    base = _scene(n=22, steps=120, src=(8, 11, 8),
+                  probes=[(0, 12, 10, 11), (0, 13, 12, 12), (2, 13, 11, 11), (4, 14, 12, 11)])
+    base.sources.append(_xsource((12, 10, 11, 15, 11, 12), R=40.0, dt=base.dt))
+    alone = _scene(n=22, steps=60, probes=base.probes)
+    alone.sources = [_xsource((12, 10, 11, 15, 11, 12), R=40.0, dt=alone.dt, amp=1e3)]
+    for name, sc in (("with z-source", base), ("alone", alone)):
+        print("x-source case:", name, flush=True)
+        sim = api.Simulation.from_scene(sc, "f32")
+        sim.run(sc.steps)
+        pr = sim.probes()
+        sim.close()
+        o = Oracle(sc, precision=32)
+        o.steps(sc.steps)
+        po = o.probes().astype(np.float32)
+        assert np.max(np.abs(po)) > 0, name
+        assert np.array_equal(pr, po), (name, np.max(np.abs(pr - po)))
+
+
+@pytest.mark.gpu
+def test_xsource_errors():
+    from fdtd_amd import api
+    sc = _scene(n=18, steps=10)
+    sim = api.Simulation.from_scene(sc, "f32", finalize=False)
+    w = scenes.gauss_deriv(sc.dt)
+    add = lambda ijk, d=1: sim.add_lumped_source(ijk, 50.0, w["wtype"], w["amp"], w["t0"], w["tau"], direction=d)
+    for ijk, d, code in (((6, 6, 6, 9, 6, 6), 2, -6),      # y-directed: not in the reference
+                         ((6, 6, 6, 6, 6, 6), 1, -1),      # ie == is: no x edge
+                         ((6, 9, 6, 8, 9, 6), 1, -1),      # js > ie: no row
+                         ((0, 3, 6, 4, 6, 6), 1, -3),      # i = 0 Ez edge is not updated
+                         ((6, 6, 6, 9, 6, 17), 1, -3)):    # past the updated Ex planes
+        with pytest.raises(api.FdtdError) as ei:
+            add(ijk, d)
+        assert ei.value.code == code, (ijk, d)
+    add((1, 6, 6, 5, 6, 6))                                # accepted, but inside the x CPML slab
+    with pytest.raises(api.FdtdError) as ei:
+        sim.finalize(sc.steps)
+    assert "CPML" in str(ei.value)
+    sim.close()

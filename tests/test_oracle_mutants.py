@@ -105,6 +105,22 @@ LOAD_MUTANTS = {
                       "s->Ceze[x]  = (two * eps[2][x] * eps_0 - dt * se[2][x] + rfactor) / D;\n"
                       "                    s->Cezhy[x] = (two * dt / dx) / D;\n"
                       "                    s->Cezhx[x] = ((R)-2 * dt / dy) / D;\n                }\n    }"),
+    # x-source rows bounded by je (the reference bounds them by ie)
+    "xsource_rows": ("i >= b[0] && i < b[3] && j >= b[1] && j <= b[3] && k >= b[2] && k <= b[5]",
+                     "i >= b[0] && i < b[3] && j >= b[1] && j <= b[4] && k >= b[2] && k <= b[5]"),
+    # x-source resistance not split over the parallel columns
+    "xsource_rcomp": ("R r_comp = (((R)1 + _je) - _js) * (((R)1 + _ke) - _ks) / (_ie - _is);",
+                      "R r_comp = (R)1 / (_ie - _is);"),
+    # x-source factor with dz in place of dx
+    "xsource_cell": ("const R rfactor = (dt * dx) / (r_comp * dz * dy);",
+                     "const R rfactor = (dt * dz) / (r_comp * dz * dy);"),
+    # x-source Ez injection over ks..ke inclusive
+    "xsource_inject": ("for (int k = b[2]; k < b[5]; ++k)\n                for (int j = b[1]; j <= b[4]; ++j)\n"
+                       "                    for (int i = b[0]; i <= b[3]; ++i) {\n                        i64 x = s->id(i, j, k);\n"
+                       "                        s->src_of[x] = q;\n                        s->Cezj[x] = (R)(float)1.2e-38;",
+                       "for (int k = b[2]; k <= b[5]; ++k)\n                for (int j = b[1]; j <= b[4]; ++j)\n"
+                       "                    for (int i = b[0]; i <= b[3]; ++i) {\n                        i64 x = s->id(i, j, k);\n"
+                       "                        s->src_of[x] = q;\n                        s->Cezj[x] = (R)(float)1.2e-38;"),
 }
 
 
