@@ -264,7 +264,7 @@ def test_xsource_errors():
         with pytest.raises(api.FdtdError) as ei:
             add(ijk, d)
         assert ei.value.code == code, (ijk, d)
-    add((1, 6, 6, 5, 6, 6))                                # accepted, but inside the x CPML slab
+    add((1, 4, 6, 5, 6, 6))                                # accepted, but inside the x CPML slab
     with pytest.raises(api.FdtdError) as ei:
         sim.finalize(sc.steps)
     assert "CPML" in str(ei.value)
