@@ -4,8 +4,9 @@
 // (fdtdrav.h:3425-4387), and its per-step driver, solver (fdtdrav.h:3188-3422),
 // for one device: the z-slab split and ghost-plane copies of the reference
 // collapse into a single domain whose neighbour planes are plain HBM reads.
-// All work goes to one non-blocking stream; a step is two launches (E, H)
-// with no host synchronisation between them.
+// All work goes to one non-blocking stream; a step is the E sweep then the
+// H sweep (one launch per non-empty region variant each), with no host
+// synchronisation between them.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -65,7 +66,6 @@ struct fdtd_ctx {
     bool pair = false;              // 16-bit storage: two cells per thread in the half-step kernels
     int pair_tiles_x = 0, pair_kchunk = 0;
     int kchunk_h = 0;               // planes per block of the one-cell H sweep (E: ka.kchunk)
-    // per sweep (0 = E, 1 = H): tiles grouped by region variant
     int box[2][6] = {};             // interior tile box per sweep (0 = E, 1 = H)
     KArgs ka_step{};                // tile geometry of the step kernel
     double step_ms = 0.0;
@@ -168,8 +168,7 @@ int zero_state(fdtd_ctx *c)
     return FDTD_OK;
 }
 
-// Rebuild the material-dependent tables (class map, E/H tables, sources)
-// after a design-eps change; buffers of unchanged size are reused.
+// Upload the material-dependent tables (class map, E/H tables, sources).
 int upload_tables(fdtd_ctx *c);
 
 // Planes per z-march block.  Many short marches keep every CU's wave slots
@@ -285,9 +284,6 @@ int do_finalize(fdtd_ctx *c)
     const size_t px_elems = (size_t)in.nz * in.ny * a.Lx;
     const size_t py_elems = (size_t)in.nz * a.Ly * in.px;
     const size_t pz_elems = (size_t)a.Lz * in.ny * in.px;
-    void **pp[12] = {&a.psi_eyx, &a.psi_ezx, &a.psi_hyx, &a.psi_hzx, &a.psi_exy, &a.psi_ezy,
-                     &a.psi_hxy, &a.psi_hzy, &a.psi_exz, &a.psi_eyz, &a.psi_hxz, &a.psi_hyz};
-    (void)pp;
     for (int p = 0; p < 12; ++p) {
         c->psi_elems[p] = p < 4 ? px_elems : p < 8 ? py_elems : pz_elems;
         const bool eside = (p % 4) < 2;
@@ -442,6 +438,16 @@ hipError_t do_step(fdtd_ctx *c, const KArgs &base, int m, hipEvent_t ev0 = nullp
     return e;
 }
 
+// The events of one call, destroyed on every return path.
+struct EventSet {
+    std::vector<hipEvent_t> v;
+    ~EventSet()
+    {
+        for (hipEvent_t e : v)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
 int run_steps(fdtd_ctx *c, int nsteps, bool profile)
 {
     if (!c->finalized) return fail(c, FDTD_E_STATE, "fdtd_finalize has not been called");
@@ -449,14 +455,11 @@ int run_steps(fdtd_ctx *c, int nsteps, bool profile)
     if (c->steps_done + nsteps > c->in.steps_cap)
         return fail(c, FDTD_E_RANGE, "run past the finalized horizon (%d + %d > %d)", c->steps_done, nsteps,
                     c->in.steps_cap);
-    hipEvent_t t0, t1;
-    HIPCHK(c, hipEventCreate(&t0));
-    HIPCHK(c, hipEventCreate(&t1));
-    std::vector<hipEvent_t> evs;
-    if (profile) {
-        evs.resize((size_t)4 * nsteps);
-        for (auto &e : evs) HIPCHK(c, hipEventCreate(&e));
-    }
+    EventSet all;
+    all.v.assign(2 + (profile ? (size_t)4 * nsteps : 0), nullptr);
+    for (auto &e : all.v) HIPCHK(c, hipEventCreate(&e));
+    hipEvent_t t0 = all.v[0], t1 = all.v[1];
+    hipEvent_t *evs = all.v.data() + 2;
     const int m0 = c->steps_done;
     HIPCHK(c, hipEventRecord(t0, c->stream));
     for (int s = 0; s < nsteps; ++s) {
@@ -488,10 +491,7 @@ int run_steps(fdtd_ctx *c, int nsteps, bool profile)
             c->h_ms = sh / nsteps;
             c->step_ms = c->e_ms + c->h_ms;
         }
-        for (auto &e : evs) (void)hipEventDestroy(e);
     }
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     c->steps_done = m0 + nsteps;
     // blow-up detection on the probe samples of this call
     const size_t np = c->in.probes.size();
@@ -682,10 +682,10 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     const int mode = c->built.mode;
     const int euni = c->built.e_uniform ? 1 : 0, huni = c->built.h_uniform ? 1 : 0;
 
-    hipEvent_t t0, t1, t2;
-    HIPCHK(c, hipEventCreate(&t0));
-    HIPCHK(c, hipEventCreate(&t1));
-    HIPCHK(c, hipEventCreate(&t2));
+    EventSet evs;
+    evs.v.assign(3, nullptr);
+    for (auto &e : evs.v) HIPCHK(c, hipEventCreate(&e));
+    hipEvent_t t0 = evs.v[0], t1 = evs.v[1], t2 = evs.v[2];
     HIPCHK(c, hipEventRecord(t0, c->stream));
     // ---- forward pass: DFT at the probes; checkpoints at segment starts, or
     // the whole design history when there is a single segment
@@ -751,9 +751,6 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     float ms1 = 0.f, ms2 = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms1, t0, t1));
     HIPCHK(c, hipEventElapsedTime(&ms2, t0, t2));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
-    (void)hipEventDestroy(t2);
     c->adj_fwd_ms = ms1;
     c->adj_total_ms = ms2;
     c->adj_segments = nseg;
@@ -994,7 +991,9 @@ int fdtd_set_design_eps(fdtd_ctx *c, const float *eps, size_t n)
     // rebuild every table on the new materials and restart from step 0
     free_device(c);
     c->finalized = false;
-    return do_finalize(c);
+    const int rc = do_finalize(c);
+    if (rc) free_device(c);
+    return rc;
 }
 
 int fdtd_finalize(fdtd_ctx *c, int n_steps)
