@@ -1,7 +1,8 @@
 """Host-side checks that need no GPU: the C ABI library loads and exports
 every function include/fdtd.h declares, the Python binding matches the
-header, argument validation in fdtd_create, the seeded scene generator and
-the benchmark's bookkeeping."""
+header, argument validation in fdtd_create, the seeded scene generator,
+the benchmark's bookkeeping, and the integration examples (the marchingLoop
+shim compiles and links; the Python snippets parse)."""
 import ctypes as C
 import math
 import os
@@ -184,3 +185,34 @@ def test_product_path_has_no_oracle_dependency():
             if f.endswith((".py", ".hip", ".cpp", ".h")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert not re.search(r"^\s*(import|from)\s+oracle|oracle_\w+\s*\(|liboracle", text, re.M), f
+
+
+def _doc_blocks(lang):
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    return re.findall(r"```%s\n(.*?)```" % lang, text, flags=re.S)
+
+
+@pytest.mark.parametrize("compiler", [["gcc", "-std=c99", "-x", "c"], ["g++", "-std=c++17", "-x", "c++"]])
+def test_integration_shim_compiles_and_links(compiler, tmp_path, lib):
+    """The marchingLoop shim of INTEGRATION.md is valid C and C++ against
+    include/fdtd.h, and every call it makes resolves in the library."""
+    blocks = _doc_blocks("c")
+    assert len(blocks) == 1
+    src = tmp_path / "shim.c"
+    src.write_text(blocks[0])
+    from fdtd_amd import api
+    libdir = os.path.dirname(api.LIB_PATH)
+    out = tmp_path / "libshim.so"
+    cmd = compiler + ["-Wall", "-Wextra", "-Wno-unused-parameter", "-Werror", "-fPIC", "-shared",
+                      "-I", os.path.join(ROOT, "include"), str(src), "-x", "none", "-L", libdir, "-lfdtd_amd",
+                      "-Wl,--no-undefined", "-Wl,-rpath," + libdir, "-lm", "-o", str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_integration_python_examples_parse():
+    import ast
+    blocks = _doc_blocks("python")
+    assert len(blocks) >= 3
+    for b in blocks:
+        ast.parse(b)
