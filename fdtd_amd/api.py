@@ -33,6 +33,7 @@ EXPORTS = {
     "fdtd_add_box": (C.c_int, [_P] + [C.c_int] * 6 + [C.POINTER(C.c_float)] * 4),
     "fdtd_add_lumped_source": (C.c_int, [_P, C.c_int] + [C.c_int] * 6 + [C.c_double, _P]),
     "fdtd_add_plate": (C.c_int, [_P, C.c_int] + [C.c_int] * 6 + [C.POINTER(C.c_float)]),
+    "fdtd_add_sector": (C.c_int, [_P] + [C.c_int] * 7 + [C.c_float, C.c_int, C.c_int, C.c_float, C.c_float]),
     "fdtd_add_resistor": (C.c_int, [_P, C.c_int] + [C.c_int] * 6 + [C.c_double]),
     "fdtd_add_probe": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "fdtd_set_dft": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
@@ -191,6 +192,11 @@ class Simulation:
         sg = (C.c_float * 3)(*sigma)
         self._check(self.lib.fdtd_add_plate(self.h, axis, *ijk, sg))
 
+    def add_sector(self, points, kz, radius, max_x, max_y, sigma_x, sigma_y):
+        """points = (i0, j0, i1, j1, i2, j2), apex (i1, j1); radius in metres."""
+        self._check(self.lib.fdtd_add_sector(self.h, *[int(v) for v in points], int(kz), float(radius),
+                                             int(max_x), int(max_y), float(sigma_x), float(sigma_y)))
+
     def add_resistor(self, ijk, R, direction=3):
         self._check(self.lib.fdtd_add_resistor(self.h, direction, *ijk, float(R)))
 
@@ -311,6 +317,8 @@ class Simulation:
             sim.add_box(b.ijk, b.eps_r, b.sigma_e, b.mu_r, b.sigma_m)
         for s in sc.sources:
             sim.add_lumped_source(s.ijk, s.R, s.wtype, s.amp, s.t0, s.tau, s.freq, direction=s.dir)
+        for q in getattr(sc, "sectors", []):
+            sim.add_sector(q.points, q.kz, q.radius, q.max_x, q.max_y, q.sigma[0], q.sigma[1])
         for p in getattr(sc, "plates", []):
             sim.add_plate(p.axis, p.ijk, p.sigma)
         for r in getattr(sc, "resistors", []):

@@ -879,6 +879,23 @@ int fdtd_add_box(fdtd_ctx *c, int is, int js, int ks, int ie, int je, int ke, co
     return FDTD_OK;
 }
 
+int fdtd_add_sector(fdtd_ctx *c, int i0, int j0, int i1, int j1, int i2, int j2, int kz, float radius, int max_x,
+                    int max_y, float sigma_x, float sigma_y)
+{
+    if (!c) return FDTD_E_INVALID;
+    if (c->finalized) return fail(c, FDTD_E_STATE, "scene is finalized");
+    if (!(radius > 0) || !std::isfinite(radius)) return fail(c, FDTD_E_INVALID, "sector radius must be > 0");
+    if (!(sigma_x >= 0) || !(sigma_y >= 0) || !std::isfinite(sigma_x) || !std::isfinite(sigma_y))
+        return fail(c, FDTD_E_INVALID, "invalid sector sigma");
+    if (kz < 1 || kz > c->in.nz - 2 || max_x < 0 || max_x >= c->in.nx || max_y < 0 || max_y >= c->in.ny)
+        return fail(c, FDTD_E_RANGE, "sector plane %d or extent (%d, %d) outside the grid", kz, max_x, max_y);
+    const int lim = 1 << 24;   // node coordinates stay exact in float
+    for (int v : {i0, j0, i1, j1, i2, j2})
+        if (v <= -lim || v >= lim) return fail(c, FDTD_E_RANGE, "sector point index out of range");
+    c->in.sectors.push_back(Sector{{i0, j0, i1, j1, i2, j2}, kz, max_x, max_y, radius, {sigma_x, sigma_y}});
+    return FDTD_OK;
+}
+
 int fdtd_add_plate(fdtd_ctx *c, int axis, int is, int js, int ks, int ie, int je, int ke, const float sigma[3])
 {
     if (!c || !sigma) return FDTD_E_INVALID;

@@ -147,6 +147,11 @@ struct Plate {
     int axis, b[6];
     float sigma[3];
 };
+struct Sector {
+    int p[6];             // i0 j0 i1 j1 i2 j2 (apex i1, j1)
+    int kz, max_x, max_y;
+    float radius, sigma[2];
+};
 struct Resistor {
     int b[6];
     double R;
@@ -188,6 +193,7 @@ struct SceneIn {
     CpmlParams cpml;
     std::vector<Box> boxes;
     std::vector<Source> sources;
+    std::vector<Sector> sectors;
     std::vector<Plate> plates;
     std::vector<Resistor> resistors;
     std::vector<Probe> probes;

@@ -14,8 +14,9 @@
 //   source loading      defineSourceZ        fdtdrav.h:1391-1421
 //   conducting plates   definePlateX/Y/Z     fdtdrav.h:1574-1678
 //   lumped resistors    defineResistor       fdtdrav.h:1429-1484
-//   x-directed source   defineSourceX        fdtdrav.h:1311-1366 (a load only:
-//                       calc_e's Ex injection is commented out, fdtdrav.h:681)
+//   conducting sectors  defineSec, tricheck  fdtdrav.h:1681-1744, 56-128
+//   x-directed source   defineSourceX        fdtdrav.h:1311-1366 (loaded Ex
+//                       edges; calc_e's Ex injection is commented out, fdtdrav.h:681)
 //   per-node waveform   setGauss / setSine   fdtdrav.h:1486-1520
 #include <hip/hip_runtime.h>
 
@@ -66,6 +67,24 @@ inline float harm(float a, float b)
 struct Consts {
     float dx, dy, dz, dt, eps0, mu0;
 };
+
+// tricheck (fdtdrav.h:56-128): 0 when the point (px, py) lies in the sector
+// with apex (x1, y1) between the rays towards (x0, y0) and (x2, y2), within
+// radius r of the apex; -1 otherwise.  As there, the coordinate differences
+// and the squared distance are float operations widened to double, and the
+// two side tests are products of double cross products.
+inline int sector_test(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2, float r)
+{
+    const double ax = x1 - x0, ay = y1 - y0, bx = x2 - x1, by = y2 - y1;
+    const double cx = x2 - x0, cy = y2 - y0, ex = x0 - x1, ey = y0 - y1;
+    const double zx1 = px - x0, zy1 = py - y0, zx2 = px - x1, zy2 = py - y1;
+    const double dist = std::sqrt((double)(((x1 - px) * (x1 - px)) + ((y1 - py) * (y1 - py))));
+    auto cross = [](double ux, double uy, double vx, double vy) { return (ux * vy) - (uy * vx); };
+    if (cross(ax, ay, cx, cy) * cross(ax, ay, zx1, zy1) < 0) return -1;
+    if (cross(bx, by, ex, ey) * cross(bx, by, zx2, zy2) < 0) return -1;
+    if (dist > (double)r) return -1;
+    return 0;
+}
 
 // (ce, cb_a, cb_b) of one E component (fdtdrav.h:1271-1305).
 inline void ecoef(const Consts &c, float eps, float sig, float da, float db, float *o)
@@ -151,6 +170,21 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
     // call order (definePlateX/Y/Z, fdtdrav.h:1574-1678, applied between
     // setSigmaEpsSigmaMu and setCoefficients, fdtdrav.h:4183-4210)
     std::unordered_map<i64, float> sig_over[3];
+    // sectors first: defineSec (fdtdrav.h:1681-1744) on the node points
+    // (i*dx, j*dy) of the plane kz, in its marchingLoop slot before the plates
+    for (const Sector &sc : in.sectors) {
+        const float dx = C.dx, dy = C.dy;
+        const float x0 = (float)sc.p[0] * dx, y0 = (float)sc.p[1] * dy, x1 = (float)sc.p[2] * dx;
+        const float y1 = (float)sc.p[3] * dy, x2 = (float)sc.p[4] * dx, y2 = (float)sc.p[5] * dy;
+        const i64 kb = (i64)ny * sc.kz;
+        for (int j = 0; j <= sc.max_y; ++j)
+            for (int i = 0; i <= sc.max_x; ++i) {
+                if (sector_test((float)i * dx, (float)j * dy, x0, y0, x1, y1, x2, y2, sc.radius) != 0) continue;
+                const i64 cell = i + nx * (j + kb);
+                if (i < sc.max_x) sig_over[0][cell] = sc.sigma[0];
+                if (j < sc.max_y) sig_over[1][cell] = sc.sigma[1];
+            }
+    }
     for (const Plate &pl : in.plates) {
         const int *b = pl.b;
         auto put = [&](int comp, int i0, int i1, int j0, int j1, int k0, int k1) {   // inclusive ranges
@@ -200,7 +234,7 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
                             err = "a resistor overlaps a lumped source";
                             return -1;
                         }
-        if (!in.design_eps.empty() && (!in.plates.empty() || !in.resistors.empty())) {
+        if (!in.design_eps.empty() && (!in.plates.empty() || !in.sectors.empty() || !in.resistors.empty())) {
             // the adjoint's edge derivatives assume the averaged materials
             const int *d = in.design;
             auto near = [&](i64 cell) {
@@ -209,7 +243,7 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
             };
             for (int c = 0; c < 3; ++c)
                 for (const auto &kv : sig_over[c])
-                    if (near(kv.first)) { err = "a plate touches the design region"; return -1; }
+                    if (near(kv.first)) { err = "a plate or sector touches the design region"; return -1; }
             for (const auto &kv : res_of)
                 if (near(kv.first)) { err = "a resistor touches the design region"; return -1; }
         }

@@ -97,6 +97,87 @@ class Source:
     dir: int = 3      # 3: z-directed (defineSourceZ); 1: x-directed (defineSourceX: loaded Ex edges)
 
 
+def _c_round(v: float) -> int:
+    """C's round(): halves away from zero (Python's round() goes to even)."""
+    a = abs(v)
+    r = math.floor(a)
+    if a - r >= 0.5:
+        r += 1
+    return int(r if v >= 0 else -r)
+
+
+def calc_index(coord, d) -> int:
+    """calc_index (fdtdrav.h:198-206): round(coord / d) of float32 values."""
+    return _c_round(float(np.float32(np.float32(coord) / np.float32(d))))
+
+
+_LIBM = None
+
+
+def _libm_f(name):
+    """glibc's float function `name` (cosf, sinf, tanf): the host build of
+    the reference calls these."""
+    global _LIBM
+    import ctypes
+    import ctypes.util
+    if _LIBM is None:
+        _LIBM = ctypes.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+    f = getattr(_LIBM, name)
+    f.restype = ctypes.c_float
+    f.argtypes = [ctypes.c_float]
+    return lambda x: np.float32(f(float(x)))
+
+
+def sector_points(raio, sec_angle, rot_angle, p1, gap, trans_x, trans_y, i, pml_n, air_n, d):
+    """sectorPoints (fdtdrav.h:2550-2618), float32 throughout: the ray ends
+    at angle +-sec_angle/2 and distance raio from the apex, offset by gap,
+    rotated by rot_angle (rotate_point, fdtdrav.h:218-232: x' = x cos + y
+    sin, y' = -x sin + y cos), slid by desloc = gap - |p1'.x| along x (to the
+    left for sectors i < 3) with the matching y shift, translated by
+    (trans_x, trans_y) and indexed with calc_index plus the air and CPML
+    offsets.  p1 is the apex (ponto1); pml_n, air_n and d are (x, y) pairs.
+    Returns (i0, j0, i1, j1, i2, j2) for Sector.points."""
+    f32 = np.float32
+    cosf, sinf, tanf = _libm_f("cosf"), _libm_f("sinf"), _libm_f("tanf")
+    raio, gap, rot = f32(raio), f32(gap), f32(rot_angle)
+    half = f32(f32(sec_angle) / f32(2))
+
+    def rotate(x, y):
+        c, s_ = cosf(rot), sinf(rot)
+        return f32(f32(x * c) + f32(y * s_)), f32(f32(f32(-x) * s_) + f32(y * c))
+
+    p2 = (f32(-f32(f32(raio * cosf(half)) + gap)), f32(raio * sinf(half)))
+    p0 = (f32(-f32(f32(raio * cosf(half)) + gap)), f32(-f32(raio * sinf(half))))
+    p0 = rotate(*p0)
+    q1 = rotate(f32(p1[0]), f32(p1[1]))
+    p2 = rotate(*p2)
+    desloc = f32(gap - abs(q1[0]))
+    sh = f32(desloc * tanf(rot))
+    sgn = 1 if i < 3 else -1
+    pts = []
+    for (x, y) in (p0, q1, p2):
+        x = f32(x - desloc) if sgn > 0 else f32(x + desloc)
+        y = f32(y + sh) if sgn > 0 else f32(y - sh)
+        x = f32(x - f32(-f32(trans_x)))        # trans_point(x, -trans_x)
+        y = f32(y - f32(-f32(trans_y)))
+        pts += [air_n[0] + pml_n[0] + calc_index(x, d[0]), air_n[1] + pml_n[1] + calc_index(y, d[1])]
+    return tuple(pts)
+
+
+@dataclass
+class Sector:
+    """Zero-thickness conducting sector in the plane k = kz (defineSec,
+    fdtdrav.h:1681-1744): apex (i1, j1), rays towards (i0, j0) and (i2, j2),
+    radius in metres; Ex edges with i < max_x, j <= max_y and Ey edges with
+    i <= max_x, j < max_y of the nodes inside get sigma = (sigma_x, sigma_y)."""
+    points: Tuple[int, int, int, int, int, int]
+    kz: int
+    radius: float
+    max_x: int
+    max_y: int
+    sigma: Tuple[float, float]
+
+
 @dataclass
 class Plate:
     """Zero-thickness conducting plate (definePlateX/Y/Z, fdtdrav.h:1574-1678):
@@ -139,6 +220,7 @@ class Scene:
     sources: List[Source] = field(default_factory=list)
     probes: List[Tuple[int, int, int, int]] = field(default_factory=list)   # (comp, i, j, k)
     freqs: List[float] = field(default_factory=list)
+    sectors: List[Sector] = field(default_factory=list)
     plates: List[Plate] = field(default_factory=list)
     resistors: List[Resistor] = field(default_factory=list)
     port: Optional[Port] = None
@@ -324,7 +406,7 @@ def patch_antenna(steps=3000) -> Scene:
     o = pml + air
 
     def ci(coord, d):   # calc_index, fdtdrav.h:198-206
-        return int(round(float(f32(f32(coord) / d))))
+        return calc_index(coord, d)
 
     X, Y, Z = ci(60 * dx, dx), ci(100 * dy, dy), ci(3 * dz, dz)
     nx, ny, nz = X + 2 * o, Y + 2 * o, Z + 2 * o

@@ -189,6 +189,19 @@ int fdtd_add_lumped_source(fdtd_ctx *ctx, int dir, int is, int js, int ks,
  * k<ke get sigma[2]) and axis 2 (plane j = js: Ez edges i<=ie, k<ke get
  * sigma[2], Ex edges i<ie, k<=ke get sigma[0]) likewise.  sigma >= 0 (S/m;
  * the reference's PEC plates use 1e10). */
+/* Zero-thickness conducting sector in the plane k = kz: a node (i, j) lies
+ * in it when it is on the side of the line (i0,j0)-(i1,j1) where (i2,j2) is,
+ * on the side of the line (i1,j1)-(i2,j2) where (i0,j0) is, and within
+ * `radius` metres of the apex (i1, j1) -- tricheck (fdtdrav.h:56-128) on the
+ * node coordinates i*dx, j*dy.  The Ex edges of such nodes with 0<=i<max_x,
+ * 0<=j<=max_y get sigma_x, the Ey edges with 0<=i<=max_x, 0<=j<max_y get
+ * sigma_y: defineSec (fdtdrav.h:1681-1744), applied after the edge averaging
+ * and before the plates (its marchingLoop slot, fdtdrav.h:4152-4180, where
+ * the call is disabled).  Sectors apply in call order.
+ * 0<kz<nz-1, 0<=max_x<nx, 0<=max_y<ny, radius > 0. */
+int fdtd_add_sector(fdtd_ctx *ctx, int i0, int j0, int i1, int j1, int i2, int j2, int kz, float radius,
+                    int max_x, int max_y, float sigma_x, float sigma_y);
+
 int fdtd_add_plate(fdtd_ctx *ctx, int axis, int is, int js, int ks, int ie, int je, int ke,
                    const float sigma[3]);
 

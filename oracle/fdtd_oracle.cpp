@@ -16,6 +16,8 @@
  *   CPML profiles, cpsi, kappa   H:1747-2160 (setABCPML)
  *   lumped z-source loading      H:1368-1426 (defineSourceZ)
  *   lumped x-source loading      H:1311-1366 (defineSourceX)
+ *   conducting sectors           H:56-128 (tricheck), H:1681-1744 (defineSec)
+ *   conducting plates            H:1574-1678 (definePlateX/Y/Z)
  *   waveforms                    H:1486-1520 (setGauss, setSine)
  *   E update + source + sample   H:577-743 (calc_e), CPML H:750-911
  *   H update + sample            H:249-395 (calc_h), CPML H:398-562
@@ -76,6 +78,9 @@ typedef struct {
     const int *res;             /* 6 per resistor: is js ks ie je ke (z-directed) */
     const double *res_R;
     const int *src_dir;         /* per source: 3 (z, defineSourceZ) or 1 (x, defineSourceX); NULL: all 3 */
+    int nsector;
+    const int *sector;          /* 9 per sector: i0 j0 i1 j1 i2 j2 kz max_x max_y */
+    const float *sector_f;      /* 3 per sector: radius sigma_x sigma_y */
 } oracle_scene;
 
 }  // extern "C"
@@ -83,6 +88,50 @@ typedef struct {
 namespace {
 
 typedef std::int64_t i64;
+
+// pvetorial2d H:24-31
+double pvetorial2d(double ax, double ay, double bx, double by)
+{
+    double result;
+    result = (ax * by) - (ay * bx);
+    return result;
+}
+
+// tricheck H:56-128: 0 inside the sector (apex p1, rays to p0 and p2,
+// radius raio), -1 outside.  The differences and the squared distance are
+// float expressions stored to double, as written there.
+int tricheck(float pontox, float pontoy, float px0, float py0, float px1, float py1, float px2, float py2,
+             float raio)
+{
+    double zx1, zy1, zx2, zy2;
+    double ax, ay, bx, by, cx, cy, dx, dy;
+    double rvetorial1, rvetorial2, rescalar;
+    double dist;
+    ax = px1 - px0;
+    ay = py1 - py0;
+    bx = px2 - px1;
+    by = py2 - py1;
+    cx = px2 - px0;
+    cy = py2 - py0;
+    dx = px0 - px1;
+    dy = py0 - py1;
+    zx1 = pontox - px0;
+    zy1 = pontoy - py0;
+    zx2 = pontox - px1;
+    zy2 = pontoy - py1;
+    dist = ((px1 - pontox) * (px1 - pontox)) + ((py1 - pontoy) * (py1 - pontoy));
+    dist = std::sqrt(dist);
+    rvetorial1 = pvetorial2d(ax, ay, cx, cy);
+    rvetorial2 = pvetorial2d(ax, ay, zx1, zy1);
+    rescalar = rvetorial1 * rvetorial2;
+    if (rescalar < 0) return -1;
+    rvetorial1 = pvetorial2d(bx, by, dx, dy);
+    rvetorial2 = pvetorial2d(bx, by, zx2, zy2);
+    rescalar = rvetorial1 * rvetorial2;
+    if (rescalar < 0) return -1;
+    if (dist > raio) return -1;
+    return 0;
+}
 
 template <class R>
 struct Sim {
@@ -298,6 +347,29 @@ static Sim<R> *build(const oracle_scene *sc)
                     sm[2][x] = harm(m_sm[2][x], m_sm[2][x - NXY]);
                 }
             }
+    // ---- conducting sectors, defineSec H:1681-1744 with tricheck H:56-128,
+    // in the "2D objects" slot of marchingLoop before the plates
+    // (H:4152-4180, where the call is disabled): each kernel's thread
+    // conditions over every cell, on the float node coordinates i*dx, j*dy
+    for (int q = 0; q < sc->nsector; ++q) {
+        const int *v = sc->sector + 9 * q;
+        const float *f = sc->sector_f + 3 * q;
+        const float fdx = (float)sc->dx, fdy = (float)sc->dy;
+        const int kz = v[6], max_x = v[7], max_y = v[8];
+        for (int k = 0; k < NZ; ++k)
+            for (int j = 0; j < NY; ++j)
+                for (int i = 0; i < NX; ++i) {
+                    i64 x = s->id(i, j, k);
+                    if (i >= 0 && i < max_x && j >= 0 && j <= max_y && kz == k)
+                        if (tricheck(i * fdx, j * fdy, v[0] * fdx, v[1] * fdy, v[2] * fdx, v[3] * fdy, v[4] * fdx,
+                                     v[5] * fdy, f[0]) == 0)
+                            se[0][x] = (R)f[1];
+                    if (i >= 0 && i <= max_x && j >= 0 && j < max_y && kz == k)
+                        if (tricheck(i * fdx, j * fdy, v[0] * fdx, v[1] * fdy, v[2] * fdx, v[3] * fdy, v[4] * fdx,
+                                     v[5] * fdy, f[0]) == 0)
+                            se[1][x] = (R)f[2];
+                }
+    }
     // ---- conducting plates, definePlateX/Y/Z H:1574-1678 (between the
     // averaging and the coefficients, H:4183-4210), each kernel's thread
     // conditions over every cell, plates in order

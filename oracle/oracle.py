@@ -30,7 +30,8 @@ class OracleScene(C.Structure):
                 ("nsteps", C.c_int),
                 ("nplate", C.c_int), ("plate", C.POINTER(C.c_int)), ("plate_sigma", C.POINTER(C.c_float)),
                 ("nres", C.c_int), ("res", C.POINTER(C.c_int)), ("res_R", C.POINTER(C.c_double)),
-                ("src_dir", C.POINTER(C.c_int))]
+                ("src_dir", C.POINTER(C.c_int)),
+                ("nsector", C.c_int), ("sector", C.POINTER(C.c_int)), ("sector_f", C.POINTER(C.c_float))]
 
 
 def build():
@@ -136,6 +137,12 @@ class Oracle:
         os_.nplate = len(plates)
         os_.plate = arr(np.array([[p.axis, *p.ijk] for p in plates], dtype=np.int32).ravel(), C.c_int, np.int32)
         os_.plate_sigma = arr(np.array([p.sigma for p in plates], dtype=np.float32).ravel(), C.c_float, np.float32)
+        secs = getattr(sc, "sectors", []) or []
+        os_.nsector = len(secs)
+        os_.sector = arr(np.array([[*q.points, q.kz, q.max_x, q.max_y] for q in secs], dtype=np.int32).ravel(),
+                         C.c_int, np.int32)
+        os_.sector_f = arr(np.array([[q.radius, *q.sigma] for q in secs], dtype=np.float32).ravel(),
+                           C.c_float, np.float32)
         os_.nres = len(res)
         os_.res = arr(np.array([r.ijk for r in res], dtype=np.int32).ravel(), C.c_int, np.int32)
         os_.res_R = arr([r.R for r in res], C.c_double, np.float64)

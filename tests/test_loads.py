@@ -1,6 +1,6 @@
-"""Conducting plates, lumped resistors and x-directed lumped sources
-(definePlateX/Y/Z, defineResistor, defineSourceX) and the reference's own
-patch-antenna scene.
+"""Conducting plates and sectors, lumped resistors and x-directed lumped
+sources (definePlateX/Y/Z, defineSec, defineResistor, defineSourceX) and the
+reference's own patch-antenna scene.
 
 CPU part (oracle, FP64), pinned by geometry and physics rather than by the
 reference's formulas:
@@ -8,6 +8,9 @@ reference's formulas:
     edge spans [i, i+1] x {j} x {k}, and so on) -- for each plate axis;
   * a plate of sigma 1e10 is a conductor: the tangential E on it stays
     below 1e-6 of the field around it;
+  * a sector changes exactly the Ex / Ey edges (within its index ranges) of
+    the nodes inside the wedge, found by angle and radius from the apex, and
+    a plate applied after it wins where they overlap;
   * a resistor edge with the port around it obeys I = -V/R minus the gap's
     displacement current, step by step (its update as a current balance);
   * an x-directed source edge obeys the same balance along x, with the
@@ -18,6 +21,8 @@ reference's formulas:
 GPU part: the engine bit for bit against the FP32 oracle on a scene with
 plates, a resistor and a port, and on the patch-antenna scene.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -164,6 +169,82 @@ def test_xsource_alone_drives_only_through_tiny_cezj():
     assert not np.any(o.array("Ex")) and not np.any(o.array("Ey"))
 
 
+SECTOR = dict(points=(31, 21, 20, 18, 23, 31), kz=9, radius=10.3e-3, max_x=28, max_y=27)
+
+
+def _sector_nodes(sc, points, radius, max_i, max_j):
+    """Nodes (i, j) of the wedge by angle: apex (i1, j1), between the rays
+    towards (i0, j0) and (i2, j2) (the arc under pi), within radius metres.
+    Independent of the side tests; asserts no node lies near an edge."""
+    i0, j0, i1, j1, i2, j2 = points
+    a0 = np.arctan2((j0 - j1) * sc.dy, (i0 - i1) * sc.dx)
+    a2 = np.arctan2((j2 - j1) * sc.dy, (i2 - i1) * sc.dx)
+    span = (a2 - a0) % (2 * np.pi)
+    assert 0 < span < np.pi
+    out = set()
+    for j in range(max_j + 1):
+        for i in range(max_i + 1):
+            wx, wy = (i - i1) * sc.dx, (j - j1) * sc.dy
+            r = np.hypot(wx, wy)
+            if r == 0:
+                out.add((i, j))
+                continue
+            t = (np.arctan2(wy, wx) - a0) % (2 * np.pi)
+            assert abs(r - radius) > 1e-6 * radius
+            assert min(abs(t), abs(t - span), abs(t - 2 * np.pi)) > 1e-6 or r > radius
+            if t <= span and r <= radius:
+                out.add((i, j))
+    return out
+
+
+def test_sector_loads_exactly_the_wedge_edges():
+    sc = _scene(n=40, steps=1, src=(8, 8, 5))
+    sc.sectors = [scenes.Sector(sigma=(5.0, 7.0), **SECTOR)]
+    # a plate over part of the sector, applied after it (later wins)
+    sc.plates = [scenes.Plate(axis=3, ijk=(18, 16, 9, 22, 20, 9), sigma=(50.0, 70.0, 0.0))]
+    o = Oracle(sc, precision=64)
+    vac = _scene(n=40, steps=1, src=(8, 8, 5))
+    ov = Oracle(vac, precision=64)
+    n = sc.nx
+    inside = _sector_nodes(sc, SECTOR["points"], SECTOR["radius"], SECTOR["max_x"], SECTOR["max_y"])
+    assert len(inside) > 40
+    kz = SECTOR["kz"]
+    for comp, name, sig, rng in ((0, "Cexe", 5.0, lambda i, j: i < SECTOR["max_x"]),
+                                 (1, "Ceye", 7.0, lambda i, j: j < SECTOR["max_y"])):
+        a = o.array(name).reshape(n, n, n)
+        v = ov.array(name).reshape(n, n, n)
+        changed = {(i, j, k) for k, j, i in zip(*np.nonzero(a != v))}
+        on_plate = {(i, j, kz) for j in range(16, 21) for i in range(18, 23)
+                    if _on_plate(comp, i, j, kz, 3, (18, 16, 9, 22, 20, 9))}
+        expect = {(i, j, kz) for (i, j) in inside if rng(i, j)} | on_plate
+        assert changed == expect, (name, sorted(changed ^ expect)[:6])
+        for (i, j, k) in changed:
+            s_ = (50.0, 70.0)[comp] if (i, j, k) in on_plate else sig
+            ce = (2 * EPS0 - sc.dt * s_) / (2 * EPS0 + sc.dt * s_)
+            assert abs(a[k, j, i] - ce) < 1e-12, (name, i, j, k)
+
+
+def test_calc_index_and_sector_points():
+    # calc_index rounds halves away from zero, as C's round() does
+    assert [scenes.calc_index(v, 0.5) for v in (1.25, -1.25, 0.75, 0.2)] == [3, -3, 2, 0]
+    z, d = (0, 0), (1e-3, 1e-3)
+    # a 60-degree sector of radius 10 cells opening towards -x from the apex
+    assert scenes.sector_points(10e-3, math.pi / 3, 0.0, (0.0, 0.0), 0.0, 0.0, 0.0, 0, z, z, d) == \
+        (-9, -5, 0, 0, -9, 5)
+    # rotate_point turns clockwise: (x, y) -> (y, -x) at 90 degrees
+    assert scenes.sector_points(10e-3, math.pi / 3, math.pi / 2, (0.0, 0.0), 0.0, 0.0, 0.0, 0, z, z, d) == \
+        (-5, 9, 0, 0, 5, 9)
+    # gap 2 cells, apex 1 cell out: every point slides by the 1-cell shortfall,
+    # away from the origin for sectors 0-2 and towards it for 3-5; then the
+    # translation and the air + CPML offsets
+    assert scenes.sector_points(10e-3, math.pi / 3, 0.0, (-1e-3, 0.0), 2e-3, 0.0, 0.0, 1, z, z, d) == \
+        (-12, -5, -2, 0, -12, 5)
+    assert scenes.sector_points(10e-3, math.pi / 3, 0.0, (-1e-3, 0.0), 2e-3, 0.0, 0.0, 4, z, z, d) == \
+        (-10, -5, 0, 0, -10, 5)
+    assert scenes.sector_points(10e-3, math.pi / 3, 0.0, (0.0, 0.0), 0.0, 50e-3, 30e-3, 0, (10, 10), (15, 15),
+                                d) == (-9 + 75, -5 + 55, 75, 55, -9 + 75, 5 + 55)
+
+
 def test_patch_antenna_scene_matches_source_cu():
     sc = scenes.patch_antenna(steps=10)
     # 60 x 100 x 3 cells of object, 15 air + 10 CPML on every side
@@ -268,4 +349,51 @@ def test_xsource_errors():
     with pytest.raises(api.FdtdError) as ei:
         sim.finalize(sc.steps)
     assert "CPML" in str(ei.value)
+    sim.close()
+
+
+@pytest.mark.gpu
+def test_sectors_bitwise_vs_oracle():
+    """Two overlapping sectors (the later wins) and a plate over part of
+    them, against the FP32 oracle bit for bit."""
+    from fdtd_amd import api
+    sc = _scene(n=40, steps=150, src=(14, 12, 9),
+                boxes=[scenes.Box(ijk=(10, 10, 3, 30, 30, 9), eps_r=(2.2, 2.2, 2.2), sigma_e=(4e-4, 4e-4, 4e-4))],
+                probes=[(0, 22, 22, 9), (1, 24, 21, 9), (2, 22, 22, 8), (0, 26, 25, 10), (4, 22, 22, 9),
+                        (1, 19, 19, 9)])
+    sc.sectors = [scenes.Sector(sigma=(1e10, 1e10), **SECTOR),
+                  scenes.Sector(points=(12, 30, 22, 22, 30, 30), kz=9, radius=7.5e-3, max_x=30, max_y=30,
+                                sigma=(3.0, 40.0))]
+    sc.plates = [scenes.Plate(axis=3, ijk=(18, 16, 9, 22, 20, 9), sigma=(50.0, 70.0, 0.0))]
+    print("sector case: 40^3, 2 sectors, 1 plate", flush=True)
+    sim = api.Simulation.from_scene(sc, "f32")
+    sim.run(sc.steps)
+    pr = sim.probes()
+    sim.close()
+    o = Oracle(sc, precision=32)
+    o.steps(sc.steps)
+    po = o.probes().astype(np.float32)
+    assert np.max(np.abs(po)) > 0
+    assert np.array_equal(pr, po), np.max(np.abs(pr - po))
+
+
+@pytest.mark.gpu
+def test_sector_errors():
+    from fdtd_amd import api
+    sc = _scene(n=18, steps=10)
+    sim = api.Simulation.from_scene(sc, "f32", finalize=False)
+    pts = (12, 9, 9, 9, 9, 12)
+    for args, code in (((pts, 0, 5e-3, 10, 10), -3),      # plane k = 0 has no updated Ex / Ey
+                       ((pts, 17, 5e-3, 10, 10), -3),     # plane k = nz-1 neither
+                       ((pts, 8, 5e-3, 18, 10), -3),      # max_x past nx-1
+                       ((pts, 8, 0.0, 10, 10), -1),       # radius must be > 0
+                       ((pts, 8, float("nan"), 10, 10), -1)):
+        with pytest.raises(api.FdtdError) as ei:
+            sim.add_sector(*args, 1e10, 1e10)
+        assert ei.value.code == code, args
+    with pytest.raises(api.FdtdError) as ei:
+        sim.add_sector(pts, 8, 5e-3, 10, 10, -1.0, 1.0)
+    assert ei.value.code == -1
+    sim.add_sector(pts, 8, 5e-3, 10, 10, 1e10, 1e10)
+    sim.finalize(sc.steps)
     sim.close()

@@ -105,6 +105,17 @@ LOAD_MUTANTS = {
                       "s->Ceze[x]  = (two * eps[2][x] * eps_0 - dt * se[2][x] + rfactor) / D;\n"
                       "                    s->Cezhy[x] = (two * dt / dx) / D;\n"
                       "                    s->Cezhx[x] = ((R)-2 * dt / dy) / D;\n                }\n    }"),
+    # sector radius measured from a ray end instead of the apex
+    "sector_radius": ("dist = ((px1 - pontox) * (px1 - pontox)) + ((py1 - pontoy) * (py1 - pontoy));",
+                      "dist = ((px0 - pontox) * (px0 - pontox)) + ((py0 - pontoy) * (py0 - pontoy));"),
+    # sector: the first side test keeps the other half-plane
+    "sector_side": ("rvetorial2 = pvetorial2d(ax, ay, zx1, zy1);\n    rescalar = rvetorial1 * rvetorial2;\n"
+                    "    if (rescalar < 0) return -1;",
+                    "rvetorial2 = pvetorial2d(ax, ay, zx1, zy1);\n    rescalar = rvetorial1 * rvetorial2;\n"
+                    "    if (rescalar > 0) return -1;"),
+    # sector Ey edges stop one column short
+    "sector_ey_range": ("if (i >= 0 && i <= max_x && j >= 0 && j < max_y && kz == k)",
+                        "if (i >= 0 && i < max_x && j >= 0 && j < max_y && kz == k)"),
     # x-source rows bounded by je (the reference bounds them by ie)
     "xsource_rows": ("i >= b[0] && i < b[3] && j >= b[1] && j <= b[3] && k >= b[2] && k <= b[5]",
                      "i >= b[0] && i < b[3] && j >= b[1] && j <= b[4] && k >= b[2] && k <= b[5]"),
