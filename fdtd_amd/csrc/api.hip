@@ -226,6 +226,16 @@ int do_finalize(fdtd_ctx *c)
     const SceneIn &in = c->in;
     Built &b = c->built;
     std::string err;
+    {
+        // The sweep kernels address every array with 32-bit byte offsets:
+        // refuse a field array (with its two guard planes) of 4 GiB or more
+        // before building anything.
+        const size_t fb = (size_t)in.px * in.ny * (in.nz + 2) * (in.storage == FDTD_STORE_F32 ? 4 : 2);
+        if (fb >= ((size_t)1 << 32))
+            return fail(c, FDTD_E_UNSUPPORTED,
+                        "grid too large: each field array (with its two guard planes) must be under 4 GiB "
+                        "(%zu bytes per field)", fb);
+    }
     if (build_scene(in, b, err) != 0) return fail(c, FDTD_E_INVALID, "%s", err.c_str());
     c->e_mode = b.e_uniform ? CLS_UNIFORM : b.mode;
     c->h_mode = b.h_uniform ? CLS_UNIFORM : b.mode;

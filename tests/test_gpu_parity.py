@@ -194,3 +194,19 @@ def test_reset_and_profile_reproduce_run(engine):
     np.testing.assert_array_equal(sim.probes(), ref)
     inf = sim.info()
     assert inf["step_kernel_ms"] > 0
+
+
+@pytest.mark.parametrize("storage,nz", [("f32", 1022), ("f16", 2046)])
+def test_grid_past_the_32bit_offsets_is_refused(storage, nz, engine):
+    """A field array of 4 GiB or more (1024 x 1024 cells a plane, guard
+    planes included) is refused at finalize, before anything is built; one
+    plane less is accepted by the same check (not run: 24 GB of fields)."""
+    sc = scenes.Scene(name="huge", nx=1024, ny=1024, nz=nz, dx=1e-3, dy=1e-3, dz=1e-3,
+                      dt=scenes.courant_dt(1e-3, 1e-3, 1e-3, 0.9), steps=1, cpml=scenes.graded_cpml(8, 1e-3),
+                      sources=[], probes=[], seed=0)
+    print(f"[limits] {storage} 1024x1024x{nz}", flush=True)
+    sim = engine.Simulation.from_scene(sc, storage, finalize=False)
+    with pytest.raises(engine.FdtdError) as ei:
+        sim.finalize(1)
+    assert ei.value.code == -6 and "4 GiB" in str(ei.value)
+    sim.close()
