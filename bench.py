@@ -89,11 +89,11 @@ def cpu_baseline(sc, seconds_target=15.0):
     from oracle.oracle import Oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    o = Oracle(sc, 32, nsteps=64)
+    o = Oracle(sc, 32, nsteps=256)
     t0 = time.perf_counter()
     o.steps(1)
     one = time.perf_counter() - t0
-    n = int(max(1, min(63, seconds_target / max(one, 1e-3))))
+    n = int(max(1, min(255, seconds_target / max(one, 1e-3))))
     t0 = time.perf_counter()
     o.steps(n)
     dt = time.perf_counter() - t0
