@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <math.h>   // sincos (glibc)
 #include <cstring>
 #include <map>
 #include <unordered_map>
@@ -724,9 +725,14 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
     out.phasor.assign((size_t)in.steps_cap * nf * 2, 0.0);
     for (int t = 0; t < in.steps_cap; ++t)
         for (int f = 0; f < nf; ++f) {
-            double th = 2.0 * M_PI * in.freqs[f] * ((double)t * in.dt);
-            out.phasor[((size_t)t * nf + f) * 2 + 0] = std::cos(th) * in.dt;
-            out.phasor[((size_t)t * nf + f) * 2 + 1] = -std::sin(th) * in.dt;
+            // one sincos call (glibc): a compiler may or may not merge
+            // separate cos and sin calls into it, and the two forms can
+            // differ in the last bit
+            double sn, cs;
+            const double th = 2.0 * M_PI * in.freqs[f] * ((double)t * in.dt);
+            ::sincos(th, &sn, &cs);
+            out.phasor[((size_t)t * nf + f) * 2 + 0] = cs * in.dt;
+            out.phasor[((size_t)t * nf + f) * 2 + 1] = -sn * in.dt;
         }
     return 0;
 }

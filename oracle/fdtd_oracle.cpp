@@ -42,6 +42,7 @@
  */
 #include <algorithm>
 #include <cmath>
+#include <math.h>   // sincos (glibc)
 #include <immintrin.h>
 #include <cstdint>
 #include <cstdlib>
@@ -633,9 +634,13 @@ static Sim<R> *build(const oracle_scene *sc)
     s->ph_im.assign((size_t)sc->nsteps * sc->nfreq, 0.0);
     for (int t = 0; t < sc->nsteps; ++t)
         for (int f = 0; f < sc->nfreq; ++f) {
-            double th = 2.0 * M_PI * sc->freqs[f] * ((double)t * sc->dt);
-            s->ph_re[(size_t)t * sc->nfreq + f] = std::cos(th) * sc->dt;
-            s->ph_im[(size_t)t * sc->nfreq + f] = -std::sin(th) * sc->dt;
+            // glibc sincos, called explicitly so the engine's table (built
+            // the same way) matches bit for bit
+            double sn, cs;
+            const double th = 2.0 * M_PI * sc->freqs[f] * ((double)t * sc->dt);
+            ::sincos(th, &sn, &cs);
+            s->ph_re[(size_t)t * sc->nfreq + f] = cs * sc->dt;
+            s->ph_im[(size_t)t * sc->nfreq + f] = -sn * sc->dt;
         }
 
     Z(s->Ex); Z(s->Ey); Z(s->Ez); Z(s->Hx); Z(s->Hy); Z(s->Hz);
