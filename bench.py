@@ -33,10 +33,10 @@ HBM_PEAK_GBS = 8000.0
 # Keyed by (workload, storage, kernel path); re-measure after kernel changes.
 MEASURED_TRAFFIC = {
     # E and H sweeps, each the sum of its interior, x-edge and general launches
-    ("slab512", "f32", "split"): ((600604.8 + 216767.6 + 215335.4) * 2.0 + (483868.2 + 188265.6 + 231352.8) +
-                                  (618467.5 + 196856.8 + 217374.2) * 2.0 + (483840.0 + 188202.5 + 239389.2)) * 1024,
-    ("slab512", "f16", "split"): ((214627.4 + 221536.3 + 112462.6) * 2.0 + (161298.1 + 174775.9 + 115634.5) +
-                                  (193385.1 + 190922.9 + 101756.9) * 2.0 + (161280.0 + 174803.6 + 119699.8)) * 1024,
+    ("slab512", "f32", "split"): ((600573.0 + 216707.7 + 215343.1) * 2.0 + (483868.2 + 188264.3 + 231356.3) +
+                                  (618460.7 + 196852.4 + 217374.5) * 2.0 + (483840.0 + 188206.0 + 239390.0)) * 1024,
+    ("slab512", "f16", "split"): ((214628.7 + 221534.2 + 111830.4) * 2.0 + (161298.1 + 174776.8 + 115636.4) +
+                                  (193349.0 + 190926.0 + 100927.2) * 2.0 + (161280.0 + 174803.8 + 119693.1)) * 1024,
 }
 METRIC = "Mcell-updates/s and % HBM peak; FP16-store vs FP32 at stated probe error"
 

@@ -28,7 +28,8 @@ hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a, c
                       int sten, hipStream_t s);
 hipError_t launch_snap(int storage, const KArgs &a, float *hist, int slot, hipStream_t s);
 hipError_t launch_step(int storage, int mode, const KArgs &a, int m, hipStream_t s);
-hipError_t launch_pair_half_step(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s);
+hipError_t launch_pair_half_step(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s,
+                                 bool gen_cells);
 void step_tiles(int nx, int ny, int &tiles_x, int &tiles_y);
 int tile_rows();
 int tile_cols();
@@ -64,6 +65,7 @@ struct fdtd_ctx {
     int cur = 0;
     bool fused = true;              // one kernel per step (else E and H half-step kernels)
     bool pair = false;              // 16-bit storage: two cells per thread in the half-step kernels
+    bool pair_gen_cells = false;    // ... except the general variant, which runs one cell per thread
     int pair_tiles_x = 0, pair_kchunk = 0;
     int kchunk_h = 0;               // planes per block of the one-cell H sweep (E: ka.kchunk)
     int box[2][6] = {};             // interior tile box per sweep (0 = E, 1 = H)
@@ -261,6 +263,13 @@ int do_finalize(fdtd_ctx *c)
     // FDTD_PAIR=0 selects the one-cell kernels instead (same stored bits)
     const char *pv = std::getenv("FDTD_PAIR");
     c->pair = in.storage != FDTD_STORE_F32 && !(pv && std::strcmp(pv, "0") == 0);
+    // The pair sweeps' general variant (slab rows and z-blocks) runs as the
+    // one-cell kernel (same stored bits): on 512x512x256 FP16 the step took
+    // 0.772 ms against 0.788 with the pair kernel's own general variant,
+    // which holds two cells of slab state per thread.  FDTD_GEN16=pair
+    // selects the latter.
+    const char *gv = std::getenv("FDTD_GEN16");
+    c->pair_gen_cells = !(gv && std::strcmp(gv, "pair") == 0);
     c->pair_tiles_x = (in.nx + pair_tile_cols() - 1) / pair_tile_cols();
     c->pair_kchunk = choose_chunks(c, c->pair_tiles_x * a.tiles_y, 8);
     a.rec = 1;
@@ -432,7 +441,10 @@ hipError_t do_step(fdtd_ctx *c, const KArgs &base, int m, hipEvent_t ev0 = nullp
     a.tiles_y = c->ka.tiles_y;
     a.kchunk = c->pair ? c->pair_kchunk : c->ka.kchunk;
     bind_sets(c, a);
-    auto half = c->pair ? launch_pair_half_step : launch_half_step;
+    auto half = [c](bool efield, int storage, int mode, const KArgs &ka, int mm, hipStream_t st) {
+        return c->pair ? launch_pair_half_step(efield, storage, mode, ka, mm, st, c->pair_gen_cells)
+                       : launch_half_step(efield, storage, mode, ka, mm, st);
+    };
     auto split = [&](int sweep) {
         for (int q = 0; q < 6; ++q) a.ib[q] = c->box[sweep][q];
     };

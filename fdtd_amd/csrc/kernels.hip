@@ -533,6 +533,20 @@ hipError_t launch_half_step(bool efield, int storage, int mode, const KArgs &a, 
     return launch_t<float>(efield, mode, a, m, s);
 }
 
+// The general variant alone (the rows and z-blocks outside the interior box;
+// a in one-cell tile units).  The 16-bit pair sweeps hand it their slab rows
+// and z-blocks: the one-cell kernel holds half the per-thread state there.
+hipError_t launch_half_step_general(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s)
+{
+    int nb, nx, ng;
+    region_tiles(a, nb, nx, ng);
+    if (ng == 0) return hipSuccess;
+    if (storage == 1) launch_v<_Float16, V_GEN>(efield, mode, a, m, ng, s);
+    else if (storage == 2) launch_v<bf16_t, V_GEN>(efield, mode, a, m, ng, s);
+    else launch_v<float, V_GEN>(efield, mode, a, m, ng, s);
+    return hipGetLastError();
+}
+
 int tile_rows() { return BY; }
 int tile_cols() { return TX; }
 

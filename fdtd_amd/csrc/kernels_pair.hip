@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <climits>
 #include <cstdint>
+#include <type_traits>
 
 #include "device_common.h"
 #include "internal.h"
@@ -25,6 +26,8 @@
 namespace fdtd {
 
 constexpr int TXP = 2 * TX;   // cells per tile row
+
+hipError_t launch_half_step_general(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s);
 
 typedef std::uint32_t u32;
 
@@ -624,21 +627,33 @@ static void launch_pair_v(bool efield, int mode, const KArgs &a, int m, int nt, 
 }
 
 template <class T>
-static hipError_t launch_pair_t(bool efield, int mode, const KArgs &a, int m, hipStream_t s)
+static hipError_t launch_pair_t(bool efield, int mode, const KArgs &a, int m, hipStream_t s, bool gen_cells)
 {
     int nb, nx, ng;
     region_tiles(a, nb, nx, ng);
     if (nb > 0) launch_pair_v<T, V_IN>(efield, mode, a, m, nb, s);
     if (nx > 0) launch_pair_v<T, V_X>(efield, mode, a, m, nx, s);
-    if (ng > 0) launch_pair_v<T, V_GEN>(efield, mode, a, m, ng, s);
+    if (ng > 0 && !gen_cells) launch_pair_v<T, V_GEN>(efield, mode, a, m, ng, s);
+    if (ng > 0 && gen_cells) {
+        // the same rows and z-blocks in one-cell tiles (TX columns): the
+        // general region depends on the box's y and z range only
+        KArgs c = a;
+        c.tiles_x = (a.nx + TX - 1) / TX;
+        c.ib[0] = a.ib[0] * (TXP / TX);
+        c.ib[3] = a.ib[3] * (TXP / TX);
+        const hipError_t e = launch_half_step_general(efield, std::is_same<T, bf16_t>::value ? 2 : 1, mode, c, m, s);
+        if (e != hipSuccess) return e;
+    }
     return hipGetLastError();
 }
 
 // storage: 1 FP16, 2 BF16.  a.tiles_x counts 128-cell tile columns.
-hipError_t launch_pair_half_step(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s)
+// gen_cells: the general variant runs as the one-cell kernel.
+hipError_t launch_pair_half_step(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s,
+                                 bool gen_cells)
 {
-    if (storage == 2) return launch_pair_t<bf16_t>(efield, mode, a, m, s);
-    return launch_pair_t<_Float16>(efield, mode, a, m, s);
+    if (storage == 2) return launch_pair_t<bf16_t>(efield, mode, a, m, s, gen_cells);
+    return launch_pair_t<_Float16>(efield, mode, a, m, s, gen_cells);
 }
 
 int pair_tile_cols() { return TXP; }

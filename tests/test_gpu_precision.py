@@ -79,29 +79,38 @@ def _pair_scenes():
     return out
 
 
+# (FDTD_PAIR, FDTD_GEN16): pair sweeps with their own general variant, pair
+# sweeps with the one-cell general variant, one-cell sweeps (last: the
+# reference the others are compared with)
+PAIR_MODES = [("1", "pair"), ("1", "cell"), ("0", "pair")]
+
+
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
 def test_pair_kernels_store_the_one_cell_bits(engine, storage, monkeypatch):
     """The two-cells-per-thread sweeps (default for 16-bit storage) write the
-    same bits as the one-cell sweeps (FDTD_PAIR=0): probes, DFT, fields."""
+    same bits as the one-cell sweeps (FDTD_PAIR=0), also with their general
+    variant run as the one-cell kernel (FDTD_GEN16=cell): probes, DFT, fields."""
     import os
     import sys
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     for name, sc in _pair_scenes().items():
         print(f"[pair] storage={storage} scene={name} grid={sc.nx}x{sc.ny}x{sc.nz} steps={sc.steps}", flush=True)
         res = []
-        for pair in ("1", "0"):
+        for pair, gen in PAIR_MODES:
             monkeypatch.setenv("FDTD_PAIR", pair)
+            monkeypatch.setenv("FDTD_GEN16", gen)
             sim = engine.Simulation.from_scene(sc, storage)
             sim.run(sc.steps)
             res.append((sim.probes(), sim.dft() if sc.freqs else None,
                         [sim.field(c) for c in range(6)]))
             sim.close()
-        (p1, d1, f1), (p0, d0, f0) = res
-        np.testing.assert_array_equal(p1, p0)
-        if d1 is not None:
-            np.testing.assert_array_equal(d1, d0)
-        for c in range(6):
-            np.testing.assert_array_equal(f1[c], f0[c], err_msg=f"{name} component {c}")
+        p0, d0, f0 = res[-1]
+        for (p1, d1, f1), mode in zip(res[:-1], PAIR_MODES):
+            np.testing.assert_array_equal(p1, p0, err_msg=f"{name} {mode}")
+            if d1 is not None:
+                np.testing.assert_array_equal(d1, d0)
+            for c in range(6):
+                np.testing.assert_array_equal(f1[c], f0[c], err_msg=f"{name} {mode} component {c}")
 
 
 def test_pair_kernels_adjoint_snapshots(engine, monkeypatch):
@@ -110,10 +119,12 @@ def test_pair_kernels_adjoint_snapshots(engine, monkeypatch):
     sc = scenes.adjoint(nx=30, ny=26, nz=22, steps=100, L=4, d=1e-3, design=(7, 5, 4), eps_range=(1.0, 4.0),
                         seed=9, probe_freqs=(12e9,), name="adjoint-pair")
     out = []
-    for pair in ("1", "0"):
+    for pair, gen in PAIR_MODES:
         monkeypatch.setenv("FDTD_PAIR", pair)
+        monkeypatch.setenv("FDTD_GEN16", gen)
         sim = engine.Simulation.from_scene(sc, "f16")
         out.append(sim.adjoint_gradient([(0, 0, 1.0)], sc.steps, 32))
         sim.close()
-    assert out[0][0] == out[1][0]
-    np.testing.assert_array_equal(out[0][1], out[1][1])
+    for o in out[:-1]:
+        assert o[0] == out[-1][0]
+        np.testing.assert_array_equal(o[1], out[-1][1])
