@@ -389,7 +389,10 @@ def patch_antenna(steps=3000) -> Scene:
     line and patch as z-plates with sigma 1e10, a 50-ohm z-source of 7 x 3
     edges driven by a Gaussian (nc = 20), the port 10 cells down the feed
     line.  CPML 10 and 15 cells of air on every side.  Constants as the
-    reference's: pi = 3.1415, Courant factor 1, FP32 cell sizes.
+    reference's: pi = 3.1415, Courant factor 1, FP32 cell sizes.  Expressions
+    follow C's arithmetic: float products, promoted to double by a double
+    literal (mu0, the third plate's y maximum) or by sqrt/pow of a float
+    (c, dt, as a C build evaluates them), rounded to float on assignment.
 
     The bounding box of source.cu:256-273 compares every maximum with `<`,
     which would keep the first object's maximum; here the maximum over the
@@ -398,7 +401,7 @@ def patch_antenna(steps=3000) -> Scene:
     f32 = np.float32
     eps0 = f32(8.854187817e-12)
     pi = f32(3.1415)
-    mu0 = f32(f32(4 * pi) * f32(1e-7))
+    mu0 = f32(float(f32(4 * pi)) * 1e-7)          # (4*pi)*1e-7: float times a double literal
     c = f32(1.0 / math.sqrt(float(f32(mu0 * eps0))))
     dx, dy, dz = f32(0.389e-3), f32(0.4e-3), f32(0.265e-3)
     dt = f32(1.0 / (float(c) * math.sqrt(1.0 / float(dx) ** 2 + 1.0 / float(dy) ** 2 + 1.0 / float(dz) ** 2)))
@@ -415,7 +418,7 @@ def patch_antenna(steps=3000) -> Scene:
                 mu_r=(1.0, 1.0, 1.0), sigma_m=(1.2e-38, 1.2e-38, 1.2e-38))
     pec = [((0.0, 0.0, 0.0), (60 * dx, 100 * dy, 0.0)),
            ((7.535e-3, 0.0, 3 * dz), (9.869e-3, 50 * dy, 3 * dz)),
-           ((5.445e-3, 50 * dy, 3 * dz), (17.895e-3, 50 * dy + 16e-3, 3 * dz))]
+           ((5.445e-3, 50 * dy, 3 * dz), (17.895e-3, f32(float(50 * dy) + 16e-3), 3 * dz))]
     plates = [Plate(axis=3, ijk=(o + ci(a[0], dx), o + ci(a[1], dy), o + ci(a[2], dz),
                                  o + ci(b[0], dx), o + ci(b[1], dy), o + ci(b[2], dz)),
                     sigma=(1e10, 1e10, 1e10)) for a, b in pec]

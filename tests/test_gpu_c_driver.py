@@ -1,0 +1,76 @@
+"""The drop-in boundary end to end from C: the reference's own scene
+(source.cu's argument values, tests/c/patch_antenna_main.c) through the
+marchingLoop-compatible shim of INTEGRATION.md and the C ABI on the GPU.
+
+Its port voltage and current must equal, bit for bit, those of the same scene
+built through the Python binding (scenes.patch_antenna, itself bitwise equal
+to the FP32 oracle in tests/test_loads.py), and its results1003.dat /
+results1004.dat must be byte-identical to fdtd_save_port's from that run.
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from fdtd_amd import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+STEPS = 300
+
+
+def build_driver(tmp_path):
+    """gcc: the shim (the one C block of INTEGRATION.md) + the C main."""
+    from fdtd_amd import api
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```c\n(.*?)```", text, flags=re.S)
+    assert len(blocks) == 1
+    shim = tmp_path / "shim.c"
+    shim.write_text(blocks[0])
+    libdir = os.path.dirname(api.LIB_PATH)
+    exe = tmp_path / "patch_antenna"
+    cmd = ["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-Wno-unused-parameter", "-Werror",
+           "-I", os.path.join(ROOT, "include"), str(shim), os.path.join(ROOT, "tests", "c", "patch_antenna_main.c"),
+           "-L", libdir, "-lfdtd_amd", "-Wl,-rpath," + libdir, "-lm", "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def test_c_driver_builds(tmp_path):
+    """CPU: the C main and the shim compile and link against the library."""
+    from fdtd_amd import api
+    api.load_library()
+    assert build_driver(tmp_path).exists()
+
+
+@pytest.mark.gpu
+def test_c_driver_matches_python_binding(engine, tmp_path):
+    exe = build_driver(tmp_path)
+    cdir = tmp_path / "c_run"
+    cdir.mkdir()
+    raw = cdir / "vi.raw"
+    print(f"[c-driver] patch antenna through marchingLoop_mi355x, {STEPS} steps", flush=True)
+    r = subprocess.run([str(exe), str(STEPS), str(raw)], cwd=cdir, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    print(f"[c-driver] {r.stdout.strip()}", flush=True)
+    vi = np.fromfile(raw, dtype=np.float32).reshape(2, STEPS)
+
+    sc = scenes.patch_antenna(steps=STEPS)
+    assert r.stdout.split()[1:4] == [str(sc.nx), str(sc.ny), str(sc.nz)]
+    sim = engine.Simulation.from_scene(sc, "f32")
+    sim.run(STEPS)
+    v, i = sim.port()
+    pdir = tmp_path / "py_run"
+    pdir.mkdir()
+    sim.save_port(str(pdir / "results1003.dat"), str(pdir / "results1004.dat"))
+    sim.close()
+
+    assert np.max(np.abs(v)) > 0 and np.max(np.abs(i)) > 0
+    np.testing.assert_array_equal(vi[0], v)
+    np.testing.assert_array_equal(vi[1], i)
+    for name in ("results1003.dat", "results1004.dat"):
+        c_bytes = (cdir / name).read_bytes()
+        assert c_bytes == (pdir / name).read_bytes(), name
+        assert len(c_bytes.splitlines()) == STEPS
