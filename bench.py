@@ -83,25 +83,49 @@ def run_form(sc, storage, steps, warmup, profile_steps):
                 step_ms=prof["step_kernel_ms"], info=inf, probes=probes, dft=dft)
 
 
-def cpu_baseline(sc, seconds_target=15.0):
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sc, seconds_target=15.0, single_seconds=8.0):
     """The CPU oracle (FP32 restatement of the reference's equations, OpenMP
-    over z-planes) on the same workload, timed over its first few steps."""
-    from oracle.oracle import Oracle
+    over z-planes) on the same workload, timed over its first steps: on the
+    box's share of host cores (16 threads), then on one thread over a few
+    further steps.  Returns (cpu_baseline, host)."""
+    from oracle.oracle import Oracle, load
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
     o = Oracle(sc, 32, nsteps=256)
+    load().oracle_set_threads(threads)
     t0 = time.perf_counter()
     o.steps(1)
     one = time.perf_counter() - t0
-    n = int(max(1, min(255, seconds_target / max(one, 1e-3))))
+    n = int(max(1, min(200, seconds_target / max(one, 1e-3))))
     t0 = time.perf_counter()
     o.steps(n)
     dt = time.perf_counter() - t0
-    o.close()
-    return {"value": sc.cells * n / dt / 1e6, "unit": "Mcell-updates/s", "cores": threads,
+    base = {"value": sc.cells * n / dt / 1e6, "unit": "Mcell-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"oracle/fdtd_oracle.cpp FP32, OpenMP {threads} threads, steps 2-{n + 1} of "
                       f"the full {sc.nx}x{sc.ny}x{sc.nz} {sc.name} grid ({dt:.1f} s)"}
+    # one thread: the per-step rate over the next few steps
+    load().oracle_set_threads(1)
+    n1 = int(max(1, min(255 - 1 - n, single_seconds / max(one * threads, 1e-3))))
+    t0 = time.perf_counter()
+    o.steps(n1)
+    dt1 = time.perf_counter() - t0
+    load().oracle_set_threads(threads)
+    o.close()
+    host = {"cpu_model": _cpu_model(), "nproc": os.cpu_count(), "omp_threads": threads,
+            "single_thread_value": round(sc.cells * n1 / dt1 / 1e6, 2), "single_thread_unit": "Mcell-updates/s",
+            "single_thread_sample": f"1 thread, steps {n + 2}-{n + n1 + 1} of the same grid ({dt1:.1f} s)"}
+    return base, host
 
 
 def main():
@@ -191,7 +215,7 @@ def main():
                        "frac": (round(b16 * cells / (f16["step_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
                                 if f16["step_ms"] > 0 else None)}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sc)
+        out["cpu_baseline"], out["cpu_host"] = cpu_baseline(sc)
     print(json.dumps(out), flush=True)
 
 

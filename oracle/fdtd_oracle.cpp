@@ -44,6 +44,7 @@
 #include <cmath>
 #include <math.h>   // sincos (glibc)
 #include <immintrin.h>
+#include <omp.h>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -1117,6 +1118,15 @@ int oracle_steps(void *hp, int n)
 {
     Handle *h = (Handle *)hp;
     return h->prec == 32 ? steps_t((Sim<float> *)h->sim, n) : steps_t((Sim<double> *)h->sim, n);
+}
+
+/* OpenMP threads of the following oracle calls (n >= 1); returns the
+ * previous setting.  Lets one process time the loop on all cores and on one. */
+int oracle_set_threads(int n)
+{
+    const int prev = omp_get_max_threads();
+    if (n >= 1) omp_set_num_threads(n);
+    return prev;
 }
 
 int oracle_step_count(void *hp)

@@ -54,6 +54,7 @@ def load():
         lib.oracle_destroy.argtypes = [C.c_void_p]
         lib.oracle_steps.argtypes = [C.c_void_p, C.c_int]
         lib.oracle_step_count.argtypes = [C.c_void_p]
+        lib.oracle_set_threads.argtypes = [C.c_int]
         lib.oracle_get_probes.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         lib.oracle_get_dft.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         lib.oracle_get_array.restype = C.c_long
