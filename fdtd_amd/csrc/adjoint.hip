@@ -6,10 +6,10 @@
 //       [finalize of the E half-step of step m+1: psi_e adjoint, dJ/deps on
 //        the design edges, dJ/dE <- Ce dJ/dE]  then the transpose of the H
 //        half-step of step m gathered into dJ/dE, then dJ/d(probe sample m).
-//   adj_src_fixup (one thread) the lumped-source edges' own Ez coefficients.
 //   adj_h_kernel  (H cells, forward differences, like the forward H sweep)
 //       [dJ/dH <- Ch dJ/dH, psi_h adjoint] then the transpose of the E
-//        half-step of step m gathered into dJ/dH.
+//        half-step of step m gathered into dJ/dH, with the lumped-source
+//        edges' own Ez coefficients where the gather meets one.
 // Every transposed term is a gather, so no two threads write one address and
 // no atomics are needed.  Forward operators: fdtdrav.h:665-911 (E) and
 // fdtdrav.h:328-562 (H); edge averaging fdtdrav.h:1135-1177.
@@ -159,6 +159,33 @@ __device__ EQ equant(const KArgs &a, const AArgs &b, bool euni, int i, int j, in
         q.ay += pze.y * (b.pb[9][s] + (uy ? (ey.y * dz) * eby : 0.f));   // psi_eyz
     }
     return q;
+}
+
+// First lumped-source event of column `col` at or after event `ev` (probe
+// events skipped), or -1.  The event list ends every column with a record of
+// another column and ends with a terminator.
+__device__ __forceinline__ int src_from(const KArgs &a, int ev, int col)
+{
+    if (ev < 0) return -1;
+    while (a.events[ev].col == col && a.events[ev].kind != EV_SOURCE) ++ev;
+    return a.events[ev].col == col ? ev : -1;
+}
+
+// equant used the material Ez curl coefficients on every edge; a lumped-
+// source edge updates Ez with its own (defineSourceZ, fdtdrav.h:1402-1420,
+// written after the CPML's kappa division).  The differences times dJ/dEz of
+// source edge `id` at (i, j, k): x = the term of dxHy, y = the term of dyHx.
+template <int CM>
+__device__ float2 src_terms(const KArgs &a, const AArgs &b, bool euni, int i, int j, int k, int id)
+{
+    const i64 c = (i64)k * a.plane + (i64)j * a.px + i;
+    float4 ex, ey, ez;
+    coefs<CM>(a, a.etab, euni, c, ex, ey, ez);
+    const float4 pxe = xs_e(a, i) ? a.px_prof[i] : NOPROF;
+    const float4 pye = ys_e(a, j) ? a.py_prof[j] : NOPROF;
+    const SrcEdge se = a.srcs[id];
+    const float ebz = b.eb[2][c];
+    return make_float2(se.cezhy * ebz - (ez.y / pxe.z) * ebz, se.cezhx * ebz - (ez.z / pye.z) * ebz);
 }
 
 }  // namespace
@@ -341,6 +368,18 @@ __global__ void __launch_bounds__(TX * BY)
     const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
     const EQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     EQ q0 = (SHARE ? valid : own) ? equant<CM, IN>(a, b, euni, i, j, k0) : zq;
+    // lumped-source edges among the Ez edges this thread gathers from: its
+    // own cell, (i+1, j) and (i, j+1); event cursor and its plane per column.
+    // Rows outside the source rows skip the lookups (no load: a kernel
+    // argument range).
+    const bool srow = own && j >= b.src_j0 && j <= b.src_j1;
+    const int scol[3] = {j * nx + i, j * nx + i + 1, (j + 1) * nx + i};
+    const bool sok[3] = {srow, srow && i + 1 < nx, srow && j + 1 < ny};
+    int sev[3], sk[3];
+    for (int q = 0; q < 3; ++q) {
+        sev[q] = sok[q] ? src_from(a, first_event(a, a.e_first, scol[q], k0), scol[q]) : -1;
+        sk[q] = sev[q] >= 0 ? a.events[sev[q]].k : INT_MAX;
+    }
     for (int k = k0; k < k1; ++k) {
         const i64 c = (i64)k * a.plane + (i64)j * px + i;
         const bool khi = IN || k < nz - 1;
@@ -393,37 +432,28 @@ __global__ void __launch_bounds__(TX * BY)
             if (uz && !ux) hbz += q0.ax - qj.ax;
         }
         if (SHARE) __syncthreads();
+        if (k == sk[0] || k == sk[1] || k == sk[2]) {
+            // source edge on this cell: +terms; at (i+1, j) or (i, j+1): -terms
+            for (int q = 0; q < 3; ++q)
+                while (k == sk[q]) {
+                    const float2 d = src_terms<CM>(a, b, euni, q == 1 ? i + 1 : i, q == 2 ? j + 1 : j, k,
+                                                   a.events[sev[q]].id);
+                    if (q == 0) {
+                        if (uy) hby += d.x;
+                        if (ux) hbx += d.y;
+                    } else if (q == 1) {
+                        if (uy) hby -= d.x;
+                    } else {
+                        if (ux) hbx -= d.y;
+                    }
+                    sev[q] = src_from(a, sev[q] + 1, scol[q]);
+                    sk[q] = sev[q] >= 0 ? a.events[sev[q]].k : INT_MAX;
+                }
+        }
         if (ux) b.hb[0][c] = hbx;
         if (uy) b.hb[1][c] = hby;
         if (uz) b.hb[2][c] = hbz;
         q0 = qk;
-    }
-}
-
-// The H-side gather used the material Ez coefficients on every edge; a
-// lumped-source edge updates Ez with its own (defineSourceZ).  One thread
-// adds the difference for every source edge, in order (deterministic).
-template <int CM>
-__global__ void adj_src_fixup(const KArgs a, const AArgs b, const int euni)
-{
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    for (int s = 0; s < b.nsrc_edges; ++s) {
-        const int i = b.src_cell[3 * s], j = b.src_cell[3 * s + 1], k = b.src_cell[3 * s + 2];
-        const i64 c = (i64)k * a.plane + (i64)j * a.px + i;
-        float4 ex, ey, ez;
-        coefs<CM>(a, a.etab, euni, c, ex, ey, ez);
-        const float4 pxe = xs_e(a, i) ? a.px_prof[i] : NOPROF;
-        const float4 pye = ys_e(a, j) ? a.py_prof[j] : NOPROF;
-        const SrcEdge se = a.srcs[s];
-        const float ebz = b.eb[2][c];
-        const float daz = se.cezhy * ebz - (ez.y / pxe.z) * ebz;   // multiplies Hy(x) - Hy(x-i)
-        const float dbz = se.cezhx * ebz - (ez.z / pye.z) * ebz;   // multiplies Hx(x) - Hx(x-j)
-        const bool khi = k < nz - 1;
-        if (i < nx - 1 && khi) b.hb[1][c] += daz;
-        if (i - 1 < nx - 1 && khi) b.hb[1][c - 1] -= daz;
-        if (j < ny - 1 && khi) b.hb[0][c] += dbz;
-        if (j - 1 < ny - 1 && khi) b.hb[0][c - a.px] -= dbz;
     }
 }
 
@@ -470,8 +500,8 @@ void adj_box(const KArgs &a, int tcols, int ib[6])
     for (int q = 0; q < 6; ++q) ib[q] = box[q];
 }
 
-// which: 0 adjoint E sweep, 1 adjoint H sweep, 2 source fix-up.  a0.ib[0] <
-// 0 asks for the general variant alone (no interior split).
+// which: 0 adjoint E sweep, 1 adjoint H sweep.  a0.ib[0] < 0 asks for the
+// general variant alone (no interior split).
 hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, const AArgs &b, int m, int fstep,
                       int sten, hipStream_t s)
 {
@@ -480,12 +510,6 @@ hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, 
     const bool share = adj_shares(a0.nx);
     KArgs a = a0;
     a.tiles_x = share ? (a.nx + TXA - 1) / TXA : (a.nx + TX - 1) / TX;
-    if (which == 2) {
-        if (mode == CLS_U32) hipLaunchKernelGGL((adj_src_fixup<CLS_U32>), dim3(1), dim3(64), 0, s, a, b, euni);
-        else if (mode == CLS_U16) hipLaunchKernelGGL((adj_src_fixup<CLS_U16>), dim3(1), dim3(64), 0, s, a, b, euni);
-        else hipLaunchKernelGGL((adj_src_fixup<CLS_UNIFORM>), dim3(1), dim3(64), 0, s, a, b, 1);
-        return hipGetLastError();
-    }
     if (a0.ib[0] < 0) {
         for (int q = 0; q < 6; ++q) a.ib[q] = 0;
     } else {

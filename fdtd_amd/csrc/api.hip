@@ -76,7 +76,6 @@ struct fdtd_ctx {
     // SI coefficient tables for the adjoint
     const float4 *etab_si = nullptr, *htab_si = nullptr;
     const SrcEdge *srcs_si = nullptr;
-    const int *src_cell = nullptr;
     const float4 *dcoef = nullptr;
     double adj_fwd_ms = 0.0, adj_total_ms = 0.0;
     int adj_segments = 0;
@@ -372,8 +371,6 @@ int upload_tables(fdtd_ctx *c)
     c->htab_si = (const float4 *)tmp;
     if ((rc = upload(c, &tmp, b.srcs_si.data(), b.srcs_si.size() * sizeof(SrcEdge)))) return rc;
     c->srcs_si = (const SrcEdge *)tmp;
-    if ((rc = upload(c, &tmp, b.src_cell.data(), b.src_cell.size() * sizeof(int)))) return rc;
-    c->src_cell = (const int *)tmp;
     c->dcoef = nullptr;
     if (!b.dcoef.empty()) {
         if ((rc = upload(c, &tmp, b.dcoef.data(), b.dcoef.size() * sizeof(float)))) return rc;
@@ -680,8 +677,8 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     b.dcoef = c->dcoef;
     b.gedge = gedge;
     b.sb[0] = d[0]; b.sb[1] = d[1]; b.sb[2] = d[2]; b.sb[3] = bx; b.sb[4] = by; b.sb[5] = bz;
-    b.src_cell = c->src_cell;
-    b.nsrc_edges = (int)c->built.srcs.size();
+    b.src_j0 = c->built.src_j0;
+    b.src_j1 = c->built.src_j1;
 
     int rc = zero_state(c);
     if (rc) return rc;
@@ -764,7 +761,6 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
         for (int m = s1 - 1; m >= s0; --m) {
             HIPCHK(c, launch_adj(0, mode, euni, huni, aa, b, m, m < s1 - 1 ? m + 1 : -1, 1, c->stream));
             HIPCHK(c, launch_adj(1, mode, euni, huni, aa, b, m, -1, 1, c->stream));
-            if (b.nsrc_edges) HIPCHK(c, launch_adj(2, mode, euni, huni, aa, b, m, -1, 1, c->stream));
         }
         HIPCHK(c, launch_adj(0, mode, euni, huni, aa, b, s0, s0, 0, c->stream));
     }

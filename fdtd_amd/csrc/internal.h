@@ -124,8 +124,7 @@ struct AArgs {
     const float4 *dcoef;       // [3][nb] (dCe/deps, 2 eps0/den, Ce, 0); zero off the design
     double *gedge;             // [3][nb] dJ/d(averaged eps) per E edge
     int sb[6];                 // dilated design box (i0, j0, k0, bx, by, bz)
-    const int *src_cell;       // per source edge (i, j, k)
-    int nsrc_edges;
+    int src_j0, src_j1;        // rows j whose row j or j+1 holds a source edge (empty: j0 > j1)
 };
 
 // Host-side scene, filled by the fdtd_* scene calls.
@@ -177,7 +176,7 @@ struct Built {
     std::vector<Event> s_events;
     std::vector<SrcEdge> srcs;
     std::vector<SrcEdge> srcs_si;   // SI coefficients (adjoint)
-    std::vector<int> src_cell;      // per source edge: (i, j, k) triples
+    int src_j0 = 1, src_j1 = 0;     // rows j whose row j or j+1 holds a source edge: [src_j0, src_j1]
     std::vector<float> vpn;         // [wave][steps_cap]
     std::vector<double> phasor;     // [step][nfreq][2]
     std::vector<float> dcoef;       // [3][nb][4] design-edge eps derivatives (adjoint)

@@ -555,7 +555,8 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
     out.events.clear();
     out.srcs.clear();
     out.srcs_si.clear();
-    out.src_cell.clear();
+    out.src_j0 = 1;
+    out.src_j1 = 0;
     std::vector<Event> ev_e, ev_h;
     std::map<i64, int> owner;
     out.vpn.assign(in.sources.size() * (size_t)in.steps_cap, 0.0f);
@@ -596,9 +597,9 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
                         se.cezj = (-2.0f * C.dt / raux) / D;
                     }
                     out.srcs_si.push_back(se);
-                    out.src_cell.push_back(i);
-                    out.src_cell.push_back(j);
-                    out.src_cell.push_back(k);
+                    // the adjoint H gather meets this edge from rows j and j-1
+                    out.src_j0 = std::min(out.src_j0 > out.src_j1 ? j : out.src_j0, std::max(0, j - 1));
+                    out.src_j1 = std::max(out.src_j1, j);
                     if (scaled) {
                         se.cezhy = (float)((double)se.cezhy * sc[2] / sc[4]);
                         se.cezhx = (float)((double)se.cezhx * sc[2] / sc[3]);

@@ -23,10 +23,25 @@ def objective(sim):
     return sum(w * abs(X[p, f]) ** 2 for p, f, w in TERMS)
 
 
-def test_adjoint_matches_fp64_oracle(engine):
+def wide_sources(sc):
+    """The first source widened to 3 x 2 x 2 Ez edges (the transposed source
+    coupling then meets source edges on the cell itself and at i+1, j+1 of
+    its neighbours), plus a single edge inside the x-CPML slab (no kappa on
+    its coefficients)."""
+    s0 = sc.sources[0]
+    i, j, k = s0.ijk[:3]
+    sc.sources[0] = scenes.Source((i, j, k, i + 2, j + 1, k + 2), 50.0, s0.wtype, s0.amp, s0.t0, s0.tau, s0.freq)
+    sc.sources.append(scenes.z_edge(2, sc.ny // 2, sc.nz // 2, sc.dt))
+    return sc
+
+
+@pytest.mark.parametrize("sources", ["single", "wide"])
+def test_adjoint_matches_fp64_oracle(engine, sources):
     from oracle.oracle import Oracle
     sc = small_design()
-    print(f"[adjoint] {sc.nx}x{sc.ny}x{sc.nz} steps={sc.steps}", flush=True)
+    if sources == "wide":
+        sc = wide_sources(sc)
+    print(f"[adjoint] {sc.nx}x{sc.ny}x{sc.nz} steps={sc.steps} sources={sources}", flush=True)
     sim = engine.Simulation.from_scene(sc, "f32")
     J, g = sim.adjoint_gradient(TERMS, sc.steps, 0)
     Jo, go = Oracle(sc, 64).adjoint(TERMS)
