@@ -457,6 +457,18 @@ hipError_t do_step(fdtd_ctx *c, const KArgs &base, int m, hipEvent_t ev0 = nullp
     return e;
 }
 
+// A stream of one call, drained and destroyed on every return path.
+struct StreamGuard {
+    hipStream_t s = nullptr;
+    ~StreamGuard()
+    {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
+};
+
 // The events of one call, destroyed on every return path.
 struct EventSet {
     std::vector<hipEvent_t> v;
@@ -643,7 +655,11 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
         for (int q = 0; q < 12; ++q) t += c->psi_elems[q] * 4;
         return t;
     }();
-    const size_t need = hist_bytes + (size_t)(nseg - 1) * ck_bytes + adj_bytes + (size_t)np * n * 4 + 3 * nb * 8;
+    // segmented runs double-buffer the history: segment s-1 is recomputed on
+    // a second stream while the reverse sweep of segment s runs
+    const int nhist = nseg > 1 ? 2 : 1;
+    const size_t need = nhist * hist_bytes + (size_t)(nseg - 1) * ck_bytes + adj_bytes + (size_t)np * n * 4 +
+                        3 * nb * 8;
     if ((double)need > 0.9 * (double)freeb)
         return fail(c, FDTD_E_NOMEM, "adjoint needs %.2f GB of device memory, %.2f GB free; use a larger "
                     "checkpoint interval", need / 1e9, freeb / 1e9);
@@ -664,12 +680,14 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
         if (!b.pb[q]) return fail(c, FDTD_E_NOMEM, "adjoint psi allocation failed");
         HIPCHK(c, hipMemsetAsync(b.pb[q], 0, std::max<size_t>(c->psi_elems[q], 1) * 4, c->stream));
     }
-    float *hist = (float *)mem.get(hist_bytes);
+    float *hbuf[2] = {(float *)mem.get(hist_bytes), nhist > 1 ? (float *)mem.get(hist_bytes) : nullptr};
+    float *hist = hbuf[0];
     double *gedge = (double *)mem.get(3 * nb * sizeof(double));
     float *gdev = (float *)mem.get((size_t)std::max(np, 1) * n * sizeof(float));
     char *pool = nseg > 1 ? (char *)mem.get((size_t)(nseg - 1) * ck_bytes) : nullptr;
-    if (!hist || !gedge || !gdev || (nseg > 1 && !pool)) return fail(c, FDTD_E_NOMEM, "adjoint buffers");
-    HIPCHK(c, hipMemsetAsync(hist, 0, hist_bytes, c->stream));
+    if (!hist || (nhist > 1 && !hbuf[1]) || !gedge || !gdev || (nseg > 1 && !pool))
+        return fail(c, FDTD_E_NOMEM, "adjoint buffers");
+    for (int q = 0; q < nhist; ++q) HIPCHK(c, hipMemsetAsync(hbuf[q], 0, hist_bytes, c->stream));
     HIPCHK(c, hipMemsetAsync(gedge, 0, 3 * nb * sizeof(double), c->stream));
     b.gsteps = n;
     b.g = gdev;
@@ -705,6 +723,18 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     evs.v.assign(3, nullptr);
     for (auto &e : evs.v) HIPCHK(c, hipEventCreate(&e));
     hipEvent_t t0 = evs.v[0], t1 = evs.v[1], t2 = evs.v[2];
+    // second stream (default priority) for the reverse sweeps of a segmented
+    // run, and the events that order it against the recompute stream:
+    // ready[q] = history buffer q filled, freed[q] = its reverse sweep done
+    StreamGuard rsg;
+    EventSet sync;
+    hipStream_t rs = c->stream;
+    if (nseg > 1) {
+        HIPCHK(c, hipStreamCreateWithFlags(&rsg.s, hipStreamNonBlocking));
+        rs = rsg.s;
+        sync.v.assign(4, nullptr);
+        for (auto &e : sync.v) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     HIPCHK(c, hipEventRecord(t0, c->stream));
     // ---- forward pass: DFT at the probes; checkpoints at segment starts, or
     // the whole design history when there is a single segment
@@ -737,32 +767,50 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     }
     for (size_t q = 0; q < g.size(); ++q) g[q] = (float)gd[q];
     HIPCHK(c, hipMemcpyAsync(gdev, g.data(), g.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    // ---- reverse sweep, segment by segment
-    for (int sgi = nseg - 1; sgi >= 0; --sgi) {
-        const int s0 = sgi * Kseg, s1 = std::min(n, s0 + Kseg);
-        if (nseg > 1) {
-            if (sgi == 0) {
-                for (int f = 0; f < 6; ++f)
-                    HIPCHK(c, hipMemsetAsync(fld(c, f), 0, (size_t)c->field_elems * c->esz, c->stream));
-                for (int q = 0; q < 12; ++q)
-                    if (c->psi_elems[q]) HIPCHK(c, hipMemsetAsync(psv(c, q), 0, c->psi_elems[q] * c->esz, c->stream));
-            } else if ((rc = copy_state(c, pool + (size_t)(sgi - 1) * ck_bytes, false))) {
-                return rc;
-            }
-            KArgs ra = fa;
-            ra.rec = 0;
-            ra.snap = hist;
-            ra.snap_base = s0;
-            bind_sets(c, ra);
-            HIPCHK(c, launch_snap(in.storage, ra, hist, 0, c->stream));
-            for (int m = s0; m < s1; ++m) HIPCHK(c, do_step(c, ra, m));
+    // ---- reverse sweep, segment by segment.  Segmented: the forward state of
+    // segment sgi is recomputed from its checkpoint on c->stream into history
+    // buffer sgi % 2 while the reverse sweep of segment sgi + 1 runs on rs.
+    auto recompute = [&](int sgi) -> int {
+        const int s0 = sgi * Kseg, s1 = std::min(n, s0 + Kseg), q = sgi % 2;
+        if (sgi + 2 < nseg) HIPCHK(c, hipStreamWaitEvent(c->stream, sync.v[2 + q], 0));   // buffer q free
+        if (sgi == 0) {
+            for (int f = 0; f < 6; ++f)
+                HIPCHK(c, hipMemsetAsync(fld(c, f), 0, (size_t)c->field_elems * c->esz, c->stream));
+            for (int p = 0; p < 12; ++p)
+                if (c->psi_elems[p]) HIPCHK(c, hipMemsetAsync(psv(c, p), 0, c->psi_elems[p] * c->esz, c->stream));
+        } else {
+            const int r = copy_state(c, pool + (size_t)(sgi - 1) * ck_bytes, false);
+            if (r) return r;
         }
+        KArgs ra = fa;
+        ra.rec = 0;
+        ra.snap = hbuf[q];
+        ra.snap_base = s0;
+        bind_sets(c, ra);
+        HIPCHK(c, launch_snap(in.storage, ra, hbuf[q], 0, c->stream));
+        for (int m = s0; m < s1; ++m) HIPCHK(c, do_step(c, ra, m));
+        HIPCHK(c, hipEventRecord(sync.v[q], c->stream));   // buffer q ready
+        return FDTD_OK;
+    };
+    if (nseg > 1 && (rc = recompute(nseg - 1))) return rc;
+    for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+        const int s0 = sgi * Kseg, s1 = std::min(n, s0 + Kseg), q = sgi % 2;
+        if (nseg > 1) HIPCHK(c, hipStreamWaitEvent(rs, sync.v[q], 0));
+        b.hist = hbuf[q];
         b.hist_base = s0;
         for (int m = s1 - 1; m >= s0; --m) {
-            HIPCHK(c, launch_adj(0, mode, euni, huni, aa, b, m, m < s1 - 1 ? m + 1 : -1, 1, c->stream));
-            HIPCHK(c, launch_adj(1, mode, euni, huni, aa, b, m, -1, 1, c->stream));
+            HIPCHK(c, launch_adj(0, mode, euni, huni, aa, b, m, m < s1 - 1 ? m + 1 : -1, 1, rs));
+            HIPCHK(c, launch_adj(1, mode, euni, huni, aa, b, m, -1, 1, rs));
         }
-        HIPCHK(c, launch_adj(0, mode, euni, huni, aa, b, s0, s0, 0, c->stream));
+        HIPCHK(c, launch_adj(0, mode, euni, huni, aa, b, s0, s0, 0, rs));
+        if (nseg > 1) {
+            HIPCHK(c, hipEventRecord(sync.v[2 + q], rs));   // buffer q free
+            if (sgi > 0 && (rc = recompute(sgi - 1))) return rc;
+        }
+    }
+    if (nseg > 1) {
+        HIPCHK(c, hipEventRecord(sync.v[2], rs));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, sync.v[2], 0));
     }
     HIPCHK(c, hipEventRecord(t2, c->stream));
     HIPCHK(c, hipEventSynchronize(t2));
