@@ -52,14 +52,20 @@ def test_adjoint_matches_fp64_oracle(engine, sources):
 
 
 def test_checkpointed_equals_stored_history(engine):
+    """Segments are recomputed on one stream while the previous segment's
+    reverse sweep runs on another, through two history buffers: bitwise the
+    stored-history result for uneven, two-segment, one-step-tail and
+    one-step segmentations."""
     sc = small_design(steps=150)
     a = engine.Simulation.from_scene(sc, "f32")
     J0, g0 = a.adjoint_gradient(TERMS, sc.steps, 0)
     assert a.info()["adjoint_segments"] == 1
-    J1, g1 = a.adjoint_gradient(TERMS, sc.steps, 32)
-    assert a.info()["adjoint_segments"] == 5
-    assert J0 == J1
-    np.testing.assert_array_equal(g0, g1)
+    for K, nseg in ((32, 5), (75, 2), (149, 2), (1, 150)):
+        print(f"[adjoint] checkpoint every {K} steps", flush=True)
+        J1, g1 = a.adjoint_gradient(TERMS, sc.steps, K)
+        assert a.info()["adjoint_segments"] == nseg
+        assert J0 == J1
+        np.testing.assert_array_equal(g0, g1)
 
 
 def test_adjoint_vs_central_differences(engine):
