@@ -11,7 +11,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfdtd_amd.so")
-SOURCES = ["api.hip", "kernels.hip", "kernels_pair.hip", "fused.hip", "adjoint.hip", "scene.cpp", "port.cpp"]
+SOURCES = ["api.hip", "kernels.hip", "kernels_pair.hip", "fused.hip", "adjoint.hip", "scene.cpp", "port.cpp",
+           "marching_loop.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -1,7 +1,7 @@
 /* A C main() for the reference's own patch-antenna scene (source.cu:17-325),
- * calling the marchingLoop-compatible shim of INTEGRATION.md with the same
- * argument values, evaluated with C's float/double rules.  Test driver:
- * tests/test_gpu_c_driver.py builds it against the shim and
+ * calling marchingLoop_mi355x (include/fdtd_marching_loop.h, the library's
+ * counterpart of marchingLoop) with the same argument values, evaluated
+ * with C's float/double rules.  tests/test_gpu_c_driver.py builds it against
  * libfdtd_amd.so and compares its output with the Python-API run.
  *
  * Differences from source.cu, each on purpose:
@@ -18,31 +18,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 
-#include "fdtd.h"
-
-int marchingLoop_mi355x(const int NUMDEV, float eps_0, float pi, float mu_0, float c, float dx, float dy, float dz,
-    float dt, int X, int Y, int Z, int n_t_steps,
-    int pml_x_n, int pml_y_n, int pml_z_n, int pml_x_p, int pml_y_p, int pml_z_p,
-    int air_buff_x_n, int air_buff_y_n, int air_buff_z_n, int air_buff_x_p, int air_buff_y_p, int air_buff_z_p,
-    int source_tp, float source_freq, float source_amp,
-    float source_min_x, float source_min_y, float source_min_z,
-    float source_max_x, float source_max_y, float source_max_z,
-    int source_direction, float rs, float nc, float tau, float t_0,
-    float *brick_min_x, float *brick_min_y, float *brick_min_z,
-    float *brick_max_x, float *brick_max_y, float *brick_max_z,
-    float *brick_sigma_e_x, float *brick_sigma_e_y, float *brick_sigma_e_z,
-    float *brick_eps_r_x, float *brick_eps_r_y, float *brick_eps_r_z,
-    float *brick_sigma_m_x, float *brick_sigma_m_y, float *brick_sigma_m_z,
-    float *brick_mu_r_x, float *brick_mu_r_y, float *brick_mu_r_z,
-    int brick_opt, int brick_num,
-    float *pec_min_x, float *pec_min_y, float *pec_min_z, float *pec_max_x, float *pec_max_y, float *pec_max_z,
-    float *pec_sigma_e_x, float *pec_sigma_e_y, float *pec_sigma_e_z, int pec_opt, int pec_num,
-    float resistor_min_x, float resistor_min_y, float resistor_min_z,
-    float resistor_max_x, float resistor_max_y, float resistor_max_z,
-    float resistor_resist, int resistor_direction, int resistor_opt,
-    float v_min_x, float v_min_y, float v_min_z, float v_max_x, float v_max_y, float v_max_z, int voltage_direction,
-    float i_min_x, float i_min_y, float i_min_z, float i_max_x, float i_max_y, float i_max_z, int current_direction,
-    float **probes_out, int *n_probes_out);
+#include "fdtd_marching_loop.h"
 
 int main(int argc, char **argv)
 {
@@ -80,8 +56,8 @@ int main(int argc, char **argv)
     /* bounding box of the objects: the maxima (see the header comment) */
     int X = (int)round(b_max[0] / dx), Y = (int)round(b_max[1] / dy), Z = (int)round(b_max[2] / dz);
 
-    float *vi = NULL;
-    int nvi = 0;
+    float *vi = malloc(sizeof(float) * 2 * (size_t)(n > 0 ? n : 1));
+    if (!vi) return 1;
     int rc = marchingLoop_mi355x(1, eps_0, pi, mu_0, c, dx, dy, dz, dt, X, Y, Z, n,
         10, 10, 10, 10, 10, 10, 15, 15, 15, 15, 15, 15,
         1, 1e8, 1, 7.535e-3, 0, 0, 9.869e-3, 0, 3 * dz, 3, 50, nc, tau, t_0,
@@ -92,13 +68,13 @@ int main(int argc, char **argv)
         7e-3, 0, 0, 8e-3, 2e-3, 4e-3, 50, 1, 0,                      /* resistor_opt = 0 (source.cu:190) */
         7.535e-3, 10 * dy, 0, 9.869e-3, 10 * dy, 3 * dz, 3,
         7.535e-3, 10 * dy, 3 * dz, 9.869e-3, 10 * dy, 3 * dz, 2,
-        &vi, &nvi);
+        vi, vi + n);
     if (rc != FDTD_OK) {
         fprintf(stderr, "marchingLoop_mi355x: %d\n", rc);
         return 1;
     }
     FILE *f = fopen(argv[2], "wb");
-    if (!f || fwrite(vi, sizeof(float), (size_t)nvi * n, f) != (size_t)nvi * n) {
+    if (!f || fwrite(vi, sizeof(float), 2 * (size_t)n, f) != 2 * (size_t)n) {
         fprintf(stderr, "cannot write %s\n", argv[2]);
         return 1;
     }

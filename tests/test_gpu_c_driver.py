@@ -1,6 +1,6 @@
 """The drop-in boundary end to end from C: the reference's own scene
-(source.cu's argument values, tests/c/patch_antenna_main.c) through the
-marchingLoop-compatible shim of INTEGRATION.md and the C ABI on the GPU.
+(source.cu's argument values, tests/c/patch_antenna_main.c) through
+marchingLoop_mi355x, the library's counterpart of marchingLoop, on the GPU.
 
 Its port voltage and current must equal, bit for bit, those of the same scene
 built through the Python binding (scenes.patch_antenna, itself bitwise equal
@@ -8,7 +8,6 @@ to the FP32 oracle in tests/test_loads.py), and its results1003.dat /
 results1004.dat must be byte-identical to fdtd_save_port's from that run.
 """
 import os
-import re
 import subprocess
 
 import numpy as np
@@ -21,17 +20,12 @@ STEPS = 300
 
 
 def build_driver(tmp_path):
-    """gcc: the shim (the one C block of INTEGRATION.md) + the C main."""
+    """gcc: the C main against include/ and libfdtd_amd.so."""
     from fdtd_amd import api
-    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
-    blocks = re.findall(r"```c\n(.*?)```", text, flags=re.S)
-    assert len(blocks) == 1
-    shim = tmp_path / "shim.c"
-    shim.write_text(blocks[0])
     libdir = os.path.dirname(api.LIB_PATH)
     exe = tmp_path / "patch_antenna"
     cmd = ["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-Wno-unused-parameter", "-Werror",
-           "-I", os.path.join(ROOT, "include"), str(shim), os.path.join(ROOT, "tests", "c", "patch_antenna_main.c"),
+           "-I", os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "c", "patch_antenna_main.c"),
            "-L", libdir, "-lfdtd_amd", "-Wl,-rpath," + libdir, "-lm", "-o", str(exe)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
@@ -39,7 +33,7 @@ def build_driver(tmp_path):
 
 
 def test_c_driver_builds(tmp_path):
-    """CPU: the C main and the shim compile and link against the library."""
+    """CPU: the C main compiles and links against the library."""
     from fdtd_amd import api
     api.load_library()
     assert build_driver(tmp_path).exists()

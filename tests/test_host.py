@@ -192,22 +192,29 @@ def _doc_blocks(lang):
     return re.findall(r"```%s\n(.*?)```" % lang, text, flags=re.S)
 
 
+ML_HEADER = os.path.join(ROOT, "include", "fdtd_marching_loop.h")
+
+
 @pytest.mark.parametrize("compiler", [["gcc", "-std=c99", "-x", "c"], ["g++", "-std=c++17", "-x", "c++"]])
-def test_integration_shim_compiles_and_links(compiler, tmp_path, lib):
-    """The marchingLoop shim of INTEGRATION.md is valid C and C++ against
-    include/fdtd.h, and every call it makes resolves in the library."""
-    blocks = _doc_blocks("c")
-    assert len(blocks) == 1
-    src = tmp_path / "shim.c"
-    src.write_text(blocks[0])
+def test_marching_loop_header_compiles_and_links(compiler, tmp_path, lib):
+    """include/fdtd_marching_loop.h is valid C and C++, and the library
+    exports marchingLoop_mi355x with C linkage (a call through the header
+    links with --no-undefined)."""
+    src = tmp_path / "use.c"
+    src.write_text('#include "fdtd_marching_loop.h"\n'
+                   "typedef void (*any_fn)(void);\n"
+                   "any_fn entry(void) { return (any_fn)&marchingLoop_mi355x; }\n")
     from fdtd_amd import api
     libdir = os.path.dirname(api.LIB_PATH)
-    out = tmp_path / "libshim.so"
-    cmd = compiler + ["-Wall", "-Wextra", "-Wno-unused-parameter", "-Werror", "-fPIC", "-shared",
-                      "-I", os.path.join(ROOT, "include"), str(src), "-x", "none", "-L", libdir, "-lfdtd_amd",
-                      "-Wl,--no-undefined", "-Wl,-rpath," + libdir, "-lm", "-o", str(out)]
+    out = tmp_path / "libuse.so"
+    cmd = compiler + ["-Wall", "-Wextra", "-Werror", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
+                      str(src), "-x", "none", "-L", libdir, "-lfdtd_amd", "-Wl,--no-undefined",
+                      "-Wl,-rpath," + libdir, "-o", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+    assert hasattr(lib, "marchingLoop_mi355x")
+    decl = re.findall(r"^int\s+(\w+)\s*\(", open(ML_HEADER).read(), flags=re.M)
+    assert decl == ["marchingLoop_mi355x"]
 
 
 def test_integration_python_examples_parse():
