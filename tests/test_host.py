@@ -1,8 +1,8 @@
 """Host-side checks that need no GPU: the C ABI library loads and exports
-every function include/fdtd.h declares, the Python binding matches the
-header, argument validation in fdtd_create, the seeded scene generator,
-the benchmark's bookkeeping, and the integration examples (the marchingLoop
-shim compiles and links; the Python snippets parse)."""
+every function include/fdtd.h and include/fdtd_marching_loop.h declare, the
+Python binding matches the header, argument validation in fdtd_create, the
+seeded scene generator, the benchmark's bookkeeping, and that the code
+examples of the repository's integration guide are valid (C and Python)."""
 import ctypes as C
 import math
 import os
@@ -217,9 +217,37 @@ def test_marching_loop_header_compiles_and_links(compiler, tmp_path, lib):
     assert decl == ["marchingLoop_mi355x"]
 
 
-def test_integration_python_examples_parse():
+def test_integration_examples_parse():
+    """The guide's Python blocks parse; its C call example compiles against
+    include/fdtd_marching_loop.h inside a function with the named variables."""
     import ast
     blocks = _doc_blocks("python")
     assert len(blocks) >= 3
     for b in blocks:
         ast.parse(b)
+
+
+def test_integration_c_call_example_compiles(tmp_path, lib):
+    """The guide's C call of marchingLoop_mi355x passes every parameter of the
+    header, in order: declared with the header's types, it compiles as C99
+    and links against the library."""
+    (block,) = _doc_blocks("c")
+    hdr = open(ML_HEADER).read()
+    params = re.search(r"int\s+marchingLoop_mi355x\s*\((.*?)\);", hdr, flags=re.S).group(1)
+    types = [" ".join(p.split()[:-1]) + (" *" if "*" in p else "") for p in params.replace("*", " * ").split(",")]
+    types = [t.replace("const ", "").replace(" *", "").strip() + (" *" if t.endswith("*") else "") for t in types]
+    call = re.search(r"marchingLoop_mi355x\((.*?)\);", block, flags=re.S).group(1)
+    call = re.sub(r"/\*.*?\*/", "", call, flags=re.S)
+    args = [a.strip() for a in call.split(",")]
+    assert len(args) == len(types), (len(args), len(types))
+    decls = "".join(f"    {t} {a} = 0;\n" for t, a in zip(types, args) if a != "NULL")
+    src = tmp_path / "example.c"
+    src.write_text('#include <stddef.h>\n#include "fdtd_marching_loop.h"\nint example(void)\n{\n' + decls +
+                   block + "    return rc;\n}\n")
+    from fdtd_amd import api
+    libdir = os.path.dirname(api.LIB_PATH)
+    cmd = ["gcc", "-std=c99", "-Wall", "-Wno-unused-variable", "-Werror=int-conversion",
+           "-Werror=incompatible-pointer-types", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
+           str(src), "-L", libdir, "-lfdtd_amd", "-Wl,--no-undefined", "-o", str(tmp_path / "libexample.so")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
