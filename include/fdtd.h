@@ -20,7 +20,8 @@
  *     during the call; output arrays are caller-allocated.
  *   - Threading: a context is not thread-safe.  Use one host thread per
  *     context.  A context runs on the HIP device that was current when
- *     fdtd_create() was called, on one internal stream.
+ *     fdtd_create() was called, on one internal stream (a checkpointed
+ *     fdtd_adjoint_gradient adds a second one for the call).
  *   - Life cycle: create -> set_cpml / add_box / add_lumped_source /
  *     add_probe / set_dft / set_design_region -> finalize -> run ... ->
  *     get_* -> destroy.  fdtd_reset() rewinds to step 0 with zero fields.
