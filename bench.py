@@ -128,6 +128,16 @@ def cpu_baseline(sc, seconds_target=15.0, single_seconds=8.0):
     return base, host
 
 
+def copy_ceiling_gbs(nbytes=1 << 30, iters=5):
+    """GB/s of the plain HBM copy kernel (non-temporal 16 B per lane, read and
+    written bytes counted), the practical ceiling the sweeps are set against."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "fdtd_amd", "libhbmcopy.so"))
+    lib.hbm_copy_variant_gbs.restype = ctypes.c_double
+    lib.hbm_copy_variant_gbs.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
+    return round(lib.hbm_copy_variant_gbs(nbytes, iters, 5), 1)   # variant 5: u1_nt
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,7 +196,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": _traffic(sc.name, "f32", f32["info"]),
                      "kernel": "step_kernel" if f32["info"]["fused"] else "e_kernel+h_kernel",
-                     "bytes_per_cell": round(bpc, 3), "kernel_ms": round(kms, 5)},
+                     "bytes_per_cell": round(bpc, 3), "kernel_ms": round(kms, 5),
+                     "copy_ceiling": copy_ceiling_gbs()},
         "timed_loop_frac": round(step_gbs / HBM_PEAK_GBS, 4),
         "wall_s": round(f32["wall_s"], 3),
     }
