@@ -25,6 +25,15 @@ MUTANTS = {
     # H update drops the Ez term of Hx
     "dropped_term": ("Hx[x] = (s->Chxh[x] * Hx[x] + s->Chxey[x] * (Ey[x + NXY] - Ey[x])) + s->Chxez[x] * (Ez[x + NX] - Ez[x]);",
                      "Hx[x] = (s->Chxh[x] * Hx[x] + s->Chxey[x] * (Ey[x + NXY] - Ey[x]));"),
+    # CPML: kappa dropped from a's denominator (sigma + alpha for sigma + kappa alpha)
+    "cpml_a_kappa": ("double dne = kpe * (sge + kpe * ale)", "double dne = kpe * (sge + ale)"),
+    # CPML: kappa dropped from b's exponent (sigma for sigma / kappa)
+    "cpml_b_kappa": ("std::exp((-ddt / e0) * ((sge / kpe) + ale))", "std::exp((-ddt / e0) * ((sge) + ale))"),
+    # CPML: alpha graded the wrong way (largest at the outer face)
+    "cpml_alpha_grading": ("double ale = amin + (amax - amin) * (1.0 - re);",
+                           "double ale = amin + (amax - amin) * (re);"),
+    # CPML: the kappa division of Ceyhz left out
+    "cpml_kappa_division": ("s->Ceyhz[x] = s->Ceyhz[x] / kex[i];", "s->Ceyhz[x] = s->Ceyhz[x];"),
 }
 
 

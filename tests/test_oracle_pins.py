@@ -15,8 +15,13 @@ fails at least one check:
   * lumped source: the resistive voltage-source update (Elsherbeni & Demir,
     eq. 4.6-4.9);
   * CPML: reduction to the plain exponential integrator (kappa = 1,
-    alpha = 0), grading monotonicity, and a reflection bound against a
-    domain large enough that nothing comes back;
+    alpha = 0); kappa alone equals a grid stretched by kappa along the
+    slab's axis; the per-cell recursion converges (first order in dt) to
+    1/s(w) of the complex-frequency-shifted stretch s = kappa +
+    sigma/(alpha + j w eps0) (Roden & Gedney 2000); sigma, kappa and alpha
+    recovered from the arrays grade in the CFS-PML's directions; grading
+    monotonicity; a reflection bound against a domain large enough that
+    nothing comes back;
   * pulse: the Gaussian derivative's closed-form peak and odd symmetry;
   * DFT: the accumulated transform against a direct FP64 sum with
     t_m = m dt;
@@ -168,6 +173,157 @@ def test_cpml_profile_grading():
         # loss grows towards each outer face, for E and for H nodes
         assert np.all(np.diff(be[lo]) > 0) and np.all(np.diff(be[hi]) < 0)
         assert np.all(np.diff(bm[lo]) > 0) and np.all(np.diff(bm[hi][:-1]) < 0)
+
+
+def _init_fields(o, sc, seed):
+    """Seeded random values in all six fields (an initial state instead of a
+    source, so no lumped-source loading enters the comparison)."""
+    rng = np.random.default_rng(seed)
+    for name in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        scale = 1.0 if name[0] == "E" else 1.0 / scenes.ETA0
+        o.set_array(name, scale * rng.standard_normal(sc.cells))
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_cpml_kappa_alone_is_a_coordinate_stretch(axis):
+    """With sigma = 0 and alpha = 0 the CPML's complex stretch
+    s = kappa + sigma/(alpha + j w eps0) reduces to the real constant kappa
+    (Roden & Gedney 2000): a slab spanning the whole extent of one axis, with
+    a flat profile (grading order 0, so kappa = kappa_max on every cell), is
+    the same grid with that axis' cell size multiplied by kappa.  Both runs
+    start from the same random fields in a lossy, anisotropic, magnetic box;
+    every field must agree to FP64 round-off after 40 steps.  This pins the
+    kappa division of all four curl coefficients of the axis, E and H side."""
+    kappa = 3.5
+    nx, ny, nz = 15, 13, 11
+    d = [1.0e-3, 1.2e-3, 0.9e-3]
+    dt = scenes.courant_dt(*d, 0.99)
+    box = Box((4, 3, 3, 10, 9, 8), (3.0, 2.0, 5.0), (0.02, 0.01, 0.03), (2.0, 1.5, 3.0), (10.0, 0.0, 5.0))
+    thick = [0] * 6
+    thick[2 * axis] = (nx, ny, nz)[axis]
+    # sigma_max <= 0 selects the reference's default, so "no loss" is 1e-300
+    stretched = Cpml(thick=tuple(thick), order=0.0, sigma_max=(1e-300,) * 3, kappa_max=kappa,
+                     alpha_max=0.0, alpha_min=0.0)
+    dd = list(d)
+    dd[axis] *= kappa
+    runs = []
+    for cells, cp in ((d, stretched), (dd, NO_CPML)):
+        sc = Scene(name="t", nx=nx, ny=ny, nz=nz, dx=cells[0], dy=cells[1], dz=cells[2], dt=dt, steps=40,
+                   cpml=cp, boxes=[box])
+        o = Oracle(sc, 64)
+        _init_fields(o, sc, 7)
+        o.steps(40)
+        runs.append({n: o.array(n) for n in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")})
+        o.close()
+    for n, want in runs[1].items():
+        got = runs[0][n]
+        assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max(), (axis, n)
+
+
+# (curl coefficient, its cpsi, the axis' 1-D profile a/b names, sign of the
+# vacuum coefficient, axis) for the 12 stretched derivatives of setABCPML
+_STRETCHED = [
+    ("Ceyhz", "cpsi_eyx", "ex", -1, 0), ("Cezhy", "cpsi_ezx", "ex", +1, 0),
+    ("Cexhz", "cpsi_exy", "ey", +1, 1), ("Cezhx", "cpsi_ezy", "ey", -1, 1),
+    ("Cexhy", "cpsi_exz", "ez", -1, 2), ("Ceyhx", "cpsi_eyz", "ez", +1, 2),
+    ("Chyez", "cpsi_hyx", "mx", +1, 0), ("Chzey", "cpsi_hzx", "mx", -1, 0),
+    ("Chxez", "cpsi_hxy", "my", -1, 1), ("Chzex", "cpsi_hzy", "my", +1, 1),
+    ("Chxey", "cpsi_hxz", "mz", +1, 2), ("Chyex", "cpsi_hyz", "mz", -1, 2),
+]
+
+
+def _stretch_error(sigma, kappa, alpha, dt_factor):
+    """Largest |G(w) s(w) / C0 - 1| over the 12 stretched derivatives and
+    four frequencies, where G(w) = C + cpsi a / (1 - b e^{-j w dt}) is the
+    per-cell transfer function of the recursion psi^n = b psi^{n-1} + a D^n
+    applied as C D + cpsi psi, C0 the unstretched (vacuum) coefficient, and
+    s(w) = kappa + sigma / (alpha + j w eps0) the CFS stretch (Roden &
+    Gedney 2000).  The profile is flat (order 0, alpha_max = alpha_min), so
+    every slab cell carries exactly (sigma, kappa, alpha)."""
+    n, L = 20, 6
+    d = (1.0e-3, 1.3e-3, 0.8e-3)
+    dt = scenes.courant_dt(*d, 0.99) * dt_factor
+    cp = Cpml(thick=(L,) * 6, order=0.0, sigma_max=(sigma,) * 3, kappa_max=kappa, alpha_max=alpha,
+              alpha_min=alpha)
+    sc = Scene(name="t", nx=n, ny=n, nz=n, dx=d[0], dy=d[1], dz=d[2], dt=dt, steps=1, cpml=cp)
+    o = Oracle(sc, 64)
+    worst = 0.0
+    for cname, pname, prof, sign, axis in _STRETCHED:
+        cell = [n // 2] * 3
+        cell[axis] = 2                          # inside this axis' slab only
+        x = idx(sc, *cell)
+        mat = sc.eps0 if cname[1] == "e" else sc.mu0
+        C0 = sign * dt / (mat * d[axis])
+        C, cpsi = o.array(cname)[x], o.array(pname)[x]
+        a, b = o.array("a_" + prof)[cell[axis]], o.array("b_" + prof)[cell[axis]]
+        for f in (0.3e9, 1e9, 3e9, 10e9):
+            w = 2 * math.pi * f
+            G = C + cpsi * a / (1.0 - b * np.exp(-1j * w * dt))
+            s = kappa + sigma / (alpha + 1j * w * sc.eps0)
+            worst = max(worst, abs(G * s / C0 - 1.0))
+    o.close()
+    return worst
+
+
+@pytest.mark.parametrize("sigma,kappa,alpha", [(8.5, 5.0, 0.05), (3.0, 2.0, 0.5), (20.0, 8.0, 0.0)])
+def test_cpml_recursion_converges_to_cfs_stretch(sigma, kappa, alpha):
+    """The discrete CPML (b, a, cpsi, C/kappa) realises 1/s(w) of the
+    complex-frequency-shifted stretch to first order in dt: the error is
+    small and halves when dt halves.  A kappa or alpha misplaced in a or b,
+    or a missing kappa division, leaves an O(1) error that does not
+    converge."""
+    e1 = _stretch_error(sigma, kappa, alpha, 0.1)
+    e2 = _stretch_error(sigma, kappa, alpha, 0.05)
+    assert e1 < 3e-2, e1
+    assert 0.4 < e2 / e1 < 0.6, (e1, e2)
+
+
+def test_cpml_grading_directions():
+    """The recovered per-cell (sigma, kappa, alpha) of the graded CPML follow
+    the CFS-PML design: sigma and kappa grow from the interface towards the
+    outer face (kappa from about 1 to about kappa_max), alpha falls from about
+    alpha_max at the interface to about alpha_min at the outer face, on both
+    sides of every axis, for E and H nodes.  Values are recovered from the
+    oracle's own arrays: kappa = cpsi / (C d), sigma/kappa + alpha =
+    -ln(b) eps0/dt, and sigma = a d kappa^2 (sigma/kappa + alpha) / (b - 1)
+    (the relations the convergence pin above fixes)."""
+    L, n = 10, 30
+    d = (1.0e-3, 1.3e-3, 0.8e-3)
+    amax = 0.05
+    cp = scenes.graded_cpml(L, d[0])
+    sc = Scene(name="t", nx=n, ny=n, nz=n, dx=d[0], dy=d[1], dz=d[2], dt=scenes.courant_dt(*d, 0.99), steps=1,
+               cpml=cp)
+    o = Oracle(sc, 64)
+    for cname, pname, prof, sign, axis in _STRETCHED:
+        mat = sc.eps0 if cname[1] == "e" else sc.mu0
+        C, cpsi = o.array(cname), o.array(pname)
+        a, b = o.array("a_" + prof), o.array("b_" + prof)
+        for side in ("low", "high"):
+            # slab cells ordered from the interface to the outer face
+            order = list(range(L - 1, -1, -1)) if side == "low" else list(range(n - L, n))
+            # nodes on the outer walls take no update (tangential E at c = 0 and
+            # n-1, H at n-1: the ranges of H:249-573 / H:577-915), so no coefficient
+            order = [c for c in order if c < n - 1 and (c > 0 or cname[1] == "h")]
+            sig, kap, alp = [], [], []
+            for c in order:
+                cell = [n // 2] * 3
+                cell[axis] = c
+                x = idx(sc, *cell)
+                k = cpsi[x] / (C[x] * d[axis])
+                g = -math.log(b[c]) * mat / sc.dt
+                s = a[c] * d[axis] * k * k * g / (b[c] - 1.0)
+                scale = sc.eps0 / mat                 # H-node sigma_m, alpha_m carry mu0/eps0
+                sig.append(s * scale)
+                kap.append(k)
+                alp.append((g - s / k) * scale)
+            sig, kap, alp = map(np.array, (sig, kap, alp))
+            where = (cname, side)
+            assert np.all(np.diff(sig) > 0) and np.all(np.diff(kap) > 0), where
+            assert np.all(np.diff(alp) < 0), where
+            assert 1.0 <= kap[0] < 1.01 and kap[-1] > 0.5 * cp.kappa_max, where
+            assert alp[0] > 0.9 * amax and 0.0 <= alp[-1] < 0.25 * amax, where
+            assert sig[0] < 0.01 * cp.sigma_max[axis] and sig[-1] > 0.4 * cp.sigma_max[axis], where
+    o.close()
 
 
 def _reflection(L, cpml=True, T=110):
