@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -872,8 +873,12 @@ int fdtd_create(const fdtd_grid *g, int storage, fdtd_ctx **out)
     if (!(dt > 0)) {
         if (!(g->courant > 0) || g->courant > 1.0) return FDTD_E_INVALID;
         dt = g->courant * cfl;
-    } else if (dt > cfl * (1.0 + 1e-12)) {
-        return FDTD_E_INVALID;   // above the 3-D Courant limit: unstable
+    } else if (dt > cfl * (1.0 + 4.0 * FLT_EPSILON)) {
+        // above the 3-D Courant limit: unstable.  A dt computed in float at
+        // Courant factor 1 (source.cu:45-46 built as C++, where sqrt of a
+        // float is a float) lands a few float ulps either side of the double
+        // limit, so the test allows float resolution.
+        return FDTD_E_INVALID;
     }
     fdtd_ctx *c = new (std::nothrow) fdtd_ctx();
     if (!c) return FDTD_E_NOMEM;

@@ -55,12 +55,12 @@ extern "C" int marchingLoop_mi355x(const int NUMDEV, float eps_0, float pi, floa
     float *v_out, float *i_out)
 {
     (void)NUMDEV;
-    (void)pi;
     (void)c;
     (void)nc;
     (void)resistor_direction;
     fdtd_ctx *ctx = nullptr;
     if (source_direction != 1 && source_direction != 3) return FDTD_E_UNSUPPORTED;
+    if (source_tp != 1 && source_tp != 3) return FDTD_E_UNSUPPORTED;   // case 2 fills nothing, fdtdrav.h:4077-4079
     const int ox = air_buff_x_n + pml_x_n, oy = air_buff_y_n + pml_y_n, oz = air_buff_z_n + pml_z_n;
     fdtd_grid g;
     std::memset(&g, 0, sizeof g);
@@ -76,9 +76,15 @@ extern "C" int marchingLoop_mi355x(const int NUMDEV, float eps_0, float pi, floa
     int rc = fdtd_create(&g, FDTD_STORE_F32, &ctx);
     if (rc != FDTD_OK) return failed(nullptr, "fdtd_create", rc);
 
-    // setABCPML's constants (fdtdrav.h:1774-1812): m = 3, sigma = 1.5 sigma_opt, kappa 5, alpha 0.05 -> 0
-    const fdtd_cpml cp = {{pml_x_n, pml_x_p, pml_y_n, pml_y_p, pml_z_n, pml_z_p}, 3.0, {0.0, 0.0, 0.0}, 1.5, 5.0,
-                          0.05, 0.0};
+    // setABCPML's constants (fdtdrav.h:1774-1812): m = 3, kappa 5, alpha 0.05 -> 0, and
+    // sigma_max = 1.5 sigma_opt with sigma_opt = (m + 1) / (150 pi d), float
+    // arithmetic with the caller's pi (fdtdrav.h:1793-1808)
+    const float npml = 3, sigma_factor = 1.5f;
+    const float smax[3] = {sigma_factor * ((npml + 1) / (150 * pi * 1 * dx)),
+                           sigma_factor * ((npml + 1) / (150 * pi * 1 * dy)),
+                           sigma_factor * ((npml + 1) / (150 * pi * 1 * dz))};
+    const fdtd_cpml cp = {{pml_x_n, pml_x_p, pml_y_n, pml_y_p, pml_z_n, pml_z_p}, 3.0,
+                          {smax[0], smax[1], smax[2]}, 1.5, 5.0, 0.05, 0.0};
     ML_CHECK(fdtd_set_cpml(ctx, &cp));
 
     for (int b = 0; brick_opt && b < brick_num; ++b) {   // setBrick loop, fdtdrav.h:4104-4128
@@ -101,7 +107,10 @@ extern "C" int marchingLoop_mi355x(const int NUMDEV, float eps_0, float pi, floa
                                    oz + cidx(resistor_min_z, dz), ox + cidx(resistor_max_x, dx),
                                    oy + cidx(resistor_max_y, dy), oz + cidx(resistor_max_z, dz), resistor_resist));
 
-    const fdtd_waveform w = {source_tp, source_amp, t_0, tau, source_freq};   // setGauss / setSine
+    // setGauss / setSine; the library's sine is sin(2 pi_lib f t), so the
+    // frequency carries the caller's pi: sin(2 pi f dt m), fdtdrav.h:1515
+    const double f_eff = (double)source_freq * ((double)pi / M_PI);
+    const fdtd_waveform w = {source_tp, source_amp, t_0, tau, f_eff};
     ML_CHECK(fdtd_add_lumped_source(ctx, source_direction, ox + cidx(source_min_x, dx),
                                     oy + cidx(source_min_y, dy), oz + cidx(source_min_z, dz),
                                     ox + cidx(source_max_x, dx), oy + cidx(source_max_y, dy),

@@ -383,13 +383,23 @@ def adjoint768(**kw) -> Scene:
 
 
 
+def reference_sigma_max(pi, d, order=3.0, sigma_factor=1.5):
+    """setABCPML's sigma_max for one axis in float arithmetic, as the
+    reference evaluates it with its own pi (fdtdrav.h:1793-1808):
+    sigma_factor * (npml + 1) / (150 * pi * 1 * d)."""
+    f32 = np.float32
+    sigma_opt = f32(f32(order) + f32(1)) / f32(f32(f32(f32(150) * f32(pi)) * f32(1)) * f32(d))
+    return float(f32(f32(sigma_factor) * f32(sigma_opt)))
+
+
 def patch_antenna(steps=3000) -> Scene:
     """The reference's own scene (source.cu:17-226): a microstrip-fed patch on
     a 60 x 100 x 3 cell substrate (eps_r 2.2, sigma 4e-4), ground plane, feed
     line and patch as z-plates with sigma 1e10, a 50-ohm z-source of 7 x 3
     edges driven by a Gaussian (nc = 20), the port 10 cells down the feed
     line.  CPML 10 and 15 cells of air on every side.  Constants as the
-    reference's: pi = 3.1415, Courant factor 1, FP32 cell sizes.  Expressions
+    reference's: pi = 3.1415 (also in the CPML's sigma_max, evaluated in
+    float), Courant factor 1, FP32 cell sizes.  Expressions
     follow C's arithmetic: float products, promoted to double by a double
     literal (mu0, the third plate's y maximum) or by sqrt/pow of a float
     (c, dt, as a C build evaluates them), rounded to float on assignment.
@@ -430,8 +440,8 @@ def patch_antenna(steps=3000) -> Scene:
             o + ci(9.869e-3, dx), o + ci(10 * dy, dy), o + ci(3 * dz, dz))
     ibox = (o + ci(7.535e-3, dx), o + ci(10 * dy, dy), o + ci(3 * dz, dz),
             o + ci(9.869e-3, dx), o + ci(10 * dy, dy), o + ci(3 * dz, dz))
-    cp = Cpml(thick=(pml,) * 6, order=3.0, sigma_max=(0.0, 0.0, 0.0), kappa_max=5.0, alpha_max=0.05,
-              alpha_min=0.0, sigma_factor=1.5)
+    cp = Cpml(thick=(pml,) * 6, order=3.0, sigma_max=tuple(reference_sigma_max(pi, d) for d in (dx, dy, dz)),
+              kappa_max=5.0, alpha_max=0.05, alpha_min=0.0, sigma_factor=1.5)
     return Scene(name="patch_antenna", nx=nx, ny=ny, nz=nz, dx=float(dx), dy=float(dy), dz=float(dz),
                  dt=float(dt), steps=steps, cpml=cp, boxes=[brick], sources=[src], plates=plates,
                  port=Port(vbox=vbox, ibox=ibox, idir=2), eps0=float(eps0), mu0=float(mu0), seed=0)

@@ -77,7 +77,9 @@ typedef struct fdtd_ctx fdtd_ctx;
 typedef struct {
     int    nx, ny, nz;      /* cells per axis, CPML included                 */
     double dx, dy, dz;      /* cell sizes [m]                                */
-    double dt;              /* time step [s]; <= 0 selects courant * 3-D CFL  */
+    double dt;              /* time step [s]; <= 0 selects courant * 3-D CFL;
+                               may exceed the CFL limit by float resolution
+                               (4 FLT_EPSILON), no more                       */
     double courant;         /* Courant factor used when dt <= 0 (source.cu:41-46) */
     double eps0, mu0;       /* <= 0 selects 8.854187817e-12 and 4*pi*1e-7     */
 } fdtd_grid;

@@ -10,11 +10,19 @@
  * Differences a caller sees:
  *  - it returns FDTD_OK or a negative FDTD_E_* code (the reference calls
  *    exit() on errors and returns nothing);
- *  - NUMDEV is ignored (one device); pi and c are implied by eps_0 and mu_0,
- *    which are used as given (the reference's pi = 3.1415 travels in mu_0);
+ *  - NUMDEV is ignored (one device); c is implied by eps_0 and mu_0, which
+ *    are used as given; pi enters where the reference uses it: the CPML's
+ *    sigma_max = 1.5 (npml + 1) / (150 pi d) per axis, evaluated in float
+ *    (fdtdrav.h:1793-1808), and the sine source sin(2 pi f dt m)
+ *    (fdtdrav.h:1505-1520);
  *  - source_direction 1 (defineSourceX) and 3 (defineSourceZ) are supported,
  *    other values return FDTD_E_UNSUPPORTED; voltage_direction must be 3 (the
  *    reference samples only Ez, fdtdrav.h:740);
+ *  - source_tp 1 (setGauss) and 3 (setSine) are supported; source_tp 2
+ *    returns FDTD_E_UNSUPPORTED: the reference's switch leaves its signal
+ *    buffer unwritten for it (fdtdrav.h:4070-4085), so there is no waveform
+ *    to reproduce (the Gaussian derivative of fdtd_add_lumped_source's
+ *    waveform type 2 is this library's own);
  *  - the resistor is added only when resistor_opt != 0: the reference runs
  *    defineResistor whenever resistor_direction != 0 but computes its indices
  *    only when resistor_opt != 0 (fdtdrav.h:3575-3587, 4296);

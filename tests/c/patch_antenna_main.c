@@ -13,7 +13,10 @@
  *    as in source.cu;
  *  - V[m] and I[m] are also written raw (float32) to argv[2] for a bitwise
  *    comparison; results1003.dat / results1004.dat go to the working
- *    directory as the reference writes them. */
+ *    directory as the reference writes them.
+ * The file is valid C and C++.  Built as C, sqrt of a float promotes to
+ * double; built as C++ (as nvcc builds source.cu) it is the float overload,
+ * so c and dt differ in the last float bits between the two builds. */
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -44,7 +47,7 @@ int main(int argc, char **argv)
     /* ground plane, feed line, patch (source.cu:105-151) */
     float p_min_x[3] = {0, 7.535e-3, 5.445e-3}, p_min_y[3] = {0, 0, 50 * dy}, p_min_z[3] = {0, 3 * dz, 3 * dz};
     float p_max_x[3] = {60 * dx, 9.869e-3, 17.895e-3};
-    float p_max_y[3] = {100 * dy, 50 * dy, (50 * dy) + (16e-3)};
+    float p_max_y[3] = {100 * dy, 50 * dy, (float)((50 * dy) + (16e-3))};
     float p_max_z[3] = {0, 3 * dz, 3 * dz};
     float p_sig[3] = {1e10, 1e10, 1e10};
 
@@ -56,7 +59,7 @@ int main(int argc, char **argv)
     /* bounding box of the objects: the maxima (see the header comment) */
     int X = (int)round(b_max[0] / dx), Y = (int)round(b_max[1] / dy), Z = (int)round(b_max[2] / dz);
 
-    float *vi = malloc(sizeof(float) * 2 * (size_t)(n > 0 ? n : 1));
+    float *vi = (float *)malloc(sizeof(float) * 2 * (size_t)(n > 0 ? n : 1));
     if (!vi) return 1;
     int rc = marchingLoop_mi355x(1, eps_0, pi, mu_0, c, dx, dy, dz, dt, X, Y, Z, n,
         10, 10, 10, 10, 10, 10, 15, 15, 15, 15, 15, 15,

@@ -19,24 +19,44 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 STEPS = 300
 
 
-def build_driver(tmp_path):
-    """gcc: the C main against include/ and libfdtd_amd.so."""
+COMPILERS = {"c": ["gcc", "-std=c99", "-x", "c"], "c++": ["g++", "-std=c++17", "-x", "c++"]}
+
+
+def build_driver(tmp_path, lang="c"):
+    """The C main against include/ and libfdtd_amd.so, built as C (gcc) or
+    as C++ (g++, the language nvcc compiles source.cu in)."""
     from fdtd_amd import api
     libdir = os.path.dirname(api.LIB_PATH)
-    exe = tmp_path / "patch_antenna"
-    cmd = ["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-Wno-unused-parameter", "-Werror",
-           "-I", os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "c", "patch_antenna_main.c"),
-           "-L", libdir, "-lfdtd_amd", "-Wl,-rpath," + libdir, "-lm", "-o", str(exe)]
+    exe = tmp_path / ("patch_antenna_" + lang.replace("+", "p"))
+    cmd = COMPILERS[lang] + ["-O2", "-Wall", "-Wextra", "-Wno-unused-parameter", "-Werror",
+                             "-I", os.path.join(ROOT, "include"),
+                             os.path.join(ROOT, "tests", "c", "patch_antenna_main.c"), "-x", "none",
+                             "-L", libdir, "-lfdtd_amd", "-Wl,-rpath," + libdir, "-lm", "-o", str(exe)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     return exe
 
 
-def test_c_driver_builds(tmp_path):
-    """CPU: the C main compiles and links against the library."""
+@pytest.mark.parametrize("lang", sorted(COMPILERS))
+def test_c_driver_builds(tmp_path, lang):
+    """CPU: the C main compiles and links against the library as C and C++."""
     from fdtd_amd import api
     api.load_library()
-    assert build_driver(tmp_path).exists()
+    assert build_driver(tmp_path, lang).exists()
+
+
+@pytest.mark.parametrize("lang", sorted(COMPILERS))
+def test_driver_dt_passes_the_courant_check(tmp_path, lang):
+    """source.cu's dt at Courant factor 1, evaluated by C's or C++'s rules
+    (C++ takes the float sqrt for c, which puts dt a few float ulps above the
+    double limit), passes fdtd_create's stability check.  Run for one step:
+    with a device it completes; on a host without one, fdtd_create gets past
+    validation and stops at the device (FDTD_E_DEVICE = -5, not
+    FDTD_E_INVALID = -1)."""
+    exe = build_driver(tmp_path, lang)
+    r = subprocess.run([str(exe), "1", str(tmp_path / "vi.raw")], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0 or "marchingLoop_mi355x: -5" in r.stderr, r.stdout + r.stderr
 
 
 @pytest.mark.gpu
@@ -68,3 +88,16 @@ def test_c_driver_matches_python_binding(engine, tmp_path):
         c_bytes = (cdir / name).read_bytes()
         assert c_bytes == (pdir / name).read_bytes(), name
         assert len(c_bytes.splitlines()) == STEPS
+
+    # the same main built as C++: c and dt differ in the last float bits, so
+    # the series agree to FP32 round-off, not bit for bit
+    exe = build_driver(tmp_path, "c++")
+    xdir = tmp_path / "cpp_run"
+    xdir.mkdir()
+    print(f"[c-driver] the same scene built as C++, {STEPS} steps", flush=True)
+    r = subprocess.run([str(exe), str(STEPS), str(xdir / "vi.raw")], cwd=xdir, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    vx = np.fromfile(xdir / "vi.raw", dtype=np.float32).reshape(2, STEPS).astype(np.float64)
+    for got, want in ((vx[0], v), (vx[1], i)):
+        assert np.linalg.norm(got - want) <= 1e-4 * np.linalg.norm(want)

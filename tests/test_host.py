@@ -251,3 +251,61 @@ def test_integration_c_call_example_compiles(tmp_path, lib):
            str(src), "-L", libdir, "-lfdtd_amd", "-Wl,--no-undefined", "-o", str(tmp_path / "libexample.so")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_create_accepts_float_dt_at_courant_one(lib):
+    """A dt evaluated in float at Courant factor 1 (source.cu:45-46 compiled as
+    C++: c = 1 / sqrtf(mu0 eps0)) may sit a few float ulps above the double
+    CFL limit; fdtd_create takes it (validation passes; without a device the
+    call then stops at FDTD_E_DEVICE), and still refuses a dt 1e-5 above."""
+    from fdtd_amd import api
+    f32 = np.float32
+    eps0, pi = f32(8.854187817e-12), f32(3.1415)
+    mu0 = f32(float(f32(4 * pi)) * 1e-7)
+    d = (f32(0.389e-3), f32(0.4e-3), f32(0.265e-3))
+    c = f32(1) / np.sqrt(f32(mu0 * eps0))            # C++: the float sqrt
+    dt = f32(1.0 / (float(c) * math.sqrt(sum(1.0 / float(x) ** 2 for x in d))))
+    cfl = 1.0 / ((1.0 / math.sqrt(float(mu0) * float(eps0))) * math.sqrt(sum(1.0 / float(x) ** 2 for x in d)))
+    assert float(dt) > cfl            # the case the check must let through
+    for scale, ok in ((1.0, True), (1.0 + 1e-5, False)):
+        g = api.FdtdGrid()
+        g.nx, g.ny, g.nz = 16, 16, 16
+        g.dx, g.dy, g.dz = (float(x) for x in d)
+        g.dt, g.eps0, g.mu0 = float(dt) * scale, float(eps0), float(mu0)
+        ctx = C.c_void_p()
+        rc = lib.fdtd_create(C.byref(g), 0, C.byref(ctx))
+        if ok:
+            assert rc in (api.FDTD_OK, api.FDTD_E_DEVICE), rc
+            if rc == api.FDTD_OK:
+                lib.fdtd_destroy(ctx)
+        else:
+            assert rc == api.FDTD_E_INVALID
+
+
+def test_reference_sigma_max_is_the_float_expression(tmp_path):
+    """scenes.reference_sigma_max (used by the patch-antenna scene, and the
+    same float expression marchingLoop_mi355x evaluates with the caller's pi)
+    equals setABCPML's sigma_factor * (npml + 1) / (150 * pi * 1 * d) as a C
+    compiler evaluates it in float (fdtdrav.h:1793-1808), and differs from
+    the same with the exact pi."""
+    from fdtd_amd import scenes
+    src = tmp_path / "smax.c"
+    src.write_text("#include <stdio.h>\n#include <stdlib.h>\n"
+                   "int main(int argc, char **argv) {\n"
+                   "    float pi = 3.1415, npml = 3, sigma_factor = 1.5;\n"
+                   "    for (int a = 1; a < argc; ++a) {\n"
+                   "        float d = strtof(argv[a], 0);\n"
+                   "        float sigma_opt = (npml + 1) / (150 * pi * 1 * d);\n"
+                   "        float sigma_max = sigma_factor * sigma_opt;\n"
+                   "        printf(\"%a\\n\", (double)sigma_max);\n"
+                   "    }\n    return 0;\n}\n")
+    exe = tmp_path / "smax"
+    subprocess.run(["gcc", "-std=c99", "-O0", "-ffp-contract=off", str(src), "-o", str(exe)], check=True)
+    cells = ["0.389e-3", "0.4e-3", "0.265e-3", "1e-3", "0.5e-3"]
+    out = subprocess.run([str(exe)] + cells, capture_output=True, text=True, check=True).stdout.split()
+    for d, want in zip(cells, out):
+        got = scenes.reference_sigma_max(np.float32(3.1415), np.float32(float(d)))
+        assert got == float.fromhex(want), d
+        assert got != scenes.reference_sigma_max(np.float32(math.pi), np.float32(float(d)))
+    sc = scenes.patch_antenna(steps=1)
+    assert sc.cpml.sigma_max == tuple(float.fromhex(v) for v in out[:3])
