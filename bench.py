@@ -55,8 +55,7 @@ def b_alg(sc, info, storage_bytes):
     psi = 8.0 * s * sum(f)
     ncls = info["n_classes"]
     cls_bytes = 0.0 if ncls <= 1 else (2.0 if ncls <= 65536 else 4.0)
-    fields = 12 * s if info["fused"] else 18 * s
-    return fields + cls_bytes + psi
+    return 18 * s + cls_bytes + psi
 
 
 def _traffic(workload, storage, info):

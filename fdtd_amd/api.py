@@ -41,6 +41,7 @@ EXPORTS = {
     "fdtd_set_design_eps": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
     "fdtd_finalize": (C.c_int, [_P, C.c_int]),
     "fdtd_run": (C.c_int, [_P, C.c_int]),
+    "fdtd_step": (C.c_int, [_P]),
     "fdtd_reset": (C.c_int, [_P]),
     "fdtd_profile": (C.c_int, [_P, C.c_int]),
     "fdtd_get_info": (C.c_int, [_P, _P]),
@@ -90,7 +91,7 @@ class FdtdInfo(C.Structure):
                 ("n_classes", C.c_int), ("storage", C.c_int), ("dt", C.c_double),
                 ("last_run_ms", C.c_double), ("e_kernel_ms", C.c_double),
                 ("h_kernel_ms", C.c_double), ("step_kernel_ms", C.c_double),
-                ("fused", C.c_int), ("pad1", C.c_int), ("adjoint_forward_ms", C.c_double),
+                ("interior_split", C.c_int), ("pad1", C.c_int), ("adjoint_forward_ms", C.c_double),
                 ("adjoint_total_ms", C.c_double), ("adjoint_segments", C.c_int), ("pad0", C.c_int),
                 ("device_bytes", C.c_uint64),
                 ("bytes_per_cell_e", C.c_uint64), ("bytes_per_cell_h", C.c_uint64),
@@ -232,6 +233,10 @@ class Simulation:
     # -- time loop ------------------------------------------------------------
     def run(self, nsteps):
         self._check(self.lib.fdtd_run(self.h, int(nsteps)))
+
+    def step(self):
+        """One step (E then H), as one call of the reference's solver."""
+        self._check(self.lib.fdtd_step(self.h))
 
     def profile(self, nsteps):
         self._check(self.lib.fdtd_profile(self.h, int(nsteps)))

@@ -28,10 +28,8 @@ hipError_t launch_half_step(bool efield, int storage, int mode, const KArgs &a, 
 hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a, const AArgs &b, int m, int fstep,
                       int sten, hipStream_t s);
 hipError_t launch_snap(int storage, const KArgs &a, float *hist, int slot, hipStream_t s);
-hipError_t launch_step(int storage, int mode, const KArgs &a, int m, hipStream_t s);
 hipError_t launch_pair_half_step(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s,
                                  bool gen_cells);
-void step_tiles(int nx, int ny, int &tiles_x, int &tiles_y);
 int tile_rows();
 int tile_cols();
 int pair_tile_cols();
@@ -57,20 +55,14 @@ struct fdtd_ctx {
     size_t esz = 4;                 // bytes per stored value
     i64 field_elems = 0;            // per field allocation, guard planes included
     std::vector<void *> allocs;     // every device allocation
-    // Two buffer sets (ping-pong for the one-kernel step): fields[s][comp]
-    // are allocation bases; psi[s][q] holds the E-side psi of set s and, in
-    // set 0 only, the H-side psi (updated in place).  `cur` is the set that
-    // holds the current state.
-    void *fields_set[2][6] = {};
-    void *psi_set[2][12] = {};
-    int cur = 0;
-    bool fused = true;              // one kernel per step (else E and H half-step kernels)
+    // Field and psi allocation bases (the sweeps update them in place).
+    void *fields[6] = {};
+    void *psi[12] = {};
     bool pair = false;              // 16-bit storage: two cells per thread in the half-step kernels
     bool pair_gen_cells = false;    // ... except the general variant, which runs one cell per thread
     int pair_tiles_x = 0, pair_kchunk = 0;
     int kchunk_h = 0;               // planes per block of the one-cell H sweep (E: ka.kchunk)
     int box[2][6] = {};             // interior tile box per sweep (0 = E, 1 = H)
-    KArgs ka_step{};                // tile geometry of the step kernel
     double step_ms = 0.0;
     size_t psi_elems[12] = {};
     uint64_t dev_bytes = 0;
@@ -134,16 +126,13 @@ void free_device(fdtd_ctx *c)
     for (void *p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
     c->dev_bytes = 0;
-    for (int s = 0; s < 2; ++s) {
-        for (auto &f : c->fields_set[s]) f = nullptr;
-        for (auto &p : c->psi_set[s]) p = nullptr;
-    }
-    c->cur = 0;
+    for (auto &f : c->fields) f = nullptr;
+    for (auto &p : c->psi) p = nullptr;
 }
 
 // The buffers holding the current state.
-void *fld(const fdtd_ctx *c, int f) { return c->fields_set[c->cur][f]; }
-void *psv(const fdtd_ctx *c, int q) { return (q % 4) < 2 ? c->psi_set[c->cur][q] : c->psi_set[0][q]; }
+void *fld(const fdtd_ctx *c, int f) { return c->fields[f]; }
+void *psv(const fdtd_ctx *c, int q) { return c->psi[q]; }
 
 bool box_in_grid(const fdtd_ctx *c, int i, int j, int k)
 {
@@ -153,15 +142,11 @@ bool box_in_grid(const fdtd_ctx *c, int i, int j, int k)
 // Zero every piece of time-dependent state.
 int zero_state(fdtd_ctx *c)
 {
-    for (int s = 0; s < 2; ++s) {
-        for (int f = 0; f < 6; ++f)
-            if (c->fields_set[s][f])
-                HIPCHK(c, hipMemsetAsync(c->fields_set[s][f], 0, (size_t)c->field_elems * c->esz, c->stream));
-        for (int p = 0; p < 12; ++p)
-            if (c->psi_elems[p] && c->psi_set[s][p])
-                HIPCHK(c, hipMemsetAsync(c->psi_set[s][p], 0, c->psi_elems[p] * c->esz, c->stream));
-    }
-    c->cur = 0;
+    for (int f = 0; f < 6; ++f)
+        if (c->fields[f]) HIPCHK(c, hipMemsetAsync(c->fields[f], 0, (size_t)c->field_elems * c->esz, c->stream));
+    for (int p = 0; p < 12; ++p)
+        if (c->psi_elems[p] && c->psi[p])
+            HIPCHK(c, hipMemsetAsync(c->psi[p], 0, c->psi_elems[p] * c->esz, c->stream));
     const size_t np = c->in.probes.size(), nf = c->in.freqs.size();
     if (np) HIPCHK(c, hipMemsetAsync(c->ka.probe_hist, 0, np * (size_t)c->in.steps_cap * sizeof(float), c->stream));
     if (np && nf) HIPCHK(c, hipMemsetAsync(c->ka.dft, 0, np * nf * sizeof(double2), c->stream));
@@ -282,12 +267,7 @@ int do_finalize(fdtd_ctx *c)
     a.rec = 1;
     a.snap = nullptr;
 
-    // fields: one zero guard plane below and above; a second set for the
-    // one-kernel step
-    // The half-step kernels are the default; FDTD_KERNEL=fused selects the
-    // one-kernel step (same results, measured slower on MI355X).
-    const char *ks = std::getenv("FDTD_KERNEL");
-    c->fused = ks && std::strcmp(ks, "fused") == 0;
+    // fields: one zero guard plane below and above
     c->field_elems = a.plane * (in.nz + 2);
     // The sweep kernels address every array with 32-bit byte offsets.
     const size_t cls_bytes = (size_t)a.plane * in.nz * (b.mode == CLS_U32 ? 4 : 2);
@@ -295,11 +275,10 @@ int do_finalize(fdtd_ctx *c)
         return fail(c, FDTD_E_UNSUPPORTED,
                     "grid too large: each field array (with its two guard planes) and the class map "
                     "must be under 4 GiB (%zu bytes per field)", (size_t)c->field_elems * c->esz);
-    for (int s = 0; s < (c->fused ? 2 : 1); ++s)
-        for (int f = 0; f < 6; ++f) {
-            int rc = dev_alloc(c, &c->fields_set[s][f], (size_t)c->field_elems * c->esz);
-            if (rc) return rc;
-        }
+    for (int f = 0; f < 6; ++f) {
+        int rc = dev_alloc(c, &c->fields[f], (size_t)c->field_elems * c->esz);
+        if (rc) return rc;
+    }
     // interior / general split of the two sweeps
     {
         const int tcols = c->pair ? pair_tile_cols() : tile_cols();
@@ -312,24 +291,11 @@ int do_finalize(fdtd_ctx *c)
     const size_t pz_elems = (size_t)a.Lz * in.ny * in.px;
     for (int p = 0; p < 12; ++p) {
         c->psi_elems[p] = p < 4 ? px_elems : p < 8 ? py_elems : pz_elems;
-        const bool eside = (p % 4) < 2;
-        for (int s = 0; s < (c->fused && eside ? 2 : 1); ++s) {
-            int rc = dev_alloc(c, &c->psi_set[s][p], std::max<size_t>(c->psi_elems[p], 1) * c->esz);
-            if (rc) return rc;
-        }
+        int rc = dev_alloc(c, &c->psi[p], std::max<size_t>(c->psi_elems[p], 1) * c->esz);
+        if (rc) return rc;
     }
     a.e_uni = b.e_uniform ? 1 : 0;
     a.h_uni = b.h_uniform ? 1 : 0;
-    c->ka_step = a;
-    step_tiles(in.nx, in.ny, c->ka_step.tiles_x, c->ka_step.tiles_y);
-    {
-        // about 1536 workgroups (4 wavefronts each): 24 wavefronts per CU
-        const int tiles = c->ka_step.tiles_x * c->ka_step.tiles_y;
-        const char *kc = std::getenv("FDTD_STEP_CHUNKS");
-        int nch = kc ? std::atoi(kc) : (1536 + tiles - 1) / tiles;
-        nch = std::max(1, std::min(nch, in.nz));
-        c->ka_step.kchunk = (in.nz + nch - 1) / nch;
-    }
     int rc;
     void *tmp;
     for (int ax = 0; ax < 3; ++ax) {
@@ -400,27 +366,16 @@ int upload_tables(fdtd_ctx *c)
     return FDTD_OK;
 }
 
-// Point a kernel argument block at the current buffer set (read) and the
-// other set (write).  The half-step kernels update the current set in place.
+// Point a kernel argument block at the field and psi buffers; the sweeps
+// read and write them in place.
 void bind_sets(const fdtd_ctx *c, KArgs &a)
 {
-    const int r = c->cur, w = c->fused ? 1 - c->cur : c->cur;
     const size_t g = (size_t)c->ka.plane * c->esz;   // guard plane
     void **rd[6] = {&a.ex, &a.ey, &a.ez, &a.hx, &a.hy, &a.hz};
-    void **wr[6] = {&a.nex, &a.ney, &a.nez, &a.nhx, &a.nhy, &a.nhz};
-    for (int f = 0; f < 6; ++f) {
-        *rd[f] = (char *)c->fields_set[r][f] + g;
-        *wr[f] = (char *)c->fields_set[w][f] + g;
-    }
+    for (int f = 0; f < 6; ++f) *rd[f] = (char *)c->fields[f] + g;
     void **pr[12] = {&a.psi_eyx, &a.psi_ezx, &a.psi_hyx, &a.psi_hzx, &a.psi_exy, &a.psi_ezy,
                      &a.psi_hxy, &a.psi_hzy, &a.psi_exz, &a.psi_eyz, &a.psi_hxz, &a.psi_hyz};
-    for (int p = 0; p < 12; ++p) *pr[p] = (p % 4) < 2 ? c->psi_set[r][p] : c->psi_set[0][p];
-    a.npsi_eyx = c->psi_set[w][0];
-    a.npsi_ezx = c->psi_set[w][1];
-    a.npsi_exy = c->psi_set[w][4];
-    a.npsi_ezy = c->psi_set[w][5];
-    a.npsi_exz = c->psi_set[w][8];
-    a.npsi_eyz = c->psi_set[w][9];
+    for (int p = 0; p < 12; ++p) *pr[p] = c->psi[p];
 }
 
 // One time step m.  base carries the per-call flags (rec, snapshot); the
@@ -429,18 +384,6 @@ hipError_t do_step(fdtd_ctx *c, const KArgs &base, int m, hipEvent_t ev0 = nullp
                    hipEvent_t ev2 = nullptr, hipEvent_t ev3 = nullptr)
 {
     hipError_t e;
-    if (c->fused) {
-        KArgs a = base;
-        a.tiles_x = c->ka_step.tiles_x;
-        a.tiles_y = c->ka_step.tiles_y;
-        a.kchunk = c->ka_step.kchunk;
-        bind_sets(c, a);
-        if (ev0) (void)hipEventRecord(ev0, c->stream);
-        e = launch_step(c->in.storage, c->built.mode, a, m, c->stream);
-        if (ev1) (void)hipEventRecord(ev1, c->stream);
-        c->cur = 1 - c->cur;
-        return e;
-    }
     KArgs a = base;
     a.tiles_x = c->pair ? c->pair_tiles_x : c->ka.tiles_x;
     a.tiles_y = c->ka.tiles_y;
@@ -518,18 +461,13 @@ int run_steps(fdtd_ctx *c, int nsteps, bool profile)
         for (int s = 0; s < nsteps; ++s) {
             float a = 0, b = 0;
             HIPCHK(c, hipEventElapsedTime(&a, evs[4 * s + 0], evs[4 * s + 1]));
-            if (!c->fused) HIPCHK(c, hipEventElapsedTime(&b, evs[4 * s + 2], evs[4 * s + 3]));
+            HIPCHK(c, hipEventElapsedTime(&b, evs[4 * s + 2], evs[4 * s + 3]));
             se += a;
             sh += b;
         }
-        if (c->fused) {
-            c->step_ms = se / nsteps;
-            c->e_ms = c->h_ms = 0.0;
-        } else {
-            c->e_ms = se / nsteps;
-            c->h_ms = sh / nsteps;
-            c->step_ms = c->e_ms + c->h_ms;
-        }
+        c->e_ms = se / nsteps;
+        c->h_ms = sh / nsteps;
+        c->step_ms = c->e_ms + c->h_ms;
     }
     c->steps_done = m0 + nsteps;
     // blow-up detection on the probe samples of this call
@@ -1108,6 +1046,12 @@ int fdtd_run(fdtd_ctx *c, int nsteps)
     return run_steps(c, nsteps, false);
 }
 
+int fdtd_step(fdtd_ctx *c)
+{
+    if (!c) return FDTD_E_INVALID;
+    return run_steps(c, 1, false);
+}
+
 int fdtd_profile(fdtd_ctx *c, int nsteps)
 {
     if (!c) return FDTD_E_INVALID;
@@ -1138,7 +1082,7 @@ int fdtd_get_info(const fdtd_ctx *c, fdtd_info *o)
     o->e_kernel_ms = c->e_ms;
     o->h_kernel_ms = c->h_ms;
     o->step_kernel_ms = c->step_ms;
-    o->fused = c->fused ? 1 : 0;
+    o->interior_split = c->box[0][3] > c->box[0][0] ? 1 : 0;
     o->device_bytes = c->dev_bytes;
     // algorithmic bytes per cell: E sweep reads E, H and writes E (9 values);
     // H sweep likewise; plus one class index per cell in a classed sweep and
@@ -1255,7 +1199,7 @@ int fdtd_get_field(fdtd_ctx *c, int comp, float *out, size_t n)
     const size_t cells = (size_t)nx * ny * nz;
     if (n < cells) return fail(c, FDTD_E_INVALID, "output too small");
     std::vector<unsigned char> raw((size_t)px * ny * nz * c->esz);
-    const char *base = (const char *)c->fields_set[c->cur][comp] + (size_t)c->ka.plane * c->esz;
+    const char *base = (const char *)c->fields[comp] + (size_t)c->ka.plane * c->esz;
     HIPCHK(c, hipMemcpy(raw.data(), base, raw.size(), hipMemcpyDeviceToHost));
     const float un = c->built.probe_unscale[comp];
     for (size_t r = 0; r < (size_t)ny * nz; ++r)

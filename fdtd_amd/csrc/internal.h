@@ -46,16 +46,10 @@ enum SweepRegion { V_IN = 0, V_X = 1, V_Y = 2, V_Z = 4, V_GEN = 7 };
 // k = nz-1, so every i+-1, j+-1, k+-1 neighbour address is inside it.
 struct KArgs {
     void *ex, *ey, *ez, *hx, *hy, *hz;
-    // the fused step kernel reads the set above and writes this one (the
-    // half-step kernels update in place: same pointers)
-    void *nex, *ney, *nez, *nhx, *nhy, *nhz;
     // psi slabs: x [nz][ny][Lx], y [nz][Ly][px], z [Lz][ny][px]
     void *psi_eyx, *psi_ezx, *psi_hyx, *psi_hzx;
     void *psi_exy, *psi_ezy, *psi_hxy, *psi_hzy;
     void *psi_exz, *psi_eyz, *psi_hxz, *psi_hyz;
-    // E-side psi written by the fused step kernel (H-side psi is updated in
-    // place: only the owning tile reads or writes it)
-    void *npsi_eyx, *npsi_ezx, *npsi_exy, *npsi_ezy, *npsi_exz, *npsi_eyz;
     const void *cls;           // class map [nz][ny][px] (u16/u32) or null
     const float4 *etab;        // [class][3] (ce, cb_a, cb_b, 0) for Ex, Ey, Ez
     const float4 *htab;        // [class][3] (ch, cb_a, cb_b, 0) for Hx, Hy, Hz

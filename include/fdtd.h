@@ -129,7 +129,8 @@ typedef struct {
     double   e_kernel_ms;        /* mean E-kernel launch time (fdtd_profile)   */
     double   h_kernel_ms;        /* mean H-kernel launch time (fdtd_profile)   */
     double   step_kernel_ms;     /* mean time of one step's launches            */
-    int      fused;              /* 1: one kernel per step; 0: E and H kernels  */
+    int      interior_split;     /* 1: the sweeps run interior / x-edge /
+                                    general region variants; 0: general only */
     int      pad1;
     double   adjoint_forward_ms; /* last fdtd_adjoint_gradient: forward pass   */
     double   adjoint_total_ms;   /* ... forward + recompute + reverse sweep     */
@@ -252,6 +253,12 @@ int fdtd_finalize(fdtd_ctx *ctx, int n_steps);
  * loop of fdtdrav.h:4320-4355).  Blocking; returns FDTD_E_NONFINITE if a
  * probe went non-finite. */
 int fdtd_run(fdtd_ctx *ctx, int nsteps);
+
+/* Advance one step: the E sweep then the H sweep of step steps_done.
+ * Replaces one call of solver (fdtdrav.h:3188-3422), which the reference's
+ * loop makes per step (fdtdrav.h:4320-4355).  Blocking; same results as
+ * fdtd_run(ctx, 1). */
+int fdtd_step(fdtd_ctx *ctx);
 
 /* Rewind to step 0: zero fields, psi, probe history and DFT accumulators. */
 int fdtd_reset(fdtd_ctx *ctx);

@@ -122,18 +122,15 @@ def test_fp32_bitwise_and_fp64_parity(name, engine, oracle_mod):
     sim.close()
 
 
-@pytest.mark.parametrize("path", [("split", None), ("fused", "1"), ("fused", "7")])
-def test_kernel_paths_and_chunking_bitwise(path, engine, oracle_mod, monkeypatch):
-    """The one-kernel step (any z-chunking) and the two half-step kernels
-    give identical FP32 results."""
-    kernel, chunks = path
-    monkeypatch.setenv("FDTD_KERNEL", kernel)
+@pytest.mark.parametrize("chunks", [None, "1", "7"])
+def test_sweep_chunking_bitwise(chunks, engine, oracle_mod, monkeypatch):
+    """Any number of z-blocks per sweep gives the FP32 oracle's results bit
+    for bit."""
     if chunks:
-        monkeypatch.setenv("FDTD_STEP_CHUNKS", chunks)
+        monkeypatch.setenv("FDTD_CHUNKS", chunks)
     sc = scene_dielectric()
-    print(f"[parity] kernel={kernel} chunks={chunks}", flush=True)
+    print(f"[parity] chunks={chunks}", flush=True)
     sim = engine.Simulation.from_scene(sc, "f32")
-    assert sim.info()["fused"] == (kernel == "fused")
     sim.run(sc.steps)
     o32 = oracle_mod.Oracle(sc, 32)
     o32.steps(sc.steps)
@@ -194,6 +191,12 @@ def test_reset_and_profile_reproduce_run(engine):
     np.testing.assert_array_equal(sim.probes(), ref)
     inf = sim.info()
     assert inf["step_kernel_ms"] > 0
+    # the same run one fdtd_step at a time
+    sim.reset()
+    for _ in range(sc.steps):
+        sim.step()
+    assert sim.info()["steps_done"] == sc.steps
+    np.testing.assert_array_equal(sim.probes(), ref)
 
 
 @pytest.mark.parametrize("storage,nz", [("f32", 1022), ("f16", 2046)])

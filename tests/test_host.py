@@ -153,12 +153,11 @@ def test_bench_byte_model():
     import bench
     from fdtd_amd import scenes
     sc = scenes.slab512()
-    info = {"n_classes": 8160, "fused": 0}
+    info = {"n_classes": 8160}
     assert bench.b_alg(sc, info, 4) == pytest.approx(82.0)
     assert bench.b_alg(sc, info, 2) == pytest.approx(42.0)
-    assert bench.b_alg(sc, {"n_classes": 8160, "fused": 1}, 4) == pytest.approx(58.0)
     sc = scenes.yee64()
-    assert bench.b_alg(sc, {"n_classes": 1, "fused": 0}, 4) == pytest.approx(102.0)
+    assert bench.b_alg(sc, {"n_classes": 1}, 4) == pytest.approx(102.0)
 
 
 def test_bench_refuses_more_than_one_gpu():
