@@ -58,6 +58,8 @@ EXPORTS = {
     "fdtd_checkpoint_restore": (C.c_int, [_P, _P, C.c_size_t]),
     "fdtd_adjoint_gradient": (C.c_int, [_P, _P, C.c_int, C.c_int, C.POINTER(C.c_double),
                                         C.POINTER(C.c_double), C.c_size_t]),
+    "fdtd_adjoint_gradient_noalloc": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_double),
+                                                C.POINTER(C.c_double), C.c_size_t]),
 }
 
 
@@ -297,6 +299,18 @@ class Simulation:
     def checkpoint_restore(self, blob: bytes):
         buf = C.create_string_buffer(bytes(blob), len(blob))
         self._check(self.lib.fdtd_checkpoint_restore(self.h, buf, len(blob)))
+
+    def adjoint_gradient_noalloc(self, terms, n_steps):
+        """fdtd_adjoint_gradient_noalloc: the stored-history adjoint on the
+        library's static device scratch.  Returns (J, grad)."""
+        n = len(terms)
+        obj = FdtdObjective(n, (C.c_int * n)(*[t[0] for t in terms]), (C.c_int * n)(*[t[1] for t in terms]),
+                            (C.c_double * n)(*[t[2] for t in terms]))
+        J = C.c_double()
+        g = np.zeros(self.design_cells, dtype=np.float64)
+        self._check(self.lib.fdtd_adjoint_gradient_noalloc(self.h, C.byref(obj), int(n_steps), C.byref(J),
+                                                           _dptr(g), g.size))
+        return J.value, g
 
     def adjoint_gradient(self, terms, n_steps, checkpoint_every):
         """terms: list of (probe, freq_index, weight).  Returns (J, grad)."""

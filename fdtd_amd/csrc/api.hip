@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <new>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -73,11 +74,46 @@ struct fdtd_ctx {
     double adj_fwd_ms = 0.0, adj_total_ms = 0.0;
     int adj_segments = 0;
     uint64_t adj_bytes = 0;         // peak device memory of the last adjoint run
+    hipEvent_t adj_ev[3] = {};      // timing events of the adjoint (made at create)
+    unsigned char *scratch = nullptr;   // this device's copy of the static scratch
 };
+
+// Device scratch of fdtd_adjoint_gradient_noalloc: a file-scope device array,
+// one copy per GPU, held from the library's first use on that GPU until the
+// process ends.  Its name is a host-side stand-in, never a device address:
+// host code uses only the per-device address hipGetSymbolAddress returns
+// (scratch_address), for kernel arguments, memsets and copies.  Every call
+// on one GPU shares that GPU's copy, and it keeps the last call's contents.
+static __device__ __align__(256) unsigned char scratch[FDTD_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
 const char *g_static_err = "no error";
+
+// The address of this device's copy of scratch, looked up once per device
+// with that device current (fdtd_create's device); nullptr on failure.
+unsigned char *scratch_address(int device)
+{
+    static std::mutex mu;
+    static std::vector<unsigned char *> addr;
+    static std::vector<char> looked;
+    std::lock_guard<std::mutex> lock(mu);
+    if (device < 0) return nullptr;
+    if ((size_t)device >= addr.size()) {
+        addr.resize((size_t)device + 1, nullptr);
+        looked.resize((size_t)device + 1, 0);
+    }
+    if (!looked[(size_t)device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) {
+            addr[(size_t)device] = (unsigned char *)p;
+            looked[(size_t)device] = 1;
+        } else {
+            (void)hipGetLastError();   // not cached: a later context may retry
+        }
+    }
+    return addr[(size_t)device];
+}
 
 int fail(fdtd_ctx *c, int code, const char *fmt, ...)
 {
@@ -530,32 +566,50 @@ int copy_state(fdtd_ctx *c, char *dst, bool save)
     return FDTD_OK;
 }
 
+// The device buffers of one adjoint call: hipMalloc'd and freed on return,
+// or (arena set) carved in 256-byte-aligned pieces from the static scratch,
+// with no allocation at all.
 struct DevBuf {
     std::vector<void *> ptrs;
     uint64_t bytes = 0;
+    unsigned char *arena = nullptr;
+    size_t arena_cap = 0;
+    hipStream_t stream = nullptr;   // the arena's work is queued here
+    static size_t rounded(size_t b) { return (std::max<size_t>(b, 256) + 255) / 256 * 256; }
     ~DevBuf()
     {
         // an early return may leave work queued on these buffers
         if (!ptrs.empty()) (void)hipDeviceSynchronize();
         for (void *p : ptrs) (void)hipFree(p);
+        if (arena && bytes && stream) (void)hipStreamSynchronize(stream);
     }
     void *get(size_t bytes_)
     {
+        const size_t b = rounded(bytes_);
+        if (arena) {
+            if (bytes + b > arena_cap) return nullptr;
+            void *p = arena + bytes;
+            bytes += b;
+            return p;
+        }
         void *p = nullptr;
-        if (hipMalloc(&p, std::max<size_t>(bytes_, 256)) != hipSuccess) {
+        if (hipMalloc(&p, b) != hipSuccess) {
             (void)hipGetLastError();
             return nullptr;
         }
         ptrs.push_back(p);
-        bytes += std::max<size_t>(bytes_, 256);
+        bytes += b;
         return p;
     }
 };
 
 }  // namespace
 
+// arena: nullptr allocates the call's buffers with hipMalloc; otherwise they
+// are carved from arena (the static scratch, arena_cap bytes), which allows
+// one segment only (no checkpoint pool, no second stream).
 int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, double *J_out, double *grad_out,
-                size_t n_grad)
+                size_t n_grad, unsigned char *arena = nullptr, size_t arena_cap = 0)
 {
     if (!c->finalized) return fail(c, FDTD_E_STATE, "fdtd_finalize has not been called");
     const SceneIn &in = c->in;
@@ -595,7 +649,6 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     const size_t hist_bytes = (size_t)(Kseg + 1) * 3 * nb * sizeof(float);
     const size_t ck_bytes = state_bytes(c);
     size_t freeb = 0, totb = 0;
-    HIPCHK(c, hipMemGetInfo(&freeb, &totb));
     const size_t adj_bytes = 6 * (size_t)c->field_elems * 4 + [&] {
         size_t t = 0;
         for (int q = 0; q < 12; ++q) t += c->psi_elems[q] * 4;
@@ -604,13 +657,31 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     // segmented runs double-buffer the history: segment s-1 is recomputed on
     // a second stream while the reverse sweep of segment s runs
     const int nhist = nseg > 1 ? 2 : 1;
-    const size_t need = nhist * hist_bytes + (size_t)(nseg - 1) * ck_bytes + adj_bytes + (size_t)np * n * 4 +
-                        3 * nb * 8;
-    if ((double)need > 0.9 * (double)freeb)
-        return fail(c, FDTD_E_NOMEM, "adjoint needs %.2f GB of device memory, %.2f GB free; use a larger "
-                    "checkpoint interval", need / 1e9, freeb / 1e9);
+    if (arena) {
+        if (nseg > 1)
+            return fail(c, FDTD_E_INVALID, "the no-allocation adjoint stores the design history of the whole run: "
+                        "checkpoint_every must be 0 or >= n_steps");
+        // exactly the pieces carved below, each rounded to 256 bytes
+        size_t carve = 6 * DevBuf::rounded((size_t)c->field_elems * 4) + DevBuf::rounded(hist_bytes) +
+                       DevBuf::rounded(3 * nb * sizeof(double)) +
+                       DevBuf::rounded((size_t)std::max(np, 1) * n * sizeof(float));
+        for (int q = 0; q < 12; ++q) carve += DevBuf::rounded(std::max<size_t>(c->psi_elems[q], 1) * 4);
+        if (carve > arena_cap)
+            return fail(c, FDTD_E_NOMEM, "the no-allocation adjoint needs %zu bytes of device scratch for this "
+                        "run; the static scratch holds %zu (FDTD_NOALLOC_SCRATCH_BYTES)", carve, arena_cap);
+    } else {
+        HIPCHK(c, hipMemGetInfo(&freeb, &totb));
+        const size_t need = nhist * hist_bytes + (size_t)(nseg - 1) * ck_bytes + adj_bytes +
+                            (size_t)np * n * 4 + 3 * nb * 8;
+        if ((double)need > 0.9 * (double)freeb)
+            return fail(c, FDTD_E_NOMEM, "adjoint needs %.2f GB of device memory, %.2f GB free; use a larger "
+                        "checkpoint interval", need / 1e9, freeb / 1e9);
+    }
 
     DevBuf mem;
+    mem.arena = arena;
+    mem.arena_cap = arena_cap;
+    mem.stream = c->stream;
     AArgs b{};
     for (int q = 0; q < 3; ++q) {
         char *e = (char *)mem.get((size_t)c->field_elems * 4);
@@ -665,10 +736,7 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     const int mode = c->built.mode;
     const int euni = c->built.e_uniform ? 1 : 0, huni = c->built.h_uniform ? 1 : 0;
 
-    EventSet evs;
-    evs.v.assign(3, nullptr);
-    for (auto &e : evs.v) HIPCHK(c, hipEventCreate(&e));
-    hipEvent_t t0 = evs.v[0], t1 = evs.v[1], t2 = evs.v[2];
+    hipEvent_t t0 = c->adj_ev[0], t1 = c->adj_ev[1], t2 = c->adj_ev[2];
     // second stream (default priority) for the reverse sweeps of a segmented
     // run, and the events that order it against the recompute stream:
     // ready[q] = history buffer q filled, freed[q] = its reverse sweep done
@@ -835,6 +903,14 @@ int fdtd_create(const fdtd_grid *g, int storage, fdtd_ctx **out)
         delete c;
         return FDTD_E_DEVICE;
     }
+    for (hipEvent_t &e : c->adj_ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            (void)hipGetLastError();
+            fdtd_destroy(c);
+            return FDTD_E_DEVICE;
+        }
+    // the no-allocation adjoint's scratch on this (current) device
+    c->scratch = scratch_address(c->device);
     *out = c;
     return FDTD_OK;
 }
@@ -843,6 +919,8 @@ int fdtd_destroy(fdtd_ctx *c)
 {
     if (!c) return FDTD_E_INVALID;
     free_device(c);
+    for (hipEvent_t e : c->adj_ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return FDTD_OK;
@@ -1311,6 +1389,15 @@ int fdtd_adjoint_gradient(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, i
 {
     if (!c || !obj || !J_out || !grad_out) return FDTD_E_INVALID;
     return run_adjoint(c, obj, n_steps, checkpoint_every, J_out, grad_out, n_grad);
+}
+
+int fdtd_adjoint_gradient_noalloc(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, double *J_out,
+                                  double *grad_out, size_t n_grad)
+{
+    if (!c || !obj || !J_out || !grad_out) return FDTD_E_INVALID;
+    if (!c->scratch)
+        return fail(c, FDTD_E_DEVICE, "the static scratch has no device address on device %d", c->device);
+    return run_adjoint(c, obj, n_steps, 0, J_out, grad_out, n_grad, c->scratch, FDTD_NOALLOC_SCRATCH_BYTES);
 }
 
 }  // extern "C"

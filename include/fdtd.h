@@ -336,6 +336,24 @@ int fdtd_adjoint_gradient(fdtd_ctx *ctx, const fdtd_objective *obj,
                           int n_steps, int checkpoint_every,
                           double *J_out, double *grad_out, size_t n_grad);
 
+/* Device scratch of fdtd_adjoint_gradient_noalloc, in bytes: a static array
+ * the library holds on every GPU it runs on, until the process ends. */
+#define FDTD_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+
+/* fdtd_adjoint_gradient with checkpoint_every = 0 (the design-region E
+ * history of the whole run is stored), without allocating: no hipMalloc or
+ * hipFree, no stream or event creation.  Its device buffers (the FP32 adjoint
+ * fields and psi, the history, the gradient accumulators) are carved from
+ * the static scratch of FDTD_NOALLOC_SCRATCH_BYTES per GPU; a run that needs
+ * more returns FDTD_E_NOMEM with the figure (adjoint128, 1000 steps, needs
+ * 247 MiB).  Same results as fdtd_adjoint_gradient(..., 0, ...), bit for
+ * bit.  Every call on one GPU shares that GPU's scratch: calls that overlap
+ * on one GPU (other threads or contexts) race.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid it). */
+int fdtd_adjoint_gradient_noalloc(fdtd_ctx *ctx, const fdtd_objective *obj, int n_steps,
+                                  double *J_out, double *grad_out, size_t n_grad);
+
 #ifdef __cplusplus
 }
 #endif
