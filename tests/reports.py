@@ -165,25 +165,28 @@ def adjoint_report(args):
             out["grad_rel_l2_vs_fp32_adjoint"] = float(np.linalg.norm(g - g32) / np.linalg.norm(g32))
         else:
             g32 = g
-        h = 5e-2
-        errs = []
-        for q in cells:
-            _progress(f"{wl}: FP32 engine central difference, cell {q}")
-            vals, ev = [], []
-            for sgn in (1, -1):
-                e2 = eps.copy()
-                e2[q] = np.float32(eps[q] + sgn * h)
-                ev.append(float(e2[q]))
-                ref.set_design_eps(e2)
-                ref.run(sc.steps)
-                X = ref.dft()
-                vals.append(sum(w * abs(X[p, f]) ** 2 for p, f, w in terms))
-            fd = (vals[0] - vals[1]) / (ev[0] - ev[1])
-            errs.append(abs(g32[q] - fd) / float(np.abs(g32).max()))
+        out["fd_fp32_engine"] = []
+        for h in args.fd_h:
+            errs = []
+            for q in cells:
+                _progress(f"{wl}: FP32 engine central difference, h {h}, cell {q}")
+                vals, ev = [], []
+                for sgn in (1, -1):
+                    e2 = eps.copy()
+                    e2[q] = np.float32(eps[q] + sgn * h)
+                    ev.append(float(e2[q]))
+                    ref.set_design_eps(e2)
+                    ref.run(sc.steps)
+                    X = ref.dft()
+                    vals.append(sum(w * abs(X[p, f]) ** 2 for p, f, w in terms))
+                fd = (vals[0] - vals[1]) / (ev[0] - ev[1])
+                errs.append(abs(g32[q] - fd) / float(np.abs(g32).max()))
+            out["fd_fp32_engine"].append(
+                {"cells": len(cells), "h": h,
+                 "fp32_adjoint_err_over_max_grad": {"median": float(np.median(errs)) if errs else None,
+                                                    "max": float(np.max(errs)) if errs else None,
+                                                    "each": [float(f"{e:.3e}") for e in errs]}})
         ref.close()
-        out["fd_fp32_engine"] = {"cells": len(cells), "h": h,
-                                 "fp32_adjoint_err_over_max_grad": {"median": float(np.median(errs)) if errs else None,
-                                                                    "max": float(np.max(errs)) if errs else None}}
     out["grad_max_abs"] = gmax
     return out
 
@@ -197,6 +200,8 @@ def main():
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--checkpoint-every", type=int, default=64)
     ap.add_argument("--fd-cells", type=int, default=16)
+    ap.add_argument("--fd-h", type=lambda v: [float(x) for x in v.split(",")], default=[5e-2],
+                    help="adjoint768: comma-separated eps_r steps of the FP32-engine central differences")
     ap.add_argument("--adjoint-steps", type=int, default=0, help="steps of the run (0: the workload's own)")
     ap.add_argument("--no-cpu", action="store_true", help="adjoint: skip the host FP64 checks")
     args = ap.parse_args()

@@ -61,3 +61,37 @@ def test_engine_matches_fixture(name, engine):
         derr = np.abs(X - R) / np.abs(R).max()
         assert derr.max() <= 1e-5, derr.max()
     sim.close()
+
+
+def test_adjoint128_fixture_is_self_consistent():
+    """The committed adjoint128 fixture (FP64 oracle): its adjoint gradient
+    agrees with its own FP64 central differences on the 16 seeded cells to
+    1e-5 of max |g| (the two are computed independently: a reverse sweep of
+    the transposed step, and two forward runs per cell), the cells are the
+    seeded ones, and every perturbed eps_r differs from the design value by
+    h rounded to float32."""
+    from fdtd_amd import scenes
+    from make_golden import FD_H, fd_cells
+    g = load("adjoint128")
+    sc = scenes.adjoint128()
+    assert int(g["steps"]) == sc.steps and int(g["seed"]) == sc.seed
+    np.testing.assert_array_equal(g["fd_cells"], fd_cells(sc))
+    gmax = np.abs(g["grad"]).max()
+    assert np.abs(g["grad"][g["fd_cells"]] - g["fd"]).max() <= 1e-5 * gmax
+    eps = sc.design_eps.astype(np.float32)[g["fd_cells"]]
+    np.testing.assert_array_equal(g["eps_plus"], (eps + np.float32(FD_H)).astype(np.float64))
+    np.testing.assert_array_equal(g["eps_minus"], (eps - np.float32(FD_H)).astype(np.float64))
+
+
+def test_slab512_fixture_shape():
+    """The committed FP64 fixture of the full slab512 run (the benchmark's
+    accuracy target) covers every probe for the workload's 3000 steps and its
+    16 DFT frequencies, and every probe carries the pulse."""
+    from fdtd_amd import scenes
+    g = load("slab512_fp64")
+    sc = scenes.slab512()
+    assert g["probes"].shape == (len(sc.probes), sc.steps)
+    assert g["dft"].shape == (len(sc.probes), len(sc.freqs))
+    np.testing.assert_array_equal(g["freqs"], np.array(sc.freqs))
+    assert int(g["seed"]) == sc.seed
+    assert np.all(np.isfinite(g["probes"])) and np.all(np.linalg.norm(g["probes"], axis=1) > 0)
