@@ -21,7 +21,7 @@ FDTD_E_DEVICE, FDTD_E_UNSUPPORTED, FDTD_E_NONFINITE, FDTD_E_CHECKSUM = -5, -6, -
 ERRORS = {-1: "FDTD_E_INVALID", -2: "FDTD_E_STATE", -3: "FDTD_E_RANGE", -4: "FDTD_E_NOMEM",
           -5: "FDTD_E_DEVICE", -6: "FDTD_E_UNSUPPORTED", -7: "FDTD_E_NONFINITE",
           -8: "FDTD_E_CHECKSUM"}
-STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
+STORAGE = {"f32": 0, "f16": 1, "bf16": 2, "bq16": 3}   # bq16: block-scaled int16 fields
 
 # Every function the header declares, with its ctypes signature.
 _P = C.c_void_p

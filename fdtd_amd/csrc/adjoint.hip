@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <climits>
 #include <cstdint>
+#include <cstdlib>
+#include <cstring>
 
 #include "device_common.h"
 #include "internal.h"
@@ -53,21 +55,20 @@ struct HQ {
     float sy, sz, tz, tx, rx, ry;
 };
 
-template <int CM, bool IN>
-__device__ HQ hquant(const KArgs &a, const AArgs &b, bool huni, int i, int j, int k)
+// From the cell's H coefficients and its dJ/dH (and the psi_h adjoint in
+// memory).
+template <bool IN>
+__device__ __forceinline__ HQ hq_of(const KArgs &a, const AArgs &b, const float4 &hx, const float4 &hy,
+                                    const float4 &hz, float hbx, float hby, float hbz, int i, int j, int k)
 {
     HQ q{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
-    const i64 c = (i64)k * a.plane + (i64)j * px + i;
-    float4 hx, hy, hz;
-    coefs<CM>(a, a.htab, huni, c, hx, hy, hz);
     const bool ux = IN || (j < ny - 1 && k < nz - 1), uy = IN || (i < nx - 1 && k < nz - 1);
     const bool uz = IN || (i < nx - 1 && j < ny - 1);
     const bool xs = !IN && xs_h(a, i), ys = !IN && ys_h(a, j), zs = !IN && zs_h(a, k);
     const float4 pxm = xs ? a.px_prof[nx + i] : NOPROF;
     const float4 pym = ys ? a.py_prof[ny + j] : NOPROF;
     const float4 pzm = zs ? a.pz_prof[nz + k] : NOPROF;
-    const float hbx = b.hb[0][c], hby = b.hb[1][c], hbz = b.hb[2][c];
     const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
     // curl coefficients over kappa, divided on the slab cells only (kappa is
     // 1 elsewhere and x / 1 = x exactly)
@@ -111,21 +112,20 @@ struct EQ {
     float ax, bx, ay, by, az, bz;
 };
 
-template <int CM, bool IN>
-__device__ EQ equant(const KArgs &a, const AArgs &b, bool euni, int i, int j, int k)
+// From the cell's E coefficients and its dJ/dE (and the psi_e adjoint in
+// memory).
+template <bool IN>
+__device__ __forceinline__ EQ eq_of(const KArgs &a, const AArgs &b, const float4 &ex, const float4 &ey,
+                                    const float4 &ez, float ebx, float eby, float ebz, int i, int j, int k)
 {
     EQ q{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
     if (!IN && (i >= nx - 1 || j >= ny - 1 || k >= nz - 1)) return q;   // no E update, no psi_e
-    const i64 c = (i64)k * a.plane + (i64)j * px + i;
-    float4 ex, ey, ez;
-    coefs<CM>(a, a.etab, euni, c, ex, ey, ez);
     const bool ux = IN || (j > 0 && k > 0), uy = IN || (i > 0 && k > 0), uz = IN || (i > 0 && j > 0);
     const bool xs = !IN && xs_e(a, i), ys = !IN && ys_e(a, j), zs = !IN && zs_e(a, k);
     const float4 pxe = xs ? a.px_prof[i] : NOPROF;
     const float4 pye = ys ? a.py_prof[j] : NOPROF;
     const float4 pze = zs ? a.pz_prof[k] : NOPROF;
-    const float ebx = b.eb[0][c], eby = b.eb[1][c], ebz = b.eb[2][c];
     const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
     float exy = ex.y, exz = ex.z, eyy = ey.y, eyz = ey.z, ezy = ez.y, ezz = ez.z;
     if (ys) { exy = exy / pye.z; ezz = ezz / pye.z; }
@@ -161,6 +161,27 @@ __device__ EQ equant(const KArgs &a, const AArgs &b, bool euni, int i, int j, in
     return q;
 }
 
+// The same quantities with the cell's coefficients and adjoint read from
+// memory (the wide-grid sweeps, which compute them where they gather).
+template <int CM, bool IN>
+__device__ HQ hquant(const KArgs &a, const AArgs &b, bool huni, int i, int j, int k)
+{
+    const i64 c = (i64)k * a.plane + (i64)j * a.px + i;
+    float4 hx, hy, hz;
+    coefs<CM>(a, a.htab, huni, c, hx, hy, hz);
+    return hq_of<IN>(a, b, hx, hy, hz, b.hb[0][c], b.hb[1][c], b.hb[2][c], i, j, k);
+}
+
+template <int CM, bool IN>
+__device__ EQ equant(const KArgs &a, const AArgs &b, bool euni, int i, int j, int k)
+{
+    if (!IN && (i >= a.nx - 1 || j >= a.ny - 1 || k >= a.nz - 1)) return EQ{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const i64 c = (i64)k * a.plane + (i64)j * a.px + i;
+    float4 ex, ey, ez;
+    coefs<CM>(a, a.etab, euni, c, ex, ey, ez);
+    return eq_of<IN>(a, b, ex, ey, ez, b.eb[0][c], b.eb[1][c], b.eb[2][c], i, j, k);
+}
+
 // First lumped-source event of column `col` at or after event `ev` (probe
 // events skipped), or -1.  The event list ends every column with a record of
 // another column and ends with a terminator.
@@ -190,26 +211,288 @@ __device__ float2 src_terms(const KArgs &a, const AArgs &b, bool euni, int i, in
 
 }  // namespace
 
-// Tiles of the adjoint sweeps.  A thread's gather needs the transposed
-// quantities of its cell and of two neighbours in the same plane ((i-1, j)
-// and (i, j-1) for the E sweep, (i+1, j) and (i, j+1) for the H sweep); the
-// plane neighbour (k-1 / k+1) is carried in registers along the z-march.
-// Each thread computes its own cell's quantities once per plane and shares
-// them through LDS: a wave covers TXA = 63 columns plus one halo lane (the
-// column before the tile for the E sweep, after it for the H sweep) that only
-// computes, and the edge row of the tile computes its missing row directly.
-// The shared values are exactly those the neighbour computes for itself.
-constexpr int TXA = TX - 1;
+__device__ __forceinline__ HQ ld_hq(const AArgs &b, i64 c)
+{
+    return HQ{b.hq[0][c], b.hq[1][c], b.hq[2][c], b.hq[3][c], b.hq[4][c], b.hq[5][c]};
+}
+__device__ __forceinline__ void st_hq(const AArgs &b, i64 c, const HQ &q)
+{
+    b.hq[0][c] = q.sy; b.hq[1][c] = q.sz; b.hq[2][c] = q.tz;
+    b.hq[3][c] = q.tx; b.hq[4][c] = q.rx; b.hq[5][c] = q.ry;
+}
+__device__ __forceinline__ EQ ld_eq(const AArgs &b, i64 c)
+{
+    return EQ{b.eq[0][c], b.eq[1][c], b.eq[2][c], b.eq[3][c], b.eq[4][c], b.eq[5][c]};
+}
+__device__ __forceinline__ void st_eq(const AArgs &b, i64 c, const EQ &q)
+{
+    b.eq[0][c] = q.ax; b.eq[1][c] = q.bx; b.eq[2][c] = q.ay;
+    b.eq[3][c] = q.by; b.eq[4][c] = q.az; b.eq[5][c] = q.bz;
+}
 
-// Whether the adjoint sweeps share through LDS for rows of nx cells.
-inline bool adj_shares(int nx) { return nx >= 8 * TX; }
+// Whether the adjoint sweeps of rows nx cells wide store every cell's
+// transposed quantities (narrow grids, where the sweeps are latency-bound:
+// plain loads replace the neighbours' coefficient lookups) or compute them
+// where they gather and share them through LDS (wide grids, where the stored
+// arrays' traffic costs more).  Measured: 128x128x64, 1000 steps, stored
+// history 98.1 -> 83.0 ms; 768x768x384 FP16, 4000 steps, checkpointed 45.5 s
+// computing against 50.6 s storing.  FDTD_ADJ_GATHER=stored / lds overrides.
+bool adj_stores_quantities(int nx)
+{
+    const char *ov = std::getenv("FDTD_ADJ_GATHER");
+    if (ov && std::strcmp(ov, "stored") == 0) return true;
+    if (ov && std::strcmp(ov, "lds") == 0) return false;
+    return nx < 8 * TX;
+}
+
+// The adjoint sweeps, one thread per (i, j) column marching a z-block.  Each
+// cell's transposed quantities are computed once, by the thread that owns
+// it, at the end of the sweep that updates its adjoint (EQ in the E sweep,
+// HQ in the H sweep), and stored: the other sweep gathers them from its own
+// cell and its neighbours ((i-1, j, k), (i, j-1, k), (i, j, k-1) for the E
+// sweep, (i+1, j, k), (i, j+1, k), (i, j, k+1) for the H sweep) as plain
+// loads, with no coefficient lookups of the neighbours.  The stored values
+// are those a recomputation would give, bit for bit.
 
 // fstep >= 0: finalize the E half-step of forward step fstep first.
 // sten: gather the transposed H half-step of step m and inject dJ/ds[m].
-template <int CM, bool SHARE, int V>
+template <int CM, int V>
 __global__ void __launch_bounds__(TX * BY)
-    adj_e_kernel(const KArgs a, const AArgs b, const int m, const int fstep, const int sten, const int euni,
-                 const int huni)
+    adj_e_kernel(const KArgs a, const AArgs b, const int m, const int fstep, const int sten, const int euni)
+{
+    constexpr bool IN = V == V_IN;   // interior tiles: no slab, face or range predicates
+    int tx, ty, tz;
+    tile_of_region<V>(a, tx, ty, tz);
+    const int i = tx * TX + (int)(threadIdx.x & (TX - 1));
+    const int j = ty * BY + (int)(threadIdx.x / TX);
+    const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
+    if (!IN && (i >= nx || j >= ny)) return;
+    const int k0 = tz * a.kchunk;
+    const int k1 = min(nz, k0 + a.kchunk);
+    const bool ilo = IN || i > 0, ihi = IN || i < nx - 1, jlo = IN || j > 0, jhi = IN || j < ny - 1;
+    const int mycol = j * nx + i;
+    int ev = first_event(a, a.e_first, mycol, k0);
+    int ek = ev >= 0 ? a.events[ev].k : INT_MAX;
+    const bool in_box = i >= b.sb[0] && i < b.sb[0] + b.sb[3] && j >= b.sb[1] && j < b.sb[1] + b.sb[4];
+    const i64 nb = (i64)b.sb[3] * b.sb[4] * b.sb[5];
+    const bool xs = !IN && xs_e(a, i), ys = !IN && ys_e(a, j);
+    const float4 pxe = xs ? a.px_prof[i] : NOPROF;
+    const float4 pye = ys ? a.py_prof[j] : NOPROF;
+    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    const HQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+
+    HQ qkm = zq;
+    if (sten && k0 > 0) qkm = ld_hq(b, (i64)(k0 - 1) * a.plane + (i64)j * px + i);
+    for (int k = k0; k < k1; ++k) {
+        const i64 c = (i64)k * a.plane + (i64)j * px + i;
+        const bool klo = IN || k > 0, khi = IN || k < nz - 1;
+        const bool ux = ihi && jlo && jhi && klo && khi;
+        const bool uy = ilo && ihi && jhi && klo && khi;
+        const bool uz = ilo && ihi && jlo && jhi && khi;
+        float ebx = b.eb[0][c], eby = b.eb[1][c], ebz = b.eb[2][c];
+        float4 ex, ey, ez;
+        coefs<CM>(a, a.etab, euni, c, ex, ey, ez);
+        // source edge and probe events of this cell
+        float cez_src = 0.f;
+        bool is_src = false;
+        int pev = -1;
+        if (k == ek) {
+            pev = ev;
+            while (a.events[ev].k == k && a.events[ev].col == mycol) {
+                if (a.events[ev].kind == EV_SOURCE) {
+                    is_src = true;
+                    cez_src = a.srcs[a.events[ev].id].ceze;
+                }
+                ++ev;
+            }
+            ek = (a.events[ev].col == mycol) ? a.events[ev].k : INT_MAX;
+        }
+        if (fstep >= 0 && (ux || uy || uz)) {
+            const bool zs = !IN && zs_e(a, k);
+            if (xs && jhi && khi) {
+                const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
+                b.pb[0][s] = pxe.x * (b.pb[0][s] + (uy ? (ey.z * dx) * eby : 0.f));
+                b.pb[1][s] = pxe.x * (b.pb[1][s] + (uz ? (ez.y * dx) * ebz : 0.f));
+            }
+            if (ys && ihi && khi) {
+                const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
+                b.pb[4][s] = pye.x * (b.pb[4][s] + (ux ? (ex.y * dy) * ebx : 0.f));
+                b.pb[5][s] = pye.x * (b.pb[5][s] + (uz ? (ez.z * dy) * ebz : 0.f));
+            }
+            if (zs && ihi && jhi) {
+                const float4 pze = a.pz_prof[k];
+                const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
+                b.pb[8][s] = pze.x * (b.pb[8][s] + (ux ? (ex.z * dz) * ebx : 0.f));
+                b.pb[9][s] = pze.x * (b.pb[9][s] + (uy ? (ey.y * dz) * eby : 0.f));
+            }
+            if (in_box && k >= b.sb[2] && k < b.sb[2] + b.sb[5]) {
+                const i64 q = (i - b.sb[0]) + (i64)b.sb[3] * ((j - b.sb[1]) + (i64)b.sb[4] * (k - b.sb[2]));
+                const float *h0 = b.hist + (i64)(fstep - b.hist_base) * 3 * nb;
+                const float *h1 = h0 + 3 * nb;
+                const float e_[3] = {ebx, eby, ebz};
+                const bool u_[3] = {ux, uy, uz};
+                for (int cc = 0; cc < 3; ++cc) {
+                    const float4 dc = b.dcoef[cc * nb + q];
+                    if (!u_[cc] || (dc.x == 0.f && dc.y == 0.f)) continue;
+                    const double em = h0[cc * nb + q], ep = h1[cc * nb + q];
+                    const double dE = (double)dc.x * em - (double)dc.y * (ep - (double)dc.z * em);
+                    b.gedge[cc * nb + q] += (double)e_[cc] * dE;
+                }
+            }
+            if (ux) ebx = ex.x * ebx;
+            if (uy) eby = ey.x * eby;
+            if (uz) ebz = (is_src ? cez_src : ez.x) * ebz;
+        }
+        if (sten) {
+            // this cell, (i-1, j, k) and (i, j-1, k)
+            const HQ q0 = ld_hq(b, c);
+            const HQ qi = ilo && (uy || uz) ? ld_hq(b, c - 1) : zq;
+            const HQ qj = jlo && (ux || uz) ? ld_hq(b, c - px) : zq;
+            if (ux) {
+                ebx += (qkm.tx - q0.tx) + (qj.rx - q0.rx);
+                if (uz) ebz += qj.sz - q0.sz;
+            }
+            if (uy) eby += (qkm.sy - q0.sy) + (qi.ry - q0.ry);
+            if (uz) ebz += qi.tz - q0.tz;
+            if (uz && !ux) ebz += qj.sz - q0.sz;
+            qkm = q0;
+            if (pev >= 0) {   // dJ/d(probe sample m) of E probes on this cell
+                for (int e = pev; a.events[e].k == k && a.events[e].col == mycol; ++e) {
+                    const int kind = a.events[e].kind;
+                    if (kind < EV_PROBE || kind > EV_PROBE + 2) continue;
+                    const float gv = b.g[(i64)a.events[e].id * b.gsteps + m];
+                    if (kind == EV_PROBE) ebx += gv;
+                    else if (kind == EV_PROBE + 1) eby += gv;
+                    else ebz += gv;
+                }
+            }
+        }
+        if (ux) b.eb[0][c] = ebx;
+        if (uy) b.eb[1][c] = eby;
+        if (uz) b.eb[2][c] = ebz;
+        // the transposed E quantities of this cell, for the H sweep's gather
+        st_eq(b, c, eq_of<IN>(a, b, ex, ey, ez, ebx, eby, ebz, i, j, k));
+    }
+}
+
+template <int CM, int V>
+__global__ void __launch_bounds__(TX * BY) adj_h_kernel(const KArgs a, const AArgs b, const int euni, const int huni)
+{
+    constexpr bool IN = V == V_IN;   // interior tiles: no slab, face or range predicates
+    int tx, ty, tz;
+    tile_of_region<V>(a, tx, ty, tz);
+    const int i = tx * TX + (int)(threadIdx.x & (TX - 1));
+    const int j = ty * BY + (int)(threadIdx.x / TX);
+    const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
+    if (!IN && (i >= nx || j >= ny)) return;
+    const int k0 = tz * a.kchunk;
+    const int k1 = min(nz, k0 + a.kchunk);
+    const bool ihi = IN || i < nx - 1, jhi = IN || j < ny - 1;
+    const bool xs = !IN && xs_h(a, i), ys = !IN && ys_h(a, j);
+    const float4 pxm = xs ? a.px_prof[nx + i] : NOPROF;
+    const float4 pym = ys ? a.py_prof[ny + j] : NOPROF;
+    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    const EQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    EQ q0 = ld_eq(b, (i64)k0 * a.plane + (i64)j * px + i);
+    // lumped-source edges among the Ez edges this thread gathers from: its
+    // own cell, (i+1, j) and (i, j+1); event cursor and its plane per column.
+    // Rows outside the source rows skip the lookups (no load: a kernel
+    // argument range).
+    const bool srow = j >= b.src_j0 && j <= b.src_j1;
+    const int scol[3] = {j * nx + i, j * nx + i + 1, (j + 1) * nx + i};
+    const bool sok[3] = {srow, srow && i + 1 < nx, srow && j + 1 < ny};
+    int sev[3], sk[3];
+    for (int q = 0; q < 3; ++q) {
+        sev[q] = sok[q] ? src_from(a, first_event(a, a.e_first, scol[q], k0), scol[q]) : -1;
+        sk[q] = sev[q] >= 0 ? a.events[sev[q]].k : INT_MAX;
+    }
+    for (int k = k0; k < k1; ++k) {
+        const i64 c = (i64)k * a.plane + (i64)j * px + i;
+        const bool khi = IN || k < nz - 1;
+        const bool ux = jhi && khi, uy = ihi && khi, uz = ihi && jhi;
+        const EQ qk = khi ? ld_eq(b, c + a.plane) : zq;
+        float4 hx, hy, hz;
+        coefs<CM>(a, a.htab, huni, c, hx, hy, hz);
+        float hbx = b.hb[0][c], hby = b.hb[1][c], hbz = b.hb[2][c];
+        // psi_h adjoint and dJ/dH <- Ch dJ/dH (the H half-step, pointwise part)
+        if (xs) {
+            const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
+            b.pb[2][s] = pxm.x * (b.pb[2][s] + (uy ? (hy.y * dx) * hby : 0.f));
+            b.pb[3][s] = pxm.x * (b.pb[3][s] + (uz ? (hz.z * dx) * hbz : 0.f));
+        }
+        if (ys) {
+            const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
+            b.pb[6][s] = pym.x * (b.pb[6][s] + (ux ? (hx.z * dy) * hbx : 0.f));
+            b.pb[7][s] = pym.x * (b.pb[7][s] + (uz ? (hz.y * dy) * hbz : 0.f));
+        }
+        if (!IN && zs_h(a, k)) {
+            const float4 pzm = a.pz_prof[nz + k];
+            const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
+            b.pb[10][s] = pzm.x * (b.pb[10][s] + (ux ? (hx.y * dz) * hbx : 0.f));
+            b.pb[11][s] = pzm.x * (b.pb[11][s] + (uy ? (hy.z * dz) * hby : 0.f));
+        }
+        if (ux) hbx = hx.x * hbx;
+        if (uy) hby = hy.x * hby;
+        if (uz) hbz = hz.x * hbz;
+        // transposed E half-step gathered from the E cells that read this H:
+        // this cell, (i+1, j, k) and (i, j+1, k)
+        {
+            const EQ qi = ihi && (uy || uz) ? ld_eq(b, c + 1) : zq;
+            const EQ qj = jhi && (ux || uz) ? ld_eq(b, c + px) : zq;
+            if (ux) {
+                hbx += (q0.ay - qk.ay) + (q0.bz - qj.bz);
+                if (uz) hbz += q0.ax - qj.ax;
+            }
+            if (uy) hby += (q0.bx - qk.bx) + (q0.az - qi.az);
+            if (uz) hbz += q0.by - qi.by;
+            if (uz && !ux) hbz += q0.ax - qj.ax;
+        }
+        if (k == sk[0] || k == sk[1] || k == sk[2]) {
+            // source edge on this cell: +terms; at (i+1, j) or (i, j+1): -terms
+            for (int q = 0; q < 3; ++q)
+                while (k == sk[q]) {
+                    const float2 d = src_terms<CM>(a, b, euni, q == 1 ? i + 1 : i, q == 2 ? j + 1 : j, k,
+                                                   a.events[sev[q]].id);
+                    if (q == 0) {
+                        if (uy) hby += d.x;
+                        if (ux) hbx += d.y;
+                    } else if (q == 1) {
+                        if (uy) hby -= d.x;
+                    } else {
+                        if (ux) hbx -= d.y;
+                    }
+                    sev[q] = src_from(a, sev[q] + 1, scol[q]);
+                    sk[q] = sev[q] >= 0 ? a.events[sev[q]].k : INT_MAX;
+                }
+        }
+        if (ux) b.hb[0][c] = hbx;
+        if (uy) b.hb[1][c] = hby;
+        if (uz) b.hb[2][c] = hbz;
+        // the transposed H quantities of this cell, for the next E sweep's gather
+        st_hq(b, c, hq_of<IN>(a, b, hx, hy, hz, hbx, hby, hbz, i, j, k));
+        q0 = qk;
+    }
+}
+
+// The adjoint sweeps of wide grids: the transposed quantities are computed
+// where they are gathered instead of stored.  A thread's gather needs those
+// of its cell and of two neighbours in the same plane ((i-1, j) and (i, j-1)
+// for the E sweep, (i+1, j) and (i, j+1) for the H sweep); the plane
+// neighbour (k-1 / k+1) is carried in registers along the z-march.  Each
+// thread computes its own cell's quantities once per plane and shares them
+// through LDS: a wave covers TXA = 63 columns plus one halo lane (the column
+// before the tile for the E sweep, after it for the H sweep) that only
+// computes, and the edge row of the tile computes its missing row directly.
+// The shared values are exactly those the neighbour computes for itself,
+// and exactly the stored ones of the narrow-grid sweeps above.
+constexpr int TXA = TX - 1;
+
+// fstep >= 0: finalize the E half-step of forward step fstep first.
+// sten: gather the transposed H half-step of step m and inject dJ/ds[m].
+template <int CM, int V>
+__global__ void __launch_bounds__(TX * BY)
+    adj_e_lds_kernel(const KArgs a, const AArgs b, const int m, const int fstep, const int sten, const int euni,
+                     const int huni)
 {
     constexpr bool IN = V == V_IN;   // interior tiles: no slab, face or range predicates
     __shared__ HQ sq[BY][TX];
@@ -217,13 +500,12 @@ __global__ void __launch_bounds__(TX * BY)
     tile_of_region<V>(a, tx, ty, tz);
     const int lane = (int)(threadIdx.x & (TX - 1));
     const int row = (int)(threadIdx.x / TX);
-    // SHARE: lane 0 is the halo column before the tile; otherwise every lane
-    // owns a cell and computes its neighbours' quantities itself
-    const int i = SHARE ? tx * TXA + lane - 1 : tx * TX + lane;
+    // lane 0 is the halo column before the tile
+    const int i = tx * TXA + lane - 1;
     const int j = ty * BY + row;
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
     const bool valid = IN || (i >= 0 && i < nx && j < ny);   // a cell whose quantities a neighbour reads
-    const bool own = valid && (!SHARE || lane > 0);          // a cell this thread updates
+    const bool own = valid && lane > 0;                       // a cell this thread updates
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
     const bool ilo = IN || i > 0, ihi = IN || i < nx - 1, jlo = IN || j > 0, jhi = IN || j < ny - 1;
@@ -306,15 +588,13 @@ __global__ void __launch_bounds__(TX * BY)
             if (uz) ebz = (is_src ? cez_src : ez.x) * ebz;
         }
         if (sten) {
-            const HQ q0 = (SHARE ? valid : own) ? hquant<CM, IN>(a, b, huni, i, j, k) : zq;
-            if (SHARE) {
-                sq[row][lane] = q0;
-                __syncthreads();
-            }
+            const HQ q0 = valid ? hquant<CM, IN>(a, b, huni, i, j, k) : zq;
+            sq[row][lane] = q0;
+            __syncthreads();
             if (own) {
                 // (i-1, j, k) and (i, j-1, k)
-                const HQ qi = SHARE ? sq[row][lane - 1] : (ilo && (uy || uz) ? hquant<CM, IN>(a, b, huni, i - 1, j, k) : zq);
-                const HQ qj = (SHARE && row > 0) ? sq[row - 1][lane]
+                const HQ qi = sq[row][lane - 1];
+                const HQ qj = row > 0 ? sq[row - 1][lane]
                                                  : (jlo && (ux || uz) ? hquant<CM, IN>(a, b, huni, i, j - 1, k) : zq);
                 if (ux) {
                     ebx += (qkm.tx - q0.tx) + (qj.rx - q0.rx);
@@ -335,7 +615,7 @@ __global__ void __launch_bounds__(TX * BY)
                     }
                 }
             }
-            if (SHARE) __syncthreads();
+            __syncthreads();
         }
         if (ux) b.eb[0][c] = ebx;
         if (uy) b.eb[1][c] = eby;
@@ -343,9 +623,9 @@ __global__ void __launch_bounds__(TX * BY)
     }
 }
 
-template <int CM, bool SHARE, int V>
+template <int CM, int V>
 __global__ void __launch_bounds__(TX * BY)
-    adj_h_kernel(const KArgs a, const AArgs b, const int euni, const int huni)
+    adj_h_lds_kernel(const KArgs a, const AArgs b, const int euni, const int huni)
 {
     constexpr bool IN = V == V_IN;   // interior tiles: no slab, face or range predicates
     __shared__ EQ sq[BY][TX];
@@ -353,12 +633,12 @@ __global__ void __launch_bounds__(TX * BY)
     tile_of_region<V>(a, tx, ty, tz);
     const int lane = (int)(threadIdx.x & (TX - 1));
     const int row = (int)(threadIdx.x / TX);
-    // SHARE: lane 63 is the halo column after the tile
-    const int i = SHARE ? tx * TXA + lane : tx * TX + lane;
+    // lane 63 is the halo column after the tile
+    const int i = tx * TXA + lane;
     const int j = ty * BY + row;
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
     const bool valid = IN || (i < nx && j < ny);
-    const bool own = valid && (!SHARE || lane < TX - 1);
+    const bool own = valid && lane < TX - 1;
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
     const bool ihi = IN || i < nx - 1, jhi = IN || j < ny - 1;
@@ -367,7 +647,7 @@ __global__ void __launch_bounds__(TX * BY)
     const float4 pym = ys ? a.py_prof[ny + j] : NOPROF;
     const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
     const EQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    EQ q0 = (SHARE ? valid : own) ? equant<CM, IN>(a, b, euni, i, j, k0) : zq;
+    EQ q0 = valid ? equant<CM, IN>(a, b, euni, i, j, k0) : zq;
     // lumped-source edges among the Ez edges this thread gathers from: its
     // own cell, (i+1, j) and (i, j+1); event cursor and its plane per column.
     // Rows outside the source rows skip the lookups (no load: a kernel
@@ -384,7 +664,7 @@ __global__ void __launch_bounds__(TX * BY)
         const i64 c = (i64)k * a.plane + (i64)j * px + i;
         const bool khi = IN || k < nz - 1;
         const bool ux = own && jhi && khi, uy = own && ihi && khi, uz = own && ihi && jhi;
-        const EQ qk = ((SHARE ? valid : own) && khi) ? equant<CM, IN>(a, b, euni, i, j, k + 1) : zq;
+        const EQ qk = (valid && khi) ? equant<CM, IN>(a, b, euni, i, j, k + 1) : zq;
         float hbx = 0.f, hby = 0.f, hbz = 0.f;
         if (own) {
             float4 hx, hy, hz;
@@ -414,14 +694,12 @@ __global__ void __launch_bounds__(TX * BY)
             if (uz) hbz = hz.x * hbz;
         }
         // transposed E half-step gathered from the E cells that read this H
-        if (SHARE) {
-            sq[row][lane] = q0;
-            __syncthreads();
-        }
+        sq[row][lane] = q0;
+        __syncthreads();
         if (own) {
             // (i+1, j, k) and (i, j+1, k)
-            const EQ qi = SHARE ? sq[row][lane + 1] : (ihi && (uy || uz) ? equant<CM, IN>(a, b, euni, i + 1, j, k) : zq);
-            const EQ qj = (SHARE && row < BY - 1) ? sq[row + 1][lane]
+            const EQ qi = sq[row][lane + 1];
+            const EQ qj = row < BY - 1 ? sq[row + 1][lane]
                                                   : (jhi && (ux || uz) ? equant<CM, IN>(a, b, euni, i, j + 1, k) : zq);
             if (ux) {
                 hbx += (q0.ay - qk.ay) + (q0.bz - qj.bz);
@@ -431,7 +709,7 @@ __global__ void __launch_bounds__(TX * BY)
             if (uz) hbz += q0.by - qi.by;
             if (uz && !ux) hbz += q0.ax - qj.ax;
         }
-        if (SHARE) __syncthreads();
+        __syncthreads();
         if (k == sk[0] || k == sk[1] || k == sk[2]) {
             // source edge on this cell: +terms; at (i+1, j) or (i, j+1): -terms
             for (int q = 0; q < 3; ++q)
@@ -505,27 +783,35 @@ void adj_box(const KArgs &a, int tcols, int ib[6])
 hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, const AArgs &b, int m, int fstep,
                       int sten, hipStream_t s)
 {
-    // LDS sharing costs one halo column per 64 and rounds nx up to 63-column
-    // tiles: used once the rows are wide enough for that to be small
-    const bool share = adj_shares(a0.nx);
+    const bool stored = adj_stores_quantities(a0.nx);
+    const int tcols = stored ? TX : TXA;
     KArgs a = a0;
-    a.tiles_x = share ? (a.nx + TXA - 1) / TXA : (a.nx + TX - 1) / TX;
+    a.tiles_x = (a.nx + tcols - 1) / tcols;
     if (a0.ib[0] < 0) {
         for (int q = 0; q < 6; ++q) a.ib[q] = 0;
     } else {
-        adj_box(a, share ? TXA : TX, a.ib);
+        adj_box(a, tcols, a.ib);
     }
     int n_in, n_x, n_gen;
     region_tiles(a, n_in, n_x, n_gen);
-#define ADJ_E(CMV, SH, V, NT, EU, HU) \
-    hipLaunchKernelGGL((adj_e_kernel<CMV, SH, V>), dim3(NT), dim3(TX * BY), 0, s, a, b, m, fstep, sten, EU, HU)
-#define ADJ_H(CMV, SH, V, NT, EU, HU) \
-    hipLaunchKernelGGL((adj_h_kernel<CMV, SH, V>), dim3(NT), dim3(TX * BY), 0, s, a, b, EU, HU)
-#define ADJ_V(MAC, V, NT)                                                                  \
-    do {                                                                                   \
-        if (mode == CLS_U32) { if (share) MAC(CLS_U32, true, V, NT, euni, huni); else MAC(CLS_U32, false, V, NT, euni, huni); } \
-        else if (mode == CLS_U16) { if (share) MAC(CLS_U16, true, V, NT, euni, huni); else MAC(CLS_U16, false, V, NT, euni, huni); } \
-        else { if (share) MAC(CLS_UNIFORM, true, V, NT, 1, 1); else MAC(CLS_UNIFORM, false, V, NT, 1, 1); } \
+#define ADJ_E(CMV, V, NT, EU, HU)                                                                                \
+    do {                                                                                                         \
+        if (stored)                                                                                              \
+            hipLaunchKernelGGL((adj_e_kernel<CMV, V>), dim3(NT), dim3(TX * BY), 0, s, a, b, m, fstep, sten, EU); \
+        else                                                                                                     \
+            hipLaunchKernelGGL((adj_e_lds_kernel<CMV, V>), dim3(NT), dim3(TX * BY), 0, s, a, b, m, fstep, sten, \
+                               EU, HU);                                                                          \
+    } while (0)
+#define ADJ_H(CMV, V, NT, EU, HU)                                                                            \
+    do {                                                                                                     \
+        if (stored) hipLaunchKernelGGL((adj_h_kernel<CMV, V>), dim3(NT), dim3(TX * BY), 0, s, a, b, EU, HU); \
+        else hipLaunchKernelGGL((adj_h_lds_kernel<CMV, V>), dim3(NT), dim3(TX * BY), 0, s, a, b, EU, HU);    \
+    } while (0)
+#define ADJ_V(MAC, V, NT)                                    \
+    do {                                                     \
+        if (mode == CLS_U32) MAC(CLS_U32, V, NT, euni, huni); \
+        else if (mode == CLS_U16) MAC(CLS_U16, V, NT, euni, huni); \
+        else MAC(CLS_UNIFORM, V, NT, 1, 1);                  \
     } while (0)
     if (which == 0) {
         if (n_in) ADJ_V(ADJ_E, V_IN, n_in);

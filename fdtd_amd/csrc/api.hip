@@ -29,6 +29,7 @@ hipError_t launch_half_step(bool efield, int storage, int mode, const KArgs &a, 
 hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a, const AArgs &b, int m, int fstep,
                       int sten, hipStream_t s);
 hipError_t launch_snap(int storage, const KArgs &a, float *hist, int slot, hipStream_t s);
+bool adj_stores_quantities(int nx);
 hipError_t launch_pair_half_step(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s,
                                  bool gen_cells);
 int tile_rows();
@@ -56,9 +57,12 @@ struct fdtd_ctx {
     size_t esz = 4;                 // bytes per stored value
     i64 field_elems = 0;            // per field allocation, guard planes included
     std::vector<void *> allocs;     // every device allocation
+    long n_allocs = 0;              // device allocations attempted (failure injection)
     // Field and psi allocation bases (the sweeps update them in place).
     void *fields[6] = {};
     void *psi[12] = {};
+    std::uint8_t *qexp[6] = {};     // block-scaled storage: exponent bytes per 8 x-cells, guard planes included
+    size_t qexp_bytes = 0;          // per field
     bool pair = false;              // 16-bit storage: two cells per thread in the half-step kernels
     bool pair_gen_cells = false;    // ... except the general variant, which runs one cell per thread
     int pair_tiles_x = 0, pair_kchunk = 0;
@@ -138,6 +142,12 @@ int dev_alloc(fdtd_ctx *c, void **p, size_t bytes)
 {
     *p = nullptr;
     if (bytes == 0) bytes = 256;
+    // failure injection: FDTD_FAIL_ALLOC=n makes the context's n-th device
+    // allocation fail as out of memory (tests of the error paths)
+    ++c->n_allocs;
+    if (const char *fa = std::getenv("FDTD_FAIL_ALLOC"))
+        if (std::atol(fa) == c->n_allocs)
+            return fail(c, FDTD_E_NOMEM, "hipMalloc(%zu) failed: injected (FDTD_FAIL_ALLOC=%s)", bytes, fa);
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -164,6 +174,7 @@ void free_device(fdtd_ctx *c)
     c->dev_bytes = 0;
     for (auto &f : c->fields) f = nullptr;
     for (auto &p : c->psi) p = nullptr;
+    for (auto &q : c->qexp) q = nullptr;
 }
 
 // The buffers holding the current state.
@@ -178,8 +189,10 @@ bool box_in_grid(const fdtd_ctx *c, int i, int j, int k)
 // Zero every piece of time-dependent state.
 int zero_state(fdtd_ctx *c)
 {
-    for (int f = 0; f < 6; ++f)
+    for (int f = 0; f < 6; ++f) {
         if (c->fields[f]) HIPCHK(c, hipMemsetAsync(c->fields[f], 0, (size_t)c->field_elems * c->esz, c->stream));
+        if (c->qexp[f]) HIPCHK(c, hipMemsetAsync(c->qexp[f], 0, c->qexp_bytes, c->stream));
+    }
     for (int p = 0; p < 12; ++p)
         if (c->psi_elems[p] && c->psi[p])
             HIPCHK(c, hipMemsetAsync(c->psi[p], 0, c->psi_elems[p] * c->esz, c->stream));
@@ -223,7 +236,11 @@ int choose_chunks(const fdtd_ctx *c, int tiles, int planes)
 // variant the rows and z-blocks outside it: z-edge variants of their own
 // measured slower on 512x512x256 (their launch tails cost more than their
 // leaner code saves).  FDTD_INTERIOR=0, or an axis without interior tiles,
-// leaves the box empty and every tile to the general variant.
+// leaves the box empty (z range [0, 0)) and every tile to the general
+// variant.  (On 128x128x64 an x-edge launch over the interior rows and
+// z-blocks beside a general one measured slower than the general variant
+// alone: 32.4 against 26.9 ms for 1000 steps; the two half-size launches
+// of a small grid leave CUs idle.)
 void sweep_split(fdtd_ctx *c, int sweep, int tile_cols, int kchunk)
 {
     const SceneIn &in = c->in;
@@ -290,7 +307,7 @@ int do_finalize(fdtd_ctx *c)
     // 16-bit storage sweeps a cell pair per thread (128-cell tile rows);
     // FDTD_PAIR=0 selects the one-cell kernels instead (same stored bits)
     const char *pv = std::getenv("FDTD_PAIR");
-    c->pair = in.storage != FDTD_STORE_F32 && !(pv && std::strcmp(pv, "0") == 0);
+    c->pair = (in.storage == FDTD_STORE_F16 || in.storage == FDTD_STORE_BF16) && !(pv && std::strcmp(pv, "0") == 0);
     // The pair sweeps' general variant (slab rows and z-blocks) runs as the
     // one-cell kernel (same stored bits): on 512x512x256 FP16 the step took
     // 0.772 ms against 0.788 with the pair kernel's own general variant,
@@ -314,6 +331,16 @@ int do_finalize(fdtd_ctx *c)
     for (int f = 0; f < 6; ++f) {
         int rc = dev_alloc(c, &c->fields[f], (size_t)c->field_elems * c->esz);
         if (rc) return rc;
+    }
+    if (in.storage == FDTD_STORE_BQ16) {
+        // one exponent byte per 8 x-cells (px is a multiple of 64)
+        c->qexp_bytes = (size_t)c->field_elems / 8;
+        for (int f = 0; f < 6; ++f) {
+            void *q = nullptr;
+            int rc = dev_alloc(c, &q, c->qexp_bytes);
+            if (rc) return rc;
+            c->qexp[f] = (std::uint8_t *)q;
+        }
     }
     // interior / general split of the two sweeps
     {
@@ -344,10 +371,6 @@ int do_finalize(fdtd_ctx *c)
     a.h_first = (const int *)tmp;
     if ((rc = upload(c, &tmp, b.events.data(), b.events.size() * sizeof(Event)))) return rc;
     a.events = (const Event *)tmp;
-    if ((rc = upload(c, &tmp, b.s_first.data(), b.s_first.size() * sizeof(int)))) return rc;
-    a.s_first = (const int *)tmp;
-    if ((rc = upload(c, &tmp, b.s_events.data(), b.s_events.size() * sizeof(Event)))) return rc;
-    a.s_events = (const Event *)tmp;
     if ((rc = upload(c, &tmp, b.vpn.data(), b.vpn.size() * sizeof(float)))) return rc;
     a.vpn = (const float *)tmp;
     if ((rc = upload(c, &tmp, b.phasor.data(), b.phasor.size() * sizeof(double)))) return rc;
@@ -409,6 +432,8 @@ void bind_sets(const fdtd_ctx *c, KArgs &a)
     const size_t g = (size_t)c->ka.plane * c->esz;   // guard plane
     void **rd[6] = {&a.ex, &a.ey, &a.ez, &a.hx, &a.hy, &a.hz};
     for (int f = 0; f < 6; ++f) *rd[f] = (char *)c->fields[f] + g;
+    std::uint8_t **qd[6] = {&a.qex, &a.qey, &a.qez, &a.qhx, &a.qhy, &a.qhz};
+    for (int f = 0; f < 6; ++f) *qd[f] = c->qexp[f] ? c->qexp[f] + (size_t)c->ka.plane / 8 : nullptr;
     void **pr[12] = {&a.psi_eyx, &a.psi_ezx, &a.psi_hyx, &a.psi_hzx, &a.psi_exy, &a.psi_ezy,
                      &a.psi_hxy, &a.psi_hzy, &a.psi_exz, &a.psi_eyz, &a.psi_hxz, &a.psi_hyz};
     for (int p = 0; p < 12; ++p) *pr[p] = c->psi[p];
@@ -614,6 +639,8 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     if (!c->finalized) return fail(c, FDTD_E_STATE, "fdtd_finalize has not been called");
     const SceneIn &in = c->in;
     if (in.design_eps.empty() || !c->dcoef) return fail(c, FDTD_E_STATE, "no design region");
+    if (in.storage == FDTD_STORE_BQ16)
+        return fail(c, FDTD_E_UNSUPPORTED, "the adjoint runs with FP32, FP16 or BF16 storage, not block-scaled int16");
     if (n_steps < 1 || n_steps > in.steps_cap) return fail(c, FDTD_E_RANGE, "n_steps outside [1, %d]", in.steps_cap);
     const int *d = in.design;
     const int dnx = d[3] - d[0], dny = d[4] - d[1], dnz = d[5] - d[2];
@@ -642,18 +669,29 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
             return fail(c, FDTD_E_INVALID, "objective probe %d lies in the CPML or on a face", p);
     }
     const int n = n_steps;
-    int Kseg = K > 0 ? std::min(K, n) : n;
-    const int nseg = (n + Kseg - 1) / Kseg;
     const int bx = dnx + 1, by = dny + 1, bz = dnz + 1;
     const i64 nb = (i64)bx * by * bz;
-    const size_t hist_bytes = (size_t)(Kseg + 1) * 3 * nb * sizeof(float);
-    const size_t ck_bytes = state_bytes(c);
-    size_t freeb = 0, totb = 0;
-    const size_t adj_bytes = 6 * (size_t)c->field_elems * 4 + [&] {
+    // adjoint fields (6) and, on narrow grids, every cell's transposed
+    // quantities (12 arrays), guard planes included
+    const int nq = adj_stores_quantities(nx) ? 12 : 0;
+    const size_t adj_bytes = (6 + nq) * (size_t)c->field_elems * 4 + [&] {
         size_t t = 0;
         for (int q = 0; q < 12; ++q) t += c->psi_elems[q] * 4;
         return t;
     }();
+    size_t freeb = 0, totb = 0;
+    if (K < 0) {
+        // automatic: the design history of the whole run when it fits in half
+        // the free device memory (no recompute), else checkpoints every
+        // ceil(sqrt(n)) steps
+        HIPCHK(c, hipMemGetInfo(&freeb, &totb));
+        const size_t all = (size_t)(n + 1) * 3 * nb * sizeof(float) + adj_bytes + (size_t)std::max(np, 1) * n * 4;
+        K = all < freeb / 2 ? 0 : (int)std::ceil(std::sqrt((double)n));
+    }
+    int Kseg = K > 0 ? std::min(K, n) : n;
+    const int nseg = (n + Kseg - 1) / Kseg;
+    const size_t hist_bytes = (size_t)(Kseg + 1) * 3 * nb * sizeof(float);
+    const size_t ck_bytes = state_bytes(c);
     // segmented runs double-buffer the history: segment s-1 is recomputed on
     // a second stream while the reverse sweep of segment s runs
     const int nhist = nseg > 1 ? 2 : 1;
@@ -662,7 +700,7 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
             return fail(c, FDTD_E_INVALID, "the no-allocation adjoint stores the design history of the whole run: "
                         "checkpoint_every must be 0 or >= n_steps");
         // exactly the pieces carved below, each rounded to 256 bytes
-        size_t carve = 6 * DevBuf::rounded((size_t)c->field_elems * 4) + DevBuf::rounded(hist_bytes) +
+        size_t carve = (6 + nq) * DevBuf::rounded((size_t)c->field_elems * 4) + DevBuf::rounded(hist_bytes) +
                        DevBuf::rounded(3 * nb * sizeof(double)) +
                        DevBuf::rounded((size_t)std::max(np, 1) * n * sizeof(float));
         for (int q = 0; q < 12; ++q) carve += DevBuf::rounded(std::max<size_t>(c->psi_elems[q], 1) * 4);
@@ -691,6 +729,12 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
         HIPCHK(c, hipMemsetAsync(h, 0, (size_t)c->field_elems * 4, c->stream));
         b.eb[q] = (float *)(e + (size_t)c->ka.plane * 4);
         b.hb[q] = (float *)(h + (size_t)c->ka.plane * 4);
+    }
+    for (int q = 0; q < nq; ++q) {
+        char *t = (char *)mem.get((size_t)c->field_elems * 4);
+        if (!t) return fail(c, FDTD_E_NOMEM, "adjoint transposed-quantity allocation failed");
+        HIPCHK(c, hipMemsetAsync(t, 0, (size_t)c->field_elems * 4, c->stream));
+        (q < 6 ? b.hq[q] : b.eq[q - 6]) = (float *)(t + (size_t)c->ka.plane * 4);
     }
     for (int q = 0; q < 12; ++q) {
         b.pb[q] = (float *)mem.get(std::max<size_t>(c->psi_elems[q], 1) * 4);
@@ -721,9 +765,15 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     fa.sb[0] = d[0]; fa.sb[1] = d[1]; fa.sb[2] = d[2]; fa.sb[3] = bx; fa.sb[4] = by; fa.sb[5] = bz;
     KArgs aa = c->ka;               // adjoint arguments: SI tables
     if (const char *ov = std::getenv("FDTD_ADJ_CHUNKS")) {
-        // z-blocks of the adjoint sweeps (default: those of the forward E sweep)
         const int nch = std::max(1, std::min(std::atoi(ov), nz));
         aa.kchunk = (nz + nch - 1) / nch;
+    } else {
+        // z-blocks of the adjoint sweeps: those of the forward E sweep, halved
+        // until a launch has 2048 workgroups (8 per CU: the reverse sweeps of a
+        // small grid are latency-bound).  Measured on 128x128x64, 1000 steps,
+        // stored history: 4-plane blocks 83.0 ms, 2-plane 78.3, 1-plane 78.2.
+        const int tiles = ((nx + tile_cols() - 1) / tile_cols()) * ((ny + tile_rows() - 1) / tile_rows());
+        while (aa.kchunk > 1 && (long)tiles * ((nz + aa.kchunk - 1) / aa.kchunk) < 2048) aa.kchunk /= 2;
     }
     aa.etab = c->etab_si;
     aa.htab = c->htab_si;
@@ -870,7 +920,7 @@ int fdtd_create(const fdtd_grid *g, int storage, fdtd_ctx **out)
     if (!g) return FDTD_E_INVALID;
     if (g->nx < 4 || g->ny < 4 || g->nz < 4) return FDTD_E_INVALID;
     if (!(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0)) return FDTD_E_INVALID;
-    if (storage < FDTD_STORE_F32 || storage > FDTD_STORE_BF16) return FDTD_E_INVALID;
+    if (storage < FDTD_STORE_F32 || storage > FDTD_STORE_BQ16) return FDTD_E_INVALID;
     double eps0 = g->eps0 > 0 ? g->eps0 : 8.854187817e-12;
     double mu0 = g->mu0 > 0 ? g->mu0 : 4.0 * M_PI * 1e-7;
     double dt = g->dt;
@@ -1160,16 +1210,19 @@ int fdtd_get_info(const fdtd_ctx *c, fdtd_info *o)
     o->e_kernel_ms = c->e_ms;
     o->h_kernel_ms = c->h_ms;
     o->step_kernel_ms = c->step_ms;
-    o->interior_split = c->box[0][3] > c->box[0][0] ? 1 : 0;
+    o->interior_split = c->box[0][5] > c->box[0][2] ? 1 : 0;
     o->device_bytes = c->dev_bytes;
     // algorithmic bytes per cell: E sweep reads E, H and writes E (9 values);
     // H sweep likewise; plus one class index per cell in a classed sweep and
     // psi read+write over the slabs (averaged over the grid).
-    const double s = (double)c->esz;
+    // block-scaled storage: 2 bytes per value plus one exponent byte per 8;
+    // its psi is FP16
+    const double s = c->in.storage == FDTD_STORE_BQ16 ? 2.125 : (double)c->esz;
+    const double sp = (double)c->esz;
     const double fx = (double)(c->in.cpml.thick[0] + c->in.cpml.thick[1]) / c->in.nx;
     const double fy = (double)(c->in.cpml.thick[2] + c->in.cpml.thick[3]) / c->in.ny;
     const double fz = (double)(c->in.cpml.thick[4] + c->in.cpml.thick[5]) / c->in.nz;
-    const double psi = 4.0 * s * (fx + fy + fz);
+    const double psi = 4.0 * sp * (fx + fy + fz);
     const double cls = c->built.mode == CLS_U16 ? 2.0 : c->built.mode == CLS_U32 ? 4.0 : 0.0;
     o->adjoint_forward_ms = c->adj_fwd_ms;
     o->adjoint_total_ms = c->adj_total_ms;
@@ -1279,6 +1332,11 @@ int fdtd_get_field(fdtd_ctx *c, int comp, float *out, size_t n)
     std::vector<unsigned char> raw((size_t)px * ny * nz * c->esz);
     const char *base = (const char *)c->fields[comp] + (size_t)c->ka.plane * c->esz;
     HIPCHK(c, hipMemcpy(raw.data(), base, raw.size(), hipMemcpyDeviceToHost));
+    std::vector<std::uint8_t> qraw;
+    if (c->qexp[comp]) {
+        qraw.resize((size_t)px * ny * nz / 8);
+        HIPCHK(c, hipMemcpy(qraw.data(), c->qexp[comp] + (size_t)c->ka.plane / 8, qraw.size(), hipMemcpyDeviceToHost));
+    }
     const float un = c->built.probe_unscale[comp];
     for (size_t r = 0; r < (size_t)ny * nz; ++r)
         for (int i = 0; i < nx; ++i) {
@@ -1292,6 +1350,8 @@ int fdtd_get_field(fdtd_ctx *c, int comp, float *out, size_t n)
                 if (c->in.storage == FDTD_STORE_BF16) {
                     uint32_t u = (uint32_t)h << 16;
                     std::memcpy(&v, &u, 4);
+                } else if (c->in.storage == FDTD_STORE_BQ16) {
+                    v = std::ldexp((float)(int16_t)h, (int)qraw[src / 8] - 128);
                 } else {
                     _Float16 f;
                     std::memcpy(&f, &h, 2);
@@ -1308,7 +1368,7 @@ int fdtd_checkpoint_size(const fdtd_ctx *c, size_t *bytes)
 {
     if (!c || !bytes) return FDTD_E_INVALID;
     if (!c->finalized) return FDTD_E_STATE;
-    size_t p = 6 * (size_t)c->field_elems * c->esz;
+    size_t p = 6 * ((size_t)c->field_elems * c->esz + c->qexp_bytes);
     for (int q = 0; q < 12; ++q) p += c->psi_elems[q] * c->esz;
     p += c->in.probes.size() * (size_t)c->in.steps_cap * sizeof(float);
     p += c->in.probes.size() * c->in.freqs.size() * sizeof(double2);
@@ -1331,7 +1391,8 @@ int fdtd_checkpoint_save(fdtd_ctx *c, void *buf, size_t bytes)
         return FDTD_OK;
     };
     for (int f = 0; f < 6; ++f)
-        if ((rc = get(fld(c, f), (size_t)c->field_elems * c->esz))) return rc;
+        if ((rc = get(fld(c, f), (size_t)c->field_elems * c->esz)) || (rc = get(c->qexp[f], c->qexp_bytes)))
+            return rc;
     for (int q = 0; q < 12; ++q)
         if ((rc = get(psv(c, q), c->psi_elems[q] * c->esz))) return rc;
     if ((rc = get(c->ka.probe_hist, c->in.probes.size() * (size_t)c->in.steps_cap * sizeof(float)))) return rc;
@@ -1375,7 +1436,8 @@ int fdtd_checkpoint_restore(fdtd_ctx *c, const void *buf, size_t bytes)
         return FDTD_OK;
     };
     for (int f = 0; f < 6; ++f)
-        if ((rc = put(fld(c, f), (size_t)c->field_elems * c->esz))) return rc;
+        if ((rc = put(fld(c, f), (size_t)c->field_elems * c->esz)) || (rc = put(c->qexp[f], c->qexp_bytes)))
+            return rc;
     for (int q = 0; q < 12; ++q)
         if ((rc = put(psv(c, q), c->psi_elems[q] * c->esz))) return rc;
     if ((rc = put(c->ka.probe_hist, c->in.probes.size() * (size_t)c->in.steps_cap * sizeof(float)))) return rc;

@@ -31,9 +31,30 @@ __device__ __forceinline__ void st(bf16_t *p, float v)
     p->v = (std::uint16_t)(u >> 16);
 }
 
+// Block-scaled 16-bit storage of the fields: an int16 mantissa per value and
+// one power-of-two exponent byte per aligned run of BQ_BLOCK cells along x,
+// value = mantissa * 2^(byte - BQ_BIAS).  The exponent array of a field has
+// the field's layout with x divided by BQ_BLOCK, so the byte of the value at
+// byte offset o (2-byte elements) is at o / (2 * BQ_BLOCK).  psi is stored
+// in FP16 (psi_type).
+struct bq16_t {
+    std::int16_t v;
+};
+constexpr int BQ_BLOCK = 8;
+constexpr int BQ_BIAS = 128;
+
+template <class T> struct psi_type { typedef T type; };
+template <> struct psi_type<bq16_t> { typedef _Float16 type; };
+template <class T> struct is_bq { static constexpr bool value = false; };
+template <> struct is_bq<bq16_t> { static constexpr bool value = true; };
+
 // v rounded to the storage type and back (what the next read will see)
 __device__ __forceinline__ float rnd(float v, const float *) { return v; }
 __device__ __forceinline__ float rnd(float v, const _Float16 *) { return (float)(_Float16)v; }
+// block-scaled storage: the stored value depends on the whole block, which
+// one lane does not see here (the adjoint, the only caller, refuses this
+// storage type)
+__device__ __forceinline__ float rnd(float v, const bq16_t *) { return v; }
 __device__ __forceinline__ float rnd(float v, const bf16_t *)
 {
     std::uint32_t u = __float_as_uint(v);
@@ -115,7 +136,7 @@ __device__ __forceinline__ void tile_of_region(const KArgs &a, int &tx, int &ty,
         tx = xe < ib[0] ? xe : ib[3] + (xe - ib[0]);
         ty = ib[1] + (t / ex) % by;
         tz = ib[2] + t / (ex * by);
-    } else if (ib[3] == ib[0]) {
+    } else if (ib[5] == ib[2]) {
         tile_of(a, tx, ty, tz);
     } else {
         const int txy = a.tiles_x * a.tiles_y, tzs = (a.nz + a.kchunk - 1) / a.kchunk;

@@ -23,12 +23,6 @@ struct Event {
 };
 enum { EV_SOURCE = 0, EV_PROBE = 8 };
 
-// Rows of a thread's strip in the one-kernel step.  The step kernel keys its
-// events by (column, strip) with the row in bits 8.. of `kind`; a source on
-// the first row of a strip is also listed, as row STRIP_R, in the strip
-// below (whose halo row recomputes it).
-constexpr int STRIP_R = 2;
-
 // Per source edge: Ez coefficients with the resistive load folded in
 // (defineSourceZ, fdtdrav.h:1402-1421) and the waveform row.
 struct SrcEdge {
@@ -46,6 +40,9 @@ enum SweepRegion { V_IN = 0, V_X = 1, V_Y = 2, V_Z = 4, V_GEN = 7 };
 // k = nz-1, so every i+-1, j+-1, k+-1 neighbour address is inside it.
 struct KArgs {
     void *ex, *ey, *ez, *hx, *hy, *hz;
+    // block-scaled storage: the fields' exponent bytes, one per 8 x-cells,
+    // [nz+2][ny][px/8] pointed at plane k = 0 (null for other storage types)
+    std::uint8_t *qex, *qey, *qez, *qhx, *qhy, *qhz;
     // psi slabs: x [nz][ny][Lx], y [nz][Ly][px], z [Lz][ny][px]
     void *psi_eyx, *psi_ezx, *psi_hyx, *psi_hzx;
     void *psi_exy, *psi_ezy, *psi_hxy, *psi_hzy;
@@ -58,8 +55,6 @@ struct KArgs {
     const float4 *px_prof, *py_prof, *pz_prof;
     const int *e_first, *h_first;   // per column: first event or -1
     const Event *events;
-    const int *s_first;             // per (strip, column): first strip event or -1
-    const Event *s_events;
     const SrcEdge *srcs;
     const float *vpn;          // [wave][steps_cap]
     float *probe_hist;         // [probe][steps_cap]
@@ -96,7 +91,7 @@ inline void region_tiles(const KArgs &a, int &n_in, int &n_x, int &n_gen)
 {
     const int *ib = a.ib;
     const int tzs = (a.nz + a.kchunk - 1) / a.kchunk;
-    if (ib[3] == ib[0]) {
+    if (ib[5] == ib[2]) {   // no box
         n_in = n_x = 0;
         n_gen = a.tiles_x * a.tiles_y * tzs;
         return;
@@ -110,6 +105,11 @@ inline void region_tiles(const KArgs &a, int &n_in, int &n_x, int &n_gen)
 // Adjoint argument block (FP32 SI adjoint state; same layouts as the forward).
 struct AArgs {
     float *eb[3], *hb[3];      // dJ/dE, dJ/dH (guard-plane layout of the fields)
+    // transposed quantities of every cell, same layout: the factors its
+    // adjoint contributes to each field difference it reads, written by the
+    // sweep that updates it, gathered by the other one
+    float *hq[6];              // H cells: sy sz tz tx rx ry
+    float *eq[6];              // E cells: ax bx ay by az bz
     float *pb[12];             // dJ/dpsi: eyx ezx hyx hzx exy ezy hxy hzy exz eyz hxz hyz
     const float *g;            // [probe][gsteps] dJ/d(probe sample m)
     int gsteps;
@@ -166,8 +166,6 @@ struct Built {
     std::vector<float> prof[3];     // [2n][4] per axis
     std::vector<int> e_first, h_first;
     std::vector<Event> events;
-    std::vector<int> s_first;       // [ny/STRIP_R][nx] first strip event or -1
-    std::vector<Event> s_events;
     std::vector<SrcEdge> srcs;
     std::vector<SrcEdge> srcs_si;   // SI coefficients (adjoint)
     int src_j0 = 1, src_j1 = 0;     // rows j whose row j or j+1 holds a source edge: [src_j0, src_j1]

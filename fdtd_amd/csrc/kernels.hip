@@ -62,6 +62,7 @@ struct ECol {
     std::uint32_t o0;    // byte offset of (i, j, 0) from the guard-plane base
     std::uint32_t ob;    // bytes per plane
     bool ilo, ihi, jlo, jhi, xs, ys, in_snap;
+    bool valid;          // i < nx (block-scaled storage keeps the lanes past nx alive)
     float4 pxe, pye;
 };
 
@@ -79,23 +80,25 @@ __device__ __forceinline__ bool e_zslab(const KArgs &a, int k)
 
 template <class T, int CM, int V>
 __device__ __forceinline__ void e_fetch(EPlane &p, const KArgs &a, const ECol &cc, const T *ex, const T *ey,
-                                        const T *ez, const T *hx, const T *hy, const T *hz, int k, int k1)
+                                        const T *ez, const T *hx, const T *hy, const T *hz, const QPtrs &qp, int k,
+                                        int k1)
 {
+    typedef typename psi_type<T>::type PT;
     ConstKArgs &ra = rare_args(a);
     constexpr std::uint32_t S = sizeof(T);
     const std::uint32_t o = cc.o0 + (std::uint32_t)k * cc.ob;
     const std::uint32_t ojm = o - (std::uint32_t)a.px * S;
     const int px = a.px, ny = a.ny;
-    p.hxc = ld(at(hx, o));
-    p.hyc = ld(at(hy, o));
-    p.hzc = ld(at(hz, o));
-    p.hy_im = ld(at(hy, o - S));
-    p.hz_im = ld(at(hz, o - S));
-    p.hx_jm = ld(at(hx, ojm));
-    p.hz_jm = ld(at(hz, ojm));
-    p.ex0 = ldn(at(ex, o));
-    p.ey0 = ldn(at(ey, o));
-    p.ez0 = ldn(at(ez, o));
+    p.hxc = fld(hx, qp.hx, o);
+    p.hyc = fld(hy, qp.hy, o);
+    p.hzc = fld(hz, qp.hz, o);
+    p.hy_im = fld(hy, qp.hy, o - S);
+    p.hz_im = fld(hz, qp.hz, o - S);
+    p.hx_jm = fld(hx, qp.hx, ojm);
+    p.hz_jm = fld(hz, qp.hz, ojm);
+    p.ex0 = fldn(ex, qp.ex, o);
+    p.ey0 = fldn(ey, qp.ey, o);
+    p.ez0 = fldn(ez, qp.ez, o);
     // class map (no guard planes): the entry of plane k+1
     p.cln = (CM != CLS_UNIFORM && k + 1 < k1)
                 ? class_at<CM>(a.cls, (std::uint32_t)(k + 1) * (std::uint32_t)a.plane + (std::uint32_t)cc.col)
@@ -104,27 +107,28 @@ __device__ __forceinline__ void e_fetch(EPlane &p, const KArgs &a, const ECol &c
     p.p_yx = p.p_zx = p.p_xy = p.p_zy = p.p_xz = p.p_yz = 0.f;
     if (cc.xs && cc.jhi && khi) {
         const std::uint32_t q = (((std::uint32_t)k * ny + cc.j) * ra.Lx + cc.sx) * S;
-        p.p_yx = ld(at((const T *)ra.psi_eyx, q));
-        p.p_zx = ld(at((const T *)ra.psi_ezx, q));
+        p.p_yx = ld(at((const PT *)ra.psi_eyx, q));
+        p.p_zx = ld(at((const PT *)ra.psi_ezx, q));
     }
     if (cc.ys && cc.ihi && khi) {
         const std::uint32_t q = (((std::uint32_t)k * ra.Ly + cc.sy) * px + cc.i) * S;
-        p.p_xy = ld(at((const T *)ra.psi_exy, q));
-        p.p_zy = ld(at((const T *)ra.psi_ezy, q));
+        p.p_xy = ld(at((const PT *)ra.psi_exy, q));
+        p.p_zy = ld(at((const PT *)ra.psi_ezy, q));
     }
     if ((V & V_Z) && e_zslab(a, k) && cc.ihi && cc.jhi) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
         const std::uint32_t q = (((std::uint32_t)sz * ny + cc.j) * px + cc.i) * S;
-        p.p_xz = ld(at((const T *)ra.psi_exz, q));
-        p.p_yz = ld(at((const T *)ra.psi_eyz, q));
+        p.p_xz = ld(at((const PT *)ra.psi_exz, q));
+        p.p_yz = ld(at((const PT *)ra.psi_eyz, q));
     }
 }
 
 template <class T, int CM, int V>
 __device__ __forceinline__ void e_update(const EPlane &p, const Coef &t, const KArgs &a, const ECol &cc, T *ex,
-                                         T *ey, T *ez, int k, int m, float &hx_km, float &hy_km, int &ev,
-                                         int &ek)
+                                         T *ey, T *ez, const QPtrs &qp, int k, int m, float &hx_km, float &hy_km,
+                                         int &ev, int &ek)
 {
+    typedef typename psi_type<T>::type PT;
     ConstKArgs &ra = rare_args(a);
     const int nz = a.nz;
     constexpr std::uint32_t S = sizeof(T);
@@ -171,8 +175,8 @@ __device__ __forceinline__ void e_update(const EPlane &p, const Coef &t, const K
         const std::uint32_t q = (((std::uint32_t)k * ra.ny + cc.j) * ra.Lx + cc.sx) * S;
         const float pey = cc.pxe.x * p.p_yx + cc.pxe.y * (hzc - hz_im);
         const float pez = cc.pxe.x * p.p_zx + cc.pxe.y * (hyc - hy_im);
-        st(at((T *)ra.psi_eyx, q), pey);
-        st(at((T *)ra.psi_ezx, q), pez);
+        st(at((PT *)ra.psi_eyx, q), pey);
+        st(at((PT *)ra.psi_ezx, q), pez);
         if (klo) eyn = eyn + (cy.z * dx) * pey;
         if (cc.jlo) ezn = ezn + (cz.y * dx) * pez;
     }
@@ -180,8 +184,8 @@ __device__ __forceinline__ void e_update(const EPlane &p, const Coef &t, const K
         const std::uint32_t q = (((std::uint32_t)k * ra.Ly + cc.sy) * ra.px + cc.i) * S;
         const float pex = cc.pye.x * p.p_xy + cc.pye.y * (hzc - hz_jm);
         const float pez = cc.pye.x * p.p_zy + cc.pye.y * (hxc - hx_jm);
-        st(at((T *)ra.psi_exy, q), pex);
-        st(at((T *)ra.psi_ezy, q), pez);
+        st(at((PT *)ra.psi_exy, q), pex);
+        st(at((PT *)ra.psi_ezy, q), pez);
         if (klo) exn = exn + (cx.y * dy) * pex;
         if (cc.ilo) ezn = ezn + (cz.z * dy) * pez;
     }
@@ -190,14 +194,14 @@ __device__ __forceinline__ void e_update(const EPlane &p, const Coef &t, const K
         const std::uint32_t q = (((std::uint32_t)sz * ra.ny + cc.j) * ra.px + cc.i) * S;
         const float pex = pze.x * p.p_xz + pze.y * (hyc - hy_km);
         const float pey = pze.x * p.p_yz + pze.y * (hxc - hx_km);
-        st(at((T *)ra.psi_exz, q), pex);
-        st(at((T *)ra.psi_eyz, q), pey);
+        st(at((PT *)ra.psi_exz, q), pex);
+        st(at((PT *)ra.psi_eyz, q), pey);
         if (cc.jlo) exn = exn + (cx.z * dz) * pex;
         if (cc.ilo) eyn = eyn + (cy.y * dz) * pey;
     }
-    if (upd_x) stn(at(ex, o), exn);
-    if (upd_y) stn(at(ey, o), eyn);
-    if (upd_z) stn(at(ez, o), ezn);
+    fst(ex, qp.ex, o, upd_x, exn, ex0, cc.valid);
+    fst(ey, qp.ey, o, upd_y, eyn, ey0, cc.valid);
+    fst(ez, qp.ez, o, upd_z, ezn, ez0, cc.valid);
     if (cc.in_snap && k >= a.sb[2] && k < a.sb[2] + a.sb[5])   // stored-precision value, as the next step reads it
         e_snapshot(ra, cc.i, cc.j, k, m, upd_x ? rnd(exn, ex) : ex0, upd_y ? rnd(eyn, ex) : ey0,
                    upd_z ? rnd(ezn, ex) : ez0);
@@ -215,13 +219,18 @@ __global__ void __launch_bounds__(TX * BY) E_OCC e_kernel(const KArgs a, const i
     cc.j = ty * BY + (int)(threadIdx.x / TX);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int i = cc.i, j = cc.j;
-    if (i >= nx || j >= ny) return;
+    // block-scaled storage encodes 8-cell blocks across lanes: lanes past nx
+    // (inside the padded pitch) stay, storing nothing
+    if (j >= ny || (!is_bq<T>::value && i >= nx)) return;
+    cc.valid = !is_bq<T>::value || i < nx;
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
     cc.col = (i64)j * a.px + i;
     cc.mycol = j * nx + i;
     cc.ob = (std::uint32_t)a.plane * (std::uint32_t)sizeof(T);
     cc.o0 = (std::uint32_t)cc.col * (std::uint32_t)sizeof(T) + cc.ob;
+    const QPtrs qp = {a.qex - a.plane / BQ_BLOCK, a.qey - a.plane / BQ_BLOCK, a.qez - a.plane / BQ_BLOCK,
+                      a.qhx - a.plane / BQ_BLOCK, a.qhy - a.plane / BQ_BLOCK, a.qhz - a.plane / BQ_BLOCK};
 
     // allocation bases (guard plane below k = 0)
     T *__restrict__ ex = (T *)a.ex - a.plane;
@@ -237,7 +246,7 @@ __global__ void __launch_bounds__(TX * BY) E_OCC e_kernel(const KArgs a, const i
     cc.jlo = !(V & V_Y) || j > 0;
     cc.jhi = !(V & V_Y) || j < ny - 1;
     cc.xs = (V & V_X) && ((cc.ilo && i < a.L[0]) || (i >= nx - a.L[1] && cc.ihi));   // E x-slab, fdtdrav.h:752,777
-    cc.ys = (V & V_Y) && ((cc.jlo && j < a.L[2]) || (j >= ny - a.L[3] && cc.jhi));   // E y-slab, fdtdrav.h:803,827
+    cc.ys = (V & V_Y) && cc.valid && ((cc.jlo && j < a.L[2]) || (j >= ny - a.L[3] && cc.jhi));   // E y-slab, fdtdrav.h:803,827
     cc.sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
     cc.sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
     cc.pxe = cc.xs ? a.px_prof[i] : make_float4(0.f, 0.f, 1.f, 0.f);
@@ -245,11 +254,11 @@ __global__ void __launch_bounds__(TX * BY) E_OCC e_kernel(const KArgs a, const i
     cc.in_snap = a.snap != nullptr && i >= a.sb[0] && i < a.sb[0] + a.sb[3] && j >= a.sb[1] &&
                  j < a.sb[1] + a.sb[4];
 
-    int ev = first_event(a, a.e_first, cc.mycol, k0);
+    int ev = cc.valid ? first_event(a, a.e_first, cc.mycol, k0) : -1;
     int ek = first_k(a, ev, cc.mycol);
 
-    float hx_km = ld(at(hx, cc.o0 + (std::uint32_t)(k0 - 1) * cc.ob));
-    float hy_km = ld(at(hy, cc.o0 + (std::uint32_t)(k0 - 1) * cc.ob));
+    float hx_km = fld(hx, qp.hx, cc.o0 + (std::uint32_t)(k0 - 1) * cc.ob);
+    float hy_km = fld(hy, qp.hy, cc.o0 + (std::uint32_t)(k0 - 1) * cc.ob);
 
     EPlane A, B;
     Coef tA, tB;
@@ -259,26 +268,26 @@ __global__ void __launch_bounds__(TX * BY) E_OCC e_kernel(const KArgs a, const i
     } else {
         tA = coef_of<CM>(a.etab, class_at<CM>(a.cls, (std::uint32_t)k0 * (std::uint32_t)a.plane + (std::uint32_t)cc.col));
     }
-    e_fetch<T, CM, V>(A, a, cc, ex, ey, ez, hx, hy, hz, k0, k1);
+    e_fetch<T, CM, V>(A, a, cc, ex, ey, ez, hx, hy, hz, qp, k0, k1);
     int k = k0;
     // Steady state: the fetches are unconditional, so on every path the
     // next plane's loads are the younger ones when this plane's are awaited.
     for (; k + 2 < k1; k += 2) {
-        e_fetch<T, CM, V>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        e_fetch<T, CM, V>(B, a, cc, ex, ey, ez, hx, hy, hz, qp, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_of<CM>(a.etab, A.cln);
-        e_update<T, CM, V>(A, tA, a, cc, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
-        e_fetch<T, CM, V>(A, a, cc, ex, ey, ez, hx, hy, hz, k + 2, k1);
+        e_update<T, CM, V>(A, tA, a, cc, ex, ey, ez, qp, k, m, hx_km, hy_km, ev, ek);
+        e_fetch<T, CM, V>(A, a, cc, ex, ey, ez, hx, hy, hz, qp, k + 2, k1);
         if (CM != CLS_UNIFORM) tA = coef_of<CM>(a.etab, B.cln);
-        e_update<T, CM, V>(B, tB, a, cc, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
+        e_update<T, CM, V>(B, tB, a, cc, ex, ey, ez, qp, k + 1, m, hx_km, hy_km, ev, ek);
     }
     // last one or two planes of the chunk
     if (k + 1 < k1) {
-        e_fetch<T, CM, V>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        e_fetch<T, CM, V>(B, a, cc, ex, ey, ez, hx, hy, hz, qp, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_of<CM>(a.etab, A.cln);
-        e_update<T, CM, V>(A, tA, a, cc, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
-        e_update<T, CM, V>(B, tB, a, cc, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
+        e_update<T, CM, V>(A, tA, a, cc, ex, ey, ez, qp, k, m, hx_km, hy_km, ev, ek);
+        e_update<T, CM, V>(B, tB, a, cc, ex, ey, ez, qp, k + 1, m, hx_km, hy_km, ev, ek);
     } else {
-        e_update<T, CM, V>(A, tA, a, cc, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
+        e_update<T, CM, V>(A, tA, a, cc, ex, ey, ez, qp, k, m, hx_km, hy_km, ev, ek);
     }
 }
 
@@ -290,6 +299,7 @@ struct HCol {
     std::uint32_t o0;    // byte offset of (i, j, 0) from the guard-plane base
     std::uint32_t ob;    // bytes per plane
     bool ihi, jhi, xs, ys;
+    bool valid;          // i < nx (block-scaled storage keeps the lanes past nx alive)
     float4 pxm, pym;
 };
 
@@ -308,50 +318,54 @@ __device__ __forceinline__ bool h_zslab(const KArgs &a, int k)
 
 template <class T, int CM, int V>
 __device__ __forceinline__ void h_fetch(HPlane &p, const KArgs &a, const HCol &cc, const T *ex, const T *ey,
-                                        const T *ez, const T *hx, const T *hy, const T *hz, int k, int k1)
+                                        const T *ez, const T *hx, const T *hy, const T *hz, const QPtrs &qp, int k,
+                                        int k1)
 {
+    typedef typename psi_type<T>::type PT;
     ConstKArgs &ra = rare_args(a);
     constexpr std::uint32_t S = sizeof(T);
     const std::uint32_t o = cc.o0 + (std::uint32_t)k * cc.ob;
     const std::uint32_t okp = o + cc.ob;
     const std::uint32_t ojp = o + (std::uint32_t)a.px * S;
     const int px = a.px, ny = a.ny;
-    p.ex_kp = ld(at(ex, okp));
-    p.ey_kp = ld(at(ey, okp));
-    p.ezc = ld(at(ez, o));
-    p.ez_ip = ld(at(ez, o + S));
-    p.ey_ip = ld(at(ey, o + S));
-    p.ez_jp = ld(at(ez, ojp));
-    p.ex_jp = ld(at(ex, ojp));
-    p.hx0 = ldn(at(hx, o));
-    p.hy0 = ldn(at(hy, o));
-    p.hz0 = ldn(at(hz, o));
+    p.ex_kp = fld(ex, qp.ex, okp);
+    p.ey_kp = fld(ey, qp.ey, okp);
+    p.ezc = fld(ez, qp.ez, o);
+    p.ez_ip = fld(ez, qp.ez, o + S);
+    p.ey_ip = fld(ey, qp.ey, o + S);
+    p.ez_jp = fld(ez, qp.ez, ojp);
+    p.ex_jp = fld(ex, qp.ex, ojp);
+    p.hx0 = fldn(hx, qp.hx, o);
+    p.hy0 = fldn(hy, qp.hy, o);
+    p.hz0 = fldn(hz, qp.hz, o);
     p.cln = (CM != CLS_UNIFORM && k + 1 < k1)
                 ? class_at<CM>(a.cls, (std::uint32_t)(k + 1) * (std::uint32_t)a.plane + (std::uint32_t)cc.col)
                 : 0;
     p.p_yx = p.p_zx = p.p_xy = p.p_zy = p.p_xz = p.p_yz = 0.f;
     if (cc.xs) {
         const std::uint32_t q = (((std::uint32_t)k * ny + cc.j) * ra.Lx + cc.sx) * S;
-        p.p_yx = ld(at((const T *)ra.psi_hyx, q));
-        p.p_zx = ld(at((const T *)ra.psi_hzx, q));
+        p.p_yx = ld(at((const PT *)ra.psi_hyx, q));
+        p.p_zx = ld(at((const PT *)ra.psi_hzx, q));
     }
     if (cc.ys) {
         const std::uint32_t q = (((std::uint32_t)k * ra.Ly + cc.sy) * px + cc.i) * S;
-        p.p_xy = ld(at((const T *)ra.psi_hxy, q));
-        p.p_zy = ld(at((const T *)ra.psi_hzy, q));
+        p.p_xy = ld(at((const PT *)ra.psi_hxy, q));
+        p.p_zy = ld(at((const PT *)ra.psi_hzy, q));
     }
-    if ((V & V_Z) && h_zslab(a, k)) {
+    if ((V & V_Z) && h_zslab(a, k) && cc.valid) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
         const std::uint32_t q = (((std::uint32_t)sz * ny + cc.j) * px + cc.i) * S;
-        p.p_xz = ld(at((const T *)ra.psi_hxz, q));
-        p.p_yz = ld(at((const T *)ra.psi_hyz, q));
+        p.p_xz = ld(at((const PT *)ra.psi_hxz, q));
+        p.p_yz = ld(at((const PT *)ra.psi_hyz, q));
     }
 }
 
 template <class T, int CM, int V>
 __device__ __forceinline__ void h_update(const HPlane &p, const Coef &t, const KArgs &a, const HCol &cc, T *hx,
-                                         T *hy, T *hz, int k, int m, float &ex_c, float &ey_c, int &ev, int &ek)
+                                         T *hy, T *hz, const QPtrs &qp, int k, int m, float &ex_c, float &ey_c,
+                                         int &ev, int &ek)
 {
+    typedef typename psi_type<T>::type PT;
     ConstKArgs &ra = rare_args(a);
     const int nz = a.nz;
     constexpr std::uint32_t S = sizeof(T);
@@ -365,7 +379,7 @@ __device__ __forceinline__ void h_update(const HPlane &p, const Coef &t, const K
     const bool upd_x = cc.jhi && khi;
     const bool upd_y = cc.ihi && khi;
     const bool upd_z = cc.ihi && cc.jhi;
-    const bool zs = (V & V_Z) && h_zslab(a, k);
+    const bool zs = (V & V_Z) && h_zslab(a, k) && cc.valid;
 
     float chx_ey = cx.y, chx_ez = cx.z;
     float chy_ez = cy.y, chy_ex = cy.z;
@@ -396,8 +410,8 @@ __device__ __forceinline__ void h_update(const HPlane &p, const Coef &t, const K
         const std::uint32_t q = (((std::uint32_t)k * ra.ny + cc.j) * ra.Lx + cc.sx) * S;
         const float phy = cc.pxm.x * p.p_yx + cc.pxm.y * (ez_ip - ezc);
         const float phz = cc.pxm.x * p.p_zx + cc.pxm.y * (ey_ip - ey_c);
-        st(at((T *)ra.psi_hyx, q), phy);
-        st(at((T *)ra.psi_hzx, q), phz);
+        st(at((PT *)ra.psi_hyx, q), phy);
+        st(at((PT *)ra.psi_hzx, q), phz);
         if (khi) hyn = hyn + (cy.y * dx) * phy;
         if (cc.jhi) hzn = hzn + (cz.z * dx) * phz;
     }
@@ -405,8 +419,8 @@ __device__ __forceinline__ void h_update(const HPlane &p, const Coef &t, const K
         const std::uint32_t q = (((std::uint32_t)k * ra.Ly + cc.sy) * ra.px + cc.i) * S;
         const float phx = cc.pym.x * p.p_xy + cc.pym.y * (ez_jp - ezc);
         const float phz = cc.pym.x * p.p_zy + cc.pym.y * (ex_jp - ex_c);
-        st(at((T *)ra.psi_hxy, q), phx);
-        st(at((T *)ra.psi_hzy, q), phz);
+        st(at((PT *)ra.psi_hxy, q), phx);
+        st(at((PT *)ra.psi_hzy, q), phz);
         if (khi) hxn = hxn + (cx.z * dy) * phx;
         if (cc.ihi) hzn = hzn + (cz.y * dy) * phz;
     }
@@ -415,14 +429,14 @@ __device__ __forceinline__ void h_update(const HPlane &p, const Coef &t, const K
         const std::uint32_t q = (((std::uint32_t)sz * ra.ny + cc.j) * ra.px + cc.i) * S;
         const float phx = pzm.x * p.p_xz + pzm.y * (ey_kp - ey_c);
         const float phy = pzm.x * p.p_yz + pzm.y * (ex_kp - ex_c);
-        st(at((T *)ra.psi_hxz, q), phx);
-        st(at((T *)ra.psi_hyz, q), phy);
+        st(at((PT *)ra.psi_hxz, q), phx);
+        st(at((PT *)ra.psi_hyz, q), phy);
         if (cc.jhi) hxn = hxn + (cx.y * dz) * phx;
         if (cc.ihi) hyn = hyn + (cy.z * dz) * phy;
     }
-    if (upd_x) stn(at(hx, o), hxn);
-    if (upd_y) stn(at(hy, o), hyn);
-    if (upd_z) stn(at(hz, o), hzn);
+    fst(hx, qp.hx, o, upd_x, hxn, hx0, cc.valid);
+    fst(hy, qp.hy, o, upd_y, hyn, hy0, cc.valid);
+    fst(hz, qp.hz, o, upd_z, hzn, hz0, cc.valid);
     ex_c = carry(ex_kp);
     ey_c = carry(ey_kp);
 }
@@ -437,13 +451,16 @@ __global__ void __launch_bounds__(TX * BY) H_OCC h_kernel(const KArgs a, const i
     cc.j = ty * BY + (int)(threadIdx.x / TX);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int i = cc.i, j = cc.j;
-    if (i >= nx || j >= ny) return;
+    if (j >= ny || (!is_bq<T>::value && i >= nx)) return;
+    cc.valid = !is_bq<T>::value || i < nx;
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
     cc.col = (i64)j * a.px + i;
     cc.mycol = j * nx + i;
     cc.ob = (std::uint32_t)a.plane * (std::uint32_t)sizeof(T);
     cc.o0 = (std::uint32_t)cc.col * (std::uint32_t)sizeof(T) + cc.ob;
+    const QPtrs qp = {a.qex - a.plane / BQ_BLOCK, a.qey - a.plane / BQ_BLOCK, a.qez - a.plane / BQ_BLOCK,
+                      a.qhx - a.plane / BQ_BLOCK, a.qhy - a.plane / BQ_BLOCK, a.qhz - a.plane / BQ_BLOCK};
 
     // allocation bases (guard plane below k = 0)
     const T *__restrict__ ex = (const T *)a.ex - a.plane;
@@ -456,17 +473,17 @@ __global__ void __launch_bounds__(TX * BY) H_OCC h_kernel(const KArgs a, const i
     cc.ihi = !(V & V_X) || i < nx - 1;
     cc.jhi = !(V & V_Y) || j < ny - 1;
     cc.xs = (V & V_X) && (i < a.L[0] || (i >= nx - a.L[1] && cc.ihi));   // H x-slab, fdtdrav.h:402,424
-    cc.ys = (V & V_Y) && (j < a.L[2] || (j >= ny - a.L[3] && cc.jhi));   // H y-slab, fdtdrav.h:447,471
+    cc.ys = (V & V_Y) && cc.valid && (j < a.L[2] || (j >= ny - a.L[3] && cc.jhi));   // H y-slab, fdtdrav.h:447,471
     cc.sx = i < a.L[0] ? i : i - (nx - a.L[1]) + a.L[0];
     cc.sy = j < a.L[2] ? j : j - (ny - a.L[3]) + a.L[2];
     cc.pxm = cc.xs ? a.px_prof[nx + i] : make_float4(0.f, 0.f, 1.f, 0.f);
     cc.pym = cc.ys ? a.py_prof[ny + j] : make_float4(0.f, 0.f, 1.f, 0.f);
 
-    int ev = first_event(a, a.h_first, cc.mycol, k0);
+    int ev = cc.valid ? first_event(a, a.h_first, cc.mycol, k0) : -1;
     int ek = first_k(a, ev, cc.mycol);
 
-    float ex_c = ld(at(ex, cc.o0 + (std::uint32_t)k0 * cc.ob));
-    float ey_c = ld(at(ey, cc.o0 + (std::uint32_t)k0 * cc.ob));
+    float ex_c = fld(ex, qp.ex, cc.o0 + (std::uint32_t)k0 * cc.ob);
+    float ey_c = fld(ey, qp.ey, cc.o0 + (std::uint32_t)k0 * cc.ob);
 
     HPlane A, B;
     Coef tA, tB;
@@ -476,23 +493,23 @@ __global__ void __launch_bounds__(TX * BY) H_OCC h_kernel(const KArgs a, const i
     } else {
         tA = coef_of<CM>(a.htab, class_at<CM>(a.cls, (std::uint32_t)k0 * (std::uint32_t)a.plane + (std::uint32_t)cc.col));
     }
-    h_fetch<T, CM, V>(A, a, cc, ex, ey, ez, hx, hy, hz, k0, k1);
+    h_fetch<T, CM, V>(A, a, cc, ex, ey, ez, hx, hy, hz, qp, k0, k1);
     int k = k0;
     for (; k + 2 < k1; k += 2) {
-        h_fetch<T, CM, V>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        h_fetch<T, CM, V>(B, a, cc, ex, ey, ez, hx, hy, hz, qp, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_of<CM>(a.htab, A.cln);
-        h_update<T, CM, V>(A, tA, a, cc, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
-        h_fetch<T, CM, V>(A, a, cc, ex, ey, ez, hx, hy, hz, k + 2, k1);
+        h_update<T, CM, V>(A, tA, a, cc, hx, hy, hz, qp, k, m, ex_c, ey_c, ev, ek);
+        h_fetch<T, CM, V>(A, a, cc, ex, ey, ez, hx, hy, hz, qp, k + 2, k1);
         if (CM != CLS_UNIFORM) tA = coef_of<CM>(a.htab, B.cln);
-        h_update<T, CM, V>(B, tB, a, cc, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
+        h_update<T, CM, V>(B, tB, a, cc, hx, hy, hz, qp, k + 1, m, ex_c, ey_c, ev, ek);
     }
     if (k + 1 < k1) {
-        h_fetch<T, CM, V>(B, a, cc, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        h_fetch<T, CM, V>(B, a, cc, ex, ey, ez, hx, hy, hz, qp, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_of<CM>(a.htab, A.cln);
-        h_update<T, CM, V>(A, tA, a, cc, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
-        h_update<T, CM, V>(B, tB, a, cc, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
+        h_update<T, CM, V>(A, tA, a, cc, hx, hy, hz, qp, k, m, ex_c, ey_c, ev, ek);
+        h_update<T, CM, V>(B, tB, a, cc, hx, hy, hz, qp, k + 1, m, ex_c, ey_c, ev, ek);
     } else {
-        h_update<T, CM, V>(A, tA, a, cc, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
+        h_update<T, CM, V>(A, tA, a, cc, hx, hy, hz, qp, k, m, ex_c, ey_c, ev, ek);
     }
 }
 
@@ -525,11 +542,12 @@ static hipError_t launch_t(bool efield, int mode, const KArgs &a, int m, hipStre
     return hipGetLastError();
 }
 
-// storage: 0 FP32, 1 FP16, 2 BF16.  e_mode/h_mode: ClassMode of each sweep.
+// storage: 0 FP32, 1 FP16, 2 BF16, 3 block-scaled int16 (fields; psi FP16).  e_mode/h_mode: ClassMode of each sweep.
 hipError_t launch_half_step(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s)
 {
     if (storage == 1) return launch_t<_Float16>(efield, mode, a, m, s);
     if (storage == 2) return launch_t<bf16_t>(efield, mode, a, m, s);
+    if (storage == 3) return launch_t<bq16_t>(efield, mode, a, m, s);
     return launch_t<float>(efield, mode, a, m, s);
 }
 

@@ -643,33 +643,6 @@ int build_scene(const SceneIn &in, Built &out, std::string &err)
         out.events.push_back(ev_h[q]);
     }
     out.events.push_back(Event{1 << 30, -1, -1, -1});
-    // the same events keyed by (column, strip of STRIP_R rows)
-    {
-        const int nstrip = (ny + STRIP_R - 1) / STRIP_R;
-        std::vector<Event> se;
-        auto add = [&](const Event &e, bool efield) {
-            const int i = e.col % nx, j = e.col / nx;
-            const int s = j / STRIP_R, r = j % STRIP_R;
-            se.push_back(Event{e.k, e.kind | (r << 8), e.id, s * nx + i});
-            if (efield && e.kind == EV_SOURCE && r == 0 && s > 0)
-                se.push_back(Event{e.k, e.kind | (STRIP_R << 8), e.id, (s - 1) * nx + i});
-        };
-        for (const Event &e : ev_e) add(e, true);
-        for (const Event &e : ev_h) add(e, false);
-        std::sort(se.begin(), se.end(), [](const Event &a, const Event &b) {
-            if (a.col != b.col) return a.col < b.col;
-            if (a.k != b.k) return a.k < b.k;
-            if ((a.kind >> 8) != (b.kind >> 8)) return (a.kind >> 8) < (b.kind >> 8);
-            return (a.kind & 0xFF) < (b.kind & 0xFF);
-        });
-        out.s_first.assign((size_t)nstrip * nx, -1);
-        out.s_events.clear();
-        for (const Event &e : se) {
-            if (out.s_first[e.col] < 0) out.s_first[e.col] = (int)out.s_events.size();
-            out.s_events.push_back(e);
-        }
-        out.s_events.push_back(Event{1 << 30, -1, -1, -1});
-    }
 
     // ---- design edges: derivative of each E edge's update w.r.t. its
     // averaged eps (fdtdrav.h:1135-1177, 1271-1305):  with den = 2 eps eps0 +

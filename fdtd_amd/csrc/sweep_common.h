@@ -47,6 +47,69 @@ __device__ __forceinline__ void stn(bf16_t *p, float v)
     __builtin_nontemporal_store((std::uint16_t)(u >> 16), (std::uint16_t *)p);
 }
 
+// ---- block-scaled 16-bit fields (bq16_t) ----------------------------------
+// Exponent arrays of the six fields, from the start of their allocations
+// (null for the other storage types).
+struct QPtrs {
+    std::uint8_t *ex, *ey, *ez, *hx, *hy, *hz;
+};
+
+// Value of a field at byte offset o from the start of its allocation.
+template <class T>
+__device__ __forceinline__ float fld(const T *base, const std::uint8_t *, std::uint32_t o)
+{
+    return ld((const T *)((const char *)base + o));
+}
+template <class T>
+__device__ __forceinline__ float fldn(const T *base, const std::uint8_t *, std::uint32_t o)
+{
+    return ldn((const T *)((const char *)base + o));
+}
+__device__ __forceinline__ float bq_value(int m, int ebyte) { return ldexpf((float)m, ebyte - BQ_BIAS); }
+template <>
+__device__ __forceinline__ float fld<bq16_t>(const bq16_t *base, const std::uint8_t *q, std::uint32_t o)
+{
+    const int m = *(const std::int16_t *)((const char *)base + o);
+    return bq_value(m, q[o / (2 * BQ_BLOCK)]);
+}
+template <>
+__device__ __forceinline__ float fldn<bq16_t>(const bq16_t *base, const std::uint8_t *q, std::uint32_t o)
+{
+    const int m = __builtin_nontemporal_load((const std::int16_t *)((const char *)base + o));
+    return bq_value(m, q[o / (2 * BQ_BLOCK)]);
+}
+
+// Store of a swept field's new value v (when upd) at byte offset o.  For
+// block-scaled storage every lane of the 8-cell block takes part (cell i
+// sits on lane i % 64 of its wave): the block is re-encoded with one
+// exponent from its largest magnitude -- the smallest e with 2^e > max/32767
+// -- and each valid cell stores upd ? v : old (its current value).
+template <class T>
+__device__ __forceinline__ void fst(T *base, std::uint8_t *, std::uint32_t o, bool upd, float v, float, bool)
+{
+    if (upd) stn((T *)((char *)base + o), v);
+}
+template <>
+__device__ __forceinline__ void fst<bq16_t>(bq16_t *base, std::uint8_t *q, std::uint32_t o, bool upd, float v,
+                                            float old, bool valid)
+{
+    const float x = !valid ? 0.f : upd ? v : old;
+    float mx = fabsf(x);
+    mx = fmaxf(mx, __shfl_xor(mx, 1, BQ_BLOCK));
+    mx = fmaxf(mx, __shfl_xor(mx, 2, BQ_BLOCK));
+    mx = fmaxf(mx, __shfl_xor(mx, 4, BQ_BLOCK));
+    int e = -BQ_BIAS;
+    if (mx > 0.f) {
+        (void)frexpf(mx * (1.0f / 32767.0f), &e);   // 2^(e-1) <= max/32767 < 2^e
+        e = max(e, -BQ_BIAS);
+    }
+    const float mf = fminf(fmaxf(rintf(ldexpf(x, -e)), -32767.f), 32767.f);
+    if (valid) {
+        __builtin_nontemporal_store((std::int16_t)(int)mf, (std::int16_t *)((char *)base + o));
+        if ((o / 2) % BQ_BLOCK == 0) q[o / (2 * BQ_BLOCK)] = (std::uint8_t)(e + BQ_BIAS);
+    }
+}
+
 // Byte-offset addressing.  Every array a sweep touches is smaller than 4 GiB
 // (checked when the scene is finalized), so a 32-bit byte offset in one
 // vector register, added to a scalar base, addresses every stream of a

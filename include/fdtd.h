@@ -55,6 +55,9 @@ typedef struct fdtd_ctx fdtd_ctx;
 #define FDTD_STORE_F32   0
 #define FDTD_STORE_F16   1  /* FP16 storage of E, H and psi, scaled       */
 #define FDTD_STORE_BF16  2  /* BF16 storage of E, H and psi, scaled       */
+#define FDTD_STORE_BQ16  3  /* E and H as int16 mantissas with one power-of-two
+                               exponent byte per 8 x-cells (2.125 B/value),
+                               psi FP16, scaled; no adjoint               */
 
 /* Field components (probe and field-access selectors). */
 #define FDTD_EX 0
@@ -330,6 +333,9 @@ int fdtd_checkpoint_restore(fdtd_ctx *ctx, const void *buf, size_t bytes);
 /* Run n_steps forward steps from step 0 (checkpointing every
  * checkpoint_every steps), then the discrete adjoint backward in segments,
  * recomputing each segment's forward fields from its checkpoint.
+ * checkpoint_every = 0 stores the design region's E of every step instead
+ * (no recompute); < 0 chooses: that history when it fits in half the free
+ * device memory, else checkpoints every ceil(sqrt(n_steps)) steps.
  * J_out receives J; grad_out receives dJ/d(eps_r) of every design cell
  * (x-fastest over the design region, FP64). */
 int fdtd_adjoint_gradient(fdtd_ctx *ctx, const fdtd_objective *obj,
@@ -338,15 +344,16 @@ int fdtd_adjoint_gradient(fdtd_ctx *ctx, const fdtd_objective *obj,
 
 /* Device scratch of fdtd_adjoint_gradient_noalloc, in bytes: a static array
  * the library holds on every GPU it runs on, until the process ends. */
-#define FDTD_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+#define FDTD_NOALLOC_SCRATCH_BYTES ((size_t)320 << 20)
 
 /* fdtd_adjoint_gradient with checkpoint_every = 0 (the design-region E
  * history of the whole run is stored), without allocating: no hipMalloc or
  * hipFree, no stream or event creation.  Its device buffers (the FP32 adjoint
- * fields and psi, the history, the gradient accumulators) are carved from
+ * fields and psi, the cells' transposed quantities, the history, the
+ * gradient accumulators) are carved from
  * the static scratch of FDTD_NOALLOC_SCRATCH_BYTES per GPU; a run that needs
  * more returns FDTD_E_NOMEM with the figure (adjoint128, 1000 steps, needs
- * 247 MiB).  Same results as fdtd_adjoint_gradient(..., 0, ...), bit for
+ * 297 MiB).  Same results as fdtd_adjoint_gradient(..., 0, ...), bit for
  * bit.  Every call on one GPU shares that GPU's scratch: calls that overlap
  * on one GPU (other threads or contexts) race.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP

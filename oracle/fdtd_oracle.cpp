@@ -209,15 +209,21 @@ static void quantize(Sim<R> *s, bool efield)
                             efield ? &s->Psi_exy : &s->Psi_hxy, efield ? &s->Psi_ezy : &s->Psi_hzy,
                             efield ? &s->Psi_exz : &s->Psi_hxz, efield ? &s->Psi_eyz : &s->Psi_hyz};
     const bool k = !s->keep.empty();
-    if (s->quant == 18) {
+    if (s->quant == 18 || s->quant == 40) {
         // block-scaled int16: one power-of-two scale per aligned run of
         // qblock cells along x, mantissas rounded to nearest in [-32767, 32767]
         const int B = s->qblock;
         const int nb = (s->NX + B - 1) / B;
+        const int na = s->quant == 18 ? 9 : 3;
         std::vector<R> *all[9] = {f[0], f[1], f[2], p[0], p[1], p[2], p[3], p[4], p[5]};
+        if (s->quant == 40) {
+#pragma omp parallel for schedule(static)
+            for (i64 x = 0; x < s->N; ++x)
+                for (int c = 0; c < 6; ++c) (*p[c])[x] = (R)(round_store((float)(*p[c])[x] * sp, 16) / sp);
+        }
 #pragma omp parallel for schedule(static)
         for (i64 r = 0; r < (i64)s->NY * s->NZ; ++r)
-            for (int a = 0; a < 9; ++a) {
+            for (int a = 0; a < na; ++a) {
                 R *row = all[a]->data() + r * s->NX;
                 for (int b = 0; b < nb; ++b) {
                     const int i0 = b * B, i1 = std::min(s->NX, i0 + B);
@@ -1215,8 +1221,12 @@ int oracle_set_quant(void *hp, int mode, double sE, double sH, int keep_radius)
     Handle *h = (Handle *)hp;
     if (h->prec != 32) return -1;
     Sim<float> *s = (Sim<float> *)h->sim;
-    s->quant = mode >= 18 ? 18 : mode;
-    if (mode > 18) s->qblock = mode - 18;   // 18 + block size selects the block length
+    // 18 + B: block-scaled int16 for fields and psi, blocks of B x-cells;
+    // 40 + B: block-scaled int16 fields, FP16 psi (scaled), as the engine's
+    // block-scaled storage does
+    s->quant = mode >= 40 ? 40 : mode >= 18 ? 18 : mode;
+    if (mode > 40) s->qblock = mode - 40;
+    else if (mode > 18 && mode < 40) s->qblock = mode - 18;
     s->qsE = (float)sE;
     s->qsH = (float)sH;
     s->keep.clear();

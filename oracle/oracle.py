@@ -196,8 +196,10 @@ class Oracle:
 
     def set_quant(self, mode, sE=1.0, sH=1.0, keep_radius=-1):
         """Round every stored E/H/psi value to FP16 (16) or BF16 (17) in
-        scaled units (FP32 oracle only) -- a model of reduced-precision
-        storage for error studies."""
+        scaled units, or to block-scaled int16 with one power-of-two scale
+        per B x-cells: 18 + B for fields and psi, 40 + B for the fields with
+        psi in FP16 (the engine's block-scaled storage) -- FP32 oracle only,
+        a model of reduced-precision storage for error studies."""
         if self.lib.oracle_set_quant(self.h, int(mode), float(sE), float(sH), int(keep_radius)) != 0:
             raise ValueError("quantisation needs the FP32 oracle")
 

@@ -123,3 +123,22 @@ def test_adjoint_interior_split(engine, monkeypatch, nx):
     rel = np.linalg.norm(g0 - go) / np.linalg.norm(go)
     print(f"[adjoint split] gradient rel L2 vs FP64 {rel:.2e}", flush=True)
     assert rel < 1e-3
+
+
+@pytest.mark.parametrize("nx", [200, 530])
+def test_adjoint_gather_paths_bitwise(engine, monkeypatch, nx):
+    """The two forms of the adjoint sweeps -- every cell's transposed
+    quantities stored by the sweep that updates it (narrow grids) and
+    computed where they are gathered, shared through LDS (wide grids) -- give
+    the same gradient bit for bit, checkpointed and not."""
+    sc = small_design(nx=nx, ny=26, nz=24, steps=100, design=(8, 6, 5))
+    out = {}
+    for path in ("stored", "lds"):
+        monkeypatch.setenv("FDTD_ADJ_GATHER", path)
+        print(f"[adjoint gather] {sc.nx}x{sc.ny}x{sc.nz} path={path}", flush=True)
+        sim = engine.Simulation.from_scene(sc, "f32")
+        out[path] = [sim.adjoint_gradient(TERMS, sc.steps, K) for K in (0, 32)]
+        sim.close()
+    for (J0, g0), (J1, g1) in zip(out["stored"], out["lds"]):
+        assert J0 == J1
+        np.testing.assert_array_equal(g0, g1)

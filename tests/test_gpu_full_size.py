@@ -6,8 +6,10 @@ and 16), where the interior, x-edge and general tile variants split the grid
 as they do in the benchmark.  FP32: bit for bit against the oracle's FP32
 restatement on probes, DFT and every field, for enough steps that the
 wavefronts of the seeded sources reach the CPML slabs (c dt / dx = 0.57 cells
-per step).  FP16: the full 3000-step slab512 run stays within the storage
-rounding band against the FP32 run.
+per step).  Storage forms: the full 3000-step slab512 run of FP32, FP16 and
+block-scaled int16 storage against the committed FP64 fixture, each inside its
+regression band, and the 1e-3 target of the 2-byte forms as an expected
+failure (the gap stays visible in every GPU test report).
 """
 import gc
 
@@ -62,25 +64,63 @@ def test_full_size_fp32_bitwise(workload, steps, engine):
         assert r.max() <= 1e-5, r.max()
 
 
-def test_full_size_fp16_storage_band(engine):
-    """slab512 at full length: the FP16-stored run against the FP32 run,
-    probe by probe (relative L2 over the probes that carry signal) and on
-    the DFT peak.  Measured 2.1e-2 / 1.2e-2 (max / median) and 5.1e-3 on
-    the DFT; a lost scale or a missed rounding is orders of magnitude off."""
+_ERRORS = {}
+
+
+def _errors_vs_fp64(engine):
+    """slab512 at full length (3000 steps) in every storage form against the
+    committed FP64 oracle fixture: per-probe relative L2 over the probes that
+    carry signal, and the DFT error over each probe's spectral peak.  Computed
+    once per session."""
+    if _ERRORS:
+        return _ERRORS
+    import os
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "slab512_fp64.npz"))
+    ref, X64 = fx["probes"], fx["dft"]
     sc = scenes.slab512()
-    print(f"[full-size] fp16 band grid={sc.nx}x{sc.ny}x{sc.nz} steps={sc.steps}", flush=True)
-    out = {}
-    for storage in ("f32", "f16"):
+    assert ref.shape == (len(sc.probes), sc.steps)
+    n = np.linalg.norm(ref, axis=1)
+    sig = n >= 1e-3 * n.max()
+    for storage in ("f32", "f16", "bq16"):
+        print(f"[full-size] slab512 {storage}, {sc.steps} steps against the FP64 fixture", flush=True)
         sim = engine.Simulation.from_scene(sc, storage)
         sim.run(sc.steps)
-        out[storage] = (sim.probes().astype(np.float64), sim.dft())
+        p = sim.probes()[:, :sc.steps].astype(np.float64)
+        X = sim.dft()
         sim.close()
-    (p32, d32), (p16, d16) = out["f32"], out["f16"]
-    n = np.linalg.norm(p32, axis=1)
-    sig = n >= 1e-3 * n.max()
-    assert sig.sum() >= 64
-    r = np.linalg.norm(p16 - p32, axis=1)[sig] / n[sig]
-    print(f"[full-size] fp16 probe rel L2 max {r.max():.3e} median {np.median(r):.3e}", flush=True)
-    assert r.max() < 5e-2 and np.median(r) < 2.5e-2
-    dr = np.abs(d16 - d32).max(axis=1)[sig] / np.abs(d32).max(axis=1)[sig]
-    assert dr.max() < 2e-2
+        r = np.linalg.norm(p - ref, axis=1)[sig] / n[sig]
+        d = (np.abs(X - X64).max(axis=1) / np.abs(X64).max(axis=1))[sig]
+        print(f"[full-size] {storage}: probe rel L2 max {r.max():.3e} median {np.median(r):.3e}, DFT {d.max():.3e}",
+              flush=True)
+        _ERRORS[storage] = (r, d)
+    return _ERRORS
+
+
+# Regression bands (what each form measures at 3000 steps, with margin), not
+# the north star's target: FP32 2.7e-5 max; FP16 2.1e-2 max / 1.2e-2 median;
+# block-scaled int16 2.6e-3 / 1.5e-3.  A lost scale or a missed rounding is
+# orders of magnitude off.
+BANDS = {"f32": (1e-4, 5e-5, 2e-4), "f16": (5e-2, 2.5e-2, 4e-2), "bq16": (5e-3, 3e-3, 5e-3)}
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bq16"])
+def test_full_size_storage_error_band(engine, storage):
+    """slab512 (512x512x256, CPML 16, 256 boxes, 8 sources, 128 probes, 16
+    DFT frequencies) for its full 3000 steps, each storage form against FP64:
+    max and median probe error and the DFT error inside the form's
+    regression band."""
+    r, d = _errors_vs_fp64(engine)[storage]
+    assert r.size >= 64
+    mx, med, dft = BANDS[storage]
+    assert r.max() < mx and np.median(r) < med and d.max() < dft
+
+
+@pytest.mark.parametrize("storage", ["f16", "bq16"])
+@pytest.mark.xfail(reason="2-byte storage misses the 1e-3 probe-error target at 3000 steps: each stored value is "
+                          "rounded once per step and the roundings accumulate as a random walk (worst probe: FP16 "
+                          "2.1e-2, block-scaled int16 2.6e-3)", strict=False)
+def test_16bit_storage_meets_the_1e3_target(engine, storage):
+    """The north star's target for the 2-byte storage forms: every probe
+    within 1e-3 relative L2 of FP64 at slab512's 3000 steps."""
+    r, _ = _errors_vs_fp64(engine)[storage]
+    assert r.max() <= 1e-3

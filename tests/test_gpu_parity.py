@@ -158,13 +158,16 @@ def test_interior_split_and_z_blocks_bitwise(interior, chunks, engine, oracle_mo
         np.testing.assert_array_equal(sim.field(c), o32.field(nm).astype(np.float32))
 
 
-def test_resume_from_checkpoint_is_bitwise(engine):
+@pytest.mark.parametrize("storage", ["f32", "bq16"])
+def test_resume_from_checkpoint_is_bitwise(engine, storage):
+    """A run resumed from a checkpoint equals the uninterrupted run bit for
+    bit (block-scaled storage: its exponent bytes travel in the blob)."""
     sc = scene_dielectric()
-    a = engine.Simulation.from_scene(sc, "f32")
+    a = engine.Simulation.from_scene(sc, storage)
     a.run(sc.steps)
     ref = a.probes()
     ref_ez = a.field(EZ)
-    b = engine.Simulation.from_scene(sc, "f32")
+    b = engine.Simulation.from_scene(sc, storage)
     b.run(60)
     blob = b.checkpoint_save()
     b.run(30)            # diverge, then rewind
@@ -212,4 +215,28 @@ def test_grid_past_the_32bit_offsets_is_refused(storage, nz, engine):
     with pytest.raises(engine.FdtdError) as ei:
         sim.finalize(1)
     assert ei.value.code == -6 and "4 GiB" in str(ei.value)
+    sim.close()
+
+
+@pytest.mark.parametrize("fail_at", [1, 4, 9, 16])
+def test_allocation_failure_is_reported_and_recoverable(engine, monkeypatch, fail_at):
+    """A device allocation that fails during fdtd_finalize (forced from the
+    host: FDTD_FAIL_ALLOC=n fails the context's n-th allocation) returns
+    FDTD_E_NOMEM with the reason, releases what was allocated, and the same
+    context finalizes and runs normally afterwards, bit for bit as a fresh one."""
+    sc = scene_dielectric()
+    ref = engine.Simulation.from_scene(sc, "f32")
+    ref.run(sc.steps)
+    want = ref.probes()
+    ref.close()
+    sim = engine.Simulation.from_scene(sc, "f32", finalize=False)
+    monkeypatch.setenv("FDTD_FAIL_ALLOC", str(fail_at))
+    with pytest.raises(engine.FdtdError) as ei:
+        sim.finalize(sc.steps)
+    assert ei.value.code == engine.FDTD_E_NOMEM and "injected" in str(ei.value)
+    assert sim.info()["device_bytes"] == 0
+    monkeypatch.delenv("FDTD_FAIL_ALLOC")
+    sim.finalize(sc.steps)
+    sim.run(sc.steps)
+    np.testing.assert_array_equal(sim.probes(), want)
     sim.close()

@@ -32,10 +32,21 @@ def rel(a, b):
     return np.linalg.norm(a - b, axis=-1) / np.linalg.norm(b, axis=-1)
 
 
-@pytest.mark.parametrize("storage,mode", [("f16", 16), ("bf16", 17)])
-def test_scaled_storage_matches_rounding_model(engine, storage, mode):
+# FP16 / BF16: the oracle's modes 16 / 17; block-scaled int16 fields with 8-cell
+# blocks and FP16 psi: mode 40 + 8.  The bounds on the error against FP64
+# are regression bands at this scene's 200 steps (what each type measures
+# here, with margin), not the 1e-3 target of the 3000-step workloads.
+STORE_BANDS = {"f16": (16, 6e-3), "bf16": (17, 5e-2), "bq16": (48, 1e-3)}
+
+
+@pytest.mark.parametrize("storage,nx", [("f16", 40), ("bf16", 40), ("bq16", 40), ("bq16", 45)])
+def test_scaled_storage_matches_rounding_model(engine, storage, nx):
+    """Engine and storage model against FP64, probe by probe (nx = 45: a
+    partial last exponent block, whose lanes past nx take part in the block
+    encode and store nothing)."""
     from oracle.oracle import Oracle
-    sc = scenes.dielectric(nx=40, ny=36, nz=32, steps=200, L=6, n_boxes=6, n_src=2, n_probes=8, n_freqs=3,
+    mode, band = STORE_BANDS[storage]
+    sc = scenes.dielectric(nx=nx, ny=36, nz=32, steps=200, L=6, n_boxes=6, n_src=2, n_probes=8, n_freqs=3,
                            seed=21, name="precision-small")
     print(f"[precision] storage={storage} grid={sc.nx}x{sc.ny}x{sc.nz} steps={sc.steps}", flush=True)
     sim = engine.Simulation.from_scene(sc, storage)
@@ -60,8 +71,8 @@ def test_scaled_storage_matches_rounding_model(engine, storage, mode):
     ratio = e_store / e_model_store
     assert np.all((ratio > 0.4) & (ratio < 2.5)), ratio
     assert 0.6 < np.median(ratio) < 1.6, ratio
-    # ... of the size the storage type gives over 200 steps
-    assert e_store.max() < (6e-3 if storage == "f16" else 5e-2)
+    # ... of the size the storage type gives over 200 steps (regression band)
+    assert e_store.max() < band
     # a defect (a wrong scale, a missed rounding) would be far off: the
     # engine is much closer to the FP64 result than to a run with the H
     # scale off by 2x
@@ -128,3 +139,29 @@ def test_pair_kernels_adjoint_snapshots(engine, monkeypatch):
     for o in out[:-1]:
         assert o[0] == out[-1][0]
         np.testing.assert_array_equal(o[1], out[-1][1])
+
+
+def test_block_scaled_storage_is_fivefold_closer_than_fp16(engine):
+    """The block-scaled int16 fields (2.125 bytes per value) cut the storage
+    error against FP64 at least fivefold against FP16 on the same scene
+    (median and worst probe), and the stored fields decode to finite values."""
+    from oracle.oracle import Oracle
+    sc = scenes.dielectric(nx=40, ny=36, nz=32, steps=300, L=6, n_boxes=6, n_src=2, n_probes=8, n_freqs=3,
+                           seed=22, name="precision-bq")
+    o64 = Oracle(sc, 64)
+    o64.steps(sc.steps)
+    ref = o64.probes()
+    sig = np.linalg.norm(ref, axis=-1) >= 1e-3 * np.abs(ref).max() * math.sqrt(ref.shape[1])
+    err = {}
+    for st in ("f16", "bq16"):
+        sim = engine.Simulation.from_scene(sc, st)
+        sim.run(sc.steps)
+        err[st] = rel(sim.probes()[:, :sc.steps].astype(np.float64), ref)[sig]
+        if st == "bq16":
+            for c in range(6):
+                assert np.all(np.isfinite(sim.field(c)))
+        sim.close()
+    print(f"[precision] 300 steps: FP16 max {err['f16'].max():.2e} median {np.median(err['f16']):.2e}; "
+          f"block int16 max {err['bq16'].max():.2e} median {np.median(err['bq16']):.2e}", flush=True)
+    assert np.median(err["bq16"]) < 0.2 * np.median(err["f16"])
+    assert err["bq16"].max() < 0.2 * err["f16"].max()

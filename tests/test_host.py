@@ -308,3 +308,48 @@ def test_reference_sigma_max_is_the_float_expression(tmp_path):
         assert got != scenes.reference_sigma_max(np.float32(math.pi), np.float32(float(d)))
     sc = scenes.patch_antenna(steps=1)
     assert sc.cpml.sigma_max == tuple(float.fromhex(v) for v in out[:3])
+
+
+def _sanitized_run(cmd_build, exe, env=None):
+    r = subprocess.run(cmd_build, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    run_env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0",
+                   UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1", OMP_NUM_THREADS="2", **(env or {}))
+    r = subprocess.run([str(exe)], capture_output=True, text=True, env=run_env, timeout=300)
+    report = r.stdout + r.stderr
+    assert r.returncode == 0, report[-4000:]
+    assert "AddressSanitizer" not in report and "runtime error" not in report, report[-4000:]
+    return report
+
+
+def test_engine_host_code_under_sanitizers(tmp_path):
+    """The engine's host code (scene builder for every storage form with boxes,
+    z- and x-sources, plates, a resistor, a sector, probes and a design
+    region; the port reductions and results-file writer) built host-only with
+    AddressSanitizer and UndefinedBehaviorSanitizer: no report, and an
+    invalid scene is refused with a message (tests/c/host_sanitize_main.cpp)."""
+    hipcc = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "bin", "hipcc")
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not found")
+    exe = tmp_path / "host_sanitize"
+    csrc = os.path.join(ROOT, "fdtd_amd", "csrc")
+    inc = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "include")
+    cmd = [hipcc, "-x", "c++", "-std=c++17", "-O1", "-g", "-D__HIP_PLATFORM_AMD__", "-I", inc, "-I", csrc,
+           "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+           "-fno-sanitize-recover=undefined", os.path.join(ROOT, "tests", "c", "host_sanitize_main.cpp"),
+           os.path.join(csrc, "scene.cpp"), os.path.join(csrc, "port.cpp"), "-o", str(exe)]
+    out = _sanitized_run(cmd, exe, env={"SANITIZE_OUT": str(tmp_path / "v.dat")})
+    assert "storage 3:" in out and (tmp_path / "v.dat").exists()
+
+
+def test_oracle_under_sanitizers(tmp_path):
+    """The CPU oracle (FP32 and FP64 runs of a scene with every kind of
+    object, the FP16 / BF16 / block-scaled storage models, the FP64 adjoint)
+    built with g++ under AddressSanitizer and UndefinedBehaviorSanitizer: no
+    report (tests/c/oracle_sanitize_main.cpp)."""
+    exe = tmp_path / "oracle_sanitize"
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-march=x86-64-v3", "-mf16c", "-fopenmp",
+           "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+           os.path.join(ROOT, "tests", "c", "oracle_sanitize_main.cpp"), "-o", str(exe)]
+    out = _sanitized_run(cmd, exe)
+    assert "oracle ok" in out

@@ -534,3 +534,38 @@ def test_adjoint_gradient_matches_finite_differences():
         em[q] = np.float32(em[q] - h)
         fd = (objective(ep) - objective(em)) / (float(ep[q]) - float(em[q]))
         assert g[q] == pytest.approx(fd, rel=1e-4, abs=1e-6 * np.abs(g).max()), q
+
+
+# ---- storage-rounding model (the checker of the 16-bit storage forms) ---------
+
+def test_block_scaled_model_stores_int16_blocks():
+    """The oracle's block-scaled int16 model (mode 40 + 8: fields in blocks of
+    8 x-cells, psi FP16): after every step each 8-cell run of every field is
+    exactly int16 mantissas times one power of two, |mantissa| <= 32767, and
+    the largest mantissa of a non-zero block uses the top binade (the scale
+    is not wasted); its largest probe error against FP64 is at most a fifth
+    of FP16's."""
+    sc = scenes.dielectric(nx=36, ny=30, nz=28, steps=150, L=5, n_boxes=4, n_src=2, n_probes=6, n_freqs=2,
+                           seed=31, name="model-bq")
+    ref = Oracle(sc, 64)
+    ref.steps(sc.steps)
+    r = ref.probes()
+    errs = {}
+    for mode in (16, 48):
+        o = Oracle(sc, 32)
+        o.set_quant(mode, 1.0, scenes.ETA0)
+        o.steps(sc.steps)
+        p = o.probes()
+        errs[mode] = np.max(np.linalg.norm(p - r, axis=1) / np.linalg.norm(r, axis=1))
+        if mode == 48:
+            for name in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+                f = o.field(name).reshape(-1, sc.nx)[:, :sc.nx // 8 * 8].reshape(-1, 8)
+                mx = np.abs(f).max(axis=1)
+                live = mx > 0
+                e = np.ceil(np.log2(mx[live] / 32767.0))
+                q = np.exp2(e)[:, None]
+                m = f[live] / q
+                assert np.all(m == np.round(m)) and np.all(np.abs(m) <= 32767), name
+                assert np.all(np.abs(m).max(axis=1) >= 16384), name
+        o.close()
+    assert errs[48] < 0.2 * errs[16], errs
