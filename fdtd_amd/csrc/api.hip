@@ -212,10 +212,13 @@ int upload_tables(fdtd_ctx *c);
 // carried z-neighbour, so a block of `planes` costs 1/planes extra on that
 // stream.  Measured on slab512: the FP32 E sweep is fastest at 4 planes, the
 // FP32 H sweep at 2, the 16-bit pair sweeps at 8 (FDTD_CHUNKS overrides all
-// three with a number of blocks along z).
-int choose_chunks(const fdtd_ctx *c, int tiles, int planes)
+// three with a number of blocks along z; FDTD_E_PLANES, FDTD_H_PLANES and
+// FDTD_PAIR_PLANES set one sweep's planes per block before the small-grid
+// halving).
+int choose_chunks(const fdtd_ctx *c, int tiles, int planes, const char *planes_env)
 {
     const char *ov = std::getenv("FDTD_CHUNKS");
+    if (const char *pe = std::getenv(planes_env)) planes = std::max(1, std::atoi(pe));
     // Small grids: halve the planes per block until a launch has 1024
     // workgroups (4 per CU), down to one plane (FDTD_MIN_WG overrides 1024).
     // Measured: 64^3 FP32 18.4 -> 22.7 G, FP16 10.3 -> 22.3 G; 128x128x64
@@ -302,8 +305,8 @@ int do_finalize(fdtd_ctx *c)
     for (int q = 0; q < 6; ++q) a.probe_unscale[q] = b.probe_unscale[q];
     a.tiles_x = (in.nx + tile_cols() - 1) / tile_cols();
     a.tiles_y = (in.ny + tile_rows() - 1) / tile_rows();
-    a.kchunk = choose_chunks(c, a.tiles_x * a.tiles_y, 4);
-    c->kchunk_h = choose_chunks(c, a.tiles_x * a.tiles_y, 2);
+    a.kchunk = choose_chunks(c, a.tiles_x * a.tiles_y, 4, "FDTD_E_PLANES");
+    c->kchunk_h = choose_chunks(c, a.tiles_x * a.tiles_y, 2, "FDTD_H_PLANES");
     // 16-bit storage sweeps a cell pair per thread (128-cell tile rows);
     // FDTD_PAIR=0 selects the one-cell kernels instead (same stored bits)
     const char *pv = std::getenv("FDTD_PAIR");
@@ -316,7 +319,7 @@ int do_finalize(fdtd_ctx *c)
     const char *gv = std::getenv("FDTD_GEN16");
     c->pair_gen_cells = !(gv && std::strcmp(gv, "pair") == 0);
     c->pair_tiles_x = (in.nx + pair_tile_cols() - 1) / pair_tile_cols();
-    c->pair_kchunk = choose_chunks(c, c->pair_tiles_x * a.tiles_y, 8);
+    c->pair_kchunk = choose_chunks(c, c->pair_tiles_x * a.tiles_y, 8, "FDTD_PAIR_PLANES");
     a.rec = 1;
     a.snap = nullptr;
 
