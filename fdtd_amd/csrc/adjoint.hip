@@ -832,9 +832,12 @@ hipError_t launch_snap(int storage, const KArgs &a, float *hist, int slot, hipSt
 {
     const i64 nb = (i64)a.sb[3] * a.sb[4] * a.sb[5];
     dim3 grid((unsigned)((nb + 255) / 256)), block(256);
-    if (storage == 1) hipLaunchKernelGGL((snap_kernel<_Float16>), grid, block, 0, s, a, hist, slot);
-    else if (storage == 2) hipLaunchKernelGGL((snap_kernel<bf16_t>), grid, block, 0, s, a, hist, slot);
-    else hipLaunchKernelGGL((snap_kernel<float>), grid, block, 0, s, a, hist, slot);
+    switch (storage) {   // block-scaled storage has no adjoint (refused before any launch)
+    case 0: hipLaunchKernelGGL((snap_kernel<float>), grid, block, 0, s, a, hist, slot); break;
+    case 1: hipLaunchKernelGGL((snap_kernel<_Float16>), grid, block, 0, s, a, hist, slot); break;
+    case 2: hipLaunchKernelGGL((snap_kernel<bf16_t>), grid, block, 0, s, a, hist, slot); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

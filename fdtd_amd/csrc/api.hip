@@ -61,8 +61,12 @@ struct fdtd_ctx {
     // Field and psi allocation bases (the sweeps update them in place).
     void *fields[6] = {};
     void *psi[12] = {};
-    std::uint8_t *qexp[6] = {};     // block-scaled storage: exponent bytes per 8 x-cells, guard planes included
-    size_t qexp_bytes = 0;          // per field
+    // block-scaled storage: the E and H exponent arrays (BQ_STRIDE bytes per
+    // 8 x-cells: the x, y, z components' exponents), guard planes included,
+    // and the component pointers into them (qexp[f] = qarr[f / 3] + f % 3)
+    std::uint8_t *qarr[2] = {};
+    std::uint8_t *qexp[6] = {};
+    size_t qarr_bytes = 0;          // per array
     bool pair = false;              // 16-bit storage: two cells per thread in the half-step kernels
     bool pair_gen_cells = false;    // ... except the general variant, which runs one cell per thread
     int pair_tiles_x = 0, pair_kchunk = 0;
@@ -175,6 +179,7 @@ void free_device(fdtd_ctx *c)
     for (auto &f : c->fields) f = nullptr;
     for (auto &p : c->psi) p = nullptr;
     for (auto &q : c->qexp) q = nullptr;
+    for (auto &q : c->qarr) q = nullptr;
 }
 
 // The buffers holding the current state.
@@ -191,8 +196,9 @@ int zero_state(fdtd_ctx *c)
 {
     for (int f = 0; f < 6; ++f) {
         if (c->fields[f]) HIPCHK(c, hipMemsetAsync(c->fields[f], 0, (size_t)c->field_elems * c->esz, c->stream));
-        if (c->qexp[f]) HIPCHK(c, hipMemsetAsync(c->qexp[f], 0, c->qexp_bytes, c->stream));
     }
+    for (int q = 0; q < 2; ++q)
+        if (c->qarr[q]) HIPCHK(c, hipMemsetAsync(c->qarr[q], 0, c->qarr_bytes, c->stream));
     for (int p = 0; p < 12; ++p)
         if (c->psi_elems[p] && c->psi[p])
             HIPCHK(c, hipMemsetAsync(c->psi[p], 0, c->psi_elems[p] * c->esz, c->stream));
@@ -307,10 +313,11 @@ int do_finalize(fdtd_ctx *c)
     a.tiles_y = (in.ny + tile_rows() - 1) / tile_rows();
     a.kchunk = choose_chunks(c, a.tiles_x * a.tiles_y, 4, "FDTD_E_PLANES");
     c->kchunk_h = choose_chunks(c, a.tiles_x * a.tiles_y, 2, "FDTD_H_PLANES");
-    // 16-bit storage sweeps a cell pair per thread (128-cell tile rows);
-    // FDTD_PAIR=0 selects the one-cell kernels instead (same stored bits)
+    // 16-bit storage (FP16, BF16, block-scaled int16) sweeps a cell pair per
+    // thread (128-cell tile rows); FDTD_PAIR=0 selects the one-cell kernels
+    // instead (same stored bits)
     const char *pv = std::getenv("FDTD_PAIR");
-    c->pair = (in.storage == FDTD_STORE_F16 || in.storage == FDTD_STORE_BF16) && !(pv && std::strcmp(pv, "0") == 0);
+    c->pair = in.storage != FDTD_STORE_F32 && !(pv && std::strcmp(pv, "0") == 0);
     // The pair sweeps' general variant (slab rows and z-blocks) runs as the
     // one-cell kernel (same stored bits): on 512x512x256 FP16 the step took
     // 0.772 ms against 0.788 with the pair kernel's own general variant,
@@ -336,14 +343,16 @@ int do_finalize(fdtd_ctx *c)
         if (rc) return rc;
     }
     if (in.storage == FDTD_STORE_BQ16) {
-        // one exponent byte per 8 x-cells (px is a multiple of 64)
-        c->qexp_bytes = (size_t)c->field_elems / 8;
-        for (int f = 0; f < 6; ++f) {
-            void *q = nullptr;
-            int rc = dev_alloc(c, &q, c->qexp_bytes);
+        // BQ_STRIDE exponent bytes per 8 x-cells of E and of H (px is a
+        // multiple of 64)
+        c->qarr_bytes = (size_t)c->field_elems / BQ_BLOCK * BQ_STRIDE;
+        for (int q = 0; q < 2; ++q) {
+            void *d = nullptr;
+            int rc = dev_alloc(c, &d, c->qarr_bytes);
             if (rc) return rc;
-            c->qexp[f] = (std::uint8_t *)q;
+            c->qarr[q] = (std::uint8_t *)d;
         }
+        for (int f = 0; f < 6; ++f) c->qexp[f] = c->qarr[f / 3] + f % 3;
     }
     // interior / general split of the two sweeps
     {
@@ -436,7 +445,8 @@ void bind_sets(const fdtd_ctx *c, KArgs &a)
     void **rd[6] = {&a.ex, &a.ey, &a.ez, &a.hx, &a.hy, &a.hz};
     for (int f = 0; f < 6; ++f) *rd[f] = (char *)c->fields[f] + g;
     std::uint8_t **qd[6] = {&a.qex, &a.qey, &a.qez, &a.qhx, &a.qhy, &a.qhz};
-    for (int f = 0; f < 6; ++f) *qd[f] = c->qexp[f] ? c->qexp[f] + (size_t)c->ka.plane / 8 : nullptr;
+    for (int f = 0; f < 6; ++f)
+        *qd[f] = c->qexp[f] ? c->qexp[f] + (size_t)c->ka.plane / BQ_BLOCK * BQ_STRIDE : nullptr;
     void **pr[12] = {&a.psi_eyx, &a.psi_ezx, &a.psi_hyx, &a.psi_hzx, &a.psi_exy, &a.psi_ezy,
                      &a.psi_hxy, &a.psi_hzy, &a.psi_exz, &a.psi_eyz, &a.psi_hxz, &a.psi_hyz};
     for (int p = 0; p < 12; ++p) *pr[p] = c->psi[p];
@@ -1218,9 +1228,9 @@ int fdtd_get_info(const fdtd_ctx *c, fdtd_info *o)
     // algorithmic bytes per cell: E sweep reads E, H and writes E (9 values);
     // H sweep likewise; plus one class index per cell in a classed sweep and
     // psi read+write over the slabs (averaged over the grid).
-    // block-scaled storage: 2 bytes per value plus one exponent byte per 8;
-    // its psi is FP16
-    const double s = c->in.storage == FDTD_STORE_BQ16 ? 2.125 : (double)c->esz;
+    // block-scaled storage: 2 bytes per value plus BQ_STRIDE exponent bytes
+    // per 8 cells of the three components; its psi is FP16
+    const double s = c->in.storage == FDTD_STORE_BQ16 ? 2.0 + (double)BQ_STRIDE / (3 * BQ_BLOCK) : (double)c->esz;
     const double sp = (double)c->esz;
     const double fx = (double)(c->in.cpml.thick[0] + c->in.cpml.thick[1]) / c->in.nx;
     const double fy = (double)(c->in.cpml.thick[2] + c->in.cpml.thick[3]) / c->in.ny;
@@ -1335,10 +1345,11 @@ int fdtd_get_field(fdtd_ctx *c, int comp, float *out, size_t n)
     std::vector<unsigned char> raw((size_t)px * ny * nz * c->esz);
     const char *base = (const char *)c->fields[comp] + (size_t)c->ka.plane * c->esz;
     HIPCHK(c, hipMemcpy(raw.data(), base, raw.size(), hipMemcpyDeviceToHost));
-    std::vector<std::uint8_t> qraw;
-    if (c->qexp[comp]) {
-        qraw.resize((size_t)px * ny * nz / 8);
-        HIPCHK(c, hipMemcpy(qraw.data(), c->qexp[comp] + (size_t)c->ka.plane / 8, qraw.size(), hipMemcpyDeviceToHost));
+    std::vector<std::uint8_t> qraw;   // the component's exponent array, k = 0 .. nz-1
+    if (c->qarr[comp / 3]) {
+        qraw.resize((size_t)px * ny * nz / BQ_BLOCK * BQ_STRIDE);
+        HIPCHK(c, hipMemcpy(qraw.data(), c->qarr[comp / 3] + (size_t)c->ka.plane / BQ_BLOCK * BQ_STRIDE,
+                            qraw.size(), hipMemcpyDeviceToHost));
     }
     const float un = c->built.probe_unscale[comp];
     for (size_t r = 0; r < (size_t)ny * nz; ++r)
@@ -1354,7 +1365,7 @@ int fdtd_get_field(fdtd_ctx *c, int comp, float *out, size_t n)
                     uint32_t u = (uint32_t)h << 16;
                     std::memcpy(&v, &u, 4);
                 } else if (c->in.storage == FDTD_STORE_BQ16) {
-                    v = std::ldexp((float)(int16_t)h, (int)qraw[src / 8] - 128);
+                    v = std::ldexp((float)(int16_t)h, (int)qraw[src / BQ_BLOCK * BQ_STRIDE + comp % 3] - BQ_BIAS);
                 } else {
                     _Float16 f;
                     std::memcpy(&f, &h, 2);
@@ -1371,7 +1382,7 @@ int fdtd_checkpoint_size(const fdtd_ctx *c, size_t *bytes)
 {
     if (!c || !bytes) return FDTD_E_INVALID;
     if (!c->finalized) return FDTD_E_STATE;
-    size_t p = 6 * ((size_t)c->field_elems * c->esz + c->qexp_bytes);
+    size_t p = 6 * (size_t)c->field_elems * c->esz + 2 * c->qarr_bytes;
     for (int q = 0; q < 12; ++q) p += c->psi_elems[q] * c->esz;
     p += c->in.probes.size() * (size_t)c->in.steps_cap * sizeof(float);
     p += c->in.probes.size() * c->in.freqs.size() * sizeof(double2);
@@ -1394,8 +1405,9 @@ int fdtd_checkpoint_save(fdtd_ctx *c, void *buf, size_t bytes)
         return FDTD_OK;
     };
     for (int f = 0; f < 6; ++f)
-        if ((rc = get(fld(c, f), (size_t)c->field_elems * c->esz)) || (rc = get(c->qexp[f], c->qexp_bytes)))
-            return rc;
+        if ((rc = get(fld(c, f), (size_t)c->field_elems * c->esz))) return rc;
+    for (int q = 0; q < 2; ++q)
+        if ((rc = get(c->qarr[q], c->qarr_bytes))) return rc;
     for (int q = 0; q < 12; ++q)
         if ((rc = get(psv(c, q), c->psi_elems[q] * c->esz))) return rc;
     if ((rc = get(c->ka.probe_hist, c->in.probes.size() * (size_t)c->in.steps_cap * sizeof(float)))) return rc;
@@ -1439,8 +1451,9 @@ int fdtd_checkpoint_restore(fdtd_ctx *c, const void *buf, size_t bytes)
         return FDTD_OK;
     };
     for (int f = 0; f < 6; ++f)
-        if ((rc = put(fld(c, f), (size_t)c->field_elems * c->esz)) || (rc = put(c->qexp[f], c->qexp_bytes)))
-            return rc;
+        if ((rc = put(fld(c, f), (size_t)c->field_elems * c->esz))) return rc;
+    for (int q = 0; q < 2; ++q)
+        if ((rc = put(c->qarr[q], c->qarr_bytes))) return rc;
     for (int q = 0; q < 12; ++q)
         if ((rc = put(psv(c, q), c->psi_elems[q] * c->esz))) return rc;
     if ((rc = put(c->ka.probe_hist, c->in.probes.size() * (size_t)c->in.steps_cap * sizeof(float)))) return rc;

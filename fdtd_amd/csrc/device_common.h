@@ -33,15 +33,17 @@ __device__ __forceinline__ void st(bf16_t *p, float v)
 
 // Block-scaled 16-bit storage of the fields: an int16 mantissa per value and
 // one power-of-two exponent byte per aligned run of BQ_BLOCK cells along x,
-// value = mantissa * 2^(byte - BQ_BIAS).  The exponent array of a field has
-// the field's layout with x divided by BQ_BLOCK, so the byte of the value at
-// byte offset o (2-byte elements) is at o / (2 * BQ_BLOCK).  psi is stored
-// in FP16 (psi_type).
+// value = mantissa * 2^(byte - BQ_BIAS).  The exponents of the three
+// components of E (and of H) share one array with the field's layout, x
+// divided by BQ_BLOCK and BQ_STRIDE bytes per block: bytes 0, 1, 2 hold the
+// x, y, z components' exponents (byte 3 is unused), so one 4-byte load
+// brings all three.  The byte of component c of the value at byte offset o
+// (2-byte elements) is at (o / (2 * BQ_BLOCK)) * BQ_STRIDE + c.  psi is
+// stored in FP16 (psi_type).
 struct bq16_t {
     std::int16_t v;
 };
-constexpr int BQ_BLOCK = 8;
-constexpr int BQ_BIAS = 128;
+// BQ_BLOCK, BQ_BIAS, BQ_STRIDE: internal.h (the host code shares them)
 
 template <class T> struct psi_type { typedef T type; };
 template <> struct psi_type<bq16_t> { typedef _Float16 type; };

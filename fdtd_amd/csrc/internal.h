@@ -10,6 +10,12 @@ namespace fdtd {
 
 typedef std::int64_t i64;
 
+// Block-scaled int16 storage: cells per exponent along x, exponent bias, and
+// exponent bytes per block in the E (or H) exponent array (x, y, z, unused).
+constexpr int BQ_BLOCK = 8;
+constexpr int BQ_BIAS = 128;
+constexpr int BQ_STRIDE = 4;
+
 // Coefficient-class map index width.  Class 0..n-1 index the E and H tables.
 enum ClassMode { CLS_UNIFORM = 0, CLS_U16 = 1, CLS_U32 = 2 };
 

@@ -229,8 +229,7 @@ __global__ void __launch_bounds__(TX * BY) E_OCC e_kernel(const KArgs a, const i
     cc.mycol = j * nx + i;
     cc.ob = (std::uint32_t)a.plane * (std::uint32_t)sizeof(T);
     cc.o0 = (std::uint32_t)cc.col * (std::uint32_t)sizeof(T) + cc.ob;
-    const QPtrs qp = {a.qex - a.plane / BQ_BLOCK, a.qey - a.plane / BQ_BLOCK, a.qez - a.plane / BQ_BLOCK,
-                      a.qhx - a.plane / BQ_BLOCK, a.qhy - a.plane / BQ_BLOCK, a.qhz - a.plane / BQ_BLOCK};
+    const QPtrs qp = qptrs(a);
 
     // allocation bases (guard plane below k = 0)
     T *__restrict__ ex = (T *)a.ex - a.plane;
@@ -459,8 +458,7 @@ __global__ void __launch_bounds__(TX * BY) H_OCC h_kernel(const KArgs a, const i
     cc.mycol = j * nx + i;
     cc.ob = (std::uint32_t)a.plane * (std::uint32_t)sizeof(T);
     cc.o0 = (std::uint32_t)cc.col * (std::uint32_t)sizeof(T) + cc.ob;
-    const QPtrs qp = {a.qex - a.plane / BQ_BLOCK, a.qey - a.plane / BQ_BLOCK, a.qez - a.plane / BQ_BLOCK,
-                      a.qhx - a.plane / BQ_BLOCK, a.qhy - a.plane / BQ_BLOCK, a.qhz - a.plane / BQ_BLOCK};
+    const QPtrs qp = qptrs(a);
 
     // allocation bases (guard plane below k = 0)
     const T *__restrict__ ex = (const T *)a.ex - a.plane;
@@ -545,10 +543,13 @@ static hipError_t launch_t(bool efield, int mode, const KArgs &a, int m, hipStre
 // storage: 0 FP32, 1 FP16, 2 BF16, 3 block-scaled int16 (fields; psi FP16).  e_mode/h_mode: ClassMode of each sweep.
 hipError_t launch_half_step(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s)
 {
-    if (storage == 1) return launch_t<_Float16>(efield, mode, a, m, s);
-    if (storage == 2) return launch_t<bf16_t>(efield, mode, a, m, s);
-    if (storage == 3) return launch_t<bq16_t>(efield, mode, a, m, s);
-    return launch_t<float>(efield, mode, a, m, s);
+    switch (storage) {
+    case 0: return launch_t<float>(efield, mode, a, m, s);
+    case 1: return launch_t<_Float16>(efield, mode, a, m, s);
+    case 2: return launch_t<bf16_t>(efield, mode, a, m, s);
+    case 3: return launch_t<bq16_t>(efield, mode, a, m, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 // The general variant alone (the rows and z-blocks outside the interior box;
@@ -559,9 +560,13 @@ hipError_t launch_half_step_general(bool efield, int storage, int mode, const KA
     int nb, nx, ng;
     region_tiles(a, nb, nx, ng);
     if (ng == 0) return hipSuccess;
-    if (storage == 1) launch_v<_Float16, V_GEN>(efield, mode, a, m, ng, s);
-    else if (storage == 2) launch_v<bf16_t, V_GEN>(efield, mode, a, m, ng, s);
-    else launch_v<float, V_GEN>(efield, mode, a, m, ng, s);
+    switch (storage) {
+    case 0: launch_v<float, V_GEN>(efield, mode, a, m, ng, s); break;
+    case 1: launch_v<_Float16, V_GEN>(efield, mode, a, m, ng, s); break;
+    case 2: launch_v<bf16_t, V_GEN>(efield, mode, a, m, ng, s); break;
+    case 3: launch_v<bq16_t, V_GEN>(efield, mode, a, m, ng, s); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

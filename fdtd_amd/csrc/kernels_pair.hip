@@ -1,5 +1,5 @@
 // kernels_pair.hip -- the E and H half-step kernels for 16-bit storage
-// (FP16 / BF16), two adjacent x cells per thread.
+// (FP16 / BF16 / block-scaled int16), two adjacent x cells per thread.
 //
 // With 2-byte values a one-cell-per-thread sweep moves half the bytes per
 // load instruction of the FP32 sweep and runs out of loads in flight long
@@ -11,7 +11,10 @@
 // the H sweep) are single 2-byte loads.  The z-march, the two-deep software
 // pipeline, the class-table prefetch, the XCD tile mapping and the per-cell
 // arithmetic are those of kernels.hip; only the packing differs, so the
-// stored bits equal the one-cell kernels' bit for bit.
+// stored bits equal the one-cell kernels' bit for bit.  Block-scaled storage
+// adds the exponent byte of the pair's 8-cell block to every packed load;
+// the four lanes of a block (one DPP quad) agree on the block's new exponent
+// with two lane permutations before they store.
 //   E: calc_e  fdtdrav.h:665-735, CPML fdtdrav.h:750-911
 //   H: calc_h  fdtdrav.h:328-386, CPML fdtdrav.h:398-562
 #include <hip/hip_runtime.h>
@@ -80,6 +83,116 @@ __device__ __forceinline__ u32 merge2(u32 old, bool ua, float va, bool ub, float
     return lo | (hi << 16);
 }
 
+// Exponent word of block-scaled storage: the exponent bytes of the three
+// components of E (or H) at one 8-cell block, one 4-byte load (empty for the
+// other storage types, which then carry no extra registers).
+template <class T>
+struct QW {
+};
+template <>
+struct QW<bq16_t> {
+    u32 w;
+};
+
+template <class T>
+__device__ __forceinline__ QW<T> ldq(const std::uint8_t *, u32)
+{
+    return QW<T>{};
+}
+template <>
+__device__ __forceinline__ QW<bq16_t> ldq<bq16_t>(const std::uint8_t *q, u32 off)
+{
+    QW<bq16_t> r;
+    r.w = *(const u32 *)(q + bq_at(off));
+    return r;
+}
+template <class T>
+__device__ __forceinline__ QW<T> carryq(const QW<T> &q)
+{
+    return q;
+}
+template <>
+__device__ __forceinline__ QW<bq16_t> carryq<bq16_t>(const QW<bq16_t> &q)
+{
+    QW<bq16_t> r;
+    r.w = carry(q.w);
+    return r;
+}
+// exponent of component c (bias removed)
+__device__ __forceinline__ int qe(const QW<bq16_t> &q, int c) { return (int)((q.w >> (8 * c)) & 0xFFu) - BQ_BIAS; }
+
+// Values of a packed pair (cell i, cell i+1) of component c.
+template <class T>
+__device__ __forceinline__ float dlo(u32 m, const QW<T> &, int)
+{
+    return lo_f(m, (const T *)nullptr);
+}
+template <class T>
+__device__ __forceinline__ float dhi(u32 m, const QW<T> &, int)
+{
+    return hi_f(m, (const T *)nullptr);
+}
+template <>
+__device__ __forceinline__ float dlo<bq16_t>(u32 m, const QW<bq16_t> &q, int c)
+{
+    return ldexpf((float)(std::int16_t)(m & 0xFFFFu), qe(q, c));
+}
+template <>
+__device__ __forceinline__ float dhi<bq16_t>(u32 m, const QW<bq16_t> &q, int c)
+{
+    return ldexpf((float)(std::int16_t)(m >> 16), qe(q, c));
+}
+
+// One value of component c at byte offset off.
+template <class T>
+__device__ __forceinline__ float ld1(const T *base, u32 off, const QW<T> &, int)
+{
+    return ld(at(base, off));
+}
+template <>
+__device__ __forceinline__ float ld1<bq16_t>(const bq16_t *base, u32 off, const QW<bq16_t> &q, int c)
+{
+    return ldexpf((float)*(const std::int16_t *)((const char *)base + off), qe(q, c));
+}
+
+// Store of a pair's new values (va / vb where ua / ub, else the old ones,
+// oa / ob decoded and old_m as stored).  FP16 / BF16: only when a cell
+// changes; the other half keeps its bits.  Block-scaled: every lane of the
+// block re-encodes its cells (a cell past nx counts as 0 and keeps its bits)
+// under the block's new exponent; returns that exponent's byte.
+template <class T>
+__device__ __forceinline__ u32 stp(T *base, u32 o, u32 old_m, float, float, bool ua, float va, bool ub, float vb,
+                                   bool, bool)
+{
+    if (ua || ub) stn2(base, o, merge2<T>(old_m, ua, va, ub, vb));
+    return 0;
+}
+template <>
+__device__ __forceinline__ u32 stp<bq16_t>(bq16_t *base, u32 o, u32 old_m, float oa, float ob, bool ua, float va,
+                                           bool ub, float vb, bool valid_a, bool valid_b)
+{
+    const float xa = !valid_a ? 0.f : ua ? va : oa;
+    const float xb = !valid_b ? 0.f : ub ? vb : ob;
+    float mx = fmaxf(fabsf(xa), fabsf(xb));
+    mx = fmaxf(mx, dpp_f<DPP_QUAD_XOR1>(mx));
+    mx = fmaxf(mx, dpp_f<DPP_QUAD_XOR2>(mx));
+    const int e = bq_exponent(mx);
+    if (valid_a) {
+        const u32 wa = (u32)(std::uint16_t)(std::int16_t)bq_mantissa(xa, e);
+        const u32 wb = valid_b ? (u32)(std::uint16_t)(std::int16_t)bq_mantissa(xb, e) : (old_m >> 16);
+        stn2(base, o, wa | (wb << 16));
+    }
+    return (u32)(e + BQ_BIAS);
+}
+
+// The three exponent bytes of a swept field at the block of byte offset o,
+// written by the lane of the block's first cell.
+template <class T>
+__device__ __forceinline__ void stq(std::uint8_t *q, u32 o, u32 bx, u32 by, u32 bz, bool first)
+{
+    if (is_bq<T>::value && first) *(u32 *)(q + bq_at(o)) = bx | (by << 8) | (bz << 16);
+}
+
 // class indices of the pair at cell index `cell` (even)
 template <int CM>
 __device__ __forceinline__ void class_pair(const void *cls, u32 cell, int &ca, int &cb)
@@ -113,7 +226,7 @@ __device__ __forceinline__ CoefPair coef_pair(const float4 *tab, int ca, int cb)
 // Per-cell column constants (cell 0 = i, cell 1 = i+1).
 struct PCell {
     int i, sx, mycol;
-    bool ilo, ihi, xs, valid, in_snap;
+    bool ilo, ihi, xs, valid, in_snap;   // valid: i < nx
     float4 px;   // CPML x profile of this cell (E or H nodes)
 };
 
@@ -128,18 +241,21 @@ struct PRow {
 
 // ============================ E sweep ======================================
 
+template <class T>
 struct EPair {
     u32 hx, hy, hz, hx_jm, hz_jm, ex, ey, ez;   // packed pairs
-    float hy_im, hz_im;                          // H at i-1 (x neighbour of cell 0)
+    QW<T> qh, qh_jm, qe;                        // exponent words of H, H at row j-1, E
+    float hy_im, hz_im;                         // H at i-1 (x neighbour of cell 0)
     float p_yx[2], p_zx[2], p_xy[2], p_zy[2], p_xz[2], p_yz[2];
     int cla, clb;                                // class indices of plane k+1
 };
 
 template <class T, int CM, int V>
-__device__ __forceinline__ void ep_fetch(EPair &p, const KArgs &a, const PRow &r, const PCell (&c)[2], const T *ex,
-                                         const T *ey, const T *ez, const T *hx, const T *hy, const T *hz, int k,
-                                         int k1)
+__device__ __forceinline__ void ep_fetch(EPair<T> &p, const KArgs &a, const PRow &r, const PCell (&c)[2],
+                                         const T *ex, const T *ey, const T *ez, const T *hx, const T *hy,
+                                         const T *hz, const QPtrs &qp, int k, int k1)
 {
+    typedef typename psi_type<T>::type PT;
     ConstKArgs &ra = rare_args(a);
     constexpr u32 S = sizeof(T);
     const u32 o = r.o0 + (u32)k * r.ob;
@@ -149,11 +265,15 @@ __device__ __forceinline__ void ep_fetch(EPair &p, const KArgs &a, const PRow &r
     p.hz = ld2(hz, o);
     p.hx_jm = ld2(hx, ojm);
     p.hz_jm = ld2(hz, ojm);
-    p.hy_im = ld(at(hy, o - S));
-    p.hz_im = ld(at(hz, o - S));
     p.ex = ldn2(ex, o);
     p.ey = ldn2(ey, o);
     p.ez = ldn2(ez, o);
+    p.qh = ldq<T>(qp.hx, o);
+    p.qh_jm = ldq<T>(qp.hx, ojm);
+    p.qe = ldq<T>(qp.ex, o);
+    const QW<T> qh_im = ldq<T>(qp.hx, o - S);
+    p.hy_im = ld1(hy, o - S, qh_im, 1);
+    p.hz_im = ld1(hz, o - S, qh_im, 2);
     p.cla = p.clb = 0;
     if (CM != CLS_UNIFORM && k + 1 < k1)
         class_pair<CM>(a.cls, (u32)(k + 1) * (u32)a.plane + (u32)r.col, p.cla, p.clb);
@@ -162,37 +282,38 @@ __device__ __forceinline__ void ep_fetch(EPair &p, const KArgs &a, const PRow &r
         p.p_yx[q] = p.p_zx[q] = p.p_xy[q] = p.p_zy[q] = p.p_xz[q] = p.p_yz[q] = 0.f;
         if (c[q].xs && r.jhi && khi) {
             const u32 qq = (((u32)k * ra.ny + r.j) * ra.Lx + c[q].sx) * S;
-            p.p_yx[q] = ld(at((const T *)ra.psi_eyx, qq));
-            p.p_zx[q] = ld(at((const T *)ra.psi_ezx, qq));
+            p.p_yx[q] = ld(at((const PT *)ra.psi_eyx, qq));
+            p.p_zx[q] = ld(at((const PT *)ra.psi_ezx, qq));
         }
     }
     if (r.ys && khi) {
         const u32 qq = (((u32)k * ra.Ly + r.sy) * ra.px + c[0].i) * S;
-        const u32 u = ld2((const T *)ra.psi_exy, qq), v = ld2((const T *)ra.psi_ezy, qq);
-        p.p_xy[0] = lo_f(u, ex);
-        p.p_xy[1] = hi_f(u, ex);
-        p.p_zy[0] = lo_f(v, ex);
-        p.p_zy[1] = hi_f(v, ex);
+        const u32 u = ld2((const PT *)ra.psi_exy, qq), v = ld2((const PT *)ra.psi_ezy, qq);
+        p.p_xy[0] = lo_f(u, (const PT *)nullptr);
+        p.p_xy[1] = hi_f(u, (const PT *)nullptr);
+        p.p_zy[0] = lo_f(v, (const PT *)nullptr);
+        p.p_zy[1] = hi_f(v, (const PT *)nullptr);
     }
     if ((V & V_Z) && ((k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && khi)) && r.jhi) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
         const u32 qq = (((u32)sz * ra.ny + r.j) * ra.px + c[0].i) * S;
-        const u32 u = ld2((const T *)ra.psi_exz, qq), v = ld2((const T *)ra.psi_eyz, qq);
-        p.p_xz[0] = lo_f(u, ex);
-        p.p_xz[1] = hi_f(u, ex);
-        p.p_yz[0] = lo_f(v, ex);
-        p.p_yz[1] = hi_f(v, ex);
+        const u32 u = ld2((const PT *)ra.psi_exz, qq), v = ld2((const PT *)ra.psi_eyz, qq);
+        p.p_xz[0] = lo_f(u, (const PT *)nullptr);
+        p.p_xz[1] = hi_f(u, (const PT *)nullptr);
+        p.p_yz[0] = lo_f(v, (const PT *)nullptr);
+        p.p_yz[1] = hi_f(v, (const PT *)nullptr);
     }
 }
 
 // E update of one cell of the pair; the arithmetic of kernels.hip e_update.
 template <class T, int V>
-__device__ __forceinline__ void e_cell(const KArgs &a, const PRow &r, const PCell &c, int q, const EPair &p,
+__device__ __forceinline__ void e_cell(const KArgs &a, const PRow &r, const PCell &c, int q, const EPair<T> &p,
                                        const Coef &t, int k, int m, float hxc, float hyc, float hzc, float hy_im,
                                        float hz_im, float hx_jm, float hz_jm, float ex0, float ey0, float ez0,
                                        float hx_km, float hy_km, int &ev, int &ek, bool zs, const float4 &pze,
                                        float &exn, float &eyn, float &ezn, bool &ux, bool &uy, bool &uz)
 {
+    typedef typename psi_type<T>::type PT;
     ConstKArgs &ra = rare_args(a);
     const int nz = a.nz;
     const float4 cx = t.x, cy = t.y, cz = t.z;
@@ -228,8 +349,8 @@ __device__ __forceinline__ void e_cell(const KArgs &a, const PRow &r, const PCel
         const u32 qq = (((u32)k * ra.ny + r.j) * ra.Lx + c.sx) * S;
         const float pey = c.px.x * p.p_yx[q] + c.px.y * (hzc - hz_im);
         const float pez = c.px.x * p.p_zx[q] + c.px.y * (hyc - hy_im);
-        st(at((T *)ra.psi_eyx, qq), pey);
-        st(at((T *)ra.psi_ezx, qq), pez);
+        st(at((PT *)ra.psi_eyx, qq), pey);
+        st(at((PT *)ra.psi_ezx, qq), pez);
         if (klo) eyn = eyn + (cy.z * dx) * pey;
         if (r.jlo) ezn = ezn + (cz.y * dx) * pez;
     }
@@ -237,8 +358,8 @@ __device__ __forceinline__ void e_cell(const KArgs &a, const PRow &r, const PCel
         const u32 qq = (((u32)k * ra.Ly + r.sy) * ra.px + c.i) * S;
         const float pex = r.py.x * p.p_xy[q] + r.py.y * (hzc - hz_jm);
         const float pez = r.py.x * p.p_zy[q] + r.py.y * (hxc - hx_jm);
-        st(at((T *)ra.psi_exy, qq), pex);
-        st(at((T *)ra.psi_ezy, qq), pez);
+        st(at((PT *)ra.psi_exy, qq), pex);
+        st(at((PT *)ra.psi_ezy, qq), pez);
         if (klo) exn = exn + (cx.y * dy) * pex;
         if (c.ilo) ezn = ezn + (cz.z * dy) * pez;
     }
@@ -247,8 +368,8 @@ __device__ __forceinline__ void e_cell(const KArgs &a, const PRow &r, const PCel
         const u32 qq = (((u32)sz * ra.ny + r.j) * ra.px + c.i) * S;
         const float pex = pze.x * p.p_xz[q] + pze.y * (hyc - hy_km);
         const float pey = pze.x * p.p_yz[q] + pze.y * (hxc - hx_km);
-        st(at((T *)ra.psi_exz, qq), pex);
-        st(at((T *)ra.psi_eyz, qq), pey);
+        st(at((PT *)ra.psi_exz, qq), pex);
+        st(at((PT *)ra.psi_eyz, qq), pey);
         if (r.jlo) exn = exn + (cx.z * dz) * pex;
         if (c.ilo) eyn = eyn + (cy.y * dz) * pey;
     }
@@ -259,30 +380,35 @@ __device__ __forceinline__ void e_cell(const KArgs &a, const PRow &r, const PCel
 }
 
 template <class T, int CM, int V>
-__device__ __forceinline__ void ep_update(const EPair &p, const CoefPair &t, const KArgs &a, const PRow &r,
-                                          const PCell (&c)[2], T *ex, T *ey, T *ez, int k, int m, u32 &hx_km,
-                                          u32 &hy_km, int (&ev)[2], int (&ek)[2])
+__device__ __forceinline__ void ep_update(const EPair<T> &p, const CoefPair &t, const KArgs &a, const PRow &r,
+                                          const PCell (&c)[2], T *ex, T *ey, T *ez, const QPtrs &qp, int k, int m,
+                                          u32 &hx_km, u32 &hy_km, QW<T> &q_km, int (&ev)[2], int (&ek)[2])
 {
-    const T *tp = nullptr;
     const u32 o = r.o0 + (u32)k * r.ob;
     const bool zs = (V & V_Z) && ((k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && k < a.nz - 1));   // fdtdrav.h:853,879
     const float4 pze = zs ? uload(a.pz_prof + k) : make_float4(0.f, 0.f, 1.f, 0.f);
-    const float hx0 = lo_f(p.hx, tp), hx1 = hi_f(p.hx, tp);
-    const float hy0 = lo_f(p.hy, tp), hy1 = hi_f(p.hy, tp);
-    const float hz0 = lo_f(p.hz, tp), hz1 = hi_f(p.hz, tp);
+    const float hx0 = dlo(p.hx, p.qh, 0), hx1 = dhi(p.hx, p.qh, 0);
+    const float hy0 = dlo(p.hy, p.qh, 1), hy1 = dhi(p.hy, p.qh, 1);
+    const float hz0 = dlo(p.hz, p.qh, 2), hz1 = dhi(p.hz, p.qh, 2);
+    const float ex0 = dlo(p.ex, p.qe, 0), ex1 = dhi(p.ex, p.qe, 0);
+    const float ey0 = dlo(p.ey, p.qe, 1), ey1 = dhi(p.ey, p.qe, 1);
+    const float ez0 = dlo(p.ez, p.qe, 2), ez1 = dhi(p.ez, p.qe, 2);
     float ex_a, ey_a, ez_a, ex_b, ey_b, ez_b;
     bool uxa, uya, uza, uxb, uyb, uzb;
-    e_cell<T, V>(a, r, c[0], 0, p, t.a, k, m, hx0, hy0, hz0, p.hy_im, p.hz_im, lo_f(p.hx_jm, tp), lo_f(p.hz_jm, tp),
-              lo_f(p.ex, tp), lo_f(p.ey, tp), lo_f(p.ez, tp), lo_f(hx_km, tp), lo_f(hy_km, tp), ev[0], ek[0], zs,
-              pze, ex_a, ey_a, ez_a, uxa, uya, uza);
-    e_cell<T, V>(a, r, c[1], 1, p, t.b, k, m, hx1, hy1, hz1, hy0, hz0, hi_f(p.hx_jm, tp), hi_f(p.hz_jm, tp),
-              hi_f(p.ex, tp), hi_f(p.ey, tp), hi_f(p.ez, tp), hi_f(hx_km, tp), hi_f(hy_km, tp), ev[1], ek[1], zs,
-              pze, ex_b, ey_b, ez_b, uxb, uyb, uzb);
-    if (uxa || uxb) stn2(ex, o, merge2<T>(p.ex, uxa, ex_a, uxb, ex_b));
-    if (uya || uyb) stn2(ey, o, merge2<T>(p.ey, uya, ey_a, uyb, ey_b));
-    if (uza || uzb) stn2(ez, o, merge2<T>(p.ez, uza, ez_a, uzb, ez_b));
+    e_cell<T, V>(a, r, c[0], 0, p, t.a, k, m, hx0, hy0, hz0, p.hy_im, p.hz_im, dlo(p.hx_jm, p.qh_jm, 0),
+                 dlo(p.hz_jm, p.qh_jm, 2), ex0, ey0, ez0, dlo(hx_km, q_km, 0), dlo(hy_km, q_km, 1), ev[0], ek[0], zs,
+                 pze, ex_a, ey_a, ez_a, uxa, uya, uza);
+    e_cell<T, V>(a, r, c[1], 1, p, t.b, k, m, hx1, hy1, hz1, hy0, hz0, dhi(p.hx_jm, p.qh_jm, 0),
+                 dhi(p.hz_jm, p.qh_jm, 2), ex1, ey1, ez1, dhi(hx_km, q_km, 0), dhi(hy_km, q_km, 1), ev[1], ek[1], zs,
+                 pze, ex_b, ey_b, ez_b, uxb, uyb, uzb);
+    const bool va = c[0].valid, vb = c[1].valid;
+    const u32 bx = stp(ex, o, p.ex, ex0, ex1, uxa, ex_a, uxb, ex_b, va, vb);
+    const u32 by = stp(ey, o, p.ey, ey0, ey1, uya, ey_a, uyb, ey_b, va, vb);
+    const u32 bz = stp(ez, o, p.ez, ez0, ez1, uza, ez_a, uzb, ez_b, va, vb);
+    stq<T>(qp.ex, o, bx, by, bz, va && (c[0].i & (BQ_BLOCK - 1)) == 0);
     hx_km = carry(p.hx);
     hy_km = carry(p.hy);
+    q_km = carryq(p.qh);
 }
 
 template <class T, int CM, int V>
@@ -294,7 +420,10 @@ __global__ void __launch_bounds__(TX * BY) e_pair_kernel(const KArgs a, const in
     const int i0 = tx * TXP + 2 * (int)(threadIdx.x & (TX - 1));
     PRow r;
     r.j = ty * BY + (int)(threadIdx.x / TX);
-    if (i0 >= nx || r.j >= ny) return;
+    // block-scaled storage encodes 8-cell blocks across the four lanes of a
+    // quad: the lanes of a block that reaches past nx stay (the block lies
+    // inside the row's padded pitch), storing nothing past nx
+    if (r.j >= ny || (is_bq<T>::value ? (i0 & ~(BQ_BLOCK - 1)) : i0) >= nx) return;
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
     const int j = r.j;
@@ -328,6 +457,7 @@ __global__ void __launch_bounds__(TX * BY) e_pair_kernel(const KArgs a, const in
     const T *__restrict__ hx = (const T *)a.hx - a.plane;
     const T *__restrict__ hy = (const T *)a.hy - a.plane;
     const T *__restrict__ hz = (const T *)a.hz - a.plane;
+    const QPtrs qp = qptrs(a);
 
     int ev[2], ek[2];
     for (int q = 0; q < 2; ++q) {
@@ -337,8 +467,9 @@ __global__ void __launch_bounds__(TX * BY) e_pair_kernel(const KArgs a, const in
 
     u32 hx_km = ld2(hx, r.o0 + (u32)(k0 - 1) * r.ob);
     u32 hy_km = ld2(hy, r.o0 + (u32)(k0 - 1) * r.ob);
+    QW<T> q_km = ldq<T>(qp.hx, r.o0 + (u32)(k0 - 1) * r.ob);
 
-    EPair A, B;
+    EPair<T> A, B;
     CoefPair tA, tB;
     if (CM == CLS_UNIFORM) {
         tA = coef_pair<CM>(a.etab, 0, 0);
@@ -348,31 +479,33 @@ __global__ void __launch_bounds__(TX * BY) e_pair_kernel(const KArgs a, const in
         class_pair<CM>(a.cls, (u32)k0 * (u32)a.plane + (u32)r.col, ca, cb);
         tA = coef_pair<CM>(a.etab, ca, cb);
     }
-    ep_fetch<T, CM, V>(A, a, r, c, ex, ey, ez, hx, hy, hz, k0, k1);
+    ep_fetch<T, CM, V>(A, a, r, c, ex, ey, ez, hx, hy, hz, qp, k0, k1);
     int k = k0;
     for (; k + 2 < k1; k += 2) {
-        ep_fetch<T, CM, V>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        ep_fetch<T, CM, V>(B, a, r, c, ex, ey, ez, hx, hy, hz, qp, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_pair<CM>(a.etab, A.cla, A.clb);
-        ep_update<T, CM, V>(A, tA, a, r, c, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
-        ep_fetch<T, CM, V>(A, a, r, c, ex, ey, ez, hx, hy, hz, k + 2, k1);
+        ep_update<T, CM, V>(A, tA, a, r, c, ex, ey, ez, qp, k, m, hx_km, hy_km, q_km, ev, ek);
+        ep_fetch<T, CM, V>(A, a, r, c, ex, ey, ez, hx, hy, hz, qp, k + 2, k1);
         if (CM != CLS_UNIFORM) tA = coef_pair<CM>(a.etab, B.cla, B.clb);
-        ep_update<T, CM, V>(B, tB, a, r, c, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
+        ep_update<T, CM, V>(B, tB, a, r, c, ex, ey, ez, qp, k + 1, m, hx_km, hy_km, q_km, ev, ek);
     }
     if (k + 1 < k1) {
-        ep_fetch<T, CM, V>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        ep_fetch<T, CM, V>(B, a, r, c, ex, ey, ez, hx, hy, hz, qp, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_pair<CM>(a.etab, A.cla, A.clb);
-        ep_update<T, CM, V>(A, tA, a, r, c, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
-        ep_update<T, CM, V>(B, tB, a, r, c, ex, ey, ez, k + 1, m, hx_km, hy_km, ev, ek);
+        ep_update<T, CM, V>(A, tA, a, r, c, ex, ey, ez, qp, k, m, hx_km, hy_km, q_km, ev, ek);
+        ep_update<T, CM, V>(B, tB, a, r, c, ex, ey, ez, qp, k + 1, m, hx_km, hy_km, q_km, ev, ek);
     } else {
-        ep_update<T, CM, V>(A, tA, a, r, c, ex, ey, ez, k, m, hx_km, hy_km, ev, ek);
+        ep_update<T, CM, V>(A, tA, a, r, c, ex, ey, ez, qp, k, m, hx_km, hy_km, q_km, ev, ek);
     }
 }
 
 // ============================ H sweep ======================================
 
+template <class T>
 struct HPair {
     u32 ex_kp, ey_kp, ez, ez_jp, ex_jp, hx, hy, hz;   // packed pairs
-    float ez_i2, ey_i2;                                // E at i+2 (x neighbour of cell 1)
+    QW<T> qe_kp, qe_jp, qh;                           // exponent words of E at k+1, E at row j+1, H
+    float ez_i2, ey_i2;                               // E at i+2 (x neighbour of cell 1)
     float p_yx[2], p_zx[2], p_xy[2], p_zy[2], p_xz[2], p_yz[2];
     int cla, clb;
 };
@@ -383,10 +516,11 @@ __device__ __forceinline__ bool hp_zslab(const KArgs &a, int k)
 }
 
 template <class T, int CM, int V>
-__device__ __forceinline__ void hp_fetch(HPair &p, const KArgs &a, const PRow &r, const PCell (&c)[2], const T *ex,
-                                         const T *ey, const T *ez, const T *hx, const T *hy, const T *hz, int k,
-                                         int k1)
+__device__ __forceinline__ void hp_fetch(HPair<T> &p, const KArgs &a, const PRow &r, const PCell (&c)[2],
+                                         const T *ex, const T *ey, const T *ez, const T *hx, const T *hy,
+                                         const T *hz, const QPtrs &qp, int k, int k1)
 {
+    typedef typename psi_type<T>::type PT;
     ConstKArgs &ra = rare_args(a);
     constexpr u32 S = sizeof(T);
     const u32 o = r.o0 + (u32)k * r.ob;
@@ -397,11 +531,15 @@ __device__ __forceinline__ void hp_fetch(HPair &p, const KArgs &a, const PRow &r
     p.ez = ld2(ez, o);
     p.ez_jp = ld2(ez, ojp);
     p.ex_jp = ld2(ex, ojp);
-    p.ez_i2 = ld(at(ez, o + 2 * S));
-    p.ey_i2 = ld(at(ey, o + 2 * S));
     p.hx = ldn2(hx, o);
     p.hy = ldn2(hy, o);
     p.hz = ldn2(hz, o);
+    p.qe_kp = ldq<T>(qp.ex, okp);
+    p.qe_jp = ldq<T>(qp.ex, ojp);
+    p.qh = ldq<T>(qp.hx, o);
+    const QW<T> qe_i2 = ldq<T>(qp.ex, o + 2 * S);
+    p.ez_i2 = ld1(ez, o + 2 * S, qe_i2, 2);
+    p.ey_i2 = ld1(ey, o + 2 * S, qe_i2, 1);
     p.cla = p.clb = 0;
     if (CM != CLS_UNIFORM && k + 1 < k1)
         class_pair<CM>(a.cls, (u32)(k + 1) * (u32)a.plane + (u32)r.col, p.cla, p.clb);
@@ -409,38 +547,39 @@ __device__ __forceinline__ void hp_fetch(HPair &p, const KArgs &a, const PRow &r
         p.p_yx[q] = p.p_zx[q] = p.p_xy[q] = p.p_zy[q] = p.p_xz[q] = p.p_yz[q] = 0.f;
         if (c[q].xs) {
             const u32 qq = (((u32)k * ra.ny + r.j) * ra.Lx + c[q].sx) * S;
-            p.p_yx[q] = ld(at((const T *)ra.psi_hyx, qq));
-            p.p_zx[q] = ld(at((const T *)ra.psi_hzx, qq));
+            p.p_yx[q] = ld(at((const PT *)ra.psi_hyx, qq));
+            p.p_zx[q] = ld(at((const PT *)ra.psi_hzx, qq));
         }
     }
     if (r.ys) {
         const u32 qq = (((u32)k * ra.Ly + r.sy) * ra.px + c[0].i) * S;
-        const u32 u = ld2((const T *)ra.psi_hxy, qq), v = ld2((const T *)ra.psi_hzy, qq);
-        p.p_xy[0] = lo_f(u, ex);
-        p.p_xy[1] = hi_f(u, ex);
-        p.p_zy[0] = lo_f(v, ex);
-        p.p_zy[1] = hi_f(v, ex);
+        const u32 u = ld2((const PT *)ra.psi_hxy, qq), v = ld2((const PT *)ra.psi_hzy, qq);
+        p.p_xy[0] = lo_f(u, (const PT *)nullptr);
+        p.p_xy[1] = hi_f(u, (const PT *)nullptr);
+        p.p_zy[0] = lo_f(v, (const PT *)nullptr);
+        p.p_zy[1] = hi_f(v, (const PT *)nullptr);
     }
     if ((V & V_Z) && hp_zslab(a, k)) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
         const u32 qq = (((u32)sz * ra.ny + r.j) * ra.px + c[0].i) * S;
-        const u32 u = ld2((const T *)ra.psi_hxz, qq), v = ld2((const T *)ra.psi_hyz, qq);
-        p.p_xz[0] = lo_f(u, ex);
-        p.p_xz[1] = hi_f(u, ex);
-        p.p_yz[0] = lo_f(v, ex);
-        p.p_yz[1] = hi_f(v, ex);
+        const u32 u = ld2((const PT *)ra.psi_hxz, qq), v = ld2((const PT *)ra.psi_hyz, qq);
+        p.p_xz[0] = lo_f(u, (const PT *)nullptr);
+        p.p_xz[1] = hi_f(u, (const PT *)nullptr);
+        p.p_yz[0] = lo_f(v, (const PT *)nullptr);
+        p.p_yz[1] = hi_f(v, (const PT *)nullptr);
     }
 }
 
 // H update of one cell of the pair; the arithmetic of kernels.hip h_update.
 template <class T, int V>
-__device__ __forceinline__ void h_cell(const KArgs &a, const PRow &r, const PCell &c, int q, const HPair &p,
+__device__ __forceinline__ void h_cell(const KArgs &a, const PRow &r, const PCell &c, int q, const HPair<T> &p,
                                        const Coef &t, int k, int m, float ex_kp, float ey_kp, float ezc,
                                        float ez_ip, float ey_ip, float ez_jp, float ex_jp, float hx0, float hy0,
                                        float hz0, float ex_c, float ey_c, int &ev, int &ek, bool zs,
                                        const float4 &pzm, float &hxn, float &hyn, float &hzn, bool &ux, bool &uy,
                                        bool &uz)
 {
+    typedef typename psi_type<T>::type PT;
     ConstKArgs &ra = rare_args(a);
     const int nz = a.nz;
     const float4 cx = t.x, cy = t.y, cz = t.z;
@@ -475,8 +614,8 @@ __device__ __forceinline__ void h_cell(const KArgs &a, const PRow &r, const PCel
         const u32 qq = (((u32)k * ra.ny + r.j) * ra.Lx + c.sx) * S;
         const float phy = c.px.x * p.p_yx[q] + c.px.y * (ez_ip - ezc);
         const float phz = c.px.x * p.p_zx[q] + c.px.y * (ey_ip - ey_c);
-        st(at((T *)ra.psi_hyx, qq), phy);
-        st(at((T *)ra.psi_hzx, qq), phz);
+        st(at((PT *)ra.psi_hyx, qq), phy);
+        st(at((PT *)ra.psi_hzx, qq), phz);
         if (khi) hyn = hyn + (cy.y * dx) * phy;
         if (r.jhi) hzn = hzn + (cz.z * dx) * phz;
     }
@@ -484,8 +623,8 @@ __device__ __forceinline__ void h_cell(const KArgs &a, const PRow &r, const PCel
         const u32 qq = (((u32)k * ra.Ly + r.sy) * ra.px + c.i) * S;
         const float phx = r.py.x * p.p_xy[q] + r.py.y * (ez_jp - ezc);
         const float phz = r.py.x * p.p_zy[q] + r.py.y * (ex_jp - ex_c);
-        st(at((T *)ra.psi_hxy, qq), phx);
-        st(at((T *)ra.psi_hzy, qq), phz);
+        st(at((PT *)ra.psi_hxy, qq), phx);
+        st(at((PT *)ra.psi_hzy, qq), phz);
         if (khi) hxn = hxn + (cx.z * dy) * phx;
         if (c.ihi) hzn = hzn + (cz.y * dy) * phz;
     }
@@ -494,38 +633,44 @@ __device__ __forceinline__ void h_cell(const KArgs &a, const PRow &r, const PCel
         const u32 qq = (((u32)sz * ra.ny + r.j) * ra.px + c.i) * S;
         const float phx = pzm.x * p.p_xz[q] + pzm.y * (ey_kp - ey_c);
         const float phy = pzm.x * p.p_yz[q] + pzm.y * (ex_kp - ex_c);
-        st(at((T *)ra.psi_hxz, qq), phx);
-        st(at((T *)ra.psi_hyz, qq), phy);
+        st(at((PT *)ra.psi_hxz, qq), phx);
+        st(at((PT *)ra.psi_hyz, qq), phy);
         if (r.jhi) hxn = hxn + (cx.y * dz) * phx;
         if (c.ihi) hyn = hyn + (cy.z * dz) * phy;
     }
 }
 
 template <class T, int CM, int V>
-__device__ __forceinline__ void hp_update(const HPair &p, const CoefPair &t, const KArgs &a, const PRow &r,
-                                          const PCell (&c)[2], T *hx, T *hy, T *hz, int k, int m, u32 &ex_c,
-                                          u32 &ey_c, int (&ev)[2], int (&ek)[2])
+__device__ __forceinline__ void hp_update(const HPair<T> &p, const CoefPair &t, const KArgs &a, const PRow &r,
+                                          const PCell (&c)[2], T *hx, T *hy, T *hz, const QPtrs &qp, int k, int m,
+                                          u32 &ex_c, u32 &ey_c, QW<T> &q_c, int (&ev)[2], int (&ek)[2])
 {
-    const T *tp = nullptr;
     const u32 o = r.o0 + (u32)k * r.ob;
     const bool zs = (V & V_Z) && hp_zslab(a, k);
     const float4 pzm = zs ? uload(a.pz_prof + a.nz + k) : make_float4(0.f, 0.f, 1.f, 0.f);
-    const float ez0 = lo_f(p.ez, tp), ez1 = hi_f(p.ez, tp);
-    const float exc0 = lo_f(ex_c, tp), exc1 = hi_f(ex_c, tp);
-    const float eyc0 = lo_f(ey_c, tp), eyc1 = hi_f(ey_c, tp);
+    // q_c: the exponent word of E at plane k (carried from the previous plane)
+    const float ez0 = dlo(p.ez, q_c, 2), ez1 = dhi(p.ez, q_c, 2);
+    const float exc0 = dlo(ex_c, q_c, 0), exc1 = dhi(ex_c, q_c, 0);
+    const float eyc0 = dlo(ey_c, q_c, 1), eyc1 = dhi(ey_c, q_c, 1);
+    const float hx0 = dlo(p.hx, p.qh, 0), hx1 = dhi(p.hx, p.qh, 0);
+    const float hy0 = dlo(p.hy, p.qh, 1), hy1 = dhi(p.hy, p.qh, 1);
+    const float hz0 = dlo(p.hz, p.qh, 2), hz1 = dhi(p.hz, p.qh, 2);
     float hx_a, hy_a, hz_a, hx_b, hy_b, hz_b;
     bool uxa, uya, uza, uxb, uyb, uzb;
-    h_cell<T, V>(a, r, c[0], 0, p, t.a, k, m, lo_f(p.ex_kp, tp), lo_f(p.ey_kp, tp), ez0, ez1, eyc1,
-              lo_f(p.ez_jp, tp), lo_f(p.ex_jp, tp), lo_f(p.hx, tp), lo_f(p.hy, tp), lo_f(p.hz, tp), exc0, eyc0,
-              ev[0], ek[0], zs, pzm, hx_a, hy_a, hz_a, uxa, uya, uza);
-    h_cell<T, V>(a, r, c[1], 1, p, t.b, k, m, hi_f(p.ex_kp, tp), hi_f(p.ey_kp, tp), ez1, p.ez_i2, p.ey_i2,
-              hi_f(p.ez_jp, tp), hi_f(p.ex_jp, tp), hi_f(p.hx, tp), hi_f(p.hy, tp), hi_f(p.hz, tp), exc1, eyc1,
-              ev[1], ek[1], zs, pzm, hx_b, hy_b, hz_b, uxb, uyb, uzb);
-    if (uxa || uxb) stn2(hx, o, merge2<T>(p.hx, uxa, hx_a, uxb, hx_b));
-    if (uya || uyb) stn2(hy, o, merge2<T>(p.hy, uya, hy_a, uyb, hy_b));
-    if (uza || uzb) stn2(hz, o, merge2<T>(p.hz, uza, hz_a, uzb, hz_b));
+    h_cell<T, V>(a, r, c[0], 0, p, t.a, k, m, dlo(p.ex_kp, p.qe_kp, 0), dlo(p.ey_kp, p.qe_kp, 1), ez0, ez1, eyc1,
+                 dlo(p.ez_jp, p.qe_jp, 2), dlo(p.ex_jp, p.qe_jp, 0), hx0, hy0, hz0, exc0, eyc0, ev[0], ek[0], zs, pzm,
+                 hx_a, hy_a, hz_a, uxa, uya, uza);
+    h_cell<T, V>(a, r, c[1], 1, p, t.b, k, m, dhi(p.ex_kp, p.qe_kp, 0), dhi(p.ey_kp, p.qe_kp, 1), ez1, p.ez_i2,
+                 p.ey_i2, dhi(p.ez_jp, p.qe_jp, 2), dhi(p.ex_jp, p.qe_jp, 0), hx1, hy1, hz1, exc1, eyc1, ev[1], ek[1],
+                 zs, pzm, hx_b, hy_b, hz_b, uxb, uyb, uzb);
+    const bool va = c[0].valid, vb = c[1].valid;
+    const u32 bx = stp(hx, o, p.hx, hx0, hx1, uxa, hx_a, uxb, hx_b, va, vb);
+    const u32 by = stp(hy, o, p.hy, hy0, hy1, uya, hy_a, uyb, hy_b, va, vb);
+    const u32 bz = stp(hz, o, p.hz, hz0, hz1, uza, hz_a, uzb, hz_b, va, vb);
+    stq<T>(qp.hx, o, bx, by, bz, va && (c[0].i & (BQ_BLOCK - 1)) == 0);
     ex_c = carry(p.ex_kp);
     ey_c = carry(p.ey_kp);
+    q_c = carryq(p.qe_kp);
 }
 
 template <class T, int CM, int V>
@@ -537,7 +682,10 @@ __global__ void __launch_bounds__(TX * BY) h_pair_kernel(const KArgs a, const in
     const int i0 = tx * TXP + 2 * (int)(threadIdx.x & (TX - 1));
     PRow r;
     r.j = ty * BY + (int)(threadIdx.x / TX);
-    if (i0 >= nx || r.j >= ny) return;
+    // block-scaled storage encodes 8-cell blocks across the four lanes of a
+    // quad: the lanes of a block that reaches past nx stay (the block lies
+    // inside the row's padded pitch), storing nothing past nx
+    if (r.j >= ny || (is_bq<T>::value ? (i0 & ~(BQ_BLOCK - 1)) : i0) >= nx) return;
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
     const int j = r.j;
@@ -570,6 +718,7 @@ __global__ void __launch_bounds__(TX * BY) h_pair_kernel(const KArgs a, const in
     T *__restrict__ hx = (T *)a.hx - a.plane;
     T *__restrict__ hy = (T *)a.hy - a.plane;
     T *__restrict__ hz = (T *)a.hz - a.plane;
+    const QPtrs qp = qptrs(a);
 
     int ev[2], ek[2];
     for (int q = 0; q < 2; ++q) {
@@ -579,8 +728,9 @@ __global__ void __launch_bounds__(TX * BY) h_pair_kernel(const KArgs a, const in
 
     u32 ex_c = ld2(ex, r.o0 + (u32)k0 * r.ob);
     u32 ey_c = ld2(ey, r.o0 + (u32)k0 * r.ob);
+    QW<T> q_c = ldq<T>(qp.ex, r.o0 + (u32)k0 * r.ob);
 
-    HPair A, B;
+    HPair<T> A, B;
     CoefPair tA, tB;
     if (CM == CLS_UNIFORM) {
         tA = coef_pair<CM>(a.htab, 0, 0);
@@ -590,23 +740,23 @@ __global__ void __launch_bounds__(TX * BY) h_pair_kernel(const KArgs a, const in
         class_pair<CM>(a.cls, (u32)k0 * (u32)a.plane + (u32)r.col, ca, cb);
         tA = coef_pair<CM>(a.htab, ca, cb);
     }
-    hp_fetch<T, CM, V>(A, a, r, c, ex, ey, ez, hx, hy, hz, k0, k1);
+    hp_fetch<T, CM, V>(A, a, r, c, ex, ey, ez, hx, hy, hz, qp, k0, k1);
     int k = k0;
     for (; k + 2 < k1; k += 2) {
-        hp_fetch<T, CM, V>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        hp_fetch<T, CM, V>(B, a, r, c, ex, ey, ez, hx, hy, hz, qp, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_pair<CM>(a.htab, A.cla, A.clb);
-        hp_update<T, CM, V>(A, tA, a, r, c, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
-        hp_fetch<T, CM, V>(A, a, r, c, ex, ey, ez, hx, hy, hz, k + 2, k1);
+        hp_update<T, CM, V>(A, tA, a, r, c, hx, hy, hz, qp, k, m, ex_c, ey_c, q_c, ev, ek);
+        hp_fetch<T, CM, V>(A, a, r, c, ex, ey, ez, hx, hy, hz, qp, k + 2, k1);
         if (CM != CLS_UNIFORM) tA = coef_pair<CM>(a.htab, B.cla, B.clb);
-        hp_update<T, CM, V>(B, tB, a, r, c, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
+        hp_update<T, CM, V>(B, tB, a, r, c, hx, hy, hz, qp, k + 1, m, ex_c, ey_c, q_c, ev, ek);
     }
     if (k + 1 < k1) {
-        hp_fetch<T, CM, V>(B, a, r, c, ex, ey, ez, hx, hy, hz, k + 1, k1);
+        hp_fetch<T, CM, V>(B, a, r, c, ex, ey, ez, hx, hy, hz, qp, k + 1, k1);
         if (CM != CLS_UNIFORM) tB = coef_pair<CM>(a.htab, A.cla, A.clb);
-        hp_update<T, CM, V>(A, tA, a, r, c, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
-        hp_update<T, CM, V>(B, tB, a, r, c, hx, hy, hz, k + 1, m, ex_c, ey_c, ev, ek);
+        hp_update<T, CM, V>(A, tA, a, r, c, hx, hy, hz, qp, k, m, ex_c, ey_c, q_c, ev, ek);
+        hp_update<T, CM, V>(B, tB, a, r, c, hx, hy, hz, qp, k + 1, m, ex_c, ey_c, q_c, ev, ek);
     } else {
-        hp_update<T, CM, V>(A, tA, a, r, c, hx, hy, hz, k, m, ex_c, ey_c, ev, ek);
+        hp_update<T, CM, V>(A, tA, a, r, c, hx, hy, hz, qp, k, m, ex_c, ey_c, q_c, ev, ek);
     }
 }
 
@@ -641,19 +791,25 @@ static hipError_t launch_pair_t(bool efield, int mode, const KArgs &a, int m, hi
         c.tiles_x = (a.nx + TX - 1) / TX;
         c.ib[0] = a.ib[0] * (TXP / TX);
         c.ib[3] = a.ib[3] * (TXP / TX);
-        const hipError_t e = launch_half_step_general(efield, std::is_same<T, bf16_t>::value ? 2 : 1, mode, c, m, s);
+        const int st = std::is_same<T, bf16_t>::value ? 2 : is_bq<T>::value ? 3 : 1;
+        const hipError_t e = launch_half_step_general(efield, st, mode, c, m, s);
         if (e != hipSuccess) return e;
     }
     return hipGetLastError();
 }
 
-// storage: 1 FP16, 2 BF16.  a.tiles_x counts 128-cell tile columns.
-// gen_cells: the general variant runs as the one-cell kernel.
+// storage: 1 FP16, 2 BF16, 3 block-scaled int16.  a.tiles_x counts
+// 128-cell tile columns.  gen_cells: the general variant runs as the
+// one-cell kernel.
 hipError_t launch_pair_half_step(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s,
                                  bool gen_cells)
 {
-    if (storage == 2) return launch_pair_t<bf16_t>(efield, mode, a, m, s, gen_cells);
-    return launch_pair_t<_Float16>(efield, mode, a, m, s, gen_cells);
+    switch (storage) {
+    case 1: return launch_pair_t<_Float16>(efield, mode, a, m, s, gen_cells);
+    case 2: return launch_pair_t<bf16_t>(efield, mode, a, m, s, gen_cells);
+    case 3: return launch_pair_t<bq16_t>(efield, mode, a, m, s, gen_cells);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 int pair_tile_cols() { return TXP; }

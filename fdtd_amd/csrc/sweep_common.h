@@ -48,11 +48,24 @@ __device__ __forceinline__ void stn(bf16_t *p, float v)
 }
 
 // ---- block-scaled 16-bit fields (bq16_t) ----------------------------------
-// Exponent arrays of the six fields, from the start of their allocations
-// (null for the other storage types).
+// Exponent bytes of the six field components, from the start of the
+// (guard-plane) allocation: ex = the E exponent array, ey = ex + 1, ez =
+// ex + 2, likewise for H (null for the other storage types).
 struct QPtrs {
     std::uint8_t *ex, *ey, *ez, *hx, *hy, *hz;
 };
+
+// The exponent pointers of the kernel arguments (which start at k = 0),
+// moved back by the guard plane below k = 0.
+__device__ __forceinline__ QPtrs qptrs(const KArgs &a)
+{
+    const i64 g = a.plane / BQ_BLOCK * BQ_STRIDE;
+    return QPtrs{a.qex - g, a.qey - g, a.qez - g, a.qhx - g, a.qhy - g, a.qhz - g};
+}
+
+// Byte index of the exponent of the value at byte offset o, from the
+// component's QPtrs entry.
+__device__ __forceinline__ std::uint32_t bq_at(std::uint32_t o) { return (o / (2 * BQ_BLOCK)) * BQ_STRIDE; }
 
 // Value of a field at byte offset o from the start of its allocation.
 template <class T>
@@ -70,20 +83,49 @@ template <>
 __device__ __forceinline__ float fld<bq16_t>(const bq16_t *base, const std::uint8_t *q, std::uint32_t o)
 {
     const int m = *(const std::int16_t *)((const char *)base + o);
-    return bq_value(m, q[o / (2 * BQ_BLOCK)]);
+    return bq_value(m, q[bq_at(o)]);
 }
 template <>
 __device__ __forceinline__ float fldn<bq16_t>(const bq16_t *base, const std::uint8_t *q, std::uint32_t o)
 {
     const int m = __builtin_nontemporal_load((const std::int16_t *)((const char *)base + o));
-    return bq_value(m, q[o / (2 * BQ_BLOCK)]);
+    return bq_value(m, q[bq_at(o)]);
+}
+
+// Lane permutations within a 16-lane row (DPP, no memory access): another
+// lane of the same quad, or the mirrored lane of the same 8-lane half row.
+// A lane whose source lane is inactive keeps its own value.
+constexpr int DPP_QUAD_XOR1 = 0xB1;     // quad_perm [1, 0, 3, 2]
+constexpr int DPP_QUAD_XOR2 = 0x4E;     // quad_perm [2, 3, 0, 1]
+constexpr int DPP_HALF_MIRROR = 0x141;  // row_half_mirror: lane i <- lane 7 - i of its 8
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Block exponent from the block's largest magnitude mx: the smallest e with
+// 2^e > mx / 32767 (e = -BQ_BIAS for an all-zero block), and the int16
+// mantissa of x under it.
+__device__ __forceinline__ int bq_exponent(float mx)
+{
+    int e = -BQ_BIAS;
+    if (mx > 0.f) {
+        (void)frexpf(mx * (1.0f / 32767.0f), &e);   // 2^(e-1) <= max/32767 < 2^e
+        e = max(e, -BQ_BIAS);
+    }
+    return e;
+}
+__device__ __forceinline__ int bq_mantissa(float x, int e)
+{
+    return (int)fminf(fmaxf(rintf(ldexpf(x, -e)), -32767.f), 32767.f);
 }
 
 // Store of a swept field's new value v (when upd) at byte offset o.  For
 // block-scaled storage every lane of the 8-cell block takes part (cell i
 // sits on lane i % 64 of its wave): the block is re-encoded with one
-// exponent from its largest magnitude -- the smallest e with 2^e > max/32767
-// -- and each valid cell stores upd ? v : old (its current value).
+// exponent from its largest magnitude and each valid cell stores
+// upd ? v : old (its current value).
 template <class T>
 __device__ __forceinline__ void fst(T *base, std::uint8_t *, std::uint32_t o, bool upd, float v, float, bool)
 {
@@ -95,18 +137,13 @@ __device__ __forceinline__ void fst<bq16_t>(bq16_t *base, std::uint8_t *q, std::
 {
     const float x = !valid ? 0.f : upd ? v : old;
     float mx = fabsf(x);
-    mx = fmaxf(mx, __shfl_xor(mx, 1, BQ_BLOCK));
-    mx = fmaxf(mx, __shfl_xor(mx, 2, BQ_BLOCK));
-    mx = fmaxf(mx, __shfl_xor(mx, 4, BQ_BLOCK));
-    int e = -BQ_BIAS;
-    if (mx > 0.f) {
-        (void)frexpf(mx * (1.0f / 32767.0f), &e);   // 2^(e-1) <= max/32767 < 2^e
-        e = max(e, -BQ_BIAS);
-    }
-    const float mf = fminf(fmaxf(rintf(ldexpf(x, -e)), -32767.f), 32767.f);
+    mx = fmaxf(mx, dpp_f<DPP_QUAD_XOR1>(mx));
+    mx = fmaxf(mx, dpp_f<DPP_QUAD_XOR2>(mx));
+    mx = fmaxf(mx, dpp_f<DPP_HALF_MIRROR>(mx));
+    const int e = bq_exponent(mx);
     if (valid) {
-        __builtin_nontemporal_store((std::int16_t)(int)mf, (std::int16_t *)((char *)base + o));
-        if ((o / 2) % BQ_BLOCK == 0) q[o / (2 * BQ_BLOCK)] = (std::uint8_t)(e + BQ_BIAS);
+        __builtin_nontemporal_store((std::int16_t)bq_mantissa(x, e), (std::int16_t *)((char *)base + o));
+        if ((o / 2) % BQ_BLOCK == 0) q[bq_at(o)] = (std::uint8_t)(e + BQ_BIAS);
     }
 }
 

@@ -55,11 +55,16 @@ def test_checkpointed_equals_stored_history(engine):
     """Segments are recomputed on one stream while the previous segment's
     reverse sweep runs on another, through two history buffers: bitwise the
     stored-history result for uneven, two-segment, one-step-tail and
-    one-step segmentations."""
+    one-step segmentations.  checkpoint_every < 0 picks the stored history
+    on a scene this small (one segment, same bits)."""
     sc = small_design(steps=150)
     a = engine.Simulation.from_scene(sc, "f32")
     J0, g0 = a.adjoint_gradient(TERMS, sc.steps, 0)
     assert a.info()["adjoint_segments"] == 1
+    Ja, ga = a.adjoint_gradient(TERMS, sc.steps, -1)
+    assert a.info()["adjoint_segments"] == 1
+    assert Ja == J0
+    np.testing.assert_array_equal(ga, g0)
     for K, nseg in ((32, 5), (75, 2), (149, 2), (1, 150)):
         print(f"[adjoint] checkpoint every {K} steps", flush=True)
         J1, g1 = a.adjoint_gradient(TERMS, sc.steps, K)

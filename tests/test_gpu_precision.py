@@ -96,11 +96,14 @@ def _pair_scenes():
 PAIR_MODES = [("1", "pair"), ("1", "cell"), ("0", "pair")]
 
 
-@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("storage", ["f16", "bf16", "bq16"])
 def test_pair_kernels_store_the_one_cell_bits(engine, storage, monkeypatch):
     """The two-cells-per-thread sweeps (default for 16-bit storage) write the
     same bits as the one-cell sweeps (FDTD_PAIR=0), also with their general
-    variant run as the one-cell kernel (FDTD_GEN16=cell): probes, DFT, fields."""
+    variant run as the one-cell kernel (FDTD_GEN16=cell): probes, DFT, fields.
+    Block-scaled storage: the pair sweeps agree on each 8-cell block's
+    exponent across a lane quad, the one-cell sweeps across eight lanes (odd
+    nx = 41: a partial last block and a pair whose second cell is past nx)."""
     import os
     import sys
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
