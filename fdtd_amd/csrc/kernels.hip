@@ -89,16 +89,20 @@ __device__ __forceinline__ void e_fetch(EPlane &p, const KArgs &a, const ECol &c
     const std::uint32_t o = cc.o0 + (std::uint32_t)k * cc.ob;
     const std::uint32_t ojm = o - (std::uint32_t)a.px * S;
     const int px = a.px, ny = a.ny;
-    p.hxc = fld(hx, qp.hx, o);
-    p.hyc = fld(hy, qp.hy, o);
-    p.hzc = fld(hz, qp.hz, o);
-    p.hy_im = fld(hy, qp.hy, o - S);
-    p.hz_im = fld(hz, qp.hz, o - S);
-    p.hx_jm = fld(hx, qp.hx, ojm);
-    p.hz_jm = fld(hz, qp.hz, ojm);
-    p.ex0 = fldn(ex, qp.ex, o);
-    p.ey0 = fldn(ey, qp.ey, o);
-    p.ez0 = fldn(ez, qp.ez, o);
+    // exponent words (block-scaled storage): H at the cell's block, at
+    // i-1's and at row j-1's; E at the cell's block
+    const QW<T> wh = ldq<T>(qp.hx, o), wim = ldq<T>(qp.hx, o - S), wjm = ldq<T>(qp.hx, ojm);
+    const QW<T> we = ldq<T>(qp.ex, o);
+    p.hxc = fldw(hx, o, wh, 0);
+    p.hyc = fldw(hy, o, wh, 1);
+    p.hzc = fldw(hz, o, wh, 2);
+    p.hy_im = fldw(hy, o - S, wim, 1);
+    p.hz_im = fldw(hz, o - S, wim, 2);
+    p.hx_jm = fldw(hx, ojm, wjm, 0);
+    p.hz_jm = fldw(hz, ojm, wjm, 2);
+    p.ex0 = fldnw(ex, o, we, 0);
+    p.ey0 = fldnw(ey, o, we, 1);
+    p.ez0 = fldnw(ez, o, we, 2);
     // class map (no guard planes): the entry of plane k+1
     p.cln = (CM != CLS_UNIFORM && k + 1 < k1)
                 ? class_at<CM>(a.cls, (std::uint32_t)(k + 1) * (std::uint32_t)a.plane + (std::uint32_t)cc.col)
@@ -199,9 +203,10 @@ __device__ __forceinline__ void e_update(const EPlane &p, const Coef &t, const K
         if (cc.jlo) exn = exn + (cx.z * dz) * pex;
         if (cc.ilo) eyn = eyn + (cy.y * dz) * pey;
     }
-    fst(ex, qp.ex, o, upd_x, exn, ex0, cc.valid);
-    fst(ey, qp.ey, o, upd_y, eyn, ey0, cc.valid);
-    fst(ez, qp.ez, o, upd_z, ezn, ez0, cc.valid);
+    const std::uint32_t bx = fst(ex, o, upd_x, exn, ex0, cc.valid);
+    const std::uint32_t by = fst(ey, o, upd_y, eyn, ey0, cc.valid);
+    const std::uint32_t bz = fst(ez, o, upd_z, ezn, ez0, cc.valid);
+    stq<T>(qp.ex, o, bx, by, bz, cc.valid && (cc.i & (BQ_BLOCK - 1)) == 0);
     if (cc.in_snap && k >= a.sb[2] && k < a.sb[2] + a.sb[5])   // stored-precision value, as the next step reads it
         e_snapshot(ra, cc.i, cc.j, k, m, upd_x ? rnd(exn, ex) : ex0, upd_y ? rnd(eyn, ex) : ey0,
                    upd_z ? rnd(ezn, ex) : ez0);
@@ -256,8 +261,10 @@ __global__ void __launch_bounds__(TX * BY) E_OCC e_kernel(const KArgs a, const i
     int ev = cc.valid ? first_event(a, a.e_first, cc.mycol, k0) : -1;
     int ek = first_k(a, ev, cc.mycol);
 
-    float hx_km = fld(hx, qp.hx, cc.o0 + (std::uint32_t)(k0 - 1) * cc.ob);
-    float hy_km = fld(hy, qp.hy, cc.o0 + (std::uint32_t)(k0 - 1) * cc.ob);
+    const std::uint32_t okm = cc.o0 + (std::uint32_t)(k0 - 1) * cc.ob;
+    const QW<T> wkm = ldq<T>(qp.hx, okm);
+    float hx_km = fldw(hx, okm, wkm, 0);
+    float hy_km = fldw(hy, okm, wkm, 1);
 
     EPlane A, B;
     Coef tA, tB;
@@ -327,16 +334,20 @@ __device__ __forceinline__ void h_fetch(HPlane &p, const KArgs &a, const HCol &c
     const std::uint32_t okp = o + cc.ob;
     const std::uint32_t ojp = o + (std::uint32_t)a.px * S;
     const int px = a.px, ny = a.ny;
-    p.ex_kp = fld(ex, qp.ex, okp);
-    p.ey_kp = fld(ey, qp.ey, okp);
-    p.ezc = fld(ez, qp.ez, o);
-    p.ez_ip = fld(ez, qp.ez, o + S);
-    p.ey_ip = fld(ey, qp.ey, o + S);
-    p.ez_jp = fld(ez, qp.ez, ojp);
-    p.ex_jp = fld(ex, qp.ex, ojp);
-    p.hx0 = fldn(hx, qp.hx, o);
-    p.hy0 = fldn(hy, qp.hy, o);
-    p.hz0 = fldn(hz, qp.hz, o);
+    // exponent words (block-scaled storage): E at plane k+1, at the cell's
+    // block, at i+1's and at row j+1's; H at the cell's block
+    const QW<T> wkp = ldq<T>(qp.ex, okp), we = ldq<T>(qp.ex, o), wip = ldq<T>(qp.ex, o + S);
+    const QW<T> wjp = ldq<T>(qp.ex, ojp), wh = ldq<T>(qp.hx, o);
+    p.ex_kp = fldw(ex, okp, wkp, 0);
+    p.ey_kp = fldw(ey, okp, wkp, 1);
+    p.ezc = fldw(ez, o, we, 2);
+    p.ez_ip = fldw(ez, o + S, wip, 2);
+    p.ey_ip = fldw(ey, o + S, wip, 1);
+    p.ez_jp = fldw(ez, ojp, wjp, 2);
+    p.ex_jp = fldw(ex, ojp, wjp, 0);
+    p.hx0 = fldnw(hx, o, wh, 0);
+    p.hy0 = fldnw(hy, o, wh, 1);
+    p.hz0 = fldnw(hz, o, wh, 2);
     p.cln = (CM != CLS_UNIFORM && k + 1 < k1)
                 ? class_at<CM>(a.cls, (std::uint32_t)(k + 1) * (std::uint32_t)a.plane + (std::uint32_t)cc.col)
                 : 0;
@@ -433,9 +444,10 @@ __device__ __forceinline__ void h_update(const HPlane &p, const Coef &t, const K
         if (cc.jhi) hxn = hxn + (cx.y * dz) * phx;
         if (cc.ihi) hyn = hyn + (cy.z * dz) * phy;
     }
-    fst(hx, qp.hx, o, upd_x, hxn, hx0, cc.valid);
-    fst(hy, qp.hy, o, upd_y, hyn, hy0, cc.valid);
-    fst(hz, qp.hz, o, upd_z, hzn, hz0, cc.valid);
+    const std::uint32_t bx = fst(hx, o, upd_x, hxn, hx0, cc.valid);
+    const std::uint32_t by = fst(hy, o, upd_y, hyn, hy0, cc.valid);
+    const std::uint32_t bz = fst(hz, o, upd_z, hzn, hz0, cc.valid);
+    stq<T>(qp.hx, o, bx, by, bz, cc.valid && (cc.i & (BQ_BLOCK - 1)) == 0);
     ex_c = carry(ex_kp);
     ey_c = carry(ey_kp);
 }
@@ -480,8 +492,10 @@ __global__ void __launch_bounds__(TX * BY) H_OCC h_kernel(const KArgs a, const i
     int ev = cc.valid ? first_event(a, a.h_first, cc.mycol, k0) : -1;
     int ek = first_k(a, ev, cc.mycol);
 
-    float ex_c = fld(ex, qp.ex, cc.o0 + (std::uint32_t)k0 * cc.ob);
-    float ey_c = fld(ey, qp.ey, cc.o0 + (std::uint32_t)k0 * cc.ob);
+    const std::uint32_t ok0 = cc.o0 + (std::uint32_t)k0 * cc.ob;
+    const QW<T> wk0 = ldq<T>(qp.ex, ok0);
+    float ex_c = fldw(ex, ok0, wk0, 0);
+    float ey_c = fldw(ey, ok0, wk0, 1);
 
     HPlane A, B;
     Coef tA, tB;

@@ -30,6 +30,13 @@ namespace fdtd {
 
 constexpr int TXP = 2 * TX;   // cells per tile row
 
+// Occupancy floor of the pair sweeps (waves per SIMD), for tuning builds.
+#ifdef FDTD_PAIR_WAVES
+#define P_OCC __attribute__((amdgpu_waves_per_eu(FDTD_PAIR_WAVES)))
+#else
+#define P_OCC
+#endif
+
 hipError_t launch_half_step_general(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s);
 
 typedef std::uint32_t u32;
@@ -83,44 +90,6 @@ __device__ __forceinline__ u32 merge2(u32 old, bool ua, float va, bool ub, float
     return lo | (hi << 16);
 }
 
-// Exponent word of block-scaled storage: the exponent bytes of the three
-// components of E (or H) at one 8-cell block, one 4-byte load (empty for the
-// other storage types, which then carry no extra registers).
-template <class T>
-struct QW {
-};
-template <>
-struct QW<bq16_t> {
-    u32 w;
-};
-
-template <class T>
-__device__ __forceinline__ QW<T> ldq(const std::uint8_t *, u32)
-{
-    return QW<T>{};
-}
-template <>
-__device__ __forceinline__ QW<bq16_t> ldq<bq16_t>(const std::uint8_t *q, u32 off)
-{
-    QW<bq16_t> r;
-    r.w = *(const u32 *)(q + bq_at(off));
-    return r;
-}
-template <class T>
-__device__ __forceinline__ QW<T> carryq(const QW<T> &q)
-{
-    return q;
-}
-template <>
-__device__ __forceinline__ QW<bq16_t> carryq<bq16_t>(const QW<bq16_t> &q)
-{
-    QW<bq16_t> r;
-    r.w = carry(q.w);
-    return r;
-}
-// exponent of component c (bias removed)
-__device__ __forceinline__ int qe(const QW<bq16_t> &q, int c) { return (int)((q.w >> (8 * c)) & 0xFFu) - BQ_BIAS; }
-
 // Values of a packed pair (cell i, cell i+1) of component c.
 template <class T>
 __device__ __forceinline__ float dlo(u32 m, const QW<T> &, int)
@@ -141,18 +110,6 @@ template <>
 __device__ __forceinline__ float dhi<bq16_t>(u32 m, const QW<bq16_t> &q, int c)
 {
     return ldexpf((float)(std::int16_t)(m >> 16), qe(q, c));
-}
-
-// One value of component c at byte offset off.
-template <class T>
-__device__ __forceinline__ float ld1(const T *base, u32 off, const QW<T> &, int)
-{
-    return ld(at(base, off));
-}
-template <>
-__device__ __forceinline__ float ld1<bq16_t>(const bq16_t *base, u32 off, const QW<bq16_t> &q, int c)
-{
-    return ldexpf((float)*(const std::int16_t *)((const char *)base + off), qe(q, c));
 }
 
 // Store of a pair's new values (va / vb where ua / ub, else the old ones,
@@ -183,14 +140,6 @@ __device__ __forceinline__ u32 stp<bq16_t>(bq16_t *base, u32 o, u32 old_m, float
         stn2(base, o, wa | (wb << 16));
     }
     return (u32)(e + BQ_BIAS);
-}
-
-// The three exponent bytes of a swept field at the block of byte offset o,
-// written by the lane of the block's first cell.
-template <class T>
-__device__ __forceinline__ void stq(std::uint8_t *q, u32 o, u32 bx, u32 by, u32 bz, bool first)
-{
-    if (is_bq<T>::value && first) *(u32 *)(q + bq_at(o)) = bx | (by << 8) | (bz << 16);
 }
 
 // class indices of the pair at cell index `cell` (even)
@@ -272,8 +221,8 @@ __device__ __forceinline__ void ep_fetch(EPair<T> &p, const KArgs &a, const PRow
     p.qh_jm = ldq<T>(qp.hx, ojm);
     p.qe = ldq<T>(qp.ex, o);
     const QW<T> qh_im = ldq<T>(qp.hx, o - S);
-    p.hy_im = ld1(hy, o - S, qh_im, 1);
-    p.hz_im = ld1(hz, o - S, qh_im, 2);
+    p.hy_im = fldw(hy, o - S, qh_im, 1);
+    p.hz_im = fldw(hz, o - S, qh_im, 2);
     p.cla = p.clb = 0;
     if (CM != CLS_UNIFORM && k + 1 < k1)
         class_pair<CM>(a.cls, (u32)(k + 1) * (u32)a.plane + (u32)r.col, p.cla, p.clb);
@@ -412,7 +361,7 @@ __device__ __forceinline__ void ep_update(const EPair<T> &p, const CoefPair &t, 
 }
 
 template <class T, int CM, int V>
-__global__ void __launch_bounds__(TX * BY) e_pair_kernel(const KArgs a, const int m)
+__global__ void __launch_bounds__(TX * BY) P_OCC e_pair_kernel(const KArgs a, const int m)
 {
     int tx, ty, tz;
     tile_of_region<V>(a, tx, ty, tz);
@@ -538,8 +487,8 @@ __device__ __forceinline__ void hp_fetch(HPair<T> &p, const KArgs &a, const PRow
     p.qe_jp = ldq<T>(qp.ex, ojp);
     p.qh = ldq<T>(qp.hx, o);
     const QW<T> qe_i2 = ldq<T>(qp.ex, o + 2 * S);
-    p.ez_i2 = ld1(ez, o + 2 * S, qe_i2, 2);
-    p.ey_i2 = ld1(ey, o + 2 * S, qe_i2, 1);
+    p.ez_i2 = fldw(ez, o + 2 * S, qe_i2, 2);
+    p.ey_i2 = fldw(ey, o + 2 * S, qe_i2, 1);
     p.cla = p.clb = 0;
     if (CM != CLS_UNIFORM && k + 1 < k1)
         class_pair<CM>(a.cls, (u32)(k + 1) * (u32)a.plane + (u32)r.col, p.cla, p.clb);
@@ -674,7 +623,7 @@ __device__ __forceinline__ void hp_update(const HPair<T> &p, const CoefPair &t, 
 }
 
 template <class T, int CM, int V>
-__global__ void __launch_bounds__(TX * BY) h_pair_kernel(const KArgs a, const int m)
+__global__ void __launch_bounds__(TX * BY) P_OCC h_pair_kernel(const KArgs a, const int m)
 {
     int tx, ty, tz;
     tile_of_region<V>(a, tx, ty, tz);

@@ -67,29 +67,65 @@ __device__ __forceinline__ QPtrs qptrs(const KArgs &a)
 // component's QPtrs entry.
 __device__ __forceinline__ std::uint32_t bq_at(std::uint32_t o) { return (o / (2 * BQ_BLOCK)) * BQ_STRIDE; }
 
-// Value of a field at byte offset o from the start of its allocation.
+// Exponent word of block-scaled storage: the exponent bytes of the three
+// components of E (or H) at one 8-cell block, one 4-byte load (empty for the
+// other storage types, which then carry no extra registers).
 template <class T>
-__device__ __forceinline__ float fld(const T *base, const std::uint8_t *, std::uint32_t o)
+struct QW {
+};
+template <>
+struct QW<bq16_t> {
+    std::uint32_t w;
+};
+
+template <class T>
+__device__ __forceinline__ QW<T> ldq(const std::uint8_t *, std::uint32_t)
+{
+    return QW<T>{};
+}
+template <>
+__device__ __forceinline__ QW<bq16_t> ldq<bq16_t>(const std::uint8_t *q, std::uint32_t off)
+{
+    QW<bq16_t> r;
+    r.w = *(const std::uint32_t *)(q + bq_at(off));
+    return r;
+}
+template <class T>
+__device__ __forceinline__ QW<T> carryq(const QW<T> &q)
+{
+    return q;
+}
+template <>
+__device__ __forceinline__ QW<bq16_t> carryq<bq16_t>(const QW<bq16_t> &q)
+{
+    QW<bq16_t> r;
+    r.w = carry(q.w);
+    return r;
+}
+// exponent of component c (bias removed)
+__device__ __forceinline__ int qe(const QW<bq16_t> &q, int c) { return (int)((q.w >> (8 * c)) & 0xFFu) - BQ_BIAS; }
+
+// Value of component c of a field at byte offset o from the start of its
+// allocation (w: the exponent word of o's block for block-scaled storage).
+template <class T>
+__device__ __forceinline__ float fldw(const T *base, std::uint32_t o, const QW<T> &, int)
 {
     return ld((const T *)((const char *)base + o));
 }
 template <class T>
-__device__ __forceinline__ float fldn(const T *base, const std::uint8_t *, std::uint32_t o)
+__device__ __forceinline__ float fldnw(const T *base, std::uint32_t o, const QW<T> &, int)
 {
     return ldn((const T *)((const char *)base + o));
 }
-__device__ __forceinline__ float bq_value(int m, int ebyte) { return ldexpf((float)m, ebyte - BQ_BIAS); }
 template <>
-__device__ __forceinline__ float fld<bq16_t>(const bq16_t *base, const std::uint8_t *q, std::uint32_t o)
+__device__ __forceinline__ float fldw<bq16_t>(const bq16_t *base, std::uint32_t o, const QW<bq16_t> &w, int c)
 {
-    const int m = *(const std::int16_t *)((const char *)base + o);
-    return bq_value(m, q[bq_at(o)]);
+    return ldexpf((float)*(const std::int16_t *)((const char *)base + o), qe(w, c));
 }
 template <>
-__device__ __forceinline__ float fldn<bq16_t>(const bq16_t *base, const std::uint8_t *q, std::uint32_t o)
+__device__ __forceinline__ float fldnw<bq16_t>(const bq16_t *base, std::uint32_t o, const QW<bq16_t> &w, int c)
 {
-    const int m = __builtin_nontemporal_load((const std::int16_t *)((const char *)base + o));
-    return bq_value(m, q[bq_at(o)]);
+    return ldexpf((float)__builtin_nontemporal_load((const std::int16_t *)((const char *)base + o)), qe(w, c));
 }
 
 // Lane permutations within a 16-lane row (DPP, no memory access): another
@@ -124,16 +160,17 @@ __device__ __forceinline__ int bq_mantissa(float x, int e)
 // Store of a swept field's new value v (when upd) at byte offset o.  For
 // block-scaled storage every lane of the 8-cell block takes part (cell i
 // sits on lane i % 64 of its wave): the block is re-encoded with one
-// exponent from its largest magnitude and each valid cell stores
-// upd ? v : old (its current value).
+// exponent from its largest magnitude, each valid cell stores upd ? v : old
+// (its current value), and the exponent's byte is returned (for stq).
 template <class T>
-__device__ __forceinline__ void fst(T *base, std::uint8_t *, std::uint32_t o, bool upd, float v, float, bool)
+__device__ __forceinline__ std::uint32_t fst(T *base, std::uint32_t o, bool upd, float v, float, bool)
 {
     if (upd) stn((T *)((char *)base + o), v);
+    return 0;
 }
 template <>
-__device__ __forceinline__ void fst<bq16_t>(bq16_t *base, std::uint8_t *q, std::uint32_t o, bool upd, float v,
-                                            float old, bool valid)
+__device__ __forceinline__ std::uint32_t fst<bq16_t>(bq16_t *base, std::uint32_t o, bool upd, float v, float old,
+                                                     bool valid)
 {
     const float x = !valid ? 0.f : upd ? v : old;
     float mx = fabsf(x);
@@ -141,10 +178,18 @@ __device__ __forceinline__ void fst<bq16_t>(bq16_t *base, std::uint8_t *q, std::
     mx = fmaxf(mx, dpp_f<DPP_QUAD_XOR2>(mx));
     mx = fmaxf(mx, dpp_f<DPP_HALF_MIRROR>(mx));
     const int e = bq_exponent(mx);
-    if (valid) {
-        __builtin_nontemporal_store((std::int16_t)bq_mantissa(x, e), (std::int16_t *)((char *)base + o));
-        if ((o / 2) % BQ_BLOCK == 0) q[bq_at(o)] = (std::uint8_t)(e + BQ_BIAS);
-    }
+    if (valid) __builtin_nontemporal_store((std::int16_t)bq_mantissa(x, e), (std::int16_t *)((char *)base + o));
+    return (std::uint32_t)(e + BQ_BIAS);
+}
+
+// The three exponent bytes of a swept field at the block of byte offset o
+// (q: the field's exponent array, component x), written by the lane of the
+// block's first cell.
+template <class T>
+__device__ __forceinline__ void stq(std::uint8_t *q, std::uint32_t o, std::uint32_t bx, std::uint32_t by,
+                                    std::uint32_t bz, bool first)
+{
+    if (is_bq<T>::value && first) *(std::uint32_t *)(q + bq_at(o)) = bx | (by << 8) | (bz << 16);
 }
 
 // Byte-offset addressing.  Every array a sweep touches is smaller than 4 GiB
