@@ -93,8 +93,9 @@ def adjoint_report(args):
     Checks (seeded design cells): central differences with h = 1e-3 of the
     FP64 oracle's forward runs (host cores) and the FP64 oracle's adjoint,
     for the small workload; for the large one, central differences of FP32
-    engine forward runs (h = 0.05) and the FP32 engine adjoint as the
-    reference of a 16-bit-storage gradient."""
+    engine forward runs (per cell at the --fd-h steps, and along whole-region
+    directions with --fd-dirs) and the FP32 engine adjoint as the reference
+    of a 16-bit-storage gradient."""
     wl = args.workload if args.workload.startswith("adjoint") else "adjoint128"
     sc = scenes.WORKLOADS[wl](**({"steps": args.adjoint_steps} if args.adjoint_steps > 0 else {}))
     terms = [(p, f, 1.0) for p in range(len(sc.probes)) for f in range(len(sc.freqs))]
@@ -186,6 +187,35 @@ def adjoint_report(args):
                  "fp32_adjoint_err_over_max_grad": {"median": float(np.median(errs)) if errs else None,
                                                     "max": float(np.max(errs)) if errs else None,
                                                     "each": [float(f"{e:.3e}") for e in errs]}})
+        if args.fd_dirs > 0:
+            # Directional central differences: J(eps + h v) - J(eps - h v)
+            # against g . (e+ - e-) over every design cell at once (v = sign
+            # of the gradient, then seeded random signs).  The change in J is
+            # a sum over the whole design region, far above the FP32
+            # forwards' rounding of J, so h = 1e-3 is usable here.
+            h = args.fd_dir_h
+            rngd = np.random.default_rng(sc.seed + 1)
+            dirs = [("sign(g)", np.sign(g32).astype(np.float32))]
+            for d in range(1, args.fd_dirs):
+                dirs.append((f"random {d}", rngd.choice(np.array([-1.0, 1.0], dtype=np.float32), g32.size)))
+            res = []
+            for name, v in dirs:
+                _progress(f"{wl}: FP32 engine directional difference, h {h}, direction {name}")
+                vals, es = [], []
+                for sgn in (1, -1):
+                    e2 = (eps + np.float32(sgn * h) * v).astype(np.float32)
+                    es.append(e2.astype(np.float64))
+                    ref.set_design_eps(e2)
+                    ref.run(sc.steps)
+                    X = ref.dft()
+                    vals.append(sum(w * abs(X[p, f]) ** 2 for p, f, w in terms))
+                fd = vals[0] - vals[1]
+                pred = float(np.dot(g32, es[0] - es[1]))
+                pred16 = float(np.dot(g, es[0] - es[1]))
+                res.append({"direction": name, "dJ_fd": fd, "dJ_fp32_adjoint": pred, "dJ_adjoint": pred16,
+                            "rel_err_fp32_adjoint": abs(fd - pred) / abs(pred),
+                            "rel_err_adjoint": abs(fd - pred16) / abs(pred16)})
+            out["fd_directional_fp32_engine"] = {"h": h, "directions": res}
         ref.close()
     out["grad_max_abs"] = gmax
     return out
@@ -203,6 +233,9 @@ def main():
     ap.add_argument("--fd-h", type=lambda v: [float(x) for x in v.split(",")], default=[5e-2],
                     help="adjoint768: comma-separated eps_r steps of the FP32-engine central differences")
     ap.add_argument("--adjoint-steps", type=int, default=0, help="steps of the run (0: the workload's own)")
+    ap.add_argument("--fd-dirs", type=int, default=0,
+                    help="adjoint, large workload: directional central differences (sign(g), then random signs)")
+    ap.add_argument("--fd-dir-h", type=float, default=1e-3, help="step of the directional differences")
     ap.add_argument("--no-cpu", action="store_true", help="adjoint: skip the host FP64 checks")
     args = ap.parse_args()
     if args.report == "accuracy":

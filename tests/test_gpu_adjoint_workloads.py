@@ -9,10 +9,11 @@ design cells and FP64 central differences (h = 1e-3) on 16 seeded cells.
 adjoint768 (768x768x384, 0.5 mm, CPML 16, 4 sources, design 128x128x64 with
 eps_r ~ U[1,6), 8 probes x {5,10,15,20} GHz, FP16 storage, 4000 steps,
 checkpoints every 64 steps): the FP16-storage gradient against the FP32
-engine's stored-history gradient.  The north star's finite-difference check at
-h = 1e-3 cannot be run with FP32 forwards at this size (the FP32 rounding of
-J, relative 1e-5, over 2h swamps the difference); tests/reports.py
-adjoint --workload adjoint768 --fd-h ... states the measured floor.
+engine's stored-history gradient, and both against a central difference at
+h = 1e-3 of FP32 forward runs along sign(g) over the whole design region.  A
+single cell's difference at h = 1e-3 is below the FP32 forwards' rounding of
+J (relative 1e-5); tests/reports.py adjoint --workload adjoint768 --fd-h ...
+states that floor.
 """
 import os
 import sys
@@ -74,8 +75,27 @@ def test_adjoint768_fp16_against_fp32(engine):
     s16.close()
     s32 = engine.Simulation.from_scene(sc, "f32")
     J32, g32 = s32.adjoint_gradient(terms, sc.steps, 0)
-    s32.close()
     rel = np.linalg.norm(g16 - g32) / np.linalg.norm(g32)
     print(f"[adjoint768] J FP16 {J16:.6e} FP32 {J32:.6e}; gradient rel L2 FP16 vs FP32 {rel:.2e}", flush=True)
     assert abs(J16 - J32) <= 1e-2 * J32
     assert rel <= 1e-3
+    # Directional central difference at h = 1e-3 of FP32 forwards along
+    # sign(g) over all 1,048,576 design cells: the change in J is a sum over
+    # the region, far above the FP32 forwards' rounding of J (a single cell's
+    # difference at h = 1e-3 is not).
+    eps = sc.design_eps.astype(np.float32).ravel()
+    v = np.sign(g32).astype(np.float32)
+    vals, es = [], []
+    for sgn in (1, -1):
+        e2 = (eps + np.float32(sgn * 1e-3) * v).astype(np.float32)
+        es.append(e2.astype(np.float64))
+        s32.set_design_eps(e2)
+        s32.run(sc.steps)
+        X = s32.dft()
+        vals.append(sum(w * abs(X[p, f]) ** 2 for p, f, w in terms))
+    s32.close()
+    fd = vals[0] - vals[1]
+    e32 = abs(fd - np.dot(g32, es[0] - es[1])) / abs(fd)
+    e16 = abs(fd - np.dot(g16, es[0] - es[1])) / abs(fd)
+    print(f"[adjoint768] directional FD (h = 1e-3, sign(g)): FP32 adjoint {e32:.2e}, FP16 adjoint {e16:.2e}", flush=True)
+    assert e32 <= 1e-3 and e16 <= 1e-3
