@@ -487,6 +487,259 @@ __global__ void __launch_bounds__(TX * BY) adj_h_kernel(const KArgs a, const AAr
 // and exactly the stored ones of the narrow-grid sweeps above.
 constexpr int TXA = TX - 1;
 
+// The wide-grid sweeps march like the forward ones: every array of a plane
+// is addressed by one 32-bit byte offset from its allocation base (the
+// adjoint arrays are under 4 GiB, checked before the run), and the loads of
+// plane k+1 -- adjoint fields, the neighbour row's values the tile's edge
+// row needs, the class index of the plane after -- are issued before plane k
+// is computed, in two register sets that alternate (loop unrolled by two),
+// with the coefficient-table entries requested one plane ahead.  The
+// arithmetic is that of the per-cell helpers above, in the same order.
+typedef std::uint32_t u32;
+
+__device__ __forceinline__ float ldf(const float *base, u32 off) { return *(const float *)((const char *)base + off); }
+__device__ __forceinline__ void stf(float *base, u32 off, float v) { *(float *)((char *)base + off) = v; }
+
+struct Tab3 {
+    float4 x, y, z;
+};
+
+// The uniform (one-class) coefficients, read once per workgroup into scalar
+// registers and hidden from the optimizer: told that they equal the table's
+// first entry, it re-reads the table through a selected pointer on every
+// plane instead of using the registers.
+__device__ __forceinline__ float4 sreg4(const float4 *p)
+{
+    typedef __attribute__((address_space(4))) const float cf;
+    const cf *q = (const cf *)p;
+    float4 v = make_float4(q[0], q[1], q[2], q[3]);
+    v.x = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v.x)));
+    v.y = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v.y)));
+    v.z = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v.z)));
+    v.w = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v.w)));
+    asm volatile("" : "+s"(v.x), "+s"(v.y), "+s"(v.z), "+s"(v.w));
+    return v;
+}
+__device__ __forceinline__ Tab3 uniform_tab3(const float4 *tab) { return Tab3{sreg4(tab), sreg4(tab + 1), sreg4(tab + 2)}; }
+__device__ __forceinline__ Tab3 tab3(const float4 *tab, int cl)
+{
+    return Tab3{tab[cl * 3 + 0], tab[cl * 3 + 1], tab[cl * 3 + 2]};
+}
+template <int CM>
+__device__ __forceinline__ int class_u(const void *cls, u32 cell)
+{
+    if (CM == CLS_U16) return *(const std::uint16_t *)((const char *)cls + cell * 2u);
+    if (CM == CLS_U32) return (int)*(const std::uint32_t *)((const char *)cls + cell * 4u);
+    return 0;
+}
+
+// Reads of one plane k of the adjoint E sweep.
+struct AEPlane {
+    float e0, e1, e2;   // dJ/dE of the own cell (i, j, k)
+    float h0, h1, h2;   // dJ/dH of (i, j, k)
+    float g0, g1, g2;   // dJ/dH of (i, j-1, k) (the tile's first row)
+    int cln;            // class index of (i, j, k+1) (E coefficients)
+};
+
+// Column constants of the adjoint E sweep.
+struct AECol {
+    int i, j, row, lane, mycol;
+    bool valid, own, ilo, ihi, jlo, jhi, in_box, xs, ys;
+    float4 pxe, pye;
+    u32 o0, ob, cell0;   // byte offset of (i, j, 0) from the allocation base; bytes per plane; cell index of (i, j, 0)
+};
+
+// Every load is issued on every path (lanes and planes that need no value
+// read a cell that exists: the first cell of the arrays, their own cell, or
+// the last plane), so that the compiler's waits for a plane's values count
+// only the younger loads of the next plane.
+// PIPE (the interior variant's two-deep pipeline): every load is issued on
+// every path (lanes and planes that need no value read a cell that exists:
+// the first cell of the arrays, their own cell, or the last plane), so that
+// the compiler's waits for a plane's values count only the younger loads of
+// the next plane.  Otherwise (the edge variants, which march one plane at a
+// time: their extra registers would cost more occupancy than the pipeline
+// hides) only the needed loads are issued.
+template <int CM, int V, bool PIPE>
+__device__ __forceinline__ void ae_fetch(AEPlane &p, const KArgs &a, const AECol &cc, const float *const (&eb)[3],
+                                         const float *const (&hb)[3], int euni, int k, int k1)
+{
+    constexpr bool IN = V == V_IN;
+    const u32 o = cc.o0 + (u32)k * cc.ob;
+    const bool ne = IN || cc.own, nh = IN || cc.valid, ng = cc.row == 0 && (IN || (cc.own && cc.jlo));
+    if (PIPE) {
+        const u32 oe = ne ? o : cc.ob;   // cc.ob: cell (0, 0, 0)
+        const u32 oh = nh ? o : cc.ob;
+        const u32 og = ng ? o - (u32)a.px * 4u : oh;
+        p.e0 = ldf(eb[0], oe);
+        p.e1 = ldf(eb[1], oe);
+        p.e2 = ldf(eb[2], oe);
+        p.h0 = ldf(hb[0], oh);
+        p.h1 = ldf(hb[1], oh);
+        p.h2 = ldf(hb[2], oh);
+        p.g0 = ldf(hb[0], og);
+        p.g1 = ldf(hb[1], og);
+        p.g2 = ldf(hb[2], og);
+        const int kn = min(k + 1, k1 - 1);
+        const int cl = class_u<CM>(a.cls, (ne ? cc.cell0 : 0u) + (u32)kn * (u32)a.plane);
+        p.cln = euni ? 0 : cl;
+    } else {
+        p.e0 = p.e1 = p.e2 = p.h0 = p.h1 = p.h2 = p.g0 = p.g1 = p.g2 = 0.f;
+        if (ne) {
+            p.e0 = ldf(eb[0], o);
+            p.e1 = ldf(eb[1], o);
+            p.e2 = ldf(eb[2], o);
+        }
+        if (nh) {
+            p.h0 = ldf(hb[0], o);
+            p.h1 = ldf(hb[1], o);
+            p.h2 = ldf(hb[2], o);
+        }
+        if (ng) {
+            const u32 og = o - (u32)a.px * 4u;
+            p.g0 = ldf(hb[0], og);
+            p.g1 = ldf(hb[1], og);
+            p.g2 = ldf(hb[2], og);
+        }
+        p.cln = (!euni && ne && k + 1 < k1) ? class_u<CM>(a.cls, cc.cell0 + (u32)(k + 1) * (u32)a.plane) : 0;
+    }
+}
+
+// H coefficients of cell (i, j, k): the uniform ones loaded before the
+// march, else through the class map.
+template <int CM>
+__device__ __forceinline__ void hcoefs(const KArgs &a, const Tab3 &hu, int huni, int i, int j, int k, float4 &hx,
+                                       float4 &hy, float4 &hz)
+{
+    if (huni) {
+        hx = hu.x;
+        hy = hu.y;
+        hz = hu.z;
+    } else {
+        coefs<CM>(a, a.htab, false, (i64)k * a.plane + (i64)j * a.px + i, hx, hy, hz);
+        // keep the two paths apart (merged, they become one load through a
+        // selected pointer into a stack copy of hu)
+        asm volatile("" : "+v"(hx.x), "+v"(hx.y), "+v"(hx.z), "+v"(hy.x), "+v"(hy.y), "+v"(hy.z), "+v"(hz.x),
+                     "+v"(hz.y), "+v"(hz.z));
+    }
+}
+
+template <int CM, int V>
+__device__ __forceinline__ void ae_update(const KArgs &a, const AArgs &b, const AECol &cc, const AEPlane &p,
+                                          const Tab3 &te, const Tab3 &hu, float *const (&eb)[3], HQ (*sq)[TX],
+                                          int k, int m, int fstep, int sten, int huni, HQ &qkm, int &ev, int &ek)
+{
+    constexpr bool IN = V == V_IN;
+    const int i = cc.i, j = cc.j, row = cc.row, lane = cc.lane, mycol = cc.mycol;
+    const int nz = a.nz, ny = a.ny, px = a.px;
+    const u32 o = cc.o0 + (u32)k * cc.ob;
+    const bool klo = IN || k > 0, khi = IN || k < nz - 1;
+    const bool own = cc.own, ilo = cc.ilo, ihi = cc.ihi, jlo = cc.jlo, jhi = cc.jhi;
+    const bool ux = own && ihi && jlo && jhi && klo && khi;
+    const bool uy = own && ilo && ihi && jhi && klo && khi;
+    const bool uz = own && ilo && ihi && jlo && jhi && khi;
+    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    const HQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float ebx = own ? p.e0 : 0.f, eby = own ? p.e1 : 0.f, ebz = own ? p.e2 : 0.f;
+    // source edge and probe events of this cell
+    float cez_src = 0.f;
+    bool is_src = false;
+    int pev = -1;
+    if (k == ek) {
+        pev = ev;
+        while (a.events[ev].k == k && a.events[ev].col == mycol) {
+            if (a.events[ev].kind == EV_SOURCE) {
+                is_src = true;
+                cez_src = a.srcs[a.events[ev].id].ceze;
+            }
+            ++ev;
+        }
+        ek = (a.events[ev].col == mycol) ? a.events[ev].k : INT_MAX;
+    }
+    if (fstep >= 0 && (ux || uy || uz)) {
+        const float4 ex = te.x, ey = te.y, ez = te.z;
+        const bool zs = !IN && zs_e(a, k);
+        if (cc.xs && jhi && khi) {
+            const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
+            b.pb[0][s] = cc.pxe.x * (b.pb[0][s] + (uy ? (ey.z * dx) * eby : 0.f));
+            b.pb[1][s] = cc.pxe.x * (b.pb[1][s] + (uz ? (ez.y * dx) * ebz : 0.f));
+        }
+        if (cc.ys && ihi && khi) {
+            const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
+            b.pb[4][s] = cc.pye.x * (b.pb[4][s] + (ux ? (ex.y * dy) * ebx : 0.f));
+            b.pb[5][s] = cc.pye.x * (b.pb[5][s] + (uz ? (ez.z * dy) * ebz : 0.f));
+        }
+        if (zs && ihi && jhi) {
+            const float4 pze = a.pz_prof[k];
+            const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
+            b.pb[8][s] = pze.x * (b.pb[8][s] + (ux ? (ex.z * dz) * ebx : 0.f));
+            b.pb[9][s] = pze.x * (b.pb[9][s] + (uy ? (ey.y * dz) * eby : 0.f));
+        }
+        if (cc.in_box && k >= b.sb[2] && k < b.sb[2] + b.sb[5]) {
+            const i64 nb = (i64)b.sb[3] * b.sb[4] * b.sb[5];
+            const i64 q = (i - b.sb[0]) + (i64)b.sb[3] * ((j - b.sb[1]) + (i64)b.sb[4] * (k - b.sb[2]));
+            const float *h0 = b.hist + (i64)(fstep - b.hist_base) * 3 * nb;
+            const float *h1 = h0 + 3 * nb;
+            const float e_[3] = {ebx, eby, ebz};
+            const bool u_[3] = {ux, uy, uz};
+            for (int c3 = 0; c3 < 3; ++c3) {
+                const float4 dc = b.dcoef[c3 * nb + q];
+                if (!u_[c3] || (dc.x == 0.f && dc.y == 0.f)) continue;
+                const double em = h0[c3 * nb + q], ep = h1[c3 * nb + q];
+                const double dE = (double)dc.x * em - (double)dc.y * (ep - (double)dc.z * em);
+                b.gedge[c3 * nb + q] += (double)e_[c3] * dE;
+            }
+        }
+        if (ux) ebx = ex.x * ebx;
+        if (uy) eby = ey.x * eby;
+        if (uz) ebz = (is_src ? cez_src : ez.x) * ebz;
+    }
+    if (sten) {
+        HQ q0 = zq;
+        if (IN || cc.valid) {
+            float4 hx, hy, hz;
+            hcoefs<CM>(a, hu, huni, i, j, k, hx, hy, hz);
+            q0 = hq_of<IN>(a, b, hx, hy, hz, p.h0, p.h1, p.h2, i, j, k);
+        }
+        sq[row][lane] = q0;
+        __syncthreads();
+        if (own) {
+            // (i-1, j, k) and (i, j-1, k)
+            const HQ qi = sq[row][lane - 1];
+            HQ qj = zq;
+            if (row > 0) {
+                qj = sq[row - 1][lane];
+            } else if (jlo && (ux || uz)) {
+                float4 hx, hy, hz;
+                hcoefs<CM>(a, hu, huni, i, j - 1, k, hx, hy, hz);
+                qj = hq_of<IN>(a, b, hx, hy, hz, p.g0, p.g1, p.g2, i, j - 1, k);
+            }
+            if (ux) {
+                ebx += (qkm.tx - q0.tx) + (qj.rx - q0.rx);
+                if (uz) ebz += qj.sz - q0.sz;
+            }
+            if (uy) eby += (qkm.sy - q0.sy) + (qi.ry - q0.ry);
+            if (uz) ebz += qi.tz - q0.tz;
+            if (uz && !ux) ebz += qj.sz - q0.sz;
+            qkm = q0;
+            if (pev >= 0) {   // dJ/d(probe sample m) of E probes on this cell
+                for (int e = pev; a.events[e].k == k && a.events[e].col == mycol; ++e) {
+                    const int kind = a.events[e].kind;
+                    if (kind < EV_PROBE || kind > EV_PROBE + 2) continue;
+                    const float gv = b.g[(i64)a.events[e].id * b.gsteps + m];
+                    if (kind == EV_PROBE) ebx += gv;
+                    else if (kind == EV_PROBE + 1) eby += gv;
+                    else ebz += gv;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (ux) stf(eb[0], o, ebx);
+    if (uy) stf(eb[1], o, eby);
+    if (uz) stf(eb[2], o, ebz);
+}
+
 // fstep >= 0: finalize the E half-step of forward step fstep first.
 // sten: gather the transposed H half-step of step m and inject dJ/ds[m].
 template <int CM, int V>
@@ -498,129 +751,229 @@ __global__ void __launch_bounds__(TX * BY)
     __shared__ HQ sq[BY][TX];
     int tx, ty, tz;
     tile_of_region<V>(a, tx, ty, tz);
-    const int lane = (int)(threadIdx.x & (TX - 1));
-    const int row = (int)(threadIdx.x / TX);
+    AECol cc;
+    cc.lane = (int)(threadIdx.x & (TX - 1));
+    cc.row = (int)(threadIdx.x / TX);
     // lane 0 is the halo column before the tile
-    const int i = tx * TXA + lane - 1;
-    const int j = ty * BY + row;
+    const int i = tx * TXA + cc.lane - 1;
+    const int j = ty * BY + cc.row;
+    cc.i = i;
+    cc.j = j;
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
-    const bool valid = IN || (i >= 0 && i < nx && j < ny);   // a cell whose quantities a neighbour reads
-    const bool own = valid && lane > 0;                       // a cell this thread updates
+    cc.valid = IN || (i >= 0 && i < nx && j < ny);   // a cell whose quantities a neighbour reads
+    cc.own = cc.valid && cc.lane > 0;                  // a cell this thread updates
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
-    const bool ilo = IN || i > 0, ihi = IN || i < nx - 1, jlo = IN || j > 0, jhi = IN || j < ny - 1;
-    const int mycol = j * nx + i;
-    int ev = own ? first_event(a, a.e_first, mycol, k0) : -1;
+    cc.ilo = IN || i > 0;
+    cc.ihi = IN || i < nx - 1;
+    cc.jlo = IN || j > 0;
+    cc.jhi = IN || j < ny - 1;
+    cc.mycol = j * nx + i;
+    cc.in_box = i >= b.sb[0] && i < b.sb[0] + b.sb[3] && j >= b.sb[1] && j < b.sb[1] + b.sb[4];
+    cc.xs = !IN && cc.own && xs_e(a, i);
+    cc.ys = !IN && cc.own && ys_e(a, j);
+    cc.pxe = cc.xs ? a.px_prof[i] : NOPROF;
+    cc.pye = cc.ys ? a.py_prof[j] : NOPROF;
+    cc.ob = (u32)a.plane * 4u;
+    cc.cell0 = (u32)(j * px + i);
+    cc.o0 = cc.cell0 * 4u + cc.ob;
+    // allocation bases (guard plane below k = 0)
+    float *const eb[3] = {b.eb[0] - a.plane, b.eb[1] - a.plane, b.eb[2] - a.plane};
+    const float *const ebc[3] = {eb[0], eb[1], eb[2]};
+    const float *const hb[3] = {b.hb[0] - a.plane, b.hb[1] - a.plane, b.hb[2] - a.plane};
+    int ev = cc.own ? first_event(a, a.e_first, cc.mycol, k0) : -1;
     int ek = ev >= 0 ? a.events[ev].k : INT_MAX;
-    const bool in_box = i >= b.sb[0] && i < b.sb[0] + b.sb[3] && j >= b.sb[1] && j < b.sb[1] + b.sb[4];
-    const i64 nb = (i64)b.sb[3] * b.sb[4] * b.sb[5];
-    const bool xs = !IN && own && xs_e(a, i), ys = !IN && own && ys_e(a, j);
-    const float4 pxe = xs ? a.px_prof[i] : NOPROF;
-    const float4 pye = ys ? a.py_prof[j] : NOPROF;
-    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
-    const HQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 
-    HQ qkm = zq;
-    if (own && sten && k0 > 0) qkm = hquant<CM, IN>(a, b, huni, i, j, k0 - 1);
-    for (int k = k0; k < k1; ++k) {
-        const i64 c = (i64)k * a.plane + (i64)j * px + i;
-        const bool klo = IN || k > 0, khi = IN || k < nz - 1;
-        const bool ux = own && ihi && jlo && jhi && klo && khi;
-        const bool uy = own && ilo && ihi && jhi && klo && khi;
-        const bool uz = own && ilo && ihi && jlo && jhi && khi;
-        float ebx = 0.f, eby = 0.f, ebz = 0.f;
-        if (own) {
-            ebx = b.eb[0][c];
-            eby = b.eb[1][c];
-            ebz = b.eb[2][c];
+    HQ qkm{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (cc.own && sten && k0 > 0) qkm = hquant<CM, IN>(a, b, huni, i, j, k0 - 1);
+    const Tab3 hu = uniform_tab3(a.htab);   // the H coefficients when they are one class
+    const int cl0 = (!euni && (IN || cc.own)) ? class_u<CM>(a.cls, cc.cell0 + (u32)k0 * (u32)a.plane) : 0;
+    if (!IN) {
+        // one plane at a time
+        int cl = cl0;
+        for (int k = k0; k < k1; ++k) {
+            AEPlane P;
+            ae_fetch<CM, V, false>(P, a, cc, ebc, hb, euni, k, k1);
+            const Tab3 t = tab3(a.etab, cl);
+            ae_update<CM, V>(a, b, cc, P, t, hu, eb, sq, k, m, fstep, sten, huni, qkm, ev, ek);
+            cl = P.cln;
         }
-        // source edge and probe events of this cell
-        float cez_src = 0.f;
-        bool is_src = false;
-        int pev = -1;
-        if (k == ek) {
-            pev = ev;
-            while (a.events[ev].k == k && a.events[ev].col == mycol) {
-                if (a.events[ev].kind == EV_SOURCE) {
-                    is_src = true;
-                    cez_src = a.srcs[a.events[ev].id].ceze;
-                }
-                ++ev;
-            }
-            ek = (a.events[ev].col == mycol) ? a.events[ev].k : INT_MAX;
-        }
-        if (fstep >= 0 && (ux || uy || uz)) {
-            float4 ex, ey, ez;
-            coefs<CM>(a, a.etab, euni, c, ex, ey, ez);
-            const bool zs = !IN && zs_e(a, k);
-            if (xs && jhi && khi) {
-                const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
-                b.pb[0][s] = pxe.x * (b.pb[0][s] + (uy ? (ey.z * dx) * eby : 0.f));
-                b.pb[1][s] = pxe.x * (b.pb[1][s] + (uz ? (ez.y * dx) * ebz : 0.f));
-            }
-            if (ys && ihi && khi) {
-                const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
-                b.pb[4][s] = pye.x * (b.pb[4][s] + (ux ? (ex.y * dy) * ebx : 0.f));
-                b.pb[5][s] = pye.x * (b.pb[5][s] + (uz ? (ez.z * dy) * ebz : 0.f));
-            }
-            if (zs && ihi && jhi) {
-                const float4 pze = a.pz_prof[k];
-                const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
-                b.pb[8][s] = pze.x * (b.pb[8][s] + (ux ? (ex.z * dz) * ebx : 0.f));
-                b.pb[9][s] = pze.x * (b.pb[9][s] + (uy ? (ey.y * dz) * eby : 0.f));
-            }
-            if (in_box && k >= b.sb[2] && k < b.sb[2] + b.sb[5]) {
-                const i64 q = (i - b.sb[0]) + (i64)b.sb[3] * ((j - b.sb[1]) + (i64)b.sb[4] * (k - b.sb[2]));
-                const float *h0 = b.hist + (i64)(fstep - b.hist_base) * 3 * nb;
-                const float *h1 = h0 + 3 * nb;
-                const float e_[3] = {ebx, eby, ebz};
-                const bool u_[3] = {ux, uy, uz};
-                for (int cc = 0; cc < 3; ++cc) {
-                    const float4 dc = b.dcoef[cc * nb + q];
-                    if (!u_[cc] || (dc.x == 0.f && dc.y == 0.f)) continue;
-                    const double em = h0[cc * nb + q], ep = h1[cc * nb + q];
-                    const double dE = (double)dc.x * em - (double)dc.y * (ep - (double)dc.z * em);
-                    b.gedge[cc * nb + q] += (double)e_[cc] * dE;
-                }
-            }
-            if (ux) ebx = ex.x * ebx;
-            if (uy) eby = ey.x * eby;
-            if (uz) ebz = (is_src ? cez_src : ez.x) * ebz;
-        }
-        if (sten) {
-            const HQ q0 = valid ? hquant<CM, IN>(a, b, huni, i, j, k) : zq;
-            sq[row][lane] = q0;
-            __syncthreads();
-            if (own) {
-                // (i-1, j, k) and (i, j-1, k)
-                const HQ qi = sq[row][lane - 1];
-                const HQ qj = row > 0 ? sq[row - 1][lane]
-                                                 : (jlo && (ux || uz) ? hquant<CM, IN>(a, b, huni, i, j - 1, k) : zq);
-                if (ux) {
-                    ebx += (qkm.tx - q0.tx) + (qj.rx - q0.rx);
-                    if (uz) ebz += qj.sz - q0.sz;
-                }
-                if (uy) eby += (qkm.sy - q0.sy) + (qi.ry - q0.ry);
-                if (uz) ebz += qi.tz - q0.tz;
-                if (uz && !ux) ebz += qj.sz - q0.sz;
-                qkm = q0;
-                if (pev >= 0) {   // dJ/d(probe sample m) of E probes on this cell
-                    for (int e = pev; a.events[e].k == k && a.events[e].col == mycol; ++e) {
-                        const int kind = a.events[e].kind;
-                        if (kind < EV_PROBE || kind > EV_PROBE + 2) continue;
-                        const float gv = b.g[(i64)a.events[e].id * b.gsteps + m];
-                        if (kind == EV_PROBE) ebx += gv;
-                        else if (kind == EV_PROBE + 1) eby += gv;
-                        else ebz += gv;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        if (ux) b.eb[0][c] = ebx;
-        if (uy) b.eb[1][c] = eby;
-        if (uz) b.eb[2][c] = ebz;
+        return;
     }
+    AEPlane A, B;
+    Tab3 tA = tab3(a.etab, cl0), tB;
+    ae_fetch<CM, V, true>(A, a, cc, ebc, hb, euni, k0, k1);
+    int k = k0;
+    // steady state: the fetches are unconditional, so on every path the next
+    // plane's loads are the younger ones when this plane's are awaited
+    for (; k + 2 < k1; k += 2) {
+        ae_fetch<CM, V, true>(B, a, cc, ebc, hb, euni, k + 1, k1);
+        tB = tab3(a.etab, A.cln);
+        ae_update<CM, V>(a, b, cc, A, tA, hu, eb, sq, k, m, fstep, sten, huni, qkm, ev, ek);
+        ae_fetch<CM, V, true>(A, a, cc, ebc, hb, euni, k + 2, k1);
+        tA = tab3(a.etab, B.cln);
+        ae_update<CM, V>(a, b, cc, B, tB, hu, eb, sq, k + 1, m, fstep, sten, huni, qkm, ev, ek);
+    }
+    if (k + 1 < k1) {
+        ae_fetch<CM, V, true>(B, a, cc, ebc, hb, euni, k + 1, k1);
+        tB = tab3(a.etab, A.cln);
+        ae_update<CM, V>(a, b, cc, A, tA, hu, eb, sq, k, m, fstep, sten, huni, qkm, ev, ek);
+        ae_update<CM, V>(a, b, cc, B, tB, hu, eb, sq, k + 1, m, fstep, sten, huni, qkm, ev, ek);
+    } else {
+        ae_update<CM, V>(a, b, cc, A, tA, hu, eb, sq, k, m, fstep, sten, huni, qkm, ev, ek);
+    }
+}
+
+// Reads of one plane k of the adjoint H sweep.
+struct AHPlane {
+    float f0, f1, f2;   // dJ/dE of (i, j, k+1)
+    float h0, h1, h2;   // dJ/dH of the own cell (i, j, k)
+    float d0, d1, d2;   // dJ/dE of (i, j+1, k) (the tile's last row)
+    int cl2;            // class index of (i, j, k+2) (E coefficients of the next plane's z-neighbour)
+    int cld;            // class index of (i, j+1, k+1) (the tile's last row, next plane)
+};
+
+// PIPE: as for ae_fetch.
+template <int CM, int V, bool PIPE>
+__device__ __forceinline__ void ah_fetch(AHPlane &p, const KArgs &a, const AECol &cc, const float *const (&eb)[3],
+                                         const float *const (&hb)[3], int euni, int k)
+{
+    constexpr bool IN = V == V_IN;
+    const int nz = a.nz;
+    const u32 o = cc.o0 + (u32)k * cc.ob;
+    const bool last = cc.row == BY - 1 && (IN || (cc.own && cc.jhi));
+    const bool nf = IN || (cc.valid && k + 1 < nz), nh = IN || cc.own;
+    if (PIPE) {
+        const u32 of = nf ? o + cc.ob : cc.ob;   // cc.ob: cell (0, 0, 0)
+        const u32 oh = nh ? o : cc.ob;
+        const u32 od = last ? o + (u32)a.px * 4u : oh;
+        p.f0 = ldf(eb[0], of);
+        p.f1 = ldf(eb[1], of);
+        p.f2 = ldf(eb[2], of);
+        p.h0 = ldf(hb[0], oh);
+        p.h1 = ldf(hb[1], oh);
+        p.h2 = ldf(hb[2], oh);
+        p.d0 = ldf(eb[0], od);
+        p.d1 = ldf(eb[1], od);
+        p.d2 = ldf(eb[2], od);
+        const u32 c0 = (IN || cc.valid) ? cc.cell0 : 0u;
+        const int c2 = class_u<CM>(a.cls, c0 + (u32)min(k + 2, nz - 1) * (u32)a.plane);
+        const int cd =
+            class_u<CM>(a.cls, (last ? cc.cell0 + (u32)a.px : c0) + (u32)min(k + 1, nz - 1) * (u32)a.plane);
+        p.cl2 = euni ? 0 : c2;
+        p.cld = euni ? 0 : cd;
+    } else {
+        p.f0 = p.f1 = p.f2 = p.h0 = p.h1 = p.h2 = p.d0 = p.d1 = p.d2 = 0.f;
+        if (nf) {
+            p.f0 = ldf(eb[0], o + cc.ob);
+            p.f1 = ldf(eb[1], o + cc.ob);
+            p.f2 = ldf(eb[2], o + cc.ob);
+        }
+        if (nh) {
+            p.h0 = ldf(hb[0], o);
+            p.h1 = ldf(hb[1], o);
+            p.h2 = ldf(hb[2], o);
+        }
+        if (last) {
+            const u32 od = o + (u32)a.px * 4u;
+            p.d0 = ldf(eb[0], od);
+            p.d1 = ldf(eb[1], od);
+            p.d2 = ldf(eb[2], od);
+        }
+        p.cl2 = (!euni && cc.valid && k + 2 < nz) ? class_u<CM>(a.cls, cc.cell0 + (u32)(k + 2) * (u32)a.plane) : 0;
+        p.cld = (!euni && last && k + 1 < nz) ? class_u<CM>(a.cls, cc.cell0 + (u32)a.px + (u32)(k + 1) * (u32)a.plane)
+                                              : 0;
+    }
+}
+
+// tk: E coefficients of (i, j, k+1); tj: of (i, j+1, k) (the tile's last row).
+template <int CM, int V>
+__device__ __forceinline__ void ah_update(const KArgs &a, const AArgs &b, const AECol &cc, const AHPlane &p,
+                                          const Tab3 &tk, const Tab3 &tj, const Tab3 &hu, float *const (&hb)[3],
+                                          EQ (*sq)[TX], int k, int euni, int huni, EQ &q0, int (&sev)[3],
+                                          int (&sk)[3], const int (&scol)[3])
+{
+    constexpr bool IN = V == V_IN;
+    const int i = cc.i, j = cc.j, row = cc.row, lane = cc.lane;
+    const int nz = a.nz, ny = a.ny, px = a.px;
+    const u32 o = cc.o0 + (u32)k * cc.ob;
+    const bool own = cc.own, ihi = cc.ihi, jhi = cc.jhi;
+    const bool khi = IN || k < nz - 1;
+    const bool ux = own && jhi && khi, uy = own && ihi && khi, uz = own && ihi && jhi;
+    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    const EQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const EQ qk = (cc.valid && khi) ? eq_of<IN>(a, b, tk.x, tk.y, tk.z, p.f0, p.f1, p.f2, i, j, k + 1) : zq;
+    float hbx = 0.f, hby = 0.f, hbz = 0.f;
+    if (own) {
+        float4 hx, hy, hz;
+        hcoefs<CM>(a, hu, huni, i, j, k, hx, hy, hz);
+        hbx = p.h0;
+        hby = p.h1;
+        hbz = p.h2;
+        // psi_h adjoint and dJ/dH <- Ch dJ/dH (the H half-step, pointwise part)
+        if (cc.xs) {
+            const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
+            b.pb[2][s] = cc.pxe.x * (b.pb[2][s] + (uy ? (hy.y * dx) * hby : 0.f));
+            b.pb[3][s] = cc.pxe.x * (b.pb[3][s] + (uz ? (hz.z * dx) * hbz : 0.f));
+        }
+        if (cc.ys) {
+            const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
+            b.pb[6][s] = cc.pye.x * (b.pb[6][s] + (ux ? (hx.z * dy) * hbx : 0.f));
+            b.pb[7][s] = cc.pye.x * (b.pb[7][s] + (uz ? (hz.y * dy) * hbz : 0.f));
+        }
+        if (!IN && zs_h(a, k)) {
+            const float4 pzm = a.pz_prof[nz + k];
+            const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
+            b.pb[10][s] = pzm.x * (b.pb[10][s] + (ux ? (hx.y * dz) * hbx : 0.f));
+            b.pb[11][s] = pzm.x * (b.pb[11][s] + (uy ? (hy.z * dz) * hby : 0.f));
+        }
+        if (ux) hbx = hx.x * hbx;
+        if (uy) hby = hy.x * hby;
+        if (uz) hbz = hz.x * hbz;
+    }
+    // transposed E half-step gathered from the E cells that read this H
+    sq[row][lane] = q0;
+    __syncthreads();
+    if (own) {
+        // (i+1, j, k) and (i, j+1, k)
+        const EQ qi = sq[row][lane + 1];
+        EQ qj = zq;
+        if (row < BY - 1) {
+            qj = sq[row + 1][lane];
+        } else if (jhi && (ux || uz)) {
+            qj = eq_of<IN>(a, b, tj.x, tj.y, tj.z, p.d0, p.d1, p.d2, i, j + 1, k);
+        }
+        if (ux) {
+            hbx += (q0.ay - qk.ay) + (q0.bz - qj.bz);
+            if (uz) hbz += q0.ax - qj.ax;
+        }
+        if (uy) hby += (q0.bx - qk.bx) + (q0.az - qi.az);
+        if (uz) hbz += q0.by - qi.by;
+        if (uz && !ux) hbz += q0.ax - qj.ax;
+    }
+    __syncthreads();
+    if (k == sk[0] || k == sk[1] || k == sk[2]) {
+        // source edge on this cell: +terms; at (i+1, j) or (i, j+1): -terms
+        for (int q = 0; q < 3; ++q)
+            while (k == sk[q]) {
+                const float2 d = src_terms<CM>(a, b, euni, q == 1 ? i + 1 : i, q == 2 ? j + 1 : j, k,
+                                               a.events[sev[q]].id);
+                if (q == 0) {
+                    if (uy) hby += d.x;
+                    if (ux) hbx += d.y;
+                } else if (q == 1) {
+                    if (uy) hby -= d.x;
+                } else {
+                    if (ux) hbx -= d.y;
+                }
+                sev[q] = src_from(a, sev[q] + 1, scol[q]);
+                sk[q] = sev[q] >= 0 ? a.events[sev[q]].k : INT_MAX;
+            }
+    }
+    if (ux) stf(hb[0], o, hbx);
+    if (uy) stf(hb[1], o, hby);
+    if (uz) stf(hb[2], o, hbz);
+    q0 = qk;
 }
 
 template <int CM, int V>
@@ -631,28 +984,42 @@ __global__ void __launch_bounds__(TX * BY)
     __shared__ EQ sq[BY][TX];
     int tx, ty, tz;
     tile_of_region<V>(a, tx, ty, tz);
-    const int lane = (int)(threadIdx.x & (TX - 1));
-    const int row = (int)(threadIdx.x / TX);
+    AECol cc;
+    cc.lane = (int)(threadIdx.x & (TX - 1));
+    cc.row = (int)(threadIdx.x / TX);
     // lane 63 is the halo column after the tile
-    const int i = tx * TXA + lane;
-    const int j = ty * BY + row;
+    const int i = tx * TXA + cc.lane;
+    const int j = ty * BY + cc.row;
+    cc.i = i;
+    cc.j = j;
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
-    const bool valid = IN || (i < nx && j < ny);
-    const bool own = valid && lane < TX - 1;
+    cc.valid = IN || (i < nx && j < ny);
+    cc.own = cc.valid && cc.lane < TX - 1;
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
-    const bool ihi = IN || i < nx - 1, jhi = IN || j < ny - 1;
-    const bool xs = !IN && own && xs_h(a, i), ys = !IN && own && ys_h(a, j);
-    const float4 pxm = xs ? a.px_prof[nx + i] : NOPROF;
-    const float4 pym = ys ? a.py_prof[ny + j] : NOPROF;
-    const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
+    cc.ilo = true;
+    cc.jlo = true;
+    cc.ihi = IN || i < nx - 1;
+    cc.jhi = IN || j < ny - 1;
+    cc.mycol = j * nx + i;
+    cc.in_box = false;
+    cc.xs = !IN && cc.own && xs_h(a, i);
+    cc.ys = !IN && cc.own && ys_h(a, j);
+    cc.pxe = cc.xs ? a.px_prof[nx + i] : NOPROF;
+    cc.pye = cc.ys ? a.py_prof[ny + j] : NOPROF;
+    cc.ob = (u32)a.plane * 4u;
+    cc.cell0 = (u32)(j * px + i);
+    cc.o0 = cc.cell0 * 4u + cc.ob;
+    const float *const eb[3] = {b.eb[0] - a.plane, b.eb[1] - a.plane, b.eb[2] - a.plane};
+    float *const hb[3] = {b.hb[0] - a.plane, b.hb[1] - a.plane, b.hb[2] - a.plane};
+    const float *const hbc[3] = {hb[0], hb[1], hb[2]};
     const EQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    EQ q0 = valid ? equant<CM, IN>(a, b, euni, i, j, k0) : zq;
+    EQ q0 = cc.valid ? equant<CM, IN>(a, b, euni, i, j, k0) : zq;
     // lumped-source edges among the Ez edges this thread gathers from: its
     // own cell, (i+1, j) and (i, j+1); event cursor and its plane per column.
     // Rows outside the source rows skip the lookups (no load: a kernel
     // argument range).
-    const bool srow = own && j >= b.src_j0 && j <= b.src_j1;
+    const bool srow = cc.own && j >= b.src_j0 && j <= b.src_j1;
     const int scol[3] = {j * nx + i, j * nx + i + 1, (j + 1) * nx + i};
     const bool sok[3] = {srow, srow && i + 1 < nx, srow && j + 1 < ny};
     int sev[3], sk[3];
@@ -660,78 +1027,50 @@ __global__ void __launch_bounds__(TX * BY)
         sev[q] = sok[q] ? src_from(a, first_event(a, a.e_first, scol[q], k0), scol[q]) : -1;
         sk[q] = sev[q] >= 0 ? a.events[sev[q]].k : INT_MAX;
     }
-    for (int k = k0; k < k1; ++k) {
-        const i64 c = (i64)k * a.plane + (i64)j * px + i;
-        const bool khi = IN || k < nz - 1;
-        const bool ux = own && jhi && khi, uy = own && ihi && khi, uz = own && ihi && jhi;
-        const EQ qk = (valid && khi) ? equant<CM, IN>(a, b, euni, i, j, k + 1) : zq;
-        float hbx = 0.f, hby = 0.f, hbz = 0.f;
-        if (own) {
-            float4 hx, hy, hz;
-            coefs<CM>(a, a.htab, huni, c, hx, hy, hz);
-            hbx = b.hb[0][c];
-            hby = b.hb[1][c];
-            hbz = b.hb[2][c];
-            // psi_h adjoint and dJ/dH <- Ch dJ/dH (the H half-step, pointwise part)
-            if (xs) {
-                const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
-                b.pb[2][s] = pxm.x * (b.pb[2][s] + (uy ? (hy.y * dx) * hby : 0.f));
-                b.pb[3][s] = pxm.x * (b.pb[3][s] + (uz ? (hz.z * dx) * hbz : 0.f));
-            }
-            if (ys) {
-                const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
-                b.pb[6][s] = pym.x * (b.pb[6][s] + (ux ? (hx.z * dy) * hbx : 0.f));
-                b.pb[7][s] = pym.x * (b.pb[7][s] + (uz ? (hz.y * dy) * hbz : 0.f));
-            }
-            if (!IN && zs_h(a, k)) {
-                const float4 pzm = a.pz_prof[nz + k];
-                const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
-                b.pb[10][s] = pzm.x * (b.pb[10][s] + (ux ? (hx.y * dz) * hbx : 0.f));
-                b.pb[11][s] = pzm.x * (b.pb[11][s] + (uy ? (hy.z * dz) * hby : 0.f));
-            }
-            if (ux) hbx = hx.x * hbx;
-            if (uy) hby = hy.x * hby;
-            if (uz) hbz = hz.x * hbz;
+    const Tab3 hu = uniform_tab3(a.htab);   // the H coefficients when they are one class
+    // E coefficients of (i, j, k0+1) and of (i, j+1, k0)
+    int ck, cj;
+    {
+        const bool last = cc.row == BY - 1 && (IN || (cc.own && cc.jhi));
+        const u32 c0 = (IN || cc.valid) ? cc.cell0 : 0u;
+        ck = euni ? 0 : class_u<CM>(a.cls, c0 + (u32)min(k0 + 1, nz - 1) * (u32)a.plane);
+        cj = euni ? 0 : class_u<CM>(a.cls, (last ? cc.cell0 + (u32)px : c0) + (u32)k0 * (u32)a.plane);
+    }
+    if (!IN) {
+        // one plane at a time
+        for (int k = k0; k < k1; ++k) {
+            AHPlane P;
+            ah_fetch<CM, V, false>(P, a, cc, eb, hbc, euni, k);
+            const Tab3 tk = tab3(a.etab, ck), tj = tab3(a.etab, cj);
+            ah_update<CM, V>(a, b, cc, P, tk, tj, hu, hb, sq, k, euni, huni, q0, sev, sk, scol);
+            ck = P.cl2;
+            cj = P.cld;
         }
-        // transposed E half-step gathered from the E cells that read this H
-        sq[row][lane] = q0;
-        __syncthreads();
-        if (own) {
-            // (i+1, j, k) and (i, j+1, k)
-            const EQ qi = sq[row][lane + 1];
-            const EQ qj = row < BY - 1 ? sq[row + 1][lane]
-                                                  : (jhi && (ux || uz) ? equant<CM, IN>(a, b, euni, i, j + 1, k) : zq);
-            if (ux) {
-                hbx += (q0.ay - qk.ay) + (q0.bz - qj.bz);
-                if (uz) hbz += q0.ax - qj.ax;
-            }
-            if (uy) hby += (q0.bx - qk.bx) + (q0.az - qi.az);
-            if (uz) hbz += q0.by - qi.by;
-            if (uz && !ux) hbz += q0.ax - qj.ax;
-        }
-        __syncthreads();
-        if (k == sk[0] || k == sk[1] || k == sk[2]) {
-            // source edge on this cell: +terms; at (i+1, j) or (i, j+1): -terms
-            for (int q = 0; q < 3; ++q)
-                while (k == sk[q]) {
-                    const float2 d = src_terms<CM>(a, b, euni, q == 1 ? i + 1 : i, q == 2 ? j + 1 : j, k,
-                                                   a.events[sev[q]].id);
-                    if (q == 0) {
-                        if (uy) hby += d.x;
-                        if (ux) hbx += d.y;
-                    } else if (q == 1) {
-                        if (uy) hby -= d.x;
-                    } else {
-                        if (ux) hbx -= d.y;
-                    }
-                    sev[q] = src_from(a, sev[q] + 1, scol[q]);
-                    sk[q] = sev[q] >= 0 ? a.events[sev[q]].k : INT_MAX;
-                }
-        }
-        if (ux) b.hb[0][c] = hbx;
-        if (uy) b.hb[1][c] = hby;
-        if (uz) b.hb[2][c] = hbz;
-        q0 = qk;
+        return;
+    }
+    AHPlane A, B;
+    // E coefficients of (i, j, k+1) and of (i, j+1, k) for plane k of the set
+    Tab3 tkA = tab3(a.etab, ck), tjA = tab3(a.etab, cj), tkB, tjB;
+    ah_fetch<CM, V, true>(A, a, cc, eb, hbc, euni, k0);
+    int k = k0;
+    for (; k + 2 < k1; k += 2) {
+        ah_fetch<CM, V, true>(B, a, cc, eb, hbc, euni, k + 1);
+        tkB = tab3(a.etab, A.cl2);
+        tjB = tab3(a.etab, A.cld);
+        ah_update<CM, V>(a, b, cc, A, tkA, tjA, hu, hb, sq, k, euni, huni, q0, sev, sk, scol);
+        ah_fetch<CM, V, true>(A, a, cc, eb, hbc, euni, k + 2);
+        tkA = tab3(a.etab, B.cl2);
+        tjA = tab3(a.etab, B.cld);
+        ah_update<CM, V>(a, b, cc, B, tkB, tjB, hu, hb, sq, k + 1, euni, huni, q0, sev, sk, scol);
+    }
+    if (k + 1 < k1) {
+        ah_fetch<CM, V, true>(B, a, cc, eb, hbc, euni, k + 1);
+        tkB = tab3(a.etab, A.cl2);
+        tjB = tab3(a.etab, A.cld);
+        ah_update<CM, V>(a, b, cc, A, tkA, tjA, hu, hb, sq, k, euni, huni, q0, sev, sk, scol);
+        ah_update<CM, V>(a, b, cc, B, tkB, tjB, hu, hb, sq, k + 1, euni, huni, q0, sev, sk, scol);
+    } else {
+        ah_update<CM, V>(a, b, cc, A, tkA, tjA, hu, hb, sq, k, euni, huni, q0, sev, sk, scol);
     }
 }
 

@@ -655,6 +655,11 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     if (in.storage == FDTD_STORE_BQ16)
         return fail(c, FDTD_E_UNSUPPORTED, "the adjoint runs with FP32, FP16 or BF16 storage, not block-scaled int16");
     if (n_steps < 1 || n_steps > in.steps_cap) return fail(c, FDTD_E_RANGE, "n_steps outside [1, %d]", in.steps_cap);
+    // the adjoint sweeps address their FP32 arrays (two guard planes
+    // included) and the class map with 32-bit byte offsets
+    if ((size_t)c->field_elems * 4 >= ((size_t)1 << 32))
+        return fail(c, FDTD_E_UNSUPPORTED, "grid too large for the adjoint: each FP32 adjoint array (with its two "
+                    "guard planes) must be under 4 GiB (%zu bytes)", (size_t)c->field_elems * 4);
     const int *d = in.design;
     const int dnx = d[3] - d[0], dny = d[4] - d[1], dnz = d[5] - d[2];
     if (n_grad < (size_t)dnx * dny * dnz) return fail(c, FDTD_E_INVALID, "gradient buffer too small");
