@@ -469,6 +469,17 @@ hipError_t do_step(fdtd_ctx *c, const KArgs &base, int m, hipEvent_t ev0 = nullp
     };
     auto split = [&](int sweep) {
         for (int q = 0; q < 6; ++q) a.ib[q] = c->box[sweep][q];
+        a.sub[3] = 0;
+        if (a.lc[3] > 0) {   // light-cone box: its tiles of this sweep's tiling, general variant
+            for (int q = 0; q < 6; ++q) a.ib[q] = 0;
+            const int tc = c->pair ? pair_tile_cols() : tile_cols(), by = tile_rows();
+            a.sub[0] = a.lc[0] / tc;
+            a.sub[3] = (a.lc[3] + tc - 1) / tc - a.sub[0];
+            a.sub[1] = a.lc[1] / by;
+            a.sub[4] = (a.lc[4] + by - 1) / by - a.sub[1];
+            a.sub[2] = a.lc[2] / a.kchunk;
+            a.sub[5] = (a.lc[5] + a.kchunk - 1) / a.kchunk - a.sub[2];
+        }
     };
     split(0);
     if (ev0) (void)hipEventRecord(ev0, c->stream);
@@ -852,6 +863,26 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     // ---- reverse sweep, segment by segment.  Segmented: the forward state of
     // segment sgi is recomputed from its checkpoint on c->stream into history
     // buffer sgi % 2 while the reverse sweep of segment sgi + 1 runs on rs.
+    // A step moves information one cell per axis (E reads H at i-1, H reads E
+    // at i+1), so over a segment of at most Kseg steps the design history
+    // depends only on the checkpoint within Kseg + 1 cells of the snapshot
+    // box: the recompute sweeps that box, dilated by Kseg + 2, alone, when
+    // it holds under 60 % of the grid's cells (FDTD_LIGHTCONE=0: the whole
+    // grid).  Cells outside it keep stale values, and every segment restarts
+    // from a full checkpoint.
+    int lcone[6] = {0, 0, 0, 0, 0, 0};
+    {
+        const char *lv = std::getenv("FDTD_LIGHTCONE");
+        const int R = Kseg + 2;
+        const int lo[3] = {std::max(0, d[0] - R), std::max(0, d[1] - R), std::max(0, d[2] - R)};
+        const int hi[3] = {std::min(nx, d[0] + bx + R), std::min(ny, d[1] + by + R), std::min(nz, d[2] + bz + R)};
+        const double frac = (double)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]) / ((double)nx * ny * nz);
+        if (nseg > 1 && frac < 0.6 && !(lv && std::strcmp(lv, "0") == 0))
+            for (int v = 0; v < 3; ++v) {
+                lcone[v] = lo[v];
+                lcone[v + 3] = hi[v];
+            }
+    }
     auto recompute = [&](int sgi) -> int {
         const int s0 = sgi * Kseg, s1 = std::min(n, s0 + Kseg), q = sgi % 2;
         if (sgi + 2 < nseg) HIPCHK(c, hipStreamWaitEvent(c->stream, sync.v[2 + q], 0));   // buffer q free
@@ -868,6 +899,7 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
         ra.rec = 0;
         ra.snap = hbuf[q];
         ra.snap_base = s0;
+        for (int v = 0; v < 6; ++v) ra.lc[v] = lcone[v];
         bind_sets(c, ra);
         HIPCHK(c, launch_snap(in.storage, ra, hbuf[q], 0, c->stream));
         for (int m = s0; m < s1; ++m) HIPCHK(c, do_step(c, ra, m));

@@ -86,9 +86,19 @@ __device__ __forceinline__ int class_of(const void *cls, i64 c)
 }
 
 // Workgroup -> tile: XCD x (= blockIdx % 8 under round-robin dispatch) gets
-// the contiguous tile range [start_x, start_x + count_x).
+// the contiguous tile range [start_x, start_x + count_x).  With a light-cone
+// sub-range (a.sub), the tiles of that range only.
 __device__ __forceinline__ void tile_of(const KArgs &a, int &tx, int &ty, int &tz)
 {
+    if (a.sub[3] > 0) {
+        const int b = blockIdx.x, nt = a.sub[3] * a.sub[4] * a.sub[5];
+        const int q = nt >> 3, r = nt & 7, x = b & 7;
+        const int t = x * q + (x < r ? x : r) + (b >> 3);
+        tx = a.sub[0] + t % a.sub[3];
+        ty = a.sub[1] + (t / a.sub[3]) % a.sub[4];
+        tz = a.sub[2] + t / (a.sub[3] * a.sub[4]);
+        return;
+    }
     const int nt = a.tiles_x * a.tiles_y * ((a.nz + a.kchunk - 1) / a.kchunk);
     const int b = blockIdx.x;
     const int q = nt >> 3, r = nt & 7, x = b & 7;

@@ -82,6 +82,12 @@ struct KArgs {
     // z-blocks), V_GEN over the rows and z-blocks outside it.  Without one
     // (all zero), V_GEN covers every tile.
     int ib[6];
+    // Light-cone recompute: only the cell box [lc0, lc3) x [lc1, lc4) x
+    // [lc2, lc5) is swept (lc[3] == 0: the whole grid), as sub = (tile x,
+    // tile row, z-block of its first tile; tiles, rows, z-blocks), set for
+    // each sweep's tiling when the step is launched (no interior box then).
+    int lc[6];
+    int sub[6];
     int rec;                   // record probes / DFT (0 during adjoint recompute)
     int e_uni, h_uni;          // E / H coefficients are one class (no class-map read)
     // design-region snapshot: E^{m+1} of the E edges in the box
@@ -97,9 +103,9 @@ inline void region_tiles(const KArgs &a, int &n_in, int &n_x, int &n_gen)
 {
     const int *ib = a.ib;
     const int tzs = (a.nz + a.kchunk - 1) / a.kchunk;
-    if (ib[5] == ib[2]) {   // no box
+    if (ib[5] == ib[2]) {   // no box: every tile, or the light-cone tiles
         n_in = n_x = 0;
-        n_gen = a.tiles_x * a.tiles_y * tzs;
+        n_gen = a.sub[3] > 0 ? a.sub[3] * a.sub[4] * a.sub[5] : a.tiles_x * a.tiles_y * tzs;
         return;
     }
     const int bx = ib[3] - ib[0], by = ib[4] - ib[1], bz = ib[5] - ib[2];

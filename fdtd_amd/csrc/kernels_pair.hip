@@ -740,6 +740,10 @@ static hipError_t launch_pair_t(bool efield, int mode, const KArgs &a, int m, hi
         c.tiles_x = (a.nx + TX - 1) / TX;
         c.ib[0] = a.ib[0] * (TXP / TX);
         c.ib[3] = a.ib[3] * (TXP / TX);
+        if (c.sub[3] > 0) {   // light-cone range in one-cell tile columns
+            c.sub[0] = a.sub[0] * (TXP / TX);
+            c.sub[3] = min(a.sub[3] * (TXP / TX), c.tiles_x - c.sub[0]);
+        }
         const int st = std::is_same<T, bf16_t>::value ? 2 : is_bq<T>::value ? 3 : 1;
         const hipError_t e = launch_half_step_general(efield, st, mode, c, m, s);
         if (e != hipSuccess) return e;

@@ -147,3 +147,29 @@ def test_adjoint_gather_paths_bitwise(engine, monkeypatch, nx):
     for (J0, g0), (J1, g1) in zip(out["stored"], out["lds"]):
         assert J0 == J1
         np.testing.assert_array_equal(g0, g1)
+
+
+@pytest.mark.parametrize("storage,gen16", [("f32", None), ("f16", None), ("f16", "pair")])
+def test_light_cone_recompute_is_bitwise(engine, monkeypatch, storage, gen16):
+    """Checkpointed segments recomputed over the snapshot box dilated by the
+    segment length plus two (a small design region in a large grid: 4 % of
+    the cells) give the stored-history gradient and the whole-grid recompute's
+    bit for bit (FP32 one-cell sweeps; FP16 pair sweeps with the one-cell or
+    their own general variant over the light-cone tiles)."""
+    if gen16:
+        monkeypatch.setenv("FDTD_GEN16", gen16)
+    sc = scenes.adjoint(nx=140, ny=120, nz=100, steps=150, L=8, d=1e-3, design=(6, 5, 4), eps_range=(1.0, 4.0),
+                        seed=31, probe_freqs=(12e9,), name="adjoint-lightcone")
+    terms = [(0, 0, 1.0)]
+    print(f"[adjoint] light cone {sc.nx}x{sc.ny}x{sc.nz} storage={storage} gen16={gen16}", flush=True)
+    sim = engine.Simulation.from_scene(sc, storage)
+    J0, g0 = sim.adjoint_gradient(terms, sc.steps, 0)
+    J1, g1 = sim.adjoint_gradient(terms, sc.steps, 16)
+    assert sim.info()["adjoint_segments"] == 10
+    monkeypatch.setenv("FDTD_LIGHTCONE", "0")
+    J2, g2 = sim.adjoint_gradient(terms, sc.steps, 16)
+    sim.close()
+    assert np.abs(g0).max() > 0
+    assert J0 == J1 == J2
+    np.testing.assert_array_equal(g1, g0)
+    np.testing.assert_array_equal(g2, g0)
