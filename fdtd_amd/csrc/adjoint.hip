@@ -57,15 +57,19 @@ struct HQ {
 
 // From the cell's H coefficients and its dJ/dH (and the psi_h adjoint in
 // memory).
-template <bool IN>
+// Region variant V: bit V_X / V_Y / V_Z set when the cell can meet an x / y
+// / z CPML slab or outer face (the predicates of the other axes compile out).
+template <int V>
 __device__ __forceinline__ HQ hq_of(const KArgs &a, const AArgs &b, const float4 &hx, const float4 &hy,
                                     const float4 &hz, float hbx, float hby, float hbz, int i, int j, int k)
 {
+    constexpr bool PX = (V & V_X) != 0, PY = (V & V_Y) != 0, PZ = (V & V_Z) != 0;
     HQ q{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
-    const bool ux = IN || (j < ny - 1 && k < nz - 1), uy = IN || (i < nx - 1 && k < nz - 1);
-    const bool uz = IN || (i < nx - 1 && j < ny - 1);
-    const bool xs = !IN && xs_h(a, i), ys = !IN && ys_h(a, j), zs = !IN && zs_h(a, k);
+    const bool ihi = !PX || i < nx - 1, jhi = !PY || j < ny - 1, khi = !PZ || k < nz - 1;
+    const bool ux = jhi && khi, uy = ihi && khi;
+    const bool uz = ihi && jhi;
+    const bool xs = PX && xs_h(a, i), ys = PY && ys_h(a, j), zs = PZ && zs_h(a, k);
     const float4 pxm = xs ? a.px_prof[nx + i] : NOPROF;
     const float4 pym = ys ? a.py_prof[ny + j] : NOPROF;
     const float4 pzm = zs ? a.pz_prof[nz + k] : NOPROF;
@@ -114,15 +118,17 @@ struct EQ {
 
 // From the cell's E coefficients and its dJ/dE (and the psi_e adjoint in
 // memory).
-template <bool IN>
+template <int V>
 __device__ __forceinline__ EQ eq_of(const KArgs &a, const AArgs &b, const float4 &ex, const float4 &ey,
                                     const float4 &ez, float ebx, float eby, float ebz, int i, int j, int k)
 {
+    constexpr bool PX = (V & V_X) != 0, PY = (V & V_Y) != 0, PZ = (V & V_Z) != 0;
     EQ q{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
-    if (!IN && (i >= nx - 1 || j >= ny - 1 || k >= nz - 1)) return q;   // no E update, no psi_e
-    const bool ux = IN || (j > 0 && k > 0), uy = IN || (i > 0 && k > 0), uz = IN || (i > 0 && j > 0);
-    const bool xs = !IN && xs_e(a, i), ys = !IN && ys_e(a, j), zs = !IN && zs_e(a, k);
+    if ((PX && i >= nx - 1) || (PY && j >= ny - 1) || (PZ && k >= nz - 1)) return q;   // no E update, no psi_e
+    const bool ilo = !PX || i > 0, jlo = !PY || j > 0, klo = !PZ || k > 0;
+    const bool ux = jlo && klo, uy = ilo && klo, uz = ilo && jlo;
+    const bool xs = PX && xs_e(a, i), ys = PY && ys_e(a, j), zs = PZ && zs_e(a, k);
     const float4 pxe = xs ? a.px_prof[i] : NOPROF;
     const float4 pye = ys ? a.py_prof[j] : NOPROF;
     const float4 pze = zs ? a.pz_prof[k] : NOPROF;
@@ -163,23 +169,24 @@ __device__ __forceinline__ EQ eq_of(const KArgs &a, const AArgs &b, const float4
 
 // The same quantities with the cell's coefficients and adjoint read from
 // memory (the wide-grid sweeps, which compute them where they gather).
-template <int CM, bool IN>
+template <int CM, int V>
 __device__ HQ hquant(const KArgs &a, const AArgs &b, bool huni, int i, int j, int k)
 {
     const i64 c = (i64)k * a.plane + (i64)j * a.px + i;
     float4 hx, hy, hz;
     coefs<CM>(a, a.htab, huni, c, hx, hy, hz);
-    return hq_of<IN>(a, b, hx, hy, hz, b.hb[0][c], b.hb[1][c], b.hb[2][c], i, j, k);
+    return hq_of<V>(a, b, hx, hy, hz, b.hb[0][c], b.hb[1][c], b.hb[2][c], i, j, k);
 }
 
-template <int CM, bool IN>
+template <int CM, int V>
 __device__ EQ equant(const KArgs &a, const AArgs &b, bool euni, int i, int j, int k)
 {
-    if (!IN && (i >= a.nx - 1 || j >= a.ny - 1 || k >= a.nz - 1)) return EQ{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (((V & V_X) && i >= a.nx - 1) || ((V & V_Y) && j >= a.ny - 1) || ((V & V_Z) && k >= a.nz - 1))
+        return EQ{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const i64 c = (i64)k * a.plane + (i64)j * a.px + i;
     float4 ex, ey, ez;
     coefs<CM>(a, a.etab, euni, c, ex, ey, ez);
-    return eq_of<IN>(a, b, ex, ey, ez, b.eb[0][c], b.eb[1][c], b.eb[2][c], i, j, k);
+    return eq_of<V>(a, b, ex, ey, ez, b.eb[0][c], b.eb[1][c], b.eb[2][c], i, j, k);
 }
 
 // First lumped-source event of column `col` at or after event `ev` (probe
@@ -261,21 +268,22 @@ __global__ void __launch_bounds__(TX * BY)
     adj_e_kernel(const KArgs a, const AArgs b, const int m, const int fstep, const int sten, const int euni)
 {
     constexpr bool IN = V == V_IN;   // interior tiles: no slab, face or range predicates
+    constexpr bool PX = (V & V_X) != 0, PY = (V & V_Y) != 0, PZ = (V & V_Z) != 0;   // predicated axes
     int tx, ty, tz;
     tile_of_region<V>(a, tx, ty, tz);
     const int i = tx * TX + (int)(threadIdx.x & (TX - 1));
     const int j = ty * BY + (int)(threadIdx.x / TX);
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
-    if (!IN && (i >= nx || j >= ny)) return;
+    if ((PX && i >= nx) || (PY && j >= ny)) return;
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
-    const bool ilo = IN || i > 0, ihi = IN || i < nx - 1, jlo = IN || j > 0, jhi = IN || j < ny - 1;
+    const bool ilo = !PX || i > 0, ihi = !PX || i < nx - 1, jlo = !PY || j > 0, jhi = !PY || j < ny - 1;
     const int mycol = j * nx + i;
     int ev = first_event(a, a.e_first, mycol, k0);
     int ek = ev >= 0 ? a.events[ev].k : INT_MAX;
     const bool in_box = i >= b.sb[0] && i < b.sb[0] + b.sb[3] && j >= b.sb[1] && j < b.sb[1] + b.sb[4];
     const i64 nb = (i64)b.sb[3] * b.sb[4] * b.sb[5];
-    const bool xs = !IN && xs_e(a, i), ys = !IN && ys_e(a, j);
+    const bool xs = PX && xs_e(a, i), ys = PY && ys_e(a, j);
     const float4 pxe = xs ? a.px_prof[i] : NOPROF;
     const float4 pye = ys ? a.py_prof[j] : NOPROF;
     const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
@@ -285,7 +293,7 @@ __global__ void __launch_bounds__(TX * BY)
     if (sten && k0 > 0) qkm = ld_hq(b, (i64)(k0 - 1) * a.plane + (i64)j * px + i);
     for (int k = k0; k < k1; ++k) {
         const i64 c = (i64)k * a.plane + (i64)j * px + i;
-        const bool klo = IN || k > 0, khi = IN || k < nz - 1;
+        const bool klo = !PZ || k > 0, khi = !PZ || k < nz - 1;
         const bool ux = ihi && jlo && jhi && klo && khi;
         const bool uy = ilo && ihi && jhi && klo && khi;
         const bool uz = ilo && ihi && jlo && jhi && khi;
@@ -308,7 +316,7 @@ __global__ void __launch_bounds__(TX * BY)
             ek = (a.events[ev].col == mycol) ? a.events[ev].k : INT_MAX;
         }
         if (fstep >= 0 && (ux || uy || uz)) {
-            const bool zs = !IN && zs_e(a, k);
+            const bool zs = PZ && zs_e(a, k);
             if (xs && jhi && khi) {
                 const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
                 b.pb[0][s] = pxe.x * (b.pb[0][s] + (uy ? (ey.z * dx) * eby : 0.f));
@@ -371,7 +379,7 @@ __global__ void __launch_bounds__(TX * BY)
         if (uy) b.eb[1][c] = eby;
         if (uz) b.eb[2][c] = ebz;
         // the transposed E quantities of this cell, for the H sweep's gather
-        st_eq(b, c, eq_of<IN>(a, b, ex, ey, ez, ebx, eby, ebz, i, j, k));
+        st_eq(b, c, eq_of<V>(a, b, ex, ey, ez, ebx, eby, ebz, i, j, k));
     }
 }
 
@@ -379,16 +387,17 @@ template <int CM, int V>
 __global__ void __launch_bounds__(TX * BY) adj_h_kernel(const KArgs a, const AArgs b, const int euni, const int huni)
 {
     constexpr bool IN = V == V_IN;   // interior tiles: no slab, face or range predicates
+    constexpr bool PX = (V & V_X) != 0, PY = (V & V_Y) != 0, PZ = (V & V_Z) != 0;   // predicated axes
     int tx, ty, tz;
     tile_of_region<V>(a, tx, ty, tz);
     const int i = tx * TX + (int)(threadIdx.x & (TX - 1));
     const int j = ty * BY + (int)(threadIdx.x / TX);
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
-    if (!IN && (i >= nx || j >= ny)) return;
+    if ((PX && i >= nx) || (PY && j >= ny)) return;
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
-    const bool ihi = IN || i < nx - 1, jhi = IN || j < ny - 1;
-    const bool xs = !IN && xs_h(a, i), ys = !IN && ys_h(a, j);
+    const bool ihi = !PX || i < nx - 1, jhi = !PY || j < ny - 1;
+    const bool xs = PX && xs_h(a, i), ys = PY && ys_h(a, j);
     const float4 pxm = xs ? a.px_prof[nx + i] : NOPROF;
     const float4 pym = ys ? a.py_prof[ny + j] : NOPROF;
     const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
@@ -408,7 +417,7 @@ __global__ void __launch_bounds__(TX * BY) adj_h_kernel(const KArgs a, const AAr
     }
     for (int k = k0; k < k1; ++k) {
         const i64 c = (i64)k * a.plane + (i64)j * px + i;
-        const bool khi = IN || k < nz - 1;
+        const bool khi = !PZ || k < nz - 1;
         const bool ux = jhi && khi, uy = ihi && khi, uz = ihi && jhi;
         const EQ qk = khi ? ld_eq(b, c + a.plane) : zq;
         float4 hx, hy, hz;
@@ -425,7 +434,7 @@ __global__ void __launch_bounds__(TX * BY) adj_h_kernel(const KArgs a, const AAr
             b.pb[6][s] = pym.x * (b.pb[6][s] + (ux ? (hx.z * dy) * hbx : 0.f));
             b.pb[7][s] = pym.x * (b.pb[7][s] + (uz ? (hz.y * dy) * hbz : 0.f));
         }
-        if (!IN && zs_h(a, k)) {
+        if (PZ && zs_h(a, k)) {
             const float4 pzm = a.pz_prof[nz + k];
             const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
             b.pb[10][s] = pzm.x * (b.pb[10][s] + (ux ? (hx.y * dz) * hbx : 0.f));
@@ -469,7 +478,7 @@ __global__ void __launch_bounds__(TX * BY) adj_h_kernel(const KArgs a, const AAr
         if (uy) b.hb[1][c] = hby;
         if (uz) b.hb[2][c] = hbz;
         // the transposed H quantities of this cell, for the next E sweep's gather
-        st_hq(b, c, hq_of<IN>(a, b, hx, hy, hz, hbx, hby, hbz, i, j, k));
+        st_hq(b, c, hq_of<V>(a, b, hx, hy, hz, hbx, hby, hbz, i, j, k));
         q0 = qk;
     }
 }
@@ -630,10 +639,11 @@ __device__ __forceinline__ void ae_update(const KArgs &a, const AArgs &b, const 
                                           int k, int m, int fstep, int sten, int huni, HQ &qkm, int &ev, int &ek)
 {
     constexpr bool IN = V == V_IN;
+    constexpr bool PZ = (V & V_Z) != 0;
     const int i = cc.i, j = cc.j, row = cc.row, lane = cc.lane, mycol = cc.mycol;
     const int nz = a.nz, ny = a.ny, px = a.px;
     const u32 o = cc.o0 + (u32)k * cc.ob;
-    const bool klo = IN || k > 0, khi = IN || k < nz - 1;
+    const bool klo = !PZ || k > 0, khi = !PZ || k < nz - 1;
     const bool own = cc.own, ilo = cc.ilo, ihi = cc.ihi, jlo = cc.jlo, jhi = cc.jhi;
     const bool ux = own && ihi && jlo && jhi && klo && khi;
     const bool uy = own && ilo && ihi && jhi && klo && khi;
@@ -658,7 +668,7 @@ __device__ __forceinline__ void ae_update(const KArgs &a, const AArgs &b, const 
     }
     if (fstep >= 0 && (ux || uy || uz)) {
         const float4 ex = te.x, ey = te.y, ez = te.z;
-        const bool zs = !IN && zs_e(a, k);
+        const bool zs = PZ && zs_e(a, k);
         if (cc.xs && jhi && khi) {
             const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
             b.pb[0][s] = cc.pxe.x * (b.pb[0][s] + (uy ? (ey.z * dx) * eby : 0.f));
@@ -699,7 +709,7 @@ __device__ __forceinline__ void ae_update(const KArgs &a, const AArgs &b, const 
         if (IN || cc.valid) {
             float4 hx, hy, hz;
             hcoefs<CM>(a, hu, huni, i, j, k, hx, hy, hz);
-            q0 = hq_of<IN>(a, b, hx, hy, hz, p.h0, p.h1, p.h2, i, j, k);
+            q0 = hq_of<V>(a, b, hx, hy, hz, p.h0, p.h1, p.h2, i, j, k);
         }
         sq[row][lane] = q0;
         __syncthreads();
@@ -712,7 +722,7 @@ __device__ __forceinline__ void ae_update(const KArgs &a, const AArgs &b, const 
             } else if (jlo && (ux || uz)) {
                 float4 hx, hy, hz;
                 hcoefs<CM>(a, hu, huni, i, j - 1, k, hx, hy, hz);
-                qj = hq_of<IN>(a, b, hx, hy, hz, p.g0, p.g1, p.g2, i, j - 1, k);
+                qj = hq_of<V>(a, b, hx, hy, hz, p.g0, p.g1, p.g2, i, j - 1, k);
             }
             if (ux) {
                 ebx += (qkm.tx - q0.tx) + (qj.rx - q0.rx);
@@ -748,6 +758,7 @@ __global__ void __launch_bounds__(TX * BY)
                      const int huni)
 {
     constexpr bool IN = V == V_IN;   // interior tiles: no slab, face or range predicates
+    constexpr bool PX = (V & V_X) != 0, PY = (V & V_Y) != 0, PZ = (V & V_Z) != 0;   // predicated axes
     __shared__ HQ sq[BY][TX];
     int tx, ty, tz;
     tile_of_region<V>(a, tx, ty, tz);
@@ -760,18 +771,18 @@ __global__ void __launch_bounds__(TX * BY)
     cc.i = i;
     cc.j = j;
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
-    cc.valid = IN || (i >= 0 && i < nx && j < ny);   // a cell whose quantities a neighbour reads
+    cc.valid = (!PX || (i >= 0 && i < nx)) && (!PY || j < ny);   // a cell whose quantities a neighbour reads
     cc.own = cc.valid && cc.lane > 0;                  // a cell this thread updates
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
-    cc.ilo = IN || i > 0;
-    cc.ihi = IN || i < nx - 1;
-    cc.jlo = IN || j > 0;
-    cc.jhi = IN || j < ny - 1;
+    cc.ilo = !PX || i > 0;
+    cc.ihi = !PX || i < nx - 1;
+    cc.jlo = !PY || j > 0;
+    cc.jhi = !PY || j < ny - 1;
     cc.mycol = j * nx + i;
     cc.in_box = i >= b.sb[0] && i < b.sb[0] + b.sb[3] && j >= b.sb[1] && j < b.sb[1] + b.sb[4];
-    cc.xs = !IN && cc.own && xs_e(a, i);
-    cc.ys = !IN && cc.own && ys_e(a, j);
+    cc.xs = PX && cc.own && xs_e(a, i);
+    cc.ys = PY && cc.own && ys_e(a, j);
     cc.pxe = cc.xs ? a.px_prof[i] : NOPROF;
     cc.pye = cc.ys ? a.py_prof[j] : NOPROF;
     cc.ob = (u32)a.plane * 4u;
@@ -785,7 +796,7 @@ __global__ void __launch_bounds__(TX * BY)
     int ek = ev >= 0 ? a.events[ev].k : INT_MAX;
 
     HQ qkm{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (cc.own && sten && k0 > 0) qkm = hquant<CM, IN>(a, b, huni, i, j, k0 - 1);
+    if (cc.own && sten && k0 > 0) qkm = hquant<CM, V>(a, b, huni, i, j, k0 - 1);
     const Tab3 hu = uniform_tab3(a.htab);   // the H coefficients when they are one class
     const int cl0 = (!euni && (IN || cc.own)) ? class_u<CM>(a.cls, cc.cell0 + (u32)k0 * (u32)a.plane) : 0;
     if (!IN) {
@@ -842,7 +853,7 @@ __device__ __forceinline__ void ah_fetch(AHPlane &p, const KArgs &a, const AECol
     const int nz = a.nz;
     const u32 o = cc.o0 + (u32)k * cc.ob;
     const bool last = cc.row == BY - 1 && (IN || (cc.own && cc.jhi));
-    const bool nf = IN || (cc.valid && k + 1 < nz), nh = IN || cc.own;
+    const bool nf = (IN || cc.valid) && (!(V & V_Z) || k + 1 < nz), nh = IN || cc.own;
     if (PIPE) {
         const u32 of = nf ? o + cc.ob : cc.ob;   // cc.ob: cell (0, 0, 0)
         const u32 oh = nh ? o : cc.ob;
@@ -894,15 +905,16 @@ __device__ __forceinline__ void ah_update(const KArgs &a, const AArgs &b, const 
                                           int (&sk)[3], const int (&scol)[3])
 {
     constexpr bool IN = V == V_IN;
+    constexpr bool PZ = (V & V_Z) != 0;
     const int i = cc.i, j = cc.j, row = cc.row, lane = cc.lane;
     const int nz = a.nz, ny = a.ny, px = a.px;
     const u32 o = cc.o0 + (u32)k * cc.ob;
     const bool own = cc.own, ihi = cc.ihi, jhi = cc.jhi;
-    const bool khi = IN || k < nz - 1;
+    const bool khi = !PZ || k < nz - 1;
     const bool ux = own && jhi && khi, uy = own && ihi && khi, uz = own && ihi && jhi;
     const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
     const EQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    const EQ qk = (cc.valid && khi) ? eq_of<IN>(a, b, tk.x, tk.y, tk.z, p.f0, p.f1, p.f2, i, j, k + 1) : zq;
+    const EQ qk = (cc.valid && khi) ? eq_of<V>(a, b, tk.x, tk.y, tk.z, p.f0, p.f1, p.f2, i, j, k + 1) : zq;
     float hbx = 0.f, hby = 0.f, hbz = 0.f;
     if (own) {
         float4 hx, hy, hz;
@@ -921,7 +933,7 @@ __device__ __forceinline__ void ah_update(const KArgs &a, const AArgs &b, const 
             b.pb[6][s] = cc.pye.x * (b.pb[6][s] + (ux ? (hx.z * dy) * hbx : 0.f));
             b.pb[7][s] = cc.pye.x * (b.pb[7][s] + (uz ? (hz.y * dy) * hbz : 0.f));
         }
-        if (!IN && zs_h(a, k)) {
+        if (PZ && zs_h(a, k)) {
             const float4 pzm = a.pz_prof[nz + k];
             const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
             b.pb[10][s] = pzm.x * (b.pb[10][s] + (ux ? (hx.y * dz) * hbx : 0.f));
@@ -941,7 +953,7 @@ __device__ __forceinline__ void ah_update(const KArgs &a, const AArgs &b, const 
         if (row < BY - 1) {
             qj = sq[row + 1][lane];
         } else if (jhi && (ux || uz)) {
-            qj = eq_of<IN>(a, b, tj.x, tj.y, tj.z, p.d0, p.d1, p.d2, i, j + 1, k);
+            qj = eq_of<V>(a, b, tj.x, tj.y, tj.z, p.d0, p.d1, p.d2, i, j + 1, k);
         }
         if (ux) {
             hbx += (q0.ay - qk.ay) + (q0.bz - qj.bz);
@@ -981,6 +993,7 @@ __global__ void __launch_bounds__(TX * BY)
     adj_h_lds_kernel(const KArgs a, const AArgs b, const int euni, const int huni)
 {
     constexpr bool IN = V == V_IN;   // interior tiles: no slab, face or range predicates
+    constexpr bool PX = (V & V_X) != 0, PY = (V & V_Y) != 0, PZ = (V & V_Z) != 0;   // predicated axes
     __shared__ EQ sq[BY][TX];
     int tx, ty, tz;
     tile_of_region<V>(a, tx, ty, tz);
@@ -993,18 +1006,18 @@ __global__ void __launch_bounds__(TX * BY)
     cc.i = i;
     cc.j = j;
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
-    cc.valid = IN || (i < nx && j < ny);
+    cc.valid = (!PX || i < nx) && (!PY || j < ny);
     cc.own = cc.valid && cc.lane < TX - 1;
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
     cc.ilo = true;
     cc.jlo = true;
-    cc.ihi = IN || i < nx - 1;
-    cc.jhi = IN || j < ny - 1;
+    cc.ihi = !PX || i < nx - 1;
+    cc.jhi = !PY || j < ny - 1;
     cc.mycol = j * nx + i;
     cc.in_box = false;
-    cc.xs = !IN && cc.own && xs_h(a, i);
-    cc.ys = !IN && cc.own && ys_h(a, j);
+    cc.xs = PX && cc.own && xs_h(a, i);
+    cc.ys = PY && cc.own && ys_h(a, j);
     cc.pxe = cc.xs ? a.px_prof[nx + i] : NOPROF;
     cc.pye = cc.ys ? a.py_prof[ny + j] : NOPROF;
     cc.ob = (u32)a.plane * 4u;
@@ -1014,7 +1027,7 @@ __global__ void __launch_bounds__(TX * BY)
     float *const hb[3] = {b.hb[0] - a.plane, b.hb[1] - a.plane, b.hb[2] - a.plane};
     const float *const hbc[3] = {hb[0], hb[1], hb[2]};
     const EQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    EQ q0 = cc.valid ? equant<CM, IN>(a, b, euni, i, j, k0) : zq;
+    EQ q0 = cc.valid ? equant<CM, V>(a, b, euni, i, j, k0) : zq;
     // lumped-source edges among the Ez edges this thread gathers from: its
     // own cell, (i+1, j) and (i, j+1); event cursor and its plane per column.
     // Rows outside the source rows skip the lookups (no load: a kernel
@@ -1131,8 +1144,13 @@ hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, 
     } else {
         adj_box(a, tcols, a.ib);
     }
-    int n_in, n_x, n_gen;
-    region_tiles(a, n_in, n_x, n_gen);
+    // wide grids: the z-slab blocks of the box's columns and rows get a
+    // variant with the z predicates only (the general variant costs 2.7x
+    // the interior's time per cell there)
+    a.zsplit = stored ? 0 : 1;
+    a.sub[3] = 0;
+    int n_in, n_x, n_z, n_gen;
+    region_tiles(a, n_in, n_x, n_z, n_gen);
 #define ADJ_E(CMV, V, NT, EU, HU)                                                                                \
     do {                                                                                                         \
         if (stored)                                                                                              \
@@ -1155,10 +1173,12 @@ hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, 
     if (which == 0) {
         if (n_in) ADJ_V(ADJ_E, V_IN, n_in);
         if (n_x) ADJ_V(ADJ_E, V_X, n_x);
+        if (n_z) ADJ_V(ADJ_E, V_Z, n_z);
         if (n_gen) ADJ_V(ADJ_E, V_GEN, n_gen);
     } else {
         if (n_in) ADJ_V(ADJ_H, V_IN, n_in);
         if (n_x) ADJ_V(ADJ_H, V_X, n_x);
+        if (n_z) ADJ_V(ADJ_H, V_Z, n_z);
         if (n_gen) ADJ_V(ADJ_H, V_GEN, n_gen);
     }
 #undef ADJ_V

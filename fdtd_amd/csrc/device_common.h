@@ -133,8 +133,11 @@ __device__ __forceinline__ int remap_tile(int nt)
 // Workgroup -> tile of region variant V, computed from the interior box (no
 // tile lists: the many short workgroups then start without a dependent
 // load).  V_X: tile columns outside [ib0, ib3) of the box's rows and
-// z-blocks.  V_GEN: every z-block outside [ib2, ib5), then the rows outside
-// [ib1, ib4) of the box's z-blocks; the whole grid when there is no box.
+// z-blocks.  V_Z (with zsplit): the box's tile columns and rows in every
+// z-block outside [ib2, ib5).  V_GEN: every z-block outside [ib2, ib5) (with
+// zsplit only its tiles off the box's columns and rows), then the rows
+// outside [ib1, ib4) of the box's z-blocks; the whole grid (or the light-cone
+// range) when there is no box.
 template <int V>
 __device__ __forceinline__ void tile_of_region(const KArgs &a, int &tx, int &ty, int &tz)
 {
@@ -148,18 +151,36 @@ __device__ __forceinline__ void tile_of_region(const KArgs &a, int &tx, int &ty,
         tx = xe < ib[0] ? xe : ib[3] + (xe - ib[0]);
         ty = ib[1] + (t / ex) % by;
         tz = ib[2] + t / (ex * by);
+    } else if (V == V_Z) {
+        const int bx = ib[3] - ib[0], by = ib[4] - ib[1], bz = ib[5] - ib[2];
+        const int tzs = (a.nz + a.kchunk - 1) / a.kchunk;
+        const int t = remap_tile(bx * by * (tzs - bz));
+        const int ze = t / (bx * by);
+        tx = ib[0] + t % bx;
+        ty = ib[1] + (t / bx) % by;
+        tz = ze < ib[2] ? ze : ib[5] + (ze - ib[2]);
     } else if (ib[5] == ib[2]) {
         tile_of(a, tx, ty, tz);
     } else {
         const int txy = a.tiles_x * a.tiles_y, tzs = (a.nz + a.kchunk - 1) / a.kchunk;
-        const int bz = ib[5] - ib[2], ey = a.tiles_y - (ib[4] - ib[1]);
-        const int n1 = txy * (tzs - bz);
+        const int bx = ib[3] - ib[0], by = ib[4] - ib[1], bz = ib[5] - ib[2], ey = a.tiles_y - by;
+        const int per = a.zsplit ? txy - bx * by : txy;   // tiles per z-block outside the box's z range
+        const int n1 = per * (tzs - bz);
         const int t = remap_tile(n1 + a.tiles_x * ey * bz);
         if (t < n1) {
-            const int ze = t / txy;
-            tx = t % a.tiles_x;
-            ty = (t % txy) / a.tiles_x;
+            const int ze = t / per, u = t % per;
             tz = ze < ib[2] ? ze : ib[5] + (ze - ib[2]);
+            if (!a.zsplit || u < ey * a.tiles_x) {
+                // every tile of the plane, or (zsplit) the rows outside the box's
+                const int w = u / a.tiles_x;
+                tx = u % a.tiles_x;
+                ty = !a.zsplit ? w : (w < ib[1] ? w : ib[4] + (w - ib[1]));
+            } else {
+                // (zsplit) the box's rows, columns outside it
+                const int v = u - ey * a.tiles_x, ex = a.tiles_x - bx, xe = v % ex;
+                tx = xe < ib[0] ? xe : ib[3] + (xe - ib[0]);
+                ty = ib[1] + v / ex;
+            }
         } else {
             const int u = t - n1;
             const int ye = (u / a.tiles_x) % ey;

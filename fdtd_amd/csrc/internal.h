@@ -82,6 +82,9 @@ struct KArgs {
     // z-blocks), V_GEN over the rows and z-blocks outside it.  Without one
     // (all zero), V_GEN covers every tile.
     int ib[6];
+    // zsplit: V_Z takes the box's tile columns and rows in the z-blocks
+    // outside it (cells that can meet only a z slab or face), V_GEN the rest.
+    int zsplit;
     // Light-cone recompute: only the cell box [lc0, lc3) x [lc1, lc4) x
     // [lc2, lc5) is swept (lc[3] == 0: the whole grid), as sub = (tile x,
     // tile row, z-block of its first tile; tiles, rows, z-blocks), set for
@@ -98,20 +101,21 @@ struct KArgs {
     int snap_base;
 };
 
-// Tile counts of the three region variants of a sweep (tile_of_region).
-inline void region_tiles(const KArgs &a, int &n_in, int &n_x, int &n_gen)
+// Tile counts of the region variants of a sweep (tile_of_region).
+inline void region_tiles(const KArgs &a, int &n_in, int &n_x, int &n_z, int &n_gen)
 {
     const int *ib = a.ib;
     const int tzs = (a.nz + a.kchunk - 1) / a.kchunk;
     if (ib[5] == ib[2]) {   // no box: every tile, or the light-cone tiles
-        n_in = n_x = 0;
+        n_in = n_x = n_z = 0;
         n_gen = a.sub[3] > 0 ? a.sub[3] * a.sub[4] * a.sub[5] : a.tiles_x * a.tiles_y * tzs;
         return;
     }
     const int bx = ib[3] - ib[0], by = ib[4] - ib[1], bz = ib[5] - ib[2];
     n_in = bx * by * bz;
     n_x = (a.tiles_x - bx) * by * bz;
-    n_gen = a.tiles_x * a.tiles_y * (tzs - bz) + a.tiles_x * (a.tiles_y - by) * bz;
+    n_z = a.zsplit ? bx * by * (tzs - bz) : 0;
+    n_gen = a.tiles_x * a.tiles_y * (tzs - bz) - n_z + a.tiles_x * (a.tiles_y - by) * bz;
 }
 
 // Adjoint argument block (FP32 SI adjoint state; same layouts as the forward).

@@ -546,8 +546,8 @@ static void launch_v(bool efield, int mode, const KArgs &a, int m, int nt, hipSt
 template <class T>
 static hipError_t launch_t(bool efield, int mode, const KArgs &a, int m, hipStream_t s)
 {
-    int nb, nx, ng;
-    region_tiles(a, nb, nx, ng);
+    int nb, nx, nzt, ng;   // nzt = 0: the forward sweeps run without zsplit
+    region_tiles(a, nb, nx, nzt, ng);
     if (nb > 0) launch_v<T, V_IN>(efield, mode, a, m, nb, s);
     if (nx > 0) launch_v<T, V_X>(efield, mode, a, m, nx, s);
     if (ng > 0) launch_v<T, V_GEN>(efield, mode, a, m, ng, s);
@@ -571,8 +571,8 @@ hipError_t launch_half_step(bool efield, int storage, int mode, const KArgs &a, 
 // and z-blocks: the one-cell kernel holds half the per-thread state there.
 hipError_t launch_half_step_general(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s)
 {
-    int nb, nx, ng;
-    region_tiles(a, nb, nx, ng);
+    int nb, nx, nzt, ng;   // nzt = 0: the forward sweeps run without zsplit
+    region_tiles(a, nb, nx, nzt, ng);
     if (ng == 0) return hipSuccess;
     switch (storage) {
     case 0: launch_v<float, V_GEN>(efield, mode, a, m, ng, s); break;

@@ -728,8 +728,8 @@ static void launch_pair_v(bool efield, int mode, const KArgs &a, int m, int nt, 
 template <class T>
 static hipError_t launch_pair_t(bool efield, int mode, const KArgs &a, int m, hipStream_t s, bool gen_cells)
 {
-    int nb, nx, ng;
-    region_tiles(a, nb, nx, ng);
+    int nb, nx, nzt, ng;   // nzt = 0: the forward sweeps run without zsplit
+    region_tiles(a, nb, nx, nzt, ng);
     if (nb > 0) launch_pair_v<T, V_IN>(efield, mode, a, m, nb, s);
     if (nx > 0) launch_pair_v<T, V_X>(efield, mode, a, m, nx, s);
     if (ng > 0 && !gen_cells) launch_pair_v<T, V_GEN>(efield, mode, a, m, ng, s);
