@@ -562,13 +562,14 @@ struct AECol {
 // read a cell that exists: the first cell of the arrays, their own cell, or
 // the last plane), so that the compiler's waits for a plane's values count
 // only the younger loads of the next plane.
-// PIPE (the interior variant's two-deep pipeline): every load is issued on
-// every path (lanes and planes that need no value read a cell that exists:
-// the first cell of the arrays, their own cell, or the last plane), so that
-// the compiler's waits for a plane's values count only the younger loads of
-// the next plane.  Otherwise (the edge variants, which march one plane at a
-// time: their extra registers would cost more occupancy than the pipeline
-// hides) only the needed loads are issued.
+// PIPE (the two-deep pipeline of the interior variants and of the E sweep's
+// x-edge and z-slab ones): every load is issued on every path (lanes and
+// planes that need no value read a cell that exists: the first cell of the
+// arrays, their own cell, or the last plane), so that the compiler's waits
+// for a plane's values count only the younger loads of the next plane.
+// Otherwise (variants that march one plane at a time, where the pipeline's
+// registers cost more occupancy than it hides) only the needed loads are
+// issued.
 template <int CM, int V, bool PIPE>
 __device__ __forceinline__ void ae_fetch(AEPlane &p, const KArgs &a, const AECol &cc, const float *const (&eb)[3],
                                          const float *const (&hb)[3], int euni, int k, int k1)
@@ -799,8 +800,9 @@ __global__ void __launch_bounds__(TX * BY)
     if (cc.own && sten && k0 > 0) qkm = hquant<CM, V>(a, b, huni, i, j, k0 - 1);
     const Tab3 hu = uniform_tab3(a.htab);   // the H coefficients when they are one class
     const int cl0 = (!euni && (IN || cc.own)) ? class_u<CM>(a.cls, cc.cell0 + (u32)k0 * (u32)a.plane) : 0;
-    if (!IN) {
-        // one plane at a time
+    if (V == V_GEN) {
+        // one plane at a time (the x-edge and z-slab variants pipeline like the
+        // interior: adjoint768 x-edge 0.575 -> 0.555 ms, z-slab 0.365 -> 0.357 ms)
         int cl = cl0;
         for (int k = k0; k < k1; ++k) {
             AEPlane P;
@@ -1050,7 +1052,8 @@ __global__ void __launch_bounds__(TX * BY)
         cj = euni ? 0 : class_u<CM>(a.cls, (last ? cc.cell0 + (u32)px : c0) + (u32)k0 * (u32)a.plane);
     }
     if (!IN) {
-        // one plane at a time
+        // one plane at a time (pipelined, the H edge variants measured
+        // slower: x-edge 0.57 -> 0.69 ms, z-slab 0.39 -> 0.48 ms on adjoint768)
         for (int k = k0; k < k1; ++k) {
             AHPlane P;
             ah_fetch<CM, V, false>(P, a, cc, eb, hbc, euni, k);
