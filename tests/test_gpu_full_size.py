@@ -97,7 +97,7 @@ def _errors_vs_fp64(engine):
 
 
 # Regression bands (what each form measures at 3000 steps, with margin), not
-# the north star's target: FP32 2.7e-5 max; FP16 2.1e-2 max / 1.2e-2 median;
+# the 1e-3 accuracy target of the 2-byte forms: FP32 2.7e-5 max; FP16 2.1e-2 max / 1.2e-2 median;
 # block-scaled int16 2.6e-3 / 1.5e-3.  A lost scale or a missed rounding is
 # orders of magnitude off.
 BANDS = {"f32": (1e-4, 5e-5, 2e-4), "f16": (5e-2, 2.5e-2, 4e-2), "bq16": (5e-3, 3e-3, 5e-3)}
@@ -120,7 +120,7 @@ def test_full_size_storage_error_band(engine, storage):
                           "rounded once per step and the roundings accumulate as a random walk (worst probe: FP16 "
                           "2.1e-2, block-scaled int16 2.6e-3)", strict=False)
 def test_16bit_storage_meets_the_1e3_target(engine, storage):
-    """The north star's target for the 2-byte storage forms: every probe
-    within 1e-3 relative L2 of FP64 at slab512's 3000 steps."""
+    """The accuracy target of the 2-byte storage forms: every probe within
+    1e-3 relative L2 of FP64 at slab512's 3000 steps."""
     r, _ = _errors_vs_fp64(engine)[storage]
     assert r.max() <= 1e-3
