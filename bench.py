@@ -13,7 +13,7 @@ Prints one JSON line with the FP32 form as `value` and, beside it:
     (whatever --steps is), its probe series and DFTs against the committed
     FP64 fixture tests/golden/slab512_fp64.npz;
   * the roofline of the sweep kernels (algorithmic bytes per step over their
-    HIP-event launch time) with the HBM traffic of this build measured live by
+    HIP-event launch time) with the HBM traffic of the running library measured live by
     rocprofv3 (FETCH_SIZE / WRITE_SIZE, calibrated on a copy kernel);
   * the adjoint128 workload: adjoint/forward wall-time ratios and gradient
     error against the committed FP64 fixture tests/golden/adjoint128.npz;
@@ -239,7 +239,7 @@ def live_traffic(workload, steps=3, timeout=240):
         finally:
             shutil.rmtree(d, ignore_errors=True)
     info = {"source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (separate passes) over `bench.py --traffic-probe`, "
-                      f"{steps} steps per form, this build",
+                      f"{steps} steps per form, the running library",
             "fetch_factor": round(factors["FETCH_SIZE"], 4), "write_factor": round(factors["WRITE_SIZE"], 4),
             "sweep_dispatches": ndisp}
     return {k: round(v) for k, v in sums.items()}, info
@@ -340,7 +340,7 @@ def main():
         print(json.dumps(out))
         return
     if args.gpus != 1:
-        print(f"bench.py runs one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
+        print(f"bench.py runs one GPU: the engine has no multi-GPU path (asked for {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
 

@@ -43,7 +43,7 @@ extern "C" {
 #define FDTD_E_RANGE       -3  /* index or box outside the grid            */
 #define FDTD_E_NOMEM       -4  /* host or device allocation failed         */
 #define FDTD_E_DEVICE      -5  /* HIP runtime error                        */
-#define FDTD_E_UNSUPPORTED -6  /* feature not provided by this build       */
+#define FDTD_E_UNSUPPORTED -6  /* feature not provided by the library       */
 #define FDTD_E_NONFINITE   -7  /* a probe recorded a non-finite value      */
 #define FDTD_E_CHECKSUM    -8  /* checkpoint blob is corrupt or mismatched */
 
