@@ -96,7 +96,17 @@ static __device__ __align__(256) unsigned char scratch[FDTD_NOALLOC_SCRATCH_BYTE
 
 namespace {
 
-const char *g_static_err = "no error";
+// Text of the last refused fdtd_create on this thread (no context to hold it).
+thread_local char g_create_err[256] = "no error";
+
+int create_fail(int rc, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_create_err, sizeof g_create_err, fmt, ap);
+    va_end(ap);
+    return rc;
+}
 
 // The address of this device's copy of scratch, looked up once per device
 // with that device current (fdtd_create's device); nullptr on failure.
@@ -961,33 +971,38 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
 // ============================================================================
 extern "C" {
 
-const char *fdtd_last_error(const fdtd_ctx *ctx) { return ctx ? ctx->err.c_str() : g_static_err; }
+const char *fdtd_last_error(const fdtd_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_err; }
 
 int fdtd_create(const fdtd_grid *g, int storage, fdtd_ctx **out)
 {
-    if (!out) return FDTD_E_INVALID;
+    if (!out) return create_fail(FDTD_E_INVALID, "null output pointer");
     *out = nullptr;
-    if (!g) return FDTD_E_INVALID;
-    if (g->nx < 4 || g->ny < 4 || g->nz < 4) return FDTD_E_INVALID;
-    if (!(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0)) return FDTD_E_INVALID;
-    if (storage < FDTD_STORE_F32 || storage > FDTD_STORE_BQ16) return FDTD_E_INVALID;
+    if (!g) return create_fail(FDTD_E_INVALID, "null grid");
+    if (g->nx < 4 || g->ny < 4 || g->nz < 4)
+        return create_fail(FDTD_E_INVALID, "the grid needs at least 4 cells along each axis (got %d x %d x %d)", g->nx,
+                           g->ny, g->nz);
+    if (!(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0))
+        return create_fail(FDTD_E_INVALID, "cell sizes must be positive");
+    if (storage < FDTD_STORE_F32 || storage > FDTD_STORE_BQ16)
+        return create_fail(FDTD_E_INVALID, "unknown storage form %d", storage);
     double eps0 = g->eps0 > 0 ? g->eps0 : 8.854187817e-12;
     double mu0 = g->mu0 > 0 ? g->mu0 : 4.0 * M_PI * 1e-7;
     double dt = g->dt;
     const double cfl = 1.0 / ((1.0 / std::sqrt(mu0 * eps0)) *
                               std::sqrt(1.0 / (g->dx * g->dx) + 1.0 / (g->dy * g->dy) + 1.0 / (g->dz * g->dz)));
     if (!(dt > 0)) {
-        if (!(g->courant > 0) || g->courant > 1.0) return FDTD_E_INVALID;
+        if (!(g->courant > 0) || g->courant > 1.0)
+            return create_fail(FDTD_E_INVALID, "courant must lie in (0, 1] when dt is not given");
         dt = g->courant * cfl;
     } else if (dt > cfl * (1.0 + 4.0 * FLT_EPSILON)) {
         // above the 3-D Courant limit: unstable.  A dt computed in float at
         // Courant factor 1 (source.cu:45-46 built as C++, where sqrt of a
         // float is a float) lands a few float ulps either side of the double
         // limit, so the test allows float resolution.
-        return FDTD_E_INVALID;
+        return create_fail(FDTD_E_INVALID, "dt %.9g s is above the 3-D Courant limit %.9g s", dt, cfl);
     }
     fdtd_ctx *c = new (std::nothrow) fdtd_ctx();
-    if (!c) return FDTD_E_NOMEM;
+    if (!c) return create_fail(FDTD_E_NOMEM, "out of host memory");
     SceneIn &in = c->in;
     in.nx = g->nx; in.ny = g->ny; in.nz = g->nz;
     in.px = (g->nx + 63) / 64 * 64;
@@ -1001,13 +1016,13 @@ int fdtd_create(const fdtd_grid *g, int storage, fdtd_ctx **out)
         hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         (void)hipGetLastError();
         delete c;
-        return FDTD_E_DEVICE;
+        return create_fail(FDTD_E_DEVICE, "no HIP device");
     }
     for (hipEvent_t &e : c->adj_ev)
         if (hipEventCreate(&e) != hipSuccess) {
             (void)hipGetLastError();
             fdtd_destroy(c);
-            return FDTD_E_DEVICE;
+            return create_fail(FDTD_E_DEVICE, "hipEventCreate failed");
         }
     // the no-allocation adjoint's scratch on this (current) device
     c->scratch = scratch_address(c->device);

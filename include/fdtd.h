@@ -156,7 +156,8 @@ int fdtd_create(const fdtd_grid *grid, int storage, fdtd_ctx **out);
 /* Release every resource.  The reference frees nothing (fdtdrav.h:4384). */
 int fdtd_destroy(fdtd_ctx *ctx);
 
-/* Text of the last error on this context (static storage when ctx is NULL). */
+/* Text of the last error on this context; with ctx NULL, of the last
+ * fdtd_create on the calling thread that failed (thread-local storage). */
 const char *fdtd_last_error(const fdtd_ctx *ctx);
 
 /* ---- scene ---------------------------------------------------------- */

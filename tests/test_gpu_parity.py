@@ -73,8 +73,44 @@ def scene_wide():
     return sc
 
 
+def scene_pec_faces():
+    """CPML on three faces only (x-, y+ and z+ are bare outer faces)."""
+    sc = scenes.dielectric(nx=30, ny=22, nz=18, steps=60, L=4, n_boxes=4, n_src=1, n_probes=6, n_freqs=2, seed=77,
+                           name="pec-faces")
+    sc.cpml = scenes.graded_cpml((0, 5, 4, 0, 3, 0), sc.dx)
+    sc.probes += [(EX, 0, 10, 9), (EY, 15, 21, 9), (EZ, 15, 10, 17)]
+    return sc
+
+
+def scene_tiny():
+    """9 x 7 x 6 cells, one-cell CPML: a single partial tile in x and y and a
+    single z-block; probes on the outer faces and corners."""
+    sc = scenes.dielectric(nx=24, ny=24, nz=24, steps=40, L=1, n_boxes=1, n_src=1, n_probes=2, n_freqs=2, seed=78,
+                           name="tiny")
+    sc.nx, sc.ny, sc.nz = 9, 7, 6
+    sc.cpml = scenes.graded_cpml(1, sc.dx)
+    sc.boxes = [scenes.Box((2, 1, 1, 6, 5, 4), (4.0,) * 3, (0.002,) * 3)]
+    sc.sources = [scenes.z_edge(4, 3, 2, sc.dt)]
+    sc.probes = [(EZ, 4, 3, 2), (EX, 5, 3, 3), (HY, 3, 3, 2), (HZ, 0, 0, 0), (EY, 8, 6, 5)]
+    return sc
+
+
+def scene_thin_z():
+    """Four planes along z (the smallest grid the engine takes) with no
+    z-CPML (bare z faces), CPML of four different thicknesses in x and y."""
+    sc = scenes.dielectric(nx=70, ny=66, nz=24, steps=60, L=4, n_boxes=3, n_src=1, n_probes=2, n_freqs=2, seed=79,
+                           name="thin-z")
+    sc.nz = 4
+    sc.boxes = [scenes.Box((b.ijk[0], b.ijk[1], 0, b.ijk[3], b.ijk[4], 3), b.eps_r, b.sigma_e) for b in sc.boxes]
+    sc.cpml = scenes.graded_cpml((4, 5, 6, 4, 0, 0), sc.dx)
+    sc.sources = [scenes.z_edge(35, 33, 1, sc.dt)]
+    sc.probes = [(EZ, 35, 33, 1), (EZ, 20, 20, 1), (HX, 40, 30, 1), (EY, 35, 62, 2), (HY, 36, 33, 2)]
+    return sc
+
+
 SCENES = {"vacuum": scene_vacuum, "dielectric": scene_dielectric, "magnetic": scene_magnetic,
-          "design_u32": scene_design_u32, "wide": scene_wide}
+          "design_u32": scene_design_u32, "wide": scene_wide, "pec_faces": scene_pec_faces, "tiny": scene_tiny,
+          "thin_z": scene_thin_z}
 
 
 @pytest.fixture(scope="module")

@@ -85,7 +85,7 @@ def _pair_scenes():
                             seed=23, name="odd-nx")
     odd.probes += [(scenes.EX, 40, 20, 15), (scenes.EY, 39, 20, 15), (scenes.HZ, 0, 18, 14), (scenes.HY, 40, 5, 5)]
     out = {"odd_nx": odd}
-    for name in ("vacuum", "magnetic", "design_u32", "wide"):
+    for name in ("vacuum", "magnetic", "design_u32", "wide", "pec_faces", "tiny", "thin_z"):
         out[name] = SCENES[name]()
     return out
 

@@ -72,20 +72,25 @@ def _grid(**kw):
     return g
 
 
-@pytest.mark.parametrize("bad", [dict(n=(3, 16, 16)), dict(d=0.0), dict(courant=1.5),
-                                 dict(dt=1e-9), dict(courant=0.0)])
-def test_create_rejects_invalid_grids(lib, bad):
+@pytest.mark.parametrize("bad,text", [(dict(n=(3, 16, 16)), "at least 4 cells"), (dict(n=(16, 16, 3)), "16 x 16 x 3"),
+                                      (dict(d=0.0), "cell sizes"), (dict(courant=1.5), "courant"),
+                                      (dict(dt=1e-9), "Courant limit"), (dict(courant=0.0), "courant")])
+def test_create_rejects_invalid_grids(lib, bad, text):
+    """Refused grids return FDTD_E_INVALID, no context, and say why through
+    fdtd_last_error(NULL)."""
     from fdtd_amd import api
     ctx = C.c_void_p()
     rc = lib.fdtd_create(C.byref(_grid(**bad)), 0, C.byref(ctx))
     assert rc == api.FDTD_E_INVALID
     assert not ctx.value
+    assert text in lib.fdtd_last_error(None).decode()
 
 
 def test_create_rejects_unknown_storage(lib):
     from fdtd_amd import api
     ctx = C.c_void_p()
     assert lib.fdtd_create(C.byref(_grid()), 7, C.byref(ctx)) == api.FDTD_E_INVALID
+    assert "storage form 7" in lib.fdtd_last_error(None).decode()
 
 
 def test_create_without_device_fails_loudly(lib):
@@ -98,6 +103,7 @@ def test_create_without_device_fails_loudly(lib):
         assert lib.fdtd_destroy(ctx) == api.FDTD_OK
     else:
         assert rc == api.FDTD_E_DEVICE and not ctx.value
+        assert "device" in lib.fdtd_last_error(None).decode()
 
 
 def test_splitmix64_published_vectors():
