@@ -706,6 +706,10 @@ __device__ __forceinline__ void ae_update(const KArgs &a, const AArgs &b, const 
         if (uz) ebz = (is_src ? cez_src : ez.x) * ebz;
     }
     if (sten) {
+        // consecutive planes use the two LDS buffers in turn (sq is one of
+        // them): a thread writes plane k's after the barrier of plane k-1,
+        // which every thread reaches only once done reading plane k-2's, so
+        // one barrier per plane
         HQ q0 = zq;
         if (IN || cc.valid) {
             float4 hx, hy, hz;
@@ -744,7 +748,6 @@ __device__ __forceinline__ void ae_update(const KArgs &a, const AArgs &b, const 
                 }
             }
         }
-        __syncthreads();
     }
     if (ux) stf(eb[0], o, ebx);
     if (uy) stf(eb[1], o, eby);
@@ -760,7 +763,7 @@ __global__ void __launch_bounds__(TX * BY)
 {
     constexpr bool IN = V == V_IN;   // interior tiles: no slab, face or range predicates
     constexpr bool PX = (V & V_X) != 0, PY = (V & V_Y) != 0, PZ = (V & V_Z) != 0;   // predicated axes
-    __shared__ HQ sq[BY][TX];
+    __shared__ HQ sq[2][BY][TX];
     int tx, ty, tz;
     tile_of_region<V>(a, tx, ty, tz);
     AECol cc;
@@ -808,7 +811,7 @@ __global__ void __launch_bounds__(TX * BY)
             AEPlane P;
             ae_fetch<CM, V, false>(P, a, cc, ebc, hb, euni, k, k1);
             const Tab3 t = tab3(a.etab, cl);
-            ae_update<CM, V>(a, b, cc, P, t, hu, eb, sq, k, m, fstep, sten, huni, qkm, ev, ek);
+            ae_update<CM, V>(a, b, cc, P, t, hu, eb, sq[(k - k0) & 1], k, m, fstep, sten, huni, qkm, ev, ek);
             cl = P.cln;
         }
         return;
@@ -822,18 +825,18 @@ __global__ void __launch_bounds__(TX * BY)
     for (; k + 2 < k1; k += 2) {
         ae_fetch<CM, V, true>(B, a, cc, ebc, hb, euni, k + 1, k1);
         tB = tab3(a.etab, A.cln);
-        ae_update<CM, V>(a, b, cc, A, tA, hu, eb, sq, k, m, fstep, sten, huni, qkm, ev, ek);
+        ae_update<CM, V>(a, b, cc, A, tA, hu, eb, sq[0], k, m, fstep, sten, huni, qkm, ev, ek);
         ae_fetch<CM, V, true>(A, a, cc, ebc, hb, euni, k + 2, k1);
         tA = tab3(a.etab, B.cln);
-        ae_update<CM, V>(a, b, cc, B, tB, hu, eb, sq, k + 1, m, fstep, sten, huni, qkm, ev, ek);
+        ae_update<CM, V>(a, b, cc, B, tB, hu, eb, sq[1], k + 1, m, fstep, sten, huni, qkm, ev, ek);
     }
     if (k + 1 < k1) {
         ae_fetch<CM, V, true>(B, a, cc, ebc, hb, euni, k + 1, k1);
         tB = tab3(a.etab, A.cln);
-        ae_update<CM, V>(a, b, cc, A, tA, hu, eb, sq, k, m, fstep, sten, huni, qkm, ev, ek);
-        ae_update<CM, V>(a, b, cc, B, tB, hu, eb, sq, k + 1, m, fstep, sten, huni, qkm, ev, ek);
+        ae_update<CM, V>(a, b, cc, A, tA, hu, eb, sq[0], k, m, fstep, sten, huni, qkm, ev, ek);
+        ae_update<CM, V>(a, b, cc, B, tB, hu, eb, sq[1], k + 1, m, fstep, sten, huni, qkm, ev, ek);
     } else {
-        ae_update<CM, V>(a, b, cc, A, tA, hu, eb, sq, k, m, fstep, sten, huni, qkm, ev, ek);
+        ae_update<CM, V>(a, b, cc, A, tA, hu, eb, sq[0], k, m, fstep, sten, huni, qkm, ev, ek);
     }
 }
 
@@ -945,7 +948,9 @@ __device__ __forceinline__ void ah_update(const KArgs &a, const AArgs &b, const 
         if (uy) hby = hy.x * hby;
         if (uz) hbz = hz.x * hbz;
     }
-    // transposed E half-step gathered from the E cells that read this H
+    // transposed E half-step gathered from the E cells that read this H;
+    // consecutive planes use the two LDS buffers in turn, one barrier per
+    // plane (as in the E sweep)
     sq[row][lane] = q0;
     __syncthreads();
     if (own) {
@@ -965,7 +970,6 @@ __device__ __forceinline__ void ah_update(const KArgs &a, const AArgs &b, const 
         if (uz) hbz += q0.by - qi.by;
         if (uz && !ux) hbz += q0.ax - qj.ax;
     }
-    __syncthreads();
     if (k == sk[0] || k == sk[1] || k == sk[2]) {
         // source edge on this cell: +terms; at (i+1, j) or (i, j+1): -terms
         for (int q = 0; q < 3; ++q)
@@ -996,7 +1000,7 @@ __global__ void __launch_bounds__(TX * BY)
 {
     constexpr bool IN = V == V_IN;   // interior tiles: no slab, face or range predicates
     constexpr bool PX = (V & V_X) != 0, PY = (V & V_Y) != 0, PZ = (V & V_Z) != 0;   // predicated axes
-    __shared__ EQ sq[BY][TX];
+    __shared__ EQ sq[2][BY][TX];
     int tx, ty, tz;
     tile_of_region<V>(a, tx, ty, tz);
     AECol cc;
@@ -1058,7 +1062,7 @@ __global__ void __launch_bounds__(TX * BY)
             AHPlane P;
             ah_fetch<CM, V, false>(P, a, cc, eb, hbc, euni, k);
             const Tab3 tk = tab3(a.etab, ck), tj = tab3(a.etab, cj);
-            ah_update<CM, V>(a, b, cc, P, tk, tj, hu, hb, sq, k, euni, huni, q0, sev, sk, scol);
+            ah_update<CM, V>(a, b, cc, P, tk, tj, hu, hb, sq[(k - k0) & 1], k, euni, huni, q0, sev, sk, scol);
             ck = P.cl2;
             cj = P.cld;
         }
@@ -1073,20 +1077,20 @@ __global__ void __launch_bounds__(TX * BY)
         ah_fetch<CM, V, true>(B, a, cc, eb, hbc, euni, k + 1);
         tkB = tab3(a.etab, A.cl2);
         tjB = tab3(a.etab, A.cld);
-        ah_update<CM, V>(a, b, cc, A, tkA, tjA, hu, hb, sq, k, euni, huni, q0, sev, sk, scol);
+        ah_update<CM, V>(a, b, cc, A, tkA, tjA, hu, hb, sq[0], k, euni, huni, q0, sev, sk, scol);
         ah_fetch<CM, V, true>(A, a, cc, eb, hbc, euni, k + 2);
         tkA = tab3(a.etab, B.cl2);
         tjA = tab3(a.etab, B.cld);
-        ah_update<CM, V>(a, b, cc, B, tkB, tjB, hu, hb, sq, k + 1, euni, huni, q0, sev, sk, scol);
+        ah_update<CM, V>(a, b, cc, B, tkB, tjB, hu, hb, sq[1], k + 1, euni, huni, q0, sev, sk, scol);
     }
     if (k + 1 < k1) {
         ah_fetch<CM, V, true>(B, a, cc, eb, hbc, euni, k + 1);
         tkB = tab3(a.etab, A.cl2);
         tjB = tab3(a.etab, A.cld);
-        ah_update<CM, V>(a, b, cc, A, tkA, tjA, hu, hb, sq, k, euni, huni, q0, sev, sk, scol);
-        ah_update<CM, V>(a, b, cc, B, tkB, tjB, hu, hb, sq, k + 1, euni, huni, q0, sev, sk, scol);
+        ah_update<CM, V>(a, b, cc, A, tkA, tjA, hu, hb, sq[0], k, euni, huni, q0, sev, sk, scol);
+        ah_update<CM, V>(a, b, cc, B, tkB, tjB, hu, hb, sq[1], k + 1, euni, huni, q0, sev, sk, scol);
     } else {
-        ah_update<CM, V>(a, b, cc, A, tkA, tjA, hu, hb, sq, k, euni, huni, q0, sev, sk, scol);
+        ah_update<CM, V>(a, b, cc, A, tkA, tjA, hu, hb, sq[0], k, euni, huni, q0, sev, sk, scol);
     }
 }
 
