@@ -489,12 +489,15 @@ __global__ void __launch_bounds__(TX * BY) adj_h_kernel(const KArgs a, const AAr
 // for the E sweep, (i+1, j) and (i, j+1) for the H sweep); the plane
 // neighbour (k-1 / k+1) is carried in registers along the z-march.  Each
 // thread computes its own cell's quantities once per plane and shares them
-// through LDS: a wave covers TXA = 63 columns plus one halo lane (the column
-// before the tile for the E sweep, after it for the H sweep) that only
-// computes, and the edge row of the tile computes its missing row directly.
-// The shared values are exactly those the neighbour computes for itself,
-// and exactly the stored ones of the narrow-grid sweeps above.
-constexpr int TXA = TX - 1;
+// through LDS.  A wave covers 64 columns starting on a 256-byte boundary, so
+// each row of a plane is two whole 128-byte lines; the tile's edge lane (lane
+// 0 of the E sweep, lane 63 of the H sweep) computes the quantities of the
+// column beyond the tile itself, as the tile's edge row computes its missing
+// row.  (Tiles of 63 columns plus a halo lane that only computes put every
+// row across three lines: adjoint768 reverse step 6.00 ms, against 5.1 ms
+// for aligned 64-column tiles without the edge-lane work.)  The shared
+// values are exactly those the neighbour computes for itself, and exactly
+// the stored ones of the narrow-grid sweeps above.
 
 // The wide-grid sweeps march like the forward ones: every array of a plane
 // is addressed by one 32-bit byte offset from its allocation base (the
@@ -547,6 +550,7 @@ struct AEPlane {
     float e0, e1, e2;   // dJ/dE of the own cell (i, j, k)
     float h0, h1, h2;   // dJ/dH of (i, j, k)
     float g0, g1, g2;   // dJ/dH of (i, j-1, k) (the tile's first row)
+    float m0, m1, m2;   // dJ/dH of (i-1, j, k) (the tile's first lane)
     int cln;            // class index of (i, j, k+1) (E coefficients)
 };
 
@@ -577,10 +581,12 @@ __device__ __forceinline__ void ae_fetch(AEPlane &p, const KArgs &a, const AECol
     constexpr bool IN = V == V_IN;
     const u32 o = cc.o0 + (u32)k * cc.ob;
     const bool ne = IN || cc.own, nh = IN || cc.valid, ng = cc.row == 0 && (IN || (cc.own && cc.jlo));
+    const bool nm = cc.lane == 0 && (IN || (cc.own && cc.ilo));
     if (PIPE) {
         const u32 oe = ne ? o : cc.ob;   // cc.ob: cell (0, 0, 0)
         const u32 oh = nh ? o : cc.ob;
         const u32 og = ng ? o - (u32)a.px * 4u : oh;
+        const u32 om = nm ? o - 4u : oh;
         p.e0 = ldf(eb[0], oe);
         p.e1 = ldf(eb[1], oe);
         p.e2 = ldf(eb[2], oe);
@@ -590,11 +596,14 @@ __device__ __forceinline__ void ae_fetch(AEPlane &p, const KArgs &a, const AECol
         p.g0 = ldf(hb[0], og);
         p.g1 = ldf(hb[1], og);
         p.g2 = ldf(hb[2], og);
+        p.m0 = ldf(hb[0], om);
+        p.m1 = ldf(hb[1], om);
+        p.m2 = ldf(hb[2], om);
         const int kn = min(k + 1, k1 - 1);
         const int cl = class_u<CM>(a.cls, (ne ? cc.cell0 : 0u) + (u32)kn * (u32)a.plane);
         p.cln = euni ? 0 : cl;
     } else {
-        p.e0 = p.e1 = p.e2 = p.h0 = p.h1 = p.h2 = p.g0 = p.g1 = p.g2 = 0.f;
+        p.e0 = p.e1 = p.e2 = p.h0 = p.h1 = p.h2 = p.g0 = p.g1 = p.g2 = p.m0 = p.m1 = p.m2 = 0.f;
         if (ne) {
             p.e0 = ldf(eb[0], o);
             p.e1 = ldf(eb[1], o);
@@ -610,6 +619,11 @@ __device__ __forceinline__ void ae_fetch(AEPlane &p, const KArgs &a, const AECol
             p.g0 = ldf(hb[0], og);
             p.g1 = ldf(hb[1], og);
             p.g2 = ldf(hb[2], og);
+        }
+        if (nm) {
+            p.m0 = ldf(hb[0], o - 4u);
+            p.m1 = ldf(hb[1], o - 4u);
+            p.m2 = ldf(hb[2], o - 4u);
         }
         p.cln = (!euni && ne && k + 1 < k1) ? class_u<CM>(a.cls, cc.cell0 + (u32)(k + 1) * (u32)a.plane) : 0;
     }
@@ -716,19 +730,28 @@ __device__ __forceinline__ void ae_update(const KArgs &a, const AArgs &b, const 
             hcoefs<CM>(a, hu, huni, i, j, k, hx, hy, hz);
             q0 = hq_of<V>(a, b, hx, hy, hz, p.h0, p.h1, p.h2, i, j, k);
         }
+        // the tile's first lane and first row: the quantities of (i-1, j, k)
+        // and (i, j-1, k), beyond the tile, computed beside the cell's own so
+        // that their reads overlap
+        HQ qim = zq, qjm = zq;
+        if (own && lane == 0 && ilo && (uy || uz)) {
+            float4 hx, hy, hz;
+            hcoefs<CM>(a, hu, huni, i - 1, j, k, hx, hy, hz);
+            qim = hq_of<V>(a, b, hx, hy, hz, p.m0, p.m1, p.m2, i - 1, j, k);
+        }
+        if (own && row == 0 && jlo && (ux || uz)) {
+            float4 hx, hy, hz;
+            hcoefs<CM>(a, hu, huni, i, j - 1, k, hx, hy, hz);
+            qjm = hq_of<V>(a, b, hx, hy, hz, p.g0, p.g1, p.g2, i, j - 1, k);
+        }
         sq[row][lane] = q0;
         __syncthreads();
         if (own) {
             // (i-1, j, k) and (i, j-1, k)
-            const HQ qi = sq[row][lane - 1];
-            HQ qj = zq;
-            if (row > 0) {
-                qj = sq[row - 1][lane];
-            } else if (jlo && (ux || uz)) {
-                float4 hx, hy, hz;
-                hcoefs<CM>(a, hu, huni, i, j - 1, k, hx, hy, hz);
-                qj = hq_of<V>(a, b, hx, hy, hz, p.g0, p.g1, p.g2, i, j - 1, k);
-            }
+            HQ qi = sq[row][lane > 0 ? lane - 1 : 0];
+            if (lane == 0) qi = qim;
+            HQ qj = sq[row > 0 ? row - 1 : 0][lane];
+            if (row == 0) qj = qjm;
             if (ux) {
                 ebx += (qkm.tx - q0.tx) + (qj.rx - q0.rx);
                 if (uz) ebz += qj.sz - q0.sz;
@@ -769,14 +792,13 @@ __global__ void __launch_bounds__(TX * BY)
     AECol cc;
     cc.lane = (int)(threadIdx.x & (TX - 1));
     cc.row = (int)(threadIdx.x / TX);
-    // lane 0 is the halo column before the tile
-    const int i = tx * TXA + cc.lane - 1;
+    const int i = tx * TX + cc.lane;
     const int j = ty * BY + cc.row;
     cc.i = i;
     cc.j = j;
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
-    cc.valid = (!PX || (i >= 0 && i < nx)) && (!PY || j < ny);   // a cell whose quantities a neighbour reads
-    cc.own = cc.valid && cc.lane > 0;                  // a cell this thread updates
+    cc.valid = (!PX || i < nx) && (!PY || j < ny);   // a cell this thread updates
+    cc.own = cc.valid;
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
     cc.ilo = !PX || i > 0;
@@ -845,8 +867,10 @@ struct AHPlane {
     float f0, f1, f2;   // dJ/dE of (i, j, k+1)
     float h0, h1, h2;   // dJ/dH of the own cell (i, j, k)
     float d0, d1, d2;   // dJ/dE of (i, j+1, k) (the tile's last row)
+    float x0, x1, x2;   // dJ/dE of (i+1, j, k) (the tile's last lane)
     int cl2;            // class index of (i, j, k+2) (E coefficients of the next plane's z-neighbour)
     int cld;            // class index of (i, j+1, k+1) (the tile's last row, next plane)
+    int cli;            // class index of (i+1, j, k+1) (the tile's last lane, next plane)
 };
 
 // PIPE: as for ae_fetch.
@@ -859,10 +883,12 @@ __device__ __forceinline__ void ah_fetch(AHPlane &p, const KArgs &a, const AECol
     const u32 o = cc.o0 + (u32)k * cc.ob;
     const bool last = cc.row == BY - 1 && (IN || (cc.own && cc.jhi));
     const bool nf = (IN || cc.valid) && (!(V & V_Z) || k + 1 < nz), nh = IN || cc.own;
+    const bool xl = cc.lane == TX - 1 && (IN || (cc.own && cc.ihi));   // column i+1 is in the grid
     if (PIPE) {
         const u32 of = nf ? o + cc.ob : cc.ob;   // cc.ob: cell (0, 0, 0)
         const u32 oh = nh ? o : cc.ob;
         const u32 od = last ? o + (u32)a.px * 4u : oh;
+        const u32 ox = xl ? o + 4u : oh;
         p.f0 = ldf(eb[0], of);
         p.f1 = ldf(eb[1], of);
         p.f2 = ldf(eb[2], of);
@@ -872,14 +898,19 @@ __device__ __forceinline__ void ah_fetch(AHPlane &p, const KArgs &a, const AECol
         p.d0 = ldf(eb[0], od);
         p.d1 = ldf(eb[1], od);
         p.d2 = ldf(eb[2], od);
+        p.x0 = ldf(eb[0], ox);
+        p.x1 = ldf(eb[1], ox);
+        p.x2 = ldf(eb[2], ox);
         const u32 c0 = (IN || cc.valid) ? cc.cell0 : 0u;
         const int c2 = class_u<CM>(a.cls, c0 + (u32)min(k + 2, nz - 1) * (u32)a.plane);
         const int cd =
             class_u<CM>(a.cls, (last ? cc.cell0 + (u32)a.px : c0) + (u32)min(k + 1, nz - 1) * (u32)a.plane);
+        const int ci = class_u<CM>(a.cls, (xl ? cc.cell0 + 1u : c0) + (u32)min(k + 1, nz - 1) * (u32)a.plane);
         p.cl2 = euni ? 0 : c2;
         p.cld = euni ? 0 : cd;
+        p.cli = euni ? 0 : ci;
     } else {
-        p.f0 = p.f1 = p.f2 = p.h0 = p.h1 = p.h2 = p.d0 = p.d1 = p.d2 = 0.f;
+        p.f0 = p.f1 = p.f2 = p.h0 = p.h1 = p.h2 = p.d0 = p.d1 = p.d2 = p.x0 = p.x1 = p.x2 = 0.f;
         if (nf) {
             p.f0 = ldf(eb[0], o + cc.ob);
             p.f1 = ldf(eb[1], o + cc.ob);
@@ -896,18 +927,26 @@ __device__ __forceinline__ void ah_fetch(AHPlane &p, const KArgs &a, const AECol
             p.d1 = ldf(eb[1], od);
             p.d2 = ldf(eb[2], od);
         }
+        if (xl) {
+            p.x0 = ldf(eb[0], o + 4u);
+            p.x1 = ldf(eb[1], o + 4u);
+            p.x2 = ldf(eb[2], o + 4u);
+        }
         p.cl2 = (!euni && cc.valid && k + 2 < nz) ? class_u<CM>(a.cls, cc.cell0 + (u32)(k + 2) * (u32)a.plane) : 0;
         p.cld = (!euni && last && k + 1 < nz) ? class_u<CM>(a.cls, cc.cell0 + (u32)a.px + (u32)(k + 1) * (u32)a.plane)
                                               : 0;
+        p.cli = (!euni && xl && k + 1 < nz) ? class_u<CM>(a.cls, cc.cell0 + 1u + (u32)(k + 1) * (u32)a.plane) : 0;
     }
 }
 
-// tk: E coefficients of (i, j, k+1); tj: of (i, j+1, k) (the tile's last row).
+// tk: E coefficients of (i, j, k+1); tj: of (i, j+1, k) (the tile's last row);
+// ti: of (i+1, j, k) (the tile's last lane).  q0: the transposed quantities
+// of (i, j, k), carried from the previous plane.
 template <int CM, int V>
 __device__ __forceinline__ void ah_update(const KArgs &a, const AArgs &b, const AECol &cc, const AHPlane &p,
-                                          const Tab3 &tk, const Tab3 &tj, const Tab3 &hu, float *const (&hb)[3],
-                                          EQ (*sq)[TX], int k, int euni, int huni, EQ &q0, int (&sev)[3],
-                                          int (&sk)[3], const int (&scol)[3])
+                                          const Tab3 &tk, const Tab3 &tj, const Tab3 &ti, const Tab3 &hu,
+                                          float *const (&hb)[3], EQ (*sq)[TX], int k, int euni, int huni, EQ &q0,
+                                          int (&sev)[3], int (&sk)[3], const int (&scol)[3])
 {
     constexpr bool IN = V == V_IN;
     constexpr bool PZ = (V & V_Z) != 0;
@@ -920,6 +959,12 @@ __device__ __forceinline__ void ah_update(const KArgs &a, const AArgs &b, const 
     const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
     const EQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const EQ qk = (cc.valid && khi) ? eq_of<V>(a, b, tk.x, tk.y, tk.z, p.f0, p.f1, p.f2, i, j, k + 1) : zq;
+    // the tile's last lane and last row: the quantities of (i+1, j, k) and
+    // (i, j+1, k), beyond the tile, computed beside the cell's own so that
+    // their reads overlap
+    EQ qil = zq, qjl = zq;
+    if (own && lane == TX - 1 && ihi && (uy || uz)) qil = eq_of<V>(a, b, ti.x, ti.y, ti.z, p.x0, p.x1, p.x2, i + 1, j, k);
+    if (own && row == BY - 1 && jhi && (ux || uz)) qjl = eq_of<V>(a, b, tj.x, tj.y, tj.z, p.d0, p.d1, p.d2, i, j + 1, k);
     float hbx = 0.f, hby = 0.f, hbz = 0.f;
     if (own) {
         float4 hx, hy, hz;
@@ -954,14 +999,12 @@ __device__ __forceinline__ void ah_update(const KArgs &a, const AArgs &b, const 
     sq[row][lane] = q0;
     __syncthreads();
     if (own) {
-        // (i+1, j, k) and (i, j+1, k)
-        const EQ qi = sq[row][lane + 1];
-        EQ qj = zq;
-        if (row < BY - 1) {
-            qj = sq[row + 1][lane];
-        } else if (jhi && (ux || uz)) {
-            qj = eq_of<V>(a, b, tj.x, tj.y, tj.z, p.d0, p.d1, p.d2, i, j + 1, k);
-        }
+        // (i+1, j, k) and (i, j+1, k): from the neighbours in LDS, the tile's
+        // last lane and last row computing theirs
+        EQ qi = sq[row][lane < TX - 1 ? lane + 1 : TX - 1];
+        if (lane == TX - 1) qi = qil;
+        EQ qj = sq[row < BY - 1 ? row + 1 : BY - 1][lane];
+        if (row == BY - 1) qj = qjl;
         if (ux) {
             hbx += (q0.ay - qk.ay) + (q0.bz - qj.bz);
             if (uz) hbz += q0.ax - qj.ax;
@@ -1006,14 +1049,13 @@ __global__ void __launch_bounds__(TX * BY)
     AECol cc;
     cc.lane = (int)(threadIdx.x & (TX - 1));
     cc.row = (int)(threadIdx.x / TX);
-    // lane 63 is the halo column after the tile
-    const int i = tx * TXA + cc.lane;
+    const int i = tx * TX + cc.lane;
     const int j = ty * BY + cc.row;
     cc.i = i;
     cc.j = j;
     const int nx = a.nx, ny = a.ny, nz = a.nz, px = a.px;
     cc.valid = (!PX || i < nx) && (!PY || j < ny);
-    cc.own = cc.valid && cc.lane < TX - 1;
+    cc.own = cc.valid;
     const int k0 = tz * a.kchunk;
     const int k1 = min(nz, k0 + a.kchunk);
     cc.ilo = true;
@@ -1047,13 +1089,15 @@ __global__ void __launch_bounds__(TX * BY)
         sk[q] = sev[q] >= 0 ? a.events[sev[q]].k : INT_MAX;
     }
     const Tab3 hu = uniform_tab3(a.htab);   // the H coefficients when they are one class
-    // E coefficients of (i, j, k0+1) and of (i, j+1, k0)
-    int ck, cj;
+    // E coefficients of (i, j, k0+1), of (i, j+1, k0) and of (i+1, j, k0)
+    int ck, cj, ci;
     {
         const bool last = cc.row == BY - 1 && (IN || (cc.own && cc.jhi));
+        const bool xl = cc.lane == TX - 1 && (IN || (cc.own && cc.ihi));
         const u32 c0 = (IN || cc.valid) ? cc.cell0 : 0u;
         ck = euni ? 0 : class_u<CM>(a.cls, c0 + (u32)min(k0 + 1, nz - 1) * (u32)a.plane);
         cj = euni ? 0 : class_u<CM>(a.cls, (last ? cc.cell0 + (u32)px : c0) + (u32)k0 * (u32)a.plane);
+        ci = euni ? 0 : class_u<CM>(a.cls, (xl ? cc.cell0 + 1u : c0) + (u32)k0 * (u32)a.plane);
     }
     if (!IN) {
         // one plane at a time (pipelined, the H edge variants measured
@@ -1061,37 +1105,45 @@ __global__ void __launch_bounds__(TX * BY)
         for (int k = k0; k < k1; ++k) {
             AHPlane P;
             ah_fetch<CM, V, false>(P, a, cc, eb, hbc, euni, k);
-            const Tab3 tk = tab3(a.etab, ck), tj = tab3(a.etab, cj);
-            ah_update<CM, V>(a, b, cc, P, tk, tj, hu, hb, sq[(k - k0) & 1], k, euni, huni, q0, sev, sk, scol);
+            const Tab3 tk = tab3(a.etab, ck), tj = tab3(a.etab, cj), ti = tab3(a.etab, ci);
+            ah_update<CM, V>(a, b, cc, P, tk, tj, ti, hu, hb, sq[(k - k0) & 1], k, euni, huni, q0, sev, sk, scol);
             ck = P.cl2;
             cj = P.cld;
+            ci = P.cli;
         }
         return;
     }
     AHPlane A, B;
-    // E coefficients of (i, j, k+1) and of (i, j+1, k) for plane k of the set
-    Tab3 tkA = tab3(a.etab, ck), tjA = tab3(a.etab, cj), tkB, tjB;
+    // E coefficients of (i, j, k+1), of (i, j+1, k) and of (i+1, j, k) for
+    // plane k of the set
+    Tab3 tkA = tab3(a.etab, ck), tjA = tab3(a.etab, cj), tiA = tab3(a.etab, ci), tkB, tjB, tiB;
     ah_fetch<CM, V, true>(A, a, cc, eb, hbc, euni, k0);
     int k = k0;
+#define AH_UPD(S, TK, TJ, TI, BUF, K) \
+    ah_update<CM, V>(a, b, cc, S, TK, TJ, TI, hu, hb, sq[BUF], K, euni, huni, q0, sev, sk, scol)
     for (; k + 2 < k1; k += 2) {
         ah_fetch<CM, V, true>(B, a, cc, eb, hbc, euni, k + 1);
         tkB = tab3(a.etab, A.cl2);
         tjB = tab3(a.etab, A.cld);
-        ah_update<CM, V>(a, b, cc, A, tkA, tjA, hu, hb, sq[0], k, euni, huni, q0, sev, sk, scol);
+        tiB = tab3(a.etab, A.cli);
+        AH_UPD(A, tkA, tjA, tiA, 0, k);
         ah_fetch<CM, V, true>(A, a, cc, eb, hbc, euni, k + 2);
         tkA = tab3(a.etab, B.cl2);
         tjA = tab3(a.etab, B.cld);
-        ah_update<CM, V>(a, b, cc, B, tkB, tjB, hu, hb, sq[1], k + 1, euni, huni, q0, sev, sk, scol);
+        tiA = tab3(a.etab, B.cli);
+        AH_UPD(B, tkB, tjB, tiB, 1, k + 1);
     }
     if (k + 1 < k1) {
         ah_fetch<CM, V, true>(B, a, cc, eb, hbc, euni, k + 1);
         tkB = tab3(a.etab, A.cl2);
         tjB = tab3(a.etab, A.cld);
-        ah_update<CM, V>(a, b, cc, A, tkA, tjA, hu, hb, sq[0], k, euni, huni, q0, sev, sk, scol);
-        ah_update<CM, V>(a, b, cc, B, tkB, tjB, hu, hb, sq[1], k + 1, euni, huni, q0, sev, sk, scol);
+        tiB = tab3(a.etab, A.cli);
+        AH_UPD(A, tkA, tjA, tiA, 0, k);
+        AH_UPD(B, tkB, tjB, tiB, 1, k + 1);
     } else {
-        ah_update<CM, V>(a, b, cc, A, tkA, tjA, hu, hb, sq[0], k, euni, huni, q0, sev, sk, scol);
+        AH_UPD(A, tkA, tjA, tiA, 0, k);
     }
+#undef AH_UPD
 }
 
 // E of the design box into snapshot slot (step - hist_base), SI units.
@@ -1143,7 +1195,7 @@ hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, 
                       int sten, hipStream_t s)
 {
     const bool stored = adj_stores_quantities(a0.nx);
-    const int tcols = stored ? TX : TXA;
+    const int tcols = TX;
     KArgs a = a0;
     a.tiles_x = (a.nx + tcols - 1) / tcols;
     if (a0.ib[0] < 0) {
