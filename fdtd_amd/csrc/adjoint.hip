@@ -585,8 +585,11 @@ __device__ __forceinline__ void ae_fetch(AEPlane &p, const KArgs &a, const AECol
     if (PIPE) {
         const u32 oe = ne ? o : cc.ob;   // cc.ob: cell (0, 0, 0)
         const u32 oh = nh ? o : cc.ob;
-        const u32 og = ng ? o - (u32)a.px * 4u : oh;
-        const u32 om = nm ? o - 4u : oh;
+        // lanes and rows that need no neighbour value read one address of
+        // the wave (one line) instead of their own cells again
+        const u32 ou = (u32)__builtin_amdgcn_readfirstlane((int)oh);
+        const u32 og = ng ? o - (u32)a.px * 4u : ou;
+        const u32 om = (u32)__builtin_amdgcn_readfirstlane((int)(nm ? o - 4u : oh));
         p.e0 = ldf(eb[0], oe);
         p.e1 = ldf(eb[1], oe);
         p.e2 = ldf(eb[2], oe);
@@ -887,8 +890,11 @@ __device__ __forceinline__ void ah_fetch(AHPlane &p, const KArgs &a, const AECol
     if (PIPE) {
         const u32 of = nf ? o + cc.ob : cc.ob;   // cc.ob: cell (0, 0, 0)
         const u32 oh = nh ? o : cc.ob;
-        const u32 od = last ? o + (u32)a.px * 4u : oh;
-        const u32 ox = xl ? o + 4u : oh;
+        // lanes and rows that need no neighbour value read one address of
+        // the wave (one line) instead of their own cells again
+        const u32 ou = (u32)__builtin_amdgcn_readfirstlane((int)oh);
+        const u32 od = last ? o + (u32)a.px * 4u : ou;
+        const u32 ox = (u32)__builtin_amdgcn_readlane((int)(xl ? o + 4u : oh), TX - 1);
         p.f0 = ldf(eb[0], of);
         p.f1 = ldf(eb[1], of);
         p.f2 = ldf(eb[2], of);
@@ -902,10 +908,12 @@ __device__ __forceinline__ void ah_fetch(AHPlane &p, const KArgs &a, const AECol
         p.x1 = ldf(eb[1], ox);
         p.x2 = ldf(eb[2], ox);
         const u32 c0 = (IN || cc.valid) ? cc.cell0 : 0u;
+        const u32 cu = (u32)__builtin_amdgcn_readfirstlane((int)c0);
         const int c2 = class_u<CM>(a.cls, c0 + (u32)min(k + 2, nz - 1) * (u32)a.plane);
         const int cd =
-            class_u<CM>(a.cls, (last ? cc.cell0 + (u32)a.px : c0) + (u32)min(k + 1, nz - 1) * (u32)a.plane);
-        const int ci = class_u<CM>(a.cls, (xl ? cc.cell0 + 1u : c0) + (u32)min(k + 1, nz - 1) * (u32)a.plane);
+            class_u<CM>(a.cls, (last ? cc.cell0 + (u32)a.px : cu) + (u32)min(k + 1, nz - 1) * (u32)a.plane);
+        const u32 cx = (u32)__builtin_amdgcn_readlane((int)(xl ? cc.cell0 + 1u : c0), TX - 1);
+        const int ci = class_u<CM>(a.cls, cx + (u32)min(k + 1, nz - 1) * (u32)a.plane);
         p.cl2 = euni ? 0 : c2;
         p.cld = euni ? 0 : cd;
         p.cli = euni ? 0 : ci;
