@@ -30,6 +30,23 @@ namespace fdtd {
 
 constexpr int TXP = 2 * TX;   // cells per tile row
 
+// The exponent word of the previous / next lane's block (block-scaled storage;
+// the other storage types carry none).
+template <class T>
+__device__ __forceinline__ QW<T> qw_prev(const QW<T> &q, const QW<T> &) { return q; }
+template <>
+__device__ __forceinline__ QW<bq16_t> qw_prev<bq16_t>(const QW<bq16_t> &q, const QW<bq16_t> &f)
+{
+    return QW<bq16_t>{lane_prev_u(q.w, f.w)};
+}
+template <class T>
+__device__ __forceinline__ QW<T> qw_next(const QW<T> &q, const QW<T> &) { return q; }
+template <>
+__device__ __forceinline__ QW<bq16_t> qw_next<bq16_t>(const QW<bq16_t> &q, const QW<bq16_t> &f)
+{
+    return QW<bq16_t>{lane_next_u(q.w, f.w)};
+}
+
 // Occupancy floor of the pair sweeps (waves per SIMD), for tuning builds.
 #ifdef FDTD_PAIR_WAVES
 #define P_OCC __attribute__((amdgpu_waves_per_eu(FDTD_PAIR_WAVES)))
@@ -194,8 +211,8 @@ template <class T>
 struct EPair {
     u32 hx, hy, hz, hx_jm, hz_jm, ex, ey, ez;   // packed pairs
     QW<T> qh, qh_jm, qe;                        // exponent words of H, H at row j-1, E
-    float hy_im, hz_im;                         // H at i-1 (x neighbour of cell 0; block-scaled storage: loaded)
-    u32 hy_fill, hz_fill;                       // H pair before the tile (lane 0's i-2, i-1; other storage: shifted)
+    u32 hy_fill, hz_fill;                       // H pair before the tile (lane 0's i-2, i-1)
+    QW<T> qh_fill;                              // its exponent word
     float p_yx[2], p_zx[2], p_xy[2], p_zy[2], p_xz[2], p_yz[2];
     int cla, clb;                                // class indices of plane k+1
 };
@@ -221,17 +238,15 @@ __device__ __forceinline__ void ep_fetch(EPair<T> &p, const KArgs &a, const PRow
     p.qh = ldq<T>(qp.hx, o);
     p.qh_jm = ldq<T>(qp.hx, ojm);
     p.qe = ldq<T>(qp.ex, o);
-    if (is_bq<T>::value) {
-        const QW<T> qh_im = ldq<T>(qp.hx, o - S);
-        p.hy_im = fldw(hy, o - S, qh_im, 1);
-        p.hz_im = fldw(hz, o - S, qh_im, 2);
-    } else {
+    {
         // H at i-1 is the previous lane's high half (lane_prev_u in
-        // ep_update); lane 0 fills from the pair before the tile, one address
+        // ep_update; block-scaled storage shifts the block's exponent word
+        // with it); lane 0 fills from the pair before the tile, one address
         // for the wave
         const u32 ou = (u32)__builtin_amdgcn_readfirstlane((int)o) - 2u * S;
         p.hy_fill = ld2(hy, ou);
         p.hz_fill = ld2(hz, ou);
+        p.qh_fill = ldq<T>(qp.hx, ou);
     }
     p.cla = p.clb = 0;
     if (CM != CLS_UNIFORM && k + 1 < k1)
@@ -354,8 +369,9 @@ __device__ __forceinline__ void ep_update(const EPair<T> &p, const CoefPair &t, 
     const float ez0 = dlo(p.ez, p.qe, 2), ez1 = dhi(p.ez, p.qe, 2);
     float ex_a, ey_a, ez_a, ex_b, ey_b, ez_b;
     bool uxa, uya, uza, uxb, uyb, uzb;
-    const float hy_im = is_bq<T>::value ? p.hy_im : dhi(lane_prev_u(p.hy, p.hy_fill), p.qh, 1);
-    const float hz_im = is_bq<T>::value ? p.hz_im : dhi(lane_prev_u(p.hz, p.hz_fill), p.qh, 2);
+    const QW<T> qim = qw_prev(p.qh, p.qh_fill);
+    const float hy_im = dhi(lane_prev_u(p.hy, p.hy_fill), qim, 1);
+    const float hz_im = dhi(lane_prev_u(p.hz, p.hz_fill), qim, 2);
     e_cell<T, V>(a, r, c[0], 0, p, t.a, k, m, hx0, hy0, hz0, hy_im, hz_im, dlo(p.hx_jm, p.qh_jm, 0),
                  dlo(p.hz_jm, p.qh_jm, 2), ex0, ey0, ez0, dlo(hx_km, q_km, 0), dlo(hy_km, q_km, 1), ev[0], ek[0], zs,
                  pze, ex_a, ey_a, ez_a, uxa, uya, uza);
@@ -468,16 +484,17 @@ struct HPair {
     QW<T> qe_kp, qe_jp, qh;                           // exponent words of E at k+1, E at row j+1, H
     float ez_i2, ey_i2;                               // E at i+2 (x neighbour of cell 1; loaded)
     u32 ez_fill, ey_fill;                             // E pair after the tile (lane 63's i+2, i+3; shifted)
+    QW<T> qe_fill;                                    // its exponent word
     float p_yx[2], p_zx[2], p_xy[2], p_zy[2], p_xz[2], p_yz[2];
     int cla, clb;
 };
 
-// E at i+2 from the next lane (lane_next_u) or loaded: block-scaled storage
-// and the x-edge variant load it (there the shift's two registers cost a wave
-// per SIMD: 116.6 against 113.9 us on 512x512x256 FP16).
+// E at i+2 from the next lane (lane_next_u) or loaded: the x-edge variant
+// loads it (there the shift's two registers cost a wave per SIMD: 116.6
+// against 113.9 us on 512x512x256 FP16).
 template <class T, int V>
 struct HpShift {
-    static constexpr bool value = !is_bq<T>::value && V != V_X;
+    static constexpr bool value = V != V_X;
 };
 
 __device__ __forceinline__ bool hp_zslab(const KArgs &a, int k)
@@ -517,6 +534,7 @@ __device__ __forceinline__ void hp_fetch(HPair<T> &p, const KArgs &a, const PRow
         const u32 ou = (u32)__builtin_amdgcn_readfirstlane((int)o) + (u32)TXP * S;
         p.ez_fill = ld2(ez, ou);
         p.ey_fill = ld2(ey, ou);
+        p.qe_fill = ldq<T>(qp.ex, ou);
     }
     p.cla = p.clb = 0;
     if (CM != CLS_UNIFORM && k + 1 < k1)
@@ -638,8 +656,9 @@ __device__ __forceinline__ void hp_update(const HPair<T> &p, const CoefPair &t, 
     h_cell<T, V>(a, r, c[0], 0, p, t.a, k, m, dlo(p.ex_kp, p.qe_kp, 0), dlo(p.ey_kp, p.qe_kp, 1), ez0, ez1, eyc1,
                  dlo(p.ez_jp, p.qe_jp, 2), dlo(p.ex_jp, p.qe_jp, 0), hx0, hy0, hz0, exc0, eyc0, ev[0], ek[0], zs, pzm,
                  hx_a, hy_a, hz_a, uxa, uya, uza);
-    const float ez_i2 = !HpShift<T, V>::value ? p.ez_i2 : dlo(lane_next_u(p.ez, p.ez_fill), q_c, 2);
-    const float ey_i2 = !HpShift<T, V>::value ? p.ey_i2 : dlo(lane_next_u(ey_c, p.ey_fill), q_c, 1);
+    const QW<T> qi2 = qw_next(q_c, p.qe_fill);
+    const float ez_i2 = !HpShift<T, V>::value ? p.ez_i2 : dlo(lane_next_u(p.ez, p.ez_fill), qi2, 2);
+    const float ey_i2 = !HpShift<T, V>::value ? p.ey_i2 : dlo(lane_next_u(ey_c, p.ey_fill), qi2, 1);
     h_cell<T, V>(a, r, c[1], 1, p, t.b, k, m, dhi(p.ex_kp, p.qe_kp, 0), dhi(p.ey_kp, p.qe_kp, 1), ez1, ez_i2,
                  ey_i2, dhi(p.ez_jp, p.qe_jp, 2), dhi(p.ex_jp, p.qe_jp, 0), hx1, hy1, hz1, exc1, eyc1, ev[1], ek[1],
                  zs, pzm, hx_b, hy_b, hz_b, uxb, uyb, uzb);
