@@ -113,19 +113,31 @@ __device__ __forceinline__ void e_fetch(EPlane &p, const KArgs &a, const ECol &c
                 : 0;
     const bool khi = !(V & V_Z) || k < a.nz - 1;
     p.p_yx = p.p_zx = p.p_xy = p.p_zy = p.p_xz = p.p_yz = 0.f;
-    if (cc.xs && cc.jhi && khi) {
-        const std::uint32_t q = (((std::uint32_t)k * ny + cc.j) * ra.Lx + cc.sx) * S;
+    // Slab psi.  16-bit storage issues these loads on every path of the
+    // variants with that axis's predicates (lanes off the slab read element
+    // 0 of the array, which e_update does not use), so that the waits for a
+    // plane count the same younger loads on every path; loaded only on the
+    // slab lanes, the waits sized for the path without them also drained
+    // part of the next plane's loads.  Measured on 512x512x256: FP16 general
+    // E sweep 128.9 -> 108.3 us; FP32 0.35 % slower, so FP32 keeps the
+    // conditional loads.
+    constexpr bool ALL = sizeof(T) == 2;
+    const bool nx_ = cc.xs && cc.jhi && khi;
+    if ((V & V_X) && (ALL || nx_)) {
+        const std::uint32_t q = nx_ ? (((std::uint32_t)k * ny + cc.j) * ra.Lx + cc.sx) * S : 0u;
         p.p_yx = ld(at((const PT *)ra.psi_eyx, q));
         p.p_zx = ld(at((const PT *)ra.psi_ezx, q));
     }
-    if (cc.ys && cc.ihi && khi) {
-        const std::uint32_t q = (((std::uint32_t)k * ra.Ly + cc.sy) * px + cc.i) * S;
+    const bool ny_ = cc.ys && cc.ihi && khi;
+    if ((V & V_Y) && (ALL || ny_)) {
+        const std::uint32_t q = ny_ ? (((std::uint32_t)k * ra.Ly + cc.sy) * px + cc.i) * S : 0u;
         p.p_xy = ld(at((const PT *)ra.psi_exy, q));
         p.p_zy = ld(at((const PT *)ra.psi_ezy, q));
     }
-    if ((V & V_Z) && e_zslab(a, k) && cc.ihi && cc.jhi) {
+    const bool nz_ = (V & V_Z) && e_zslab(a, k) && cc.ihi && cc.jhi;
+    if ((V & V_Z) && (ALL || nz_)) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
-        const std::uint32_t q = (((std::uint32_t)sz * ny + cc.j) * px + cc.i) * S;
+        const std::uint32_t q = nz_ ? (((std::uint32_t)sz * ny + cc.j) * px + cc.i) * S : 0u;
         p.p_xz = ld(at((const PT *)ra.psi_exz, q));
         p.p_yz = ld(at((const PT *)ra.psi_eyz, q));
     }
@@ -362,19 +374,22 @@ __device__ __forceinline__ void h_fetch(HPlane &p, const KArgs &a, const HCol &c
                 ? class_at<CM>(a.cls, (std::uint32_t)(k + 1) * (std::uint32_t)a.plane + (std::uint32_t)cc.col)
                 : 0;
     p.p_yx = p.p_zx = p.p_xy = p.p_zy = p.p_xz = p.p_yz = 0.f;
-    if (cc.xs) {
-        const std::uint32_t q = (((std::uint32_t)k * ny + cc.j) * ra.Lx + cc.sx) * S;
+    // slab psi: on every path for 16-bit storage (as in e_fetch)
+    constexpr bool ALL = sizeof(T) == 2;
+    if ((V & V_X) && (ALL || cc.xs)) {
+        const std::uint32_t q = cc.xs ? (((std::uint32_t)k * ny + cc.j) * ra.Lx + cc.sx) * S : 0u;
         p.p_yx = ld(at((const PT *)ra.psi_hyx, q));
         p.p_zx = ld(at((const PT *)ra.psi_hzx, q));
     }
-    if (cc.ys) {
-        const std::uint32_t q = (((std::uint32_t)k * ra.Ly + cc.sy) * px + cc.i) * S;
+    if ((V & V_Y) && (ALL || cc.ys)) {
+        const std::uint32_t q = cc.ys ? (((std::uint32_t)k * ra.Ly + cc.sy) * px + cc.i) * S : 0u;
         p.p_xy = ld(at((const PT *)ra.psi_hxy, q));
         p.p_zy = ld(at((const PT *)ra.psi_hzy, q));
     }
-    if ((V & V_Z) && h_zslab(a, k) && cc.valid) {
+    const bool nz_ = (V & V_Z) && h_zslab(a, k) && cc.valid;
+    if ((V & V_Z) && (ALL || nz_)) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
-        const std::uint32_t q = (((std::uint32_t)sz * ny + cc.j) * px + cc.i) * S;
+        const std::uint32_t q = nz_ ? (((std::uint32_t)sz * ny + cc.j) * px + cc.i) * S : 0u;
         p.p_xz = ld(at((const PT *)ra.psi_hxz, q));
         p.p_yz = ld(at((const PT *)ra.psi_hyz, q));
     }

@@ -252,25 +252,33 @@ __device__ __forceinline__ void ep_fetch(EPair<T> &p, const KArgs &a, const PRow
     if (CM != CLS_UNIFORM && k + 1 < k1)
         class_pair<CM>(a.cls, (u32)(k + 1) * (u32)a.plane + (u32)r.col, p.cla, p.clb);
     const bool khi = !(V & V_Z) || k < a.nz - 1;
+    // Slab psi, issued on every path of the variants with that axis's
+    // predicates: lanes off the slab read element 0 of the array, which
+    // e_cell does not use, so that the waits for a plane count the same
+    // younger loads on every path (loaded only on the slab lanes, the waits
+    // sized for the path without them also drained part of the next plane's
+    // loads)
     for (int q = 0; q < 2; ++q) {
         p.p_yx[q] = p.p_zx[q] = p.p_xy[q] = p.p_zy[q] = p.p_xz[q] = p.p_yz[q] = 0.f;
-        if (c[q].xs && r.jhi && khi) {
-            const u32 qq = (((u32)k * ra.ny + r.j) * ra.Lx + c[q].sx) * S;
+        if (V & V_X) {
+            const bool n = c[q].xs && r.jhi && khi;
+            const u32 qq = n ? (((u32)k * ra.ny + r.j) * ra.Lx + c[q].sx) * S : 0u;
             p.p_yx[q] = ld(at((const PT *)ra.psi_eyx, qq));
             p.p_zx[q] = ld(at((const PT *)ra.psi_ezx, qq));
         }
     }
-    if (r.ys && khi) {
-        const u32 qq = (((u32)k * ra.Ly + r.sy) * ra.px + c[0].i) * S;
+    if (V & V_Y) {
+        const u32 qq = (r.ys && khi) ? (((u32)k * ra.Ly + r.sy) * ra.px + c[0].i) * S : 0u;
         const u32 u = ld2((const PT *)ra.psi_exy, qq), v = ld2((const PT *)ra.psi_ezy, qq);
         p.p_xy[0] = lo_f(u, (const PT *)nullptr);
         p.p_xy[1] = hi_f(u, (const PT *)nullptr);
         p.p_zy[0] = lo_f(v, (const PT *)nullptr);
         p.p_zy[1] = hi_f(v, (const PT *)nullptr);
     }
-    if ((V & V_Z) && ((k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && khi)) && r.jhi) {
+    if (V & V_Z) {
+        const bool n = ((k > 0 && k < a.L[4]) || (k >= a.nz - a.L[5] && khi)) && r.jhi;
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
-        const u32 qq = (((u32)sz * ra.ny + r.j) * ra.px + c[0].i) * S;
+        const u32 qq = n ? (((u32)sz * ra.ny + r.j) * ra.px + c[0].i) * S : 0u;
         const u32 u = ld2((const PT *)ra.psi_exz, qq), v = ld2((const PT *)ra.psi_eyz, qq);
         p.p_xz[0] = lo_f(u, (const PT *)nullptr);
         p.p_xz[1] = hi_f(u, (const PT *)nullptr);
@@ -539,25 +547,27 @@ __device__ __forceinline__ void hp_fetch(HPair<T> &p, const KArgs &a, const PRow
     p.cla = p.clb = 0;
     if (CM != CLS_UNIFORM && k + 1 < k1)
         class_pair<CM>(a.cls, (u32)(k + 1) * (u32)a.plane + (u32)r.col, p.cla, p.clb);
+    // slab psi on every path of the variants with that axis's predicates
+    // (as in ep_fetch)
     for (int q = 0; q < 2; ++q) {
         p.p_yx[q] = p.p_zx[q] = p.p_xy[q] = p.p_zy[q] = p.p_xz[q] = p.p_yz[q] = 0.f;
-        if (c[q].xs) {
-            const u32 qq = (((u32)k * ra.ny + r.j) * ra.Lx + c[q].sx) * S;
+        if (V & V_X) {
+            const u32 qq = c[q].xs ? (((u32)k * ra.ny + r.j) * ra.Lx + c[q].sx) * S : 0u;
             p.p_yx[q] = ld(at((const PT *)ra.psi_hyx, qq));
             p.p_zx[q] = ld(at((const PT *)ra.psi_hzx, qq));
         }
     }
-    if (r.ys) {
-        const u32 qq = (((u32)k * ra.Ly + r.sy) * ra.px + c[0].i) * S;
+    if (V & V_Y) {
+        const u32 qq = r.ys ? (((u32)k * ra.Ly + r.sy) * ra.px + c[0].i) * S : 0u;
         const u32 u = ld2((const PT *)ra.psi_hxy, qq), v = ld2((const PT *)ra.psi_hzy, qq);
         p.p_xy[0] = lo_f(u, (const PT *)nullptr);
         p.p_xy[1] = hi_f(u, (const PT *)nullptr);
         p.p_zy[0] = lo_f(v, (const PT *)nullptr);
         p.p_zy[1] = hi_f(v, (const PT *)nullptr);
     }
-    if ((V & V_Z) && hp_zslab(a, k)) {
+    if (V & V_Z) {
         const int sz = k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4];
-        const u32 qq = (((u32)sz * ra.ny + r.j) * ra.px + c[0].i) * S;
+        const u32 qq = hp_zslab(a, k) ? (((u32)sz * ra.ny + r.j) * ra.px + c[0].i) * S : 0u;
         const u32 u = ld2((const PT *)ra.psi_hxz, qq), v = ld2((const PT *)ra.psi_hyz, qq);
         p.p_xz[0] = lo_f(u, (const PT *)nullptr);
         p.p_xz[1] = hi_f(u, (const PT *)nullptr);
