@@ -37,6 +37,75 @@ __device__ __forceinline__ bool zs_e(const KArgs &a, int k) { return (k > 0 && k
 __device__ __forceinline__ int slab_x(const KArgs &a, int i) { return i < a.L[0] ? i : i - (a.nx - a.L[1]) + a.L[0]; }
 __device__ __forceinline__ int slab_y(const KArgs &a, int j) { return j < a.L[2] ? j : j - (a.ny - a.L[3]) + a.L[2]; }
 __device__ __forceinline__ int slab_z(const KArgs &a, int k) { return k < a.L[4] ? k : k - (a.nz - a.L[5]) + a.L[4]; }
+// element of cell (i, j, k) in the x / y / z slab arrays of psi
+__device__ __forceinline__ i64 psi_sx(const KArgs &a, int i, int j, int k) { return ((i64)k * a.ny + j) * a.Lx + slab_x(a, i); }
+__device__ __forceinline__ i64 psi_sy(const KArgs &a, int i, int j, int k) { return ((i64)k * a.Ly + slab_y(a, j)) * a.px + i; }
+__device__ __forceinline__ i64 psi_sz(const KArgs &a, int i, int j, int k) { return ((i64)slab_z(a, k) * a.ny + j) * a.px + i; }
+
+// The adjoint psi values of one cell, two per slab axis (E cells: pb[0], pb[1]
+// in x, pb[4], pb[5] in y, pb[8], pb[9] in z; H cells: pb[2], pb[3], pb[6],
+// pb[7], pb[10], pb[11]).  The transposed quantities take them as arguments
+// so that the sweeps can read them with the rest of a plane (a read where
+// they are used waits for every load in flight, the next plane's included).
+struct Psi6 {
+    float x0, x1, y0, y1, z0, z1;
+};
+
+// psi of H cell (i, j, k) on the slabs it is in (zero elsewhere)
+template <int V>
+__device__ __forceinline__ Psi6 psih_at(const KArgs &a, const AArgs &b, int i, int j, int k)
+{
+    Psi6 p{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if ((V & V_X) && xs_h(a, i)) { const i64 q = psi_sx(a, i, j, k); p.x0 = b.pb[2][q]; p.x1 = b.pb[3][q]; }
+    if ((V & V_Y) && ys_h(a, j)) { const i64 q = psi_sy(a, i, j, k); p.y0 = b.pb[6][q]; p.y1 = b.pb[7][q]; }
+    if ((V & V_Z) && zs_h(a, k)) { const i64 q = psi_sz(a, i, j, k); p.z0 = b.pb[10][q]; p.z1 = b.pb[11][q]; }
+    return p;
+}
+// psi of E cell (i, j, k) on the slabs it is in (zero elsewhere)
+template <int V>
+__device__ __forceinline__ Psi6 psie_at(const KArgs &a, const AArgs &b, int i, int j, int k)
+{
+    Psi6 p{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if ((V & V_X) && xs_e(a, i)) { const i64 q = psi_sx(a, i, j, k); p.x0 = b.pb[0][q]; p.x1 = b.pb[1][q]; }
+    if ((V & V_Y) && ys_e(a, j)) { const i64 q = psi_sy(a, i, j, k); p.y0 = b.pb[4][q]; p.y1 = b.pb[5][q]; }
+    if ((V & V_Z) && zs_e(a, k)) { const i64 q = psi_sz(a, i, j, k); p.z0 = b.pb[8][q]; p.z1 = b.pb[9][q]; }
+    return p;
+}
+// The same, issued on every path (the two-deep pipelines): a lane that needs
+// no value of an axis (need false, or the cell off that slab) reads element
+// 0 of the array, so that the compiler's waits count the same loads on every
+// path; only values on the cell's slabs are used.
+template <int V>
+__device__ __forceinline__ Psi6 psih_all(const KArgs &a, const AArgs &b, bool need, int i, int j, int k)
+{
+    Psi6 p{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (V & V_X) { const i64 q = need && xs_h(a, i) ? psi_sx(a, i, j, k) : 0; p.x0 = b.pb[2][q]; p.x1 = b.pb[3][q]; }
+    if (V & V_Y) { const i64 q = need && ys_h(a, j) ? psi_sy(a, i, j, k) : 0; p.y0 = b.pb[6][q]; p.y1 = b.pb[7][q]; }
+    if (V & V_Z) { const i64 q = need && zs_h(a, k) ? psi_sz(a, i, j, k) : 0; p.z0 = b.pb[10][q]; p.z1 = b.pb[11][q]; }
+    return p;
+}
+template <int V>
+__device__ __forceinline__ Psi6 psie_all(const KArgs &a, const AArgs &b, bool need, int i, int j, int k)
+{
+    Psi6 p{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (V & V_X) { const i64 q = need && xs_e(a, i) ? psi_sx(a, i, j, k) : 0; p.x0 = b.pb[0][q]; p.x1 = b.pb[1][q]; }
+    if (V & V_Y) { const i64 q = need && ys_e(a, j) ? psi_sy(a, i, j, k) : 0; p.y0 = b.pb[4][q]; p.y1 = b.pb[5][q]; }
+    if (V & V_Z) { const i64 q = need && zs_e(a, k) ? psi_sz(a, i, j, k) : 0; p.z0 = b.pb[8][q]; p.z1 = b.pb[9][q]; }
+    return p;
+}
+// conditional loads (the one-plane-at-a-time marches): only what is needed
+template <int V, bool PIPE>
+__device__ __forceinline__ Psi6 psih_get(const KArgs &a, const AArgs &b, bool need, int i, int j, int k)
+{
+    if (PIPE) return psih_all<V>(a, b, need, i, j, k);
+    return need ? psih_at<V>(a, b, i, j, k) : Psi6{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+}
+template <int V, bool PIPE>
+__device__ __forceinline__ Psi6 psie_get(const KArgs &a, const AArgs &b, bool need, int i, int j, int k)
+{
+    if (PIPE) return psie_all<V>(a, b, need, i, j, k);
+    return need ? psie_at<V>(a, b, i, j, k) : Psi6{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+}
 
 template <int CM>
 __device__ __forceinline__ void coefs(const KArgs &a, const float4 *tab, bool uni, i64 c, float4 &cx, float4 &cy,
@@ -60,8 +129,8 @@ struct HQ {
 // Region variant V: bit V_X / V_Y / V_Z set when the cell can meet an x / y
 // / z CPML slab or outer face (the predicates of the other axes compile out).
 template <int V>
-__device__ __forceinline__ HQ hq_of(const KArgs &a, const AArgs &b, const float4 &hx, const float4 &hy,
-                                    const float4 &hz, float hbx, float hby, float hbz, int i, int j, int k)
+__device__ __forceinline__ HQ hq_of(const KArgs &a, const float4 &hx, const float4 &hy, const float4 &hz, float hbx,
+                                    float hby, float hbz, int i, int j, int k, const Psi6 &ps)
 {
     constexpr bool PX = (V & V_X) != 0, PY = (V & V_Y) != 0, PZ = (V & V_Z) != 0;
     HQ q{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -93,19 +162,16 @@ __device__ __forceinline__ HQ hq_of(const KArgs &a, const AArgs &b, const float4
         q.ry = hzz * hbz;
     }
     if (xs) {
-        const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
-        q.tz += pxm.y * (b.pb[2][s] + (uy ? (hy.y * dx) * hby : 0.f));   // psi_hyx
-        q.ry += pxm.y * (b.pb[3][s] + (uz ? (hz.z * dx) * hbz : 0.f));   // psi_hzx
+        q.tz += pxm.y * (ps.x0 + (uy ? (hy.y * dx) * hby : 0.f));   // psi_hyx
+        q.ry += pxm.y * (ps.x1 + (uz ? (hz.z * dx) * hbz : 0.f));   // psi_hzx
     }
     if (ys) {
-        const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
-        q.sz += pym.y * (b.pb[6][s] + (ux ? (hx.z * dy) * hbx : 0.f));   // psi_hxy
-        q.rx += pym.y * (b.pb[7][s] + (uz ? (hz.y * dy) * hbz : 0.f));   // psi_hzy
+        q.sz += pym.y * (ps.y0 + (ux ? (hx.z * dy) * hbx : 0.f));   // psi_hxy
+        q.rx += pym.y * (ps.y1 + (uz ? (hz.y * dy) * hbz : 0.f));   // psi_hzy
     }
     if (zs) {
-        const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
-        q.sy += pzm.y * (b.pb[10][s] + (ux ? (hx.y * dz) * hbx : 0.f));  // psi_hxz
-        q.tx += pzm.y * (b.pb[11][s] + (uy ? (hy.z * dz) * hby : 0.f));  // psi_hyz
+        q.sy += pzm.y * (ps.z0 + (ux ? (hx.y * dz) * hbx : 0.f));   // psi_hxz
+        q.tx += pzm.y * (ps.z1 + (uy ? (hy.z * dz) * hby : 0.f));   // psi_hyz
     }
     return q;
 }
@@ -119,8 +185,8 @@ struct EQ {
 // From the cell's E coefficients and its dJ/dE (and the psi_e adjoint in
 // memory).
 template <int V>
-__device__ __forceinline__ EQ eq_of(const KArgs &a, const AArgs &b, const float4 &ex, const float4 &ey,
-                                    const float4 &ez, float ebx, float eby, float ebz, int i, int j, int k)
+__device__ __forceinline__ EQ eq_of(const KArgs &a, const float4 &ex, const float4 &ey, const float4 &ez, float ebx,
+                                    float eby, float ebz, int i, int j, int k, const Psi6 &ps)
 {
     constexpr bool PX = (V & V_X) != 0, PY = (V & V_Y) != 0, PZ = (V & V_Z) != 0;
     EQ q{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -150,19 +216,16 @@ __device__ __forceinline__ EQ eq_of(const KArgs &a, const AArgs &b, const float4
         q.bz = ezz * ebz;
     }
     if (xs) {
-        const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
-        q.by += pxe.y * (b.pb[0][s] + (uy ? (ey.z * dx) * eby : 0.f));   // psi_eyx
-        q.az += pxe.y * (b.pb[1][s] + (uz ? (ez.y * dx) * ebz : 0.f));   // psi_ezx
+        q.by += pxe.y * (ps.x0 + (uy ? (ey.z * dx) * eby : 0.f));   // psi_eyx
+        q.az += pxe.y * (ps.x1 + (uz ? (ez.y * dx) * ebz : 0.f));   // psi_ezx
     }
     if (ys) {
-        const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
-        q.ax += pye.y * (b.pb[4][s] + (ux ? (ex.y * dy) * ebx : 0.f));   // psi_exy
-        q.bz += pye.y * (b.pb[5][s] + (uz ? (ez.z * dy) * ebz : 0.f));   // psi_ezy
+        q.ax += pye.y * (ps.y0 + (ux ? (ex.y * dy) * ebx : 0.f));   // psi_exy
+        q.bz += pye.y * (ps.y1 + (uz ? (ez.z * dy) * ebz : 0.f));   // psi_ezy
     }
     if (zs) {
-        const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
-        q.bx += pze.y * (b.pb[8][s] + (ux ? (ex.z * dz) * ebx : 0.f));   // psi_exz
-        q.ay += pze.y * (b.pb[9][s] + (uy ? (ey.y * dz) * eby : 0.f));   // psi_eyz
+        q.bx += pze.y * (ps.z0 + (ux ? (ex.z * dz) * ebx : 0.f));   // psi_exz
+        q.ay += pze.y * (ps.z1 + (uy ? (ey.y * dz) * eby : 0.f));   // psi_eyz
     }
     return q;
 }
@@ -175,7 +238,7 @@ __device__ HQ hquant(const KArgs &a, const AArgs &b, bool huni, int i, int j, in
     const i64 c = (i64)k * a.plane + (i64)j * a.px + i;
     float4 hx, hy, hz;
     coefs<CM>(a, a.htab, huni, c, hx, hy, hz);
-    return hq_of<V>(a, b, hx, hy, hz, b.hb[0][c], b.hb[1][c], b.hb[2][c], i, j, k);
+    return hq_of<V>(a, hx, hy, hz, b.hb[0][c], b.hb[1][c], b.hb[2][c], i, j, k, psih_at<V>(a, b, i, j, k));
 }
 
 template <int CM, int V>
@@ -186,7 +249,7 @@ __device__ EQ equant(const KArgs &a, const AArgs &b, bool euni, int i, int j, in
     const i64 c = (i64)k * a.plane + (i64)j * a.px + i;
     float4 ex, ey, ez;
     coefs<CM>(a, a.etab, euni, c, ex, ey, ez);
-    return eq_of<V>(a, b, ex, ey, ez, b.eb[0][c], b.eb[1][c], b.eb[2][c], i, j, k);
+    return eq_of<V>(a, ex, ey, ez, b.eb[0][c], b.eb[1][c], b.eb[2][c], i, j, k, psie_at<V>(a, b, i, j, k));
 }
 
 // First lumped-source event of column `col` at or after event `ev` (probe
@@ -379,7 +442,7 @@ __global__ void __launch_bounds__(TX * BY)
         if (uy) b.eb[1][c] = eby;
         if (uz) b.eb[2][c] = ebz;
         // the transposed E quantities of this cell, for the H sweep's gather
-        st_eq(b, c, eq_of<V>(a, b, ex, ey, ez, ebx, eby, ebz, i, j, k));
+        st_eq(b, c, eq_of<V>(a, ex, ey, ez, ebx, eby, ebz, i, j, k, psie_at<V>(a, b, i, j, k)));
     }
 }
 
@@ -478,7 +541,7 @@ __global__ void __launch_bounds__(TX * BY) adj_h_kernel(const KArgs a, const AAr
         if (uy) b.hb[1][c] = hby;
         if (uz) b.hb[2][c] = hbz;
         // the transposed H quantities of this cell, for the next E sweep's gather
-        st_hq(b, c, hq_of<V>(a, b, hx, hy, hz, hbx, hby, hbz, i, j, k));
+        st_hq(b, c, hq_of<V>(a, hx, hy, hz, hbx, hby, hbz, i, j, k, psih_at<V>(a, b, i, j, k)));
         q0 = qk;
     }
 }
@@ -552,6 +615,8 @@ struct AEPlane {
     float g0, g1, g2;   // dJ/dH of (i, j-1, k) (the tile's first row)
     float m0, m1, m2;   // dJ/dH of (i-1, j, k) (the tile's first lane)
     int cln;            // class index of (i, j, k+1) (E coefficients)
+    Psi6 ph, phm, phg;  // psi of H cells (i, j, k), (i-1, j, k), (i, j-1, k)
+    Psi6 pe;            // psi of E cell (i, j, k), before this sweep updates it
 };
 
 // Column constants of the adjoint E sweep.
@@ -575,8 +640,9 @@ struct AECol {
 // registers cost more occupancy than it hides) only the needed loads are
 // issued.
 template <int CM, int V, bool PIPE>
-__device__ __forceinline__ void ae_fetch(AEPlane &p, const KArgs &a, const AECol &cc, const float *const (&eb)[3],
-                                         const float *const (&hb)[3], int euni, int k, int k1)
+__device__ __forceinline__ void ae_fetch(AEPlane &p, const KArgs &a, const AArgs &b, const AECol &cc,
+                                         const float *const (&eb)[3], const float *const (&hb)[3], int euni, int k,
+                                         int k1)
 {
     constexpr bool IN = V == V_IN;
     const u32 o = cc.o0 + (u32)k * cc.ob;
@@ -629,6 +695,14 @@ __device__ __forceinline__ void ae_fetch(AEPlane &p, const KArgs &a, const AECol
             p.m2 = ldf(hb[2], o - 4u);
         }
         p.cln = (!euni && ne && k + 1 < k1) ? class_u<CM>(a.cls, cc.cell0 + (u32)(k + 1) * (u32)a.plane) : 0;
+    }
+    // slab psi of the plane, read with its other values
+    if (V != V_IN) {
+        const int i = cc.i, j = cc.j;
+        p.ph = psih_get<V, PIPE>(a, b, cc.valid, i, j, k);
+        p.phm = psih_get<V, PIPE>(a, b, nm, i - 1, j, k);
+        p.phg = psih_get<V, PIPE>(a, b, ng, i, j - 1, k);
+        p.pe = psie_get<V, PIPE>(a, b, cc.own, i, j, k);
     }
 }
 
@@ -687,21 +761,22 @@ __device__ __forceinline__ void ae_update(const KArgs &a, const AArgs &b, const 
     if (fstep >= 0 && (ux || uy || uz)) {
         const float4 ex = te.x, ey = te.y, ez = te.z;
         const bool zs = PZ && zs_e(a, k);
+        // psi_e adjoint: the old values read with the plane (p.pe)
         if (cc.xs && jhi && khi) {
-            const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
-            b.pb[0][s] = cc.pxe.x * (b.pb[0][s] + (uy ? (ey.z * dx) * eby : 0.f));
-            b.pb[1][s] = cc.pxe.x * (b.pb[1][s] + (uz ? (ez.y * dx) * ebz : 0.f));
+            const i64 s = psi_sx(a, i, j, k);
+            b.pb[0][s] = cc.pxe.x * (p.pe.x0 + (uy ? (ey.z * dx) * eby : 0.f));
+            b.pb[1][s] = cc.pxe.x * (p.pe.x1 + (uz ? (ez.y * dx) * ebz : 0.f));
         }
         if (cc.ys && ihi && khi) {
-            const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
-            b.pb[4][s] = cc.pye.x * (b.pb[4][s] + (ux ? (ex.y * dy) * ebx : 0.f));
-            b.pb[5][s] = cc.pye.x * (b.pb[5][s] + (uz ? (ez.z * dy) * ebz : 0.f));
+            const i64 s = psi_sy(a, i, j, k);
+            b.pb[4][s] = cc.pye.x * (p.pe.y0 + (ux ? (ex.y * dy) * ebx : 0.f));
+            b.pb[5][s] = cc.pye.x * (p.pe.y1 + (uz ? (ez.z * dy) * ebz : 0.f));
         }
         if (zs && ihi && jhi) {
             const float4 pze = a.pz_prof[k];
-            const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
-            b.pb[8][s] = pze.x * (b.pb[8][s] + (ux ? (ex.z * dz) * ebx : 0.f));
-            b.pb[9][s] = pze.x * (b.pb[9][s] + (uy ? (ey.y * dz) * eby : 0.f));
+            const i64 s = psi_sz(a, i, j, k);
+            b.pb[8][s] = pze.x * (p.pe.z0 + (ux ? (ex.z * dz) * ebx : 0.f));
+            b.pb[9][s] = pze.x * (p.pe.z1 + (uy ? (ey.y * dz) * eby : 0.f));
         }
         if (cc.in_box && k >= b.sb[2] && k < b.sb[2] + b.sb[5]) {
             const i64 nb = (i64)b.sb[3] * b.sb[4] * b.sb[5];
@@ -731,7 +806,7 @@ __device__ __forceinline__ void ae_update(const KArgs &a, const AArgs &b, const 
         if (IN || cc.valid) {
             float4 hx, hy, hz;
             hcoefs<CM>(a, hu, huni, i, j, k, hx, hy, hz);
-            q0 = hq_of<V>(a, b, hx, hy, hz, p.h0, p.h1, p.h2, i, j, k);
+            q0 = hq_of<V>(a, hx, hy, hz, p.h0, p.h1, p.h2, i, j, k, p.ph);
         }
         // the tile's first lane and first row: the quantities of (i-1, j, k)
         // and (i, j-1, k), beyond the tile, computed beside the cell's own so
@@ -740,12 +815,12 @@ __device__ __forceinline__ void ae_update(const KArgs &a, const AArgs &b, const 
         if (own && lane == 0 && ilo && (uy || uz)) {
             float4 hx, hy, hz;
             hcoefs<CM>(a, hu, huni, i - 1, j, k, hx, hy, hz);
-            qim = hq_of<V>(a, b, hx, hy, hz, p.m0, p.m1, p.m2, i - 1, j, k);
+            qim = hq_of<V>(a, hx, hy, hz, p.m0, p.m1, p.m2, i - 1, j, k, p.phm);
         }
         if (own && row == 0 && jlo && (ux || uz)) {
             float4 hx, hy, hz;
             hcoefs<CM>(a, hu, huni, i, j - 1, k, hx, hy, hz);
-            qjm = hq_of<V>(a, b, hx, hy, hz, p.g0, p.g1, p.g2, i, j - 1, k);
+            qjm = hq_of<V>(a, hx, hy, hz, p.g0, p.g1, p.g2, i, j - 1, k, p.phg);
         }
         sq[row][lane] = q0;
         __syncthreads();
@@ -834,7 +909,7 @@ __global__ void __launch_bounds__(TX * BY)
         int cl = cl0;
         for (int k = k0; k < k1; ++k) {
             AEPlane P;
-            ae_fetch<CM, V, false>(P, a, cc, ebc, hb, euni, k, k1);
+            ae_fetch<CM, V, false>(P, a, b, cc, ebc, hb, euni, k, k1);
             const Tab3 t = tab3(a.etab, cl);
             ae_update<CM, V>(a, b, cc, P, t, hu, eb, sq[(k - k0) & 1], k, m, fstep, sten, huni, qkm, ev, ek);
             cl = P.cln;
@@ -843,20 +918,20 @@ __global__ void __launch_bounds__(TX * BY)
     }
     AEPlane A, B;
     Tab3 tA = tab3(a.etab, cl0), tB;
-    ae_fetch<CM, V, true>(A, a, cc, ebc, hb, euni, k0, k1);
+    ae_fetch<CM, V, true>(A, a, b, cc, ebc, hb, euni, k0, k1);
     int k = k0;
     // steady state: the fetches are unconditional, so on every path the next
     // plane's loads are the younger ones when this plane's are awaited
     for (; k + 2 < k1; k += 2) {
-        ae_fetch<CM, V, true>(B, a, cc, ebc, hb, euni, k + 1, k1);
+        ae_fetch<CM, V, true>(B, a, b, cc, ebc, hb, euni, k + 1, k1);
         tB = tab3(a.etab, A.cln);
         ae_update<CM, V>(a, b, cc, A, tA, hu, eb, sq[0], k, m, fstep, sten, huni, qkm, ev, ek);
-        ae_fetch<CM, V, true>(A, a, cc, ebc, hb, euni, k + 2, k1);
+        ae_fetch<CM, V, true>(A, a, b, cc, ebc, hb, euni, k + 2, k1);
         tA = tab3(a.etab, B.cln);
         ae_update<CM, V>(a, b, cc, B, tB, hu, eb, sq[1], k + 1, m, fstep, sten, huni, qkm, ev, ek);
     }
     if (k + 1 < k1) {
-        ae_fetch<CM, V, true>(B, a, cc, ebc, hb, euni, k + 1, k1);
+        ae_fetch<CM, V, true>(B, a, b, cc, ebc, hb, euni, k + 1, k1);
         tB = tab3(a.etab, A.cln);
         ae_update<CM, V>(a, b, cc, A, tA, hu, eb, sq[0], k, m, fstep, sten, huni, qkm, ev, ek);
         ae_update<CM, V>(a, b, cc, B, tB, hu, eb, sq[1], k + 1, m, fstep, sten, huni, qkm, ev, ek);
@@ -874,12 +949,14 @@ struct AHPlane {
     int cl2;            // class index of (i, j, k+2) (E coefficients of the next plane's z-neighbour)
     int cld;            // class index of (i, j+1, k+1) (the tile's last row, next plane)
     int cli;            // class index of (i+1, j, k+1) (the tile's last lane, next plane)
+    Psi6 pf, pfx, pfd;  // psi of E cells (i, j, k+1), (i+1, j, k), (i, j+1, k)
+    Psi6 ph;            // psi of H cell (i, j, k), before this sweep updates it
 };
 
 // PIPE: as for ae_fetch.
 template <int CM, int V, bool PIPE>
-__device__ __forceinline__ void ah_fetch(AHPlane &p, const KArgs &a, const AECol &cc, const float *const (&eb)[3],
-                                         const float *const (&hb)[3], int euni, int k)
+__device__ __forceinline__ void ah_fetch(AHPlane &p, const KArgs &a, const AArgs &b, const AECol &cc,
+                                         const float *const (&eb)[3], const float *const (&hb)[3], int euni, int k)
 {
     constexpr bool IN = V == V_IN;
     const int nz = a.nz;
@@ -945,6 +1022,14 @@ __device__ __forceinline__ void ah_fetch(AHPlane &p, const KArgs &a, const AECol
                                               : 0;
         p.cli = (!euni && xl && k + 1 < nz) ? class_u<CM>(a.cls, cc.cell0 + 1u + (u32)(k + 1) * (u32)a.plane) : 0;
     }
+    // slab psi of the plane, read with its other values
+    if (V != V_IN) {
+        const int i = cc.i, j = cc.j;
+        p.pf = psie_get<V, PIPE>(a, b, nf, i, j, k + 1);
+        p.pfx = psie_get<V, PIPE>(a, b, xl, i + 1, j, k);
+        p.pfd = psie_get<V, PIPE>(a, b, last, i, j + 1, k);
+        p.ph = psih_get<V, PIPE>(a, b, nh, i, j, k);
+    }
 }
 
 // tk: E coefficients of (i, j, k+1); tj: of (i, j+1, k) (the tile's last row);
@@ -966,13 +1051,15 @@ __device__ __forceinline__ void ah_update(const KArgs &a, const AArgs &b, const 
     const bool ux = own && jhi && khi, uy = own && ihi && khi, uz = own && ihi && jhi;
     const float dx = a.d[0], dy = a.d[1], dz = a.d[2];
     const EQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    const EQ qk = (cc.valid && khi) ? eq_of<V>(a, b, tk.x, tk.y, tk.z, p.f0, p.f1, p.f2, i, j, k + 1) : zq;
+    const EQ qk = (cc.valid && khi) ? eq_of<V>(a, tk.x, tk.y, tk.z, p.f0, p.f1, p.f2, i, j, k + 1, p.pf) : zq;
     // the tile's last lane and last row: the quantities of (i+1, j, k) and
     // (i, j+1, k), beyond the tile, computed beside the cell's own so that
     // their reads overlap
     EQ qil = zq, qjl = zq;
-    if (own && lane == TX - 1 && ihi && (uy || uz)) qil = eq_of<V>(a, b, ti.x, ti.y, ti.z, p.x0, p.x1, p.x2, i + 1, j, k);
-    if (own && row == BY - 1 && jhi && (ux || uz)) qjl = eq_of<V>(a, b, tj.x, tj.y, tj.z, p.d0, p.d1, p.d2, i, j + 1, k);
+    if (own && lane == TX - 1 && ihi && (uy || uz))
+        qil = eq_of<V>(a, ti.x, ti.y, ti.z, p.x0, p.x1, p.x2, i + 1, j, k, p.pfx);
+    if (own && row == BY - 1 && jhi && (ux || uz))
+        qjl = eq_of<V>(a, tj.x, tj.y, tj.z, p.d0, p.d1, p.d2, i, j + 1, k, p.pfd);
     float hbx = 0.f, hby = 0.f, hbz = 0.f;
     if (own) {
         float4 hx, hy, hz;
@@ -981,21 +1068,22 @@ __device__ __forceinline__ void ah_update(const KArgs &a, const AArgs &b, const 
         hby = p.h1;
         hbz = p.h2;
         // psi_h adjoint and dJ/dH <- Ch dJ/dH (the H half-step, pointwise part)
+        // psi_h adjoint: the old values read with the plane (p.ph)
         if (cc.xs) {
-            const i64 s = ((i64)k * ny + j) * a.Lx + slab_x(a, i);
-            b.pb[2][s] = cc.pxe.x * (b.pb[2][s] + (uy ? (hy.y * dx) * hby : 0.f));
-            b.pb[3][s] = cc.pxe.x * (b.pb[3][s] + (uz ? (hz.z * dx) * hbz : 0.f));
+            const i64 s = psi_sx(a, i, j, k);
+            b.pb[2][s] = cc.pxe.x * (p.ph.x0 + (uy ? (hy.y * dx) * hby : 0.f));
+            b.pb[3][s] = cc.pxe.x * (p.ph.x1 + (uz ? (hz.z * dx) * hbz : 0.f));
         }
         if (cc.ys) {
-            const i64 s = ((i64)k * a.Ly + slab_y(a, j)) * px + i;
-            b.pb[6][s] = cc.pye.x * (b.pb[6][s] + (ux ? (hx.z * dy) * hbx : 0.f));
-            b.pb[7][s] = cc.pye.x * (b.pb[7][s] + (uz ? (hz.y * dy) * hbz : 0.f));
+            const i64 s = psi_sy(a, i, j, k);
+            b.pb[6][s] = cc.pye.x * (p.ph.y0 + (ux ? (hx.z * dy) * hbx : 0.f));
+            b.pb[7][s] = cc.pye.x * (p.ph.y1 + (uz ? (hz.y * dy) * hbz : 0.f));
         }
         if (PZ && zs_h(a, k)) {
             const float4 pzm = a.pz_prof[nz + k];
-            const i64 s = ((i64)slab_z(a, k) * ny + j) * px + i;
-            b.pb[10][s] = pzm.x * (b.pb[10][s] + (ux ? (hx.y * dz) * hbx : 0.f));
-            b.pb[11][s] = pzm.x * (b.pb[11][s] + (uy ? (hy.z * dz) * hby : 0.f));
+            const i64 s = psi_sz(a, i, j, k);
+            b.pb[10][s] = pzm.x * (p.ph.z0 + (ux ? (hx.y * dz) * hbx : 0.f));
+            b.pb[11][s] = pzm.x * (p.ph.z1 + (uy ? (hy.z * dz) * hby : 0.f));
         }
         if (ux) hbx = hx.x * hbx;
         if (uy) hby = hy.x * hby;
@@ -1112,7 +1200,7 @@ __global__ void __launch_bounds__(TX * BY)
         // slower: x-edge 0.57 -> 0.69 ms, z-slab 0.39 -> 0.48 ms on adjoint768)
         for (int k = k0; k < k1; ++k) {
             AHPlane P;
-            ah_fetch<CM, V, false>(P, a, cc, eb, hbc, euni, k);
+            ah_fetch<CM, V, false>(P, a, b, cc, eb, hbc, euni, k);
             const Tab3 tk = tab3(a.etab, ck), tj = tab3(a.etab, cj), ti = tab3(a.etab, ci);
             ah_update<CM, V>(a, b, cc, P, tk, tj, ti, hu, hb, sq[(k - k0) & 1], k, euni, huni, q0, sev, sk, scol);
             ck = P.cl2;
@@ -1125,24 +1213,24 @@ __global__ void __launch_bounds__(TX * BY)
     // E coefficients of (i, j, k+1), of (i, j+1, k) and of (i+1, j, k) for
     // plane k of the set
     Tab3 tkA = tab3(a.etab, ck), tjA = tab3(a.etab, cj), tiA = tab3(a.etab, ci), tkB, tjB, tiB;
-    ah_fetch<CM, V, true>(A, a, cc, eb, hbc, euni, k0);
+    ah_fetch<CM, V, true>(A, a, b, cc, eb, hbc, euni, k0);
     int k = k0;
 #define AH_UPD(S, TK, TJ, TI, BUF, K) \
     ah_update<CM, V>(a, b, cc, S, TK, TJ, TI, hu, hb, sq[BUF], K, euni, huni, q0, sev, sk, scol)
     for (; k + 2 < k1; k += 2) {
-        ah_fetch<CM, V, true>(B, a, cc, eb, hbc, euni, k + 1);
+        ah_fetch<CM, V, true>(B, a, b, cc, eb, hbc, euni, k + 1);
         tkB = tab3(a.etab, A.cl2);
         tjB = tab3(a.etab, A.cld);
         tiB = tab3(a.etab, A.cli);
         AH_UPD(A, tkA, tjA, tiA, 0, k);
-        ah_fetch<CM, V, true>(A, a, cc, eb, hbc, euni, k + 2);
+        ah_fetch<CM, V, true>(A, a, b, cc, eb, hbc, euni, k + 2);
         tkA = tab3(a.etab, B.cl2);
         tjA = tab3(a.etab, B.cld);
         tiA = tab3(a.etab, B.cli);
         AH_UPD(B, tkB, tjB, tiB, 1, k + 1);
     }
     if (k + 1 < k1) {
-        ah_fetch<CM, V, true>(B, a, cc, eb, hbc, euni, k + 1);
+        ah_fetch<CM, V, true>(B, a, b, cc, eb, hbc, euni, k + 1);
         tkB = tab3(a.etab, A.cl2);
         tjB = tab3(a.etab, A.cld);
         tiB = tab3(a.etab, A.cli);
