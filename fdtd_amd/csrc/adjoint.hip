@@ -249,7 +249,8 @@ __device__ EQ equant(const KArgs &a, const AArgs &b, bool euni, int i, int j, in
     const i64 c = (i64)k * a.plane + (i64)j * a.px + i;
     float4 ex, ey, ez;
     coefs<CM>(a, a.etab, euni, c, ex, ey, ez);
-    return eq_of<V>(a, ex, ey, ez, b.eb[0][c], b.eb[1][c], b.eb[2][c], i, j, k, psie_at<V>(a, b, i, j, k));
+    // dJ/dE after this step's E sweep (the H sweep's operand)
+    return eq_of<V>(a, ex, ey, ez, b.ebn[0][c], b.ebn[1][c], b.ebn[2][c], i, j, k, psie_at<V>(a, b, i, j, k));
 }
 
 // First lumped-source event of column `col` at or after event `ev` (probe
@@ -275,7 +276,7 @@ __device__ float2 src_terms(const KArgs &a, const AArgs &b, bool euni, int i, in
     const float4 pxe = xs_e(a, i) ? a.px_prof[i] : NOPROF;
     const float4 pye = ys_e(a, j) ? a.py_prof[j] : NOPROF;
     const SrcEdge se = a.srcs[id];
-    const float ebz = b.eb[2][c];
+    const float ebz = b.ebn[2][c];   // after this step's E sweep
     return make_float2(se.cezhy * ebz - (ez.y / pxe.z) * ebz, se.cezhx * ebz - (ez.z / pye.z) * ebz);
 }
 
@@ -627,10 +628,17 @@ struct AECol {
     u32 o0, ob, cell0;   // byte offset of (i, j, 0) from the allocation base; bytes per plane; cell index of (i, j, 0)
 };
 
-// Every load is issued on every path (lanes and planes that need no value
-// read a cell that exists: the first cell of the arrays, their own cell, or
-// the last plane), so that the compiler's waits for a plane's values count
-// only the younger loads of the next plane.
+// Interior tile of a workgroup from a column list (tx | ty << 16), each
+// column for every z-block of the box, XCD-aware as remap_tile.
+__device__ __forceinline__ void tile_of_list(const KArgs &a, const int *tl, int ntl, int &tx, int &ty, int &tz)
+{
+    const int t = remap_tile(ntl * (a.ib[5] - a.ib[2]));
+    const int c = tl[t % ntl];
+    tx = c & 0xffff;
+    ty = c >> 16;
+    tz = a.ib[2] + t / ntl;
+}
+
 // PIPE (the two-deep pipeline of the interior variants and of the E sweep's
 // x-edge and z-slab ones): every load is issued on every path (lanes and
 // planes that need no value read a cell that exists: the first cell of the
@@ -646,7 +654,9 @@ __device__ __forceinline__ void ae_fetch(AEPlane &p, const KArgs &a, const AArgs
 {
     constexpr bool IN = V == V_IN;
     const u32 o = cc.o0 + (u32)k * cc.ob;
-    const bool ne = IN || cc.own, nh = IN || cc.valid, ng = cc.row == 0 && (IN || (cc.own && cc.jlo));
+    // (i, j-1): the tile's first row, and every row of an interior tile
+    // (which takes its y-neighbour from memory, not from LDS)
+    const bool ne = IN || cc.own, nh = IN || cc.valid, ng = IN || (cc.row == 0 && cc.own && cc.jlo);
     const bool nm = cc.lane == 0 && (IN || (cc.own && cc.ilo));
     if (PIPE) {
         const u32 oe = ne ? o : cc.ob;   // cc.ob: cell (0, 0, 0)
@@ -663,7 +673,7 @@ __device__ __forceinline__ void ae_fetch(AEPlane &p, const KArgs &a, const AArgs
         p.h1 = ldf(hb[1], oh);
         p.h2 = ldf(hb[2], oh);
         p.g0 = ldf(hb[0], og);
-        p.g1 = ldf(hb[1], og);
+        p.g1 = IN ? 0.f : ldf(hb[1], og);   // (the y-neighbour quantities read need no Hy)
         p.g2 = ldf(hb[2], og);
         p.m0 = ldf(hb[0], om);
         p.m1 = ldf(hb[1], om);
@@ -817,19 +827,29 @@ __device__ __forceinline__ void ae_update(const KArgs &a, const AArgs &b, const 
             hcoefs<CM>(a, hu, huni, i - 1, j, k, hx, hy, hz);
             qim = hq_of<V>(a, hx, hy, hz, p.m0, p.m1, p.m2, i - 1, j, k, p.phm);
         }
-        if (own && row == 0 && jlo && (ux || uz)) {
+        if (IN || (own && row == 0 && jlo && (ux || uz))) {
             float4 hx, hy, hz;
             hcoefs<CM>(a, hu, huni, i, j - 1, k, hx, hy, hz);
             qjm = hq_of<V>(a, hx, hy, hz, p.g0, p.g1, p.g2, i, j - 1, k, p.phg);
         }
-        sq[row][lane] = q0;
-        __syncthreads();
-        if (own) {
-            // (i-1, j, k) and (i, j-1, k)
-            HQ qi = sq[row][lane > 0 ? lane - 1 : 0];
+        // (i-1, j, k) and (i, j-1, k).  Interior tiles: the x-neighbour's from
+        // the adjacent lane (a DPP wave shift), the y-neighbour's computed
+        // from its H read above, no LDS and no barrier.  Otherwise both
+        // through LDS, the tile's first lane and row taking their own.
+        HQ qi = zq, qj = zq;
+        if (IN) {
+            qi.tz = lane_prev(q0.tz, qim.tz);
+            qi.ry = lane_prev(q0.ry, qim.ry);
+            qj = qjm;
+        } else {
+            sq[row][lane] = q0;
+            __syncthreads();
+            qi = sq[row][lane > 0 ? lane - 1 : 0];
             if (lane == 0) qi = qim;
-            HQ qj = sq[row > 0 ? row - 1 : 0][lane];
+            qj = sq[row > 0 ? row - 1 : 0][lane];
             if (row == 0) qj = qjm;
+        }
+        if (own) {
             if (ux) {
                 ebx += (qkm.tx - q0.tx) + (qj.rx - q0.rx);
                 if (uz) ebz += qj.sz - q0.sz;
@@ -866,7 +886,8 @@ __global__ void __launch_bounds__(TX * BY)
     constexpr bool PX = (V & V_X) != 0, PY = (V & V_Y) != 0, PZ = (V & V_Z) != 0;   // predicated axes
     __shared__ HQ sq[2][BY][TX];
     int tx, ty, tz;
-    tile_of_region<V>(a, tx, ty, tz);
+    if (IN && b.ntl > 0) tile_of_list(a, b.tl, b.ntl, tx, ty, tz);
+    else tile_of_region<V>(a, tx, ty, tz);
     AECol cc;
     cc.lane = (int)(threadIdx.x & (TX - 1));
     cc.row = (int)(threadIdx.x / TX);
@@ -892,9 +913,9 @@ __global__ void __launch_bounds__(TX * BY)
     cc.ob = (u32)a.plane * 4u;
     cc.cell0 = (u32)(j * px + i);
     cc.o0 = cc.cell0 * 4u + cc.ob;
-    // allocation bases (guard plane below k = 0)
-    float *const eb[3] = {b.eb[0] - a.plane, b.eb[1] - a.plane, b.eb[2] - a.plane};
-    const float *const ebc[3] = {eb[0], eb[1], eb[2]};
+    // allocation bases (guard plane below k = 0): read eb and hb, write ebn
+    float *const eb[3] = {b.ebn[0] - a.plane, b.ebn[1] - a.plane, b.ebn[2] - a.plane};
+    const float *const ebc[3] = {b.eb[0] - a.plane, b.eb[1] - a.plane, b.eb[2] - a.plane};
     const float *const hb[3] = {b.hb[0] - a.plane, b.hb[1] - a.plane, b.hb[2] - a.plane};
     int ev = cc.own ? first_event(a, a.e_first, cc.mycol, k0) : -1;
     int ek = ev >= 0 ? a.events[ev].k : INT_MAX;
@@ -961,7 +982,9 @@ __device__ __forceinline__ void ah_fetch(AHPlane &p, const KArgs &a, const AArgs
     constexpr bool IN = V == V_IN;
     const int nz = a.nz;
     const u32 o = cc.o0 + (u32)k * cc.ob;
-    const bool last = cc.row == BY - 1 && (IN || (cc.own && cc.jhi));
+    // (i, j+1): the tile's last row, and every row of an interior tile
+    // (which takes its y-neighbour from memory, not from LDS)
+    const bool last = IN || (cc.row == BY - 1 && cc.own && cc.jhi);
     const bool nf = (IN || cc.valid) && (!(V & V_Z) || k + 1 < nz), nh = IN || cc.own;
     const bool xl = cc.lane == TX - 1 && (IN || (cc.own && cc.ihi));   // column i+1 is in the grid
     if (PIPE) {
@@ -979,7 +1002,7 @@ __device__ __forceinline__ void ah_fetch(AHPlane &p, const KArgs &a, const AArgs
         p.h1 = ldf(hb[1], oh);
         p.h2 = ldf(hb[2], oh);
         p.d0 = ldf(eb[0], od);
-        p.d1 = ldf(eb[1], od);
+        p.d1 = IN ? 0.f : ldf(eb[1], od);   // (the y-neighbour quantities read need no Ey)
         p.d2 = ldf(eb[2], od);
         p.x0 = ldf(eb[0], ox);
         p.x1 = ldf(eb[1], ox);
@@ -1058,7 +1081,7 @@ __device__ __forceinline__ void ah_update(const KArgs &a, const AArgs &b, const 
     EQ qil = zq, qjl = zq;
     if (own && lane == TX - 1 && ihi && (uy || uz))
         qil = eq_of<V>(a, ti.x, ti.y, ti.z, p.x0, p.x1, p.x2, i + 1, j, k, p.pfx);
-    if (own && row == BY - 1 && jhi && (ux || uz))
+    if (IN || (own && row == BY - 1 && jhi && (ux || uz)))
         qjl = eq_of<V>(a, tj.x, tj.y, tj.z, p.d0, p.d1, p.d2, i, j + 1, k, p.pfd);
     float hbx = 0.f, hby = 0.f, hbz = 0.f;
     if (own) {
@@ -1089,18 +1112,27 @@ __device__ __forceinline__ void ah_update(const KArgs &a, const AArgs &b, const 
         if (uy) hby = hy.x * hby;
         if (uz) hbz = hz.x * hbz;
     }
-    // transposed E half-step gathered from the E cells that read this H;
+    // transposed E half-step gathered from the E cells that read this H:
+    // (i+1, j, k) and (i, j+1, k).  Interior tiles: the x-neighbour's from
+    // the adjacent lane (a DPP wave shift), the y-neighbour's computed from
+    // its E read with the plane, no LDS and no barrier.  Otherwise from the
+    // neighbours in LDS, the tile's last lane and last row computing theirs;
     // consecutive planes use the two LDS buffers in turn, one barrier per
-    // plane (as in the E sweep)
-    sq[row][lane] = q0;
-    __syncthreads();
-    if (own) {
-        // (i+1, j, k) and (i, j+1, k): from the neighbours in LDS, the tile's
-        // last lane and last row computing theirs
-        EQ qi = sq[row][lane < TX - 1 ? lane + 1 : TX - 1];
+    // plane (as in the E sweep).
+    EQ qi = zq, qj = zq;
+    if (IN) {
+        qi.by = lane_next(q0.by, qil.by);
+        qi.az = lane_next(q0.az, qil.az);
+        qj = qjl;
+    } else {
+        sq[row][lane] = q0;
+        __syncthreads();
+        qi = sq[row][lane < TX - 1 ? lane + 1 : TX - 1];
         if (lane == TX - 1) qi = qil;
-        EQ qj = sq[row < BY - 1 ? row + 1 : BY - 1][lane];
+        qj = sq[row < BY - 1 ? row + 1 : BY - 1][lane];
         if (row == BY - 1) qj = qjl;
+    }
+    if (own) {
         if (ux) {
             hbx += (q0.ay - qk.ay) + (q0.bz - qj.bz);
             if (uz) hbz += q0.ax - qj.ax;
@@ -1141,7 +1173,8 @@ __global__ void __launch_bounds__(TX * BY)
     constexpr bool PX = (V & V_X) != 0, PY = (V & V_Y) != 0, PZ = (V & V_Z) != 0;   // predicated axes
     __shared__ EQ sq[2][BY][TX];
     int tx, ty, tz;
-    tile_of_region<V>(a, tx, ty, tz);
+    if (IN && b.ntl > 0) tile_of_list(a, b.tl, b.ntl, tx, ty, tz);
+    else tile_of_region<V>(a, tx, ty, tz);
     AECol cc;
     cc.lane = (int)(threadIdx.x & (TX - 1));
     cc.row = (int)(threadIdx.x / TX);
@@ -1167,9 +1200,10 @@ __global__ void __launch_bounds__(TX * BY)
     cc.ob = (u32)a.plane * 4u;
     cc.cell0 = (u32)(j * px + i);
     cc.o0 = cc.cell0 * 4u + cc.ob;
-    const float *const eb[3] = {b.eb[0] - a.plane, b.eb[1] - a.plane, b.eb[2] - a.plane};
-    float *const hb[3] = {b.hb[0] - a.plane, b.hb[1] - a.plane, b.hb[2] - a.plane};
-    const float *const hbc[3] = {hb[0], hb[1], hb[2]};
+    // read ebn (this step's dJ/dE) and hb, write hbn
+    const float *const eb[3] = {b.ebn[0] - a.plane, b.ebn[1] - a.plane, b.ebn[2] - a.plane};
+    float *const hb[3] = {b.hbn[0] - a.plane, b.hbn[1] - a.plane, b.hbn[2] - a.plane};
+    const float *const hbc[3] = {b.hb[0] - a.plane, b.hb[1] - a.plane, b.hb[2] - a.plane};
     const EQ zq{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     EQ q0 = cc.valid ? equant<CM, V>(a, b, euni, i, j, k0) : zq;
     // lumped-source edges among the Ez edges this thread gathers from: its
@@ -1188,7 +1222,7 @@ __global__ void __launch_bounds__(TX * BY)
     // E coefficients of (i, j, k0+1), of (i, j+1, k0) and of (i+1, j, k0)
     int ck, cj, ci;
     {
-        const bool last = cc.row == BY - 1 && (IN || (cc.own && cc.jhi));
+        const bool last = IN || (cc.row == BY - 1 && cc.own && cc.jhi);
         const bool xl = cc.lane == TX - 1 && (IN || (cc.own && cc.ihi));
         const u32 c0 = (IN || cc.valid) ? cc.cell0 : 0u;
         ck = euni ? 0 : class_u<CM>(a.cls, c0 + (u32)min(k0 + 1, nz - 1) * (u32)a.plane);
@@ -1258,6 +1292,215 @@ __global__ void snap_kernel(const KArgs a, float *hist, const int slot)
     o[2 * nb] = ld((const T *)a.ez + c) * a.probe_unscale[2];
 }
 
+// Fused interior reverse step: the E sweep and the H sweep of one reverse
+// step in one march over an interior tile whose cells, and the cells one
+// beyond them at i+1 and j+1, hold no source, probe or design edge (the
+// host's list).  Plane kk's E update runs one plane ahead of plane kk-1's H
+// update, which reads dJ/dE of (i, j, kk).  The E values a tile's H cells
+// read beyond the tile -- (i+1, j) for the last lane, (i, j+1) for every row
+// (the row above is another wave's, or another tile's) -- are evaluated here
+// as well, from the old values: the pass reads eb / hb and writes ebn / hbn,
+// the other buffers of a pair, so no workgroup reads a value that another
+// has already overwritten.  A cell's arithmetic is the E and H sweeps' (the
+// same products and sums in the same order), so the result is theirs bit
+// for bit, at about two thirds of their memory traffic: each cell's dJ/dE
+// and dJ/dH are read once and written once per step.  H coefficients are
+// one class (the host fuses only then).
+
+// Reads of plane kk.  Every load is issued on every lane: a lane that needs
+// no neighbour value reads the one address of its wave's lane that does.
+struct FPlane {
+    float e0, e1, e2;   // old dJ/dE of (i, j, kk)
+    float h0, h1, h2;   // old dJ/dH of (i, j, kk)
+    float g0, g2;       // old dJ/dH x, z of (i, j-1, kk)
+    float m1, m2;       // old dJ/dH y, z of (i-1, j, kk) (the first lane's)
+    float y0, y2;       // old dJ/dE x, z of (i, j+1, kk)
+    float u0, u1, u2;   // old dJ/dH of (i, j+1, kk)
+    float c1;           // old dJ/dH y of (i-1, j+1, kk) (the first lane's)
+    float x1, x2;       // old dJ/dE y, z of (i+1, j, kk) (the last lane's)
+    float w0, w1, w2;   // old dJ/dH of (i+1, j, kk) (the last lane's)
+    float d0;           // old dJ/dH x of (i+1, j-1, kk) (the last lane's)
+    int cl, cly, clx;   // E classes of (i, j, kk), (i, j+1, kk), (i+1, j, kk)
+};
+
+// Values carried from plane kk-1 to plane kk.
+struct FCarry {
+    float qtx, qsy;            // H quantities tx, sy of (i, j, kk-1)
+    float ytx;                 // tx of (i, j+1, kk-1)
+    float xsy;                 // sy of (i+1, j, kk-1)
+    EQ q;                      // E quantities of (i, j, kk-1)
+    float ybz, yax;            // bz, ax of (i, j+1, kk-1)
+    float xaz, xby;            // az, by of (i+1, j, kk-1)
+    float hx, hy, hz;          // old dJ/dH of (i, j, kk-1)
+};
+
+template <int CM>
+__device__ __forceinline__ void fz_fetch(FPlane &p, const KArgs &a, const float *const (&eo)[3],
+                                         const float *const (&ho)[3], u32 o0, u32 cell0, u32 ob, int kk, int euni)
+{
+    const u32 o = o0 + (u32)kk * ob;
+    const u32 rw = (u32)a.px * 4u;
+    const u32 om = (u32)__builtin_amdgcn_readfirstlane((int)(o - 4u));   // (i-1, j) of the first lane
+    const u32 ox = (u32)__builtin_amdgcn_readlane((int)(o + 4u), TX - 1);   // (i+1, j) of the last lane
+    p.e0 = ldf(eo[0], o);
+    p.e1 = ldf(eo[1], o);
+    p.e2 = ldf(eo[2], o);
+    p.h0 = ldf(ho[0], o);
+    p.h1 = ldf(ho[1], o);
+    p.h2 = ldf(ho[2], o);
+    p.g0 = ldf(ho[0], o - rw);
+    p.g2 = ldf(ho[2], o - rw);
+    p.m1 = ldf(ho[1], om);
+    p.m2 = ldf(ho[2], om);
+    p.y0 = ldf(eo[0], o + rw);
+    p.y2 = ldf(eo[2], o + rw);
+    p.u0 = ldf(ho[0], o + rw);
+    p.u1 = ldf(ho[1], o + rw);
+    p.u2 = ldf(ho[2], o + rw);
+    p.c1 = ldf(ho[1], om + rw);
+    p.x1 = ldf(eo[1], ox);
+    p.x2 = ldf(eo[2], ox);
+    p.w0 = ldf(ho[0], ox);
+    p.w1 = ldf(ho[1], ox);
+    p.w2 = ldf(ho[2], ox);
+    p.d0 = ldf(ho[0], ox - rw);
+    const u32 c = cell0 + (u32)kk * (u32)a.plane;
+    const int cl = class_u<CM>(a.cls, c);
+    const int cly = class_u<CM>(a.cls, c + (u32)a.px);
+    const int clx = class_u<CM>(a.cls, (u32)__builtin_amdgcn_readlane((int)(c + 1u), TX - 1));
+    p.cl = euni ? 0 : cl;
+    p.cly = euni ? 0 : cly;
+    p.clx = euni ? 0 : clx;
+}
+
+// The E update of plane kk (the own cell; x and z of (i, j+1); y and z of
+// (i+1, j)), the E quantities of the three, and the H update of plane kk-1.
+// fin: finalize the forward E half-step (ae_update's fstep >= 0).  e_st:
+// store the own cell's dJ/dE (plane kk is the tile's); h_st: plane kk-1 is
+// the tile's (not the plane below its first).
+__device__ __forceinline__ void fz_step(const FPlane &p, const Tab3 &hu, const float4 *etab, bool fin, FCarry &c,
+                                        float *const (&en)[3], float *const (&hn)[3], u32 o, u32 ob, bool e_st,
+                                        bool h_st)
+{
+    const float hxy = hu.x.y, hxz = hu.x.z, hyy = hu.y.y, hyz = hu.y.z, hzy = hu.z.y, hzz = hu.z.z;
+    // H quantities of the own cell, of (i, j-1) and (i-1, j) (the adjacent
+    // lane's; the first lane's from its load)
+    const float q0sy = hxy * p.h0, q0sz = hxz * p.h0, q0tz = hyy * p.h1, q0tx = hyz * p.h1;
+    const float q0rx = hzy * p.h2, q0ry = hzz * p.h2;
+    const float qjrx = hzy * p.g2, qjsz = hxz * p.g0;
+    const float qitz = lane_prev(q0tz, hyy * p.m1), qiry = lane_prev(q0ry, hzz * p.m2);
+    const Tab3 te = tab3(etab, p.cl);
+    float ebx = p.e0, eby = p.e1, ebz = p.e2;
+    if (fin) {
+        ebx = te.x.x * ebx;
+        eby = te.y.x * eby;
+        ebz = te.z.x * ebz;
+    }
+    ebx += (c.qtx - q0tx) + (qjrx - q0rx);
+    ebz += qjsz - q0sz;
+    eby += (c.qsy - q0sy) + (qiry - q0ry);
+    ebz += qitz - q0tz;
+    c.qtx = q0tx;
+    c.qsy = q0sy;
+    if (e_st) {
+        stf(en[0], o, ebx);
+        stf(en[1], o, eby);
+        stf(en[2], o, ebz);
+    }
+    const EQ q{te.x.y * ebx, te.x.z * ebx, te.y.y * eby, te.y.z * eby, te.z.y * ebz, te.z.z * ebz};
+    // (i, j+1): its H quantities, (i-1, j+1)'s tz from the adjacent lane
+    const float ytx = hyz * p.u1, yrx = hzy * p.u2, ysz = hxz * p.u0, ytz = hyy * p.u1;
+    const float yitz = lane_prev(ytz, hyy * p.c1);
+    const Tab3 ty = tab3(etab, p.cly);
+    float ybx = p.y0, ybz = p.y2;
+    if (fin) {
+        ybx = ty.x.x * ybx;
+        ybz = ty.z.x * ybz;
+    }
+    ybx += (c.ytx - ytx) + (q0rx - yrx);
+    ybz += q0sz - ysz;
+    ybz += yitz - ytz;
+    c.ytx = ytx;
+    // (i+1, j) (the last lane's): its H quantities, (i+1, j-1)'s sz
+    const float xsy = hxy * p.w0, xsz = hxz * p.w0, xtz = hyy * p.w1, xry = hzz * p.w2;
+    const float xjsz = hxz * p.d0;
+    const Tab3 tx = tab3(etab, p.clx);
+    float xby = p.x1, xbz = p.x2;
+    if (fin) {
+        xby = tx.y.x * xby;
+        xbz = tx.z.x * xbz;
+    }
+    xbz += xjsz - xsz;
+    xby += (c.xsy - xsy) + (q0ry - xry);
+    xbz += q0tz - xtz;
+    c.xsy = xsy;
+    // H update of plane kk-1: (i+1, j)'s quantities from the adjacent lane,
+    // the last lane's evaluated above
+    if (h_st) {
+        float hbx = hu.x.x * c.hx, hby = hu.y.x * c.hy, hbz = hu.z.x * c.hz;
+        const float qiaz = lane_next(c.q.az, c.xaz), qiby = lane_next(c.q.by, c.xby);
+        hbx += (c.q.ay - q.ay) + (c.q.bz - c.ybz);
+        hbz += c.q.ax - c.yax;
+        hby += (c.q.bx - q.bx) + (c.q.az - qiaz);
+        hbz += c.q.by - qiby;
+        stf(hn[0], o - ob, hbx);
+        stf(hn[1], o - ob, hby);
+        stf(hn[2], o - ob, hbz);
+    }
+    c.q = q;
+    c.ybz = ty.z.z * ybz;
+    c.yax = ty.x.y * ybx;
+    c.xaz = tx.z.y * xbz;
+    c.xby = tx.y.z * xby;
+    c.hx = p.h0;
+    c.hy = p.h1;
+    c.hz = p.h2;
+}
+
+template <int CM>
+__global__ void __launch_bounds__(TX * BY) adj_fused_kernel(const KArgs a, const AArgs b, const int fin, const int euni)
+{
+    int tx, ty, tz;
+    tile_of_list(a, b.fl, b.nfl, tx, ty, tz);
+    const int lane = (int)(threadIdx.x & (TX - 1)), row = (int)(threadIdx.x / TX);
+    const int i = tx * TX + lane, j = ty * BY + row;
+    const int k0 = tz * a.kchunk, k1 = min(a.nz, k0 + a.kchunk);   // box z-blocks end below the top slab
+    const u32 ob = (u32)a.plane * 4u, cell0 = (u32)(j * a.px + i), o0 = cell0 * 4u + ob;
+    const float *const eo[3] = {b.eb[0] - a.plane, b.eb[1] - a.plane, b.eb[2] - a.plane};
+    const float *const ho[3] = {b.hb[0] - a.plane, b.hb[1] - a.plane, b.hb[2] - a.plane};
+    float *const en[3] = {b.ebn[0] - a.plane, b.ebn[1] - a.plane, b.ebn[2] - a.plane};
+    float *const hn[3] = {b.hbn[0] - a.plane, b.hbn[1] - a.plane, b.hbn[2] - a.plane};
+    const Tab3 hu = uniform_tab3(a.htab);
+    const bool fn = fin != 0;
+    FCarry c;
+    {
+        // z-neighbour quantities of plane k0-1 (the E sweep's qkm)
+        const u32 o = o0 + (u32)(k0 - 1) * ob;
+        const u32 ox = (u32)__builtin_amdgcn_readlane((int)(o + 4u), TX - 1);
+        c.qtx = hu.y.z * ldf(ho[1], o);
+        c.qsy = hu.x.y * ldf(ho[0], o);
+        c.ytx = hu.y.z * ldf(ho[1], o + (u32)a.px * 4u);
+        c.xsy = hu.x.y * ldf(ho[0], ox);
+        c.q = EQ{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        c.ybz = c.yax = c.xaz = c.xby = c.hx = c.hy = c.hz = 0.f;
+    }
+    FPlane A, B;
+    fz_fetch<CM>(A, a, eo, ho, o0, cell0, ob, k0, euni);
+    fz_fetch<CM>(B, a, eo, ho, o0, cell0, ob, k0 + 1, euni);
+    fz_step(A, hu, a.etab, fn, c, en, hn, o0 + (u32)k0 * ob, ob, true, false);
+    fz_fetch<CM>(A, a, eo, ho, o0, cell0, ob, min(k0 + 2, k1), euni);
+    int kk = k0 + 1;
+    // B holds plane kk, A plane kk+1 (the fetches past k1 re-read plane k1,
+    // so that every path issues the same loads)
+    for (; kk + 1 <= k1; kk += 2) {
+        fz_step(B, hu, a.etab, fn, c, en, hn, o0 + (u32)kk * ob, ob, true, true);
+        fz_fetch<CM>(B, a, eo, ho, o0, cell0, ob, min(kk + 2, k1), euni);
+        fz_step(A, hu, a.etab, fn, c, en, hn, o0 + (u32)(kk + 1) * ob, ob, kk + 1 < k1, true);
+        fz_fetch<CM>(A, a, eo, ho, o0, cell0, ob, min(kk + 3, k1), euni);
+    }
+    if (kk == k1) fz_step(B, hu, a.etab, fn, c, en, hn, o0 + (u32)kk * ob, ob, false, true);
+}
+
 // Interior tile box of the adjoint tiling (tiles tcols wide, rows BY, z-blocks
 // of a.kchunk planes): the own cells of a tile and the neighbour quantities it
 // evaluates (one cell beyond it on each side: the E gather reads H at i-1,
@@ -1285,6 +1528,35 @@ void adj_box(const KArgs &a, int tcols, int ib[6])
     for (int q = 0; q < 6; ++q) ib[q] = box[q];
 }
 
+// Interior tile columns of the adjoint box, split into those the fused
+// sweep takes (quiet: no event column -- source or probe -- among the
+// tile's cells and the cells at i+1 and j+1 beyond it, and no design edge
+// among its cells) and the others.  Packed tx | ty << 16.
+void adj_fuse_tiles(const KArgs &a0, const std::vector<int> &e_first, const std::vector<int> &h_first,
+                    const int sb[6], std::vector<int> &quiet, std::vector<int> &noisy)
+{
+    KArgs a = a0;
+    a.tiles_x = (a.nx + TX - 1) / TX;
+    adj_box(a, TX, a.ib);
+    quiet.clear();
+    noisy.clear();
+    if (a.ib[5] == a.ib[2]) return;
+    for (int ty = a.ib[1]; ty < a.ib[4]; ++ty)
+        for (int tx = a.ib[0]; tx < a.ib[3]; ++tx) {
+            const int i0 = tx * TX, j0 = ty * BY;
+            bool q = true;
+            for (int j = j0; j <= j0 + BY && q; ++j)
+                for (int i = i0; i <= i0 + TX && q; ++i) {
+                    const size_t col = (size_t)j * a.nx + i;
+                    if ((col < e_first.size() && e_first[col] >= 0) || (col < h_first.size() && h_first[col] >= 0))
+                        q = false;
+                }
+            // the design box's x-y footprint over the tile's own cells
+            if (sb[3] > 0 && i0 < sb[0] + sb[3] && sb[0] < i0 + TX && j0 < sb[1] + sb[4] && sb[1] < j0 + BY) q = false;
+            (q ? quiet : noisy).push_back(tx | (ty << 16));
+        }
+}
+
 // which: 0 adjoint E sweep, 1 adjoint H sweep.  a0.ib[0] < 0 asks for the
 // general variant alone (no interior split).
 hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, const AArgs &b, int m, int fstep,
@@ -1309,15 +1581,15 @@ hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, 
 #define ADJ_E(CMV, V, NT, EU, HU)                                                                                \
     do {                                                                                                         \
         if (stored)                                                                                              \
-            hipLaunchKernelGGL((adj_e_kernel<CMV, V>), dim3(NT), dim3(TX * BY), 0, s, a, b, m, fstep, sten, EU); \
+            hipLaunchKernelGGL((adj_e_kernel<CMV, V>), dim3(NT), dim3(TX * BY), 0, s, a, bb, m, fstep, sten, EU); \
         else                                                                                                     \
-            hipLaunchKernelGGL((adj_e_lds_kernel<CMV, V>), dim3(NT), dim3(TX * BY), 0, s, a, b, m, fstep, sten, \
+            hipLaunchKernelGGL((adj_e_lds_kernel<CMV, V>), dim3(NT), dim3(TX * BY), 0, s, a, bb, m, fstep, sten, \
                                EU, HU);                                                                          \
     } while (0)
 #define ADJ_H(CMV, V, NT, EU, HU)                                                                            \
     do {                                                                                                     \
-        if (stored) hipLaunchKernelGGL((adj_h_kernel<CMV, V>), dim3(NT), dim3(TX * BY), 0, s, a, b, EU, HU); \
-        else hipLaunchKernelGGL((adj_h_lds_kernel<CMV, V>), dim3(NT), dim3(TX * BY), 0, s, a, b, EU, HU);    \
+        if (stored) hipLaunchKernelGGL((adj_h_kernel<CMV, V>), dim3(NT), dim3(TX * BY), 0, s, a, bb, EU, HU); \
+        else hipLaunchKernelGGL((adj_h_lds_kernel<CMV, V>), dim3(NT), dim3(TX * BY), 0, s, a, bb, EU, HU);    \
     } while (0)
 #define ADJ_V(MAC, V, NT)                                    \
     do {                                                     \
@@ -1325,11 +1597,26 @@ hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, 
         else if (mode == CLS_U16) MAC(CLS_U16, V, NT, euni, huni); \
         else MAC(CLS_UNIFORM, V, NT, 1, 1);                  \
     } while (0)
+    // fused interior sweep (b.nfl > 0): the interior E and H sweeps take the
+    // other columns (b.tl); it runs after the E sweep's launches and before
+    // the H sweep's.  An E-only pass (sten 0) covers the whole box.
+    const bool fuse = !stored && b.nfl > 0 && sten;
+    AArgs bb = b;
+    if (!fuse) bb.ntl = 0;
+    if (fuse && n_in) n_in = b.ntl * (a.ib[5] - a.ib[2]);
     if (which == 0) {
         if (n_in) ADJ_V(ADJ_E, V_IN, n_in);
         if (n_x) ADJ_V(ADJ_E, V_X, n_x);
         if (n_z) ADJ_V(ADJ_E, V_Z, n_z);
         if (n_gen) ADJ_V(ADJ_E, V_GEN, n_gen);
+        if (fuse) {
+            const int nt = b.nfl * (a.ib[5] - a.ib[2]), fin = fstep >= 0 ? 1 : 0;
+            if (mode == CLS_U32)
+                hipLaunchKernelGGL((adj_fused_kernel<CLS_U32>), dim3(nt), dim3(TX * BY), 0, s, a, b, fin, euni);
+            else if (mode == CLS_U16)
+                hipLaunchKernelGGL((adj_fused_kernel<CLS_U16>), dim3(nt), dim3(TX * BY), 0, s, a, b, fin, euni);
+            else hipLaunchKernelGGL((adj_fused_kernel<CLS_UNIFORM>), dim3(nt), dim3(TX * BY), 0, s, a, b, fin, 1);
+        }
     } else {
         if (n_in) ADJ_V(ADJ_H, V_IN, n_in);
         if (n_x) ADJ_V(ADJ_H, V_X, n_x);

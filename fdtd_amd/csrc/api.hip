@@ -30,6 +30,8 @@ hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a, c
                       int sten, hipStream_t s);
 hipError_t launch_snap(int storage, const KArgs &a, float *hist, int slot, hipStream_t s);
 bool adj_stores_quantities(int nx);
+void adj_fuse_tiles(const KArgs &a, const std::vector<int> &e_first, const std::vector<int> &h_first,
+                    const int sb[6], std::vector<int> &quiet, std::vector<int> &noisy);
 hipError_t launch_pair_half_step(bool efield, int storage, int mode, const KArgs &a, int m, hipStream_t s,
                                  bool gen_cells);
 int tile_rows();
@@ -768,6 +770,8 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
         HIPCHK(c, hipMemsetAsync(h, 0, (size_t)c->field_elems * 4, c->stream));
         b.eb[q] = (float *)(e + (size_t)c->ka.plane * 4);
         b.hb[q] = (float *)(h + (size_t)c->ka.plane * 4);
+        b.ebn[q] = b.eb[q];   // in place unless the fused interior sweep is set up below
+        b.hbn[q] = b.hb[q];
     }
     for (int q = 0; q < nq; ++q) {
         char *t = (char *)mem.get((size_t)c->field_elems * 4);
@@ -824,6 +828,44 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     }
     const int mode = c->built.mode;
     const int euni = c->built.e_uniform ? 1 : 0, huni = c->built.h_uniform ? 1 : 0;
+    // fused interior sweep (LDS gather path, one H class, FDTD_ADJ_FUSE=0
+    // disables): its tile columns and the others, and the second dJ/dE,
+    // dJ/dH buffers it writes (the pair is swapped after every step).  Not
+    // with the no-allocation scratch, nor when the pair would take more than
+    // half the free memory.
+    bool pp = false;
+    {
+        const char *fv = std::getenv("FDTD_ADJ_FUSE");
+        const bool want = !(fv && std::strcmp(fv, "0") == 0);
+        std::vector<int> quiet, noisy;
+        if (want && !arena && huni && !adj_stores_quantities(nx) && aa.ib[0] >= 0)
+            adj_fuse_tiles(aa, c->built.e_first, c->built.h_first, b.sb, quiet, noisy);
+        const size_t pair_bytes = 6 * (size_t)c->field_elems * 4 + (quiet.size() + noisy.size() + 2) * 4;
+        if (!quiet.empty()) HIPCHK(c, hipMemGetInfo(&freeb, &totb));
+        if (!quiet.empty() && pair_bytes < freeb / 2) {
+            for (int q = 0; q < 6; ++q) {
+                char *f = (char *)mem.get((size_t)c->field_elems * 4);
+                if (!f) return fail(c, FDTD_E_NOMEM, "adjoint field allocation failed");
+                HIPCHK(c, hipMemsetAsync(f, 0, (size_t)c->field_elems * 4, c->stream));
+                (q < 3 ? b.ebn[q] : b.hbn[q - 3]) = (float *)(f + (size_t)c->ka.plane * 4);
+            }
+            int *fl = (int *)mem.get(quiet.size() * 4);
+            int *tl = (int *)mem.get(std::max<size_t>(noisy.size(), 1) * 4);
+            if (!fl || !tl) return fail(c, FDTD_E_NOMEM, "adjoint tile lists");
+            HIPCHK(c, hipMemcpyAsync(fl, quiet.data(), quiet.size() * 4, hipMemcpyHostToDevice, c->stream));
+            if (!noisy.empty())
+                HIPCHK(c, hipMemcpyAsync(tl, noisy.data(), noisy.size() * 4, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));   // the host vectors go out of scope
+            b.fl = fl;
+            b.nfl = (int)quiet.size();
+            b.tl = tl;
+            b.ntl = (int)noisy.size();
+            pp = true;
+        }
+    }
+    // after a step the buffers just written hold the current state
+    auto swap_e = [&]() { for (int q = 0; q < 3; ++q) std::swap(b.eb[q], b.ebn[q]); };
+    auto swap_h = [&]() { for (int q = 0; q < 3; ++q) std::swap(b.hb[q], b.hbn[q]); };
 
     hipEvent_t t0 = c->adj_ev[0], t1 = c->adj_ev[1], t2 = c->adj_ev[2];
     // second stream (default priority) for the reverse sweeps of a segmented
@@ -925,8 +967,13 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
         for (int m = s1 - 1; m >= s0; --m) {
             HIPCHK(c, launch_adj(0, mode, euni, huni, aa, b, m, m < s1 - 1 ? m + 1 : -1, 1, rs));
             HIPCHK(c, launch_adj(1, mode, euni, huni, aa, b, m, -1, 1, rs));
+            if (pp) {
+                swap_e();
+                swap_h();
+            }
         }
         HIPCHK(c, launch_adj(0, mode, euni, huni, aa, b, s0, s0, 0, rs));
+        if (pp) swap_e();
         if (nseg > 1) {
             HIPCHK(c, hipEventRecord(sync.v[2 + q], rs));   // buffer q free
             if (sgi > 0 && (rc = recompute(sgi - 1))) return rc;

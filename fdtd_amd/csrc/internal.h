@@ -121,6 +121,18 @@ inline void region_tiles(const KArgs &a, int &n_in, int &n_x, int &n_z, int &n_g
 // Adjoint argument block (FP32 SI adjoint state; same layouts as the forward).
 struct AArgs {
     float *eb[3], *hb[3];      // dJ/dE, dJ/dH (guard-plane layout of the fields)
+    // where a reverse step writes its dJ/dE and dJ/dH: eb / hb themselves
+    // (in place), or, for the fused interior sweep, the other buffer of a
+    // pair (the host swaps the two after each step).  The H sweep reads
+    // dJ/dE from ebn (the values the E sweep just wrote).
+    float *ebn[3], *hbn[3];
+    // interior tiles (tx | ty << 16) of the box's columns, each for every
+    // z-block of the box: ntl > 0 restricts the interior E / H sweeps to
+    // tl (the columns the fused sweep does not take), nfl the fused sweep's
+    const int *tl;
+    int ntl;
+    const int *fl;
+    int nfl;
     // transposed quantities of every cell, same layout: the factors its
     // adjoint contributes to each field difference it reads, written by the
     // sweep that updates it, gathered by the other one

@@ -140,29 +140,6 @@ __device__ __forceinline__ float dpp_f(float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
-// The x-neighbour of a lane's cell as the adjacent lane's own value (a wave
-// is one 64-cell tile row): a DPP wave shift, no memory access.  The tile's
-// edge lane, which has no source lane, takes `fill`.  Loading the neighbour
-// instead reads each row at a 4-byte offset, three 128-byte lines where the
-// wave's own loads touch two.
-__device__ __forceinline__ float lane_prev(float v, float fill)   // lane l <- lane l-1; lane 0 <- fill
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x138, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float lane_next(float v, float fill)   // lane l <- lane l+1; lane 63 <- fill
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x130, 0xF, 0xF, false));
-}
-// the same for packed 16-bit pairs (two cells per lane)
-__device__ __forceinline__ std::uint32_t lane_prev_u(std::uint32_t v, std::uint32_t fill)
-{
-    return (std::uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x138, 0xF, 0xF, false);
-}
-__device__ __forceinline__ std::uint32_t lane_next_u(std::uint32_t v, std::uint32_t fill)
-{
-    return (std::uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x130, 0xF, 0xF, false);
-}
-
 // Block exponent from the block's largest magnitude mx: the smallest e with
 // 2^e > mx / 32767 (e = -BQ_BIAS for an all-zero block), and the int16
 // mantissa of x under it.

@@ -130,6 +130,28 @@ def test_adjoint_interior_split(engine, monkeypatch, nx):
     assert rel < 1e-3
 
 
+@pytest.mark.parametrize("K,chunks", [(0, None), (32, "5"), (0, "6")])
+def test_adjoint_fused_interior_bitwise(engine, monkeypatch, K, chunks):
+    """The fused interior reverse step (E and H sweeps in one march over the
+    tiles with no source, probe or design cell, reading one buffer of a
+    pair and writing the other) gives the separate sweeps' gradient bit for
+    bit, stored history and checkpointed, with z-blocks of 1, 8 and 7
+    planes; the widened source puts source edges at i+1 and j+1 of some
+    tiles' cells."""
+    sc = wide_sources(small_design(nx=530, ny=40, nz=40, steps=90, design=(8, 6, 5)))
+    if chunks:
+        monkeypatch.setenv("FDTD_ADJ_CHUNKS", chunks)
+    out = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("FDTD_ADJ_FUSE", fuse)
+        print(f"[adjoint fused] {sc.nx}x{sc.ny}x{sc.nz} K={K} chunks={chunks} fuse={fuse}", flush=True)
+        sim = engine.Simulation.from_scene(sc, "f32")
+        out[fuse] = sim.adjoint_gradient(TERMS, sc.steps, K)
+        sim.close()
+    assert out["1"][0] == out["0"][0]
+    np.testing.assert_array_equal(out["1"][1], out["0"][1])
+
+
 @pytest.mark.parametrize("nx", [200, 530])
 def test_adjoint_gather_paths_bitwise(engine, monkeypatch, nx):
     """The two forms of the adjoint sweeps -- every cell's transposed
