@@ -628,15 +628,15 @@ struct AECol {
     u32 o0, ob, cell0;   // byte offset of (i, j, 0) from the allocation base; bytes per plane; cell index of (i, j, 0)
 };
 
-// Interior tile of a workgroup from a column list (tx | ty << 16), each
-// column for every z-block of the box, XCD-aware as remap_tile.
+// Interior tile of a workgroup from a list of tiles of the box (each its
+// index in the box, x fastest), XCD-aware as remap_tile.
 __device__ __forceinline__ void tile_of_list(const KArgs &a, const int *tl, int ntl, int &tx, int &ty, int &tz)
 {
-    const int t = remap_tile(ntl * (a.ib[5] - a.ib[2]));
-    const int c = tl[t % ntl];
-    tx = c & 0xffff;
-    ty = c >> 16;
-    tz = a.ib[2] + t / ntl;
+    const int e = tl[remap_tile(ntl)];
+    const int bx = a.ib[3] - a.ib[0], by = a.ib[4] - a.ib[1];
+    tx = a.ib[0] + e % bx;
+    ty = a.ib[1] + (e / bx) % by;
+    tz = a.ib[2] + e / (bx * by);
 }
 
 // PIPE (the two-deep pipeline of the interior variants and of the E sweep's
@@ -1528,10 +1528,10 @@ void adj_box(const KArgs &a, int tcols, int ib[6])
     for (int q = 0; q < 6; ++q) ib[q] = box[q];
 }
 
-// Interior tile columns of the adjoint box, split into those the fused
-// sweep takes (quiet: no event column -- source or probe -- among the
-// tile's cells and the cells at i+1 and j+1 beyond it, and no design edge
-// among its cells) and the others.  Packed tx | ty << 16.
+// Interior tiles of the adjoint box, split into those the fused sweep takes
+// (quiet: no event column -- source or probe -- among the tile's cells and
+// the cells at i+1 and j+1 beyond it, and no design edge among its cells)
+// and the others.  Each is its index in the box, x fastest.
 void adj_fuse_tiles(const KArgs &a0, const std::vector<int> &e_first, const std::vector<int> &h_first,
                     const int sb[6], std::vector<int> &quiet, std::vector<int> &noisy)
 {
@@ -1541,6 +1541,8 @@ void adj_fuse_tiles(const KArgs &a0, const std::vector<int> &e_first, const std:
     quiet.clear();
     noisy.clear();
     if (a.ib[5] == a.ib[2]) return;
+    const int bx = a.ib[3] - a.ib[0], by = a.ib[4] - a.ib[1], bz = a.ib[5] - a.ib[2];
+    std::vector<char> cq((size_t)bx * by, 1);   // column without events
     for (int ty = a.ib[1]; ty < a.ib[4]; ++ty)
         for (int tx = a.ib[0]; tx < a.ib[3]; ++tx) {
             const int i0 = tx * TX, j0 = ty * BY;
@@ -1551,10 +1553,21 @@ void adj_fuse_tiles(const KArgs &a0, const std::vector<int> &e_first, const std:
                     if ((col < e_first.size() && e_first[col] >= 0) || (col < h_first.size() && h_first[col] >= 0))
                         q = false;
                 }
-            // the design box's x-y footprint over the tile's own cells
-            if (sb[3] > 0 && i0 < sb[0] + sb[3] && sb[0] < i0 + TX && j0 < sb[1] + sb[4] && sb[1] < j0 + BY) q = false;
-            (q ? quiet : noisy).push_back(tx | (ty << 16));
+            cq[(size_t)(ty - a.ib[1]) * bx + (tx - a.ib[0])] = q;
         }
+    for (int z = 0; z < bz; ++z) {
+        const int k0 = (a.ib[2] + z) * a.kchunk, k1 = std::min(a.nz, k0 + a.kchunk);
+        for (int y = 0; y < by; ++y)
+            for (int x = 0; x < bx; ++x) {
+                const int i0 = (a.ib[0] + x) * TX, j0 = (a.ib[1] + y) * BY;
+                bool q = cq[(size_t)y * bx + x];
+                // the (dilated) design box over the tile's own cells
+                if (sb[3] > 0 && i0 < sb[0] + sb[3] && sb[0] < i0 + TX && j0 < sb[1] + sb[4] && sb[1] < j0 + BY &&
+                    k0 < sb[2] + sb[5] && sb[2] < k1)
+                    q = false;
+                (q ? quiet : noisy).push_back((z * by + y) * bx + x);
+            }
+    }
 }
 
 // which: 0 adjoint E sweep, 1 adjoint H sweep.  a0.ib[0] < 0 asks for the
@@ -1603,14 +1616,14 @@ hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, 
     const bool fuse = !stored && b.nfl > 0 && sten;
     AArgs bb = b;
     if (!fuse) bb.ntl = 0;
-    if (fuse && n_in) n_in = b.ntl * (a.ib[5] - a.ib[2]);
+    if (fuse && n_in) n_in = b.ntl;
     if (which == 0) {
         if (n_in) ADJ_V(ADJ_E, V_IN, n_in);
         if (n_x) ADJ_V(ADJ_E, V_X, n_x);
         if (n_z) ADJ_V(ADJ_E, V_Z, n_z);
         if (n_gen) ADJ_V(ADJ_E, V_GEN, n_gen);
         if (fuse) {
-            const int nt = b.nfl * (a.ib[5] - a.ib[2]), fin = fstep >= 0 ? 1 : 0;
+            const int nt = b.nfl, fin = fstep >= 0 ? 1 : 0;
             if (mode == CLS_U32)
                 hipLaunchKernelGGL((adj_fused_kernel<CLS_U32>), dim3(nt), dim3(TX * BY), 0, s, a, b, fin, euni);
             else if (mode == CLS_U16)

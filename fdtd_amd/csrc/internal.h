@@ -126,9 +126,9 @@ struct AArgs {
     // pair (the host swaps the two after each step).  The H sweep reads
     // dJ/dE from ebn (the values the E sweep just wrote).
     float *ebn[3], *hbn[3];
-    // interior tiles (tx | ty << 16) of the box's columns, each for every
-    // z-block of the box: ntl > 0 restricts the interior E / H sweeps to
-    // tl (the columns the fused sweep does not take), nfl the fused sweep's
+    // interior tiles (each its index in the box, x fastest): ntl > 0
+    // restricts the interior E / H sweeps to tl (the tiles the fused sweep
+    // does not take), fl the fused sweep's
     const int *tl;
     int ntl;
     const int *fl;
