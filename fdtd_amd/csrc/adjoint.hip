@@ -1570,7 +1570,7 @@ void adj_fuse_tiles(const KArgs &a0, const std::vector<int> &e_first, const std:
     }
 }
 
-// which: 0 adjoint E sweep, 1 adjoint H sweep.  a0.ib[0] < 0 asks for the
+// which: 0 adjoint E sweep, 1 adjoint H sweep, 2 the fused interior pass.  a0.ib[0] < 0 asks for the
 // general variant alone (no interior split).
 hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, const AArgs &b, int m, int fstep,
                       int sten, hipStream_t s)
@@ -1610,18 +1610,16 @@ hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, 
         else if (mode == CLS_U16) MAC(CLS_U16, V, NT, euni, huni); \
         else MAC(CLS_UNIFORM, V, NT, 1, 1);                  \
     } while (0)
-    // fused interior sweep (b.nfl > 0): the interior E and H sweeps take the
-    // other columns (b.tl); it runs after the E sweep's launches and before
-    // the H sweep's.  An E-only pass (sten 0) covers the whole box.
+    // fused interior sweep (b.nfl > 0, which 2): the interior E and H sweeps
+    // take the other tiles (b.tl).  The caller runs it after the previous
+    // step's H sweep and before this step's, beside or after this step's E
+    // sweep (they touch different cells).  An E-only pass (sten 0) covers the
+    // whole box.
     const bool fuse = !stored && b.nfl > 0 && sten;
     AArgs bb = b;
     if (!fuse) bb.ntl = 0;
     if (fuse && n_in) n_in = b.ntl;
-    if (which == 0) {
-        if (n_in) ADJ_V(ADJ_E, V_IN, n_in);
-        if (n_x) ADJ_V(ADJ_E, V_X, n_x);
-        if (n_z) ADJ_V(ADJ_E, V_Z, n_z);
-        if (n_gen) ADJ_V(ADJ_E, V_GEN, n_gen);
+    if (which == 2) {
         if (fuse) {
             const int nt = b.nfl, fin = fstep >= 0 ? 1 : 0;
             if (mode == CLS_U32)
@@ -1630,6 +1628,13 @@ hipError_t launch_adj(int which, int mode, int euni, int huni, const KArgs &a0, 
                 hipLaunchKernelGGL((adj_fused_kernel<CLS_U16>), dim3(nt), dim3(TX * BY), 0, s, a, b, fin, euni);
             else hipLaunchKernelGGL((adj_fused_kernel<CLS_UNIFORM>), dim3(nt), dim3(TX * BY), 0, s, a, b, fin, 1);
         }
+        return hipGetLastError();
+    }
+    if (which == 0) {
+        if (n_in) ADJ_V(ADJ_E, V_IN, n_in);
+        if (n_x) ADJ_V(ADJ_E, V_X, n_x);
+        if (n_z) ADJ_V(ADJ_E, V_Z, n_z);
+        if (n_gen) ADJ_V(ADJ_E, V_GEN, n_gen);
     } else {
         if (n_in) ADJ_V(ADJ_H, V_IN, n_in);
         if (n_x) ADJ_V(ADJ_H, V_X, n_x);

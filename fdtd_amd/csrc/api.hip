@@ -880,6 +880,19 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
         sync.v.assign(4, nullptr);
         for (auto &e : sync.v) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
+    // the fused interior pass runs on a stream of its own (default priority)
+    // beside the E sweep's other launches, which touch other cells; the H
+    // sweep waits for it (FDTD_ADJ_STREAMS=1: all on one stream).  fev[0]:
+    // the previous step's H sweep done, fev[1]: the fused pass done.
+    StreamGuard fsg;
+    EventSet fev;
+    const char *sv = std::getenv("FDTD_ADJ_STREAMS");
+    const bool fz_async = pp && !(sv && std::strcmp(sv, "1") == 0);
+    if (fz_async) {
+        HIPCHK(c, hipStreamCreateWithFlags(&fsg.s, hipStreamNonBlocking));
+        fev.v.assign(2, nullptr);
+        for (auto &e : fev.v) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     HIPCHK(c, hipEventRecord(t0, c->stream));
     // ---- forward pass: DFT at the probes; checkpoints at segment starts, or
     // the whole design history when there is a single segment
@@ -965,11 +978,23 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
         b.hist = hbuf[q];
         b.hist_base = s0;
         for (int m = s1 - 1; m >= s0; --m) {
-            HIPCHK(c, launch_adj(0, mode, euni, huni, aa, b, m, m < s1 - 1 ? m + 1 : -1, 1, rs));
-            HIPCHK(c, launch_adj(1, mode, euni, huni, aa, b, m, -1, 1, rs));
+            const int fst = m < s1 - 1 ? m + 1 : -1;
             if (pp) {
+                const hipStream_t fs = fz_async ? fsg.s : rs;
+                if (fz_async) {
+                    HIPCHK(c, hipEventRecord(fev.v[0], rs));
+                    HIPCHK(c, hipStreamWaitEvent(fs, fev.v[0], 0));
+                }
+                HIPCHK(c, launch_adj(2, mode, euni, huni, aa, b, m, fst, 1, fs));
+                if (fz_async) HIPCHK(c, hipEventRecord(fev.v[1], fs));
+                HIPCHK(c, launch_adj(0, mode, euni, huni, aa, b, m, fst, 1, rs));
+                if (fz_async) HIPCHK(c, hipStreamWaitEvent(rs, fev.v[1], 0));
+                HIPCHK(c, launch_adj(1, mode, euni, huni, aa, b, m, -1, 1, rs));
                 swap_e();
                 swap_h();
+            } else {
+                HIPCHK(c, launch_adj(0, mode, euni, huni, aa, b, m, fst, 1, rs));
+                HIPCHK(c, launch_adj(1, mode, euni, huni, aa, b, m, -1, 1, rs));
             }
         }
         HIPCHK(c, launch_adj(0, mode, euni, huni, aa, b, s0, s0, 0, rs));
