@@ -931,11 +931,15 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     // A step moves information one cell per axis (E reads H at i-1, H reads E
     // at i+1), so over a segment of at most Kseg steps the design history
     // depends only on the checkpoint within Kseg + 1 cells of the snapshot
-    // box: the recompute sweeps that box, dilated by Kseg + 2, alone, when
-    // it holds under 60 % of the grid's cells (FDTD_LIGHTCONE=0: the whole
-    // grid).  Cells outside it keep stale values, and every segment restarts
-    // from a full checkpoint.
+    // box: the recompute sweeps that box alone, dilated by Kseg + 2 at a
+    // segment's first step and a cell less at each later one, when the first
+    // box holds under 60 % of the grid's cells (FDTD_LIGHTCONE=0: the whole
+    // grid; FDTD_LIGHTCONE=fixed: the first box for every step).  Cells
+    // outside it keep stale values, and every segment restarts from a full
+    // checkpoint.
     int lcone[6] = {0, 0, 0, 0, 0, 0};
+    const char *lce = std::getenv("FDTD_LIGHTCONE");
+    const bool lc_fixed = lce && std::strcmp(lce, "fixed") == 0;
     {
         const char *lv = std::getenv("FDTD_LIGHTCONE");
         const int R = Kseg + 2;
@@ -967,7 +971,21 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
         for (int v = 0; v < 6; ++v) ra.lc[v] = lcone[v];
         bind_sets(c, ra);
         HIPCHK(c, launch_snap(in.storage, ra, hbuf[q], 0, c->stream));
-        for (int m = s0; m < s1; ++m) HIPCHK(c, do_step(c, ra, m));
+        for (int m = s0; m < s1; ++m) {
+            if (lcone[3] > 0 && !lc_fixed) {
+                // the cone narrows a cell per step: step m feeds only the
+                // snapshots of steps m .. s1-1, so the design box dilated by
+                // s1 - m + 2 (K + 2 at the segment's first step) suffices
+                const int R = s1 - m + 2;
+                ra.lc[0] = std::max(0, d[0] - R);
+                ra.lc[1] = std::max(0, d[1] - R);
+                ra.lc[2] = std::max(0, d[2] - R);
+                ra.lc[3] = std::min(nx, d[0] + bx + R);
+                ra.lc[4] = std::min(ny, d[1] + by + R);
+                ra.lc[5] = std::min(nz, d[2] + bz + R);
+            }
+            HIPCHK(c, do_step(c, ra, m));
+        }
         HIPCHK(c, hipEventRecord(sync.v[q], c->stream));   // buffer q ready
         return FDTD_OK;
     };

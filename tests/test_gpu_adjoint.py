@@ -174,10 +174,11 @@ def test_adjoint_gather_paths_bitwise(engine, monkeypatch, nx):
 @pytest.mark.parametrize("storage,gen16", [("f32", None), ("f16", None), ("f16", "pair")])
 def test_light_cone_recompute_is_bitwise(engine, monkeypatch, storage, gen16):
     """Checkpointed segments recomputed over the snapshot box dilated by the
-    segment length plus two (a small design region in a large grid: 4 % of
-    the cells) give the stored-history gradient and the whole-grid recompute's
-    bit for bit (FP32 one-cell sweeps; FP16 pair sweeps with the one-cell or
-    their own general variant over the light-cone tiles)."""
+    segment length plus two at a segment's first step and a cell less at
+    each later one (a small design region in a large grid: 4 % of the cells)
+    give the stored-history gradient, the fixed box's and the whole-grid
+    recompute's bit for bit (FP32 one-cell sweeps; FP16 pair sweeps with the
+    one-cell or their own general variant over the light-cone tiles)."""
     if gen16:
         monkeypatch.setenv("FDTD_GEN16", gen16)
     sc = scenes.adjoint(nx=140, ny=120, nz=100, steps=150, L=8, d=1e-3, design=(6, 5, 4), eps_range=(1.0, 4.0),
@@ -190,8 +191,11 @@ def test_light_cone_recompute_is_bitwise(engine, monkeypatch, storage, gen16):
     assert sim.info()["adjoint_segments"] == 10
     monkeypatch.setenv("FDTD_LIGHTCONE", "0")
     J2, g2 = sim.adjoint_gradient(terms, sc.steps, 16)
+    monkeypatch.setenv("FDTD_LIGHTCONE", "fixed")
+    J3, g3 = sim.adjoint_gradient(terms, sc.steps, 16)
     sim.close()
     assert np.abs(g0).max() > 0
-    assert J0 == J1 == J2
+    assert J0 == J1 == J2 == J3
     np.testing.assert_array_equal(g1, g0)
     np.testing.assert_array_equal(g2, g0)
+    np.testing.assert_array_equal(g3, g0)
