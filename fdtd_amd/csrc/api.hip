@@ -932,26 +932,26 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
     // at i+1), so over a segment of at most Kseg steps the design history
     // depends only on the checkpoint within Kseg + 1 cells of the snapshot
     // box: the recompute sweeps that box alone, dilated by Kseg + 2 at a
-    // segment's first step and a cell less at each later one, when the first
-    // box holds under 60 % of the grid's cells (FDTD_LIGHTCONE=0: the whole
-    // grid; FDTD_LIGHTCONE=fixed: the first box for every step).  Cells
-    // outside it keep stale values, and every segment restarts from a full
-    // checkpoint.
-    int lcone[6] = {0, 0, 0, 0, 0, 0};
+    // segment's first step and a cell less at each later one, whenever the
+    // step's box holds under 60 % of the grid's cells (the light-cone tiles
+    // run the general variant; a larger box runs the whole grid).
+    // FDTD_LIGHTCONE=0: the whole grid always; FDTD_LIGHTCONE=fixed: the
+    // first box for every step.  Cells outside it keep stale values, and every
+    // segment restarts from a full checkpoint.
     const char *lce = std::getenv("FDTD_LIGHTCONE");
+    const bool lc_on = nseg > 1 && !(lce && std::strcmp(lce, "0") == 0);
     const bool lc_fixed = lce && std::strcmp(lce, "fixed") == 0;
-    {
-        const char *lv = std::getenv("FDTD_LIGHTCONE");
-        const int R = Kseg + 2;
+    // the box dilated by R, or all zero (the whole grid)
+    auto cone = [&](int R, int lc[6]) {
         const int lo[3] = {std::max(0, d[0] - R), std::max(0, d[1] - R), std::max(0, d[2] - R)};
         const int hi[3] = {std::min(nx, d[0] + bx + R), std::min(ny, d[1] + by + R), std::min(nz, d[2] + bz + R)};
         const double frac = (double)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]) / ((double)nx * ny * nz);
-        if (nseg > 1 && frac < 0.6 && !(lv && std::strcmp(lv, "0") == 0))
-            for (int v = 0; v < 3; ++v) {
-                lcone[v] = lo[v];
-                lcone[v + 3] = hi[v];
-            }
-    }
+        const bool use = lc_on && frac < 0.6;
+        for (int v = 0; v < 3; ++v) {
+            lc[v] = use ? lo[v] : 0;
+            lc[v + 3] = use ? hi[v] : 0;
+        }
+    };
     auto recompute = [&](int sgi) -> int {
         const int s0 = sgi * Kseg, s1 = std::min(n, s0 + Kseg), q = sgi % 2;
         if (sgi + 2 < nseg) HIPCHK(c, hipStreamWaitEvent(c->stream, sync.v[2 + q], 0));   // buffer q free
@@ -968,22 +968,14 @@ int run_adjoint(fdtd_ctx *c, const fdtd_objective *obj, int n_steps, int K, doub
         ra.rec = 0;
         ra.snap = hbuf[q];
         ra.snap_base = s0;
-        for (int v = 0; v < 6; ++v) ra.lc[v] = lcone[v];
+        cone(Kseg + 2, ra.lc);
         bind_sets(c, ra);
         HIPCHK(c, launch_snap(in.storage, ra, hbuf[q], 0, c->stream));
         for (int m = s0; m < s1; ++m) {
-            if (lcone[3] > 0 && !lc_fixed) {
-                // the cone narrows a cell per step: step m feeds only the
-                // snapshots of steps m .. s1-1, so the design box dilated by
-                // s1 - m + 2 (K + 2 at the segment's first step) suffices
-                const int R = s1 - m + 2;
-                ra.lc[0] = std::max(0, d[0] - R);
-                ra.lc[1] = std::max(0, d[1] - R);
-                ra.lc[2] = std::max(0, d[2] - R);
-                ra.lc[3] = std::min(nx, d[0] + bx + R);
-                ra.lc[4] = std::min(ny, d[1] + by + R);
-                ra.lc[5] = std::min(nz, d[2] + bz + R);
-            }
+            // the cone narrows a cell per step: step m feeds only the
+            // snapshots of steps m .. s1-1, so the design box dilated by
+            // s1 - m + 2 (Kseg + 2 at most) suffices
+            if (!lc_fixed) cone(s1 - m + 2, ra.lc);
             HIPCHK(c, do_step(c, ra, m));
         }
         HIPCHK(c, hipEventRecord(sync.v[q], c->stream));   // buffer q ready
