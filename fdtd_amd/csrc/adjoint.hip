@@ -1293,19 +1293,19 @@ __global__ void snap_kernel(const KArgs a, float *hist, const int slot)
 }
 
 // Fused interior reverse step: the E sweep and the H sweep of one reverse
-// step in one march over an interior tile whose cells, and the cells one
-// beyond them at i+1 and j+1, hold no source, probe or design edge (the
-// host's list).  Plane kk's E update runs one plane ahead of plane kk-1's H
-// update, which reads dJ/dE of (i, j, kk).  The E values a tile's H cells
-// read beyond the tile -- (i+1, j) for the last lane, (i, j+1) for every row
-// (the row above is another wave's, or another tile's) -- are evaluated here
-// as well, from the old values: the pass reads eb / hb and writes ebn / hbn,
-// the other buffers of a pair, so no workgroup reads a value that another
-// has already overwritten.  A cell's arithmetic is the E and H sweeps' (the
-// same products and sums in the same order), so the result is theirs bit
-// for bit, at about two thirds of their memory traffic: each cell's dJ/dE
-// and dJ/dH are read once and written once per step.  H coefficients are
-// one class (the host fuses only then).
+// step in one march over an interior tile whose cells hold no design edge
+// and whose cells and the cells one beyond them at i+1 and j+1 hold no
+// source or probe (the host's list).  Plane kk's E update runs one plane
+// ahead of plane kk-1's H update, which reads dJ/dE of (i, j, kk).  The E
+// values a tile's H cells read beyond the tile -- (i+1, j) for the last
+// lane, (i, j+1) for every row (the row above is another wave's, or another
+// tile's) -- are evaluated here as well, from the old values: the pass reads
+// eb / hb and writes ebn / hbn, the other buffers of a pair, so no workgroup
+// reads a value that another has already overwritten.  A cell's arithmetic
+// is the E and H sweeps' (the same products and sums in the same order), so
+// the result is theirs bit for bit, at about two thirds of their memory
+// traffic: each cell's dJ/dE and dJ/dH are read once and written once per
+// step.  H coefficients are one class (the host fuses only then).
 
 // Reads of plane kk.  Every load is issued on every lane: a lane that needs
 // no neighbour value reads the one address of its wave's lane that does.
